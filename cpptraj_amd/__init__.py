@@ -1,0 +1,283 @@
+"""cpptraj_amd -- MI355X (gfx950) best-fit RMSD for cpptraj's rms2d / cluster / rmsd path.
+
+The product is the C ABI library ``_lib/libcpptraj_rmsd.so`` (include/cpptraj_rmsd.h).
+This module is a thin ctypes binding over it for tests, benchmarks and Python
+callers.  It mirrors the reference's command options:
+
+* ``rms2d(...)``         Analysis_Rms2d (src/Analysis_Rms2d.cpp): mask / refmask,
+                         ``mass``, ``nofit`` / ``qrmsd``, reference trajectory,
+                         triangle or full matrix.
+* ``cluster_cache(...)`` Cluster::MetricArray::CacheDistances
+                         (src/Cluster/MetricArray.cpp:722-801).
+* ``rmsd_fit(...)``      Action_Rmsd (src/Action_Rmsd.cpp): ``nofit``, ``mass``,
+                         ``norotate`` / ``nomod``, ``savematrices``, ``savevectors``.
+
+Tensors are torch CUDA tensors (HIP on ROCm).  There is no CPU fallback: if the
+library is missing or no GPU is visible every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+__all__ = ["RmsdError", "Context", "frames_desc", "rms2d", "cluster_cache", "rmsd_fit",
+           "library_path", "load_library", "MODES", "PRECISIONS"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_lib", "libcpptraj_rmsd.so")
+
+MODES = {"fit": 0, "nofit": 1, "qrmsd": 2}
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+MODIFY = {"nomod": 0, "rottrans": 1, "norotate": 2}
+
+
+class RmsdError(RuntimeError):
+    pass
+
+
+class Frames(C.Structure):
+    """rmsd_frames (include/cpptraj_rmsd.h)."""
+    _fields_ = [
+        ("xyz", C.c_void_p),
+        ("stride", C.c_int64),
+        ("ld", C.c_int64),
+        ("nframes", C.c_int32),
+        ("natom", C.c_int32),
+        ("sel", C.c_void_p),
+        ("nsel", C.c_int32),
+        ("mass", C.c_void_p),
+        ("fidx", C.c_void_p),
+    ]
+
+
+_lib = None
+
+
+def library_path() -> str:
+    return _LIB_PATH
+
+
+def load_library():
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise RmsdError(f"{_LIB_PATH} is missing: build it with python -m cpptraj_amd.build")
+    L = C.CDLL(_LIB_PATH)
+    P = C.c_void_p
+    L.rmsd_ctx_create.argtypes = [C.POINTER(P), C.c_int, P]
+    L.rmsd_ctx_create.restype = C.c_int
+    L.rmsd_ctx_destroy.argtypes = [P]
+    L.rmsd_ctx_destroy.restype = None
+    L.rmsd_last_error.argtypes = [P]
+    L.rmsd_last_error.restype = C.c_char_p
+    L.rmsd_ctx_synchronize.argtypes = [P]
+    L.rmsd_ctx_synchronize.restype = C.c_int
+    L.rmsd_ctx_scratch_bytes.argtypes = [P]
+    L.rmsd_ctx_scratch_bytes.restype = C.c_int64
+    FP = C.POINTER(Frames)
+    L.rmsd_pairwise.argtypes = [P, FP, FP, C.c_int, C.c_int, C.c_int, P]
+    L.rmsd_pairwise.restype = C.c_int
+    L.rmsd_cluster_cache.argtypes = [P, FP, C.c_int, C.c_int, C.c_int, C.c_double, P]
+    L.rmsd_cluster_cache.restype = C.c_int
+    L.rmsd_fit_to_ref.argtypes = [P, FP, FP, C.c_int, C.c_int, C.c_int, C.c_int, P, P, P]
+    L.rmsd_fit_to_ref.restype = C.c_int
+    _optional(L)
+    _lib = L
+    return L
+
+
+def _optional(L):
+    """Entry points that later parts of the library add (batch, backward)."""
+    P = C.c_void_p
+    FP = C.POINTER(Frames)
+    if hasattr(L, "rmsd_pairwise_batch"):
+        L.rmsd_pairwise_batch.argtypes = [P, FP, P, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.rmsd_pairwise_batch.restype = C.c_int
+    if hasattr(L, "rmsd_fit_backward"):
+        L.rmsd_fit_backward.argtypes = [P, FP, FP, C.c_int, C.c_int, P, P]
+        L.rmsd_fit_backward.restype = C.c_int
+    if hasattr(L, "rmsd_pairwise_backward"):
+        L.rmsd_pairwise_backward.argtypes = [P, FP, FP, C.c_int, P, P, P]
+        L.rmsd_pairwise_backward.restype = C.c_int
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def frames_desc(xyz, sel=None, mass=None, fidx=None, soa=False, nsel=None, natom=None):
+    """Builds an rmsd_frames for a CUDA tensor.
+
+    xyz: AoS (F, natom*3) or (F, natom, 3) float32, or SoA (F, 3, ld) with soa=True
+         (natom atoms used of each padded row; default ld).
+    sel: int32 CUDA tensor of selected atoms; mass: float64 CUDA tensor (nsel)."""
+    import torch
+
+    if not xyz.is_cuda or xyz.dtype != torch.float32:
+        raise RmsdError("frames must be a float32 CUDA tensor")
+    F = xyz.shape[0]
+    if soa:
+        if xyz.dim() != 3 or xyz.shape[1] != 3 or xyz.stride(2) != 1 or xyz.stride(1) < 1:
+            raise RmsdError("SoA frames must be (F, 3, ld) with unit atom stride")
+        ld = xyz.stride(1)
+        natom = xyz.shape[2] if natom is None else natom
+        stride = xyz.stride(0) if F > 1 else 3 * ld
+    else:
+        x = xyz.reshape(F, -1) if xyz.dim() == 3 else xyz
+        if x.stride(-1) != 1:
+            raise RmsdError("AoS frames need unit coordinate stride")
+        ld = 0
+        natom = x.shape[1] // 3
+        stride = x.stride(0) if F > 1 else 3 * natom
+    for t, dt in ((sel, torch.int32), (mass, torch.float64), (fidx, torch.int32)):
+        if t is not None and (not t.is_cuda or t.dtype != dt or not t.is_contiguous()):
+            raise RmsdError("sel/fidx must be contiguous int32 and mass float64 CUDA tensors")
+    if nsel is None:
+        nsel = sel.numel() if sel is not None else natom
+    nfr = fidx.numel() if fidx is not None else F
+    d = Frames(xyz.data_ptr(), stride, ld, nfr, natom, None if sel is None else sel.data_ptr(),
+               nsel, None if mass is None else mass.data_ptr(),
+               None if fidx is None else fidx.data_ptr())
+    return d
+
+
+class Context:
+    """An rmsd_ctx bound to the current torch stream of `device`."""
+
+    def __init__(self, device: int = 0, stream=None):
+        import torch
+
+        L = load_library()
+        if not torch.cuda.is_available():
+            raise RmsdError("no GPU visible: the RMSD library has no CPU path")
+        self.device = device
+        if stream is None:
+            stream = torch.cuda.current_stream(device)
+        self.stream = stream
+        h = C.c_void_p()
+        rc = L.rmsd_ctx_create(C.byref(h), device, C.c_void_p(stream.cuda_stream))
+        if rc != 0:
+            raise RmsdError(f"rmsd_ctx_create failed ({rc})")
+        self._h = h
+        self._L = L
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.rmsd_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        if rc != 0:
+            msg = self._L.rmsd_last_error(self._h).decode()
+            raise RmsdError(f"rc={rc}: {msg}")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self):
+        self.check(self._L.rmsd_ctx_synchronize(self._h))
+
+    def scratch_bytes(self) -> int:
+        return int(self._L.rmsd_ctx_scratch_bytes(self._h))
+
+    # -- raw entry points on descriptors ----------------------------------------
+    def pairwise_desc(self, tgt: Frames, ref: Frames | None, mode: int, use_mass: bool,
+                      precision: int, out):
+        self.check(self._L.rmsd_pairwise(self._h, C.byref(tgt),
+                                         None if ref is None else C.byref(ref), mode,
+                                         int(use_mass), precision, _ptr(out)))
+
+    def cluster_desc(self, frames: Frames, nofit: bool, use_mass: bool, precision: int,
+                     weight: float, out):
+        self.check(self._L.rmsd_cluster_cache(self._h, C.byref(frames), int(nofit),
+                                              int(use_mass), precision, weight, _ptr(out)))
+
+    def fit_desc(self, frames: Frames, ref: Frames, ref_frame: int, fit: bool, use_mass: bool,
+                 modify: int, rmsd, U=None, tvec=None):
+        self.check(self._L.rmsd_fit_to_ref(self._h, C.byref(frames), C.byref(ref), ref_frame,
+                                           int(fit), int(use_mass), modify, _ptr(rmsd), _ptr(U),
+                                           _ptr(tvec)))
+
+
+_default_ctx = {}
+
+
+def _ctx(device=0) -> Context:
+    c = _default_ctx.get(device)
+    if c is None:
+        c = Context(device)
+        _default_ctx[device] = c
+    return c
+
+
+def rms2d(frames, mask=None, mass=None, mode="fit", use_mass=False, precision="fp64",
+          ref_frames=None, ref_mask=None, ref_mass=None, soa=False, natom=None,
+          ctx: Context | None = None):
+    """All-pairs RMSD (Analysis_Rms2d::Calculate_2D).
+
+    Returns the packed upper triangle (same frames and mask) or the full
+    (n_tgt, n_ref) matrix (reference trajectory or different masks) as a float32
+    CUDA tensor."""
+    import torch
+
+    ctx = ctx or _ctx(frames.device.index or 0)
+    tgt = frames_desc(frames, mask, mass, soa=soa, natom=natom)
+    full = ref_frames is not None or (ref_mask is not None and (
+        mask is None or ref_mask.numel() != mask.numel() or not torch.equal(ref_mask, mask)))
+    if not full:
+        F = tgt.nframes
+        out = torch.empty(F * (F - 1) // 2, dtype=torch.float32, device=frames.device)
+        ctx.pairwise_desc(tgt, None, MODES[mode], use_mass, PRECISIONS[precision], out)
+        return out
+    rf = ref_frames if ref_frames is not None else frames
+    ref = frames_desc(rf, ref_mask if ref_mask is not None else mask,
+                      ref_mass if ref_mass is not None else mass, soa=soa, natom=natom)
+    out = torch.empty((tgt.nframes, ref.nframes), dtype=torch.float32, device=frames.device)
+    ctx.pairwise_desc(tgt, ref, MODES[mode], use_mass, PRECISIONS[precision], out)
+    return out
+
+
+def cluster_cache(frames, frames_to_cache=None, mask=None, mass=None, nofit=False,
+                  use_mass=False, precision="fp64", ctx: Context | None = None):
+    """Pairwise-distance cache over `frames_to_cache` (int32 CUDA tensor of frame
+    indices, or all frames): packed upper triangle, float32."""
+    import torch
+
+    ctx = ctx or _ctx(frames.device.index or 0)
+    d = frames_desc(frames, mask, mass, fidx=frames_to_cache)
+    n = d.nframes
+    out = torch.empty(n * (n - 1) // 2, dtype=torch.float32, device=frames.device)
+    ctx.cluster_desc(d, nofit, use_mass, PRECISIONS[precision], 1.0, out)
+    return out
+
+
+def rmsd_fit(frames, ref_frames=None, ref_index=0, mask=None, mass=None, ref_mask=None,
+             ref_mass=None, fit=True, use_mass=False, modify="rottrans", save_matrices=False,
+             save_vectors=False, soa=False, natom=None, ctx: Context | None = None):
+    """Action_Rmsd over a batch of frames against one reference frame.
+
+    modify: 'rottrans' (default action), 'norotate', 'nomod'; frames are
+    rewritten in place unless 'nomod'.  Returns (rmsd, U or None, tvec or None)."""
+    import torch
+
+    ctx = ctx or _ctx(frames.device.index or 0)
+    d = frames_desc(frames, mask, mass, soa=soa, natom=natom)
+    rf = frames if ref_frames is None else ref_frames
+    r = frames_desc(rf, ref_mask if ref_mask is not None else mask,
+                    ref_mass if ref_mass is not None else mass, soa=soa, natom=natom)
+    F = d.nframes
+    dev = frames.device
+    rmsd = torch.empty(F, dtype=torch.float64, device=dev)
+    U = torch.empty((F, 3, 3), dtype=torch.float64, device=dev) if save_matrices else None
+    tv = torch.empty((F, 6), dtype=torch.float64, device=dev) if save_vectors else None
+    ctx.fit_desc(d, r, ref_index, fit, use_mass, MODIFY[modify], rmsd, U, tv)
+    return rmsd, U, tv

@@ -1,0 +1,436 @@
+// capi.hip -- implementation of the C ABI declared in include/cpptraj_rmsd.h.
+//
+// Host-side orchestration only: argument checks, scratch management, tile
+// schedules and the launch sequence of the kernels in k_*.hip.  Every entry
+// point enqueues work on the context's stream and returns without waiting
+// (except rmsd_ctx_synchronize).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/cpptraj_rmsd.h"
+#include "rmsd_kernels.hpp"
+
+namespace {
+
+constexpr int TILE = 64;      // frames per tile side (all pairwise kernels)
+constexpr int GROUP = 8;      // tile rows per band of the grouped schedule
+constexpr int NXCD = 8;       // MI355X: 8 XCDs, workgroups dealt round-robin
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct TileSchedule {
+  int nbr = -1, nbc = -1, tri = -1;
+  int2* dev = nullptr;
+  int ntiles = 0;
+};
+
+}  // namespace
+
+struct rmsd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::map<std::string, DevBuf> bufs;
+  TileSchedule sched;
+};
+
+namespace {
+
+int fail(rmsd_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_fail(rmsd_ctx* c, hipError_t e, const char* where) {
+  return fail(c, RMSD_ERR_DEVICE, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, expr)                                   \
+  do {                                                       \
+    hipError_t e_ = (expr);                                  \
+    if (e_ != hipSuccess) return hip_fail((ctx), e_, #expr); \
+  } while (0)
+
+// Grow-only named scratch buffer.  A buffer that has to grow is released only
+// after the stream has drained, since queued kernels may still read it.
+void* scratch(rmsd_ctx* c, const char* name, size_t bytes, int* status) {
+  DevBuf& b = c->bufs[name];
+  if (bytes == 0) bytes = 16;
+  if (b.bytes >= bytes) return b.p;
+  if (b.p) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  const size_t rounded = (bytes + 255) & ~size_t(255);
+  if (hipMalloc(&b.p, rounded) != hipSuccess) {
+    b.p = nullptr;
+    *status = fail(c, RMSD_ERR_ALLOC, std::string("cannot allocate scratch '") + name + "' of " +
+                                          std::to_string(rounded) + " bytes");
+    return nullptr;
+  }
+  b.bytes = rounded;
+  return b.p;
+}
+
+int check_frames(rmsd_ctx* c, const rmsd_frames* f, const char* what) {
+  if (!f) return fail(c, RMSD_ERR_ARG, std::string(what) + " is NULL");
+  if (f->nframes < 0 || f->nsel < 1 || f->natom < 1)
+    return fail(c, RMSD_ERR_ARG, std::string(what) + ": need nframes >= 0, nsel >= 1, natom >= 1");
+  if (!f->xyz && f->nframes > 0) return fail(c, RMSD_ERR_ARG, std::string(what) + ": xyz is NULL");
+  if (!f->sel && f->nsel > f->natom)
+    return fail(c, RMSD_ERR_ARG, std::string(what) + ": nsel > natom without a selection");
+  const int64_t need = f->ld ? (2 * f->ld + f->natom) : 3 * (int64_t)f->natom;
+  if (f->ld && f->ld < f->natom) return fail(c, RMSD_ERR_ARG, std::string(what) + ": ld < natom");
+  if (f->nframes > 1 && f->stride < need)
+    return fail(c, RMSD_ERR_ARG, std::string(what) + ": frame stride smaller than one frame");
+  return RMSD_OK;
+}
+
+FrameSetDev to_dev(const rmsd_frames& f) {
+  FrameSetDev d;
+  d.xyz = f.xyz;
+  d.stride = f.stride;
+  d.ld = f.ld;
+  d.nframes = f.nframes;
+  d.sel = f.sel;
+  d.nsel = f.nsel;
+  d.fidx = f.fidx;
+  return d;
+}
+
+// Grouped tile order: bands of GROUP tile rows, each band swept column by
+// column, so that consecutive tiles reuse the band's row blocks and one
+// column block.  Triangle schedules keep only tiles with row <= column.
+std::vector<int2> grouped_tiles(int nbr, int nbc, int tri) {
+  std::vector<int2> t;
+  t.reserve(tri ? (size_t)nbr * (nbr + 1) / 2 : (size_t)nbr * nbc);
+  for (int b0 = 0; b0 < nbr; b0 += GROUP) {
+    const int b1 = std::min(nbr, b0 + GROUP);
+    for (int j = tri ? b0 : 0; j < nbc; ++j)
+      for (int i = b0; i < b1; ++i)
+        if (!tri || i <= j) t.push_back(make_int2(i, j));
+  }
+  return t;
+}
+
+int tiles_for(rmsd_ctx* c, int FA, int FB, int tri, const int2** tiles, int* ntiles) {
+  const int nbr = (FA + TILE - 1) / TILE, nbc = (FB + TILE - 1) / TILE;
+  TileSchedule& s = c->sched;
+  if (s.nbr != nbr || s.nbc != nbc || s.tri != tri) {
+    std::vector<int2> h = grouped_tiles(nbr, nbc, tri);
+    int st = RMSD_OK;
+    s.nbr = s.nbc = s.tri = -1;
+    int2* d = (int2*)scratch(c, "tiles", h.size() * sizeof(int2), &st);
+    if (!d) return st;
+    HIP_TRY(c, hipMemcpyAsync(d, h.data(), h.size() * sizeof(int2), hipMemcpyHostToDevice,
+                              c->stream));
+    // The host vector dies here: make the copy complete first.
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    s.nbr = nbr;
+    s.nbc = nbc;
+    s.tri = tri;
+    s.dev = d;
+    s.ntiles = (int)h.size();
+  }
+  *tiles = s.dev;
+  *ntiles = s.ntiles;
+  return RMSD_OK;
+}
+
+int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+// Per-set operands of the pairwise contraction.
+struct Packed {
+  void* data;
+  double* g;
+  double* u;
+};
+
+int prepare_set(rmsd_ctx* c, const FrameSetDev& fs, const double* cmass, const double* smass,
+                int prec, int Kp, const double* T, const double* gT, const char* tag,
+                Packed* out) {
+  const int CH = chunk_atoms(prec);
+  const int Fp = (int)round_up(fs.nframes, TILE);
+  const size_t esz = prec == PREC_F64 ? 8 : (prec == PREC_F32 ? 4 : 2);
+  const size_t bytes = (size_t)Fp * Kp * 3 * esz * chunk_parts(prec);
+  int st = RMSD_OK;
+  std::string n(tag);
+  out->data = scratch(c, (n + ".packed").c_str(), bytes, &st);
+  if (!out->data) return st;
+  out->g = (double*)scratch(c, (n + ".g").c_str(), sizeof(double) * Fp, &st);
+  if (!out->g) return st;
+  out->u = (double*)scratch(c, (n + ".u").c_str(), sizeof(double) * 9 * Fp, &st);
+  if (!out->u) return st;
+  PrepArgs a;
+  a.fs = fs;
+  a.cmass = cmass;
+  a.smass = smass;
+  a.prec = prec;
+  a.Kp = Kp;
+  a.KB = Kp / CH;
+  a.Fp = Fp;
+  a.packed = out->data;
+  a.g = out->g;
+  a.u = out->u;
+  a.T = T;
+  a.gT = gT;
+  HIP_TRY(c, rmsd_launch_prep(a, c->stream));
+  return RMSD_OK;
+}
+
+// The pairwise matrix over frame sets A (rows) and B (columns).  tri: B is A
+// and only i < j is produced.  amass / bmass centre A / B; wmass (the target
+// masses) weights every covariance term.
+int pairwise_impl(rmsd_ctx* c, const FrameSetDev& A, const FrameSetDev& B, int tri,
+                  const double* amass, const double* bmass, const double* wmass, int mode,
+                  int prec, float* out, int rows_are_ref) {
+  const int FA = A.nframes, FB = B.nframes;
+  if (FA == 0 || FB == 0 || (tri && FA < 2)) return RMSD_OK;
+  int st = RMSD_OK;
+  const int2* tiles = nullptr;
+  int ntiles = 0;
+  if ((st = tiles_for(c, FA, FB, tri, &tiles, &ntiles)) != RMSD_OK) return st;
+  const int per_xcd = (ntiles + NXCD - 1) / NXCD;
+  double* consts = (double*)scratch(c, "pair.consts", sizeof(double) * 16, &st);
+  if (!consts) return st;
+  double* M = consts;        // total weight
+  double* P = consts + 1;    // template Gram matrix (9)
+  double* gT = consts + 10;  // template norm
+
+  if (mode == RMSD_MODE_NOFIT) {
+    SetupArgs s{};
+    s.fs = A;
+    s.smass = wmass;
+    s.total_mass = M;
+    HIP_TRY(c, rmsd_launch_setup(s, c->stream));
+    HIP_TRY(c, rmsd_launch_pair_nofit_tiles(A, B, wmass, M, tiles, ntiles, per_xcd, NXCD, tri, out,
+                                            c->stream));
+    return RMSD_OK;
+  }
+
+  const int CH = chunk_atoms(prec);
+  const int Kp = (int)round_up(A.nsel, CH);
+  double* T = nullptr;
+  if (prec != PREC_F64) {
+    T = (double*)scratch(c, "pair.template", sizeof(double) * 3 * Kp, &st);
+    if (!T) return st;
+  }
+  SetupArgs s{};
+  s.fs = B;  // template: first column (reference) frame
+  s.f0 = 0;
+  s.cmass = bmass;
+  s.smass = wmass;
+  s.Kp = Kp;
+  s.T = T;
+  s.P = P;
+  s.gT = gT;
+  s.total_mass = M;
+  HIP_TRY(c, rmsd_launch_setup(s, c->stream));
+
+  Packed pa, pb;
+  if ((st = prepare_set(c, A, amass, wmass, prec, Kp, T, gT, "A", &pa)) != RMSD_OK) return st;
+  if (tri) {
+    pb = pa;
+  } else if ((st = prepare_set(c, B, bmass, wmass, prec, Kp, T, gT, "B", &pb)) != RMSD_OK) {
+    return st;
+  }
+  PairArgs p;
+  p.A = pa.data;
+  p.B = pb.data;
+  p.KB = Kp / CH;
+  p.gA = pa.g;
+  p.gB = pb.g;
+  p.uA = pa.u;
+  p.uB = pb.u;
+  p.P = P;
+  p.total_mass = M;
+  p.tiles = tiles;
+  p.ntiles = ntiles;
+  p.per_xcd = per_xcd;
+  p.nxcd = NXCD;
+  p.FA = FA;
+  p.FB = FB;
+  p.tri = tri;
+  p.clamp = mode == RMSD_MODE_FIT ? 1 : 0;
+  p.out = out;
+  (void)rows_are_ref;
+  HIP_TRY(c, rmsd_launch_pairwise(prec, p, c->stream));
+  return RMSD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rmsd_ctx_create(rmsd_ctx** ctx, int device, void* stream) {
+  if (!ctx) return RMSD_ERR_ARG;
+  *ctx = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return RMSD_ERR_DEVICE;
+  if (hipSetDevice(device) != hipSuccess) return RMSD_ERR_DEVICE;
+  rmsd_ctx* c = new rmsd_ctx();
+  c->device = device;
+  c->stream = (hipStream_t)stream;
+  *ctx = c;
+  return RMSD_OK;
+}
+
+void rmsd_ctx_destroy(rmsd_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->bufs)
+    if (kv.second.p) (void)hipFree(kv.second.p);
+  delete ctx;
+}
+
+const char* rmsd_last_error(const rmsd_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int rmsd_ctx_synchronize(rmsd_ctx* ctx) {
+  if (!ctx) return RMSD_ERR_ARG;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return RMSD_OK;
+}
+
+int64_t rmsd_ctx_scratch_bytes(const rmsd_ctx* ctx) {
+  if (!ctx) return 0;
+  int64_t s = 0;
+  for (const auto& kv : ctx->bufs) s += (int64_t)kv.second.bytes;
+  return s;
+}
+
+int rmsd_pairwise(rmsd_ctx* ctx, const rmsd_frames* tgt, const rmsd_frames* ref, int mode,
+                  int use_mass, int precision, float* out) {
+  if (!ctx) return RMSD_ERR_ARG;
+  int st;
+  if ((st = check_frames(ctx, tgt, "tgt")) != RMSD_OK) return st;
+  if (ref && (st = check_frames(ctx, ref, "ref")) != RMSD_OK) return st;
+  if (ref && ref->nsel != tgt->nsel)
+    return fail(ctx, RMSD_ERR_ARG, "reference and target select different atom counts");
+  if (mode != RMSD_MODE_FIT && mode != RMSD_MODE_NOFIT && mode != RMSD_MODE_QUAT)
+    return fail(ctx, RMSD_ERR_ARG, "unknown mode");
+  if (precision < RMSD_PREC_FP64 || precision > RMSD_PREC_BF16X3)
+    return fail(ctx, RMSD_ERR_ARG, "unknown precision");
+  if (use_mass && (!tgt->mass || (ref && !ref->mass)))
+    return fail(ctx, RMSD_ERR_ARG, "use_mass needs masses for every frame set");
+  if (!out && tgt->nframes > 0) return fail(ctx, RMSD_ERR_ARG, "out is NULL");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const FrameSetDev T = to_dev(*tgt);
+  const double* tm = use_mass ? tgt->mass : nullptr;
+  if (!ref) return pairwise_impl(ctx, T, T, 1, tm, tm, tm, mode, precision, out, 1);
+  const FrameSetDev R = to_dev(*ref);
+  const double* rm = use_mass ? ref->mass : nullptr;
+  // Rows are targets, columns references: element (ntgt, nref) at ntgt*nref_total + nref.
+  return pairwise_impl(ctx, T, R, 0, tm, rm, tm, mode, precision, out, 0);
+}
+
+int rmsd_cluster_cache(rmsd_ctx* ctx, const rmsd_frames* frames, int nofit, int use_mass,
+                       int precision, double weight, float* out) {
+  if (!ctx) return RMSD_ERR_ARG;
+  if (weight != 1.0)
+    return fail(ctx, RMSD_ERR_ARG, "metric weights other than 1 are not supported");
+  return rmsd_pairwise(ctx, frames, nullptr, nofit ? RMSD_MODE_NOFIT : RMSD_MODE_FIT, use_mass,
+                       precision, out);
+}
+
+int rmsd_fit_to_ref(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* ref,
+                    int ref_frame, int fit, int use_mass, int modify, double* rmsd, double* U,
+                    double* tvec) {
+  if (!ctx) return RMSD_ERR_ARG;
+  int st;
+  if ((st = check_frames(ctx, frames, "frames")) != RMSD_OK) return st;
+  if ((st = check_frames(ctx, ref, "ref")) != RMSD_OK) return st;
+  if (ref->nsel != frames->nsel)
+    return fail(ctx, RMSD_ERR_ARG, "reference and target select different atom counts");
+  if (ref_frame < 0 || ref_frame >= ref->nframes)
+    return fail(ctx, RMSD_ERR_ARG, "reference frame out of range");
+  if (use_mass && (!frames->mass || !ref->mass))
+    return fail(ctx, RMSD_ERR_ARG, "use_mass needs target and reference masses");
+  if (modify < RMSD_MODIFY_NONE || modify > RMSD_MODIFY_TRANS)
+    return fail(ctx, RMSD_ERR_ARG, "unknown modify mode");
+  if (modify != RMSD_MODIFY_NONE && frames->fidx)
+    return fail(ctx, RMSD_ERR_ARG, "coordinate update needs frames in storage order (fidx NULL)");
+  if (!rmsd && frames->nframes > 0) return fail(ctx, RMSD_ERR_ARG, "rmsd is NULL");
+  if (frames->nframes == 0) return RMSD_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const int n = frames->nsel;
+  const int ldr = (int)round_up(n, 4);
+  const int64_t nf = frames->nframes;
+  double* refbuf = (double*)scratch(ctx, "fit.ref", sizeof(double) * (3 * ldr + 8), &st);
+  if (!refbuf) return st;
+  double* ref_trans = refbuf + 3 * ldr;
+  double* consts = (double*)scratch(ctx, "fit.consts", sizeof(double) * 8, &st);
+  if (!consts) return st;
+  double* wpad = nullptr;
+  if (use_mass) {
+    wpad = (double*)scratch(ctx, "fit.mass", sizeof(double) * ldr, &st);
+    if (!wpad) return st;
+    HIP_TRY(ctx, hipMemsetAsync(wpad, 0, sizeof(double) * ldr, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(wpad, frames->mass, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                                ctx->stream));
+  }
+  const FrameSetDev F = to_dev(*frames);
+  const FrameSetDev R = to_dev(*ref);
+  HIP_TRY(ctx, rmsd_launch_ref_prep(R, ref_frame, use_mass ? ref->mass : nullptr, wpad, fit ? 1 : 0,
+                                    refbuf, ref_trans, consts, ctx->stream));
+  if (!fit) {
+    NofitArgs a;
+    a.fs = F;
+    a.mass = wpad;
+    a.ref = refbuf;
+    HIP_TRY(ctx, rmsd_launch_nofit_frames(a, consts, rmsd, ctx->stream));
+    return RMSD_OK;
+  }
+  double* moments = (double*)scratch(ctx, "fit.moments", sizeof(double) * 16 * nf, &st);
+  if (!moments) return st;
+  double* xform = nullptr;
+  if (modify != RMSD_MODIFY_NONE) {
+    xform = (double*)scratch(ctx, "fit.xform", sizeof(double) * 15 * nf, &st);
+    if (!xform) return st;
+  }
+  FitArgs fa;
+  fa.fs = F;
+  fa.mass = wpad;
+  fa.ref = refbuf;
+  fa.consts = consts;
+  fa.moments = moments;
+  HIP_TRY(ctx, rmsd_launch_fit_moments(fa, ctx->stream));
+  SolveArgs sa;
+  sa.moments = moments;
+  sa.consts = consts;
+  sa.ref_trans = ref_trans;
+  sa.nframes = (int)nf;
+  sa.fit = 1;
+  sa.clamp = 1;
+  sa.rmsd = rmsd;
+  sa.U = U;
+  sa.tvec = tvec;
+  sa.xform = xform;
+  HIP_TRY(ctx, rmsd_launch_fit_solve(sa, ctx->stream));
+  if (modify != RMSD_MODIFY_NONE) {
+    ApplyArgs ap;
+    ap.xyz = const_cast<float*>(frames->xyz);
+    ap.stride = frames->stride;
+    ap.ld = frames->ld;
+    ap.natom = frames->natom;
+    ap.nframes = frames->nframes;
+    ap.xform = xform;
+    HIP_TRY(ctx, rmsd_launch_apply(ap, modify == RMSD_MODIFY_ROT_TRANS ? 1 : 2, ctx->stream));
+  }
+  return RMSD_OK;
+}
+
+}  // extern "C"

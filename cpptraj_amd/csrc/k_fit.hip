@@ -1,0 +1,355 @@
+// k_fit.hip -- streaming fit of many frames to one reference (the 'rmsd' action).
+//
+// Replaces the per-frame body of Action_Rmsd::DoAction (src/Action_Rmsd.cpp:
+// 361-417) for a batch of frames and a fixed reference ('first', 'ref' and
+// 'refindex' modes of ReferenceAction, src/ReferenceAction.h:73-84):
+//   k_ref_prep      gather + centre the reference once (ReferenceAction::
+//                   SelectRefAtoms, src/ReferenceAction.cpp:155-169)
+//   k_fit_moments   one HBM pass over each frame: FP64 sums w x, w x y^T and
+//                   w |x|^2 (the reference's centring + covariance + E0 passes,
+//                   src/Frame.cpp:1141-1209, folded into one pass)
+//   k_fit_solve     per frame FP64 QCP: RMSD, U (Kabsch convention), Trans
+//   k_nofit_frames  Frame::RMSD_NoFit (src/Frame.cpp:1279-1307) per frame
+//   k_apply         Frame::Trans_Rot_Trans (src/Frame.h:572-581) or the
+//                   translate-only update of 'norotate' on all atoms
+//
+// Single-pass centring: with the reference centred (sum_k w_k y_k = h, not
+// necessarily 0 when target and reference masses differ) the covariance of the
+// centred target is  R = sum w x y^T - c h^T  and  sum w |x-c|^2 =
+// sum w |x|^2 - W |c|^2.  Every frame is first shifted by its first selected
+// atom (an exact identity for both formulas) so that the sums carry the
+// molecule's extent only, not its absolute position.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "rmsd_device.hpp"
+#include "rmsd_kernels.hpp"
+
+namespace {
+
+constexpr int FIT_THREADS = 256;
+
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+
+__device__ __forceinline__ float fcoord(const FrameSetDev& fs, int64_t f, int k, int c) {
+  const int a = fs.sel ? fs.sel[k] : k;
+  const float* fr = fs.xyz + frame_of(fs, f) * fs.stride;
+  return fs.ld ? fr[c * fs.ld + a] : fr[3 * a + c];
+}
+
+// ---- reference ---------------------------------------------------------------
+// ref_out: SoA [3][ldr] FP64 (ldr = nsel rounded up to 4, zero padded).
+// consts: [0..2] sum w y, [3] sum w |y|^2, [4] W = sum w (sequential sum, so
+// that it rounds exactly as the reference's running total).
+__global__ __launch_bounds__(FIT_THREADS) void k_ref_prep(FrameSetDev fs, int f, const double* rmass,
+                                                          const double* tmass, int center, int ldr,
+                                                          double* ref_out, double* ref_trans,
+                                                          double* consts) {
+  __shared__ double red[8 * (FIT_THREADS / 64)];
+  double v[4] = {0, 0, 0, 0};
+  if (center) {
+    for (int k = threadIdx.x; k < fs.nsel; k += blockDim.x) {
+      const double m = rmass ? rmass[k] : 1.0;
+      v[0] += (double)fcoord(fs, f, k, 0) * m;
+      v[1] += (double)fcoord(fs, f, k, 1) * m;
+      v[2] += (double)fcoord(fs, f, k, 2) * m;
+      v[3] += m;
+    }
+    rmsd_dev::block_sum<4>(v, red);
+  }
+  double cen[3] = {0, 0, 0};
+  if (center) {
+    const double tot = rmass ? v[3] : (double)fs.nsel;
+    if (tot != 0.0) {
+      cen[0] = v[0] / tot;
+      cen[1] = v[1] / tot;
+      cen[2] = v[2] / tot;
+    }
+  }
+  double h[4] = {0, 0, 0, 0};
+  for (int k = threadIdx.x; k < ldr; k += blockDim.x) {
+    double y[3] = {0, 0, 0};
+    if (k < fs.nsel) {
+      y[0] = (double)fcoord(fs, f, k, 0) - cen[0];
+      y[1] = (double)fcoord(fs, f, k, 1) - cen[1];
+      y[2] = (double)fcoord(fs, f, k, 2) - cen[2];
+      const double w = tmass ? tmass[k] : 1.0;
+      h[0] += w * y[0];
+      h[1] += w * y[1];
+      h[2] += w * y[2];
+      h[3] += w * (y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+    }
+    ref_out[k] = y[0];
+    ref_out[ldr + k] = y[1];
+    ref_out[2 * ldr + k] = y[2];
+  }
+  rmsd_dev::block_sum<4>(h, red);
+  if (threadIdx.x == 0) {
+    consts[0] = h[0];
+    consts[1] = h[1];
+    consts[2] = h[2];
+    consts[3] = h[3];
+    double W = 0.0;
+    if (tmass) {
+      for (int k = 0; k < fs.nsel; ++k) W += tmass[k];
+    } else {
+      W = (double)fs.nsel;
+    }
+    consts[4] = W;
+    ref_trans[0] = cen[0];
+    ref_trans[1] = cen[1];
+    ref_trans[2] = cen[2];
+  }
+}
+
+// ---- moments -----------------------------------------------------------------
+template <bool WEIGHTED>
+__device__ __forceinline__ void accum(double (&v)[13], double x0, double x1, double x2, double w,
+                                      double y0, double y1, double y2) {
+  double wx0 = x0, wx1 = x1, wx2 = x2;
+  if (WEIGHTED) {
+    wx0 = w * x0;
+    wx1 = w * x1;
+    wx2 = w * x2;
+  }
+  v[0] += wx0;
+  v[1] += wx1;
+  v[2] += wx2;
+  v[3] += wx0 * y0; v[4] += wx0 * y1; v[5] += wx0 * y2;
+  v[6] += wx1 * y0; v[7] += wx1 * y1; v[8] += wx1 * y2;
+  v[9] += wx2 * y0; v[10] += wx2 * y1; v[11] += wx2 * y2;
+  v[12] += wx0 * x0 + wx1 * x1 + wx2 * x2;
+}
+
+// Vector path: SoA rows, no selection, 16-byte aligned rows.
+template <bool WEIGHTED>
+__global__ __launch_bounds__(FIT_THREADS) void k_fit_moments_soa4(FitArgs a, int ldr) {
+  __shared__ double red[13 * (FIT_THREADS / 64)];
+  const int64_t f = blockIdx.x;
+  const float* fr = a.fs.xyz + frame_of(a.fs, f) * a.fs.stride;
+  const int64_t ld = a.fs.ld;
+  const int n = a.fs.nsel;
+  const double s0 = fr[0], s1 = fr[ld], s2 = fr[2 * ld];
+  double v[13];
+#pragma unroll
+  for (int i = 0; i < 13; ++i) v[i] = 0.0;
+  const int n4 = n >> 2;
+  // Frames are read once: non-temporal loads keep the reference and the
+  // weights resident in L2.
+  const f32x4* X = reinterpret_cast<const f32x4*>(fr);
+  const f32x4* Y = reinterpret_cast<const f32x4*>(fr + ld);
+  const f32x4* Z = reinterpret_cast<const f32x4*>(fr + 2 * ld);
+  const double2* r0 = reinterpret_cast<const double2*>(a.ref);
+  const double2* r1 = reinterpret_cast<const double2*>(a.ref + ldr);
+  const double2* r2 = reinterpret_cast<const double2*>(a.ref + 2 * ldr);
+  const double2* mw = reinterpret_cast<const double2*>(a.mass);
+  for (int q = threadIdx.x; q < n4; q += blockDim.x) {
+    const f32x4 x = __builtin_nontemporal_load(X + q);
+    const f32x4 y = __builtin_nontemporal_load(Y + q);
+    const f32x4 z = __builtin_nontemporal_load(Z + q);
+    const double2 ya = r0[2 * q], yb = r0[2 * q + 1];
+    const double2 za = r1[2 * q], zb = r1[2 * q + 1];
+    const double2 wa = r2[2 * q], wb = r2[2 * q + 1];
+    double2 ma = {1.0, 1.0}, mb = {1.0, 1.0};
+    if (WEIGHTED) {
+      ma = mw[2 * q];
+      mb = mw[2 * q + 1];
+    }
+    accum<WEIGHTED>(v, x.x - s0, y.x - s1, z.x - s2, ma.x, ya.x, za.x, wa.x);
+    accum<WEIGHTED>(v, x.y - s0, y.y - s1, z.y - s2, ma.y, ya.y, za.y, wa.y);
+    accum<WEIGHTED>(v, x.z - s0, y.z - s1, z.z - s2, mb.x, yb.x, zb.x, wb.x);
+    accum<WEIGHTED>(v, x.w - s0, y.w - s1, z.w - s2, mb.y, yb.y, zb.y, wb.y);
+  }
+  for (int k = 4 * n4 + threadIdx.x; k < n; k += blockDim.x) {
+    const double w = WEIGHTED ? a.mass[k] : 1.0;
+    accum<WEIGHTED>(v, (double)fr[k] - s0, (double)fr[ld + k] - s1, (double)fr[2 * ld + k] - s2, w,
+                    a.ref[k], a.ref[ldr + k], a.ref[2 * ldr + k]);
+  }
+  rmsd_dev::block_sum<13>(v, red);
+  if (threadIdx.x == 0)
+    for (int i = 0; i < 13; ++i) a.moments[13 * f + i] = v[i];
+}
+
+// Generic path: any layout, optional selection.
+template <bool WEIGHTED>
+__global__ __launch_bounds__(FIT_THREADS) void k_fit_moments_any(FitArgs a, int ldr) {
+  __shared__ double red[13 * (FIT_THREADS / 64)];
+  const int64_t f = blockIdx.x;
+  const double s0 = fcoord(a.fs, f, 0, 0), s1 = fcoord(a.fs, f, 0, 1), s2 = fcoord(a.fs, f, 0, 2);
+  double v[13];
+#pragma unroll
+  for (int i = 0; i < 13; ++i) v[i] = 0.0;
+  for (int k = threadIdx.x; k < a.fs.nsel; k += blockDim.x) {
+    const double w = WEIGHTED ? a.mass[k] : 1.0;
+    accum<WEIGHTED>(v, (double)fcoord(a.fs, f, k, 0) - s0, (double)fcoord(a.fs, f, k, 1) - s1,
+                    (double)fcoord(a.fs, f, k, 2) - s2, w, a.ref[k], a.ref[ldr + k],
+                    a.ref[2 * ldr + k]);
+  }
+  rmsd_dev::block_sum<13>(v, red);
+  if (threadIdx.x == 0)
+    for (int i = 0; i < 13; ++i) a.moments[13 * f + i] = v[i];
+}
+
+// Per-frame shift used by the moment kernels (first selected atom).
+__global__ void k_fit_shift(FrameSetDev fs, double* shift) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= fs.nframes) return;
+  shift[3 * f] = fcoord(fs, f, 0, 0);
+  shift[3 * f + 1] = fcoord(fs, f, 0, 1);
+  shift[3 * f + 2] = fcoord(fs, f, 0, 2);
+}
+
+// ---- solve ---------------------------------------------------------------------
+// xform (15 doubles per frame): U (9), t1 = Trans (3), t2 = refTrans (3).
+__global__ void k_fit_solve(SolveArgs a, const double* shift) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= a.nframes) return;
+  const double* m = a.moments + 13 * f;
+  const double* K = a.consts;
+  const double W = K[4];
+  double U[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  double t1[3] = {0, 0, 0};
+  double r;
+  if (W < RMSD_SMALL) {
+    r = -1.0;
+  } else {
+    // centroid in shifted coordinates, then the centred moments
+    const double c0 = m[0] / W, c1 = m[1] / W, c2 = m[2] / W;
+    double R[9];
+    R[0] = m[3] - c0 * K[0]; R[1] = m[4] - c0 * K[1]; R[2] = m[5] - c0 * K[2];
+    R[3] = m[6] - c1 * K[0]; R[4] = m[7] - c1 * K[1]; R[5] = m[8] - c1 * K[2];
+    R[6] = m[9] - c2 * K[0]; R[7] = m[10] - c2 * K[1]; R[8] = m[11] - c2 * K[2];
+    const double sx = m[12] - W * (c0 * c0 + c1 * c1 + c2 * c2);
+    const double E0 = 0.5 * (sx + K[3]);
+    const rmsd_dev::QcpCoef co = rmsd_dev::qcp_coefficients(R);
+    const double lam = rmsd_dev::qcp_lambda(co, E0);
+    r = rmsd_dev::rmsd_from_lambda(E0, lam, W, a.clamp);
+    rmsd_dev::qcp_rotation(R, lam, U);
+    t1[0] = -(c0 + shift[3 * f]);
+    t1[1] = -(c1 + shift[3 * f + 1]);
+    t1[2] = -(c2 + shift[3 * f + 2]);
+  }
+  a.rmsd[f] = r;
+  if (a.U)
+    for (int k = 0; k < 9; ++k) a.U[9 * f + k] = U[k];
+  if (a.tvec) {
+    for (int k = 0; k < 3; ++k) {
+      a.tvec[6 * f + k] = t1[k];
+      a.tvec[6 * f + 3 + k] = a.ref_trans[k];
+    }
+  }
+  if (a.xform) {
+    double* x = a.xform + 15 * f;
+    for (int k = 0; k < 9; ++k) x[k] = U[k];
+    for (int k = 0; k < 3; ++k) {
+      x[9 + k] = t1[k];
+      x[12 + k] = a.ref_trans[k];
+    }
+  }
+}
+
+// ---- no-fit ------------------------------------------------------------------
+// sum w |y - x|^2 accumulated atom by atom per thread in the reference's
+// operation order (no contraction), then reduced over the block.
+template <bool WEIGHTED>
+__global__ __launch_bounds__(FIT_THREADS) void k_nofit_frames(NofitArgs a, int ldr,
+                                                              const double* consts, double* rmsd) {
+#pragma clang fp contract(off)
+  __shared__ double red[FIT_THREADS / 64];
+  const int64_t f = blockIdx.x;
+  double v[1] = {0.0};
+  for (int k = threadIdx.x; k < a.fs.nsel; k += blockDim.x) {
+    const double dx = a.ref[k] - (double)fcoord(a.fs, f, k, 0);
+    const double dy = a.ref[ldr + k] - (double)fcoord(a.fs, f, k, 1);
+    const double dz = a.ref[2 * ldr + k] - (double)fcoord(a.fs, f, k, 2);
+    const double w = WEIGHTED ? a.mass[k] : 1.0;
+    v[0] += (w * (dx * dx + dy * dy + dz * dz));
+  }
+  rmsd_dev::block_sum<1>(v, red);
+  if (threadIdx.x == 0) {
+    const double W = consts[4];
+    double r;
+    if (W < RMSD_SMALL) r = -1.0;
+    else if (v[0] < 0) r = 0.0;
+    else r = sqrt(v[0] / W);
+    rmsd[f] = r;
+  }
+}
+
+// ---- coordinate update -----------------------------------------------------------
+// mode 1: x' = U (x + t1) + t2 (Frame::Trans_Rot_Trans); mode 2: x' = x + (t1 + t2).
+__global__ __launch_bounds__(FIT_THREADS) void k_apply(ApplyArgs a, int mode) {
+#pragma clang fp contract(off)
+  const int64_t f = blockIdx.x;
+  const double* X = a.xform + 15 * f;
+  float* fr = a.xyz + f * a.stride;  // all stored frames, in order
+  for (int k = threadIdx.x; k < a.natom; k += blockDim.x) {
+    float* px = a.ld ? fr + k : fr + 3 * k;
+    float* py = a.ld ? fr + a.ld + k : fr + 3 * k + 1;
+    float* pz = a.ld ? fr + 2 * a.ld + k : fr + 3 * k + 2;
+    if (mode == 1) {
+      const double x = (double)*px + X[9];
+      const double y = (double)*py + X[10];
+      const double z = (double)*pz + X[11];
+      *px = (float)(x * X[0] + y * X[1] + z * X[2] + X[12]);
+      *py = (float)(x * X[3] + y * X[4] + z * X[5] + X[13]);
+      *pz = (float)(x * X[6] + y * X[7] + z * X[8] + X[14]);
+    } else {
+      *px = (float)((double)*px + (X[9] + X[12]));
+      *py = (float)((double)*py + (X[10] + X[13]));
+      *pz = (float)((double)*pz + (X[11] + X[14]));
+    }
+  }
+}
+
+}  // namespace
+
+hipError_t rmsd_launch_ref_prep(const FrameSetDev& fs, int f, const double* rmass,
+                                const double* tmass, int center, double* ref_out,
+                                double* ref_trans, double* consts, hipStream_t s) {
+  const int ldr = (fs.nsel + 3) & ~3;
+  hipLaunchKernelGGL(k_ref_prep, dim3(1), dim3(FIT_THREADS), 0, s, fs, f, rmass, tmass, center, ldr,
+                     ref_out, ref_trans, consts);
+  return hipGetLastError();
+}
+
+// moments must hold 13 doubles per frame followed by 3 per frame of shift.
+hipError_t rmsd_launch_fit_moments(const FitArgs& a, hipStream_t s) {
+  const int ldr = (a.fs.nsel + 3) & ~3;
+  const int nf = a.fs.nframes;
+  double* shift = a.moments + 13 * (int64_t)nf;
+  hipLaunchKernelGGL(k_fit_shift, dim3((nf + 255) / 256), dim3(256), 0, s, a.fs, shift);
+  const bool vec = a.fs.sel == nullptr && a.fs.ld > 0 && (a.fs.ld % 4) == 0 &&
+                   (a.fs.stride % 4) == 0 && ((uintptr_t)a.fs.xyz % 16) == 0;
+  const bool w = a.mass != nullptr;
+  if (vec) {
+    if (w) hipLaunchKernelGGL(k_fit_moments_soa4<true>, dim3(nf), dim3(FIT_THREADS), 0, s, a, ldr);
+    else hipLaunchKernelGGL(k_fit_moments_soa4<false>, dim3(nf), dim3(FIT_THREADS), 0, s, a, ldr);
+  } else {
+    if (w) hipLaunchKernelGGL(k_fit_moments_any<true>, dim3(nf), dim3(FIT_THREADS), 0, s, a, ldr);
+    else hipLaunchKernelGGL(k_fit_moments_any<false>, dim3(nf), dim3(FIT_THREADS), 0, s, a, ldr);
+  }
+  return hipGetLastError();
+}
+
+hipError_t rmsd_launch_fit_solve(const SolveArgs& a, hipStream_t s) {
+  const double* shift = a.moments + 13 * (int64_t)a.nframes;
+  hipLaunchKernelGGL(k_fit_solve, dim3((a.nframes + 127) / 128), dim3(128), 0, s, a, shift);
+  return hipGetLastError();
+}
+
+hipError_t rmsd_launch_nofit_frames(const NofitArgs& a, const double* consts, double* rmsd,
+                                    hipStream_t s) {
+  const int ldr = (a.fs.nsel + 3) & ~3;
+  if (a.mass)
+    hipLaunchKernelGGL(k_nofit_frames<true>, dim3(a.fs.nframes), dim3(FIT_THREADS), 0, s, a, ldr,
+                       consts, rmsd);
+  else
+    hipLaunchKernelGGL(k_nofit_frames<false>, dim3(a.fs.nframes), dim3(FIT_THREADS), 0, s, a, ldr,
+                       consts, rmsd);
+  return hipGetLastError();
+}
+
+hipError_t rmsd_launch_apply(const ApplyArgs& a, int mode, hipStream_t s) {
+  hipLaunchKernelGGL(k_apply, dim3(a.nframes), dim3(FIT_THREADS), 0, s, a, mode);
+  return hipGetLastError();
+}

@@ -1,0 +1,276 @@
+// k_pairwise.hip -- all-pairs best-fit RMSD on MFMA (gfx950).
+//
+// Replaces the O(F^2 N) loop nest of Analysis_Rms2d::Calculate_2D
+// (src/Analysis_Rms2d.cpp:247-287) and Cluster::MetricArray::calcFrameDistances
+// (src/Cluster/MetricArray.cpp:776-797).  The 3x3 covariances of all frame pairs
+// are one contraction  [3F x N] x [N x 3F]: for a 16x16 block of frame pairs and
+// a coordinate pair (c, d) the block  S_cd(i, j) = sum_k X_i[k][c] Y_j[k][d]  is
+// one chain of MFMA accumulations over the atoms k.  Each wave owns
+// (16*WR) x (16*WC) frame pairs and all nine (c, d) blocks, so every lane ends
+// the contraction holding the complete 3x3 covariance of each of its pairs and
+// runs the FP64 QCP epilogue in registers (no cross-lane shuffle).
+//
+// Operands come from the fragment-packed layout written by k_prep.hip: for a
+// block of 16 frames, one "chunk" of atoms and one coordinate, the 64 lanes of a
+// wave read 64 consecutive 16-byte pieces (one dwordx4 load per lane, 1 KiB per
+// wave, fully coalesced).  Lane l holds frame (l % 16) of the block and atoms
+// (l / 16) * CHUNK/4 ... of the chunk, so A (rows) and B (columns) share one
+// atom order and the contraction is an exact permutation of the atom sum.
+//
+// Precisions:
+//   PREC_F64   - centred, sqrt(w)-scaled FP64 coordinates, v_mfma_f64_16x16x4f64,
+//                FP64 accumulation.  8 atoms per chunk.
+//   PREC_F32   - per-frame pre-aligned deviations D from a template, FP32,
+//                v_mfma_f32_16x16x4f32, FP32 accumulation.  16 atoms per chunk.
+//   PREC_BF16X3- the same deviations split as hi + lo bfloat16,
+//                hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16, FP32
+//                accumulation.  32 atoms per chunk.
+// For the two MFMA-reduced forms the full covariance is reassembled in FP64:
+//   S(i,j) = P + u_j + u_i^T + D_i^T D_j,  P = T^T T,  u_f = T^T D_f
+// (T the template, all per-frame terms FP64), then E0 and QCP run in FP64.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "rmsd_device.hpp"
+#include "rmsd_kernels.hpp"
+
+namespace {
+
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+typedef __attribute__((ext_vector_type(4))) double f64x4;
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
+
+template <int PREC> struct PrecTraits;
+template <> struct PrecTraits<PREC_F64> {
+  typedef f64x4 acc_t;
+  static constexpr int PARTS = 1;
+};
+template <> struct PrecTraits<PREC_F32> {
+  typedef f32x4 acc_t;
+  static constexpr int PARTS = 1;
+};
+template <> struct PrecTraits<PREC_BF16X3> {
+  typedef f32x4 acc_t;
+  static constexpr int PARTS = 2;
+};
+
+// One 16-byte piece per lane (cached: row and column blocks are re-read by
+// neighbouring workgroups through L2).
+__device__ __forceinline__ u32x4 ld16(const u32x4* p) { return *p; }
+
+template <int PREC>
+__device__ __forceinline__ void mma(typename PrecTraits<PREC>::acc_t& acc, const u32x4* a,
+                                    const u32x4* b);
+
+template <>
+__device__ __forceinline__ void mma<PREC_F64>(f64x4& acc, const u32x4* a, const u32x4* b) {
+  const double2 av = __builtin_bit_cast(double2, a[0]);
+  const double2 bv = __builtin_bit_cast(double2, b[0]);
+  acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, bv.x, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av.y, bv.y, acc, 0, 0, 0);
+}
+
+template <>
+__device__ __forceinline__ void mma<PREC_F32>(f32x4& acc, const u32x4* a, const u32x4* b) {
+  const f32x4 av = __builtin_bit_cast(f32x4, a[0]);
+  const f32x4 bv = __builtin_bit_cast(f32x4, b[0]);
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bv.x, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bv.y, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bv.z, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bv.w, acc, 0, 0, 0);
+}
+
+template <>
+__device__ __forceinline__ void mma<PREC_BF16X3>(f32x4& acc, const u32x4* a, const u32x4* b) {
+  const bf16x8 ahi = __builtin_bit_cast(bf16x8, a[0]);
+  const bf16x8 alo = __builtin_bit_cast(bf16x8, a[1]);
+  const bf16x8 bhi = __builtin_bit_cast(bf16x8, b[0]);
+  const bf16x8 blo = __builtin_bit_cast(bf16x8, b[1]);
+  // Small terms first so that they are not absorbed by the large one.
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bhi, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, blo, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bhi, acc, 0, 0, 0);
+}
+
+template <int PREC>
+__device__ __forceinline__ double acc_elem(const typename PrecTraits<PREC>::acc_t& a, int v) {
+  return (double)a[v];
+}
+
+// Fragment pointer for (frame block fb, chunk kb, coordinate c, part p).
+__device__ __forceinline__ const u32x4* frag_ptr(const u32x4* base, int64_t fb, int kb,
+                                                 int KB, int c, int p, int PARTS) {
+  return base + ((((fb * KB + kb) * 3 + c) * PARTS + p) << 6);
+}
+
+template <int PREC, int WR, int WC, int NWR, int NWC>
+__global__ __launch_bounds__(64 * NWR * NWC) void k_pairwise(PairArgs args) {
+  typedef PrecTraits<PREC> T;
+  typedef typename T::acc_t acc_t;
+  constexpr int PARTS = T::PARTS;
+  constexpr int TILE_R = 16 * WR * NWR;
+  constexpr int TILE_C = 16 * WC * NWC;
+
+  // XCD-aware tile pick: the hardware deals workgroups round-robin over the
+  // XCDs, so XCD x receives ids x, x+NXCD, ...; give each XCD one contiguous
+  // run of the (grouped) tile list so that concurrently running workgroups on
+  // one XCD share row and column blocks in its L2.
+  const int w = blockIdx.x;
+  const int t = (w % args.nxcd) * args.per_xcd + (w / args.nxcd);
+  if (t >= args.ntiles) return;
+  const int2 tile = args.tiles[t];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wr = wave / NWC, wc = wave % NWC;
+  const int64_t fbA = (int64_t)tile.x * (TILE_R / 16) + wr * WR;  // first 16-frame row block
+  const int64_t fbB = (int64_t)tile.y * (TILE_C / 16) + wc * WC;  // first 16-frame col block
+
+  const u32x4* A = reinterpret_cast<const u32x4*>(args.A);
+  const u32x4* B = reinterpret_cast<const u32x4*>(args.B);
+  const int KB = args.KB;
+
+  acc_t acc[WR][WC][3][3];
+#pragma unroll
+  for (int r = 0; r < WR; ++r)
+#pragma unroll
+    for (int q = 0; q < WC; ++q)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) acc[r][q][c][d] = acc_t{0, 0, 0, 0};
+
+  u32x4 fa[WR][3][PARTS], fb[WC][3][PARTS];
+  auto load = [&](int kb) {
+#pragma unroll
+    for (int r = 0; r < WR; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int p = 0; p < PARTS; ++p)
+          fa[r][c][p] = ld16(frag_ptr(A, fbA + r, kb, KB, c, p, PARTS) + lane);
+#pragma unroll
+    for (int q = 0; q < WC; ++q)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int p = 0; p < PARTS; ++p)
+          fb[q][c][p] = ld16(frag_ptr(B, fbB + q, kb, KB, c, p, PARTS) + lane);
+  };
+
+  load(0);
+  for (int kb = 0; kb < KB; ++kb) {
+    u32x4 ca[WR][3][PARTS], cb[WC][3][PARTS];
+#pragma unroll
+    for (int r = 0; r < WR; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int p = 0; p < PARTS; ++p) ca[r][c][p] = fa[r][c][p];
+#pragma unroll
+    for (int q = 0; q < WC; ++q)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int p = 0; p < PARTS; ++p) cb[q][c][p] = fb[q][c][p];
+    if (kb + 1 < KB) load(kb + 1);
+#pragma unroll
+    for (int r = 0; r < WR; ++r)
+#pragma unroll
+      for (int q = 0; q < WC; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) mma<PREC>(acc[r][q][c][d], ca[r][c], cb[q][d]);
+  }
+
+  // ---- FP64 epilogue: reassemble S, E0, QCP, store ------------------------
+  const double M = args.total_mass[0];
+  double Pm[9];
+  if (PREC != PREC_F64) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Pm[k] = args.P[k];
+  }
+#pragma unroll
+  for (int q = 0; q < WC; ++q) {
+    const int64_t j = (fbB + q) * 16 + (lane & 15);  // column frame
+    double gj = 0.0, uj[9];
+    const bool jok = j < args.FB;
+    if (jok) {
+      gj = args.gB[j];
+      if (PREC != PREC_F64) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) uj[k] = args.uB[9 * j + k];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < WR; ++r) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        // Accumulator layout (checked on gfx950): FP32 16x16 blocks hold row
+        // 4*(lane/16) + v in item v; the FP64 16x16x4 block holds row
+        // (lane/16) + 4*v.  Columns are lane % 16 in both.
+        const int rr = PREC == PREC_F64 ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v;
+        const int64_t i = (fbA + r) * 16 + rr;  // row frame
+        const bool ok = jok && i < args.FA && (!args.tri || i < j);
+        if (!ok) continue;
+        double S[9];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) S[3 * c + d] = acc_elem<PREC>(acc[r][q][c][d], v);
+        if (PREC != PREC_F64) {
+          const double* ui = args.uA + 9 * i;
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+              S[3 * c + d] += Pm[3 * c + d] + uj[3 * c + d] + ui[3 * d + c];
+        }
+        // Row frames are the reference in the triangle layout (the reference
+        // stores element (nref, ntgt) with ntgt > nref) and the target in the
+        // full layout; QCP takes R = sum w t r^T.
+        double R[9];
+        if (args.tri) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) R[3 * c + d] = S[3 * d + c];
+        } else {
+#pragma unroll
+          for (int k = 0; k < 9; ++k) R[k] = S[k];
+        }
+        const double E0 = 0.5 * (args.gA[i] + gj);
+        float out;
+        if (M < RMSD_SMALL)
+          out = -1.0f;
+        else
+          out = (float)rmsd_dev::qcp_rmsd(R, E0, M, args.clamp);
+        const int64_t idx = args.tri ? rmsd_dev::tri_index(args.FA, i, j) : i * args.FB + j;
+        args.out[idx] = out;
+      }
+    }
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launchers.  Tile shapes: FP64 waves own 16x32 pairs (72 FP64 accumulators),
+// the FP32 / BF16 waves 32x32 pairs (144 FP32 accumulators); every workgroup
+// covers a 64x64 block of frame pairs.
+template <int PREC, int WR, int WC, int NWR, int NWC>
+static hipError_t launch_tiles(const PairArgs& a, hipStream_t s) {
+  const int nblocks = a.per_xcd * a.nxcd;
+  hipLaunchKernelGGL((k_pairwise<PREC, WR, WC, NWR, NWC>), dim3(nblocks), dim3(64 * NWR * NWC), 0,
+                     s, a);
+  return hipGetLastError();
+}
+
+hipError_t rmsd_launch_pairwise(int prec, const PairArgs& a, hipStream_t s) {
+  switch (prec) {
+    case PREC_F64: return launch_tiles<PREC_F64, 1, 2, 4, 2>(a, s);
+    case PREC_F32: return launch_tiles<PREC_F32, 2, 2, 2, 2>(a, s);
+    case PREC_BF16X3: return launch_tiles<PREC_BF16X3, 2, 2, 2, 2>(a, s);
+    default: return hipErrorInvalidValue;
+  }
+}

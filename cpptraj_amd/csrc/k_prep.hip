@@ -1,0 +1,232 @@
+// k_prep.hip -- per-frame stage of the pairwise RMSD: centroid, E0 term and the
+// fragment-packed operands of the MFMA contraction.
+//
+// Replaces the per-pair work the reference repeats inside its loop nest
+// (Analysis_Rms2d::Calculate_2D, src/Analysis_Rms2d.cpp:262-275): the masked
+// float->double gather (CompactFrameArray::GetToMaskDblPtr,
+// src/CompactFrameArray.cpp:244-262), centring (Frame::CenterOnOrigin,
+// src/Frame.cpp:1043-1055; the target centroid in Frame::RMSD_CenteredRef,
+// src/Frame.cpp:1141-1171) and the E0 half-sums (src/Frame.cpp:1194).  These
+// depend only on the frame, so they run once per frame here.
+//
+// One workgroup per frame.  All sums are FP64.  With weights the coordinates
+// are stored pre-multiplied by sqrt(w), so sum_k w_k t_k r_k^T becomes a plain
+// dot product of the stored rows.
+//
+// For the MFMA-reduced precisions each frame is additionally rotated (FP64
+// QCP) onto a common template T and only the deviation D = U x~ - T is
+// stored at reduced precision.  The best-fit RMSD of a pair is unchanged by
+// rigid motions of either frame, and D is a few Angstrom where x~ is tens, so
+// the reduced-precision contraction only sees small numbers.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "rmsd_device.hpp"
+#include "rmsd_kernels.hpp"
+
+namespace {
+
+constexpr int PREP_THREADS = 256;
+
+__device__ __forceinline__ float load_coord(const FrameSetDev& fs, int64_t f, int k, int c) {
+  const int a = fs.sel ? fs.sel[k] : k;
+  const float* fr = fs.xyz + frame_of(fs, f) * fs.stride;
+  return fs.ld ? fr[c * fs.ld + a] : fr[3 * a + c];
+}
+
+template <int PREC>
+__device__ __forceinline__ int64_t packed_elem(int64_t f, int k, int c, int p, int KB) {
+  constexpr int CH = chunk_atoms(PREC);
+  constexpr int PER = CH / 4;
+  constexpr int PARTS = chunk_parts(PREC);
+  const int64_t fb = f >> 4;
+  const int r = (int)(f & 15);
+  const int kb = k / CH, kk = k % CH;
+  const int lane = r + 16 * (kk / PER);
+  const int e = kk % PER;
+  const int64_t piece = ((((fb * KB + kb) * 3 + c) * PARTS + p) << 6) + lane;
+  return piece * PER + e;
+}
+
+// Weighted centroid of frame f: returns (sum w x, sum w).
+__device__ __forceinline__ void centroid(const FrameSetDev& fs, int64_t f, const double* w,
+                                         double* red, double cen[3]) {
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int k = threadIdx.x; k < fs.nsel; k += blockDim.x) {
+    const double m = w ? w[k] : 1.0;
+    v[0] += m * (double)load_coord(fs, f, k, 0);
+    v[1] += m * (double)load_coord(fs, f, k, 1);
+    v[2] += m * (double)load_coord(fs, f, k, 2);
+    v[3] += m;
+  }
+  rmsd_dev::block_sum<4>(v, red);
+  const double tot = w ? v[3] : (double)fs.nsel;
+  const double inv = tot != 0.0 ? 1.0 / tot : 0.0;
+  cen[0] = v[0] * inv;
+  cen[1] = v[1] * inv;
+  cen[2] = v[2] * inv;
+}
+
+__device__ __forceinline__ void centred(const FrameSetDev& fs, int64_t f, int k, const double cen[3],
+                                        const double* sw, double x[3]) {
+  if (k < fs.nsel) {
+    const double s = sw ? sqrt(sw[k]) : 1.0;
+    x[0] = ((double)load_coord(fs, f, k, 0) - cen[0]) * s;
+    x[1] = ((double)load_coord(fs, f, k, 1) - cen[1]) * s;
+    x[2] = ((double)load_coord(fs, f, k, 2) - cen[2]) * s;
+  } else {
+    x[0] = x[1] = x[2] = 0.0;
+  }
+}
+
+// ---- template / total weight -----------------------------------------------
+__global__ __launch_bounds__(PREP_THREADS) void k_setup(SetupArgs a) {
+  __shared__ double red[16 * (PREP_THREADS / 64)];
+  if (threadIdx.x == 0) {
+    // Sequential running total, rounded as the reference's per-pair sum.
+    double tw = 0.0;
+    if (a.smass) {
+      for (int k = 0; k < a.fs.nsel; ++k) tw += a.smass[k];
+    } else {
+      tw = (double)a.fs.nsel;
+    }
+    a.total_mass[0] = tw;
+  }
+  if (!a.T) return;
+  double cen[3];
+  centroid(a.fs, a.f0, a.cmass, red, cen);
+  double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int k = threadIdx.x; k < a.Kp; k += blockDim.x) {
+    double x[3];
+    centred(a.fs, a.f0, k, cen, a.smass, x);
+    a.T[k] = x[0];
+    a.T[a.Kp + k] = x[1];
+    a.T[2 * a.Kp + k] = x[2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) v[3 * c + d] += x[c] * x[d];
+    v[9] += x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+  }
+  rmsd_dev::block_sum<10>(v, red);
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 9; ++k) a.P[k] = v[k];
+    a.gT[0] = v[9];
+  }
+}
+
+// ---- per-frame preparation -----------------------------------------------------
+template <int PREC>
+__global__ __launch_bounds__(PREP_THREADS) void k_prep(PrepArgs a) {
+  __shared__ double red[16 * (PREP_THREADS / 64)];
+  __shared__ double Ush[9];
+  const int64_t f = blockIdx.x;
+  const int Kp = a.Kp;
+
+  if (f >= a.fs.nframes) {
+    // Padding frames: zero operands so that padded rows and columns of a tile
+    // contribute nothing.
+    for (int k = threadIdx.x; k < Kp; k += blockDim.x)
+      for (int c = 0; c < 3; ++c)
+        for (int p = 0; p < chunk_parts(PREC); ++p) {
+          const int64_t e = packed_elem<PREC>(f, k, c, p, a.KB);
+          if (PREC == PREC_F64) reinterpret_cast<double*>(a.packed)[e] = 0.0;
+          else if (PREC == PREC_F32) reinterpret_cast<float*>(a.packed)[e] = 0.0f;
+          else reinterpret_cast<__bf16*>(a.packed)[e] = (__bf16)0.0f;
+        }
+    return;
+  }
+
+  double cen[3];
+  centroid(a.fs, f, a.cmass, red, cen);
+
+  if (PREC == PREC_F64) {
+    double g[1] = {0.0};
+    double* out = reinterpret_cast<double*>(a.packed);
+    for (int k = threadIdx.x; k < Kp; k += blockDim.x) {
+      double x[3];
+      centred(a.fs, f, k, cen, a.smass, x);
+      out[packed_elem<PREC>(f, k, 0, 0, a.KB)] = x[0];
+      out[packed_elem<PREC>(f, k, 1, 0, a.KB)] = x[1];
+      out[packed_elem<PREC>(f, k, 2, 0, a.KB)] = x[2];
+      g[0] += x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+    }
+    rmsd_dev::block_sum<1>(g, red);
+    if (threadIdx.x == 0) a.g[f] = g[0];
+    return;
+  }
+
+  // MFMA-reduced forms: align to the template in FP64.
+  double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int k = threadIdx.x; k < a.fs.nsel; k += blockDim.x) {
+    double x[3];
+    centred(a.fs, f, k, cen, a.smass, x);
+    const double t0 = a.T[k], t1 = a.T[Kp + k], t2 = a.T[2 * Kp + k];
+    v[0] += x[0] * t0; v[1] += x[0] * t1; v[2] += x[0] * t2;
+    v[3] += x[1] * t0; v[4] += x[1] * t1; v[5] += x[1] * t2;
+    v[6] += x[2] * t0; v[7] += x[2] * t1; v[8] += x[2] * t2;
+    v[9] += x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+  }
+  rmsd_dev::block_sum<10>(v, red);
+  if (threadIdx.x == 0) {
+    a.g[f] = v[9];
+    const double E0 = 0.5 * (v[9] + a.gT[0]);
+    const rmsd_dev::QcpCoef co = rmsd_dev::qcp_coefficients(v);
+    const double lam = rmsd_dev::qcp_lambda(co, E0);
+    double U[9];
+    rmsd_dev::qcp_rotation(v, lam, U);
+    for (int k = 0; k < 9; ++k) Ush[k] = U[k];
+  }
+  __syncthreads();
+  double U[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) U[k] = Ush[k];
+
+  double u[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int k = threadIdx.x; k < Kp; k += blockDim.x) {
+    double x[3];
+    centred(a.fs, f, k, cen, a.smass, x);
+    const double t[3] = {a.T[k], a.T[Kp + k], a.T[2 * Kp + k]};
+    double D[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      D[c] = U[3 * c] * x[0] + U[3 * c + 1] * x[1] + U[3 * c + 2] * x[2] - t[c];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) u[3 * c + d] += t[c] * D[d];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (PREC == PREC_F32) {
+        reinterpret_cast<float*>(a.packed)[packed_elem<PREC>(f, k, c, 0, a.KB)] = (float)D[c];
+      } else {
+        const __bf16 hi = (__bf16)(float)D[c];
+        const __bf16 lo = (__bf16)(float)(D[c] - (double)(float)hi);
+        __bf16* out = reinterpret_cast<__bf16*>(a.packed);
+        out[packed_elem<PREC>(f, k, c, 0, a.KB)] = hi;
+        out[packed_elem<PREC>(f, k, c, 1, a.KB)] = lo;
+      }
+    }
+  }
+  rmsd_dev::block_sum<9>(u, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 9; ++k) a.u[9 * f + k] = u[k];
+}
+
+}  // namespace
+
+hipError_t rmsd_launch_setup(const SetupArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_setup, dim3(1), dim3(PREP_THREADS), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t rmsd_launch_prep(const PrepArgs& a, hipStream_t s) {
+  switch (a.prec) {
+    case PREC_F64: hipLaunchKernelGGL(k_prep<PREC_F64>, dim3(a.Fp), dim3(PREP_THREADS), 0, s, a); break;
+    case PREC_F32: hipLaunchKernelGGL(k_prep<PREC_F32>, dim3(a.Fp), dim3(PREP_THREADS), 0, s, a); break;
+    case PREC_BF16X3:
+      hipLaunchKernelGGL(k_prep<PREC_BF16X3>, dim3(a.Fp), dim3(PREP_THREADS), 0, s, a);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}

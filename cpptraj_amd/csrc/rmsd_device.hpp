@@ -1,0 +1,193 @@
+// rmsd_device.hpp -- device-side FP64 math shared by the RMSD kernels (gfx950).
+//
+// * qcp_lambda / qcp_rmsd: largest eigenvalue of the QCP key matrix by Newton
+//   iteration from E0, following the quaternion characteristic polynomial of
+//   Theobald (Acta Cryst. A61:478, 2005) and Liu, Agrafiotis & Theobald
+//   (J. Comput. Chem. 31:1561, 2010), with the same coefficient formulas,
+//   Newton start, stopping rule (|step| < 1e-11 lambda, <= 50 steps) and
+//   adjugate-column eigenvector fallbacks as the reference's
+//   Cpptraj::QCPRot::FastCalcRMSDAndRotation (src/qcprot.cpp:167-343).
+// * qcp_rotation: the optimal rotation from the same eigenvector, returned in
+//   the Kabsch convention of Frame::RMSD_CenteredRef (src/Frame.cpp:1247-1258):
+//   x' = U x maps the centred target onto the centred reference.
+// * wave / block reductions for doubles on 64-wide wavefronts.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define RMSD_SMALL 1.0e-14  // Constants::SMALL, src/Constants.h:35
+
+namespace rmsd_dev {
+
+struct QcpCoef {
+  double c0, c1, c2;
+};
+
+// Quartic coefficients of the QCP key matrix for covariance S (row-major,
+// S_ij = sum_k w_k t_ki r_kj).
+__device__ __forceinline__ QcpCoef qcp_coefficients(const double* S) {
+  const double Sxx = S[0], Sxy = S[1], Sxz = S[2];
+  const double Syx = S[3], Syy = S[4], Syz = S[5];
+  const double Szx = S[6], Szy = S[7], Szz = S[8];
+  const double Sxx2 = Sxx * Sxx, Syy2 = Syy * Syy, Szz2 = Szz * Szz;
+  const double Sxy2 = Sxy * Sxy, Syz2 = Syz * Syz, Sxz2 = Sxz * Sxz;
+  const double Syx2 = Syx * Syx, Szy2 = Szy * Szy, Szx2 = Szx * Szx;
+  const double pYZ = 2.0 * (Syz * Szy - Syy * Szz);
+  const double qD = Syy2 + Szz2 - Sxx2 + Syz2 + Szy2;
+  QcpCoef k;
+  k.c2 = -2.0 * (Sxx2 + Syy2 + Szz2 + Sxy2 + Syx2 + Sxz2 + Szx2 + Syz2 + Szy2);
+  k.c1 = 8.0 * (Sxx * Syz * Szy + Syy * Szx * Sxz + Szz * Sxy * Syx -
+                Sxx * Syy * Szz - Syz * Szx * Sxy - Szy * Syx * Sxz);
+  const double xzP = Sxz + Szx, yzP = Syz + Szy, xyP = Sxy + Syx;
+  const double yzM = Syz - Szy, xzM = Sxz - Szx, xyM = Sxy - Syx;
+  const double xxPyy = Sxx + Syy, xxMyy = Sxx - Syy;
+  const double od = Sxy2 + Sxz2 - Syx2 - Szx2;
+  k.c0 = od * od + (qD + pYZ) * (qD - pYZ) +
+         (-(xzP) * (yzM) + (xyM) * (xxMyy - Szz)) * (-(xzM) * (yzP) + (xyM) * (xxMyy + Szz)) +
+         (-(xzP) * (yzP) - (xyP) * (xxPyy - Szz)) * (-(xzM) * (yzM) - (xyP) * (xxPyy + Szz)) +
+         (+(xyP) * (yzP) + (xzP) * (xxMyy + Szz)) * (-(xyM) * (yzM) + (xzP) * (xxPyy + Szz)) +
+         (+(xyP) * (yzM) + (xzM) * (xxMyy - Szz)) * (-(xyM) * (yzP) + (xzM) * (xxPyy - Szz));
+  return k;
+}
+
+// Newton iteration for the largest root, started at E0.
+__device__ __forceinline__ double qcp_lambda(const QcpCoef& k, double E0) {
+  double lam = E0;
+  for (int it = 0; it < 50; ++it) {
+    const double prev = lam;
+    const double l2 = lam * lam;
+    const double b = (l2 + k.c2) * lam;
+    const double a = b + k.c1;
+    lam -= (a * lam + k.c0) / (2.0 * l2 * lam + b + a);
+    if (fabs(lam - prev) < fabs(1e-11 * lam)) break;
+  }
+  return lam;
+}
+
+// RMSD from E0 - lambda.  clamp != 0 reproduces the Kabsch path's
+// "negative -> 0" (src/Frame.cpp:1262-1267), which also yields 0 where every
+// atom sits on the centroid (one atom: E0 = 0 and Newton's 0/0); otherwise the
+// QCP path's fabs (src/qcprot.cpp:246), NaN included.
+__device__ __forceinline__ double rmsd_from_lambda(double E0, double lam, double M,
+                                                   int clamp) {
+  const double e = E0 - lam;
+  if (clamp) return e > 0.0 ? sqrt((2.0 * e) / M) : 0.0;
+  return sqrt(fabs(2.0 * e / M));
+}
+
+__device__ __forceinline__ double qcp_rmsd(const double* S, double E0, double M,
+                                           int clamp) {
+  const QcpCoef k = qcp_coefficients(S);
+  return rmsd_from_lambda(E0, qcp_lambda(k, E0), M, clamp);
+}
+
+// Optimal rotation for covariance S at eigenvalue lam, written as the Kabsch
+// U (x' = U x for the centred target).  Returns 0 when every adjugate column
+// is degenerate and the identity is returned (QCP fallback), 1 otherwise.
+__device__ __forceinline__ int qcp_rotation(const double* S, double lam, double* U) {
+  const double Sxx = S[0], Sxy = S[1], Sxz = S[2];
+  const double Syx = S[3], Syy = S[4], Syz = S[5];
+  const double Szx = S[6], Szy = S[7], Szz = S[8];
+  const double xzP = Sxz + Szx, yzP = Syz + Szy, xyP = Sxy + Syx;
+  const double yzM = Syz - Szy, xzM = Sxz - Szx, xyM = Sxy - Syx;
+  const double xxPyy = Sxx + Syy, xxMyy = Sxx - Syy;
+  const double a11 = xxPyy + Szz - lam, a12 = yzM, a13 = -xzM, a14 = xyM;
+  const double a21 = yzM, a22 = xxMyy - Szz - lam, a23 = xyP, a24 = xzP;
+  const double a31 = a13, a32 = a23, a33 = Syy - Sxx - Szz - lam, a34 = yzP;
+  const double a41 = a14, a42 = a24, a43 = a34, a44 = Szz - xxPyy - lam;
+  const double m3344 = a33 * a44 - a43 * a34, m3244 = a32 * a44 - a42 * a34;
+  const double m3243 = a32 * a43 - a42 * a33, m3143 = a31 * a43 - a41 * a33;
+  const double m3144 = a31 * a44 - a41 * a34, m3142 = a31 * a42 - a41 * a32;
+  double q1 = a22 * m3344 - a23 * m3244 + a24 * m3243;
+  double q2 = -a21 * m3344 + a23 * m3144 - a24 * m3143;
+  double q3 = a21 * m3244 - a22 * m3144 + a24 * m3142;
+  double q4 = -a21 * m3243 + a22 * m3143 - a23 * m3142;
+  double qq = q1 * q1 + q2 * q2 + q3 * q3 + q4 * q4;
+  const double eps = 1e-6;
+  if (qq < eps) {
+    q1 = a12 * m3344 - a13 * m3244 + a14 * m3243;
+    q2 = -a11 * m3344 + a13 * m3144 - a14 * m3143;
+    q3 = a11 * m3244 - a12 * m3144 + a14 * m3142;
+    q4 = -a11 * m3243 + a12 * m3143 - a13 * m3142;
+    qq = q1 * q1 + q2 * q2 + q3 * q3 + q4 * q4;
+    if (qq < eps) {
+      const double m1324 = a13 * a24 - a14 * a23, m1224 = a12 * a24 - a14 * a22;
+      const double m1223 = a12 * a23 - a13 * a22, m1124 = a11 * a24 - a14 * a21;
+      const double m1123 = a11 * a23 - a13 * a21, m1122 = a11 * a22 - a12 * a21;
+      q1 = a42 * m1324 - a43 * m1224 + a44 * m1223;
+      q2 = -a41 * m1324 + a43 * m1124 - a44 * m1123;
+      q3 = a41 * m1224 - a42 * m1124 + a44 * m1122;
+      q4 = -a41 * m1223 + a42 * m1123 - a43 * m1122;
+      qq = q1 * q1 + q2 * q2 + q3 * q3 + q4 * q4;
+      if (qq < eps) {
+        q1 = a32 * m1324 - a33 * m1224 + a34 * m1223;
+        q2 = -a31 * m1324 + a33 * m1124 - a34 * m1123;
+        q3 = a31 * m1224 - a32 * m1124 + a34 * m1122;
+        q4 = -a31 * m1223 + a32 * m1123 - a33 * m1122;
+        qq = q1 * q1 + q2 * q2 + q3 * q3 + q4 * q4;
+        if (qq < eps) {
+          U[0] = U[4] = U[8] = 1.0;
+          U[1] = U[2] = U[3] = U[5] = U[6] = U[7] = 0.0;
+          return 0;
+        }
+      }
+    }
+  }
+  const double nq = sqrt(qq);
+  q1 /= nq;
+  q2 /= nq;
+  q3 /= nq;
+  q4 /= nq;
+  const double w2 = q1 * q1, x2 = q2 * q2, y2 = q3 * q3, z2 = q4 * q4;
+  const double xy = q2 * q3, wz = q1 * q4, zx = q4 * q2, wy = q1 * q3, yz = q3 * q4,
+               wx = q1 * q2;
+  // QCP's matrix (rows) is the transpose of the Kabsch U; store U directly.
+  U[0] = w2 + x2 - y2 - z2;
+  U[3] = 2 * (xy + wz);
+  U[6] = 2 * (zx - wy);
+  U[1] = 2 * (xy - wz);
+  U[4] = w2 - x2 + y2 - z2;
+  U[7] = 2 * (yz + wx);
+  U[2] = 2 * (zx + wy);
+  U[5] = 2 * (yz - wx);
+  U[8] = w2 - x2 - y2 + z2;
+  return 1;
+}
+
+// ---- reductions -----------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sums NV doubles per thread over the whole block.  Every thread receives the
+// totals.  red must hold NV * (blockDim.x / 64) doubles of shared memory.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* red) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) red[wave * NV + i] = v[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += red[w * NV + i];
+    v[i] = s;
+  }
+  __syncthreads();
+}
+
+// Packed upper-triangle index without the diagonal (Matrix<T>::calcTriIndex,
+// src/Matrix.h:110-122), for i < j.
+__device__ __forceinline__ int64_t tri_index(int64_t n, int64_t i, int64_t j) {
+  return n * i - ((i + 1) * i) / 2 + j - i - 1;
+}
+
+}  // namespace rmsd_dev

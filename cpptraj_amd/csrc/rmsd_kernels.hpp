@@ -1,0 +1,134 @@
+// rmsd_kernels.hpp -- argument blocks and launchers shared by the HIP kernels
+// and the C ABI implementation (capi.hip).  Internal: not part of the public
+// header include/cpptraj_rmsd.h.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+enum { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16X3 = 2 };
+
+// Atoms per fragment chunk for each precision: one 16-byte piece per lane
+// holds 2 doubles, 4 floats or 8 bfloat16 (x 4 lane groups).
+inline constexpr int chunk_atoms(int prec) {
+  return prec == PREC_F64 ? 8 : (prec == PREC_F32 ? 16 : 32);
+}
+inline constexpr int chunk_parts(int prec) { return prec == PREC_BF16X3 ? 2 : 1; }
+
+// A set of frames as cpptraj stores them.  AoS (ld == 0): coordinate c of atom
+// a in frame f at xyz[f*stride + 3a + c] (CompactFrameArray / Frame layout).
+// SoA (ld > 0): at xyz[f*stride + c*ld + a].  sel (may be null) lists the
+// selected atoms; nsel atoms are used.
+struct FrameSetDev {
+  const float* xyz;
+  int64_t stride;
+  int64_t ld;
+  int nframes;
+  const int* sel;
+  int nsel;
+  const int* fidx;  // frame f of the set is stored frame fidx[f] (null: f)
+};
+
+__host__ __device__ inline int64_t frame_of(const FrameSetDev& fs, int64_t f) {
+  return fs.fidx ? (int64_t)fs.fidx[f] : f;
+}
+
+// Per-frame preparation for the pairwise contraction (k_prep.hip).
+struct PrepArgs {
+  FrameSetDev fs;
+  const double* cmass;  // centroid weights (null: geometric centre)
+  const double* smass;  // covariance weights (null: unit)
+  int prec;
+  int KB;               // chunks along the atoms
+  int Fp;               // padded frame count (multiple of 64)
+  void* packed;         // fragment-packed output
+  double* g;            // per frame: sum w |x~|^2
+  double* u;            // per frame (MFMA forms): T^T D, 9 doubles
+  const double* T;      // template, SoA [3][Kp] FP64 (MFMA forms)
+  int Kp;
+  const double* gT;     // sum |T|^2 (device scalar)
+};
+
+struct SetupArgs {
+  FrameSetDev fs;       // template source set
+  int f0;               // template frame
+  const double* cmass;
+  const double* smass;
+  int Kp;
+  double* T;            // out: [3][Kp] (may be null when no template is needed)
+  double* P;            // out: T^T T (9)
+  double* gT;           // out: sum |T|^2
+  double* total_mass;   // out: sum w (or nsel)
+};
+
+struct PairArgs {
+  const void* A;        // packed row frames
+  const void* B;        // packed column frames
+  int KB;
+  const double* gA;
+  const double* gB;
+  const double* uA;
+  const double* uB;
+  const double* P;
+  const double* total_mass;
+  const int2* tiles;    // (row tile, column tile), grouped order
+  int ntiles;
+  int per_xcd;
+  int nxcd;
+  int FA, FB;           // real frame counts
+  int tri;              // packed upper triangle (A == B); else full FA x FB
+  int clamp;            // 1: Kabsch-style clamp at 0; 0: QCP-style fabs
+  float* out;
+};
+
+// Streaming fit of every frame to one reference (k_fit.hip).
+struct FitArgs {
+  FrameSetDev fs;
+  const double* mass;    // covariance / centroid weights of the target (null: unit)
+  const double* ref;     // centred reference, SoA [3][nsel] FP64
+  const double* consts;  // [0..2] sum w y, [3] sum w |y|^2, [4] total weight
+  double* moments;       // out: 13 per frame, then 3 per frame of shift
+};
+
+struct SolveArgs {
+  const double* moments;
+  const double* consts;
+  const double* ref_trans;  // origin -> reference centre (3)
+  int nframes;
+  int fit;                  // 0: no-fit RMSD
+  int clamp;
+  double* rmsd;
+  double* U;                // 9 per frame or null
+  double* tvec;             // 6 per frame or null
+  double* xform;            // 15 per frame (U, Trans, refTrans) for the coordinate update, or null
+};
+
+struct NofitArgs {
+  FrameSetDev fs;
+  const double* mass;
+  const double* ref;     // SoA [3][ldr] FP64, not centred
+};
+
+struct ApplyArgs {
+  float* xyz;             // frames to rewrite in place (all natom atoms)
+  int64_t stride;
+  int64_t ld;
+  int natom;
+  int nframes;
+  const double* xform;    // 15 per frame: U, t1, t2 with x' = U (x + t1) + t2
+};
+
+hipError_t rmsd_launch_setup(const SetupArgs& a, hipStream_t s);
+hipError_t rmsd_launch_prep(const PrepArgs& a, hipStream_t s);
+hipError_t rmsd_launch_pairwise(int prec, const PairArgs& a, hipStream_t s);
+hipError_t rmsd_launch_pair_nofit_tiles(const FrameSetDev& A, const FrameSetDev& B,
+                                        const double* mass, const double* total_mass,
+                                        const int2* tiles, int ntiles, int per_xcd, int nxcd,
+                                        int tri, float* out, hipStream_t s);
+hipError_t rmsd_launch_fit_moments(const FitArgs& a, hipStream_t s);
+hipError_t rmsd_launch_fit_solve(const SolveArgs& a, hipStream_t s);
+hipError_t rmsd_launch_nofit_frames(const NofitArgs& a, const double* consts, double* rmsd,
+                                    hipStream_t s);
+hipError_t rmsd_launch_apply(const ApplyArgs& a, int mode, hipStream_t s);
+hipError_t rmsd_launch_ref_prep(const FrameSetDev& fs, int f, const double* rmass,
+                                const double* tmass, int center, double* ref_out,
+                                double* ref_trans, double* consts, hipStream_t s);

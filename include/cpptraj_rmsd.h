@@ -1,0 +1,150 @@
+/*
+ * cpptraj_rmsd.h -- C ABI of libcpptraj_rmsd.so, the MI355X (gfx950) best-fit
+ * RMSD library.
+ *
+ * Drop-in boundary for cpptraj's best-fit RMSD path.  Each entry point names
+ * the reference loop or function it replaces (paths relative to the cpptraj
+ * tree).  The caller keeps its own option handling (masks, 'mass', 'nofit',
+ * 'qrmsd', reference modes) and passes the resolved atom selection, masses and
+ * frame storage; the library owns the loop nest and the device work.
+ *
+ * Conventions
+ *   - Every array argument is a device pointer (HBM) unless noted.  The caller
+ *     owns all input and output buffers; the library owns the scratch memory of
+ *     its context.  Nothing is allocated or freed across the boundary.
+ *   - Frames are FP32, as cpptraj's COORDS sets store them
+ *     (CompactFrameArray, src/CompactFrameArray.h:75-76).  Internally every
+ *     sum, centroid, E0 and the QCP step are FP64.
+ *   - 3x3 matrices are row-major doubles.  U is the Kabsch rotation of
+ *     Frame::RMSD_CenteredRef (src/Frame.cpp:1247-1258): x' = U x maps the
+ *     centred target onto the centred reference.
+ *   - Output matrices use the packed layouts of Matrix<float>
+ *     (src/Matrix.h:94-122): TRIANGLE index n*i - (i+1)*i/2 + j - i - 1 for
+ *     i < j, FULL index ntgt*nref_total + nref.
+ *   - Functions return RMSD_OK (0) or an error code; rmsd_last_error() gives
+ *     the message.  Numerical sentinels follow the reference: -1 when the total
+ *     mass is below 1e-14 (src/Frame.cpp:1160-1163), 0 for a negative
+ *     residual in the fit mode (src/Frame.cpp:1264-1267).
+ *   - A context is used by one host thread at a time; different contexts are
+ *     independent (one HIP stream each) and may be used concurrently.
+ */
+#ifndef CPPTRAJ_RMSD_H
+#define CPPTRAJ_RMSD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RMSD_OK 0
+#define RMSD_ERR_ARG 1
+#define RMSD_ERR_DEVICE 2
+#define RMSD_ERR_ALLOC 3
+
+/* Distance metric.  Analysis_Rms2d modes RMS_FIT, RMS_NOFIT and QUAT
+ * (src/Analysis_Rms2d.cpp:33-35, :275-281); the cluster metric's
+ * fit / nofit (src/Cluster/Metric_RMS.cpp:44-63). */
+#define RMSD_MODE_FIT 0
+#define RMSD_MODE_NOFIT 1
+#define RMSD_MODE_QUAT 2
+
+/* Arithmetic of the pairwise covariance contraction.  FP64 is the plain
+ * kernel; FP32 and BF16X3 (bfloat16 hi+lo split, three products, FP32
+ * accumulation) run on MFMA over per-frame pre-aligned deviations.  E0,
+ * centroids and QCP are FP64 in every case. */
+#define RMSD_PREC_FP64 0
+#define RMSD_PREC_FP32 1
+#define RMSD_PREC_BF16X3 2
+
+/* Coordinate update of the fit (Action_Rmsd::DoAction, src/Action_Rmsd.cpp:
+ * 378-389): none ('nomod'), rotate + translate (default), translate only
+ * ('norotate'). */
+#define RMSD_MODIFY_NONE 0
+#define RMSD_MODIFY_ROT_TRANS 1
+#define RMSD_MODIFY_TRANS 2
+
+/* A set of frames in device memory.
+ *   AoS (ld == 0): coordinate c of atom a of stored frame f at
+ *                  xyz[f*stride + 3*a + c]   (cpptraj CRD frames)
+ *   SoA (ld >  0): at xyz[f*stride + c*ld + a]
+ * sel     nsel selected atom indices (AtomMask::Selected(), src/AtomMask.h:32),
+ *         or NULL for atoms 0..nsel-1.
+ * mass    nsel masses of the selected atoms (Frame::SetupFrameFromMask,
+ *         src/Frame.cpp:502-512), or NULL; used only with use_mass.
+ * fidx    nframes stored-frame indices (a sieved or reordered frame list, as
+ *         in Cluster::Cframes), or NULL for frames 0..nframes-1. */
+typedef struct rmsd_frames {
+  const float* xyz;
+  int64_t stride;
+  int64_t ld;
+  int32_t nframes;
+  int32_t natom;
+  const int32_t* sel;
+  int32_t nsel;
+  const double* mass;
+  const int32_t* fidx;
+} rmsd_frames;
+
+typedef struct rmsd_ctx rmsd_ctx;
+
+/* Context on HIP device `device` using stream `stream` (a hipStream_t, or NULL
+ * for the null stream).  No reference counterpart: holds the stream, scratch
+ * buffers and tile schedules that cpptraj would re-create per call. */
+int rmsd_ctx_create(rmsd_ctx** ctx, int device, void* stream);
+void rmsd_ctx_destroy(rmsd_ctx* ctx);
+const char* rmsd_last_error(const rmsd_ctx* ctx);
+/* Blocks until the context's stream is idle. */
+int rmsd_ctx_synchronize(rmsd_ctx* ctx);
+/* Bytes of device scratch currently held by the context. */
+int64_t rmsd_ctx_scratch_bytes(const rmsd_ctx* ctx);
+
+/* All-pairs RMSD matrix.
+ * Replaces the nref/ntgt loop nest of Analysis_Rms2d::Calculate_2D
+ * (src/Analysis_Rms2d.cpp:196-295):
+ *   ref == NULL  -> same frames and mask: packed upper triangle of
+ *                   tgt->nframes*(tgt->nframes-1)/2 floats (AllocateTriangle);
+ *   ref != NULL  -> full matrix of tgt->nframes rows x ref->nframes columns,
+ *                   element (ntgt, nref) at ntgt*ref->nframes + nref
+ *                   (Allocate2D(totalref, totaltgt) + SetElement(nref, ntgt)).
+ * The reference frames are centred with their own masses, the targets with
+ * theirs, and the target masses weight the covariance and E0
+ * (Frame::RMSD_CenteredRef / QuaternionRMSD_CenteredRef).  ref->nsel must
+ * equal tgt->nsel.  precision is ignored for RMSD_MODE_NOFIT (FP64 direct
+ * sum, bit-exact with Frame::RMSD_NoFit). */
+int rmsd_pairwise(rmsd_ctx* ctx, const rmsd_frames* tgt, const rmsd_frames* ref, int mode,
+                  int use_mass, int precision, float* out);
+
+/* Cluster pairwise-distance cache fill.
+ * Replaces Cluster::MetricArray::calcFrameDistances (src/Cluster/
+ * MetricArray.cpp:766-801) for one RMS metric (Metric_RMS::FrameDist,
+ * src/Cluster/Metric_RMS.cpp:44-63) with weight `weight` (Manhattan, one
+ * metric: d = weight * RMSD).  frames->fidx lists the frames to cache
+ * (framesToCache); out receives the packed upper triangle over that list
+ * (DataSet_PairwiseCache_MEM::SetElement order). */
+int rmsd_cluster_cache(rmsd_ctx* ctx, const rmsd_frames* frames, int nofit, int use_mass,
+                       int precision, double weight, float* out);
+
+/* Fit every frame of `frames` to one reference frame.
+ * Replaces Action_Rmsd::DoAction (src/Action_Rmsd.cpp:361-417) over a batch of
+ * frames for the 'first' / 'ref' / 'refindex' reference modes
+ * (ReferenceAction, src/ReferenceAction.h:73-84):
+ *   ref, ref_frame  the reference set and frame; its selection and masses
+ *                   centre the reference (ReferenceAction::SelectRefAtoms);
+ *   fit             0 for 'nofit' (Frame::RMSD_NoFit);
+ *   modify          RMSD_MODIFY_*: frames->xyz is rewritten in place (all
+ *                   natom atoms of every frame; the buffer must be writable);
+ *   rmsd            nframes doubles (DataSet_double of the action);
+ *   U               9 doubles per frame or NULL ('savematrices');
+ *   tvec            6 doubles per frame or NULL: Trans (target -> origin) and
+ *                   refTrans (origin -> reference), 'savevectors separate';
+ *                   'combined' is their sum. */
+int rmsd_fit_to_ref(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* ref,
+                    int ref_frame, int fit, int use_mass, int modify, double* rmsd, double* U,
+                    double* tvec);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CPPTRAJ_RMSD_H */

@@ -1,0 +1,229 @@
+// ref_shim.cpp -- C entry points onto the compiled reference sources.
+//
+// TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles this file together with
+// the reference's own hot-path translation units (Frame.cpp, Matrix_3x3.cpp,
+// qcprot.cpp, QuaternionRMSD.cpp and the few files they pull in) straight from
+// the reference tree into oracle/_ref/libcpptraj_ref.so.  Nothing from the
+// reference is copied into this repository; the library is built here and is
+// used only (a) to pin the C oracle in rmsd_oracle.c and (b) as the CPU
+// baseline timed beside the GPU.  Each wrapper names the reference function it
+// calls.
+#include <vector>
+#include <cstring>
+#include <cstdint>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include "Frame.h"
+#include "Matrix_3x3.h"
+#include "Vec3.h"
+#include "QuaternionRMSD.h"
+#include "qcprot.h"
+
+namespace {
+
+Frame make_frame(const double* xyz, const double* mass, int n) {
+  std::vector<double> X(xyz, xyz + 3 * (size_t)n);
+  std::vector<double> M((size_t)n, 1.0);
+  if (mass) M.assign(mass, mass + n);
+  Frame f;
+  f.SetupFrameXM(X, M);
+  return f;
+}
+
+void frame_from_floats(Frame& f, const float* fr, const int* sel, int nsel) {
+  double* X = f.xAddress();
+  for (int j = 0; j < nsel; ++j) {
+    int a = sel ? sel[j] : j;
+    X[3 * j] = (double)fr[3 * a];
+    X[3 * j + 1] = (double)fr[3 * a + 1];
+    X[3 * j + 2] = (double)fr[3 * a + 2];
+  }
+}
+
+int64_t tri_index(int64_t nx, int64_t x, int64_t y) {
+  int64_t i = x < y ? x : y, j = x < y ? y : x;
+  return ((nx * i) - (((i + 1) * i) / 2)) + j - (i + 1);
+}
+
+}  // namespace
+
+extern "C" {
+
+// Frame::RMSD_CenteredRef(Frame const&, Matrix_3x3&, Vec3&, bool)
+double cref_rmsd_centered_ref(double* tgt, const double* ref, const double* tmass,
+                              const double* rmass, int n, int use_mass,
+                              double* U, double* trans) {
+  Frame T = make_frame(tgt, tmass, n);
+  Frame R = make_frame(ref, rmass, n);
+  Matrix_3x3 u;
+  Vec3 tr;
+  double r = T.RMSD_CenteredRef(R, u, tr, use_mass != 0);
+  for (int k = 0; k < 9; ++k) U[k] = u[k];
+  for (int k = 0; k < 3; ++k) trans[k] = tr[k];
+  std::memcpy(tgt, T.xAddress(), sizeof(double) * 3 * (size_t)n);
+  return r;
+}
+
+// Frame::RMSD(Frame&, Matrix_3x3&, Vec3&, Vec3&, bool)
+double cref_rmsd(double* tgt, double* ref, const double* tmass, const double* rmass,
+                 int n, int use_mass, double* U, double* trans, double* ref_trans) {
+  Frame T = make_frame(tgt, tmass, n);
+  Frame R = make_frame(ref, rmass, n);
+  Matrix_3x3 u;
+  Vec3 tr, rtr;
+  double r = T.RMSD(R, u, tr, rtr, use_mass != 0);
+  for (int k = 0; k < 9; ++k) U[k] = u[k];
+  for (int k = 0; k < 3; ++k) { trans[k] = tr[k]; ref_trans[k] = rtr[k]; }
+  std::memcpy(tgt, T.xAddress(), sizeof(double) * 3 * (size_t)n);
+  std::memcpy(ref, R.xAddress(), sizeof(double) * 3 * (size_t)n);
+  return r;
+}
+
+// Frame::RMSD_NoFit(Frame const&, bool)
+double cref_rmsd_nofit(const double* tgt, const double* ref, const double* tmass,
+                       int n, int use_mass) {
+  Frame T = make_frame(tgt, tmass, n);
+  Frame R = make_frame(ref, 0, n);
+  return T.RMSD_NoFit(R, use_mass != 0);
+}
+
+// Frame::CenterOnOrigin(bool)
+void cref_center_on_origin(double* xyz, const double* mass, int n, int use_mass,
+                           double* center) {
+  Frame F = make_frame(xyz, mass, n);
+  Vec3 c = F.CenterOnOrigin(use_mass != 0);
+  for (int k = 0; k < 3; ++k) center[k] = c[k];
+  std::memcpy(xyz, F.xAddress(), sizeof(double) * 3 * (size_t)n);
+}
+
+// QuaternionRMSD_CenteredRef(Frame const&, Frame&, Matrix_3x3&, Vec3&, bool)
+double cref_qrmsd_centered_ref(const double* ref, double* tgt, const double* tmass,
+                               int n, int use_mass, double* rot, double* trans) {
+  Frame R = make_frame(ref, 0, n);
+  Frame T = make_frame(tgt, tmass, n);
+  double r;
+  Vec3 tr;
+  if (rot) {
+    Matrix_3x3 u;
+    r = QuaternionRMSD_CenteredRef(R, T, u, tr, use_mass != 0);
+    for (int k = 0; k < 9; ++k) rot[k] = u[k];
+  } else {
+    r = QuaternionRMSD_CenteredRef(R, T, use_mass != 0);
+  }
+  for (int k = 0; k < 3; ++k) trans[k] = tr[k];
+  std::memcpy(tgt, T.xAddress(), sizeof(double) * 3 * (size_t)n);
+  return r;
+}
+
+// Cpptraj::QCPRot::FastCalcRMSDAndRotation
+int cref_qcp(double* rot, const double* A, double* rmsd, double E0, double len) {
+  double a[9];
+  std::memcpy(a, A, sizeof(a));
+  return Cpptraj::QCPRot::FastCalcRMSDAndRotation(rot, a, rmsd, E0, len);
+}
+
+// Matrix_3x3::Diagonalize_Sort(Vec3&)
+int cref_diagonalize_sort(const double* in, double* E, double* ev) {
+  Matrix_3x3 m(in);
+  Vec3 e;
+  int err = m.Diagonalize_Sort(e);
+  for (int k = 0; k < 9; ++k) E[k] = m[k];
+  for (int k = 0; k < 3; ++k) ev[k] = e[k];
+  return err;
+}
+
+// The loop nest of Analysis_Rms2d::Calculate_2D (src/Analysis_Rms2d.cpp:
+// 247-287) driving the reference's Frame routines: OpenMP over reference rows
+// with a dynamic schedule, reference centred once per row, target gathered per
+// pair, (float) store in the packed triangle or full layout.
+// mode: 0 fit, 1 nofit, 2 quaternion.
+int cref_rms2d(const float* tgt_frames, int64_t tgt_stride, int ntgt,
+               const float* ref_frames, int64_t ref_stride, int nref,
+               const int* tsel, const int* rsel, int nsel,
+               const double* tmass, const double* rmass,
+               int mode, int use_mass, int full, float* out, int nthreads,
+               int row_begin, int row_end) {
+  if (ref_frames == 0) { ref_frames = tgt_frames; ref_stride = tgt_stride; nref = ntgt; }
+  if (row_end < 0 || row_end > nref) row_end = nref;
+  std::vector<double> zero(3 * (size_t)nsel, 0.0);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+#endif
+  {
+    Frame SelRef = make_frame(zero.data(), rmass, nsel);
+    Frame SelTgt = make_frame(zero.data(), tmass, nsel);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic)
+#endif
+    for (int r = row_begin; r < row_end; ++r) {
+      frame_from_floats(SelRef, ref_frames + (int64_t)r * ref_stride, rsel, nsel);
+      if (mode == 0 || mode == 2) SelRef.CenterOnOrigin(use_mass != 0);
+      int t0 = full ? 0 : r + 1;
+      for (int t = t0; t < ntgt; ++t) {
+        frame_from_floats(SelTgt, tgt_frames + (int64_t)t * tgt_stride, tsel, nsel);
+        float v;
+        if (mode == 0) v = (float)SelTgt.RMSD_CenteredRef(SelRef, use_mass != 0);
+        else if (mode == 1) v = (float)SelTgt.RMSD_NoFit(SelRef, use_mass != 0);
+        else v = (float)QuaternionRMSD_CenteredRef(SelRef, SelTgt, use_mass != 0);
+        int64_t idx = full ? (int64_t)t * nref + r : tri_index(ntgt, r, t);
+        if (out) out[idx] = v;
+      }
+    }
+  }
+  return 0;
+}
+
+// Per-frame body of Action_Rmsd::DoAction (src/Action_Rmsd.cpp:361-417) for a
+// fixed, already centred reference: RMSD_CenteredRef (or RMSD_NoFit), then the
+// coordinate update of the chosen mode (1 = Trans_Rot_Trans, 2 = translate).
+int cref_fit_frames(const float* frames, int64_t stride, int nframes, int natom,
+                    const int* sel, int nsel, const double* mass,
+                    const double* ref_sel, const double* ref_trans,
+                    int fit, int use_mass, int modify, double* rmsd, double* U,
+                    double* tvec, float* frames_out, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+#endif
+  {
+    Frame Ref = make_frame(ref_sel, 0, nsel);
+    std::vector<double> zero(3 * (size_t)nsel, 0.0);
+    Frame Tgt = make_frame(zero.data(), mass, nsel);
+    std::vector<double> all(3 * (size_t)natom, 0.0);
+    Frame Full = make_frame(all.data(), 0, natom);
+    Vec3 refT(ref_trans[0], ref_trans[1], ref_trans[2]);
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (int f = 0; f < nframes; ++f) {
+      const float* fr = frames + (int64_t)f * stride;
+      frame_from_floats(Tgt, fr, sel, nsel);
+      Matrix_3x3 u(1.0);
+      u.Zero(); u[0] = u[4] = u[8] = 1.0;
+      Vec3 tr(0.0, 0.0, 0.0);
+      double v = fit ? Tgt.RMSD_CenteredRef(Ref, u, tr, use_mass != 0)
+                     : Tgt.RMSD_NoFit(Ref, use_mass != 0);
+      rmsd[f] = v;
+      if (fit && U) for (int k = 0; k < 9; ++k) U[9 * (int64_t)f + k] = u[k];
+      if (fit && tvec) {
+        for (int k = 0; k < 3; ++k) {
+          tvec[6 * (int64_t)f + k] = tr[k];
+          tvec[6 * (int64_t)f + 3 + k] = refT[k];
+        }
+      }
+      if (frames_out) {
+        frame_from_floats(Full, fr, 0, natom);
+        if (fit && modify == 1) Full.Trans_Rot_Trans(tr, u, refT);
+        else if (fit && modify == 2) Full.Translate(tr + refT);
+        const double* X = Full.xAddress();
+        float* fo = frames_out + (int64_t)f * stride;
+        for (int k = 0; k < 3 * natom; ++k) fo[k] = (float)X[k];
+      }
+    }
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,237 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures under tests/golden/ from the reference's own test data.
+
+Run in the build container (where the reference tree exists):
+
+    python tests/golden/make_fixtures.py [--ref /root/reference]
+
+Inputs (read as data, never executed):
+  test/tz2.parm7, test/tz2.nc                  (223 atoms, 101 frames)
+  test/tz2.truncoct.parm7, test/tz2.truncoct.nc (5827 atoms, 10 frames)
+  test/rGACC.full.parm7, test/rGACC.full.nc     (7622 atoms, 1 frame)
+  test/Test_2DRMS/*.save, test/Test_RMSD/*.save (expected outputs)
+
+Outputs (numpy .npz, no pickles):
+  tz2_2drms.npz    frames 1-10 of tz2.nc, masses, masks :3-7 / :2 / :11 /
+                   :2-12@CA, and the Test_2DRMS expected matrices
+  tz2_rmsd.npz     tz2.truncoct.nc :2-11 atoms and masses, plus expected RMSD
+                   (plain, mass, reftraj) and combined translation vectors
+  tz2_rotate.npz   reference = tz2.nc frame 1, targets = tz2.truncoct.nc with
+                   waters stripped, @CA mask, expected rotation matrices and the
+                   expected rotated / translated coordinates
+  tz2_camask.npz   tz2.nc all 101 frames, :2-12@CA atoms, expected 'nomod' and
+                   'previous' RMSD series
+  rgacc.npz        rGACC.full.nc frame 1, :1-4&!@H= atoms (9 identical frames ->
+                   all-zero matrix)
+
+The atom masks used by these tests are resolved here with a few lines of
+parm7 parsing (residue ranges, atom names, hydrogen exclusion); the general
+mask language is outside the scope of the RMSD path.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import re
+
+import numpy as np
+from scipy.io import netcdf_file
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+# --------------------------------------------------------------------------
+def read_parm7(path):
+    """Minimal Amber parm7 reader: returns dict flag -> list of tokens."""
+    flags = {}
+    cur = None
+    fmt_w = None
+    fmt_kind = None
+    with open(path) as fh:
+        for line in fh:
+            line = line.rstrip("\n")
+            if line.startswith("%FLAG"):
+                cur = line.split()[1]
+                flags[cur] = []
+                continue
+            if line.startswith("%FORMAT"):
+                m = re.match(r"%FORMAT\((\d+)([aIE])(\d+)", line)
+                fmt_kind = m.group(2)
+                fmt_w = int(m.group(3))
+                continue
+            if line.startswith("%VERSION") or cur is None:
+                continue
+            for k in range(0, len(line), fmt_w):
+                tok = line[k:k + fmt_w]
+                if fmt_kind == "a":
+                    flags[cur].append(tok.strip())
+                elif tok.strip():
+                    flags[cur].append(tok.strip())
+    return flags
+
+
+class Parm:
+    def __init__(self, path):
+        f = read_parm7(path)
+        self.names = f["ATOM_NAME"]
+        self.natom = len(self.names)
+        self.mass = np.array([float(x) for x in f["MASS"]], dtype=np.float64)
+        ptr = [int(x) - 1 for x in f["RESIDUE_POINTER"]]
+        self.resnames = f["RESIDUE_LABEL"]
+        self.res_of_atom = np.zeros(self.natom, dtype=np.int64)
+        bounds = ptr + [self.natom]
+        for r in range(len(ptr)):
+            self.res_of_atom[bounds[r]:bounds[r + 1]] = r
+
+    def select(self, res=None, names=None, no_h=False, exclude_res=None):
+        """Residues are 1-based inclusive ranges [(a, b), ...]."""
+        keep = np.ones(self.natom, dtype=bool)
+        if res is not None:
+            rk = np.zeros(self.natom, dtype=bool)
+            for a, b in res:
+                rk |= (self.res_of_atom >= a - 1) & (self.res_of_atom <= b - 1)
+            keep &= rk
+        if names is not None:
+            keep &= np.array([n in names for n in self.names])
+        if no_h:
+            keep &= np.array([not n.startswith("H") for n in self.names])
+        if exclude_res is not None:
+            keep &= np.array([self.resnames[r] not in exclude_res for r in self.res_of_atom])
+        return np.nonzero(keep)[0].astype(np.int32)
+
+
+def read_nc(path):
+    f = netcdf_file(path, "r", mmap=False)
+    x = np.array(f.variables["coordinates"].data, dtype=np.float32)
+    f.close()
+    return x
+
+
+def read_square2d(path):
+    rows = []
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("#"):
+                continue
+            tok = line.split()
+            rows.append([float(t) for t in tok[1:]])
+    return np.array(rows, dtype=np.float64)
+
+
+def read_series(path, ncols=None):
+    rows = []
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("#"):
+                continue
+            tok = line.split()
+            rows.append([float(t) for t in tok[1:]])
+    return np.array(rows, dtype=np.float64)
+
+
+def read_crd(path, natom, nbox=0):
+    """Amber ASCII trajectory (%8.3f, 10 per line); nbox trailing box values
+    per frame are dropped."""
+    vals = []
+    with open(path) as fh:
+        fh.readline()
+        for line in fh:
+            line = line.rstrip("\n")
+            for k in range(0, len(line), 8):
+                tok = line[k:k + 8]
+                if tok.strip():
+                    vals.append(float(tok))
+    a = np.array(vals, dtype=np.float64).reshape(-1, 3 * natom + nbox)
+    return a[:, :3 * natom].reshape(-1, natom, 3)
+
+
+# --------------------------------------------------------------------------
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ref", default="/root/reference")
+    args = ap.parse_args()
+    T = os.path.join(args.ref, "test")
+
+    # ---- Test_2DRMS --------------------------------------------------------
+    p = Parm(os.path.join(T, "tz2.parm7"))
+    x = read_nc(os.path.join(T, "tz2.nc"))  # (101, 223, 3)
+    frames10 = x[:10].reshape(10, -1)
+    np.savez_compressed(
+        os.path.join(OUT, "tz2_2drms.npz"),
+        frames=frames10,
+        mass=p.mass,
+        sel_3_7=p.select(res=[(3, 7)]),
+        sel_2=p.select(res=[(2, 2)]),
+        sel_11=p.select(res=[(11, 11)]),
+        sel_2_12_ca=p.select(res=[(2, 12)], names={"CA"}),
+        rmsd=read_square2d(os.path.join(T, "Test_2DRMS", "rmsd.dat.save")),
+        rmsd_mass=read_square2d(os.path.join(T, "Test_2DRMS", "rmsd.mass.dat.save")),
+        trp=read_square2d(os.path.join(T, "Test_2DRMS", "trp.dat.save")),
+        nofit=read_square2d(os.path.join(T, "Test_2DRMS", "nofit.dat.save")),
+    )
+
+    # ---- Test_RMSD basic (truncoct, :2-11) -------------------------------------
+    pt = Parm(os.path.join(T, "tz2.truncoct.parm7"))
+    xt = read_nc(os.path.join(T, "tz2.truncoct.nc"))  # (10, 5827, 3)
+    s = pt.select(res=[(2, 11)])
+    vec = read_series(os.path.join(T, "Test_RMSD", "vecs.dat.save"))
+    np.savez_compressed(
+        os.path.join(OUT, "tz2_rmsd.npz"),
+        frames=xt[:, s, :].reshape(xt.shape[0], -1),
+        mass=pt.mass[s],
+        rmsd=read_series(os.path.join(T, "Test_RMSD", "rmsd.dat.save"))[:, 0],
+        rmsd_mass=read_series(os.path.join(T, "Test_RMSD", "rmsd.mass.dat.save"))[:, 0],
+        rmsd_reftraj=read_series(os.path.join(T, "Test_RMSD", "rmsd.reftraj.dat.save"))[:, 0],
+        vecs_mass=vec,
+    )
+
+    # ---- Test_RMSD rotation matrices / coordinates ----------------------------
+    solute = pt.select(exclude_res={"WAT"})
+    tgt = xt[:, solute, :]
+    ca_ref = p.select(names={"CA"})
+    ca_tgt = pt.select(names={"CA"}, exclude_res={"WAT"})
+    assert len(ca_ref) == len(ca_tgt)
+    # index of each target CA within the stripped (solute-only) frame
+    ca_tgt = np.searchsorted(solute, ca_tgt).astype(np.int32)
+    np.savez_compressed(
+        os.path.join(OUT, "tz2_rotate.npz"),
+        ref_frame=x[0].reshape(-1),
+        frames=tgt.reshape(tgt.shape[0], -1),
+        sel_ca_ref=ca_ref,
+        sel_ca=ca_tgt,
+        rmatrices=read_series(os.path.join(T, "Test_RMSD", "rmatrices.dat.save")),
+        norotate=read_crd(os.path.join(T, "Test_RMSD", "tz2.norotate.crd.save"), len(solute), 3),
+        rotate=read_crd(os.path.join(T, "Test_RMSD", "tz2.rotate.crd.save"), len(solute), 3),
+    )
+
+    # ---- Test_RMSD nomod / previous (:2-12@CA over 101 frames) ---------------
+    s_ca = p.select(res=[(2, 12)], names={"CA"})
+    np.savez_compressed(
+        os.path.join(OUT, "tz2_camask.npz"),
+        frames=x[:, s_ca, :].reshape(x.shape[0], -1),
+        mass=p.mass[s_ca],
+        nomod=read_series(os.path.join(T, "Test_RMSD", "NoMod.dat.save"))[:, 0],
+        previous=read_series(os.path.join(T, "Test_RMSD", "Previous.dat.save"))[:, 0],
+    )
+
+    # ---- Test_2DRMS rGACC (:1-4&!@H=) ------------------------------------------
+    pg = Parm(os.path.join(T, "rGACC.full.parm7"))
+    xg = read_nc(os.path.join(T, "rGACC.full.nc"))
+    sg = pg.select(res=[(1, 4)], no_h=True)
+    vals = []
+    with open(os.path.join(T, "Test_2DRMS", "rgacc.dat.save")) as fh:
+        for line in fh:
+            if not line.startswith("#"):
+                vals.append([float(t) for t in line.split()])
+    np.savez_compressed(
+        os.path.join(OUT, "rgacc.npz"),
+        frame=xg[0, sg, :].reshape(-1),
+        expected=np.array(vals),
+    )
+    for fn in sorted(os.listdir(OUT)):
+        if fn.endswith(".npz"):
+            print(fn, os.path.getsize(os.path.join(OUT, fn)))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,159 @@
+"""GPU parity of the all-pairs kernels (rms2d / cluster cache) with the FP64 oracle.
+
+Tolerances (stated per test):
+  * plain FP64 kernel: |GPU - oracle| <= 1e-6 A on every pair (the outputs are
+    floats, as in the reference, so this also absorbs one float ulp);
+  * nofit: bit-exact (same FP64 operation order as Frame::RMSD_NoFit);
+  * FP32-MFMA and split-BF16 (3-term) kernels: <= 1e-4 A on every pair.
+"""
+import numpy as np
+import pytest
+
+from cpptraj_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+MODE = {"fit": 0, "nofit": 1, "qrmsd": 2}
+
+
+def dev(a, dt=None):
+    import torch
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a)
+    if dt is None:
+        dt = {np.float32: torch.float32, np.float64: torch.float64,
+              np.int32: torch.int32}[a.dtype.type]
+    return torch.as_tensor(a, dtype=dt).cuda()
+
+
+def maxerr(got, exp):
+    """Largest |got - exp|, with NaN matching NaN (the reference's QCP returns
+    NaN for a single atom) and NaN against a number counted as infinite."""
+    got = np.asarray(got, np.float64)
+    exp = np.asarray(exp, np.float64)
+    both = np.isnan(got) & np.isnan(exp)
+    d = np.abs(got - exp)
+    d[both] = 0.0
+    d[np.isnan(d)] = np.inf
+    return d.max() if d.size else 0.0
+
+
+def run_rms2d(ctx, X, sel=None, mass=None, mode="fit", use_mass=False, precision="fp64",
+              ref=None, ref_sel=None, ref_mass=None, soa=False, natom=None):
+    import cpptraj_amd as ca
+    out = ca.rms2d(dev(X), dev(sel), dev(mass), mode=mode, use_mass=use_mass,
+                   precision=precision, ref_frames=dev(ref), ref_mask=dev(ref_sel),
+                   ref_mass=dev(ref_mass), soa=soa, natom=natom, ctx=ctx)
+    return out.cpu().numpy()
+
+
+@pytest.mark.parametrize("use_mass", [False, True])
+@pytest.mark.parametrize("mode", ["fit", "qrmsd"])
+def test_config0_fp64_matches_oracle(oracle, ctx, mode, use_mass):
+    """1000 frames x 300 atoms, sigma 0.5, rigid motion: the CPU-runnable workload."""
+    F, n = 1000, 300
+    X = synth.noisy(0, F, n, 0.5).reshape(F, -1)
+    m = synth.masses(np.random.default_rng(100), n)
+    got = run_rms2d(ctx, X, None, m, mode, use_mass)
+    exp = oracle.rms2d(X, None, m, MODE[mode], use_mass)
+    assert got.shape == (F * (F - 1) // 2,)
+    assert np.abs(got.astype(np.float64) - exp).max() <= 1e-6
+
+
+@pytest.mark.parametrize("F,n", [(2, 3), (3, 1), (3, 2), (65, 7), (129, 33), (200, 257)])
+@pytest.mark.parametrize("use_mass", [False, True])
+def test_edges_fp64(oracle, ctx, F, n, use_mass):
+    """Tile edges (64-frame tiles), atom counts below / across the 8-atom chunk,
+    one and two atoms."""
+    X = synth.noisy(F + n, F, n + 5, 0.8).reshape(F, -1)
+    rng = np.random.default_rng(F * 7 + n)
+    sel = np.sort(rng.choice(n + 5, n, replace=False)).astype(np.int32)
+    m = synth.masses(rng, n)
+    for mode in ("fit", "qrmsd", "nofit"):
+        got = run_rms2d(ctx, X, sel, m, mode, use_mass).astype(np.float64)
+        exp = oracle.rms2d(X, sel, m, MODE[mode], use_mass)
+        if mode == "nofit":
+            assert np.array_equal(got, exp), mode
+        elif n >= 3:
+            assert maxerr(got, exp) <= 1e-6, mode
+        else:
+            # one or two atoms: no unique rotation; Kabsch and QCP still agree on
+            # the distance up to the accuracy of sqrt near 0
+            assert maxerr(got, exp) <= 1e-4, mode
+
+
+@pytest.mark.parametrize("use_mass", [False, True])
+def test_full_matrix_two_sets(oracle, ctx, use_mass):
+    """Reference trajectory with its own frames and mask (full Fref x Ftgt)."""
+    rng = np.random.default_rng(7)
+    A = synth.noisy(8, 150, 120, 0.6).reshape(150, -1)
+    B = synth.noisy(9, 97, 140, 0.6).reshape(97, -1)
+    tsel = np.sort(rng.choice(120, 90, replace=False)).astype(np.int32)
+    rsel = np.sort(rng.choice(140, 90, replace=False)).astype(np.int32)
+    tm, rm = synth.masses(rng, 90), synth.masses(rng, 90)
+    for mode in ("fit", "qrmsd", "nofit"):
+        got = run_rms2d(ctx, A, tsel, tm, mode, use_mass, ref=B, ref_sel=rsel, ref_mass=rm)
+        exp = oracle.rms2d(A, tsel, tm, MODE[mode], use_mass, ref_frames=B, ref_sel=rsel,
+                           ref_mass=rm).reshape(150, 97)
+        assert got.shape == (150, 97)
+        if mode == "nofit":
+            assert np.array_equal(got.astype(np.float64), exp.astype(np.float64))
+        else:
+            assert np.abs(got.astype(np.float64) - exp).max() <= 1e-6
+
+
+def test_soa_input_matches_aos(ctx):
+    X = synth.noisy(3, 300, 77, 0.5)
+    aos = run_rms2d(ctx, X.reshape(300, -1))
+    soa = run_rms2d(ctx, synth.to_soa(X, ld=96), soa=True, natom=77)
+    assert np.array_equal(aos, soa)
+
+
+def test_sentinels(ctx):
+    X = synth.noisy(4, 70, 20, 0.5).reshape(70, -1)
+    z = np.zeros(20)
+    for mode in ("fit", "qrmsd", "nofit"):
+        got = run_rms2d(ctx, X, None, z, mode, True)
+        assert np.all(got == -1.0), mode
+    same = np.repeat(X[:1], 70, axis=0)
+    for prec in ("fp64", "fp32", "bf16x3"):
+        got = run_rms2d(ctx, same, precision=prec)
+        assert np.abs(got).max() <= 1e-4 if prec != "fp64" else np.abs(got).max() <= 2e-6
+
+
+@pytest.mark.parametrize("use_mass", [False, True])
+def test_cluster_cache_sieve(oracle, ctx, use_mass):
+    import cpptraj_amd as ca
+    X = synth.noisy(5, 300, 64, 0.7).reshape(300, -1)
+    rng = np.random.default_rng(5)
+    m = synth.masses(rng, 64)
+    frames = np.arange(0, 300, 3, dtype=np.int32)  # sieve 3
+    for nofit in (False, True):
+        got = ca.cluster_cache(dev(X), dev(frames), None, dev(m), nofit=nofit, use_mass=use_mass,
+                               ctx=ctx).cpu().numpy().astype(np.float64)
+        exp = oracle.cluster_cache(X, frames, None, m, nofit=nofit, use_mass=use_mass)
+        if nofit:
+            assert np.array_equal(got, exp.astype(np.float64))
+        else:
+            assert np.abs(got - exp).max() <= 1e-6
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("use_mass", [False, True])
+def test_mixed_precision_bound(oracle, ctx, precision, use_mass):
+    """Drift workload at reduced frame count (1500 frames x 1200 atoms): every
+    pair within 1e-4 A of the FP64 kernel, which is itself checked against the
+    oracle on a sample of rows."""
+    F, n = 1500, 1200
+    X = synth.drift(2, F, n).reshape(F, -1)
+    m = synth.masses(np.random.default_rng(2), n)
+    ref64 = run_rms2d(ctx, X, None, m, "fit", use_mass, "fp64").astype(np.float64)
+    got = run_rms2d(ctx, X, None, m, "fit", use_mass, precision).astype(np.float64)
+    err = np.abs(got - ref64)
+    assert err.max() <= 1e-4, (precision, err.max())
+    # FP64 kernel vs oracle on the first rows (row r holds F-1-r pairs)
+    rows = 40
+    exp = oracle.rms2d(X, None, m, 0, use_mass)
+    k = rows * F - rows * (rows + 1) // 2
+    assert np.abs(ref64[:k] - exp[:k]).max() <= 1e-6

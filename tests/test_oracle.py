@@ -1,0 +1,234 @@
+"""Pins the FP64 oracle (oracle/rmsd_oracle.c).
+
+Each check is chosen so that a plausible slip in the restatement (a dropped
+term, a wrong sign or index, a transposed operand) fails:
+  * bit-for-bit agreement with the reference's own compiled sources
+    (oracle/_ref) on random and degenerate inputs;
+  * agreement with an independent textbook SVD-Kabsch (numpy.linalg.svd);
+  * closed forms: exact rigid copies give RMSD 0 and recover the rotation;
+    mirror images exercise the reflection branch;
+  * symmetry and rigid-motion invariance;
+  * the analytic gradient against central finite differences.
+"""
+import numpy as np
+import pytest
+
+from cpptraj_amd import synth
+
+
+def svd_kabsch(tgt, ref, w=None):
+    """Textbook weighted Kabsch by SVD: returns (rmsd, U) with U mapping the
+    centred target onto the centred reference."""
+    w = np.ones(len(tgt)) if w is None else np.asarray(w, float)
+    W = w.sum()
+    t = tgt - (w[:, None] * tgt).sum(0) / W
+    r = ref - (w[:, None] * ref).sum(0) / W
+    H = (w[:, None] * t).T @ r
+    V, S, Wt = np.linalg.svd(H)
+    d = np.sign(np.linalg.det(V @ Wt))
+    D = np.diag([1.0, 1.0, d])
+    U = (V @ D @ Wt).T
+    resid = (w * ((t @ U.T - r) ** 2).sum(1)).sum() / W
+    return np.sqrt(max(resid, 0.0)), U
+
+
+def same(a, b):
+    """Bitwise-equal values, NaN matching NaN (degenerate 1-atom fits give NaN
+    in the reference too)."""
+    return np.array_equal(np.asarray(a, float), np.asarray(b, float), equal_nan=True)
+
+
+def rand_case(rng, n, spread=5.0, noise=0.5):
+    ref = rng.normal(size=(n, 3)) * spread
+    Q = synth.rotations(rng, 1)[0]
+    tgt = (ref + rng.normal(size=(n, 3)) * noise) @ Q.T + rng.uniform(-20, 20, 3)
+    m = synth.masses(rng, n)
+    return ref, tgt, m
+
+
+# -- bit-exact agreement with the compiled reference ------------------------------
+@pytest.mark.parametrize("use_mass", [False, True])
+def test_rmsd_centered_ref_bitexact(oracle, reflib, use_mass):
+    rng = np.random.default_rng(11)
+    for _ in range(300):
+        n = int(rng.integers(1, 80))
+        ref, tgt, m = rand_case(rng, n, noise=float(rng.choice([0.0, 1e-3, 0.5, 5.0])))
+        rc_o, c_o = oracle.center_on_origin(ref, m, use_mass)
+        rc_r, c_r = reflib.center_on_origin(ref, m, use_mass)
+        assert np.array_equal(rc_o, rc_r) and np.array_equal(c_o, c_r)
+        a = oracle.rmsd_centered_ref(tgt, rc_o, m, use_mass)
+        b = reflib.rmsd_centered_ref(tgt, rc_o, m, m, use_mass)
+        assert same(a[0], b[0])
+        if a[0] > 0:  # U is left unset on the Jacobi-failure / sentinel paths
+            assert np.array_equal(a[1], b[1], equal_nan=True)
+        assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
+
+
+@pytest.mark.parametrize("use_mass", [False, True])
+def test_qcp_and_nofit_bitexact(oracle, reflib, use_mass):
+    rng = np.random.default_rng(12)
+    for _ in range(300):
+        n = int(rng.integers(1, 80))
+        ref, tgt, m = rand_case(rng, n, noise=float(rng.choice([0.0, 1e-4, 0.5, 3.0])))
+        rc, _ = oracle.center_on_origin(ref, m, use_mass)
+        a = oracle.qrmsd_centered_ref(rc, tgt, m, use_mass, want_rot=True)
+        b = reflib.qrmsd_centered_ref(rc, tgt, m, use_mass, want_rot=True)
+        assert same(a[0], b[0]) and same(a[1], b[1]) and same(a[2], b[2])
+        assert same(oracle.rmsd_nofit(tgt, ref, m, use_mass), reflib.rmsd_nofit(tgt, ref, m, use_mass))
+
+
+def test_jacobi_bitexact_and_eigh(oracle, reflib):
+    rng = np.random.default_rng(13)
+    for k in range(300):
+        A = rng.normal(size=(3, 3))
+        S = A @ A.T
+        if k % 5 == 0:  # repeated eigenvalues / rank deficiency
+            S = np.diag(rng.choice([0.0, 1.0, 2.0], 3))
+        e1, E1, ev1 = oracle.jacobi3_sorted(S)
+        e2, E2, ev2 = reflib.diagonalize_sort(S)
+        assert e1 == e2 and same(E1, E2) and same(ev1, ev2)
+        w = np.linalg.eigh(S)[0][::-1]
+        # Ties are not reordered by the reference's strict '>' comparisons,
+        # so compare the sets; distinct eigenvalues come out descending.
+        np.testing.assert_allclose(np.sort(ev1), np.sort(w), atol=1e-10 * max(1.0, abs(w).max()))
+        if len(set(np.round(w, 6))) == 3:
+            np.testing.assert_allclose(ev1, w, atol=1e-10 * max(1.0, abs(w).max()))
+        # eigenvectors are the rows
+        for r in range(3):
+            np.testing.assert_allclose(S @ E1[r], ev1[r] * E1[r], atol=1e-9 * max(1, abs(w).max()))
+
+
+def test_jacobi_tie_order_quirk(oracle, reflib):
+    # Matrix_3x3::Diagonalize_Sort (src/Matrix_3x3.cpp:229-245) uses strict
+    # '>' tests, so a tie for the largest eigenvalue leaves them ascending.
+    for diag, order in (((2.0, 2.0, 1.0), (1.0, 2.0, 2.0)), ((3.0, 1.0, 2.0), (3.0, 2.0, 1.0))):
+        e, E, ev = oracle.jacobi3_sorted(np.diag(diag))
+        assert tuple(ev) == order
+        assert same(ev, reflib.diagonalize_sort(np.diag(diag))[2])
+
+
+def test_rms2d_loop_bitexact(oracle, reflib):
+    rng = np.random.default_rng(14)
+    X = synth.noisy(5, 40, 50, 0.7).reshape(40, -1)
+    sel = np.array(sorted(rng.choice(50, 33, replace=False)), dtype=np.int32)
+    m = synth.masses(rng, 33)
+    for mode in (0, 1, 2):
+        for um in (False, True):
+            assert same(oracle.rms2d(X, sel, m, mode, um), reflib.rms2d(X, sel, m, mode, um))
+    sel2 = np.arange(17, 50, dtype=np.int32)
+    m2 = synth.masses(rng, 33)
+    a = oracle.rms2d(X, sel, m, 0, True, ref_sel=sel2, ref_mass=m2)
+    b = reflib.rms2d(X, sel, m, 0, True, ref_sel=sel2, ref_mass=m2)
+    assert a.shape == (1600,) and same(a, b)
+
+
+def test_fit_frames_bitexact(oracle, reflib):
+    X = synth.noisy(6, 20, 60, 1.0).reshape(20, -1)
+    rng = np.random.default_rng(15)
+    m = synth.masses(rng, 60)
+    for um in (False, True):
+        rc, rt = oracle.center_on_origin(X[0].reshape(-1, 3).astype(np.float64), m, um)
+        for mod in (0, 1, 2):
+            a = oracle.fit_frames(X, rc, rt, mass=m, use_mass=um, modify=mod, want_frames=True)
+            b = reflib.fit_frames(X, rc, rt, mass=m, use_mass=um, modify=mod, want_frames=True)
+            for x, y in zip(a, b):
+                assert same(x, y)
+
+
+# -- independent textbook and closed forms -----------------------------------------
+@pytest.mark.parametrize("use_mass", [False, True])
+def test_matches_svd_kabsch(oracle, use_mass):
+    rng = np.random.default_rng(16)
+    for _ in range(100):
+        n = int(rng.integers(5, 200))
+        ref, tgt, m = rand_case(rng, n)
+        w = m if use_mass else None
+        r_svd, U_svd = svd_kabsch(tgt, ref, w)
+        rc, _ = oracle.center_on_origin(ref, m, use_mass)
+        r, U, _, _ = oracle.rmsd_centered_ref(tgt, rc, m, use_mass)
+        assert abs(r - r_svd) < 1e-10
+        np.testing.assert_allclose(U, U_svd, atol=1e-9)
+        rq, Uq, _ = oracle.qrmsd_centered_ref(rc, tgt, m, use_mass, want_rot=True)
+        assert abs(rq - r_svd) < 1e-8
+        np.testing.assert_allclose(Uq, U_svd.T, atol=1e-8)  # QCP returns U^T
+
+
+def test_rigid_copy_is_zero_and_recovers_rotation(oracle):
+    rng = np.random.default_rng(17)
+    for _ in range(50):
+        ref = rng.normal(size=(40, 3)) * 8
+        Q = synth.rotations(rng, 1)[0]
+        tgt = ref @ Q.T + rng.uniform(-20, 20, 3)
+        rc, _ = oracle.center_on_origin(ref)
+        r, U, _, _ = oracle.rmsd_centered_ref(tgt, rc)
+        assert r < 1e-6
+        np.testing.assert_allclose(U, Q.T, atol=1e-9)  # U undoes Q
+
+
+def test_mirror_image_uses_reflection_branch(oracle):
+    rng = np.random.default_rng(18)
+    ref = rng.normal(size=(30, 3)) * 5
+    tgt = ref * np.array([1.0, 1.0, -1.0])  # improper copy
+    rc, _ = oracle.center_on_origin(ref)
+    r, U, _, _ = oracle.rmsd_centered_ref(tgt, rc)
+    r_svd, U_svd = svd_kabsch(tgt, ref)
+    assert r > 0.1 and abs(r - r_svd) < 1e-9
+    assert abs(np.linalg.det(U) - 1.0) < 1e-9
+
+
+def test_symmetry_and_invariance(oracle):
+    rng = np.random.default_rng(19)
+    ref, tgt, _ = rand_case(rng, 50)
+    a = oracle.rmsd_centered_ref(tgt, oracle.center_on_origin(ref)[0])[0]
+    b = oracle.rmsd_centered_ref(ref, oracle.center_on_origin(tgt)[0])[0]
+    Q = synth.rotations(rng, 1)[0]
+    c = oracle.rmsd_centered_ref(tgt @ Q.T + 3.0, oracle.center_on_origin(ref)[0])[0]
+    assert abs(a - b) < 1e-12 and abs(a - c) < 1e-12
+
+
+def test_sentinels(oracle):
+    rng = np.random.default_rng(20)
+    ref, tgt, _ = rand_case(rng, 10)
+    z = np.zeros(10)
+    assert oracle.rmsd_centered_ref(tgt, ref, z, True)[0] == -1.0
+    assert oracle.rmsd_nofit(tgt, ref, z, True) == -1.0
+    # collinear atoms: RMSD is still defined
+    line = np.outer(np.arange(8.0), [1.0, 2.0, 0.5])
+    rc, _ = oracle.center_on_origin(line)
+    Q = synth.rotations(rng, 1)[0]
+    assert oracle.rmsd_centered_ref(line @ Q.T, rc)[0] < 1e-6
+
+
+def test_nofit_closed_form(oracle):
+    rng = np.random.default_rng(21)
+    ref, tgt, m = rand_case(rng, 33)
+    exp = np.sqrt((m * ((ref - tgt) ** 2).sum(1)).sum() / m.sum())
+    assert abs(oracle.rmsd_nofit(tgt, ref, m, True) - exp) < 1e-12
+
+
+def test_tri_index_exhaustive(oracle):
+    for n in (2, 3, 7, 64, 65):
+        iu = np.triu_indices(n, 1)
+        got = [oracle.tri_index(n, i, j) for i, j in zip(*iu)]
+        assert got == list(range(n * (n - 1) // 2))
+        assert all(oracle.tri_index(n, j, i) == oracle.tri_index(n, i, j) for i, j in zip(*iu))
+        assert oracle.tri_index(n, 1, 1) == -1
+
+
+@pytest.mark.parametrize("use_mass", [False, True])
+def test_gradient_central_differences(oracle, use_mass):
+    rng = np.random.default_rng(22)
+    ref, tgt, m = rand_case(rng, 25)
+    rc, _ = oracle.center_on_origin(ref, m, use_mass)
+    r, g = oracle.rmsd_grad_target(tgt, rc, m, use_mass)
+    h = 1e-4
+    fd = np.zeros_like(tgt)
+    for a in range(len(tgt)):
+        for c in range(3):
+            p = tgt.copy(); p[a, c] += h
+            q = tgt.copy(); q[a, c] -= h
+            fd[a, c] = (oracle.rmsd_centered_ref(p, rc, m, use_mass)[0] -
+                        oracle.rmsd_centered_ref(q, rc, m, use_mass)[0]) / (2 * h)
+    np.testing.assert_allclose(g, fd, atol=1e-7, rtol=1e-6)
+    # translation invariance: the gradient sums to zero
+    assert np.abs(g.sum(0)).max() < 1e-12
