@@ -1,0 +1,284 @@
+#!/usr/bin/env python3
+"""Benchmark: all-pairs best-fit RMSD (rms2d) on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Headline workload: the rms2d matrix of 50,000 frames x 1,200 atoms (seeded
+random-walk chain, AR(1) drift rho=0.99 sigma=0.3 A, uniform random rigid
+motion), packed upper triangle of 1,249,975,000 float RMSDs.  One step = one
+complete rmsd_pairwise call (per-frame stage + contraction + FP64 QCP) with the
+frames already resident in HBM.
+
+value = pairs/s of the split-BF16 MFMA kernel (hi*hi + hi*lo + lo*hi, FP32
+accumulation; FP64 centroid / E0 / QCP).  Reported beside it: the plain FP64
+kernel and the FP32-MFMA kernel on the same matrix, the max / RMS error of the
+split-BF16 matrix against the FP64 one over every pair, the FP64 kernel against
+the CPU oracle on sampled rows, the contraction kernel's roofline, the
+reference's own per-pair code timed on the host cores (cpu_baseline), and the
+streaming fit-to-reference workload (1,000,000 frames x 3,000 atoms) with its
+HBM bandwidth.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "RMSD pairs/s: MFMA split-BF16 vs plain FP64 kernel at ≤1e-4 Å max error"
+PEAK_BF16_TFLOPS = 2500.0   # MI355X dense bf16 MFMA
+PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def timed_steps(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(steps):
+        fn()
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / steps  # ms per step
+
+
+def profiled_main_ms(ctx, fn, reps):
+    """Average (per-frame stage, main kernel) ms over `reps` profiled calls."""
+    ctx.set_profiling(True)
+    pre, main = [], []
+    for _ in range(reps):
+        fn()
+        s = ctx.stage_ms()
+        pre.append(s[0])
+        main.append(s[1])
+    ctx.set_profiling(False)
+    return float(np.mean(pre)), float(np.mean(main))
+
+
+def chunked_err(a, b, chunk=1 << 27):
+    import torch
+    mx = 0.0
+    ss = 0.0
+    n = a.numel()
+    for s in range(0, n, chunk):
+        d = (a[s:s + chunk].double() - b[s:s + chunk].double()).abs()
+        mx = max(mx, float(d.max()))
+        ss += float((d * d).sum())
+    return mx, (ss / max(n, 1)) ** 0.5
+
+
+def cpu_baseline(X, n_threads, target_s=12.0):
+    """The reference's Frame::RMSD_CenteredRef (compiled from its sources) in the
+    rms2d loop nest, on evenly spaced reference rows; pairs/s extrapolates exactly
+    because every pair of a row is computed."""
+    F = X.shape[0]
+    try:
+        from oracle.oracle import RefLib, ref_available
+        if not ref_available():
+            raise FileNotFoundError
+        lib, kind = RefLib(), "reference"
+        run = lambda rows: lib.rms2d_rows(X, rows, nthreads=n_threads, want_output=False)
+    except Exception:
+        from oracle.oracle import Oracle
+        lib, kind = Oracle(), "port"
+        run = lambda rows: len(lib.rms2d_rows(X, rows, nthreads=n_threads))
+    probe = np.linspace(0, F - 2, max(2, n_threads), dtype=np.int32)
+    t0 = time.perf_counter()
+    n = run(probe)
+    rate = n / (time.perf_counter() - t0)
+    avg_row = (F - 1) / 2.0
+    nrows = int(max(n_threads, min(F - 1, target_s * rate / avg_row)))
+    rows = np.unique(np.linspace(0, F - 2, nrows).astype(np.int32))
+    t0 = time.perf_counter()
+    npairs = run(rows)
+    dt = time.perf_counter() - t0
+    return {"value": npairs / dt, "unit": "pairs/s", "cores": n_threads, "kind": kind,
+            "sample": f"{len(rows)} evenly spaced reference rows of the 50000-frame matrix "
+                      f"({npairs} pairs, {dt:.1f} s), OpenMP dynamic over rows, "
+                      f"FP64 Kabsch per pair as in Analysis_Rms2d::Calculate_2D"}
+
+
+def fit_stream(ctx, F, n, steps, warmup):
+    """Streaming fit of F frames x n atoms (FP32 SoA) to frame 0, unweighted and
+    mass-weighted, nomod.  Returns throughput and HBM bandwidth of the moments
+    kernel (algorithmic bytes: 12 n per frame read + 8 B RMSD written)."""
+    import torch
+    import cpptraj_amd as ca
+    from cpptraj_amd import synth
+
+    X, ref = synth.noisy_soa_torch(1, F, n, 1.0, ld=n)
+    m = torch.as_tensor(synth.masses(np.random.default_rng(1), n), dtype=torch.float64).cuda()
+    d = ca.frames_desc(X, None, m, soa=True)
+    rmsd = torch.empty(F, dtype=torch.float64, device="cuda")
+    out = {}
+    for use_mass in (False, True):
+        fn = lambda: ctx.fit_desc(d, d, 0, True, use_mass, 0, rmsd)
+        ms = timed_steps(fn, steps, warmup)
+        pre, main = profiled_main_ms(ctx, fn, 3)
+        algo_bytes = F * (12.0 * n + 8.0)
+        key = "mass" if use_mass else "nomass"
+        out[key] = {"frames_per_s": F / (ms / 1e3), "ms_per_step": ms,
+                    "moments_kernel_ms": main, "hbm_gbs": algo_bytes / (main / 1e3) / 1e9}
+    # parity on a sample of frames: FP64 oracle on the host
+    from oracle.oracle import Oracle
+    o = Oracle()
+    idx = np.linspace(0, F - 1, 64).astype(np.int64)
+    Xs = X[idx].cpu().numpy()  # (64, 3, n)
+    aos = np.transpose(Xs, (0, 2, 1)).reshape(64, -1)
+    ref0 = np.transpose(X[0].cpu().numpy(), (1, 0)).astype(np.float64)
+    rc, rt = o.center_on_origin(ref0)
+    r_o, _, _, _ = o.fit_frames(aos, rc, rt)
+    ctx.fit_desc(d, d, 0, True, False, 0, rmsd)
+    torch.cuda.synchronize()
+    got = rmsd[idx].cpu().numpy()
+    err = np.abs(got - r_o)
+    err[0] = 0.0 if abs(got[0]) < 1e-5 else err[0]  # frame 0 fitted to itself: RMSD ~ 0
+    out["max_err_vs_oracle_64_frames"] = float(err.max())
+    out["workload"] = f"{F} frames x {n} atoms FP32 SoA, fit to frame 0, FP64 QCP, nomod"
+    del X
+    torch.cuda.empty_cache()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--frames", type=int, default=50000)
+    ap.add_argument("--atoms", type=int, default=1200)
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-fit", action="store_true")
+    ap.add_argument("--only", choices=["bf16x3", "fp32", "fp64"], default=None,
+                    help="time one precision only (profiling runs)")
+    ap.add_argument("--fit-frames", type=int, default=1000000)
+    ap.add_argument("--fit-atoms", type=int, default=3000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(see DESIGN.md for how the pair tiles would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import cpptraj_amd as ca
+    from cpptraj_amd import synth
+
+    assert torch.cuda.is_available(), "bench needs a GPU"
+    F, n = args.frames, args.atoms
+    npairs = F * (F - 1) // 2
+    log(f"generating drift workload {F} x {n}")
+    t0 = time.time()
+    X = torch.as_tensor(synth.drift(2, F, n).reshape(F, -1)).cuda()
+    log(f"generated in {time.time() - t0:.1f} s")
+    ctx = ca.Context(0)
+    d = ca.frames_desc(X)
+    outs = {}
+    res = {}
+
+    def run(prec):
+        out = outs.get(prec)
+        if out is None:
+            out = torch.empty(npairs, dtype=torch.float32, device="cuda")
+            outs[prec] = out
+        fn = lambda: ctx.pairwise_desc(d, None, 0, False, ca.PRECISIONS[prec], out)
+        return fn
+
+    precs = [args.only] if args.only else ["bf16x3", "fp32", "fp64"]
+    for prec in precs:
+        fn = run(prec)
+        steps = args.fp64_steps if prec == "fp64" else args.steps
+        warm = 1 if prec == "fp64" else args.warmup
+        ms = timed_steps(fn, steps, warm)
+        pre, main = profiled_main_ms(ctx, fn, 2 if prec == "fp64" else 3)
+        res[prec] = {"ms_per_step": ms, "pairs_per_s": npairs / (ms / 1e3),
+                     "prep_ms": pre, "contraction_ms": main}
+        log(f"{prec}: {ms:.2f} ms/step, {npairs / (ms / 1e3):.4g} pairs/s "
+            f"(prep {pre:.2f} ms, contraction {main:.2f} ms)")
+
+    head = args.only or "bf16x3"
+    flops_pair = 18.0 * n
+    algo_tflops = npairs * flops_pair / (res[head]["contraction_ms"] / 1e3) / 1e12
+    peak = {"bf16x3": PEAK_BF16_TFLOPS, "fp32": PEAK_FP32_TFLOPS, "fp64": PEAK_FP64_TFLOPS}[head]
+    terms = 3 if head == "bf16x3" else 1
+    line = {
+        "metric": METRIC,
+        "value": res[head]["pairs_per_s"],
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res[head]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"bf16x3": "bf16", "fp32": "f32", "fp64": "f64"}[head],
+        "data": "synthetic (seeded random-walk chain + AR(1) drift + rigid motion)",
+        "config": {"workload": f"rms2d all-pairs best-fit RMSD, {F} frames x {n} atoms, "
+                               "unweighted, fit, packed upper-triangle float output",
+                   "frames": F, "atoms": n, "pairs": npairs,
+                   "precision": {"bf16x3": "split-BF16 3-term MFMA, FP32 accumulate, "
+                                           "FP64 centroid/E0/QCP",
+                                 "fp32": "FP32 MFMA, FP64 centroid/E0/QCP",
+                                 "fp64": "FP64 MFMA, FP64 QCP"}[head]},
+        "kernels": res,
+        "roofline": {"bound": "mfma", "achieved": algo_tflops, "peak": peak,
+                     "unit": "TFLOP/s", "frac": algo_tflops / peak, "traffic": None,
+                     "kernel": f"k_pairwise<{head}>",
+                     "algorithmic_flops_per_launch": npairs * flops_pair,
+                     "executed_mfma_tflops": algo_tflops * terms},
+    }
+    if "fp64" in res and head != "fp64":
+        line["fp64_pairs_per_s"] = res["fp64"]["pairs_per_s"]
+        line["speedup_vs_fp64"] = res[head]["pairs_per_s"] / res["fp64"]["pairs_per_s"]
+        for prec in ("bf16x3", "fp32"):
+            if prec in outs:
+                mx, rms = chunked_err(outs[prec], outs["fp64"])
+                line[f"{prec}_max_err_vs_fp64"] = mx
+                line[f"{prec}_rms_err_vs_fp64"] = rms
+                log(f"{prec} vs fp64: max {mx:.3e} A, rms {rms:.3e} A over {npairs} pairs")
+        # FP64 kernel vs CPU oracle on sampled rows
+        from oracle.oracle import Oracle
+        o = Oracle()
+        rows = np.linspace(0, F - 2, 8).astype(np.int32)
+        Xh = X.cpu().numpy()
+        exp = o.rms2d_rows(Xh, rows, nthreads=min(16, os.cpu_count() or 1))
+        starts = [int(r) * F - int(r) * (int(r) + 1) // 2 for r in rows]
+        got = np.concatenate([outs["fp64"][s:s + F - 1 - int(r)].cpu().numpy()
+                              for s, r in zip(starts, rows)])
+        line["fp64_max_err_vs_oracle_sampled_rows"] = float(np.abs(got.astype(np.float64) - exp).max())
+        line["fp64_oracle_sample"] = f"rows {rows.tolist()} ({len(exp)} pairs)"
+    if not args.skip_cpu:
+        n_threads = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline on {n_threads} threads")
+        line["cpu_baseline"] = cpu_baseline(X.cpu().numpy(), n_threads)
+        log(f"cpu baseline {line['cpu_baseline']['value']:.4g} pairs/s")
+    for o_ in list(outs):
+        del outs[o_]
+    del X
+    torch.cuda.empty_cache()
+    if not args.skip_fit:
+        log(f"fit stream {args.fit_frames} x {args.fit_atoms}")
+        line["fit_stream"] = fit_stream(ctx, args.fit_frames, args.fit_atoms, args.steps,
+                                        args.warmup)
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -76,6 +76,10 @@ def load_library():
     L.rmsd_ctx_synchronize.restype = C.c_int
     L.rmsd_ctx_scratch_bytes.argtypes = [P]
     L.rmsd_ctx_scratch_bytes.restype = C.c_int64
+    L.rmsd_ctx_set_profiling.argtypes = [P, C.c_int]
+    L.rmsd_ctx_set_profiling.restype = C.c_int
+    L.rmsd_ctx_stage_ms.argtypes = [P, C.POINTER(C.c_double)]
+    L.rmsd_ctx_stage_ms.restype = C.c_int
     FP = C.POINTER(Frames)
     L.rmsd_pairwise.argtypes = [P, FP, FP, C.c_int, C.c_int, C.c_int, P]
     L.rmsd_pairwise.restype = C.c_int
@@ -188,6 +192,15 @@ class Context:
 
     def scratch_bytes(self) -> int:
         return int(self._L.rmsd_ctx_scratch_bytes(self._h))
+
+    def set_profiling(self, on: bool = True):
+        self.check(self._L.rmsd_ctx_set_profiling(self._h, int(on)))
+
+    def stage_ms(self):
+        """(per-frame stage, main kernel, epilogue) milliseconds of the last call."""
+        ms = (C.c_double * 3)()
+        self.check(self._L.rmsd_ctx_stage_ms(self._h, ms))
+        return tuple(ms)
 
     # -- raw entry points on descriptors ----------------------------------------
     def pairwise_desc(self, tgt: Frames, ref: Frames | None, mode: int, use_mass: bool,

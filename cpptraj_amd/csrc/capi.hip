@@ -42,7 +42,23 @@ struct rmsd_ctx {
   std::string err;
   std::map<std::string, DevBuf> bufs;
   TileSchedule sched;
+  // Stage timing: events recorded on the context's stream around the per-frame
+  // stage, the main kernel and the epilogue stage of the last call.
+  bool profiling = false;
+  bool events_made = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  int marks = 0;
 };
+
+namespace {
+
+void mark(rmsd_ctx* c, int k) {
+  if (!c->profiling || !c->events_made) return;
+  (void)hipEventRecord(c->ev[k], c->stream);
+  c->marks |= 1 << k;
+}
+
+}  // namespace
 
 namespace {
 
@@ -209,14 +225,19 @@ int pairwise_impl(rmsd_ctx* c, const FrameSetDev& A, const FrameSetDev& B, int t
   double* P = consts + 1;    // template Gram matrix (9)
   double* gT = consts + 10;  // template norm
 
+  c->marks = 0;
+  mark(c, 0);
   if (mode == RMSD_MODE_NOFIT) {
     SetupArgs s{};
     s.fs = A;
     s.smass = wmass;
     s.total_mass = M;
     HIP_TRY(c, rmsd_launch_setup(s, c->stream));
+    mark(c, 1);
     HIP_TRY(c, rmsd_launch_pair_nofit_tiles(A, B, wmass, M, tiles, ntiles, per_xcd, NXCD, tri, out,
                                             c->stream));
+    mark(c, 2);
+    mark(c, 3);
     return RMSD_OK;
   }
 
@@ -266,7 +287,10 @@ int pairwise_impl(rmsd_ctx* c, const FrameSetDev& A, const FrameSetDev& B, int t
   p.clamp = mode == RMSD_MODE_FIT ? 1 : 0;
   p.out = out;
   (void)rows_are_ref;
+  mark(c, 1);
   HIP_TRY(c, rmsd_launch_pairwise(prec, p, c->stream));
+  mark(c, 2);
+  mark(c, 3);
   return RMSD_OK;
 }
 
@@ -293,6 +317,8 @@ void rmsd_ctx_destroy(rmsd_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   for (auto& kv : ctx->bufs)
     if (kv.second.p) (void)hipFree(kv.second.p);
+  if (ctx->events_made)
+    for (int k = 0; k < 4; ++k) (void)hipEventDestroy(ctx->ev[k]);
   delete ctx;
 }
 
@@ -302,6 +328,32 @@ int rmsd_ctx_synchronize(rmsd_ctx* ctx) {
   if (!ctx) return RMSD_ERR_ARG;
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return RMSD_OK;
+}
+
+int rmsd_ctx_set_profiling(rmsd_ctx* ctx, int enable) {
+  if (!ctx) return RMSD_ERR_ARG;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (enable && !ctx->events_made) {
+    for (int k = 0; k < 4; ++k) HIP_TRY(ctx, hipEventCreate(&ctx->ev[k]));
+    ctx->events_made = true;
+  }
+  ctx->profiling = enable != 0;
+  ctx->marks = 0;
+  return RMSD_OK;
+}
+
+int rmsd_ctx_stage_ms(rmsd_ctx* ctx, double* ms) {
+  if (!ctx || !ms) return RMSD_ERR_ARG;
+  if (!ctx->profiling || ctx->marks != 15)
+    return fail(ctx, RMSD_ERR_ARG, "no profiled call recorded (rmsd_ctx_set_profiling)");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipEventSynchronize(ctx->ev[3]));
+  for (int k = 0; k < 3; ++k) {
+    float t = 0.0f;
+    HIP_TRY(ctx, hipEventElapsedTime(&t, ctx->ev[k], ctx->ev[k + 1]));
+    ms[k] = (double)t;
+  }
   return RMSD_OK;
 }
 
@@ -384,6 +436,8 @@ int rmsd_fit_to_ref(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames*
   }
   const FrameSetDev F = to_dev(*frames);
   const FrameSetDev R = to_dev(*ref);
+  ctx->marks = 0;
+  mark(ctx, 0);
   HIP_TRY(ctx, rmsd_launch_ref_prep(R, ref_frame, use_mass ? ref->mass : nullptr, wpad, fit ? 1 : 0,
                                     refbuf, ref_trans, consts, ctx->stream));
   if (!fit) {
@@ -391,7 +445,10 @@ int rmsd_fit_to_ref(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames*
     a.fs = F;
     a.mass = wpad;
     a.ref = refbuf;
+    mark(ctx, 1);
     HIP_TRY(ctx, rmsd_launch_nofit_frames(a, consts, rmsd, ctx->stream));
+    mark(ctx, 2);
+    mark(ctx, 3);
     return RMSD_OK;
   }
   double* moments = (double*)scratch(ctx, "fit.moments", sizeof(double) * 16 * nf, &st);
@@ -407,7 +464,9 @@ int rmsd_fit_to_ref(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames*
   fa.ref = refbuf;
   fa.consts = consts;
   fa.moments = moments;
+  mark(ctx, 1);
   HIP_TRY(ctx, rmsd_launch_fit_moments(fa, ctx->stream));
+  mark(ctx, 2);
   SolveArgs sa;
   sa.moments = moments;
   sa.consts = consts;
@@ -430,6 +489,7 @@ int rmsd_fit_to_ref(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames*
     ap.xform = xform;
     HIP_TRY(ctx, rmsd_launch_apply(ap, modify == RMSD_MODIFY_ROT_TRANS ? 1 : 2, ctx->stream));
   }
+  mark(ctx, 3);
   return RMSD_OK;
 }
 

@@ -99,6 +99,16 @@ int rmsd_ctx_synchronize(rmsd_ctx* ctx);
 /* Bytes of device scratch currently held by the context. */
 int64_t rmsd_ctx_scratch_bytes(const rmsd_ctx* ctx);
 
+/* Stage timing (the counterpart of cpptraj's Timer around an analysis,
+ * src/Timer.h:10-20, at kernel granularity).  With profiling on, every call
+ * records HIP events on the context's stream; rmsd_ctx_stage_ms waits for the
+ * last call and returns three durations in milliseconds:
+ *   ms[0] per-frame stage (setup, gather/centre/pack; reference preparation),
+ *   ms[1] the main kernel (pair contraction + QCP; streaming moments),
+ *   ms[2] per-frame epilogue (fit solve / coordinate update; 0 for pairwise). */
+int rmsd_ctx_set_profiling(rmsd_ctx* ctx, int enable);
+int rmsd_ctx_stage_ms(rmsd_ctx* ctx, double* ms);
+
 /* All-pairs RMSD matrix.
  * Replaces the nref/ntgt loop nest of Analysis_Rms2d::Calculate_2D
  * (src/Analysis_Rms2d.cpp:196-295):

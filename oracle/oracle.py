@@ -91,6 +91,9 @@ class Oracle:
         L.orc_fit_frames.restype = C.c_int
         L.orc_rmsd_grad_target.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, _dp]
         L.orc_rmsd_grad_target.restype = C.c_double
+        L.orc_rms2d_rows.argtypes = [_fp, C.c_int64, C.c_int, _ip, C.c_int, _dp, C.c_int,
+                                     C.c_int, _ip, C.c_int, _fp, C.c_int]
+        L.orc_rms2d_rows.restype = C.c_int
 
     # -- per-frame routines -------------------------------------------------
     def center_on_origin(self, xyz, mass=None, use_mass=False):
@@ -175,6 +178,21 @@ class Oracle:
             raise MemoryError("orc_rms2d failed")
         return out
 
+    def rms2d_rows(self, frames, rows, sel=None, mass=None, mode=MODE_FIT, use_mass=False,
+                   nthreads=0):
+        """Rows r of the triangle: pairs (r, t > r) concatenated in row order."""
+        fr = _c32(frames)
+        F = fr.shape[0]
+        rw = np.ascontiguousarray(rows, dtype=np.int32)
+        s = _ci(sel)
+        nsel = len(s) if s is not None else fr.shape[1] // 3
+        m = _c64(mass) if mass is not None else np.ones(nsel)
+        out = np.zeros(int(sum(F - 1 - int(r) for r in rw)), dtype=np.float32)
+        if self.L.orc_rms2d_rows(_f(fr), fr.shape[1], F, _i(s), nsel, _d(m), mode,
+                                 int(use_mass), _i(rw), len(rw), _f(out), nthreads):
+            raise MemoryError("orc_rms2d_rows failed")
+        return out
+
     def cluster_cache(self, frames, frame_idx=None, sel=None, mass=None, nofit=False,
                       use_mass=False, weight=1.0, nthreads=0):
         fr = _c32(frames)
@@ -238,6 +256,9 @@ class RefLib:
                                  _ip, _ip, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int,
                                  _fp, C.c_int, C.c_int, C.c_int]
         L.cref_rms2d.restype = C.c_int
+        L.cref_rms2d_rows.argtypes = [_fp, C.c_int64, C.c_int, _ip, C.c_int, _dp, C.c_int,
+                                      C.c_int, _ip, C.c_int, _fp, C.c_int]
+        L.cref_rms2d_rows.restype = C.c_int
         L.cref_fit_frames.argtypes = [_fp, C.c_int64, C.c_int, C.c_int, _ip, C.c_int, _dp,
                                       _dp, _dp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp,
                                       _fp, C.c_int]
@@ -320,6 +341,20 @@ class RefLib:
                           nsel, _d(tm), _d(rm), mode, int(use_mass), int(bool(full)),
                           _f(out), nthreads, r0, r1)
         return out
+
+    def rms2d_rows(self, frames, rows, sel=None, mass=None, mode=MODE_FIT, use_mass=False,
+                   nthreads=0, want_output=True):
+        fr = _c32(frames)
+        F = fr.shape[0]
+        rw = np.ascontiguousarray(rows, dtype=np.int32)
+        s = _ci(sel)
+        nsel = len(s) if s is not None else fr.shape[1] // 3
+        m = _c64(mass) if mass is not None else np.ones(nsel)
+        n_out = int(sum(F - 1 - int(r) for r in rw))
+        out = np.zeros(n_out, dtype=np.float32) if want_output else None
+        self.L.cref_rms2d_rows(_f(fr), fr.shape[1], F, _i(s), nsel, _d(m), mode, int(use_mass),
+                               _i(rw), len(rw), _f(out), nthreads)
+        return out if want_output else n_out
 
     def fit_frames(self, frames, ref_sel_centered, ref_trans, sel=None, mass=None,
                    fit=True, use_mass=False, modify=0, want_frames=False, nthreads=0):

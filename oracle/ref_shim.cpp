@@ -175,6 +175,41 @@ int cref_rms2d(const float* tgt_frames, int64_t tgt_stride, int ntgt,
   return 0;
 }
 
+// Listed reference rows of the same loop nest (triangle case), writing each
+// row's pairs (r, t > r) consecutively; out may be NULL (timing only).
+int cref_rms2d_rows(const float* frames, int64_t stride, int nframes, const int* sel, int nsel,
+                    const double* mass, int mode, int use_mass, const int* rows, int nrows,
+                    float* out, int nthreads) {
+  std::vector<int64_t> off((size_t)nrows + 1, 0);
+  for (int q = 0; q < nrows; ++q) off[q + 1] = off[q] + (nframes - 1 - rows[q]);
+  std::vector<double> zero(3 * (size_t)nsel, 0.0);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+#endif
+  {
+    Frame SelRef = make_frame(zero.data(), mass, nsel);
+    Frame SelTgt = make_frame(zero.data(), mass, nsel);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic)
+#endif
+    for (int q = 0; q < nrows; ++q) {
+      const int r = rows[q];
+      frame_from_floats(SelRef, frames + (int64_t)r * stride, sel, nsel);
+      if (mode == 0 || mode == 2) SelRef.CenterOnOrigin(use_mass != 0);
+      for (int t = r + 1; t < nframes; ++t) {
+        frame_from_floats(SelTgt, frames + (int64_t)t * stride, sel, nsel);
+        float v;
+        if (mode == 0) v = (float)SelTgt.RMSD_CenteredRef(SelRef, use_mass != 0);
+        else if (mode == 1) v = (float)SelTgt.RMSD_NoFit(SelRef, use_mass != 0);
+        else v = (float)QuaternionRMSD_CenteredRef(SelRef, SelTgt, use_mass != 0);
+        if (out) out[off[q] + (t - r - 1)] = v;
+      }
+    }
+  }
+  return 0;
+}
+
 // Per-frame body of Action_Rmsd::DoAction (src/Action_Rmsd.cpp:361-417) for a
 // fixed, already centred reference: RMSD_CenteredRef (or RMSD_NoFit), then the
 // coordinate update of the chosen mode (1 = Trans_Rot_Trans, 2 = translate).

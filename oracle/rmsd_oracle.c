@@ -711,3 +711,59 @@ double orc_rmsd_grad_target(const double *tgt_in, const double *ref_centered,
   free(T);
   return r;
 }
+
+/* Selected rows of the triangle of orc_rms2d (same frames and mask as target
+ * and reference): for each listed reference row r, the pairs (r, t > r) are
+ * written consecutively into out (row after row, in list order).  Used to
+ * check sampled rows of matrices too large for the whole oracle.            */
+int orc_rms2d_rows(const float *frames, int64_t stride, int nframes, const int *sel,
+                   int nsel, const double *mass, int mode, int use_mass,
+                   const int *rows, int nrows, float *out, int nthreads)
+{
+  int64_t *off = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nrows + 1));
+  if (!off) return 1;
+  off[0] = 0;
+  for (int q = 0; q < nrows; ++q) off[q + 1] = off[q] + (nframes - 1 - rows[q]);
+  int err = 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+#endif
+  {
+    double *R = (double *)malloc(sizeof(double) * 3 * (size_t)nsel);
+    double *T = (double *)malloc(sizeof(double) * 3 * (size_t)nsel);
+    if (!R || !T) {
+#ifdef _OPENMP
+#pragma omp atomic write
+#endif
+      err = 1;
+    }
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic)
+#endif
+    for (int q = 0; q < nrows; ++q) {
+      if (!R || !T) continue;
+      const int r = rows[q];
+      orc_gather(frames + (int64_t)r * stride, sel, nsel, R);
+      if (mode == ORC_MODE_FIT || mode == ORC_MODE_QUAT) {
+        double c[3];
+        orc_center_on_origin(R, mass, nsel, use_mass, c);
+      }
+      for (int t = r + 1; t < nframes; ++t) {
+        orc_gather(frames + (int64_t)t * stride, sel, nsel, T);
+        double v, U[9], tr[3];
+        if (mode == ORC_MODE_FIT)
+          v = orc_rmsd_centered_ref(T, R, mass, nsel, use_mass, U, tr);
+        else if (mode == ORC_MODE_NOFIT)
+          v = orc_rmsd_nofit(T, R, mass, nsel, use_mass);
+        else
+          v = orc_qrmsd_centered_ref(R, T, mass, nsel, use_mass, 0, tr);
+        out[off[q] + (t - r - 1)] = (float)v;
+      }
+    }
+    free(R);
+    free(T);
+  }
+  free(off);
+  return err;
+}
