@@ -21,10 +21,11 @@ import ctypes as C
 import os
 
 __all__ = ["RmsdError", "Context", "frames_desc", "rms2d", "cluster_cache", "rmsd_fit",
-           "library_path", "load_library", "MODES", "PRECISIONS"]
+           "library_path", "load_library", "MODES", "PRECISIONS", "MODIFY"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_lib", "libcpptraj_rmsd.so")
+# CPPTRAJ_RMSD_LIB selects another build of the same library (e.g. a tuned variant).
+_LIB_PATH = os.environ.get("CPPTRAJ_RMSD_LIB") or os.path.join(_HERE, "_lib", "libcpptraj_rmsd.so")
 
 MODES = {"fit": 0, "nofit": 1, "qrmsd": 2}
 PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2}

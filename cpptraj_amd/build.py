@@ -15,6 +15,7 @@ from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libcpptraj_rmsd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -31,7 +32,7 @@ FLAGS = [
 
 def _compile(src: str, verbose: bool) -> str:
     obj = os.path.join(LIBDIR, "obj", os.path.basename(src) + ".o")
-    cmd = [HIPCC, *FLAGS, "-I", CSRC, "-c", src, "-o", obj]
+    cmd = [HIPCC, *FLAGS, "-I", CSRC, "-I", INCLUDE, "-c", src, "-o", obj]
     if verbose:
         cmd.append("-Rpass-analysis=kernel-resource-usage")
     r = subprocess.run(cmd, capture_output=True, text=True)

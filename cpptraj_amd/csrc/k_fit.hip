@@ -189,6 +189,101 @@ __global__ __launch_bounds__(FIT_THREADS) void k_fit_moments_any(FitArgs a, int 
     for (int i = 0; i < 13; ++i) a.moments[13 * f + i] = v[i];
 }
 
+// Persistent path: each thread owns up to APT atoms (k = tid + 256 i, or with
+// VEC groups of four consecutive atoms) and keeps their reference coordinates,
+// weights and storage offsets in registers for every frame it visits, so HBM
+// carries the frames only.  A block walks frames f = blockIdx.x, + gridDim.x,
+// ... and prefetches the next frame's coordinates before reducing the current
+// one.  It also stores the per-frame shift (first selected atom).
+template <int NT, int APT, bool VEC, bool WEIGHTED>
+__global__ __launch_bounds__(NT, 512 / NT) void k_fit_moments_persist(FitArgs a, int ldr,
+                                                                       double* shift) {
+  __shared__ double red[13 * (NT / 64)];
+  const int tid = threadIdx.x;
+  const int n = a.fs.nsel;
+  const int64_t ld = a.fs.ld;
+  int off[APT][3];
+  bool act[APT];
+  double yr[APT][3], wt[APT];
+#pragma unroll
+  for (int i = 0; i < APT; ++i) {
+    const int k = VEC ? 4 * (tid + NT * (i / 4)) + (i % 4) : tid + NT * i;
+    act[i] = k < n;
+    const int atom = act[i] ? (a.fs.sel ? a.fs.sel[k] : k) : 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      off[i][c] = (int)(ld ? c * ld + atom : 3 * atom + c);
+      yr[i][c] = act[i] ? a.ref[c * ldr + k] : 0.0;
+    }
+    wt[i] = act[i] ? (WEIGHTED ? a.mass[k] : 1.0) : 0.0;
+  }
+  const int a0 = a.fs.sel ? a.fs.sel[0] : 0;
+  int off0[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) off0[c] = (int)(ld ? c * ld + a0 : 3 * a0 + c);
+
+  float xv[APT][3], s0[3];
+  auto fetch = [&](int64_t f) {
+    const float* fr = a.fs.xyz + frame_of(a.fs, f) * a.fs.stride;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) s0[c] = fr[off0[c]];
+    if (VEC) {
+#pragma unroll
+      for (int g = 0; g < APT / 4; ++g) {
+        if (!act[4 * g]) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) xv[4 * g + e][0] = xv[4 * g + e][1] = xv[4 * g + e][2] = 0.f;
+          continue;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(fr + off[4 * g][c]));
+          xv[4 * g][c] = v.x;
+          xv[4 * g + 1][c] = v.y;
+          xv[4 * g + 2][c] = v.z;
+          xv[4 * g + 3][c] = v.w;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < APT; ++i)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xv[i][c] = act[i] ? __builtin_nontemporal_load(fr + off[i][c]) : 0.f;
+    }
+  };
+
+  int64_t f = blockIdx.x;
+  if (f >= a.fs.nframes) return;
+  fetch(f);
+  for (; f < a.fs.nframes; f += gridDim.x) {
+    float cur[APT][3], sc[3];
+#pragma unroll
+    for (int i = 0; i < APT; ++i)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) cur[i][c] = xv[i][c];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sc[c] = s0[c];
+    const int64_t fn = f + gridDim.x;
+    if (fn < a.fs.nframes) fetch(fn);
+    double v[13];
+#pragma unroll
+    for (int i = 0; i < 13; ++i) v[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < APT; ++i) {
+      // inactive atoms carry weight 0 and zero coordinates
+      const double x0 = act[i] ? (double)cur[i][0] - sc[0] : 0.0;
+      const double x1 = act[i] ? (double)cur[i][1] - sc[1] : 0.0;
+      const double x2 = act[i] ? (double)cur[i][2] - sc[2] : 0.0;
+      accum<true>(v, x0, x1, x2, wt[i], yr[i][0], yr[i][1], yr[i][2]);
+    }
+    rmsd_dev::block_sum<13>(v, red);
+    if (tid == 0) {
+      for (int i = 0; i < 13; ++i) a.moments[13 * f + i] = v[i];
+      for (int c = 0; c < 3; ++c) shift[3 * f + c] = sc[c];
+    }
+  }
+}
+
 // Per-frame shift used by the moment kernels (first selected atom).
 __global__ void k_fit_shift(FrameSetDev fs, double* shift) {
   const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -317,10 +412,32 @@ hipError_t rmsd_launch_fit_moments(const FitArgs& a, hipStream_t s) {
   const int ldr = (a.fs.nsel + 3) & ~3;
   const int nf = a.fs.nframes;
   double* shift = a.moments + 13 * (int64_t)nf;
-  hipLaunchKernelGGL(k_fit_shift, dim3((nf + 255) / 256), dim3(256), 0, s, a.fs, shift);
   const bool vec = a.fs.sel == nullptr && a.fs.ld > 0 && (a.fs.ld % 4) == 0 &&
                    (a.fs.stride % 4) == 0 && ((uintptr_t)a.fs.xyz % 16) == 0;
   const bool w = a.mass != nullptr;
+  // Persistent register-resident path: 256 or 512 threads with up to 12
+  // (vector path) or 8 (scalar path) atoms per thread, i.e. up to 6144 / 4096
+  // selected atoms; larger selections take the one-block-per-frame kernels.
+  const int grid = nf < 1024 ? nf : 1024;
+  const int n = a.fs.nsel;
+#define RMSD_FIT_PERSIST(NT, N, V)                                                          \
+  if (vec == V && n <= NT * N) {                                                           \
+    if (w) hipLaunchKernelGGL((k_fit_moments_persist<NT, N, V, true>), dim3(grid), dim3(NT), \
+                              0, s, a, ldr, shift);                                        \
+    else hipLaunchKernelGGL((k_fit_moments_persist<NT, N, V, false>), dim3(grid), dim3(NT), \
+                            0, s, a, ldr, shift);                                          \
+    return hipGetLastError();                                                              \
+  }
+  RMSD_FIT_PERSIST(256, 4, true)
+  RMSD_FIT_PERSIST(256, 8, true)
+  RMSD_FIT_PERSIST(256, 12, true)
+  RMSD_FIT_PERSIST(512, 8, true)
+  RMSD_FIT_PERSIST(512, 12, true)
+  RMSD_FIT_PERSIST(256, 4, false)
+  RMSD_FIT_PERSIST(256, 8, false)
+  RMSD_FIT_PERSIST(512, 8, false)
+#undef RMSD_FIT_PERSIST
+  hipLaunchKernelGGL(k_fit_shift, dim3((nf + 255) / 256), dim3(256), 0, s, a.fs, shift);
   if (vec) {
     if (w) hipLaunchKernelGGL(k_fit_moments_soa4<true>, dim3(nf), dim3(FIT_THREADS), 0, s, a, ldr);
     else hipLaunchKernelGGL(k_fit_moments_soa4<false>, dim3(nf), dim3(FIT_THREADS), 0, s, a, ldr);

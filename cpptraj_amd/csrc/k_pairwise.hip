@@ -30,6 +30,7 @@
 // (T the template, all per-frame terms FP64), then E0 and QCP run in FP64.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 #include "rmsd_device.hpp"
 #include "rmsd_kernels.hpp"
 
@@ -101,6 +102,90 @@ __device__ __forceinline__ double acc_elem(const typename PrecTraits<PREC>::acc_
 __device__ __forceinline__ const u32x4* frag_ptr(const u32x4* base, int64_t fb, int kb,
                                                  int KB, int c, int p, int PARTS) {
   return base + ((((fb * KB + kb) * 3 + c) * PARTS + p) << 6);
+}
+
+// ---- FP64 epilogue --------------------------------------------------------------
+// Reassembles each pair's 3x3 covariance (plus the FP64 template terms for the
+// MFMA-reduced forms), forms E0 and runs QCP for the four pairs a lane holds in
+// one accumulator column block together (G at a time), so that their
+// independent Newton iterations overlap.
+// The per-frame terms are read from gA/uA (row frames, indexed i - rbase) and
+// gB/uB (column frames, j - cbase): global memory or an LDS copy.
+template <int PREC, int WR, int WC, int G = (PREC == PREC_F64 ? 1 : 4)>
+__device__ __forceinline__ void pair_epilogue(const PairArgs& args,
+                                              typename PrecTraits<PREC>::acc_t (&acc)[WR][WC][3][3],
+                                              int64_t fbA, int64_t fbB, int lane,
+                                              const double* gA, const double* uA, int64_t rbase,
+                                              const double* gB, const double* uB, int64_t cbase) {
+  const double M = args.total_mass[0];
+  double Pm[9];
+  if (PREC != PREC_F64) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Pm[k] = args.P[k];
+  }
+#pragma unroll
+  for (int q = 0; q < WC; ++q) {
+    const int64_t j = (fbB + q) * 16 + (lane & 15);  // column frame
+    double gj = 0.0, uj[9];
+    const bool jok = j < args.FB;
+    if (jok) {
+      gj = gB[j - cbase];
+      if (PREC != PREC_F64) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) uj[k] = uB[9 * (j - cbase) + k];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < WR; ++r) {
+#pragma unroll
+      for (int v0 = 0; v0 < 4; v0 += G) {
+        double R[G][9], E0[G], res[G];
+        bool act[G];
+        int64_t idx[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const int v = v0 + g;
+          // Accumulator layout (checked on gfx950): FP32 16x16 blocks hold row
+          // 4*(lane/16) + v in item v; the FP64 16x16x4 block holds row
+          // (lane/16) + 4*v.  Columns are lane % 16 in both.
+          const int rr = PREC == PREC_F64 ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v;
+          const int64_t i = (fbA + r) * 16 + rr;  // row frame
+          act[g] = jok && i < args.FA && (!args.tri || i < j);
+          idx[g] = args.tri ? rmsd_dev::tri_index(args.FA, i, j) : i * args.FB + j;
+          double S[9];
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) S[3 * c + d] = acc_elem<PREC>(acc[r][q][c][d], v);
+          if (PREC != PREC_F64 && act[g]) {
+            const double* ui = uA + 9 * (i - rbase);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+              for (int d = 0; d < 3; ++d)
+                S[3 * c + d] += Pm[3 * c + d] + uj[3 * c + d] + ui[3 * d + c];
+          }
+          // Row frames are the reference in the triangle layout (the reference
+          // stores element (nref, ntgt) with ntgt > nref) and the target in
+          // the full layout; QCP takes R = sum w t r^T.
+          if (args.tri) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+              for (int d = 0; d < 3; ++d) R[g][3 * c + d] = S[3 * d + c];
+          } else {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) R[g][k] = S[k];
+          }
+          E0[g] = act[g] ? 0.5 * (gA[i - rbase] + gj) : 0.0;
+        }
+        rmsd_dev::qcp_rmsd_n<G>(R, E0, act, M, args.clamp, res);
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+          if (act[g]) args.out[idx[g]] = M < RMSD_SMALL ? -1.0f : (float)res[g];
+      }
+    }
+  }
 }
 
 template <int PREC, int WR, int WC, int NWR, int NWC>
@@ -183,73 +268,242 @@ __global__ __launch_bounds__(64 * NWR * NWC) void k_pairwise(PairArgs args) {
           for (int d = 0; d < 3; ++d) mma<PREC>(acc[r][q][c][d], ca[r][c], cb[q][d]);
   }
 
-  // ---- FP64 epilogue: reassemble S, E0, QCP, store ------------------------
-  const double M = args.total_mass[0];
-  double Pm[9];
-  if (PREC != PREC_F64) {
+  pair_epilogue<PREC, WR, WC>(args, acc, fbA, fbB, lane, args.gA, args.uA, 0, args.gB, args.uB, 0);
+}
+
+// ---- LDS-staged contraction for the MFMA-reduced forms ----------------------------
+// A 256-thread workgroup owns 64 x 64 frame pairs; each of its four waves owns
+// 32 x 32 (WR = WC = 2).  Per chunk of atoms the workgroup needs 4 row and 4
+// column frame blocks x 3 coordinates x PARTS = NFRAG fragments of 1 KiB.  Each
+// wave fetches a quarter of them from HBM/L2 into registers two chunks ahead,
+// stores them into one of two LDS stages one chunk ahead, and every wave then
+// reads its own 24 (BF16) or 12 (FP32) fragments from LDS.  Every fragment is
+// fetched once per workgroup (instead of once per wave that uses it) and a
+// global load has two chunks of MFMA work to land in.
+template <int PREC>
+__global__ __launch_bounds__(256) void k_pairwise_lds(PairArgs args) {
+  typedef PrecTraits<PREC> T;
+  typedef typename T::acc_t acc_t;
+  constexpr int PARTS = T::PARTS;
+  constexpr int HALF = 4 * 3 * PARTS;       // fragments of one side (A or B)
+  constexpr int NFRAG = 2 * HALF;           // per chunk and workgroup
+  constexpr int PER_WAVE = NFRAG / 4;       // fetched by each wave
+  __shared__ u32x4 lds[2][NFRAG][64];
+
+  const int w = blockIdx.x;
+  const int t = (w % args.nxcd) * args.per_xcd + (w / args.nxcd);
+  if (t >= args.ntiles) return;
+  const int2 tile = args.tiles[t];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int64_t fbA0 = (int64_t)tile.x * 4, fbB0 = (int64_t)tile.y * 4;
+  const u32x4* A = reinterpret_cast<const u32x4*>(args.A);
+  const u32x4* B = reinterpret_cast<const u32x4*>(args.B);
+  const int KB = args.KB;
+
+  // Fetch plan: waves 0-1 fetch the A side, waves 2-3 the B side; each wave
+  // takes two frame blocks, whose 3 * PARTS fragments of one chunk are
+  // contiguous (CS pieces per chunk), so a chunk's fetch is two pointers plus
+  // immediate offsets.
+  constexpr int BF = 3 * PARTS;             // fragments per block and chunk
+  constexpr int64_t CS = BF * 64;           // 16-byte pieces per block and chunk
+  const int side = wave >> 1;
+  const int blk0 = (wave & 1) * 2;
+  const int64_t fb0 = (side ? fbB0 : fbA0) + blk0;
+  const u32x4* src0 = (side ? B : A) + fb0 * KB * CS + lane;
+  const u32x4* src1 = src0 + (int64_t)KB * CS;
+  const int dst0 = side * HALF + blk0 * BF;  // first LDS fragment of this wave
+  u32x4 stg[2][PER_WAVE];
+  // Register stage S holds the chunks of parity S; LDS stage S likewise.
+  auto fetch = [&](auto S_, int kb) {
+    constexpr int S = decltype(S_)::value;
+    const u32x4* p0 = src0 + (int64_t)kb * CS;
+    const u32x4* p1 = src1 + (int64_t)kb * CS;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) Pm[k] = args.P[k];
-  }
-#pragma unroll
-  for (int q = 0; q < WC; ++q) {
-    const int64_t j = (fbB + q) * 16 + (lane & 15);  // column frame
-    double gj = 0.0, uj[9];
-    const bool jok = j < args.FB;
-    if (jok) {
-      gj = args.gB[j];
-      if (PREC != PREC_F64) {
-#pragma unroll
-        for (int k = 0; k < 9; ++k) uj[k] = args.uB[9 * j + k];
-      }
+    for (int e = 0; e < BF; ++e) {
+      stg[S][e] = ld16(p0 + 64 * e);
+      stg[S][BF + e] = ld16(p1 + 64 * e);
     }
+  };
+  auto stash = [&](auto S_) {
+    constexpr int S = decltype(S_)::value;
 #pragma unroll
-    for (int r = 0; r < WR; ++r) {
+    for (int e = 0; e < PER_WAVE; ++e) lds[S][dst0 + e][lane] = stg[S][e];
+  };
+
+  acc_t acc[2][2][3][3];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        // Accumulator layout (checked on gfx950): FP32 16x16 blocks hold row
-        // 4*(lane/16) + v in item v; the FP64 16x16x4 block holds row
-        // (lane/16) + 4*v.  Columns are lane % 16 in both.
-        const int rr = PREC == PREC_F64 ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v;
-        const int64_t i = (fbA + r) * 16 + rr;  // row frame
-        const bool ok = jok && i < args.FA && (!args.tri || i < j);
-        if (!ok) continue;
-        double S[9];
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) acc[r][q][c][d] = acc_t{0, 0, 0, 0};
+
+  auto compute = [&](auto S_) {
+    constexpr int S = decltype(S_)::value;
+    u32x4 fbv[2][3][PARTS];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int p = 0; p < PARTS; ++p)
+          fbv[q][c][p] = lds[S][HALF + ((2 * wc + q) * 3 + c) * PARTS + p][lane];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      u32x4 fav[3][PARTS];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int p = 0; p < PARTS; ++p) fav[c][p] = lds[S][((2 * wr + r) * 3 + c) * PARTS + p][lane];
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
-          for (int d = 0; d < 3; ++d) S[3 * c + d] = acc_elem<PREC>(acc[r][q][c][d], v);
-        if (PREC != PREC_F64) {
-          const double* ui = args.uA + 9 * i;
+          for (int d = 0; d < 3; ++d) mma<PREC>(acc[r][q][c][d], fav[c], fbv[q][d]);
+    }
+  };
+  // One chunk: fetch chunk kb + 2 into the register stage that held chunk kb
+  // (already in LDS), contract chunk kb from LDS stage S, move chunk kb + 1
+  // (fetched one chunk ago) into the other LDS stage, which every wave
+  // finished reading before the previous barrier.
+  // The fetch and the stash are issued unconditionally (past the last chunk
+  // they re-read chunk KB - 1 and fill a stage nobody reads again), so the
+  // load counter waits stay exact: a stash waits only for the loads of the
+  // chunk it stores, never for the chunk fetched in the same step.
+  auto step = [&](auto S_, int kb) {
+    constexpr int S = decltype(S_)::value;
+    std::integral_constant<int, 1 - S> O;
+    fetch(S_, kb + 2 < KB ? kb + 2 : KB - 1);
+    compute(S_);
+    stash(O);
+    __syncthreads();
+  };
+  std::integral_constant<int, 0> S0;
+  std::integral_constant<int, 1> S1;
+  fetch(S0, 0);
+  stash(S0);
+  fetch(S1, KB > 1 ? 1 : 0);
+  __syncthreads();
+  for (int kb = 0; kb < KB; kb += 2) {
+    step(S0, kb);
+    if (kb + 1 < KB) step(S1, kb + 1);
+  }
+
+  // Per-frame FP64 terms of the tile's 64 row and 64 column frames, staged in
+  // LDS so that the epilogue does not wait on global loads pair by pair.
+  __shared__ double eg[2][64];
+  __shared__ double eu[2][64][9];
+  const int64_t r0 = fbA0 * 16, c0 = fbB0 * 16;
+  for (int e = threadIdx.x; e < 2 * 64; e += blockDim.x) {
+    const int side = e >> 6, f = e & 63;
+    const int64_t fr = (side ? c0 : r0) + f;
+    const bool ok = fr < (side ? args.FB : args.FA);
+    eg[side][f] = ok ? (side ? args.gB : args.gA)[fr] : 0.0;
+    const double* u = (side ? args.uB : args.uA) + 9 * fr;
 #pragma unroll
-          for (int c = 0; c < 3; ++c)
+    for (int k = 0; k < 9; ++k) eu[side][f][k] = ok ? u[k] : 0.0;
+  }
+  __syncthreads();
+  pair_epilogue<PREC, 2, 2>(args, acc, fbA0 + 2 * wr, fbB0 + 2 * wc, lane, eg[0], &eu[0][0][0], r0,
+                            eg[1], &eu[1][0][0], c0);
+}
+
+// ---- register-pipelined contraction ---------------------------------------------
+// Same wave tile as k_pairwise (WR x WC blocks of 16 x 16 pairs, all nine
+// (c, d) blocks), no LDS and no barriers: every wave streams its own fragments
+// through NS register stages, refilling a stage right after the chunk in it has
+// been contracted, so each load has NS - 1 chunks of MFMA work to land in.
+template <int PREC, int WR, int WC, int NWR, int NWC, int NS>
+__global__ __launch_bounds__(64 * NWR * NWC) void k_pairwise_rs(PairArgs args) {
+  typedef PrecTraits<PREC> T;
+  typedef typename T::acc_t acc_t;
+  constexpr int PARTS = T::PARTS;
+  constexpr int TILE_R = 16 * WR * NWR;
+  constexpr int TILE_C = 16 * WC * NWC;
+  const int w = blockIdx.x;
+  const int t = (w % args.nxcd) * args.per_xcd + (w / args.nxcd);
+  if (t >= args.ntiles) return;
+  const int2 tile = args.tiles[t];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wr = wave / NWC, wc = wave % NWC;
+  const int64_t fbA = (int64_t)tile.x * (TILE_R / 16) + wr * WR;
+  const int64_t fbB = (int64_t)tile.y * (TILE_C / 16) + wc * WC;
+  const int KB = args.KB;
+  constexpr int64_t CS = 3 * PARTS * 64;  // pieces per block and chunk
+  const u32x4* pA = reinterpret_cast<const u32x4*>(args.A) + fbA * KB * CS + lane;
+  const u32x4* pB = reinterpret_cast<const u32x4*>(args.B) + fbB * KB * CS + lane;
+  const int64_t BS = (int64_t)KB * CS;    // pieces per frame block
+
+  acc_t acc[WR][WC][3][3];
 #pragma unroll
-            for (int d = 0; d < 3; ++d)
-              S[3 * c + d] += Pm[3 * c + d] + uj[3 * c + d] + ui[3 * d + c];
-        }
-        // Row frames are the reference in the triangle layout (the reference
-        // stores element (nref, ntgt) with ntgt > nref) and the target in the
-        // full layout; QCP takes R = sum w t r^T.
-        double R[9];
-        if (args.tri) {
+  for (int r = 0; r < WR; ++r)
 #pragma unroll
-          for (int c = 0; c < 3; ++c)
+    for (int q = 0; q < WC; ++q)
 #pragma unroll
-            for (int d = 0; d < 3; ++d) R[3 * c + d] = S[3 * d + c];
-        } else {
+      for (int c = 0; c < 3; ++c)
 #pragma unroll
-          for (int k = 0; k < 9; ++k) R[k] = S[k];
-        }
-        const double E0 = 0.5 * (args.gA[i] + gj);
-        float out;
-        if (M < RMSD_SMALL)
-          out = -1.0f;
-        else
-          out = (float)rmsd_dev::qcp_rmsd(R, E0, M, args.clamp);
-        const int64_t idx = args.tri ? rmsd_dev::tri_index(args.FA, i, j) : i * args.FB + j;
-        args.out[idx] = out;
+        for (int d = 0; d < 3; ++d) acc[r][q][c][d] = acc_t{0, 0, 0, 0};
+
+  u32x4 fa[NS][WR][3][PARTS], fb[NS][WC][3][PARTS];
+  auto load = [&](auto S_, int kb) {
+    constexpr int S = decltype(S_)::value;
+    const int64_t o = (int64_t)kb * CS;
+#pragma unroll
+    for (int r = 0; r < WR; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int p = 0; p < PARTS; ++p) fa[S][r][c][p] = ld16(pA + r * BS + o + (c * PARTS + p) * 64);
+#pragma unroll
+    for (int q = 0; q < WC; ++q)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int p = 0; p < PARTS; ++p) fb[S][q][c][p] = ld16(pB + q * BS + o + (c * PARTS + p) * 64);
+  };
+  auto compute = [&](auto S_) {
+    constexpr int S = decltype(S_)::value;
+#pragma unroll
+    for (int r = 0; r < WR; ++r)
+#pragma unroll
+      for (int q = 0; q < WC; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) mma<PREC>(acc[r][q][c][d], fa[S][r][c], fb[S][q][d]);
+  };
+  // Loads past the last chunk re-read chunk KB - 1 (in bounds, unused) so that
+  // every path issues the same loads and the counter waits stay exact.
+  auto clampk = [&](int kb) { return kb < KB ? kb : KB - 1; };
+  std::integral_constant<int, 0> S0;
+  std::integral_constant<int, 1> S1;
+  std::integral_constant<int, 2> S2;
+  load(S0, 0);
+  if constexpr (NS > 1) load(S1, clampk(1));
+  if constexpr (NS > 2) load(S2, clampk(2));
+  for (int kb = 0; kb < KB; kb += NS) {
+    compute(S0);
+    load(S0, clampk(kb + NS));
+    if constexpr (NS > 1) {
+      if (kb + 1 < KB) {
+        compute(S1);
+        load(S1, clampk(kb + 1 + NS));
+      }
+    }
+    if constexpr (NS > 2) {
+      if (kb + 2 < KB) {
+        compute(S2);
+        load(S2, clampk(kb + 2 + NS));
       }
     }
   }
+  pair_epilogue<PREC, WR, WC>(args, acc, fbA, fbB, lane, args.gA, args.uA, 0, args.gB, args.uB, 0);
 }
 
 }  // namespace
@@ -266,11 +520,24 @@ static hipError_t launch_tiles(const PairArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <int PREC, int NS>
+static hipError_t launch_rs(const PairArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((k_pairwise_rs<PREC, 2, 2, 2, 2, NS>), dim3(a.per_xcd * a.nxcd), dim3(256), 0,
+                     s, a);
+  return hipGetLastError();
+}
+
+template <int PREC>
+static hipError_t launch_lds(const PairArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((k_pairwise_lds<PREC>), dim3(a.per_xcd * a.nxcd), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 hipError_t rmsd_launch_pairwise(int prec, const PairArgs& a, hipStream_t s) {
   switch (prec) {
     case PREC_F64: return launch_tiles<PREC_F64, 1, 2, 4, 2>(a, s);
-    case PREC_F32: return launch_tiles<PREC_F32, 2, 2, 2, 2>(a, s);
-    case PREC_BF16X3: return launch_tiles<PREC_BF16X3, 2, 2, 2, 2>(a, s);
+    case PREC_F32: return launch_lds<PREC_F32>(a, s);
+    case PREC_BF16X3: return launch_lds<PREC_BF16X3>(a, s);
     default: return hipErrorInvalidValue;
   }
 }

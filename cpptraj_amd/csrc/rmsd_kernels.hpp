@@ -132,3 +132,14 @@ hipError_t rmsd_launch_apply(const ApplyArgs& a, int mode, hipStream_t s);
 hipError_t rmsd_launch_ref_prep(const FrameSetDev& fs, int f, const double* rmass,
                                 const double* tmass, int center, double* ref_out,
                                 double* ref_trans, double* consts, hipStream_t s);
+
+// Gradients (k_grad.hip).
+hipError_t rmsd_launch_fit_grad(const FrameSetDev& fs, const double* mass, const double* ref,
+                                const double* xform, const double* rmsd, const double* gout,
+                                const double* consts, double* grad, hipStream_t s);
+hipError_t rmsd_launch_center(const FrameSetDev& fs, const double* w, double* xc, double* g,
+                              hipStream_t s);
+hipError_t rmsd_launch_pair_grad(const double* XA, const double* XB, const double* gA,
+                                 const double* gB, const double* w, int n, int FA, int FB,
+                                 const double* M, const double* G, double* cw, double* gradA,
+                                 double* gradB, hipStream_t s);

@@ -153,6 +153,25 @@ int rmsd_fit_to_ref(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames*
                     int ref_frame, int fit, int use_mass, int modify, double* rmsd, double* U,
                     double* tvec);
 
+/* Gradient of the fit: no counterpart in the reference (cpptraj has no RMSD
+ * derivative).  For loss = sum_f grad_rmsd[f] * RMSD_f of rmsd_fit_to_ref
+ * (fit, same frames / reference / masses), writes d loss / d x for the selected
+ * atoms of every frame: grad_xyz[(f*nsel + k)*3 + c], FP64.  The rotations are
+ * recomputed from the frames, not taken from the forward call.  Envelope
+ * theorem: w_k (x~_k - U^T y~_k) / (M RMSD_f) per atom; 0 where RMSD_f = 0. */
+int rmsd_fit_backward(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* ref,
+                      int ref_frame, int use_mass, const double* grad_rmsd, double* grad_xyz);
+
+/* Gradient of a full pairwise block (rows = A as targets, columns = B as
+ * references, as in rmsd_pairwise with ref != NULL, fit mode): for
+ * loss = sum_ij grad_out[i*B->nframes + j] * RMSD_ij writes d loss / d A
+ * (grad_a, A->nframes x nsel x 3) and d loss / d B (grad_b, may be NULL),
+ * FP64.  Both sets are centred and weighted with A's masses (same selection
+ * and weights on both sides).  Every pair's rotation is recomputed in FP64. */
+int rmsd_pairwise_backward(rmsd_ctx* ctx, const rmsd_frames* A, const rmsd_frames* B,
+                           int use_mass, const double* grad_out, double* grad_a,
+                           double* grad_b);
+
 #ifdef __cplusplus
 }
 #endif
