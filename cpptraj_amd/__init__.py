@@ -20,8 +20,9 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-__all__ = ["RmsdError", "Context", "frames_desc", "rms2d", "cluster_cache", "rmsd_fit",
-           "library_path", "load_library", "MODES", "PRECISIONS", "MODIFY"]
+__all__ = ["RmsdError", "Context", "Batch", "frames_desc", "rms2d", "rms2d_batch",
+           "cluster_cache", "rmsd_fit", "library_path", "load_library", "MODES", "PRECISIONS",
+           "MODIFY", "BATCH_GRAPH", "BATCH_AUTOTUNE"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 # CPPTRAJ_RMSD_LIB selects another build of the same library (e.g. a tuned variant).
@@ -30,6 +31,8 @@ _LIB_PATH = os.environ.get("CPPTRAJ_RMSD_LIB") or os.path.join(_HERE, "_lib", "l
 MODES = {"fit": 0, "nofit": 1, "qrmsd": 2}
 PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 MODIFY = {"nomod": 0, "rottrans": 1, "norotate": 2}
+BATCH_GRAPH = 1
+BATCH_AUTOTUNE = 2
 
 
 class RmsdError(RuntimeError):
@@ -97,8 +100,19 @@ def _optional(L):
     """Entry points that later parts of the library add (batch, backward)."""
     P = C.c_void_p
     FP = C.POINTER(Frames)
-    if hasattr(L, "rmsd_pairwise_batch"):
-        L.rmsd_pairwise_batch.argtypes = [P, FP, P, C.c_int, C.c_int, C.c_int, C.c_int]
+    if hasattr(L, "rmsd_batch_create"):
+        L.rmsd_batch_create.argtypes = [P, FP, C.c_int, C.c_int, C.c_int, C.c_int, P, C.c_int,
+                                        C.POINTER(P)]
+        L.rmsd_batch_create.restype = C.c_int
+        L.rmsd_batch_run.argtypes = [P]
+        L.rmsd_batch_run.restype = C.c_int
+        L.rmsd_batch_set_variant.argtypes = [P, C.c_int]
+        L.rmsd_batch_set_variant.restype = C.c_int
+        L.rmsd_batch_info.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+        L.rmsd_batch_info.restype = C.c_int
+        L.rmsd_batch_destroy.argtypes = [P]
+        L.rmsd_batch_destroy.restype = None
+        L.rmsd_pairwise_batch.argtypes = [P, FP, C.c_int, C.c_int, C.c_int, C.c_int, P]
         L.rmsd_pairwise_batch.restype = C.c_int
     if hasattr(L, "rmsd_fit_backward"):
         L.rmsd_fit_backward.argtypes = [P, FP, FP, C.c_int, C.c_int, P, P]
@@ -124,14 +138,16 @@ def frames_desc(xyz, sel=None, mass=None, fidx=None, soa=False, nsel=None, natom
         raise RmsdError("frames must be a float32 CUDA tensor")
     F = xyz.shape[0]
     if soa:
-        if xyz.dim() != 3 or xyz.shape[1] != 3 or xyz.stride(2) != 1 or xyz.stride(1) < 1:
+        empty = xyz.numel() == 0
+        if xyz.dim() != 3 or xyz.shape[1] != 3 or (
+                not empty and (xyz.stride(2) != 1 or xyz.stride(1) < 1)):
             raise RmsdError("SoA frames must be (F, 3, ld) with unit atom stride")
-        ld = xyz.stride(1)
+        ld = xyz.stride(1) if not empty else max(xyz.shape[2], 1)
         natom = xyz.shape[2] if natom is None else natom
         stride = xyz.stride(0) if F > 1 else 3 * ld
     else:
         x = xyz.reshape(F, -1) if xyz.dim() == 3 else xyz
-        if x.stride(-1) != 1:
+        if x.stride(-1) != 1 and x.numel() > 0:
             raise RmsdError("AoS frames need unit coordinate stride")
         ld = 0
         natom = x.shape[1] // 3
@@ -295,3 +311,74 @@ def rmsd_fit(frames, ref_frames=None, ref_index=0, mask=None, mass=None, ref_mas
     tv = torch.empty((F, 6), dtype=torch.float64, device=dev) if save_vectors else None
     ctx.fit_desc(d, r, ref_index, fit, use_mass, MODIFY[modify], rmsd, U, tv)
     return rmsd, U, tv
+
+
+class Batch:
+    """An rmsd_batch plan: packed upper triangles of many frame sets from one
+    batched launch (rmsd_batch_create / rmsd_batch_run)."""
+
+    def __init__(self, ctx: Context, sets, outs, mode="fit", use_mass=False, precision="bf16x3",
+                 graph=True, autotune=False):
+        if len(sets) != len(outs):
+            raise RmsdError("one output per set")
+        self.ctx = ctx
+        self._L = ctx._L
+        n = len(sets)
+        self._sets = (Frames * max(n, 1))(*sets)
+        self._outs_t = list(outs)
+        self._outs = (C.c_void_p * max(n, 1))(*[o.data_ptr() for o in outs])
+        flags = (BATCH_GRAPH if graph else 0) | (BATCH_AUTOTUNE if autotune else 0)
+        h = C.c_void_p()
+        ctx.check(self._L.rmsd_batch_create(ctx.handle, self._sets, n, MODES[mode], int(use_mass),
+                                            PRECISIONS[precision], self._outs, flags, C.byref(h)))
+        self._h = h
+
+    def run(self):
+        self.ctx.check(self._L.rmsd_batch_run(self._h))
+
+    def set_variant(self, v: int):
+        self.ctx.check(self._L.rmsd_batch_set_variant(self._h, v))
+
+    def info(self) -> dict:
+        i = (C.c_int64 * 6)()
+        t = (C.c_double * 4)()
+        self.ctx.check(self._L.rmsd_batch_info(self._h, i, t))
+        return {"variant": i[0], "tile": i[1], "tiles": i[2], "graph": bool(i[3]), "queues": i[4],
+                "longest_queue": i[5], "tune_ms": list(t)}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.rmsd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rms2d_batch(frame_sets, masks=None, masses=None, mode="fit", use_mass=False,
+                precision="bf16x3", graph=False, autotune=False, ctx: Context | None = None):
+    """Packed upper-triangle rms2d matrices of several trajectories in one batched
+    pass.  frame_sets: list of CUDA tensors (AoS or (F, 3, ld) SoA with unit atom
+    stride detected by shape); returns the list of float32 triangles."""
+    import torch
+
+    ctx = ctx or _ctx(frame_sets[0].device.index or 0) if frame_sets else None
+    n = len(frame_sets)
+    masks = masks or [None] * n
+    masses = masses or [None] * n
+    descs, outs = [], []
+    for X, m, w in zip(frame_sets, masks, masses):
+        soa = X.dim() == 3 and X.shape[1] == 3 and X.shape[2] != 3
+        d = frames_desc(X, m, w, soa=soa)
+        descs.append(d)
+        F = d.nframes
+        outs.append(torch.empty(max(F * (F - 1) // 2, 1), dtype=torch.float32, device=X.device))
+    if n == 0:
+        return []
+    b = Batch(ctx, descs, outs, mode, use_mass, precision, graph=graph, autotune=autotune)
+    b.run()
+    b.close()
+    return [o[: d.nframes * (d.nframes - 1) // 2] for o, d in zip(outs, descs)]

@@ -2,8 +2,12 @@
 
     python -m cpptraj_amd.build [--verbose]
 
-Output: cpptraj_amd/_lib/libcpptraj_rmsd.so.  Every HIP source is compiled for
-gfx950 only; packed-FP32 instruction selection is disabled for all device code.
+Outputs in cpptraj_amd/_lib/:
+  libcpptraj_rmsd.so       the kernels and the C ABI (include/cpptraj_rmsd.h);
+                           every HIP source is compiled for gfx950 only, with
+                           packed-FP32 instruction selection disabled;
+  libcpptraj_rmsd_host.so  the C++ host layer (host/*.cpp: rms2d / rms /
+                           cluster-cache classes) linked against the former.
 """
 from __future__ import annotations
 
@@ -18,10 +22,14 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libcpptraj_rmsd.so")
+HOSTLIB = os.path.join(LIBDIR, "libcpptraj_rmsd_host.so")
+HOSTDIR = os.path.join(HERE, "host")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["capi.hip", "k_prep.hip", "k_pairwise.hip", "k_nofit.hip", "k_fit.hip",
-           "k_grad.hip", "k_batch.hip"]
+           "k_grad.hip", "k_matrix.hip"]
+HOST_SOURCES = ["host_core.cpp", "host_rms.cpp", "host_capi.cpp"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
@@ -47,7 +55,7 @@ def _compile(src: str, verbose: bool) -> str:
 
 def build(verbose: bool = False) -> str:
     os.makedirs(os.path.join(LIBDIR, "obj"), exist_ok=True)
-    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
     tmp = LIB + ".tmp"
@@ -56,7 +64,33 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
+
+
+def build_host(verbose: bool = False) -> str:
+    """The host layer: plain C++ (g++), HIP runtime API only, linked to LIB."""
+    objs = []
+    for s in HOST_SOURCES:
+        obj = os.path.join(LIBDIR, "obj", s + ".o")
+        cmd = ["g++", "-O2", "-fPIC", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__",
+               "-I", INCLUDE, "-I", HOSTDIR, "-I", os.path.join(ROCM, "include"), "-c",
+               os.path.join(HOSTDIR, s), "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {s}:\n" + r.stdout + r.stderr)
+        if verbose and (r.stdout or r.stderr):
+            print(r.stdout + r.stderr)
+        objs.append(obj)
+    tmp = HOSTLIB + ".tmp"
+    cmd = ["g++", "-shared", "-fPIC", "-o", tmp, *objs, "-L", LIBDIR, "-lcpptraj_rmsd",
+           "-L", os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN",
+           "-Wl,-rpath," + os.path.join(ROCM, "lib")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("host link failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, HOSTLIB)
+    return HOSTLIB
 
 
 if __name__ == "__main__":

@@ -189,99 +189,82 @@ __global__ __launch_bounds__(FIT_THREADS) void k_fit_moments_any(FitArgs a, int 
     for (int i = 0; i < 13; ++i) a.moments[13 * f + i] = v[i];
 }
 
-// Persistent path: each thread owns up to APT atoms (k = tid + 256 i, or with
-// VEC groups of four consecutive atoms) and keeps their reference coordinates,
-// weights and storage offsets in registers for every frame it visits, so HBM
-// carries the frames only.  A block walks frames f = blockIdx.x, + gridDim.x,
-// ... and prefetches the next frame's coordinates before reducing the current
-// one.  It also stores the per-frame shift (first selected atom).
-template <int NT, int APT, bool VEC, bool WEIGHTED>
-__global__ __launch_bounds__(NT, 512 / NT) void k_fit_moments_persist(FitArgs a, int ldr,
+// Wave-per-frame path: the centred reference (and the weights) sit in LDS for
+// the whole launch, each wave reduces one frame at a time and walks frames
+// f = wave id, + total waves, ...  A frame's 16 sums are combined with the
+// butterfly of wave_reduce16, so there is no block barrier per frame; HBM and
+// the vector memory pipe carry the frames only (12 B per atom), LDS serves the
+// reference (24 or 32 B per atom).  VEC: SoA rows, no selection, 16-byte
+// aligned rows, four atoms per lane per step.  Also stores the per-frame shift.
+constexpr int WAVE_FIT_THREADS = 1024;
+
+template <bool VEC, bool WEIGHTED>
+__global__ __launch_bounds__(WAVE_FIT_THREADS) void k_fit_moments_wave(FitArgs a, int ldr,
                                                                        double* shift) {
-  __shared__ double red[13 * (NT / 64)];
-  const int tid = threadIdx.x;
+  extern __shared__ double sref[];  // [3][ldr] centred reference, then [ldr] weights
   const int n = a.fs.nsel;
+  for (int i = threadIdx.x; i < 3 * ldr; i += blockDim.x) sref[i] = a.ref[i];
+  if (WEIGHTED)
+    for (int i = threadIdx.x; i < ldr; i += blockDim.x) sref[3 * ldr + i] = i < n ? a.mass[i] : 0.0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
   const int64_t ld = a.fs.ld;
-  int off[APT][3];
-  bool act[APT];
-  double yr[APT][3], wt[APT];
-#pragma unroll
-  for (int i = 0; i < APT; ++i) {
-    const int k = VEC ? 4 * (tid + NT * (i / 4)) + (i % 4) : tid + NT * i;
-    act[i] = k < n;
-    const int atom = act[i] ? (a.fs.sel ? a.fs.sel[k] : k) : 0;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      off[i][c] = (int)(ld ? c * ld + atom : 3 * atom + c);
-      yr[i][c] = act[i] ? a.ref[c * ldr + k] : 0.0;
-    }
-    wt[i] = act[i] ? (WEIGHTED ? a.mass[k] : 1.0) : 0.0;
-  }
-  const int a0 = a.fs.sel ? a.fs.sel[0] : 0;
-  int off0[3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) off0[c] = (int)(ld ? c * ld + a0 : 3 * a0 + c);
-
-  float xv[APT][3], s0[3];
-  auto fetch = [&](int64_t f) {
+  const double* Y0 = sref;
+  const double* Y1 = sref + ldr;
+  const double* Y2 = sref + 2 * ldr;
+  const double* Wt = sref + 3 * ldr;
+  for (int64_t f = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+       f < a.fs.nframes; f += nw) {
     const float* fr = a.fs.xyz + frame_of(a.fs, f) * a.fs.stride;
+    const double s0 = fcoord(a.fs, f, 0, 0), s1 = fcoord(a.fs, f, 0, 1), s2 = fcoord(a.fs, f, 0, 2);
+    double v[16];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) s0[c] = fr[off0[c]];
+    for (int i = 0; i < 16; ++i) v[i] = 0.0;
+    double (&m)[13] = *reinterpret_cast<double(*)[13]>(v);
+    int k0 = 0;
     if (VEC) {
-#pragma unroll
-      for (int g = 0; g < APT / 4; ++g) {
-        if (!act[4 * g]) {
-#pragma unroll
-          for (int e = 0; e < 4; ++e) xv[4 * g + e][0] = xv[4 * g + e][1] = xv[4 * g + e][2] = 0.f;
-          continue;
-        }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(fr + off[4 * g][c]));
-          xv[4 * g][c] = v.x;
-          xv[4 * g + 1][c] = v.y;
-          xv[4 * g + 2][c] = v.z;
-          xv[4 * g + 3][c] = v.w;
-        }
+      const int n4 = n >> 2;
+      const f32x4* X = reinterpret_cast<const f32x4*>(fr);
+      const f32x4* Y = reinterpret_cast<const f32x4*>(fr + ld);
+      const f32x4* Z = reinterpret_cast<const f32x4*>(fr + 2 * ld);
+#pragma unroll 4
+      for (int q = lane; q < n4; q += 64) {
+        const f32x4 x = __builtin_nontemporal_load(X + q);
+        const f32x4 y = __builtin_nontemporal_load(Y + q);
+        const f32x4 z = __builtin_nontemporal_load(Z + q);
+        const int k = 4 * q;
+        accum<WEIGHTED>(m, x.x - s0, y.x - s1, z.x - s2, WEIGHTED ? Wt[k] : 1.0, Y0[k], Y1[k], Y2[k]);
+        accum<WEIGHTED>(m, x.y - s0, y.y - s1, z.y - s2, WEIGHTED ? Wt[k + 1] : 1.0, Y0[k + 1],
+                        Y1[k + 1], Y2[k + 1]);
+        accum<WEIGHTED>(m, x.z - s0, y.z - s1, z.z - s2, WEIGHTED ? Wt[k + 2] : 1.0, Y0[k + 2],
+                        Y1[k + 2], Y2[k + 2]);
+        accum<WEIGHTED>(m, x.w - s0, y.w - s1, z.w - s2, WEIGHTED ? Wt[k + 3] : 1.0, Y0[k + 3],
+                        Y1[k + 3], Y2[k + 3]);
       }
-    } else {
-#pragma unroll
-      for (int i = 0; i < APT; ++i)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) xv[i][c] = act[i] ? __builtin_nontemporal_load(fr + off[i][c]) : 0.f;
+      k0 = 4 * n4;
     }
-  };
-
-  int64_t f = blockIdx.x;
-  if (f >= a.fs.nframes) return;
-  fetch(f);
-  for (; f < a.fs.nframes; f += gridDim.x) {
-    float cur[APT][3], sc[3];
-#pragma unroll
-    for (int i = 0; i < APT; ++i)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) cur[i][c] = xv[i][c];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) sc[c] = s0[c];
-    const int64_t fn = f + gridDim.x;
-    if (fn < a.fs.nframes) fetch(fn);
-    double v[13];
-#pragma unroll
-    for (int i = 0; i < 13; ++i) v[i] = 0.0;
-#pragma unroll
-    for (int i = 0; i < APT; ++i) {
-      // inactive atoms carry weight 0 and zero coordinates
-      const double x0 = act[i] ? (double)cur[i][0] - sc[0] : 0.0;
-      const double x1 = act[i] ? (double)cur[i][1] - sc[1] : 0.0;
-      const double x2 = act[i] ? (double)cur[i][2] - sc[2] : 0.0;
-      accum<true>(v, x0, x1, x2, wt[i], yr[i][0], yr[i][1], yr[i][2]);
+#pragma unroll 4
+    for (int k = k0 + lane; k < n; k += 64) {
+      accum<WEIGHTED>(m, (double)fcoord(a.fs, f, k, 0) - s0, (double)fcoord(a.fs, f, k, 1) - s1,
+                      (double)fcoord(a.fs, f, k, 2) - s2, WEIGHTED ? Wt[k] : 1.0, Y0[k], Y1[k],
+                      Y2[k]);
     }
-    rmsd_dev::block_sum<13>(v, red);
-    if (tid == 0) {
-      for (int i = 0; i < 13; ++i) a.moments[13 * f + i] = v[i];
-      for (int c = 0; c < 3; ++c) shift[3 * f + c] = sc[c];
+    const double t = rmsd_dev::wave_reduce16(v);
+    const int idx = lane >> 2;
+    if ((lane & 3) == 0 && idx < 13) a.moments[13 * f + idx] = t;
+    if (lane == 0) {
+      shift[3 * f] = s0;
+      shift[3 * f + 1] = s1;
+      shift[3 * f + 2] = s2;
     }
   }
+}
+
+// LDS bytes of the wave-per-frame path, 0 when the reference does not fit.
+static inline size_t wave_fit_lds(int ldr, bool weighted) {
+  const size_t b = sizeof(double) * (size_t)ldr * (weighted ? 4 : 3);
+  return b <= 160 * 1024 ? b : 0;
 }
 
 // Per-frame shift used by the moment kernels (first selected atom).
@@ -412,31 +395,41 @@ hipError_t rmsd_launch_fit_moments(const FitArgs& a, hipStream_t s) {
   const int ldr = (a.fs.nsel + 3) & ~3;
   const int nf = a.fs.nframes;
   double* shift = a.moments + 13 * (int64_t)nf;
+  if (nf == 0) return hipSuccess;
   const bool vec = a.fs.sel == nullptr && a.fs.ld > 0 && (a.fs.ld % 4) == 0 &&
                    (a.fs.stride % 4) == 0 && ((uintptr_t)a.fs.xyz % 16) == 0;
   const bool w = a.mass != nullptr;
-  // Persistent register-resident path: 256 or 512 threads with up to 12
-  // (vector path) or 8 (scalar path) atoms per thread, i.e. up to 6144 / 4096
-  // selected atoms; larger selections take the one-block-per-frame kernels.
-  const int grid = nf < 1024 ? nf : 1024;
-  const int n = a.fs.nsel;
-#define RMSD_FIT_PERSIST(NT, N, V)                                                          \
-  if (vec == V && n <= NT * N) {                                                           \
-    if (w) hipLaunchKernelGGL((k_fit_moments_persist<NT, N, V, true>), dim3(grid), dim3(NT), \
-                              0, s, a, ldr, shift);                                        \
-    else hipLaunchKernelGGL((k_fit_moments_persist<NT, N, V, false>), dim3(grid), dim3(NT), \
-                            0, s, a, ldr, shift);                                          \
-    return hipGetLastError();                                                              \
+  // Wave-per-frame path whenever the reference fits in LDS (up to 5120
+  // weighted / 6826 unweighted atoms); one-block-per-frame kernels otherwise.
+  const size_t lds = wave_fit_lds(ldr, w);
+  if (lds && a.variant != 1) {
+    int dev = 0, ncu = 256;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const int64_t waves = WAVE_FIT_THREADS / 64;
+    const int64_t want = (nf + waves - 1) / waves;
+    // Blocks loop over frames, so the grid must not exceed what is resident.
+#define RMSD_FIT_WAVE(V, W)                                                                   \
+  {                                                                                          \
+    const void* fn = reinterpret_cast<const void*>(&k_fit_moments_wave<V, W>);               \
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+    if (e != hipSuccess) return e;                                                           \
+    int bpc = 1;                                                                             \
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, fn, WAVE_FIT_THREADS, lds);       \
+    if (e != hipSuccess) return e;                                                           \
+    if (bpc < 1) bpc = 1;                                                                    \
+    const int grid = (int)(want < (int64_t)ncu * bpc ? want : (int64_t)ncu * bpc);           \
+    hipLaunchKernelGGL((k_fit_moments_wave<V, W>), dim3(grid), dim3(WAVE_FIT_THREADS), lds, s, a, \
+                       ldr, shift);                                                          \
   }
-  RMSD_FIT_PERSIST(256, 4, true)
-  RMSD_FIT_PERSIST(256, 8, true)
-  RMSD_FIT_PERSIST(256, 12, true)
-  RMSD_FIT_PERSIST(512, 8, true)
-  RMSD_FIT_PERSIST(512, 12, true)
-  RMSD_FIT_PERSIST(256, 4, false)
-  RMSD_FIT_PERSIST(256, 8, false)
-  RMSD_FIT_PERSIST(512, 8, false)
-#undef RMSD_FIT_PERSIST
+    if (vec) {
+      if (w) RMSD_FIT_WAVE(true, true) else RMSD_FIT_WAVE(true, false)
+    } else {
+      if (w) RMSD_FIT_WAVE(false, true) else RMSD_FIT_WAVE(false, false)
+    }
+#undef RMSD_FIT_WAVE
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_fit_shift, dim3((nf + 255) / 256), dim3(256), 0, s, a.fs, shift);
   if (vec) {
     if (w) hipLaunchKernelGGL(k_fit_moments_soa4<true>, dim3(nf), dim3(FIT_THREADS), 0, s, a, ldr);

@@ -188,22 +188,15 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
   }
 }
 
+// One 64 x 64 tile of frame pairs: contraction over all atom chunks, then
+// the FP64 epilogue.  Shared by the single-matrix and the batched kernels.
 template <int PREC, int WR, int WC, int NWR, int NWC>
-__global__ __launch_bounds__(64 * NWR * NWC) void k_pairwise(PairArgs args) {
+__device__ __forceinline__ void pair_tile(const PairArgs& args, int2 tile) {
   typedef PrecTraits<PREC> T;
   typedef typename T::acc_t acc_t;
   constexpr int PARTS = T::PARTS;
   constexpr int TILE_R = 16 * WR * NWR;
   constexpr int TILE_C = 16 * WC * NWC;
-
-  // XCD-aware tile pick: the hardware deals workgroups round-robin over the
-  // XCDs, so XCD x receives ids x, x+NXCD, ...; give each XCD one contiguous
-  // run of the (grouped) tile list so that concurrently running workgroups on
-  // one XCD share row and column blocks in its L2.
-  const int w = blockIdx.x;
-  const int t = (w % args.nxcd) * args.per_xcd + (w / args.nxcd);
-  if (t >= args.ntiles) return;
-  const int2 tile = args.tiles[t];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int wr = wave / NWC, wc = wave % NWC;
@@ -269,6 +262,62 @@ __global__ __launch_bounds__(64 * NWR * NWC) void k_pairwise(PairArgs args) {
   }
 
   pair_epilogue<PREC, WR, WC>(args, acc, fbA, fbB, lane, args.gA, args.uA, 0, args.gB, args.uB, 0);
+}
+
+template <int PREC, int WR, int WC, int NWR, int NWC>
+__global__ __launch_bounds__(64 * NWR * NWC) void k_pairwise(PairArgs args) {
+  // XCD-aware tile pick: the hardware deals workgroups round-robin over the
+  // XCDs, so XCD x receives ids x, x+NXCD, ...; give each XCD one contiguous
+  // run of the (grouped) tile list so that concurrently running workgroups on
+  // one XCD share row and column blocks in its L2.
+  const int w = blockIdx.x;
+  const int t = (w % args.nxcd) * args.per_xcd + (w / args.nxcd);
+  if (t >= args.ntiles) return;
+  pair_tile<PREC, WR, WC, NWR, NWC>(args, args.tiles[t]);
+}
+
+// ---- batched matrices ---------------------------------------------------------------
+// Tiles of many independent matrices (one PairArgs per set) in nq queues, one
+// per XCD, each a contiguous, cost-balanced run of the cost-sorted tile list.
+// Grid form: workgroup w takes position w / nq of queue w % nq (the hardware
+// hands out workgroups in order, so each queue drains most expensive first).
+template <int PREC, int WR, int WC, int NWR, int NWC>
+__global__ __launch_bounds__(64 * NWR * NWC) void k_pairwise_batch(BatchArgs b) {
+  const int q = blockIdx.x % b.nq;
+  const int t = b.qbeg[q] + blockIdx.x / b.nq;
+  if (t >= b.qbeg[q + 1]) return;
+  const int4 tl = b.tiles[t];
+  const PairArgs args = b.sets[tl.x];
+  pair_tile<PREC, WR, WC, NWR, NWC>(args, make_int2(tl.y, tl.z));
+}
+
+// Persistent form: as many workgroups as are resident; each drains its own
+// queue through an atomic position counter, then steals from the others.
+// Every workgroup leaves once every queue is exhausted.  The counters must be
+// zero at launch (rmsd_launch_batch_reset).
+template <int PREC, int WR, int WC, int NWR, int NWC>
+__global__ __launch_bounds__(64 * NWR * NWC) void k_pairwise_batch_persist(BatchArgs b) {
+  __shared__ int slot;
+  const int q0 = blockIdx.x % b.nq;
+  for (int k = 0; k < b.nq; ++k) {
+    const int q = (q0 + k) % b.nq;
+    const int beg = b.qbeg[q], len = b.qbeg[q + 1] - beg;
+    for (;;) {
+      __syncthreads();
+      if (threadIdx.x == 0) slot = atomicAdd(b.counters + q, 1);
+      __syncthreads();
+      const int pos = __builtin_amdgcn_readfirstlane(slot);
+      if (pos >= len) break;
+      const int4 tl = b.tiles[beg + pos];
+      const PairArgs args = b.sets[tl.x];
+      pair_tile<PREC, WR, WC, NWR, NWC>(args, make_int2(tl.y, tl.z));
+    }
+  }
+}
+
+__global__ void k_fill_int(int* p, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 // ---- LDS-staged contraction for the MFMA-reduced forms ----------------------------
@@ -536,8 +585,79 @@ static hipError_t launch_lds(const PairArgs& a, hipStream_t s) {
 hipError_t rmsd_launch_pairwise(int prec, const PairArgs& a, hipStream_t s) {
   switch (prec) {
     case PREC_F64: return launch_tiles<PREC_F64, 1, 2, 4, 2>(a, s);
-    case PREC_F32: return launch_lds<PREC_F32>(a, s);
-    case PREC_BF16X3: return launch_lds<PREC_BF16X3>(a, s);
+    case PREC_F32: return launch_tiles<PREC_F32, 2, 2, 2, 2>(a, s);
+    case PREC_BF16X3: return launch_tiles<PREC_BF16X3, 2, 2, 2, 2>(a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// Variants of the alternative single-matrix kernels, for timing studies and
+// the tuner: 0 register tiles (default), 1 register pipeline (three chunks in
+// flight), 2 LDS-staged operands.
+hipError_t rmsd_launch_pairwise_variant(int prec, int variant, const PairArgs& a, hipStream_t s) {
+  if (variant == 0 || prec == PREC_F64) return rmsd_launch_pairwise(prec, a, s);
+  if (variant == 1)
+    return prec == PREC_F32 ? launch_rs<PREC_F32, 3>(a, s) : launch_rs<PREC_BF16X3, 3>(a, s);
+  if (variant == 2)
+    return prec == PREC_F32 ? launch_lds<PREC_F32>(a, s) : launch_lds<PREC_BF16X3>(a, s);
+  return hipErrorInvalidValue;
+}
+
+// Batched launches.  Variant v: tile shape v / 2 (shape 0: 64 x 64 pair tiles,
+// shape 1: 32 x 32 pair tiles, with the same per-wave tile: FP64 waves own
+// 16 x 32 pairs, FP32 / BF16 waves 32 x 32), grid form for even v, persistent
+// form for odd v.  The tile list must be built for the shape's tile side.
+template <int PREC, int WR, int WC, int NWR, int NWC>
+static hipError_t launch_batch_shape(const BatchArgs& b, bool persist, hipStream_t s) {
+  constexpr int NT = 64 * NWR * NWC;
+  if (!persist) {
+    const int64_t grid = (int64_t)b.nq * b.maxq;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_pairwise_batch<PREC, WR, WC, NWR, NWC>), dim3((unsigned)grid), dim3(NT),
+                       0, s, b);
+    return hipGetLastError();
+  }
+  int dev = 0, ncu = 256, bpc = 1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+    return e;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &bpc, reinterpret_cast<const void*>(&k_pairwise_batch_persist<PREC, WR, WC, NWR, NWC>), NT, 0);
+  if (e != hipSuccess) return e;
+  if (bpc < 1) bpc = 1;
+  int64_t grid = (int64_t)ncu * bpc;
+  const int64_t ntiles = (int64_t)b.nq * b.maxq;
+  if (grid > ntiles) grid = ntiles;
+  if (grid < b.nq) grid = b.nq;
+  hipLaunchKernelGGL((k_pairwise_batch_persist<PREC, WR, WC, NWR, NWC>), dim3((unsigned)grid),
+                     dim3(NT), 0, s, b);
+  return hipGetLastError();
+}
+
+int rmsd_batch_variant_count() { return 4; }
+
+int rmsd_batch_tile_side(int variant) { return (variant >> 1) == 0 ? 64 : 32; }
+
+hipError_t rmsd_launch_batch_reset(const BatchArgs& b, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, b.counters, b.nq, 0);
+  return hipGetLastError();
+}
+
+hipError_t rmsd_launch_pairwise_batch(int prec, int variant, const BatchArgs& b, hipStream_t s) {
+  if (variant < 0 || variant >= rmsd_batch_variant_count()) return hipErrorInvalidValue;
+  const bool persist = variant & 1;
+  const int shape = variant >> 1;
+  switch (prec) {
+    case PREC_F64:
+      return shape == 0 ? launch_batch_shape<PREC_F64, 1, 2, 4, 2>(b, persist, s)
+                        : launch_batch_shape<PREC_F64, 1, 2, 2, 1>(b, persist, s);
+    case PREC_F32:
+      return shape == 0 ? launch_batch_shape<PREC_F32, 2, 2, 2, 2>(b, persist, s)
+                        : launch_batch_shape<PREC_F32, 2, 2, 1, 1>(b, persist, s);
+    case PREC_BF16X3:
+      return shape == 0 ? launch_batch_shape<PREC_BF16X3, 2, 2, 2, 2>(b, persist, s)
+                        : launch_batch_shape<PREC_BF16X3, 2, 2, 1, 1>(b, persist, s);
     default: return hipErrorInvalidValue;
   }
 }

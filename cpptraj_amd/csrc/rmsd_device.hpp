@@ -197,6 +197,38 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Butterfly reduce-scatter of 16 doubles per lane over the wave: each step
+// sends the half of the remaining values that the partner keeps, so the wave
+// moves 8+4+2+1+1+1 values per lane instead of 16 x 6.  On return lane l holds
+// the wave total of value (l >> 2); the four lanes 4m..4m+3 hold the same one.
+__device__ __forceinline__ double wave_reduce16(const double (&v)[16]) {
+  const int lane = threadIdx.x & 63;
+  double a8[8], a4[4], a2[2];
+  const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const double keep = b5 ? v[i + 8] : v[i];
+    const double send = b5 ? v[i] : v[i + 8];
+    a8[i] = keep + __shfl_xor(send, 32, 64);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double keep = b4 ? a8[i + 4] : a8[i];
+    const double send = b4 ? a8[i] : a8[i + 4];
+    a4[i] = keep + __shfl_xor(send, 16, 64);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double keep = b3 ? a4[i + 2] : a4[i];
+    const double send = b3 ? a4[i] : a4[i + 2];
+    a2[i] = keep + __shfl_xor(send, 8, 64);
+  }
+  double r = (b2 ? a2[1] : a2[0]) + __shfl_xor(b2 ? a2[0] : a2[1], 4, 64);
+  r += __shfl_xor(r, 2, 64);
+  r += __shfl_xor(r, 1, 64);
+  return r;
+}
+
 // Sums NV doubles per thread over the whole block.  Every thread receives the
 // totals.  red must hold NV * (blockDim.x / 64) doubles of shared memory.
 template <int NV>

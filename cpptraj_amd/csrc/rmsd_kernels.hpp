@@ -80,6 +80,16 @@ struct PairArgs {
   float* out;
 };
 
+// Tiles of many matrices (k_pairwise.hip, batched kernels).
+struct BatchArgs {
+  const PairArgs* sets;  // one per matrix (device array)
+  const int4* tiles;     // (set, row tile, column tile, 0), queue after queue
+  const int* qbeg;       // nq + 1 queue offsets into tiles
+  int nq;                // queues (one per XCD)
+  int maxq;              // longest queue
+  int* counters;         // nq position counters (persistent form)
+};
+
 // Streaming fit of every frame to one reference (k_fit.hip).
 struct FitArgs {
   FrameSetDev fs;
@@ -87,6 +97,7 @@ struct FitArgs {
   const double* ref;     // centred reference, SoA [3][nsel] FP64
   const double* consts;  // [0..2] sum w y, [3] sum w |y|^2, [4] total weight
   double* moments;       // out: 13 per frame, then 3 per frame of shift
+  int variant = 0;       // 0: fastest fitting kernel, 1: one block per frame
 };
 
 struct SolveArgs {
@@ -120,6 +131,13 @@ struct ApplyArgs {
 hipError_t rmsd_launch_setup(const SetupArgs& a, hipStream_t s);
 hipError_t rmsd_launch_prep(const PrepArgs& a, hipStream_t s);
 hipError_t rmsd_launch_pairwise(int prec, const PairArgs& a, hipStream_t s);
+int rmsd_autocorr_chunks(int n, int* rows);
+hipError_t rmsd_launch_autocorr(const float* tri, int n, double* part, double* ct, hipStream_t s);
+hipError_t rmsd_launch_pairwise_variant(int prec, int variant, const PairArgs& a, hipStream_t s);
+int rmsd_batch_variant_count();
+int rmsd_batch_tile_side(int variant);
+hipError_t rmsd_launch_batch_reset(const BatchArgs& b, hipStream_t s);
+hipError_t rmsd_launch_pairwise_batch(int prec, int variant, const BatchArgs& b, hipStream_t s);
 hipError_t rmsd_launch_pair_nofit_tiles(const FrameSetDev& A, const FrameSetDev& B,
                                         const double* mass, const double* total_mass,
                                         const int2* tiles, int ntiles, int per_xcd, int nxcd,

@@ -1,0 +1,134 @@
+// host_capi.cpp -- C entry points over the host layer's Session.
+#include <string.h>
+
+#include <string>
+
+#include "host.hpp"
+#include "host_capi.h"
+
+using namespace cpptraj_amd;
+
+struct chost_session {
+  explicit chost_session(int device) : s(device) {}
+  Session s;
+  std::string err;
+};
+
+extern "C" {
+
+int chost_create(int device, chost_session** out) {
+  if (!out) return 1;
+  *out = new chost_session(device);
+  if (!(*out)->s.gpu.ok()) {
+    (*out)->err = (*out)->s.gpu.error();
+    return 2;
+  }
+  return 0;
+}
+
+void chost_destroy(chost_session* s) { delete s; }
+
+const char* chost_last_error(const chost_session* s) { return s ? s->err.c_str() : "no session"; }
+
+int chost_add_coords(chost_session* s, const char* name, const float* xyz, int64_t nframes,
+                     int64_t stride, int natom, const double* mass) {
+  if (!s || !name || natom < 1 || stride < 3 * (int64_t)natom || nframes < 0 ||
+      (nframes > 0 && !xyz)) {
+    if (s) s->err = "chost_add_coords: bad arguments";
+    return 1;
+  }
+  CoordsSet* c = s->s.dsl.AddCoords(name);
+  c->top.natom = natom;
+  c->top.mass.assign(natom, 1.0);
+  if (mass) c->top.mass.assign(mass, mass + natom);
+  c->stride = stride;
+  c->xyz.assign(xyz, xyz + nframes * stride);
+  return 0;
+}
+
+int chost_add_reference(chost_session* s, const char* name, const float* xyz, int natom,
+                        const double* mass) {
+  if (!s || !name || !xyz || natom < 1) {
+    if (s) s->err = "chost_add_reference: bad arguments";
+    return 1;
+  }
+  CoordsSet* c = s->s.dsl.AddReference(name);
+  c->top.natom = natom;
+  c->top.mass.assign(natom, 1.0);
+  if (mass) c->top.mass.assign(mass, mass + natom);
+  c->stride = 3 * (int64_t)natom;
+  c->xyz.assign(xyz, xyz + c->stride);
+  return 0;
+}
+
+int chost_add_mask(chost_session* s, const char* set, const char* expr, const int* sel, int nsel) {
+  if (!s || !expr || nsel < 0 || (nsel > 0 && !sel)) return 1;
+  const std::string key = (set && *set) ? std::string(set) + "|" + expr : std::string(expr);
+  s->s.masks[key] = AtomMask(sel, sel + nsel);
+  return 0;
+}
+
+int chost_run(chost_session* s, const char* line) {
+  if (!s || !line) return 1;
+  const int rc = s->s.Run(line);
+  s->err = s->s.err;
+  return rc;
+}
+
+int chost_get_matrix(chost_session* s, const char* name, float* out, int64_t* ncols,
+                     int64_t* nrows, int* is_tri) {
+  if (!s || !name) return 1;
+  const MatrixFlt* m = s->s.dsl.FindMatrix(name);
+  if (!m) {
+    s->err = std::string("no matrix '") + name + "'";
+    return 1;
+  }
+  if (ncols) *ncols = (int64_t)m->Ncols();
+  if (nrows) *nrows = (int64_t)m->Nrows();
+  if (is_tri) *is_tri = m->kind() == MatrixFlt::TRI;
+  if (out && m->size()) memcpy(out, m->data(), sizeof(float) * m->size());
+  return 0;
+}
+
+int chost_get_series(chost_session* s, const char* name, double* out, int64_t* nframes,
+                     int* ncomp) {
+  if (!s || !name) return 1;
+  const Series* d = s->s.dsl.FindSeries(name);
+  if (!d) {
+    s->err = std::string("no data set '") + name + "'";
+    return 1;
+  }
+  if (nframes) *nframes = (int64_t)d->nframes();
+  if (ncomp) *ncomp = d->ncomp;
+  if (out && !d->v.empty()) memcpy(out, d->v.data(), sizeof(double) * d->v.size());
+  return 0;
+}
+
+int chost_get_coords(chost_session* s, const char* name, float* out, int64_t* nfloats) {
+  if (!s || !name) return 1;
+  const CoordsSet* c = s->s.dsl.FindCoordsSet(name);
+  if (!c) {
+    s->err = std::string("no COORDS set '") + name + "'";
+    return 1;
+  }
+  if (nfloats) *nfloats = (int64_t)c->xyz.size();
+  if (out && !c->xyz.empty()) memcpy(out, c->xyz.data(), sizeof(float) * c->xyz.size());
+  return 0;
+}
+
+int chost_read_cmatrix(const char* path, float* tri, int64_t* nrows, int64_t* actual_nrows,
+                       int* sieve, int* frames) {
+  MatrixFlt m;
+  size_t actual = 0;
+  int sv = 1;
+  std::vector<int> fr;
+  if (!path || cmatrix::Read(path, &m, &actual, &sv, &fr)) return 1;
+  if (nrows) *nrows = (int64_t)m.Nrows();
+  if (actual_nrows) *actual_nrows = (int64_t)actual;
+  if (sieve) *sieve = sv;
+  if (tri && m.size()) memcpy(tri, m.data(), sizeof(float) * m.size());
+  if (frames && !fr.empty()) memcpy(frames, fr.data(), sizeof(int) * fr.size());
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,49 @@
+/*
+ * host_capi.h -- C entry points of the host layer (libcpptraj_rmsd_host.so),
+ * for bindings that cannot use the C++ classes of host.hpp directly (the
+ * Python tests).  A session holds named COORDS sets, reference frames, a mask
+ * table (expression -> resolved selection) and the data sets that commands
+ * produce; chost_run executes one command line of the reference's syntax
+ * ("rms2d ...", "2drms ...", "rms ...", "rmsd ...", "cluster ...").
+ */
+#ifndef CPPTRAJ_RMSD_HOST_CAPI_H
+#define CPPTRAJ_RMSD_HOST_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct chost_session chost_session;
+
+int chost_create(int device, chost_session** s);
+void chost_destroy(chost_session* s);
+const char* chost_last_error(const chost_session* s);
+/* COORDS set: nframes frames of `stride` floats (3*natom coordinates first),
+ * natom masses (NULL: all 1). */
+int chost_add_coords(chost_session* s, const char* name, const float* xyz, int64_t nframes,
+                     int64_t stride, int natom, const double* mass);
+/* Reference frame ('ref <name>' / 'reference'): one frame of 3*natom floats. */
+int chost_add_reference(chost_session* s, const char* name, const float* xyz, int natom,
+                        const double* mass);
+/* Mask expression -> selected atom indices, for the set (COORDS or reference)
+ * named `set`, or for every set when `set` is NULL or empty. */
+int chost_add_mask(chost_session* s, const char* set, const char* expr, const int* sel, int nsel);
+int chost_run(chost_session* s, const char* line);
+/* Data set queries.  Pass NULL buffers to get the sizes first. */
+int chost_get_matrix(chost_session* s, const char* name, float* out, int64_t* ncols,
+                     int64_t* nrows, int* is_tri);
+int chost_get_series(chost_session* s, const char* name, double* out, int64_t* nframes,
+                     int* ncomp);
+int chost_get_coords(chost_session* s, const char* name, float* out, int64_t* nfloats);
+/* Cmatrix file reader: fills sizes, then (second call) the triangle and the
+ * cached frame list. */
+int chost_read_cmatrix(const char* path, float* tri, int64_t* nrows, int64_t* actual_nrows,
+                       int* sieve, int* frames);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

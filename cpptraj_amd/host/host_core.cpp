@@ -1,0 +1,411 @@
+// host_core.cpp -- ArgList, data containers, text and Cmatrix output, and the
+// device session of the host layer.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <sstream>
+
+#include "host.hpp"
+
+namespace cpptraj_amd {
+
+// ---- ArgList -----------------------------------------------------------------
+ArgList::ArgList(const std::string& line) {
+  std::istringstream is(line);
+  std::string tok;
+  while (is >> tok) args_.push_back(tok);
+  marked_.assign(args_.size(), false);
+  if (!marked_.empty()) marked_[0] = true;  // the command itself
+}
+
+const std::string& ArgList::Command() const {
+  static const std::string empty;
+  return args_.empty() ? empty : args_[0];
+}
+
+bool ArgList::hasKey(const char* key) {
+  for (size_t i = 0; i < args_.size(); ++i)
+    if (!marked_[i] && args_[i] == key) {
+      marked_[i] = true;
+      return true;
+    }
+  return false;
+}
+
+bool ArgList::Contains(const char* key) const {
+  for (size_t i = 0; i < args_.size(); ++i)
+    if (!marked_[i] && args_[i] == key) return true;
+  return false;
+}
+
+std::string ArgList::GetStringKey(const char* key) {
+  for (size_t i = 0; i + 1 < args_.size(); ++i)
+    if (!marked_[i] && args_[i] == key) {
+      marked_[i] = marked_[i + 1] = true;
+      return args_[i + 1];
+    }
+  return std::string();
+}
+
+std::string ArgList::GetStringNext() {
+  for (size_t i = 0; i < args_.size(); ++i)
+    if (!marked_[i]) {
+      marked_[i] = true;
+      return args_[i];
+    }
+  return std::string();
+}
+
+bool ArgList::StrIsMask(const std::string& s) {
+  size_t p = 0;
+  while (p < s.size() && (s[p] == '!' || s[p] == '(')) ++p;
+  if (p >= s.size()) return false;
+  const char c = s[p];
+  return c == '@' || c == ':' || c == '^' || c == '*' || c == '=';
+}
+
+std::string ArgList::GetMaskNext() {
+  for (size_t i = 0; i < args_.size(); ++i)
+    if (!marked_[i] && StrIsMask(args_[i])) {
+      marked_[i] = true;
+      return args_[i];
+    }
+  return std::string();
+}
+
+int ArgList::getKeyInt(const char* key, int def) {
+  std::string v = GetStringKey(key);
+  if (v.empty()) return def;
+  return atoi(v.c_str());
+}
+
+std::vector<std::string> ArgList::Unmarked() const {
+  std::vector<std::string> r;
+  for (size_t i = 0; i < args_.size(); ++i)
+    if (!marked_[i]) r.push_back(args_[i]);
+  return r;
+}
+
+// ---- MatrixFlt ------------------------------------------------------------------
+void MatrixFlt::AllocateTriangle(size_t n) {
+  kind_ = TRI;
+  ncols_ = nrows_ = n;
+  v_.assign(n > 1 ? n * (n - 1) / 2 : 0, 0.0f);
+}
+
+void MatrixFlt::Allocate2D(size_t ncols, size_t nrows) {
+  kind_ = FULL;
+  ncols_ = ncols;
+  nrows_ = nrows;
+  v_.assign(ncols * nrows, 0.0f);
+}
+
+double MatrixFlt::GetElement(size_t x, size_t y) const {
+  if (kind_ == FULL) return (double)v_[y * ncols_ + x];
+  if (x == y) return 0.0;
+  const size_t i = std::min(x, y), j = std::max(x, y);
+  return (double)v_[ncols_ * i - (i * (i + 1)) / 2 + j - i - 1];
+}
+
+// ---- DataSetList ------------------------------------------------------------------
+CoordsSet* DataSetList::AddCoords(const std::string& name) {
+  coords_.emplace_back(new CoordsSet());
+  coords_.back()->name = name;
+  return coords_.back().get();
+}
+
+CoordsSet* DataSetList::FindCoordsSet(const std::string& name) const {
+  if (name.empty()) return coords_.empty() ? nullptr : coords_.front().get();
+  for (const auto& c : coords_)
+    if (c->name == name) return c.get();
+  return nullptr;
+}
+
+CoordsSet* DataSetList::AddReference(const std::string& name) {
+  refs_.emplace_back(new CoordsSet());
+  refs_.back()->name = name;
+  return refs_.back().get();
+}
+
+CoordsSet* DataSetList::FindReference(const std::string& name) const {
+  if (name.empty()) return refs_.empty() ? nullptr : refs_.front().get();
+  for (const auto& c : refs_)
+    if (c->name == name) return c.get();
+  return nullptr;
+}
+
+MatrixFlt* DataSetList::AddMatrix(const std::string& name) {
+  mats_.emplace_back(new MatrixFlt());
+  mats_.back()->name = name;
+  return mats_.back().get();
+}
+
+Series* DataSetList::AddSeries(const std::string& name) {
+  series_.emplace_back(new Series());
+  series_.back()->name = name;
+  return series_.back().get();
+}
+
+const MatrixFlt* DataSetList::FindMatrix(const std::string& name) const {
+  for (auto it = mats_.rbegin(); it != mats_.rend(); ++it)
+    if ((*it)->name == name) return it->get();
+  return nullptr;
+}
+
+const Series* DataSetList::FindSeries(const std::string& name) const {
+  for (auto it = series_.rbegin(); it != series_.rend(); ++it)
+    if ((*it)->name == name) return it->get();
+  return nullptr;
+}
+
+std::string DataSetList::GenerateDefaultName(const std::string& base) {
+  char buf[64];
+  snprintf(buf, sizeof(buf), "%s_%05d", base.c_str(), counters_[base]++);
+  return buf;
+}
+
+// ---- text output ------------------------------------------------------------------
+namespace dataio {
+
+// Width of a frame-number column (TextFormat::SetCoordFormat,
+// src/TextFormat.cpp:52-82): the digits of the largest coordinate
+// (step * n + min), widened for the precision, at least the default width.
+static int coord_width(size_t n, int def_width, int prec) {
+  long maxc = (long)n + 1;
+  int w = 1;
+  while (maxc >= 10) {
+    maxc /= 10;
+    ++w;
+  }
+  if (prec != 0 && w + prec + 1 > def_width) w = w + prec + 1;
+  return std::max(w, def_width);
+}
+
+int WriteMatrix(const std::string& path, const MatrixFlt& m, bool square2d) {
+  FILE* f = fopen(path.c_str(), "w");
+  if (!f) return 1;
+  const size_t nr = m.Nrows(), nc = m.Ncols();
+  if (square2d) {
+    // "#Frame" left-aligned in 8 columns, then the column numbers in the
+    // value columns' width (9 = leading space + %8.3f); rows: frame number
+    // %8.0f, then " %8.3f" per value.
+    const int yw = coord_width(nr, 8, 0);
+    const int xw = coord_width(nc, 9, 0);
+    fprintf(f, "%-*s", 8, "#Frame");
+    for (size_t x = 0; x < nc; ++x) fprintf(f, "%*.0f", xw, (double)(x + 1));
+    fprintf(f, "\n");
+    for (size_t y = 0; y < nr; ++y) {
+      fprintf(f, "%*.0f", yw, (double)(y + 1));
+      for (size_t x = 0; x < nc; ++x) fprintf(f, " %8.3f", m.GetElement(x, y));
+      fprintf(f, "\n");
+    }
+  } else {
+    // "#<xlabel> <ylabel> <legend>" (empty labels), then "x y value" lines
+    // with the coordinates as %8.3f.
+    fprintf(f, "#  %s\n", m.name.c_str());
+    const int xw = coord_width(nc, 8, 3), yw = coord_width(nr, 8, 3);
+    for (size_t y = 0; y < nr; ++y)
+      for (size_t x = 0; x < nc; ++x)
+        fprintf(f, "%*.3f %*.3f  %8.3f\n", xw, (double)(x + 1), yw, (double)(y + 1),
+                m.GetElement(x, y));
+  }
+  return fclose(f) != 0;
+}
+
+int WriteSeries(const std::string& path, const std::vector<const Series*>& sets) {
+  if (sets.empty()) return 0;
+  FILE* f = fopen(path.c_str(), "w");
+  if (!f) return 1;
+  size_t n = 0;
+  for (const Series* s : sets) n = std::max(n, s->nframes());
+  const int fw = coord_width(n, 8, 0);
+  // Header: "#Frame", then each set's legend right-aligned over its columns
+  // (ncomp x (1 + width)); spaces in names become underscores.
+  fprintf(f, "%-*s", fw, "#Frame");
+  for (const Series* s : sets) {
+    std::string nm = s->name;
+    std::replace(nm.begin(), nm.end(), ' ', '_');
+    const int cw = s->ncomp * (s->width + 1);
+    fprintf(f, "%*s", cw, nm.c_str());
+  }
+  fprintf(f, "\n");
+  for (size_t i = 0; i < n; ++i) {
+    fprintf(f, "%*.0f", fw, (double)(i + 1));
+    for (const Series* s : sets)
+      for (int c = 0; c < s->ncomp; ++c) {
+        const double v = i < s->nframes() ? s->v[i * s->ncomp + c] : 0.0;
+        fprintf(f, " %*.*f", s->width, s->prec, v);
+      }
+    fprintf(f, "\n");
+  }
+  return fclose(f) != 0;
+}
+
+}  // namespace dataio
+
+// ---- Cmatrix binary -----------------------------------------------------------------
+namespace cmatrix {
+
+static const unsigned char kMagic[4] = {'C', 'T', 'M', 2};
+
+int Write(const std::string& path, const MatrixFlt& tri, size_t actual_nrows, int sieve,
+          const std::vector<int>& frames) {
+  FILE* f = fopen(path.c_str(), "wb");
+  if (!f) return 1;
+  const uint64_t original = actual_nrows;
+  const uint64_t nrows = tri.Nrows();
+  const int64_t sv = sieve;
+  bool ok = fwrite(kMagic, 1, 4, f) == 4 && fwrite(&original, 8, 1, f) == 1 &&
+            fwrite(&nrows, 8, 1, f) == 1 && fwrite(&sv, 8, 1, f) == 1;
+  if (ok && tri.size()) ok = fwrite(tri.data(), sizeof(float), tri.size(), f) == tri.size();
+  if (ok && sieve != 1) {
+    // 'T' for a frame that was sieved out, 'F' for a cached one.
+    std::vector<char> present(original, 'T');
+    for (int fr : frames)
+      if (fr >= 0 && (size_t)fr < original) present[fr] = 'F';
+    ok = fwrite(present.data(), 1, present.size(), f) == present.size();
+  }
+  return (fclose(f) != 0 || !ok) ? 1 : 0;
+}
+
+int Read(const std::string& path, MatrixFlt* tri, size_t* actual_nrows, int* sieve,
+         std::vector<int>* frames) {
+  FILE* f = fopen(path.c_str(), "rb");
+  if (!f) return 1;
+  unsigned char magic[4];
+  uint64_t original = 0, nrows = 0;
+  int64_t sv = 1;
+  bool ok = fread(magic, 1, 4, f) == 4 && memcmp(magic, kMagic, 4) == 0 &&
+            fread(&original, 8, 1, f) == 1 && fread(&nrows, 8, 1, f) == 1 &&
+            fread(&sv, 8, 1, f) == 1;
+  if (ok) {
+    tri->AllocateTriangle(nrows);
+    if (tri->size()) ok = fread(tri->data(), sizeof(float), tri->size(), f) == tri->size();
+  }
+  frames->clear();
+  if (ok && sv != 1) {
+    std::vector<char> present(original);
+    ok = fread(present.data(), 1, original, f) == original;
+    for (size_t i = 0; ok && i < original; ++i)
+      if (present[i] == 'F') frames->push_back((int)i);
+  } else if (ok) {
+    for (uint64_t i = 0; i < nrows; ++i) frames->push_back((int)i);
+  }
+  fclose(f);
+  if (!ok) return 1;
+  *actual_nrows = original;
+  *sieve = (int)sv;
+  return 0;
+}
+
+}  // namespace cmatrix
+
+// ---- GpuSession -------------------------------------------------------------------
+GpuSession::GpuSession(int device) : device_(device) {
+  if (rmsd_ctx_create(&ctx_, device, nullptr) != RMSD_OK) {
+    ctx_ = nullptr;
+    err_ = "cannot create the RMSD context on device " + std::to_string(device);
+  }
+}
+
+GpuSession::~GpuSession() {
+  if (ctx_) {
+    rmsd_ctx_synchronize(ctx_);
+    rmsd_ctx_destroy(ctx_);
+  }
+  (void)hipSetDevice(device_);
+  for (auto& kv : bufs_)
+    if (kv.second.p) (void)hipFree(kv.second.p);
+}
+
+void* GpuSession::Buffer(const std::string& name, size_t bytes) {
+  Buf& b = bufs_[name];
+  if (bytes == 0) bytes = 16;
+  if (b.bytes >= bytes) return b.p;
+  (void)hipSetDevice(device_);
+  if (b.p) {
+    Sync();
+    (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  if (hipMalloc(&b.p, bytes) != hipSuccess) {
+    b.p = nullptr;
+    err_ = "cannot allocate " + std::to_string(bytes) + " device bytes for " + name;
+    return nullptr;
+  }
+  b.bytes = bytes;
+  return b.p;
+}
+
+int GpuSession::Upload(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return 0;
+  (void)hipSetDevice(device_);
+  // The context's stream is the null stream: a synchronous copy is ordered
+  // with the library's work.
+  if (hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    err_ = "host to device copy failed";
+    return 1;
+  }
+  return 0;
+}
+
+int GpuSession::Download(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return 0;
+  (void)hipSetDevice(device_);
+  if (hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+    err_ = "device to host copy failed";
+    return 1;
+  }
+  return 0;
+}
+
+int GpuSession::Sync() {
+  if (!ctx_) return 1;
+  return Check(rmsd_ctx_synchronize(ctx_), "synchronize");
+}
+
+int GpuSession::Check(int rc, const char* what) {
+  if (rc == RMSD_OK) return 0;
+  err_ = std::string(what) + ": " + (ctx_ ? rmsd_last_error(ctx_) : "no context");
+  return 1;
+}
+
+int GpuSession::FramesOnDevice(const std::string& tag, const CoordsSet& set, const AtomMask& sel,
+                               rmsd_frames* out, const float* xyz_override, int nframes) {
+  const int nf = nframes < 0 ? set.nframes() : nframes;
+  const float* src = xyz_override ? xyz_override : set.xyz.data();
+  const size_t fbytes = sizeof(float) * (size_t)set.stride * (size_t)nf;
+  float* dx = (float*)Buffer(tag + ".xyz", fbytes);
+  int32_t* ds = (int32_t*)Buffer(tag + ".sel", sizeof(int32_t) * sel.size());
+  double* dm = (double*)Buffer(tag + ".mass", sizeof(double) * sel.size());
+  if (!dx || !ds || !dm) return 1;
+  std::vector<int32_t> s32(sel.begin(), sel.end());
+  std::vector<double> m(sel.size(), 1.0);
+  for (size_t k = 0; k < sel.size(); ++k) {
+    if (sel[k] < 0 || sel[k] >= set.top.natom) {
+      err_ = "selection index out of range for set " + set.name;
+      return 1;
+    }
+    if (!set.top.mass.empty()) m[k] = set.top.mass[sel[k]];
+  }
+  if (Upload(dx, src, fbytes) || Upload(ds, s32.data(), sizeof(int32_t) * s32.size()) ||
+      Upload(dm, m.data(), sizeof(double) * m.size()))
+    return 1;
+  memset(out, 0, sizeof(*out));
+  out->xyz = dx;
+  out->stride = set.stride;
+  out->ld = 0;
+  out->nframes = nf;
+  out->natom = set.top.natom;
+  out->sel = ds;
+  out->nsel = (int32_t)sel.size();
+  out->mass = dm;
+  out->fidx = nullptr;
+  return 0;
+}
+
+}  // namespace cpptraj_amd

@@ -1,0 +1,526 @@
+// host_rms.cpp -- Analysis_Rms2d, Action_Rmsd and Metric_RMS over the C ABI,
+// and the command dispatch of a Session.
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "host.hpp"
+
+namespace cpptraj_amd {
+
+namespace {
+
+// Optional precision keyword of the GPU path ('prec fp64|fp32|bf16x3');
+// absent means FP64, the reference's arithmetic.
+bool parse_prec(ArgList& args, int* prec, std::string* err) {
+  const std::string p = args.GetStringKey("prec");
+  if (p.empty() || p == "fp64") *prec = RMSD_PREC_FP64;
+  else if (p == "fp32") *prec = RMSD_PREC_FP32;
+  else if (p == "bf16x3") *prec = RMSD_PREC_BF16X3;
+  else {
+    *err = "unknown precision '" + p + "' (fp64, fp32, bf16x3)";
+    return false;
+  }
+  return true;
+}
+
+// The whole topology when the expression is empty (an unset AtomMask selects
+// every atom, src/ReferenceAction.cpp:141-144).
+bool resolve_or_all(const MaskResolver& r, const std::string& expr, const CoordsSet& set,
+                    AtomMask* sel) {
+  if (expr.empty() || expr == "*") {
+    sel->resize(set.top.natom);
+    for (int i = 0; i < set.top.natom; ++i) (*sel)[i] = i;
+    return true;
+  }
+  return r && r(expr, set, sel);
+}
+
+}  // namespace
+
+// ---- Analysis_Rms2d ------------------------------------------------------------
+// Setup follows src/Analysis_Rms2d.cpp:38-148; 'dme' and 'srmsd' are other
+// metrics and are rejected here.
+RetType Analysis_Rms2d::Setup(ArgList& args, DataSetList& dsl, MaskResolver resolve,
+                              GpuSession& gpu) {
+  gpu_ = &gpu;
+  resolve_ = resolve;
+  const std::string setname = args.GetStringKey("crdset");
+  TgtTraj_ = dsl.FindCoordsSet(setname);
+  if (!TgtTraj_) {
+    gpu.SetError("rms2d: could not locate COORDS set corresponding to " + setname);
+    return ERR;
+  }
+  if (args.hasKey("nofit")) mode_ = RMS_NOFIT;
+  else if (args.hasKey("dme")) mode_ = DME;
+  else if (args.hasKey("srmsd")) mode_ = SRMSD;
+  else if (args.hasKey("qrmsd")) mode_ = QUAT;
+  else mode_ = RMS_FIT;
+  if (mode_ == DME || mode_ == SRMSD) {
+    gpu.SetError("rms2d: 'dme' and 'srmsd' are not on the GPU RMSD path");
+    return ERR;
+  }
+  useMass_ = args.hasKey("mass");
+  std::string perr;
+  if (!parse_prec(args, &precision_, &perr)) {
+    gpu.SetError("rms2d: " + perr);
+    return ERR;
+  }
+  outfile_ = args.GetStringKey("out");
+  if (outfile_.empty()) outfile_ = args.GetStringKey("rmsout");
+  if (args.hasKey("nosquare2d")) square2d_ = false;
+  else if (args.hasKey("square2d")) square2d_ = true;
+  const std::string reftraj = args.GetStringKey("reftraj");
+  useReferenceTraj_ = !reftraj.empty();
+  corrfile_ = args.GetStringKey("corr");
+  tgtMaskExpr_ = args.GetMaskNext();
+  refMaskExpr_ = args.GetMaskNext();
+  if (refMaskExpr_.empty()) refMaskExpr_ = tgtMaskExpr_;
+  if (!corrfile_.empty()) {
+    if (useReferenceTraj_) {
+      gpu.SetError("Keyword 'corr' not supported with 'reftraj'");
+      return ERR;
+    }
+    if (tgtMaskExpr_ != refMaskExpr_) {
+      gpu.SetError("Keyword 'corr' not supported when masks differ if not using 'reftraj'");
+      return ERR;
+    }
+  }
+  if (useReferenceTraj_) {
+    RefTraj_ = dsl.FindCoordsSet(reftraj);
+    if (!RefTraj_) {
+      gpu.SetError("rms2d: reference trajectory '" + reftraj + "' is not a loaded COORDS set");
+      return ERR;
+    }
+  }
+  std::string name = args.GetStringNext();
+  if (name.empty()) name = dsl.GenerateDefaultName("Rms2d");
+  rmsdataset_ = dsl.AddMatrix(name);
+  if (!corrfile_.empty()) {
+    Ct_ = dsl.AddSeries(name + "[Corr]");
+    Ct_->width = 8;
+    Ct_->prec = 3;
+  }
+  return OK;
+}
+
+// Analyze + Calculate_2D (src/Analysis_Rms2d.cpp:151-295): the triangle when
+// the masks are equal and there is no reference trajectory, else the full
+// matrix (columns = reference frames, rows = targets); one library call.
+RetType Analysis_Rms2d::Analyze() {
+  GpuSession& g = *gpu_;
+  AtomMask tsel, rsel;
+  if (!resolve_or_all(resolve_, tgtMaskExpr_, *TgtTraj_, &tsel) || tsel.empty()) {
+    g.SetError("No atoms selected for [" + tgtMaskExpr_ + "]");
+    return ERR;
+  }
+  CoordsSet* ref = useReferenceTraj_ ? RefTraj_ : TgtTraj_;
+  if (!resolve_or_all(resolve_, refMaskExpr_, *ref, &rsel) || rsel.empty()) {
+    g.SetError("Could not set up reference mask [" + refMaskExpr_ + "]");
+    return ERR;
+  }
+  if (rsel.size() != tsel.size()) {
+    g.SetError("# Selected atoms in '" + tgtMaskExpr_ + "' not equal to selected # atoms in '" +
+               refMaskExpr_ + "'");
+    return ERR;
+  }
+  const int mode = mode_ == RMS_NOFIT ? RMSD_MODE_NOFIT
+                                      : (mode_ == QUAT ? RMSD_MODE_QUAT : RMSD_MODE_FIT);
+  const bool tri = !useReferenceTraj_ && tgtMaskExpr_ == refMaskExpr_;
+  rmsd_frames T, R;
+  if (g.FramesOnDevice("rms2d.tgt", *TgtTraj_, tsel, &T)) return ERR;
+  const int ntgt = TgtTraj_->nframes();
+  if (tri) {
+    rmsdataset_->AllocateTriangle(ntgt);
+    float* d = (float*)g.Buffer("rms2d.out", sizeof(float) * std::max<size_t>(1, rmsdataset_->size()));
+    if (!d) return ERR;
+    if (g.Check(rmsd_pairwise(g.ctx(), &T, nullptr, mode, useMass_, precision_, d), "rms2d"))
+      return ERR;
+    if (Ct_) {
+      double* ct = (double*)g.Buffer("rms2d.corr", sizeof(double) * std::max(ntgt, 1));
+      if (!ct) return ERR;
+      if (g.Check(rmsd_matrix_autocorr(g.ctx(), d, ntgt, ct), "rms2d corr")) return ERR;
+      Ct_->v.resize(ntgt);
+      if (g.Download(Ct_->v.data(), ct, sizeof(double) * ntgt)) return ERR;
+    }
+    if (g.Download(rmsdataset_->data(), d, sizeof(float) * rmsdataset_->size())) return ERR;
+  } else {
+    if (g.FramesOnDevice("rms2d.ref", *ref, rsel, &R)) return ERR;
+    const int nref = ref->nframes();
+    rmsdataset_->Allocate2D(nref, ntgt);
+    float* d = (float*)g.Buffer("rms2d.out", sizeof(float) * std::max<size_t>(1, rmsdataset_->size()));
+    if (!d) return ERR;
+    if (g.Check(rmsd_pairwise(g.ctx(), &T, &R, mode, useMass_, precision_, d), "rms2d"))
+      return ERR;
+    if (g.Download(rmsdataset_->data(), d, sizeof(float) * rmsdataset_->size())) return ERR;
+  }
+  if (!outfile_.empty() && dataio::WriteMatrix(outfile_, *rmsdataset_, square2d_)) {
+    g.SetError("cannot write " + outfile_);
+    return ERR;
+  }
+  if (Ct_ && dataio::WriteSeries(corrfile_, {Ct_})) {
+    g.SetError("cannot write " + corrfile_);
+    return ERR;
+  }
+  return OK;
+}
+
+// ---- Action_Rmsd ------------------------------------------------------------------
+// Init follows src/Action_Rmsd.cpp:39-189 and ReferenceAction::InitRef
+// (src/ReferenceAction.cpp:72-130); 'perres' is another analysis and is
+// rejected here.
+RetType Action_Rmsd::Init(ArgList& args, DataSetList& dsl, MaskResolver resolve,
+                          GpuSession& gpu) {
+  gpu_ = &gpu;
+  resolve_ = resolve;
+  fit_ = !args.hasKey("nofit");
+  if (fit_) {
+    if (args.hasKey("norotate")) mode_ = TRANS_ONLY;
+    else if (args.hasKey("nomod")) mode_ = NONE;
+  }
+  useMass_ = args.hasKey("mass");
+  std::string perr;
+  if (!parse_prec(args, &precision_, &perr)) {
+    gpu.SetError("rms: " + perr);
+    return ERR;
+  }
+  if (precision_ != RMSD_PREC_FP64) {
+    gpu.SetError("rms: the frame-to-reference fit is FP64 only");
+    return ERR;
+  }
+  outfile_ = args.GetStringKey("out");
+  const bool saveMatrices = args.hasKey("savematrices");
+  if (saveMatrices) matricesout_ = args.GetStringKey("matricesout");
+  const std::string sv = args.GetStringKey("savevectors");
+  if (sv.empty()) tvecType_ = NO_TVEC;
+  else if (sv == "combined") tvecType_ = COMBINED;
+  else if (sv == "separate") tvecType_ = SEPARATE;
+  else {
+    gpu.SetError("Expected 'combined' or 'separate' for 'savevectors'");
+    return ERR;
+  }
+  if (tvecType_ != NO_TVEC) vecsout_ = args.GetStringKey("vecsout");
+  // Reference modes.
+  refMode_ = FIRST;
+  if (args.hasKey("previous")) {
+    refMode_ = PREVIOUS;
+  } else if (!args.hasKey("first")) {
+    if (args.Contains("reftraj")) {
+      const std::string rt = args.GetStringKey("reftraj");
+      refCrd_ = dsl.FindCoordsSet(rt);
+      if (!refCrd_) {
+        gpu.SetError("reference trajectory '" + rt + "' is not a loaded COORDS set");
+        return ERR;
+      }
+      refMode_ = TRAJ;
+    } else {
+      std::string rn;
+      bool want = false;
+      if (args.Contains("ref")) {
+        rn = args.GetStringKey("ref");
+        want = true;
+      } else if (args.hasKey("reference")) {
+        want = true;
+      }
+      if (want) {
+        refCrd_ = dsl.FindReference(rn);
+        if (!refCrd_) {
+          gpu.SetError("reference '" + rn + "' not found");
+          return ERR;
+        }
+        refMode_ = FRAME;
+      }
+    }
+  }
+  if (args.hasKey("perres")) {
+    gpu.SetError("rms: 'perres' is not on the GPU RMSD path");
+    return ERR;
+  }
+  tgtMaskExpr_ = args.GetMaskNext();
+  refMaskExpr_ = args.GetMaskNext();
+  if (refMaskExpr_.empty()) refMaskExpr_ = tgtMaskExpr_;
+  std::string name = args.GetStringNext();
+  if (name.empty()) name = dsl.GenerateDefaultName("RMSD");
+  rmsd_ = dsl.AddSeries(name);
+  rmsd_->width = 12;
+  rmsd_->prec = 4;
+  if (saveMatrices) {
+    if (!fit_) {
+      gpu.SetError("Must be fitting in order to save rotation matrices.");
+      return ERR;
+    }
+    rmatrices_ = dsl.AddSeries(name + "[RM]");
+    rmatrices_->ncomp = 9;
+    rmatrices_->width = 12;
+    rmatrices_->prec = 9;
+  }
+  if (tvecType_ != NO_TVEC) {
+    if (!fit_) {
+      gpu.SetError("Must be fitting in order to save translation vectors.");
+      return ERR;
+    }
+    tvecs_ = dsl.AddSeries(name + "[TV]");
+    tvecs_->ncomp = 6;
+    tvecs_->width = 8;
+    tvecs_->prec = 4;
+  }
+  return OK;
+}
+
+// Frames f0 .. f0+nf-1 of crd against one reference frame (refxyz, whose
+// selected atoms are the reference mask of reftop): one library call.
+RetType Action_Rmsd::RunBatch(CoordsSet& crd, const AtomMask& tsel, int f0, int nf,
+                              const float* refxyz, const Topology& reftop,
+                              const std::string& refname) {
+  GpuSession& g = *gpu_;
+  AtomMask rsel;
+  CoordsSet refset;
+  refset.name = refname;
+  refset.top = reftop;
+  refset.stride = 3 * (int64_t)reftop.natom;
+  refset.xyz.assign(refxyz, refxyz + refset.stride);
+  if (!resolve_or_all(resolve_, refMaskExpr_, refset, &rsel) || rsel.empty()) {
+    g.SetError("No reference atoms selected for [" + refMaskExpr_ + "]");
+    return ERR;
+  }
+  if (rsel.size() != tsel.size()) {
+    g.SetError("Number of atoms in target mask does not equal number of atoms in reference mask");
+    return ERR;
+  }
+  rmsd_frames F, R;
+  if (g.FramesOnDevice("rms.tgt", crd, tsel, &F, crd.frame(f0), nf)) return ERR;
+  if (g.FramesOnDevice("rms.ref", refset, rsel, &R)) return ERR;
+  double* drmsd = (double*)g.Buffer("rms.rmsd", sizeof(double) * nf);
+  double* dU = rmatrices_ ? (double*)g.Buffer("rms.U", sizeof(double) * 9 * nf) : nullptr;
+  double* dT = tvecs_ ? (double*)g.Buffer("rms.T", sizeof(double) * 6 * nf) : nullptr;
+  if (!drmsd || (rmatrices_ && !dU) || (tvecs_ && !dT)) return ERR;
+  const int modify = !fit_ ? RMSD_MODIFY_NONE
+                           : (mode_ == ROT_AND_TRANS ? RMSD_MODIFY_ROT_TRANS
+                                                     : (mode_ == TRANS_ONLY ? RMSD_MODIFY_TRANS
+                                                                            : RMSD_MODIFY_NONE));
+  if (g.Check(rmsd_fit_to_ref(g.ctx(), &F, &R, 0, fit_, useMass_, modify, drmsd, dU, dT), "rms"))
+    return ERR;
+  const size_t base = rmsd_->v.size();
+  rmsd_->v.resize(base + nf);
+  if (g.Download(rmsd_->v.data() + base, drmsd, sizeof(double) * nf)) return ERR;
+  if (rmatrices_) {
+    const size_t b = rmatrices_->v.size();
+    rmatrices_->v.resize(b + 9 * (size_t)nf);
+    if (g.Download(rmatrices_->v.data() + b, dU, sizeof(double) * 9 * nf)) return ERR;
+  }
+  if (tvecs_) {
+    std::vector<double> tv(6 * (size_t)nf);
+    if (g.Download(tv.data(), dT, sizeof(double) * tv.size())) return ERR;
+    const size_t b = tvecs_->v.size();
+    tvecs_->v.resize(b + 6 * (size_t)nf);
+    for (int f = 0; f < nf; ++f)
+      for (int c = 0; c < 3; ++c) {
+        double* o = tvecs_->v.data() + b + 6 * (size_t)f;
+        const double t = tv[6 * f + c], r = tv[6 * f + 3 + c];
+        if (tvecType_ == COMBINED) {
+          // Vxyz = Trans + refTrans; the origin columns stay 0.
+          o[c] = t + r;
+          o[3 + c] = 0.0;
+        } else {
+          o[c] = t;
+          o[3 + c] = r;
+        }
+      }
+  }
+  if (modify != RMSD_MODIFY_NONE &&
+      g.Download(crd.frame(f0), F.xyz, sizeof(float) * (size_t)crd.stride * nf))
+    return ERR;
+  return OK;
+}
+
+RetType Action_Rmsd::DoFrames(CoordsSet& crd) {
+  GpuSession& g = *gpu_;
+  AtomMask tsel;
+  if (!resolve_or_all(resolve_, tgtMaskExpr_, crd, &tsel) || tsel.empty()) {
+    g.SetError("No atoms selected for [" + tgtMaskExpr_ + "]");
+    return ERR;
+  }
+  const int nf = crd.nframes();
+  if (nf == 0) return OK;
+  switch (refMode_) {
+    case FIRST: {
+      // Reference = the first frame as read, before this action moves it.
+      std::vector<float> first(crd.frame(0), crd.frame(0) + crd.stride);
+      return RunBatch(crd, tsel, 0, nf, first.data(), crd.top, crd.name);
+    }
+    case FRAME:
+      return RunBatch(crd, tsel, 0, nf, refCrd_->frame(0), refCrd_->top, refCrd_->name);
+    case TRAJ:
+      // Frame i against frame i of the reference trajectory
+      // (ReferenceAction::ActionRef, src/ReferenceAction.h:72-84).
+      if (refCrd_->nframes() < nf) {
+        g.SetError("reference trajectory has fewer frames than the target");
+        return ERR;
+      }
+      for (int f = 0; f < nf; ++f)
+        if (RunBatch(crd, tsel, f, 1, refCrd_->frame(f), refCrd_->top, refCrd_->name) != OK)
+          return ERR;
+      return OK;
+    case PREVIOUS: {
+      // Reference = the previous frame after this action's update of it
+      // (ReferenceAction::PreviousRef, src/ReferenceAction.h:86-88).
+      std::vector<float> prev(crd.frame(0), crd.frame(0) + crd.stride);
+      for (int f = 0; f < nf; ++f) {
+        if (RunBatch(crd, tsel, f, 1, prev.data(), crd.top, crd.name) != OK) return ERR;
+        std::copy(crd.frame(f), crd.frame(f) + crd.stride, prev.begin());
+      }
+      return OK;
+    }
+  }
+  return ERR;
+}
+
+int Action_Rmsd::WriteOutput() const {
+  if (!outfile_.empty() && dataio::WriteSeries(outfile_, {rmsd_})) return 1;
+  if (rmatrices_ && !matricesout_.empty() && dataio::WriteSeries(matricesout_, {rmatrices_}))
+    return 1;
+  if (tvecs_ && !vecsout_.empty() && dataio::WriteSeries(vecsout_, {tvecs_})) return 1;
+  return 0;
+}
+
+// ---- Metric_RMS + pairwise cache -------------------------------------------------------
+RetType Metric_RMS::Setup(ArgList& args, DataSetList& dsl, MaskResolver resolve,
+                          GpuSession& gpu) {
+  gpu_ = &gpu;
+  resolve_ = resolve;
+  crd_ = dsl.FindCoordsSet(args.GetStringKey("crdset"));
+  if (!crd_) {
+    gpu.SetError("cluster: no COORDS set");
+    return ERR;
+  }
+  if (args.hasKey("dme") || args.hasKey("srmsd") || args.hasKey("data")) {
+    gpu.SetError("cluster: only the 'rms' metric is on the GPU RMSD path");
+    return ERR;
+  }
+  (void)args.hasKey("rms");
+  useMass_ = args.hasKey("mass");
+  nofit_ = args.hasKey("nofit");
+  sieve_ = args.getKeyInt("sieve", 1);
+  if (sieve_ < 1) {
+    gpu.SetError("cluster: only regular sieves (sieve >= 1) are supported");
+    return ERR;
+  }
+  if (args.hasKey("random")) {
+    gpu.SetError("cluster: random sieving is not supported");
+    return ERR;
+  }
+  std::string perr;
+  if (!parse_prec(args, &precision_, &perr)) {
+    gpu.SetError("cluster: " + perr);
+    return ERR;
+  }
+  pairdist_ = args.GetStringKey("pairdist");
+  save_ = args.hasKey("savepairdist");
+  maskExpr_ = args.GetMaskNext();
+  if (!resolve_or_all(resolve_, maskExpr_, *crd_, &sel_) || sel_.empty()) {
+    gpu.SetError("cluster: no atoms selected for [" + maskExpr_ + "]");
+    return ERR;
+  }
+  std::string name = args.GetStringNext();
+  if (name.empty()) name = dsl.GenerateDefaultName("CpptrajPairDist");
+  cache_ = dsl.AddMatrix(name);
+  // Regular sieve (Cluster::Sieve::SetFramesToCluster, src/Cluster/Sieve.cpp:42-55).
+  frames_.clear();
+  for (int f = 0; f < crd_->nframes(); f += sieve_) frames_.push_back(f);
+  return OK;
+}
+
+// MetricArray::CacheDistances + calcFrameDistances (src/Cluster/
+// MetricArray.cpp:722-801): the triangle over framesToCache in one call.
+RetType Metric_RMS::CacheDistances() {
+  GpuSession& g = *gpu_;
+  rmsd_frames F;
+  if (g.FramesOnDevice("cluster.crd", *crd_, sel_, &F)) return ERR;
+  const int n = (int)frames_.size();
+  int32_t* fidx = (int32_t*)g.Buffer("cluster.fidx", sizeof(int32_t) * std::max(n, 1));
+  if (!fidx) return ERR;
+  std::vector<int32_t> f32(frames_.begin(), frames_.end());
+  if (g.Upload(fidx, f32.data(), sizeof(int32_t) * n)) return ERR;
+  F.fidx = fidx;
+  F.nframes = n;
+  cache_->AllocateTriangle(n);
+  float* d = (float*)g.Buffer("cluster.out", sizeof(float) * std::max<size_t>(1, cache_->size()));
+  if (!d) return ERR;
+  if (g.Check(rmsd_cluster_cache(g.ctx(), &F, nofit_, useMass_, precision_, 1.0, d), "cluster"))
+    return ERR;
+  if (g.Download(cache_->data(), d, sizeof(float) * cache_->size())) return ERR;
+  if (save_ || !pairdist_.empty()) {
+    const std::string path = pairdist_.empty() ? "CpptrajPairDist" : pairdist_;
+    if (cmatrix::Write(path, *cache_, crd_->nframes(), sieve_, frames_)) {
+      g.SetError("cannot write " + path);
+      return ERR;
+    }
+  }
+  return OK;
+}
+
+double Metric_RMS::FrameDist(int f1, int f2) {
+  GpuSession& g = *gpu_;
+  rmsd_frames A, B;
+  if (g.FramesOnDevice("metric.a", *crd_, sel_, &A, crd_->frame(f1), 1) ||
+      g.FramesOnDevice("metric.b", *crd_, sel_, &B, crd_->frame(f2), 1))
+    return -1.0;
+  float* d = (float*)g.Buffer("metric.out", sizeof(float));
+  if (!d) return -1.0;
+  // Full 1 x 1 block: row = f1 as the target, column = f2 as the reference
+  // (Frame::RMSD(Ref) centres the reference, src/Cluster/Metric_RMS.cpp:44-63).
+  if (g.Check(rmsd_pairwise(g.ctx(), &A, &B, nofit_ ? RMSD_MODE_NOFIT : RMSD_MODE_FIT, useMass_,
+                            RMSD_PREC_FP64, d),
+              "FrameDist"))
+    return -1.0;
+  float v = 0.f;
+  if (g.Download(&v, d, sizeof(float))) return -1.0;
+  return v;
+}
+
+// ---- Session ------------------------------------------------------------------------------
+int Session::Run(const std::string& line) {
+  err.clear();
+  if (!gpu.ok()) {
+    err = gpu.error();
+    return 1;
+  }
+  ArgList args(line);
+  const std::string& cmd = args.Command();
+  MaskResolver resolve = [this](const std::string& expr, const CoordsSet& set, AtomMask* sel) {
+    auto it = masks.find(set.name + "|" + expr);  // set-specific entry first
+    if (it == masks.end()) it = masks.find(expr);
+    if (it == masks.end()) return false;
+    for (int a : it->second)
+      if (a < 0 || a >= set.top.natom) return false;
+    *sel = it->second;
+    return true;
+  };
+  int rc = 1;
+  if (cmd == "rms2d" || cmd == "2drms") {
+    Analysis_Rms2d a;
+    rc = a.Setup(args, dsl, resolve, gpu) == OK && a.Analyze() == OK ? 0 : 1;
+  } else if (cmd == "rms" || cmd == "rmsd") {
+    Action_Rmsd a;
+    const std::string crdset = args.GetStringKey("crdset");
+    rc = 1;
+    if (a.Init(args, dsl, resolve, gpu) == OK) {
+      CoordsSet* crd = dsl.FindCoordsSet(crdset);
+      if (!crd) gpu.SetError("rms: no COORDS set");
+      else if (a.DoFrames(*crd) == OK && a.WriteOutput() == 0) rc = 0;
+      else if (gpu.error().empty()) gpu.SetError("rms: cannot write output");
+    }
+  } else if (cmd == "cluster") {
+    Metric_RMS m;
+    rc = m.Setup(args, dsl, resolve, gpu) == OK && m.CacheDistances() == OK ? 0 : 1;
+  } else {
+    err = "unknown command '" + cmd + "'";
+    return 1;
+  }
+  if (rc) err = gpu.error().empty() ? "command failed" : gpu.error();
+  return rc;
+}
+
+}  // namespace cpptraj_amd

@@ -125,6 +125,12 @@ int rmsd_ctx_stage_ms(rmsd_ctx* ctx, double* ms);
 int rmsd_pairwise(rmsd_ctx* ctx, const rmsd_frames* tgt, const rmsd_frames* ref, int mode,
                   int use_mass, int precision, float* out);
 
+/* RMSD auto-correlation of a packed triangle (rms2d 'corr').
+ * Replaces Analysis_Rms2d::CalcAutoCorr (src/Analysis_Rms2d.cpp:301-319):
+ * ct[0] = 1 and ct[lag] = sum_j exp(-R(j, j+lag)) / (n - lag) for
+ * lag = 1 .. n-1, FP64, over the n*(n-1)/2 floats of `tri` (device). */
+int rmsd_matrix_autocorr(rmsd_ctx* ctx, const float* tri, int n, double* ct);
+
 /* Cluster pairwise-distance cache fill.
  * Replaces Cluster::MetricArray::calcFrameDistances (src/Cluster/
  * MetricArray.cpp:766-801) for one RMS metric (Metric_RMS::FrameDist,
@@ -171,6 +177,46 @@ int rmsd_fit_backward(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frame
 int rmsd_pairwise_backward(rmsd_ctx* ctx, const rmsd_frames* A, const rmsd_frames* B,
                            int use_mass, const double* grad_out, double* grad_a,
                            double* grad_b);
+
+/* Batched all-pairs matrices: many trajectories or masks, of uneven frame and
+ * atom counts, in one launch of the contraction.  No single reference
+ * counterpart: replaces one Analysis_Rms2d::Calculate_2D pass
+ * (src/Analysis_Rms2d.cpp:196-295) per COORDS set, for the same-mask case
+ * (packed upper triangle of each set into outs[s], nframes*(nframes-1)/2
+ * floats, as rmsd_pairwise with ref == NULL).  Sets may differ in nframes,
+ * natom, nsel, selection and masses.
+ *
+ * A plan binds the sets and outputs (device pointers) and owns one device
+ * allocation with every set's operands and the tile queues: every set's
+ * triangle tiles, sorted by cost and cut into one cost-balanced queue per XCD.
+ * rmsd_batch_run recomputes all matrices from the current contents of the
+ * bound frame buffers (per-set preparation kernels, then one batched kernel).
+ * Flags:
+ *   RMSD_BATCH_GRAPH     capture the pass as a hipGraph at creation; runs
+ *                        replay it (ordered after prior work on the context's
+ *                        stream, and later work waits for it);
+ *   RMSD_BATCH_AUTOTUNE  time every kernel variant (tile side 64 or 32, grid or
+ *                        persistent work-stealing form) on the plan's own data
+ *                        at creation and keep the fastest.
+ * RMSD_MODE_NOFIT runs the FP64 direct-sum kernel set by set (no batching). */
+#define RMSD_BATCH_GRAPH 1
+#define RMSD_BATCH_AUTOTUNE 2
+typedef struct rmsd_batch rmsd_batch;
+int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mode, int use_mass,
+                      int precision, float* const* outs, int flags, rmsd_batch** plan);
+int rmsd_batch_run(rmsd_batch* plan);
+/* Selects kernel variant 0..3 (bit 0: persistent form, bit 1: 32-frame tile
+ * side) and re-captures the graph. */
+int rmsd_batch_set_variant(rmsd_batch* plan, int variant);
+/* info (6 values): variant, tile side, tiles, graph captured (0/1), queues,
+ * longest queue.  tune_ms (4 values): the tuner's time per variant, -1 when
+ * not tuned. */
+int rmsd_batch_info(const rmsd_batch* plan, int64_t* info, double* tune_ms);
+/* Waits for the plan's work, then frees it. */
+void rmsd_batch_destroy(rmsd_batch* plan);
+/* One-shot form: create (no graph, no tuning), run, destroy. */
+int rmsd_pairwise_batch(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mode, int use_mass,
+                        int precision, float* const* outs);
 
 #ifdef __cplusplus
 }

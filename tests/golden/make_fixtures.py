@@ -23,6 +23,9 @@ Outputs (numpy .npz, no pickles):
                    'previous' RMSD series
   rgacc.npz        rGACC.full.nc frame 1, :1-4&!@H= atoms (9 identical frames ->
                    all-zero matrix)
+  text/            the expected text outputs themselves (Test_2DRMS and
+                   Test_RMSD .save files), for byte-level checks of the host
+                   layer's DataIO_Std writers
 
 The atom masks used by these tests are resolved here with a few lines of
 parm7 parsing (residue ranges, atom names, hydrogen exclusion); the general
@@ -228,6 +231,20 @@ def main():
         frame=xg[0, sg, :].reshape(-1),
         expected=np.array(vals),
     )
+    # ---- expected text outputs, verbatim -------------------------------------------
+    txt = os.path.join(OUT, "text")
+    os.makedirs(txt, exist_ok=True)
+    for sub, names in (("Test_2DRMS", ["rmsd.dat.save", "rmsd.mass.dat.save", "trp.dat.save",
+                                       "nofit.dat.save", "rgacc.dat.save"]),
+                       ("Test_RMSD", ["rmsd.dat.save", "rmsd.mass.dat.save",
+                                      "rmsd.reftraj.dat.save", "vecs.dat.save",
+                                      "rmatrices.dat.save", "NoMod.dat.save",
+                                      "Previous.dat.save"])):
+        for n in names:
+            with open(os.path.join(T, sub, n), "rb") as fi:
+                data = fi.read()
+            with open(os.path.join(txt, sub + "__" + n), "wb") as fo:
+                fo.write(data)
     for fn in sorted(os.listdir(OUT)):
         if fn.endswith(".npz"):
             print(fn, os.path.getsize(os.path.join(OUT, fn)))

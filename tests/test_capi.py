@@ -50,3 +50,27 @@ def test_no_cpu_fallback_without_gpu():
         pytest.skip("GPU present")
     with pytest.raises(cpptraj_amd.RmsdError):
         cpptraj_amd.Context(0)
+
+
+def test_host_library_exports_its_c_entry_points():
+    """libcpptraj_rmsd_host.so (the C++ host layer) exports every chost_*
+    function of host/host_capi.h and links the kernels library."""
+    from cpptraj_amd.host import _HOST, load_host
+    if not os.path.exists(_HOST):
+        pytest.fail(f"{_HOST} not built (python -m cpptraj_amd.build)")
+    src = open(os.path.join(ROOT, "cpptraj_amd", "host", "host_capi.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = sorted(set(re.findall(r"\b(chost_\w+)\s*\(", src)))
+    assert "chost_run" in names
+    lib = load_host()
+    assert not [n for n in names if not hasattr(lib, n)]
+
+
+def test_host_session_needs_a_gpu():
+    import torch
+    from cpptraj_amd import RmsdError
+    from cpptraj_amd.host import HostSession
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    with pytest.raises(RmsdError):
+        HostSession(0)

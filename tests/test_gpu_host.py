@@ -1,0 +1,160 @@
+"""The C++ host layer (Analysis_Rms2d / Action_Rmsd / cluster cache classes)
+driven with the commands of the reference's own tests on the same frames:
+test/Test_2DRMS/RunTest.sh and test/Test_RMSD/RunTest.sh.  Their DoTest
+compares outputs with plain diff, so the text files written here must equal
+the expected .save files byte for byte (vecs.dat: |diff| <= 1e-6, the
+tolerance its DoTest passes).  Fitted coordinates are compared with the
+.crd.save values (written %8.3f) to half a unit of the last digit."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, tri_to_square
+
+pytestmark = pytest.mark.gpu
+TXT = os.path.join(GOLDEN, "text")
+
+
+def expected(name):
+    with open(os.path.join(TXT, name)) as fh:
+        return fh.read()
+
+
+def text(path):
+    with open(path) as fh:
+        return fh.read()
+
+
+@pytest.fixture
+def hs():
+    from cpptraj_amd.host import HostSession
+    s = HostSession(0)
+    yield s
+    s.close()
+
+
+def load_tz2(hs, golden):
+    g = golden("tz2_2drms")
+    hs.add_coords("tz2", g["frames"], 223, g["mass"])
+    for expr, key in ((":3-7", "sel_3_7"), (":2", "sel_2"), (":11", "sel_11"),
+                      (":2-12@CA", "sel_2_12_ca")):
+        hs.add_mask(expr, g[key])
+    return g
+
+
+# Test_2DRMS tests 1, 2, 5 and 8.
+@pytest.mark.parametrize("cmd,save", [
+    ("2drms crd1 :3-7 rmsout {out}", "Test_2DRMS__rmsd.dat.save"),
+    ("2drms crd1 :3-7 rmsout {out} mass", "Test_2DRMS__rmsd.mass.dat.save"),
+    ("2drms :2 :11 out {out}", "Test_2DRMS__trp.dat.save"),
+    ("2drms crd1 :3-7 rmsout {out} qrmsd", "Test_2DRMS__rmsd.dat.save"),
+])
+def test_2drms_outputs(hs, golden, tmp_path, cmd, save):
+    load_tz2(hs, golden)
+    out = str(tmp_path / "out.dat")
+    hs.run(cmd.format(out=out))
+    assert text(out) == expected(save)
+
+
+# Test_2DRMS test 6: 'rms first :2-12@CA' rewrites the frames, then a no-fit matrix.
+def test_2drms_nofit_after_fit(hs, golden, tmp_path):
+    load_tz2(hs, golden)
+    out = str(tmp_path / "nofit.dat")
+    hs.run("rms first :2-12@CA")
+    hs.run(f"2drms crd1 :2 nofit out {out}")
+    assert text(out) == expected("Test_2DRMS__nofit.dat.save")
+
+
+# Test_2DRMS test 7: nine identical frames, XYZ ('nosquare2d') output.
+def test_2drms_nosquare2d(hs, golden, tmp_path):
+    g = golden("rgacc")
+    fr = np.tile(g["frame"][None], (9, 1))
+    hs.add_coords("rgacc", fr, fr.shape[1] // 3)
+    out = str(tmp_path / "rgacc.dat")
+    hs.run(f"2drms crd1 * out {out} nosquare2d")
+    assert text(out) == expected("Test_2DRMS__rgacc.dat.save")
+
+
+# Test_RMSD 'Basic RMSD tests': three actions over the same frames in order.
+def test_rms_basic(hs, golden, tmp_path):
+    g = golden("tz2_rmsd")
+    n = g["frames"].shape[1] // 3
+    hs.add_coords("trunc", g["frames"], n, g["mass"])
+    hs.add_coords("raw", g["frames"].copy(), n, g["mass"])
+    hs.add_mask(":2-11", np.arange(n))
+    d = str(tmp_path)
+    hs.run(f"rms Res2-11 first :2-11 out {d}/rmsd.dat crdset trunc")
+    hs.run(f"rms Res2-11_mass first :2-11 out {d}/rmsd.mass.dat mass savevectors combined "
+           f"vecsout {d}/vecs.dat crdset trunc")
+    hs.run(f"rms Res2_11_traj reftraj raw :2-11 out {d}/rmsd.reftraj.dat crdset trunc")
+    assert text(f"{d}/rmsd.dat") == expected("Test_RMSD__rmsd.dat.save")
+    assert text(f"{d}/rmsd.mass.dat") == expected("Test_RMSD__rmsd.mass.dat.save")
+    assert text(f"{d}/rmsd.reftraj.dat") == expected("Test_RMSD__rmsd.reftraj.dat.save")
+    got, exp = text(f"{d}/vecs.dat").splitlines(), expected("Test_RMSD__vecs.dat.save").splitlines()
+    assert got[0] == exp[0] and len(got) == len(exp)
+    a = np.array([[float(t) for t in l.split()] for l in got[1:]])
+    b = np.array([[float(t) for t in l.split()] for l in exp[1:]])
+    assert np.abs(a - b).max() <= 1e-6
+
+
+# Test_RMSD 'RMS coordinate rotation/rotation matrices test'.
+def test_rms_rotation(hs, golden, tmp_path):
+    g = golden("tz2_rotate")
+    hs.add_reference("[first]", g["ref_frame"], 223)
+    hs.add_coords("WAT", g["frames"], 220)
+    hs.add_mask("@CA", g["sel_ca_ref"], set_name="[first]")
+    hs.add_mask("@CA", g["sel_ca"], set_name="WAT")
+    d = str(tmp_path)
+    hs.run("rms NOROT ref [first] norotate @CA crdset WAT")
+    nr = hs.coords("WAT").reshape(10, 220, 3)
+    assert np.abs(nr - g["norotate"]).max() <= 5e-4 + 1e-6
+    hs.run(f"rms ROT ref [first] out {d}/rms.dat @CA savematrices matricesout {d}/rmatrices.dat "
+           "crdset WAT")
+    assert text(f"{d}/rmatrices.dat") == expected("Test_RMSD__rmatrices.dat.save")
+    rot = hs.coords("WAT").reshape(10, 220, 3)
+    assert np.abs(rot - g["rotate"]).max() <= 5e-4 + 1e-6
+
+
+# Test_RMSD 'nomod' and 'previous'.
+@pytest.mark.parametrize("cmd,save", [
+    ("rms First_CA :2-12@CA out {out} nomod", "Test_RMSD__NoMod.dat.save"),
+    ("rms ToPrevious :2-12@CA previous out {out}", "Test_RMSD__Previous.dat.save"),
+])
+def test_rms_nomod_previous(hs, golden, tmp_path, cmd, save):
+    g = golden("tz2_camask")
+    n = g["frames"].shape[1] // 3
+    hs.add_coords("tz2", g["frames"], n, g["mass"])
+    hs.add_mask(":2-12@CA", np.arange(n))
+    out = str(tmp_path / "o.dat")
+    hs.run(cmd.format(out=out))
+    assert text(out) == expected(save)
+
+
+def test_rms2d_corr(hs, golden, tmp_path):
+    load_tz2(hs, golden)
+    out, corr = str(tmp_path / "m.dat"), str(tmp_path / "corr.dat")
+    hs.run(f"2drms :3-7 out {out} corr {corr} M")
+    tri = hs.matrix("M").astype(np.float64)
+    ct = hs.series("M[Corr]")
+    R = tri_to_square(tri, 10)
+    exp = [1.0] + [np.mean([np.exp(-R[j, j + i]) for j in range(10 - i)]) for i in range(1, 10)]
+    assert np.allclose(ct, exp, rtol=1e-13, atol=0)
+    assert len(text(corr).splitlines()) == 11
+
+
+def test_cluster_cache_cmatrix(hs, golden, oracle, tmp_path):
+    from cpptraj_amd.host import read_cmatrix
+    g = load_tz2(hs, golden)
+    path = str(tmp_path / "pd.cmatrix")
+    hs.run(f"cluster crdset tz2 rms :3-7 mass sieve 3 pairdist {path} savepairdist")
+    tri, actual, sieve, frames = read_cmatrix(path)
+    assert actual == 10 and sieve == 3 and list(frames) == [0, 3, 6, 9]
+    exp = oracle.cluster_cache(g["frames"], np.array([0, 3, 6, 9], np.int32), g["sel_3_7"],
+                               g["mass"][g["sel_3_7"]], use_mass=True)
+    assert np.abs(tri - exp).max() <= 1e-6
+    raw = open(path, "rb").read()
+    assert raw[:4] == b"CTM\x02"
+    assert np.frombuffer(raw[4:28], dtype="<u8")[:2].tolist() == [10, 4]
+    assert np.frombuffer(raw[20:28], dtype="<i8")[0] == 3
+    assert raw[28 + 4 * 6:] == b"FTTFTTFTTF"
