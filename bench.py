@@ -155,6 +155,109 @@ def fit_stream(ctx, F, n, steps, warmup):
     return out
 
 
+def batch_bench(ctx, seed=3, nsets=64, precision="bf16x3"):
+    """64 trajectories of uneven size (frames log-uniform in 500-20000, atoms
+    log-uniform in 50-5000, each its own chain, sigma 0.5 A, rigid motion):
+    one rms2d triangle each.  Times the tuned, graph-captured batch plan
+    against one rmsd_pairwise call per trajectory, and checks the batch
+    against the FP64 batch over every pair."""
+    import torch
+    import cpptraj_amd as ca
+    from cpptraj_amd import synth
+
+    rng = np.random.default_rng(seed)
+    Fs = np.exp(rng.uniform(np.log(500), np.log(20000), nsets)).astype(int)
+    Ns = np.exp(rng.uniform(np.log(50), np.log(5000), nsets)).astype(int)
+    sets, descs, outs = [], [], []
+    for k in range(nsets):
+        X, _ = synth.noisy_soa_torch(seed * 1000 + k, int(Fs[k]), int(Ns[k]), 0.5, ld=int(Ns[k]))
+        sets.append(X)
+        descs.append(ca.frames_desc(X, soa=True))
+        outs.append(torch.empty(int(Fs[k]) * (int(Fs[k]) - 1) // 2, dtype=torch.float32,
+                                device="cuda"))
+    pairs = Fs * (Fs - 1) // 2
+    flops = float(np.sum(18.0 * Ns * pairs))
+    t0 = time.perf_counter()
+    plan = ca.Batch(ctx, descs, outs, "fit", False, precision, graph=True, autotune=True)
+    tune_s = time.perf_counter() - t0
+    info = plan.info()
+    ms = timed_steps(plan.run, 5, 1)
+    # one call per trajectory, same precision
+    sep = [torch.empty_like(o) for o in outs]
+
+    def separate():
+        for d, o in zip(descs, sep):
+            ctx.pairwise_desc(d, None, 0, False, ca.PRECISIONS[precision], o)
+    ms_sep = timed_steps(separate, 3, 1)
+    # FP64 batch (plain kernel) and the error of the batch against it
+    ref = [torch.empty_like(o) for o in outs]
+    p64 = ca.Batch(ctx, descs, ref, "fit", False, "fp64", graph=False, autotune=False)
+    ms64 = timed_steps(p64.run, 1, 0)
+    plan.run()
+    torch.cuda.synchronize()
+    mx, ss, cnt = 0.0, 0.0, 0
+    for a, b in zip(outs, ref):
+        d = (a.double() - b.double()).abs()
+        mx = max(mx, float(d.max()))
+        ss += float((d * d).sum())
+        cnt += d.numel()
+    # separate calls must agree bit for bit with the batch
+    same = all(torch.equal(a, b) for a, b in zip(outs, sep))
+    peak_exec = PEAK_BF16_TFLOPS if precision == "bf16x3" else (
+        PEAK_FP32_TFLOPS if precision == "fp32" else PEAK_FP64_TFLOPS)
+    terms = 3 if precision == "bf16x3" else 1
+    floor_ms = flops * terms / (peak_exec * 1e12) * 1e3
+    plan.close()
+    p64.close()
+    out = {"workload": f"{nsets} trajectories, frames logU[500,20000], atoms logU[50,5000], "
+                       f"seed {seed}, FP32 SoA, fit, unweighted, {precision}",
+           "pairs": int(pairs.sum()), "algorithmic_flops": flops,
+           "batch_ms": ms, "pairs_per_s": float(pairs.sum()) / (ms / 1e3),
+           "algo_tflops": flops / (ms / 1e3) / 1e12,
+           "separate_calls_ms": ms_sep, "speedup_vs_separate": ms_sep / ms,
+           "sum_of_floors_ms": floor_ms, "frac_of_floor": floor_ms / ms,
+           "fp64_batch_ms": ms64, "max_err_vs_fp64": mx,
+           "rms_err_vs_fp64": (ss / max(cnt, 1)) ** 0.5,
+           "bit_identical_to_separate_calls": bool(same),
+           "plan": info, "plan_create_s": tune_s}
+    del sets, outs, sep, ref
+    torch.cuda.empty_cache()
+    return out
+
+
+def grad_bench(ctx, steps=5, warmup=2):
+    """4096 frames x 2000 atoms, masses: loss = sum_f RMSD(frame f, frame 0);
+    forward (fit, nomod) and backward through rmsd_fit_backward; and the
+    512 x 512 pairwise block gradient with rotations recomputed."""
+    import torch
+    import cpptraj_amd as ca
+    from cpptraj_amd import synth
+    from cpptraj_amd.autograd import fit_backward, pairwise_backward
+
+    F, n = 4096, 2000
+    X = torch.as_tensor(synth.noisy(4, F, n, 1.0)).cuda()
+    m = torch.as_tensor(synth.masses(np.random.default_rng(4), n), dtype=torch.float64).cuda()
+    d = ca.frames_desc(X, None, m)
+    r = ca.frames_desc(X[:1], None, m)
+    rmsd = torch.empty(F, dtype=torch.float64, device="cuda")
+    g = torch.ones(F, dtype=torch.float64, device="cuda")
+    fwd = timed_steps(lambda: ctx.fit_desc(d, r, 0, True, True, 0, rmsd), steps, warmup)
+    bwd = timed_steps(lambda: fit_backward(X, X[:1], g, m, True, ctx=ctx), steps, warmup)
+    A, B = X[:512], X[512:1024]
+    G = torch.ones((512, 512), dtype=torch.float64, device="cuda")
+    blk = timed_steps(lambda: pairwise_backward(A, B, G, m, True, ctx=ctx), steps, warmup)
+    # bytes: forward reads 12 n per frame; backward re-reads 12 n and writes 24 n (FP64)
+    out = {"workload": f"{F} frames x {n} atoms AoS, masses, loss = sum RMSD to frame 0; "
+                       "512 x 512 block gradient",
+           "forward_ms": fwd, "backward_ms": bwd,
+           "backward_gbs": F * (12.0 * n + 24.0 * n) / (bwd / 1e3) / 1e9,
+           "block_grad_ms": blk,
+           "block_grad_pairs_per_s": 512 * 512 / (blk / 1e3)}
+    del X
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +272,8 @@ def main():
                     help="time one precision only (profiling runs)")
     ap.add_argument("--fit-frames", type=int, default=1000000)
     ap.add_argument("--fit-atoms", type=int, default=3000)
+    ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--skip-grad", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -276,6 +381,14 @@ def main():
         log(f"fit stream {args.fit_frames} x {args.fit_atoms}")
         line["fit_stream"] = fit_stream(ctx, args.fit_frames, args.fit_atoms, args.steps,
                                         args.warmup)
+    if not args.skip_batch:
+        log("batch of 64 trajectories")
+        line["batch"] = batch_bench(ctx)
+        log(f"batch {line['batch']['batch_ms']:.2f} ms vs separate "
+            f"{line['batch']['separate_calls_ms']:.2f} ms")
+    if not args.skip_grad:
+        log("gradients")
+        line["grad"] = grad_bench(ctx, args.steps, args.warmup)
     ctx.close()
     print(json.dumps(line), flush=True)
 

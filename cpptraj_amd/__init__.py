@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 __all__ = ["RmsdError", "Context", "Batch", "frames_desc", "rms2d", "rms2d_batch",
            "cluster_cache", "rmsd_fit", "library_path", "load_library", "MODES", "PRECISIONS",
@@ -183,8 +184,12 @@ class Context:
             raise RmsdError(f"rmsd_ctx_create failed ({rc})")
         self._h = h
         self._L = L
+        self._plans = weakref.WeakSet()
 
     def close(self):
+        """Destroys the context and every batch plan still bound to it."""
+        for p in list(getattr(self, "_plans", ())):
+            p.close()
         if getattr(self, "_h", None):
             self._L.rmsd_ctx_destroy(self._h)
             self._h = None
@@ -332,6 +337,7 @@ class Batch:
         ctx.check(self._L.rmsd_batch_create(ctx.handle, self._sets, n, MODES[mode], int(use_mass),
                                             PRECISIONS[precision], self._outs, flags, C.byref(h)))
         self._h = h
+        ctx._plans.add(self)
 
     def run(self):
         self.ctx.check(self._L.rmsd_batch_run(self._h))
@@ -347,9 +353,10 @@ class Batch:
                 "longest_queue": i[5], "tune_ms": list(t)}
 
     def close(self):
-        if getattr(self, "_h", None):
+        # A plan whose context is gone was freed with it.
+        if getattr(self, "_h", None) and getattr(self.ctx, "_h", None):
             self._L.rmsd_batch_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:

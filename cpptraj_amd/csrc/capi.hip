@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <map>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -48,7 +49,13 @@ struct rmsd_ctx {
   bool events_made = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int marks = 0;
+  // Batch plans bound to this context; freed with it if still alive.
+  std::set<rmsd_batch*> plans;
 };
+
+namespace {
+void batch_free(rmsd_batch* b);
+}  // namespace
 
 namespace {
 
@@ -313,6 +320,7 @@ int rmsd_ctx_create(rmsd_ctx** ctx, int device, void* stream) {
 
 void rmsd_ctx_destroy(rmsd_ctx* ctx) {
   if (!ctx) return;
+  while (!ctx->plans.empty()) batch_free(*ctx->plans.begin());
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   for (auto& kv : ctx->bufs)
@@ -809,6 +817,7 @@ int batch_run_on(rmsd_batch* b, int v) {
 void batch_free(rmsd_batch* b) {
   if (!b) return;
   if (b->ctx) {
+    b->ctx->plans.erase(b);
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
   }
@@ -847,6 +856,7 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   rmsd_batch* b = new rmsd_batch();
   b->ctx = ctx;
+  ctx->plans.insert(b);
   b->mode = mode;
   b->prec = mode == RMSD_MODE_NOFIT ? RMSD_PREC_FP64 : precision;
   b->use_mass = use_mass;

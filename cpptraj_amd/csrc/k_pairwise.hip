@@ -557,6 +557,114 @@ __global__ __launch_bounds__(64 * NWR * NWC) void k_pairwise_rs(PairArgs args) {
 
 }  // namespace
 
+// ---- LDS-DMA contraction for the MFMA-reduced forms --------------------------------
+// A 256-thread workgroup owns 64 x 64 frame pairs (four 32 x 32 waves).  Per
+// chunk of atoms the workgroup needs NF fragments of 1 KiB (4 row and 4 column
+// frame blocks x 3 coordinates x PARTS).  They travel from HBM/L2 straight into
+// an NS-stage LDS ring with global_load_lds_dwordx4 (64 lanes x 16 B = one
+// fragment per instruction, no register staging); each wave issues NF / 4 of
+// them NS - 1 chunks ahead, so every fragment is fetched once per workgroup
+// and the vector memory path carries half the bytes of the register-tile
+// kernel.  One barrier per chunk: after it, the chunk's fragments are in LDS
+// and the stage of the previous chunk is free for the next DMA.
+template <int PREC, int NS>
+__global__ __launch_bounds__(256) void k_pairwise_dma(PairArgs args) {
+  typedef PrecTraits<PREC> T;
+  typedef typename T::acc_t acc_t;
+  constexpr int PARTS = T::PARTS;
+  constexpr int WR = 2, WC = 2;
+  constexpr int NF = 2 * 4 * 3 * PARTS;  // fragments per chunk and workgroup
+  constexpr int LPW = NF / 4;            // DMA instructions per wave and chunk
+  __shared__ u32x4 lds[NS][NF][64];
+
+  const int w = blockIdx.x;
+  const int t = (w % args.nxcd) * args.per_xcd + (w / args.nxcd);
+  if (t >= args.ntiles) return;
+  const int2 tile = args.tiles[t];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int64_t fb0A = (int64_t)tile.x * 4, fb0B = (int64_t)tile.y * 4;
+  const int64_t fbA = fb0A + wr * WR, fbB = fb0B + wc * WC;
+  const u32x4* A = reinterpret_cast<const u32x4*>(args.A);
+  const u32x4* B = reinterpret_cast<const u32x4*>(args.B);
+  const int KB = args.KB;
+
+  // This wave's DMA fragments: f = wave * LPW + i, f = ((side * 4 + blk) * 3 + c) * PARTS + p.
+  const u32x4* src[LPW];
+#pragma unroll
+  for (int i = 0; i < LPW; ++i) {
+    const int f = wave * LPW + i;
+    const int p = f % PARTS, c = (f / PARTS) % 3, blk = (f / (3 * PARTS)) % 4;
+    const int side = f / (12 * PARTS);
+    src[i] = frag_ptr(side ? B : A, (side ? fb0B : fb0A) + blk, 0, KB, c, p, PARTS) + lane;
+  }
+  // Chunk stride of one fragment sequence: 3 * PARTS fragments of 64 lanes.
+  const int64_t kstride = (int64_t)3 * PARTS * 64;
+  auto issue = [&](int kb, int stage) {
+    const int k = kb < KB ? kb : KB - 1;  // past the end: reload the last chunk (never read)
+#pragma unroll
+    for (int i = 0; i < LPW; ++i)
+      __builtin_amdgcn_global_load_lds(
+          (const void*)(src[i] + k * kstride),
+          (__attribute__((address_space(3))) void*)&lds[stage][wave * LPW + i][0], 16, 0, 0);
+  };
+
+  acc_t acc[WR][WC][3][3];
+#pragma unroll
+  for (int r = 0; r < WR; ++r)
+#pragma unroll
+    for (int q = 0; q < WC; ++q)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) acc[r][q][c][d] = acc_t{0, 0, 0, 0};
+
+#pragma unroll
+  for (int s = 0; s < NS - 1; ++s) issue(s, s);
+  int stage = 0;
+  for (int kb = 0; kb < KB; ++kb) {
+    // Own DMA of chunk kb done (NS - 2 later chunks may still be in flight),
+    // then every wave's: a bare s_barrier (a __syncthreads fence would wait for
+    // every DMA in flight).  The asm's memory clobber keeps the compiler from
+    // moving LDS reads across it.  vmcnt in the gfx9 encoding; expcnt and
+    // lgkmcnt not waited.
+    constexpr int VM = (NS - 2) * LPW;
+    __builtin_amdgcn_s_waitcnt((VM & 15) | ((VM >> 4) << 14) | (7 << 4) | (15 << 8));
+    asm volatile("s_barrier" ::: "memory");
+    int nstage = stage + NS - 1;
+    if (nstage >= NS) nstage -= NS;
+    issue(kb + NS - 1, nstage);
+    u32x4 ca[WR][3][PARTS], cb[WC][3][PARTS];
+#pragma unroll
+    for (int r = 0; r < WR; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int p = 0; p < PARTS; ++p)
+          ca[r][c][p] = lds[stage][((0 * 4 + wr * WR + r) * 3 + c) * PARTS + p][lane];
+#pragma unroll
+    for (int q = 0; q < WC; ++q)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int p = 0; p < PARTS; ++p)
+          cb[q][c][p] = lds[stage][((1 * 4 + wc * WC + q) * 3 + c) * PARTS + p][lane];
+#pragma unroll
+    for (int r = 0; r < WR; ++r)
+#pragma unroll
+      for (int q = 0; q < WC; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) mma<PREC>(acc[r][q][c][d], ca[r][c], cb[q][d]);
+    stage = stage + 1 == NS ? 0 : stage + 1;
+  }
+  // Drain the tail DMA before the workgroup's LDS can be reused.
+  __builtin_amdgcn_s_waitcnt((7 << 4) | (15 << 8));
+  pair_epilogue<PREC, WR, WC>(args, acc, fbA, fbB, lane, args.gA, args.uA, 0, args.gB, args.uB, 0);
+}
+
 // ---------------------------------------------------------------------------
 // Launchers.  Tile shapes: FP64 waves own 16x32 pairs (72 FP64 accumulators),
 // the FP32 / BF16 waves 32x32 pairs (144 FP32 accumulators); every workgroup
@@ -573,6 +681,12 @@ template <int PREC, int NS>
 static hipError_t launch_rs(const PairArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((k_pairwise_rs<PREC, 2, 2, 2, 2, NS>), dim3(a.per_xcd * a.nxcd), dim3(256), 0,
                      s, a);
+  return hipGetLastError();
+}
+
+template <int PREC, int NS>
+static hipError_t launch_dma(const PairArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((k_pairwise_dma<PREC, NS>), dim3(a.per_xcd * a.nxcd), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -593,13 +707,18 @@ hipError_t rmsd_launch_pairwise(int prec, const PairArgs& a, hipStream_t s) {
 
 // Variants of the alternative single-matrix kernels, for timing studies and
 // the tuner: 0 register tiles (default), 1 register pipeline (three chunks in
-// flight), 2 LDS-staged operands.
+// flight), 2 LDS-staged operands, 3 LDS-DMA ring (three stages), 4 LDS-DMA
+// ring (two stages).
 hipError_t rmsd_launch_pairwise_variant(int prec, int variant, const PairArgs& a, hipStream_t s) {
   if (variant == 0 || prec == PREC_F64) return rmsd_launch_pairwise(prec, a, s);
   if (variant == 1)
     return prec == PREC_F32 ? launch_rs<PREC_F32, 3>(a, s) : launch_rs<PREC_BF16X3, 3>(a, s);
   if (variant == 2)
     return prec == PREC_F32 ? launch_lds<PREC_F32>(a, s) : launch_lds<PREC_BF16X3>(a, s);
+  if (variant == 3)
+    return prec == PREC_F32 ? launch_dma<PREC_F32, 3>(a, s) : launch_dma<PREC_BF16X3, 3>(a, s);
+  if (variant == 4)
+    return prec == PREC_F32 ? launch_dma<PREC_F32, 2>(a, s) : launch_dma<PREC_BF16X3, 2>(a, s);
   return hipErrorInvalidValue;
 }
 

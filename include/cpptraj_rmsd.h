@@ -212,7 +212,8 @@ int rmsd_batch_set_variant(rmsd_batch* plan, int variant);
  * longest queue.  tune_ms (4 values): the tuner's time per variant, -1 when
  * not tuned. */
 int rmsd_batch_info(const rmsd_batch* plan, int64_t* info, double* tune_ms);
-/* Waits for the plan's work, then frees it. */
+/* Waits for the plan's work, then frees it.  rmsd_ctx_destroy frees every
+ * plan still bound to the context; such a plan must not be used afterwards. */
 void rmsd_batch_destroy(rmsd_batch* plan);
 /* One-shot form: create (no graph, no tuning), run, destroy. */
 int rmsd_pairwise_batch(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mode, int use_mass,

@@ -73,21 +73,38 @@ def test_batch_equals_single_calls(ctx, prec, variant):
     b.close()
 
 
+def maxerr(got, exp):
+    """Largest |got - exp|; NaN matches NaN (the reference's QCP gives NaN for a
+    single atom), NaN against a number counts as infinite."""
+    got = np.asarray(got, np.float64)
+    exp = np.asarray(exp, np.float64)
+    d = np.abs(got - exp)
+    d[np.isnan(got) & np.isnan(exp)] = 0.0
+    d[np.isnan(d)] = np.inf
+    return d.max() if d.size else 0.0
+
+
 @pytest.mark.parametrize("mode", ["fit", "qrmsd", "nofit"])
 def test_batch_fp64_matches_oracle(oracle, ctx, mode):
     sets = make_sets(5)
     b, outs = plan(ctx, sets, mode, True, "fp64", graph=True)
-    b.run()
-    ctx.synchronize()
-    for k, (X, sel, m, Xt, st, mt) in enumerate(sets):
-        F = Xt.shape[0]
-        if F < 2:
-            continue
-        exp = oracle.rms2d(X.reshape(F, -1), sel, m, mode={"fit": 0, "nofit": 1, "qrmsd": 2}[mode],
-                           use_mass=True)
-        got = outs[k][: F * (F - 1) // 2].cpu().numpy()
-        assert np.abs(got - exp).max() <= 1e-6, (mode, k)
-    b.close()
+    try:
+        b.run()
+        ctx.synchronize()
+        for k, (X, sel, m, Xt, st, mt) in enumerate(sets):
+            F = Xt.shape[0]
+            if F < 2:
+                continue
+            exp = oracle.rms2d(X.reshape(F, -1), sel, m,
+                               mode={"fit": 0, "nofit": 1, "qrmsd": 2}[mode], use_mass=True)
+            got = outs[k][: F * (F - 1) // 2].cpu().numpy()
+            nsel = len(sel) if sel is not None else X.shape[1]
+            # one or two atoms: no unique rotation, the distances agree to sqrt's
+            # accuracy near 0 (as for the single-matrix call)
+            tol = 1e-6 if (nsel >= 3 or mode == "nofit") else 1e-4
+            assert maxerr(got, exp) <= tol, (mode, k)
+    finally:
+        b.close()
 
 
 @pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
