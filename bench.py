@@ -70,6 +70,20 @@ def profiled_main_ms(ctx, fn, reps):
     return float(np.mean(pre)), float(np.mean(main))
 
 
+def profiled_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3
+    counter summaries (profiles/*.md lines '<!-- traffic <kernel> <workload>
+    <bytes> -->', FETCH_SIZE + WRITE_SIZE scaled by a copy kernel), or None."""
+    import glob
+    import re
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as fh:
+            for m in re.finditer(r"<!-- traffic (\S+) (\S+) (\d+) -->", fh.read()):
+                if m.group(1) == kernel and m.group(2) == workload:
+                    return int(m.group(3))
+    return None
+
+
 def chunked_err(a, b, chunk=1 << 27):
     import torch
     mx = 0.0
@@ -258,6 +272,45 @@ def grad_bench(ctx, steps=5, warmup=2):
     return out
 
 
+def host_stream_bench(ctx, F=200000, n=3000):
+    """The C++ host layer's 'rms' over a host-resident COORDS set (pinned FP32
+    AoS, as cpptraj keeps frames): frames stream to the device in 256 MB
+    chunks on two transfer slots, overlapped with the fit.  PCIe-inclusive
+    rate, beside the same fit on device-resident frames."""
+    import torch
+    import cpptraj_amd as ca
+    from cpptraj_amd import synth
+    from cpptraj_amd.host import HostSession
+
+    X, _ = synth.noisy_soa_torch(5, F, n, 1.0, ld=n)
+    aos = X.permute(0, 2, 1).contiguous()  # (F, n, 3)
+    del X
+    d = ca.frames_desc(aos)
+    rmsd = torch.empty(F, dtype=torch.float64, device="cuda")
+    dev_ms = timed_steps(lambda: ctx.fit_desc(d, d, 0, True, False, 0, rmsd), 3, 1)
+    host = aos.cpu().numpy().reshape(F, -1)
+    del aos
+    torch.cuda.empty_cache()
+    hs = HostSession(0)
+    hs.add_coords("big", host, n)
+    del host
+    hs.run("rms W first * nomod crdset big")  # warm-up: buffers, contexts
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        hs.run("rms W first * nomod crdset big")
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    got = hs.series("W")[-F:]
+    hs.close()
+    ref = rmsd.cpu().numpy()
+    return {"workload": f"{F} frames x {n} atoms, host COORDS set (pinned FP32 AoS), "
+                        "rms first nomod through the C++ host layer",
+            "host_ms": ms, "host_frames_per_s": F / (ms / 1e3),
+            "pcie_inclusive_gbs": F * 12.0 * n / (ms / 1e3) / 1e9,
+            "device_resident_ms": dev_ms,
+            "max_diff_vs_device_resident": float(np.abs(got - ref).max())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -274,6 +327,7 @@ def main():
     ap.add_argument("--fit-atoms", type=int, default=3000)
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-grad", action="store_true")
+    ap.add_argument("--skip-host", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -343,7 +397,8 @@ def main():
                                  "fp64": "FP64 MFMA, FP64 QCP"}[head]},
         "kernels": res,
         "roofline": {"bound": "mfma", "achieved": algo_tflops, "peak": peak,
-                     "unit": "TFLOP/s", "frac": algo_tflops / peak, "traffic": None,
+                     "unit": "TFLOP/s", "frac": algo_tflops / peak,
+                     "traffic": profiled_traffic(f"k_pairwise<{head}>", f"{F}x{n}"),
                      "kernel": f"k_pairwise<{head}>",
                      "algorithmic_flops_per_launch": npairs * flops_pair,
                      "executed_mfma_tflops": algo_tflops * terms},
@@ -389,6 +444,9 @@ def main():
     if not args.skip_grad:
         log("gradients")
         line["grad"] = grad_bench(ctx, args.steps, args.warmup)
+    if not args.skip_host:
+        log("host-resident fit stream")
+        line["host_stream"] = host_stream_bench(ctx)
     ctx.close()
     print(json.dumps(line), flush=True)
 

@@ -515,6 +515,33 @@ int rmsd_fit_to_ref(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames*
   return RMSD_OK;
 }
 
+int rmsd_transform(rmsd_ctx* ctx, const rmsd_frames* frames, const int32_t* atoms, int natoms,
+                   const double* U, const double* tvec, int mode) {
+  if (!ctx) return RMSD_ERR_ARG;
+  int st;
+  if ((st = check_frames(ctx, frames, "frames")) != RMSD_OK) return st;
+  if (mode != RMSD_MODIFY_ROT_TRANS && mode != RMSD_MODIFY_TRANS)
+    return fail(ctx, RMSD_ERR_ARG, "transform mode must be ROT_TRANS or TRANS");
+  if (!tvec || (mode == RMSD_MODIFY_ROT_TRANS && !U))
+    return fail(ctx, RMSD_ERR_ARG, "transform needs tvec (and U for ROT_TRANS)");
+  if (atoms && natoms < 0) return fail(ctx, RMSD_ERR_ARG, "natoms < 0");
+  if (frames->nframes == 0 || (atoms && natoms == 0)) return RMSD_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  ApplyArgs ap;
+  ap.xyz = const_cast<float*>(frames->xyz);
+  ap.stride = frames->stride;
+  ap.ld = frames->ld;
+  ap.natom = frames->natom;
+  ap.nframes = frames->nframes;
+  ap.xform = nullptr;
+  ap.atoms = atoms;
+  ap.natoms = natoms;
+  ap.U = mode == RMSD_MODIFY_ROT_TRANS ? U : nullptr;
+  ap.tvec = tvec;
+  HIP_TRY(ctx, rmsd_launch_apply(ap, mode == RMSD_MODIFY_ROT_TRANS ? 1 : 2, ctx->stream));
+  return RMSD_OK;
+}
+
 int rmsd_fit_backward(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* ref,
                       int ref_frame, int use_mass, const double* grad_rmsd, double* grad_xyz) {
   if (!ctx) return RMSD_ERR_ARG;

@@ -355,12 +355,23 @@ __global__ __launch_bounds__(FIT_THREADS) void k_nofit_frames(NofitArgs a, int l
 
 // ---- coordinate update -----------------------------------------------------------
 // mode 1: x' = U (x + t1) + t2 (Frame::Trans_Rot_Trans); mode 2: x' = x + (t1 + t2).
+// The transform comes from xform (15 per frame) or from U (9) + tvec (6); the
+// atoms are all natom or the list atoms[0..natoms-1] (Trans_Rot_Trans with an
+// AtomMask, as Action_Align's 'move' uses it).
 __global__ __launch_bounds__(FIT_THREADS) void k_apply(ApplyArgs a, int mode) {
 #pragma clang fp contract(off)
   const int64_t f = blockIdx.x;
-  const double* X = a.xform + 15 * f;
+  double X[15];
+  if (a.xform) {
+    for (int k = 0; k < 15; ++k) X[k] = a.xform[15 * f + k];
+  } else {
+    for (int k = 0; k < 9; ++k) X[k] = a.U ? a.U[9 * f + k] : (k % 4 == 0 ? 1.0 : 0.0);
+    for (int k = 0; k < 6; ++k) X[9 + k] = a.tvec[6 * f + k];
+  }
   float* fr = a.xyz + f * a.stride;  // all stored frames, in order
-  for (int k = threadIdx.x; k < a.natom; k += blockDim.x) {
+  const int nmove = a.atoms ? a.natoms : a.natom;
+  for (int i = threadIdx.x; i < nmove; i += blockDim.x) {
+    const int k = a.atoms ? a.atoms[i] : i;
     float* px = a.ld ? fr + k : fr + 3 * k;
     float* py = a.ld ? fr + a.ld + k : fr + 3 * k + 1;
     float* pz = a.ld ? fr + 2 * a.ld + k : fr + 3 * k + 2;

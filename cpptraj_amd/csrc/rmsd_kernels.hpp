@@ -126,6 +126,10 @@ struct ApplyArgs {
   int natom;
   int nframes;
   const double* xform;    // 15 per frame: U, t1, t2 with x' = U (x + t1) + t2
+  const int32_t* atoms = nullptr;  // atoms to move (null: all natom)
+  int natoms = 0;
+  const double* U = nullptr;       // instead of xform: U (9 per frame) ...
+  const double* tvec = nullptr;    // ... and t1, t2 (6 per frame)
 };
 
 hipError_t rmsd_launch_setup(const SetupArgs& a, hipStream_t s);

@@ -3,7 +3,8 @@
 // The reference's command classes for this path, with the reference's names,
 // options and outputs, built on the C ABI of libcpptraj_rmsd.so:
 //   Analysis_Rms2d   rms2d / 2drms            (src/Analysis_Rms2d.cpp)
-//   Action_Rmsd      rms / rmsd               (src/Action_Rmsd.cpp, src/ReferenceAction.cpp)
+//   Action_Rmsd      rms / rmsd, align        (src/Action_Rmsd.cpp, src/Action_Align.cpp,
+//                                              src/ReferenceAction.cpp)
 //   Metric_RMS       cluster pairwise cache   (src/Cluster/Metric_RMS.cpp,
 //                                              src/Cluster/MetricArray.cpp:722-801)
 // Each owns its loop nest and calls the library once per matrix, batch of
@@ -50,6 +51,26 @@ class ArgList {
   std::vector<bool> marked_;
 };
 
+// Page-locked host memory (hipHostMalloc): COORDS sets live in it so that
+// their copies to and from the device run asynchronously at full link speed.
+void* pinned_alloc(size_t bytes);
+void pinned_free(void* p) noexcept;
+template <class T>
+struct PinnedAllocator {
+  using value_type = T;
+  PinnedAllocator() = default;
+  template <class U>
+  PinnedAllocator(const PinnedAllocator<U>&) {}
+  T* allocate(size_t n) { return static_cast<T*>(pinned_alloc(n * sizeof(T))); }
+  void deallocate(T* p, size_t) noexcept { pinned_free(p); }
+  template <class U>
+  bool operator==(const PinnedAllocator<U>&) const { return true; }
+  template <class U>
+  bool operator!=(const PinnedAllocator<U>&) const { return false; }
+};
+using PinnedFloats = std::vector<float, PinnedAllocator<float>>;
+using PinnedDoubles = std::vector<double, PinnedAllocator<double>>;
+
 // Topology facts the path reads: atom count and masses.
 struct Topology {
   int natom = 0;
@@ -62,7 +83,7 @@ struct CoordsSet {
   std::string name;
   Topology top;
   int64_t stride = 0;
-  std::vector<float> xyz;
+  PinnedFloats xyz;
   int nframes() const { return stride ? (int)(xyz.size() / (size_t)stride) : 0; }
   const float* frame(int f) const { return xyz.data() + (int64_t)f * stride; }
   float* frame(int f) { return xyz.data() + (int64_t)f * stride; }
@@ -156,6 +177,9 @@ class GpuSession {
   // Records an error from a library return code.
   int Check(int rc, const char* what);
   void SetError(const std::string& e) { err_ = e; }
+  // Transfer slot k (0 or 1): a non-blocking stream with its own library
+  // context, for double-buffered host <-> device streaming.
+  int Slot(int k, rmsd_ctx** ctx, void** stream);
 
  private:
   struct Buf {
@@ -164,6 +188,8 @@ class GpuSession {
   };
   int device_ = 0;
   rmsd_ctx* ctx_ = nullptr;
+  rmsd_ctx* slot_ctx_[2] = {nullptr, nullptr};
+  void* slot_stream_[2] = {nullptr, nullptr};
   std::map<std::string, Buf> bufs_;
   std::string err_;
 };
@@ -201,6 +227,9 @@ class Action_Rmsd {
   enum TvecType { NO_TVEC = 0, COMBINED, SEPARATE };
   enum RefModeType { FIRST = 0, PREVIOUS, TRAJ, FRAME };
   RetType Init(ArgList& args, DataSetList& dsl, MaskResolver resolve, GpuSession& gpu);
+  // 'align' (src/Action_Align.cpp): the same fit, always applied, to every atom
+  // or to the atoms of 'move <mask>' only.
+  RetType InitAlign(ArgList& args, DataSetList& dsl, MaskResolver resolve, GpuSession& gpu);
   // Runs the action over every frame of `crd` (rewriting its coordinates
   // unless 'nofit' or 'nomod').  Frames go to the library in batches when the
   // reference is fixed (first / ref), one at a time otherwise.
@@ -209,6 +238,7 @@ class Action_Rmsd {
   const Series* Rmsd() const { return rmsd_; }
 
  private:
+  RetType InitRef(ArgList& args, DataSetList& dsl, GpuSession& gpu);
   RetType RunBatch(CoordsSet& crd, const AtomMask& tsel, int f0, int nf, const float* refxyz,
                    const Topology& reftop, const std::string& refname);
   bool fit_ = true, useMass_ = false;
@@ -216,7 +246,8 @@ class Action_Rmsd {
   TvecType tvecType_ = NO_TVEC;
   RefModeType refMode_ = FIRST;
   int precision_ = RMSD_PREC_FP64;
-  std::string outfile_, matricesout_, vecsout_, tgtMaskExpr_, refMaskExpr_;
+  int chunkFrames_ = 0;
+  std::string outfile_, matricesout_, vecsout_, tgtMaskExpr_, refMaskExpr_, moveMaskExpr_;
   CoordsSet* refCrd_ = nullptr;
   Series* rmsd_ = nullptr;
   Series* rmatrices_ = nullptr;

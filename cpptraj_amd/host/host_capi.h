@@ -4,7 +4,7 @@
  * Python tests).  A session holds named COORDS sets, reference frames, a mask
  * table (expression -> resolved selection) and the data sets that commands
  * produce; chost_run executes one command line of the reference's syntax
- * ("rms2d ...", "2drms ...", "rms ...", "rmsd ...", "cluster ...").
+ * ("rms2d ...", "2drms ...", "rms ...", "rmsd ...", "align ...", "cluster ...").
  */
 #ifndef CPPTRAJ_RMSD_HOST_CAPI_H
 #define CPPTRAJ_RMSD_HOST_CAPI_H

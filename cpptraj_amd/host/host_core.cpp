@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <new>
 #include <sstream>
 
 #include "host.hpp"
@@ -303,6 +304,18 @@ int Read(const std::string& path, MatrixFlt* tri, size_t* actual_nrows, int* sie
 
 }  // namespace cmatrix
 
+// ---- pinned host memory ---------------------------------------------------------------
+void* pinned_alloc(size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  void* p = nullptr;
+  if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess || !p) throw std::bad_alloc();
+  return p;
+}
+
+void pinned_free(void* p) noexcept {
+  if (p) (void)hipHostFree(p);
+}
+
 // ---- GpuSession -------------------------------------------------------------------
 GpuSession::GpuSession(int device) : device_(device) {
   if (rmsd_ctx_create(&ctx_, device, nullptr) != RMSD_OK) {
@@ -312,6 +325,13 @@ GpuSession::GpuSession(int device) : device_(device) {
 }
 
 GpuSession::~GpuSession() {
+  for (int k = 0; k < 2; ++k) {
+    if (slot_ctx_[k]) {
+      rmsd_ctx_synchronize(slot_ctx_[k]);
+      rmsd_ctx_destroy(slot_ctx_[k]);
+    }
+    if (slot_stream_[k]) (void)hipStreamDestroy((hipStream_t)slot_stream_[k]);
+  }
   if (ctx_) {
     rmsd_ctx_synchronize(ctx_);
     rmsd_ctx_destroy(ctx_);
@@ -366,6 +386,28 @@ int GpuSession::Download(void* dst, const void* src, size_t bytes) {
 int GpuSession::Sync() {
   if (!ctx_) return 1;
   return Check(rmsd_ctx_synchronize(ctx_), "synchronize");
+}
+
+int GpuSession::Slot(int k, rmsd_ctx** ctx, void** stream) {
+  if (k < 0 || k > 1) return 1;
+  if (!slot_ctx_[k]) {
+    (void)hipSetDevice(device_);
+    hipStream_t st = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+      err_ = "cannot create a transfer stream";
+      return 1;
+    }
+    if (rmsd_ctx_create(&slot_ctx_[k], device_, st) != RMSD_OK) {
+      (void)hipStreamDestroy(st);
+      slot_ctx_[k] = nullptr;
+      err_ = "cannot create a transfer context";
+      return 1;
+    }
+    slot_stream_[k] = st;
+  }
+  *ctx = slot_ctx_[k];
+  *stream = slot_stream_[k];
+  return 0;
 }
 
 int GpuSession::Check(int rc, const char* what) {

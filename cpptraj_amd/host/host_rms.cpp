@@ -1,5 +1,6 @@
 // host_rms.cpp -- Analysis_Rms2d, Action_Rmsd and Metric_RMS over the C ABI,
 // and the command dispatch of a Session.
+#include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -191,6 +192,8 @@ RetType Action_Rmsd::Init(ArgList& args, DataSetList& dsl, MaskResolver resolve,
     return ERR;
   }
   outfile_ = args.GetStringKey("out");
+  // GPU path option: frames per streamed chunk (default: 256 MB of frames).
+  chunkFrames_ = args.getKeyInt("chunk", 0);
   const bool saveMatrices = args.hasKey("savematrices");
   if (saveMatrices) matricesout_ = args.GetStringKey("matricesout");
   const std::string sv = args.GetStringKey("savevectors");
@@ -202,38 +205,7 @@ RetType Action_Rmsd::Init(ArgList& args, DataSetList& dsl, MaskResolver resolve,
     return ERR;
   }
   if (tvecType_ != NO_TVEC) vecsout_ = args.GetStringKey("vecsout");
-  // Reference modes.
-  refMode_ = FIRST;
-  if (args.hasKey("previous")) {
-    refMode_ = PREVIOUS;
-  } else if (!args.hasKey("first")) {
-    if (args.Contains("reftraj")) {
-      const std::string rt = args.GetStringKey("reftraj");
-      refCrd_ = dsl.FindCoordsSet(rt);
-      if (!refCrd_) {
-        gpu.SetError("reference trajectory '" + rt + "' is not a loaded COORDS set");
-        return ERR;
-      }
-      refMode_ = TRAJ;
-    } else {
-      std::string rn;
-      bool want = false;
-      if (args.Contains("ref")) {
-        rn = args.GetStringKey("ref");
-        want = true;
-      } else if (args.hasKey("reference")) {
-        want = true;
-      }
-      if (want) {
-        refCrd_ = dsl.FindReference(rn);
-        if (!refCrd_) {
-          gpu.SetError("reference '" + rn + "' not found");
-          return ERR;
-        }
-        refMode_ = FRAME;
-      }
-    }
-  }
+  if (InitRef(args, dsl, gpu) != OK) return ERR;
   if (args.hasKey("perres")) {
     gpu.SetError("rms: 'perres' is not on the GPU RMSD path");
     return ERR;
@@ -269,8 +241,51 @@ RetType Action_Rmsd::Init(ArgList& args, DataSetList& dsl, MaskResolver resolve,
   return OK;
 }
 
+// ReferenceAction::InitRef (src/ReferenceAction.cpp:72-130): first (default),
+// previous, reftraj <COORDS set>, ref <name> / reference.
+RetType Action_Rmsd::InitRef(ArgList& args, DataSetList& dsl, GpuSession& gpu) {
+  refMode_ = FIRST;
+  if (args.hasKey("previous")) {
+    refMode_ = PREVIOUS;
+  } else if (!args.hasKey("first")) {
+    if (args.Contains("reftraj")) {
+      const std::string rt = args.GetStringKey("reftraj");
+      refCrd_ = dsl.FindCoordsSet(rt);
+      if (!refCrd_) {
+        gpu.SetError("reference trajectory '" + rt + "' is not a loaded COORDS set");
+        return ERR;
+      }
+      refMode_ = TRAJ;
+    } else {
+      std::string rn;
+      bool want = false;
+      if (args.Contains("ref")) {
+        rn = args.GetStringKey("ref");
+        want = true;
+      } else if (args.hasKey("reference")) {
+        want = true;
+      }
+      if (want) {
+        refCrd_ = dsl.FindReference(rn);
+        if (!refCrd_) {
+          gpu.SetError("reference '" + rn + "' not found");
+          return ERR;
+        }
+        refMode_ = FRAME;
+      }
+    }
+  }
+  return OK;
+}
+
 // Frames f0 .. f0+nf-1 of crd against one reference frame (refxyz, whose
-// selected atoms are the reference mask of reftop): one library call.
+// selected atoms are the reference mask of reftop).  The frames stream
+// through the device in chunks of at most kStreamBytes on two transfer slots
+// (stream + library context each): while one chunk is fitted, the next one
+// is uploaded and the previous one's results (and rewritten coordinates) come
+// back.  One library call per chunk.
+static const size_t kStreamBytes = size_t(256) << 20;
+
 RetType Action_Rmsd::RunBatch(CoordsSet& crd, const AtomMask& tsel, int f0, int nf,
                               const float* refxyz, const Topology& reftop,
                               const std::string& refname) {
@@ -289,49 +304,155 @@ RetType Action_Rmsd::RunBatch(CoordsSet& crd, const AtomMask& tsel, int f0, int 
     g.SetError("Number of atoms in target mask does not equal number of atoms in reference mask");
     return ERR;
   }
-  rmsd_frames F, R;
-  if (g.FramesOnDevice("rms.tgt", crd, tsel, &F, crd.frame(f0), nf)) return ERR;
+  // Reference, target selection and masses: uploaded once (synchronously).
+  rmsd_frames R, T0;
   if (g.FramesOnDevice("rms.ref", refset, rsel, &R)) return ERR;
-  double* drmsd = (double*)g.Buffer("rms.rmsd", sizeof(double) * nf);
-  double* dU = rmatrices_ ? (double*)g.Buffer("rms.U", sizeof(double) * 9 * nf) : nullptr;
-  double* dT = tvecs_ ? (double*)g.Buffer("rms.T", sizeof(double) * 6 * nf) : nullptr;
-  if (!drmsd || (rmatrices_ && !dU) || (tvecs_ && !dT)) return ERR;
+  if (g.FramesOnDevice("rms.tsel", crd, tsel, &T0, crd.frame(f0), 0)) return ERR;
   const int modify = !fit_ ? RMSD_MODIFY_NONE
                            : (mode_ == ROT_AND_TRANS ? RMSD_MODIFY_ROT_TRANS
                                                      : (mode_ == TRANS_ONLY ? RMSD_MODIFY_TRANS
                                                                             : RMSD_MODIFY_NONE));
-  if (g.Check(rmsd_fit_to_ref(g.ctx(), &F, &R, 0, fit_, useMass_, modify, drmsd, dU, dT), "rms"))
-    return ERR;
-  const size_t base = rmsd_->v.size();
-  rmsd_->v.resize(base + nf);
-  if (g.Download(rmsd_->v.data() + base, drmsd, sizeof(double) * nf)) return ERR;
-  if (rmatrices_) {
-    const size_t b = rmatrices_->v.size();
-    rmatrices_->v.resize(b + 9 * (size_t)nf);
-    if (g.Download(rmatrices_->v.data() + b, dU, sizeof(double) * 9 * nf)) return ERR;
+  // 'align ... move <mask>': fit without update, then move the listed atoms.
+  const bool moving = !moveMaskExpr_.empty();
+  int32_t* dmove = nullptr;
+  int nmove = 0;
+  if (moving) {
+    AtomMask mv;
+    if (!resolve_or_all(resolve_, moveMaskExpr_, crd, &mv) || mv.empty()) {
+      g.SetError("No atoms selected for move mask [" + moveMaskExpr_ + "]");
+      return ERR;
+    }
+    for (int a : mv)
+      if (a < 0 || a >= crd.top.natom) {
+        g.SetError("move mask atom out of range");
+        return ERR;
+      }
+    std::vector<int32_t> m32(mv.begin(), mv.end());
+    nmove = (int)m32.size();
+    dmove = (int32_t*)g.Buffer("rms.move", sizeof(int32_t) * nmove);
+    if (!dmove || g.Upload(dmove, m32.data(), sizeof(int32_t) * nmove)) return ERR;
   }
-  if (tvecs_) {
-    std::vector<double> tv(6 * (size_t)nf);
-    if (g.Download(tv.data(), dT, sizeof(double) * tv.size())) return ERR;
-    const size_t b = tvecs_->v.size();
-    tvecs_->v.resize(b + 6 * (size_t)nf);
-    for (int f = 0; f < nf; ++f)
-      for (int c = 0; c < 3; ++c) {
-        double* o = tvecs_->v.data() + b + 6 * (size_t)f;
-        const double t = tv[6 * f + c], r = tv[6 * f + 3 + c];
-        if (tvecType_ == COMBINED) {
-          // Vxyz = Trans + refTrans; the origin columns stay 0.
-          o[c] = t + r;
-          o[3 + c] = 0.0;
-        } else {
-          o[c] = t;
-          o[3 + c] = r;
+  const size_t fbytes = sizeof(float) * (size_t)crd.stride;
+  int chunk = (int)std::max<size_t>(1, std::min<size_t>(nf, kStreamBytes / fbytes));
+  if (chunkFrames_ > 0) chunk = std::min(chunk, chunkFrames_);
+  const int nchunks = (nf + chunk - 1) / chunk;
+  const size_t base_r = rmsd_->v.size();
+  rmsd_->v.resize(base_r + nf);
+  const size_t base_u = rmatrices_ ? rmatrices_->v.size() : 0;
+  if (rmatrices_) rmatrices_->v.resize(base_u + 9 * (size_t)nf);
+  const size_t base_t = tvecs_ ? tvecs_->v.size() : 0;
+  if (tvecs_) tvecs_->v.resize(base_t + 6 * (size_t)nf);
+
+  struct SlotBufs {
+    rmsd_ctx* ctx = nullptr;
+    hipStream_t st = nullptr;
+    float* dx = nullptr;
+    double *dr = nullptr, *dU = nullptr, *dT = nullptr;
+    PinnedDoubles hr, hU, hT;
+    int c = -1;  // chunk in flight
+  } sl[2];
+  for (int k = 0; k < 2 && k < nchunks; ++k) {
+    void* st = nullptr;
+    if (g.Slot(k, &sl[k].ctx, &st)) return ERR;
+    sl[k].st = (hipStream_t)st;
+    const std::string t = "rms.slot" + std::to_string(k);
+    sl[k].dx = (float*)g.Buffer(t + ".xyz", fbytes * chunk);
+    sl[k].dr = (double*)g.Buffer(t + ".rmsd", sizeof(double) * chunk);
+    if (rmatrices_ || moving) sl[k].dU = (double*)g.Buffer(t + ".U", sizeof(double) * 9 * chunk);
+    if (tvecs_ || moving) sl[k].dT = (double*)g.Buffer(t + ".T", sizeof(double) * 6 * chunk);
+    if (!sl[k].dx || !sl[k].dr || ((rmatrices_ || moving) && !sl[k].dU) ||
+        ((tvecs_ || moving) && !sl[k].dT))
+      return ERR;
+    sl[k].hr.resize(chunk);
+    if (rmatrices_) sl[k].hU.resize(9 * (size_t)chunk);
+    if (tvecs_) sl[k].hT.resize(6 * (size_t)chunk);
+  }
+  auto hip_ok = [&](hipError_t e, const char* what) {
+    if (e == hipSuccess) return true;
+    g.SetError(std::string(what) + ": " + hipGetErrorString(e));
+    return false;
+  };
+  // Waits for a slot's chunk and moves its results into the data sets.
+  auto harvest = [&](SlotBufs& b) -> bool {
+    if (b.c < 0) return true;
+    if (!hip_ok(hipStreamSynchronize(b.st), "rms stream")) return false;
+    const int c0 = b.c * chunk, n = std::min(chunk, nf - c0);
+    std::copy(b.hr.begin(), b.hr.begin() + n, rmsd_->v.begin() + base_r + c0);
+    if (rmatrices_)
+      std::copy(b.hU.begin(), b.hU.begin() + 9 * (size_t)n,
+                rmatrices_->v.begin() + base_u + 9 * (size_t)c0);
+    if (tvecs_)
+      for (int f = 0; f < n; ++f) {
+        double* o = tvecs_->v.data() + base_t + 6 * (size_t)(c0 + f);
+        for (int c = 0; c < 3; ++c) {
+          const double tr = b.hT[6 * f + c], rf = b.hT[6 * f + 3 + c];
+          if (tvecType_ == COMBINED) {
+            // Vxyz = Trans + refTrans; the origin columns stay 0.
+            o[c] = tr + rf;
+            o[3 + c] = 0.0;
+          } else {
+            o[c] = tr;
+            o[3 + c] = rf;
+          }
         }
       }
+    b.c = -1;
+    return true;
+  };
+  for (int c = 0; c < nchunks; ++c) {
+    SlotBufs& b = sl[c & 1];
+    if (!harvest(b)) return ERR;
+    const int c0 = c * chunk, n = std::min(chunk, nf - c0);
+    float* host = crd.frame(f0 + c0);
+    if (!hip_ok(hipMemcpyAsync(b.dx, host, fbytes * n, hipMemcpyHostToDevice, b.st), "upload"))
+      return ERR;
+    rmsd_frames F = T0;
+    F.xyz = b.dx;
+    F.nframes = n;
+    if (g.Check(rmsd_fit_to_ref(b.ctx, &F, &R, 0, fit_, useMass_,
+                                moving ? RMSD_MODIFY_NONE : modify, b.dr, b.dU, b.dT),
+                "rms"))
+      return ERR;
+    if (moving && g.Check(rmsd_transform(b.ctx, &F, dmove, nmove, b.dU, b.dT,
+                                         RMSD_MODIFY_ROT_TRANS),
+                          "align"))
+      return ERR;
+    if (!hip_ok(hipMemcpyAsync(b.hr.data(), b.dr, sizeof(double) * n, hipMemcpyDeviceToHost, b.st),
+                "download") ||
+        (rmatrices_ && !hip_ok(hipMemcpyAsync(b.hU.data(), b.dU, sizeof(double) * 9 * n,
+                                              hipMemcpyDeviceToHost, b.st),
+                               "download")) ||
+        (tvecs_ && !hip_ok(hipMemcpyAsync(b.hT.data(), b.dT, sizeof(double) * 6 * n,
+                                          hipMemcpyDeviceToHost, b.st),
+                           "download")) ||
+        ((modify != RMSD_MODIFY_NONE || moving) &&
+         !hip_ok(hipMemcpyAsync(host, b.dx, fbytes * n, hipMemcpyDeviceToHost, b.st), "download")))
+      return ERR;
+    b.c = c;
   }
-  if (modify != RMSD_MODIFY_NONE &&
-      g.Download(crd.frame(f0), F.xyz, sizeof(float) * (size_t)crd.stride * nf))
-    return ERR;
+  return harvest(sl[0]) && harvest(sl[1]) ? OK : ERR;
+}
+
+// Action_Align::Init (src/Action_Align.cpp): 'align <mask> [<refmask>]
+// [move <mask>] [mass]' plus the reference keywords; always fits and moves
+// (all atoms, or those of the move mask).  Its RMSD values are kept in a data
+// set as a by-product.
+RetType Action_Rmsd::InitAlign(ArgList& args, DataSetList& dsl, MaskResolver resolve,
+                               GpuSession& gpu) {
+  gpu_ = &gpu;
+  resolve_ = resolve;
+  fit_ = true;
+  mode_ = ROT_AND_TRANS;
+  useMass_ = args.hasKey("mass");
+  chunkFrames_ = args.getKeyInt("chunk", 0);
+  if (InitRef(args, dsl, gpu) != OK) return ERR;
+  moveMaskExpr_ = args.GetStringKey("move");
+  tgtMaskExpr_ = args.GetMaskNext();
+  refMaskExpr_ = args.GetMaskNext();
+  if (refMaskExpr_.empty()) refMaskExpr_ = tgtMaskExpr_;
+  std::string name = args.GetStringNext();
+  if (name.empty()) name = dsl.GenerateDefaultName("Align");
+  rmsd_ = dsl.AddSeries(name);
   return OK;
 }
 
@@ -511,6 +632,15 @@ int Session::Run(const std::string& line) {
       if (!crd) gpu.SetError("rms: no COORDS set");
       else if (a.DoFrames(*crd) == OK && a.WriteOutput() == 0) rc = 0;
       else if (gpu.error().empty()) gpu.SetError("rms: cannot write output");
+    }
+  } else if (cmd == "align") {
+    Action_Rmsd a;
+    const std::string crdset = args.GetStringKey("crdset");
+    rc = 1;
+    if (a.InitAlign(args, dsl, resolve, gpu) == OK) {
+      CoordsSet* crd = dsl.FindCoordsSet(crdset);
+      if (!crd) gpu.SetError("align: no COORDS set");
+      else if (a.DoFrames(*crd) == OK) rc = 0;
     }
   } else if (cmd == "cluster") {
     Metric_RMS m;

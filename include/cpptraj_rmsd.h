@@ -159,6 +159,18 @@ int rmsd_fit_to_ref(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames*
                     int ref_frame, int fit, int use_mass, int modify, double* rmsd, double* U,
                     double* tvec);
 
+/* Per-frame rigid update of selected atoms.
+ * Replaces Frame::Trans_Rot_Trans(AtomMask, ...) as Action_Align::DoAction
+ * uses it for 'align ... move <mask>' (src/Action_Align.cpp): for every stored
+ * frame f of `frames` (xyz rewritten in place; fidx, sel and mass ignored),
+ * atoms atoms[0..natoms-1] (every natom atom when atoms is NULL) become
+ *   RMSD_MODIFY_ROT_TRANS  x' = U_f (x + t_f) + r_f
+ *   RMSD_MODIFY_TRANS      x' = x + (t_f + r_f)
+ * with U (9 doubles per frame) and tvec (6 per frame: t = Trans, r = refTrans)
+ * as rmsd_fit_to_ref returns them. */
+int rmsd_transform(rmsd_ctx* ctx, const rmsd_frames* frames, const int32_t* atoms, int natoms,
+                   const double* U, const double* tvec, int mode);
+
 /* Gradient of the fit: no counterpart in the reference (cpptraj has no RMSD
  * derivative).  For loss = sum_f grad_rmsd[f] * RMSD_f of rmsd_fit_to_ref
  * (fit, same frames / reference / masses), writes d loss / d x for the selected

@@ -172,3 +172,44 @@ def test_cluster_cache_cmatrix(hs, golden, oracle, tmp_path):
     assert np.frombuffer(raw[4:28], dtype="<u8")[:2].tolist() == [10, 4]
     assert np.frombuffer(raw[20:28], dtype="<i8")[0] == 3
     assert raw[28 + 4 * 6:] == b"FTTFTTFTTF"
+
+
+@pytest.mark.parametrize("extra", ["", "mass savematrices savevectors separate", "norotate"])
+def test_rms_streamed_chunks_match_one_batch(hs, extra):
+    """The host layer's double-buffered streaming (chunks on two transfer
+    slots) gives the same RMSD, matrices, vectors and coordinates as a single
+    chunk: 1000 frames x 300 atoms in chunks of 64 frames (16 chunks, odd tail)."""
+    from cpptraj_amd import synth
+    X = synth.noisy(17, 1000, 300, 1.0).reshape(1000, -1)
+    m = synth.masses(np.random.default_rng(17), 300)
+    hs.add_coords("a", X, 300, m)
+    hs.add_coords("b", X.copy(), 300, m)
+    hs.add_mask(":1-20", np.arange(0, 300, 2))
+    hs.run(f"rms A first :1-20 {extra} crdset a")
+    hs.run(f"rms B first :1-20 {extra} chunk 64 crdset b")
+    assert np.array_equal(hs.series("A"), hs.series("B"))
+    if "savematrices" in extra:
+        assert np.array_equal(hs.series("A[RM]"), hs.series("B[RM]"))
+        assert np.array_equal(hs.series("A[TV]"), hs.series("B[TV]"))
+    assert np.array_equal(hs.coords("a"), hs.coords("b"))
+
+
+def test_align_move_mask(hs):
+    """'align <mask> move <mask>' (Action_Align): the moved atoms get exactly the
+    coordinates 'rms' gives them, the others stay where they were; without
+    'move' align equals rms on every atom."""
+    from cpptraj_amd import synth
+    X = synth.noisy(23, 50, 120, 1.0).reshape(50, -1)
+    for name in ("r", "a", "m"):
+        hs.add_coords(name, X.copy(), 120)
+    hs.add_mask(":1-5", np.arange(0, 60))
+    hs.add_mask(":6-10", np.arange(60, 120))
+    hs.run("rms R first :1-5 crdset r")
+    hs.run("align :1-5 crdset a")
+    hs.run("align :1-5 move :6-10 chunk 16 crdset m")
+    r = hs.coords("r").reshape(50, 120, 3)
+    a = hs.coords("a").reshape(50, 120, 3)
+    m = hs.coords("m").reshape(50, 120, 3)
+    assert np.array_equal(a, r)
+    assert np.array_equal(m[:, 60:], r[:, 60:])
+    assert np.array_equal(m[:, :60], X.reshape(50, 120, 3)[:, :60])
