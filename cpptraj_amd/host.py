@@ -49,6 +49,12 @@ def load_host():
     L.chost_get_series.argtypes = [P, C.c_char_p, DP, C.POINTER(I64), IP]
     L.chost_get_coords.argtypes = [P, C.c_char_p, FP, C.POINTER(I64)]
     L.chost_read_cmatrix.argtypes = [C.c_char_p, FP, C.POINTER(I64), C.POINTER(I64), IP, IP]
+    L.chost_write_matrix.argtypes = [C.c_char_p, FP, I64, I64, I, I, C.c_char_p]
+    L.chost_write_series.argtypes = [C.c_char_p, I, C.POINTER(C.c_char_p), IP, IP, IP,
+                                     C.POINTER(DP), I64]
+    L.chost_write_cmatrix.argtypes = [C.c_char_p, FP, I64, I64, I, IP]
+    for f in ("chost_write_matrix", "chost_write_series", "chost_write_cmatrix"):
+        getattr(L, f).restype = I
     for f in ("chost_create", "chost_add_coords", "chost_add_reference", "chost_add_mask",
               "chost_run", "chost_get_matrix", "chost_get_series", "chost_get_coords",
               "chost_read_cmatrix"):
@@ -152,3 +158,36 @@ def read_cmatrix(path):
                             fr.ctypes.data_as(C.POINTER(C.c_int))):
         raise RmsdError(f"cannot read {path}")
     return tri[: n * (n - 1) // 2], an.value, sv.value, fr[:n]
+
+
+def write_matrix(path, data, ncols, nrows=None, tri=True, square2d=True, name=""):
+    """DataIO_Std text of a float matrix (host layer writer, no GPU needed)."""
+    L = load_host()
+    a = np.ascontiguousarray(data, dtype=np.float32).reshape(-1)
+    if L.chost_write_matrix(path.encode(), _fp(a), ncols, ncols if nrows is None else nrows,
+                            int(tri), int(square2d), name.encode()):
+        raise RmsdError(f"cannot write {path}")
+
+
+def write_series(path, sets):
+    """sets: list of (name, values (nframes, ncomp), width, precision)."""
+    L = load_host()
+    n = len(sets)
+    vals = [np.ascontiguousarray(np.asarray(v, np.float64).reshape(len(v), -1)) for _, v, _, _ in sets]
+    names = (C.c_char_p * n)(*[s[0].encode() for s in sets])
+    ncomp = (C.c_int * n)(*[v.shape[1] for v in vals])
+    width = (C.c_int * n)(*[s[2] for s in sets])
+    prec = (C.c_int * n)(*[s[3] for s in sets])
+    ptrs = (C.POINTER(C.c_double) * n)(*[_dp(v) for v in vals])
+    nf = vals[0].shape[0] if vals else 0
+    if L.chost_write_series(path.encode(), n, names, ncomp, width, prec, ptrs, nf):
+        raise RmsdError(f"cannot write {path}")
+
+
+def write_cmatrix(path, tri, nrows, actual_nrows, sieve, frames):
+    L = load_host()
+    t = np.ascontiguousarray(tri, dtype=np.float32)
+    fr = np.ascontiguousarray(frames, dtype=np.int32)
+    if L.chost_write_cmatrix(path.encode(), _fp(t), nrows, actual_nrows, sieve,
+                             fr.ctypes.data_as(C.POINTER(C.c_int))):
+        raise RmsdError(f"cannot write {path}")

@@ -116,6 +116,46 @@ int chost_get_coords(chost_session* s, const char* name, float* out, int64_t* nf
   return 0;
 }
 
+int chost_write_matrix(const char* path, const float* data, int64_t ncols, int64_t nrows,
+                       int is_tri, int square2d, const char* name) {
+  if (!path || ncols < 0 || nrows < 0) return 1;
+  MatrixFlt m;
+  if (is_tri) m.AllocateTriangle((size_t)ncols);
+  else m.Allocate2D((size_t)ncols, (size_t)nrows);
+  if (m.size() && !data) return 1;
+  if (m.size()) memcpy(m.data(), data, sizeof(float) * m.size());
+  m.name = name ? name : "";
+  return dataio::WriteMatrix(path, m, square2d != 0);
+}
+
+int chost_write_series(const char* path, int nsets, const char* const* names, const int* ncomp,
+                       const int* width, const int* prec, const double* const* values,
+                       int64_t nframes) {
+  if (!path || nsets < 0 || nframes < 0) return 1;
+  std::vector<Series> sets(nsets);
+  std::vector<const Series*> ptrs;
+  for (int k = 0; k < nsets; ++k) {
+    Series& s = sets[k];
+    s.name = names[k];
+    s.ncomp = ncomp[k];
+    s.width = width[k];
+    s.prec = prec[k];
+    s.v.assign(values[k], values[k] + nframes * s.ncomp);
+    ptrs.push_back(&s);
+  }
+  return dataio::WriteSeries(path, ptrs);
+}
+
+int chost_write_cmatrix(const char* path, const float* tri, int64_t nrows, int64_t actual_nrows,
+                        int sieve, const int* frames) {
+  if (!path || nrows < 0 || (nrows > 1 && !tri) || (nrows > 0 && !frames)) return 1;
+  MatrixFlt m;
+  m.AllocateTriangle((size_t)nrows);
+  if (m.size()) memcpy(m.data(), tri, sizeof(float) * m.size());
+  std::vector<int> fr(frames, frames + nrows);
+  return cmatrix::Write(path, m, (size_t)actual_nrows, sieve, fr);
+}
+
 int chost_read_cmatrix(const char* path, float* tri, int64_t* nrows, int64_t* actual_nrows,
                        int* sieve, int* frames) {
   MatrixFlt m;

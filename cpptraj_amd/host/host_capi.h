@@ -37,6 +37,19 @@ int chost_get_matrix(chost_session* s, const char* name, float* out, int64_t* nc
 int chost_get_series(chost_session* s, const char* name, double* out, int64_t* nframes,
                      int* ncomp);
 int chost_get_coords(chost_session* s, const char* name, float* out, int64_t* nfloats);
+/* Writers of the host layer, usable without a GPU: a float matrix (packed
+ * triangle of n x n when is_tri, else nrows x ncols row-major) in DataIO_Std's
+ * square2d or nosquare2d text; per-frame series (nsets sets of ncomp[s] values
+ * per frame, widths / precisions per set, names) in its 1D text; a Cmatrix
+ * file (triangle over `frames` (nrows entries) of a run over actual_nrows
+ * frames with the given sieve). */
+int chost_write_matrix(const char* path, const float* data, int64_t ncols, int64_t nrows,
+                       int is_tri, int square2d, const char* name);
+int chost_write_series(const char* path, int nsets, const char* const* names, const int* ncomp,
+                       const int* width, const int* prec, const double* const* values,
+                       int64_t nframes);
+int chost_write_cmatrix(const char* path, const float* tri, int64_t nrows, int64_t actual_nrows,
+                        int sieve, const int* frames);
 /* Cmatrix file reader: fills sizes, then (second call) the triangle and the
  * cached frame list. */
 int chost_read_cmatrix(const char* path, float* tri, int64_t* nrows, int64_t* actual_nrows,
