@@ -106,12 +106,12 @@ __device__ __forceinline__ const u32x4* frag_ptr(const u32x4* base, int64_t fb, 
 
 // ---- FP64 epilogue --------------------------------------------------------------
 // Reassembles each pair's 3x3 covariance (plus the FP64 template terms for the
-// MFMA-reduced forms), forms E0 and runs QCP for the four pairs a lane holds in
-// one accumulator column block together (G at a time), so that their
-// independent Newton iterations overlap.
-// The per-frame terms are read from gA/uA (row frames, indexed i - rbase) and
-// gB/uB (column frames, j - cbase): global memory or an LDS copy.
-template <int PREC, int WR, int WC, int G = (PREC == PREC_F64 ? 1 : 4)>
+// MFMA-reduced forms), forms E0 and runs QCP pair by pair in registers.  The
+// per-frame terms are read from gA/uA (row frames, indexed i - rbase) and
+// gB/uB (column frames, j - cbase): global memory or an LDS copy.  (Grouping
+// several pairs' Newton iterations was measured slower on the 50k-frame
+// matrix.)
+template <int PREC, int WR, int WC>
 __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
                                               typename PrecTraits<PREC>::acc_t (&acc)[WR][WC][3][3],
                                               int64_t fbA, int64_t fbB, int lane,
@@ -138,51 +138,42 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
 #pragma unroll
     for (int r = 0; r < WR; ++r) {
 #pragma unroll
-      for (int v0 = 0; v0 < 4; v0 += G) {
-        double R[G][9], E0[G], res[G];
-        bool act[G];
-        int64_t idx[G];
+      for (int v = 0; v < 4; ++v) {
+        // Accumulator layout (checked on gfx950): FP32 16x16 blocks hold row
+        // 4*(lane/16) + v in item v; the FP64 16x16x4 block holds row
+        // (lane/16) + 4*v.  Columns are lane % 16 in both.
+        const int rr = PREC == PREC_F64 ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v;
+        const int64_t i = (fbA + r) * 16 + rr;  // row frame
+        if (!(jok && i < args.FA && (!args.tri || i < j))) continue;
+        double S[9];
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-          const int v = v0 + g;
-          // Accumulator layout (checked on gfx950): FP32 16x16 blocks hold row
-          // 4*(lane/16) + v in item v; the FP64 16x16x4 block holds row
-          // (lane/16) + 4*v.  Columns are lane % 16 in both.
-          const int rr = PREC == PREC_F64 ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v;
-          const int64_t i = (fbA + r) * 16 + rr;  // row frame
-          act[g] = jok && i < args.FA && (!args.tri || i < j);
-          idx[g] = args.tri ? rmsd_dev::tri_index(args.FA, i, j) : i * args.FB + j;
-          double S[9];
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) S[3 * c + d] = acc_elem<PREC>(acc[r][q][c][d], v);
+        if (PREC != PREC_F64) {
+          const double* ui = uA + 9 * (i - rbase);
 #pragma unroll
           for (int c = 0; c < 3; ++c)
 #pragma unroll
-            for (int d = 0; d < 3; ++d) S[3 * c + d] = acc_elem<PREC>(acc[r][q][c][d], v);
-          if (PREC != PREC_F64 && act[g]) {
-            const double* ui = uA + 9 * (i - rbase);
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-              for (int d = 0; d < 3; ++d)
-                S[3 * c + d] += Pm[3 * c + d] + uj[3 * c + d] + ui[3 * d + c];
-          }
-          // Row frames are the reference in the triangle layout (the reference
-          // stores element (nref, ntgt) with ntgt > nref) and the target in
-          // the full layout; QCP takes R = sum w t r^T.
-          if (args.tri) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-              for (int d = 0; d < 3; ++d) R[g][3 * c + d] = S[3 * d + c];
-          } else {
-#pragma unroll
-            for (int k = 0; k < 9; ++k) R[g][k] = S[k];
-          }
-          E0[g] = act[g] ? 0.5 * (gA[i - rbase] + gj) : 0.0;
+            for (int d = 0; d < 3; ++d)
+              S[3 * c + d] += Pm[3 * c + d] + uj[3 * c + d] + ui[3 * d + c];
         }
-        rmsd_dev::qcp_rmsd_n<G>(R, E0, act, M, args.clamp, res);
+        // Row frames are the reference in the triangle layout (the reference
+        // stores element (nref, ntgt) with ntgt > nref) and the target in the
+        // full layout; QCP takes R = sum w t r^T.
+        double R[9];
+        if (args.tri) {
 #pragma unroll
-        for (int g = 0; g < G; ++g)
-          if (act[g]) args.out[idx[g]] = M < RMSD_SMALL ? -1.0f : (float)res[g];
+          for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) R[3 * c + d] = S[3 * d + c];
+        } else {
+#pragma unroll
+          for (int k = 0; k < 9; ++k) R[k] = S[k];
+        }
+        const double E0 = 0.5 * (gA[i - rbase] + gj);
+        const int64_t idx = args.tri ? rmsd_dev::tri_index(args.FA, i, j) : i * args.FB + j;
+        args.out[idx] = M < RMSD_SMALL ? -1.0f : (float)rmsd_dev::qcp_rmsd(R, E0, M, args.clamp);
       }
     }
   }

@@ -81,42 +81,6 @@ __device__ __forceinline__ double qcp_rmsd(const double* S, double E0, double M,
   return rmsd_from_lambda(E0, qcp_lambda(k, E0), M, clamp);
 }
 
-// QCP RMSD of NP independent pairs at once: the Newton iterations of the pairs
-// are interleaved (same per-pair arithmetic and stopping rule as qcp_lambda),
-// so their dependent FP64 chains overlap.  Inactive pairs are skipped.
-template <int NP>
-__device__ __forceinline__ void qcp_rmsd_n(const double (&S)[NP][9], const double (&E0)[NP],
-                                           const bool (&act)[NP], double M, int clamp,
-                                           double (&out)[NP]) {
-  QcpCoef k[NP];
-  double lam[NP];
-  bool done[NP];
-#pragma unroll
-  for (int p = 0; p < NP; ++p) {
-    k[p] = qcp_coefficients(S[p]);
-    lam[p] = E0[p];
-    done[p] = !act[p];
-  }
-  for (int it = 0; it < 50; ++it) {
-    bool all = true;
-#pragma unroll
-    for (int p = 0; p < NP; ++p) {
-      const double l = lam[p];
-      const double l2 = l * l;
-      const double b = (l2 + k[p].c2) * l;
-      const double a = b + k[p].c1;
-      const double nl = l - (a * l + k[p].c0) / (2.0 * l2 * l + b + a);
-      const bool conv = fabs(nl - l) < fabs(1e-11 * nl);
-      lam[p] = done[p] ? l : nl;
-      done[p] = done[p] || conv;
-      all = all && done[p];
-    }
-    if (all) break;
-  }
-#pragma unroll
-  for (int p = 0; p < NP; ++p) out[p] = act[p] ? rmsd_from_lambda(E0[p], lam[p], M, clamp) : 0.0;
-}
-
 // Optimal rotation for covariance S at eigenvalue lam, written as the Kabsch
 // U (x' = U x for the centred target).  Returns 0 when every adjugate column
 // is degenerate and the identity is returned (QCP fallback), 1 otherwise.
