@@ -663,6 +663,11 @@ struct rmsd_batch {
   std::vector<int> Kp;
   std::vector<int2*> nofit_tiles;
   std::vector<int> nofit_ntiles;
+  // batched per-set stage: one setup and one prep launch for all sets
+  SetupArgs* dsetup = nullptr;
+  PrepArgs* dprep = nullptr;
+  int* dfbase = nullptr;
+  int nactive = 0, total_frames = 0;
   // batched kernel
   PairArgs* dargs = nullptr;
   int4* dtiles[2] = {nullptr, nullptr};  // per tile side (64, 32)
@@ -733,14 +738,13 @@ void build_queues(const rmsd_batch& b, int S, std::vector<int4>* tiles, std::vec
 int batch_enqueue(rmsd_batch* b, int v, hipStream_t st) {
   rmsd_ctx* c = b->ctx;
   const int ns = (int)b->sets.size();
-  const int CH = chunk_atoms(b->prec);
-  for (int s = 0; s < ns; ++s) {
-    const rmsd_frames& f = b->sets[s];
-    if (f.nframes < 2) continue;
-    const FrameSetDev fs = to_dev(f);
-    const double* m = b->use_mass ? f.mass : nullptr;
-    double* M = b->consts[s];
-    if (b->mode == RMSD_MODE_NOFIT) {
+  if (b->mode == RMSD_MODE_NOFIT) {
+    for (int s = 0; s < ns; ++s) {
+      const rmsd_frames& f = b->sets[s];
+      if (f.nframes < 2) continue;
+      const FrameSetDev fs = to_dev(f);
+      const double* m = b->use_mass ? f.mass : nullptr;
+      double* M = b->consts[s];
       SetupArgs su{};
       su.fs = fs;
       su.smass = m;
@@ -749,35 +753,13 @@ int batch_enqueue(rmsd_batch* b, int v, hipStream_t st) {
       const int nt = b->nofit_ntiles[s];
       HIP_TRY(c, rmsd_launch_pair_nofit_tiles(fs, fs, m, M, b->nofit_tiles[s], nt,
                                               (nt + NXCD - 1) / NXCD, NXCD, 1, b->outs[s], st));
-      continue;
     }
-    SetupArgs su{};
-    su.fs = fs;
-    su.f0 = 0;
-    su.cmass = m;
-    su.smass = m;
-    su.Kp = b->Kp[s];
-    su.T = b->T[s];
-    su.P = M + 1;
-    su.gT = M + 10;
-    su.total_mass = M;
-    HIP_TRY(c, rmsd_launch_setup(su, st));
-    PrepArgs a;
-    a.fs = fs;
-    a.cmass = m;
-    a.smass = m;
-    a.prec = b->prec;
-    a.Kp = b->Kp[s];
-    a.KB = b->Kp[s] / CH;
-    a.Fp = (int)round_up(f.nframes, TILE);
-    a.packed = b->packed[s];
-    a.g = b->g[s];
-    a.u = b->u[s];
-    a.T = b->T[s];
-    a.gT = M + 10;
-    HIP_TRY(c, rmsd_launch_prep(a, st));
+    return RMSD_OK;
   }
-  if (b->mode == RMSD_MODE_NOFIT) return RMSD_OK;
+  // Every set's template / total weight, then every set's frames: two launches.
+  HIP_TRY(c, rmsd_launch_setup_batch(b->dsetup, b->nactive, st));
+  HIP_TRY(c, rmsd_launch_prep_batch(b->prec, b->dprep, b->dfbase, b->nactive, b->total_frames,
+                                    st));
   mark(c, 1);
   const int sh = v >> 1;
   BatchArgs ba;
@@ -924,7 +906,16 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
   std::vector<int4> th[2];
   std::vector<int> qh[2];
   size_t o_args = 0, o_tiles[2] = {0, 0}, o_q[2] = {0, 0}, o_cnt = 0;
+  size_t o_setup = 0, o_prep = 0, o_fbase = 0;
+  int nactive = 0;
+  for (int s = 0; s < nsets; ++s) nactive += (mode != RMSD_MODE_NOFIT && sets[s].nframes >= 2);
   if (mode != RMSD_MODE_NOFIT) {
+    o_setup = off;
+    off += al(sizeof(SetupArgs) * std::max(1, nactive));
+    o_prep = off;
+    off += al(sizeof(PrepArgs) * std::max(1, nactive));
+    o_fbase = off;
+    off += al(sizeof(int) * (nactive + 1));
     for (int sh = 0; sh < 2; ++sh) {
       build_queues(*b, sh == 0 ? 64 : 32, &th[sh], &qh[sh], &b->maxq[sh]);
       b->ntiles[sh] = (int)th[sh].size();
@@ -985,6 +976,54 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
   };
   for (int s = 0; s < nsets; ++s)
     if (b->nofit_tiles[s]) up(b->nofit_tiles[s], nt_host[s].data(), sizeof(int2) * nt_host[s].size());
+  if (mode != RMSD_MODE_NOFIT) {
+    // Per-set setup / prep arguments of the sets with pairs, and the prefix of
+    // their padded frame counts (one prep workgroup per padded frame).
+    std::vector<SetupArgs> su;
+    std::vector<PrepArgs> pa;
+    std::vector<int> fbase(1, 0);
+    for (int s = 0; s < nsets; ++s) {
+      const rmsd_frames& f = sets[s];
+      if (f.nframes < 2) continue;
+      const double* m = use_mass ? f.mass : nullptr;
+      double* M = b->consts[s];
+      SetupArgs x{};
+      x.fs = to_dev(f);
+      x.f0 = 0;
+      x.cmass = m;
+      x.smass = m;
+      x.Kp = b->Kp[s];
+      x.T = b->T[s];
+      x.P = M + 1;
+      x.gT = M + 10;
+      x.total_mass = M;
+      su.push_back(x);
+      PrepArgs a;
+      memset(&a, 0, sizeof(a));
+      a.fs = to_dev(f);
+      a.cmass = m;
+      a.smass = m;
+      a.prec = b->prec;
+      a.Kp = b->Kp[s];
+      a.KB = b->Kp[s] / CH;
+      a.Fp = (int)round_up(f.nframes, TILE);
+      a.packed = b->packed[s];
+      a.g = b->g[s];
+      a.u = b->u[s];
+      a.T = b->T[s];
+      a.gT = M + 10;
+      pa.push_back(a);
+      fbase.push_back(fbase.back() + a.Fp);
+    }
+    b->nactive = (int)su.size();
+    b->total_frames = fbase.back();
+    b->dsetup = (SetupArgs*)(base + o_setup);
+    b->dprep = (PrepArgs*)(base + o_prep);
+    b->dfbase = (int*)(base + o_fbase);
+    up(b->dsetup, su.data(), sizeof(SetupArgs) * su.size());
+    up(b->dprep, pa.data(), sizeof(PrepArgs) * pa.size());
+    up(b->dfbase, fbase.data(), sizeof(int) * fbase.size());
+  }
   if (mode != RMSD_MODE_NOFIT) {
     b->dargs = (PairArgs*)(base + o_args);
     up(b->dargs, args.data(), sizeof(PairArgs) * nsets);

@@ -79,7 +79,7 @@ __device__ __forceinline__ void centred(const FrameSetDev& fs, int64_t f, int k,
 }
 
 // ---- template / total weight -----------------------------------------------
-__global__ __launch_bounds__(PREP_THREADS) void k_setup(SetupArgs a) {
+__device__ __forceinline__ void setup_body(const SetupArgs& a) {
   __shared__ double red[16 * (PREP_THREADS / 64)];
   if (threadIdx.x == 0) {
     // Sequential running total, rounded as the reference's per-pair sum.
@@ -114,12 +114,19 @@ __global__ __launch_bounds__(PREP_THREADS) void k_setup(SetupArgs a) {
   }
 }
 
+__global__ __launch_bounds__(PREP_THREADS) void k_setup(SetupArgs a) { setup_body(a); }
+
+// One workgroup per set (batched matrices).
+__global__ __launch_bounds__(PREP_THREADS) void k_setup_batch(const SetupArgs* sets) {
+  const SetupArgs a = sets[blockIdx.x];
+  setup_body(a);
+}
+
 // ---- per-frame preparation -----------------------------------------------------
 template <int PREC>
-__global__ __launch_bounds__(PREP_THREADS) void k_prep(PrepArgs a) {
+__device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
   __shared__ double red[16 * (PREP_THREADS / 64)];
   __shared__ double Ush[9];
-  const int64_t f = blockIdx.x;
   const int Kp = a.Kp;
 
   if (f >= a.fs.nframes) {
@@ -212,10 +219,58 @@ __global__ __launch_bounds__(PREP_THREADS) void k_prep(PrepArgs a) {
     for (int k = 0; k < 9; ++k) a.u[9 * f + k] = u[k];
 }
 
+template <int PREC>
+__global__ __launch_bounds__(PREP_THREADS) void k_prep(PrepArgs a) {
+  prep_body<PREC>(a, blockIdx.x);
+}
+
+// Every padded frame of every set of a batch in one launch: workgroup b is
+// frame b - fbase[s] of the set s with fbase[s] <= b < fbase[s + 1].
+template <int PREC>
+__global__ __launch_bounds__(PREP_THREADS) void k_prep_batch(const PrepArgs* sets,
+                                                             const int* fbase, int nsets) {
+  const int b = blockIdx.x;
+  int lo = 0, hi = nsets - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (fbase[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const PrepArgs a = sets[lo];
+  prep_body<PREC>(a, b - fbase[lo]);
+}
+
 }  // namespace
 
 hipError_t rmsd_launch_setup(const SetupArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_setup, dim3(1), dim3(PREP_THREADS), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t rmsd_launch_setup_batch(const SetupArgs* sets, int nsets, hipStream_t s) {
+  if (nsets <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_setup_batch, dim3(nsets), dim3(PREP_THREADS), 0, s, sets);
+  return hipGetLastError();
+}
+
+hipError_t rmsd_launch_prep_batch(int prec, const PrepArgs* sets, const int* fbase, int nsets,
+                                  int total_frames, hipStream_t s) {
+  if (nsets <= 0 || total_frames <= 0) return hipSuccess;
+  switch (prec) {
+    case PREC_F64:
+      hipLaunchKernelGGL(k_prep_batch<PREC_F64>, dim3(total_frames), dim3(PREP_THREADS), 0, s,
+                         sets, fbase, nsets);
+      break;
+    case PREC_F32:
+      hipLaunchKernelGGL(k_prep_batch<PREC_F32>, dim3(total_frames), dim3(PREP_THREADS), 0, s,
+                         sets, fbase, nsets);
+      break;
+    case PREC_BF16X3:
+      hipLaunchKernelGGL(k_prep_batch<PREC_BF16X3>, dim3(total_frames), dim3(PREP_THREADS), 0, s,
+                         sets, fbase, nsets);
+      break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

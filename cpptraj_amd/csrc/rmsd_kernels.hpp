@@ -134,6 +134,10 @@ struct ApplyArgs {
 
 hipError_t rmsd_launch_setup(const SetupArgs& a, hipStream_t s);
 hipError_t rmsd_launch_prep(const PrepArgs& a, hipStream_t s);
+// Batched forms: one workgroup per set / per padded frame of every set.
+hipError_t rmsd_launch_setup_batch(const SetupArgs* sets, int nsets, hipStream_t s);
+hipError_t rmsd_launch_prep_batch(int prec, const PrepArgs* sets, const int* fbase, int nsets,
+                                  int total_frames, hipStream_t s);
 hipError_t rmsd_launch_pairwise(int prec, const PairArgs& a, hipStream_t s);
 int rmsd_autocorr_chunks(int n, int* rows);
 hipError_t rmsd_launch_autocorr(const float* tri, int n, double* part, double* ct, hipStream_t s);
