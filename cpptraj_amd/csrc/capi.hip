@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <map>
+#include <numeric>
 #include <set>
 #include <string>
 #include <vector>
@@ -30,7 +31,7 @@ struct DevBuf {
 };
 
 struct TileSchedule {
-  int nbr = -1, nbc = -1, tri = -1;
+  int nbr = -1, nbc = -1, tri = -1, tc = -1;
   int2* dev = nullptr;
   int ntiles = 0;
 };
@@ -135,24 +136,25 @@ FrameSetDev to_dev(const rmsd_frames& f) {
 
 // Grouped tile order: bands of GROUP tile rows, each band swept column by
 // column, so that consecutive tiles reuse the band's row blocks and one
-// column block.  Triangle schedules keep only tiles with row <= column.
-std::vector<int2> grouped_tiles(int nbr, int nbc, int tri) {
+// column block.  Tiles are tr x tc frames; triangle schedules keep only tiles
+// that hold a pair (row < column).
+std::vector<int2> grouped_tiles(int nbr, int nbc, int tri, int tr = TILE, int tc = TILE) {
   std::vector<int2> t;
-  t.reserve(tri ? (size_t)nbr * (nbr + 1) / 2 : (size_t)nbr * nbc);
   for (int b0 = 0; b0 < nbr; b0 += GROUP) {
     const int b1 = std::min(nbr, b0 + GROUP);
-    for (int j = tri ? b0 : 0; j < nbc; ++j)
+    const int j0 = tri ? (b0 * tr) / tc : 0;
+    for (int j = j0; j < nbc; ++j)
       for (int i = b0; i < b1; ++i)
-        if (!tri || i <= j) t.push_back(make_int2(i, j));
+        if (!tri || i * tr < j * tc + tc - 1) t.push_back(make_int2(i, j));
   }
   return t;
 }
 
-int tiles_for(rmsd_ctx* c, int FA, int FB, int tri, const int2** tiles, int* ntiles) {
-  const int nbr = (FA + TILE - 1) / TILE, nbc = (FB + TILE - 1) / TILE;
+int tiles_for(rmsd_ctx* c, int FA, int FB, int tri, int tc, const int2** tiles, int* ntiles) {
+  const int nbr = (FA + TILE - 1) / TILE, nbc = (FB + tc - 1) / tc;
   TileSchedule& s = c->sched;
-  if (s.nbr != nbr || s.nbc != nbc || s.tri != tri) {
-    std::vector<int2> h = grouped_tiles(nbr, nbc, tri);
+  if (s.nbr != nbr || s.nbc != nbc || s.tri != tri || s.tc != tc) {
+    std::vector<int2> h = grouped_tiles(nbr, nbc, tri, TILE, tc);
     int st = RMSD_OK;
     s.nbr = s.nbc = s.tri = -1;
     int2* d = (int2*)scratch(c, "tiles", h.size() * sizeof(int2), &st);
@@ -164,6 +166,7 @@ int tiles_for(rmsd_ctx* c, int FA, int FB, int tri, const int2** tiles, int* nti
     s.nbr = nbr;
     s.nbc = nbc;
     s.tri = tri;
+    s.tc = tc;
     s.dev = d;
     s.ntiles = (int)h.size();
   }
@@ -185,7 +188,9 @@ int prepare_set(rmsd_ctx* c, const FrameSetDev& fs, const double* cmass, const d
                 int prec, int Kp, const double* T, const double* gT, const char* tag,
                 Packed* out) {
   const int CH = chunk_atoms(prec);
-  const int Fp = (int)round_up(fs.nframes, TILE);
+  // Padded to whole row and column tiles: 64 and rmsd_pairwise_tile_cols.
+  const int tc = rmsd_pairwise_tile_cols(prec);
+  const int Fp = (int)round_up(fs.nframes, std::lcm(TILE, tc));
   const size_t esz = prec == PREC_F64 ? 8 : (prec == PREC_F32 ? 4 : 2);
   const size_t bytes = (size_t)Fp * Kp * 3 * esz * chunk_parts(prec);
   int st = RMSD_OK;
@@ -224,7 +229,8 @@ int pairwise_impl(rmsd_ctx* c, const FrameSetDev& A, const FrameSetDev& B, int t
   int st = RMSD_OK;
   const int2* tiles = nullptr;
   int ntiles = 0;
-  if ((st = tiles_for(c, FA, FB, tri, &tiles, &ntiles)) != RMSD_OK) return st;
+  const int tc = mode == RMSD_MODE_NOFIT ? TILE : rmsd_pairwise_tile_cols(prec);
+  if ((st = tiles_for(c, FA, FB, tri, tc, &tiles, &ntiles)) != RMSD_OK) return st;
   const int per_xcd = (ntiles + NXCD - 1) / NXCD;
   double* consts = (double*)scratch(c, "pair.consts", sizeof(double) * 16, &st);
   if (!consts) return st;

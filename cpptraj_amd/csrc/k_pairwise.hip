@@ -311,355 +311,12 @@ __global__ void k_fill_int(int* p, int n, int v) {
   if (i < n) p[i] = v;
 }
 
-// ---- LDS-staged contraction for the MFMA-reduced forms ----------------------------
-// A 256-thread workgroup owns 64 x 64 frame pairs; each of its four waves owns
-// 32 x 32 (WR = WC = 2).  Per chunk of atoms the workgroup needs 4 row and 4
-// column frame blocks x 3 coordinates x PARTS = NFRAG fragments of 1 KiB.  Each
-// wave fetches a quarter of them from HBM/L2 into registers two chunks ahead,
-// stores them into one of two LDS stages one chunk ahead, and every wave then
-// reads its own 24 (BF16) or 12 (FP32) fragments from LDS.  Every fragment is
-// fetched once per workgroup (instead of once per wave that uses it) and a
-// global load has two chunks of MFMA work to land in.
-template <int PREC>
-__global__ __launch_bounds__(256) void k_pairwise_lds(PairArgs args) {
-  typedef PrecTraits<PREC> T;
-  typedef typename T::acc_t acc_t;
-  constexpr int PARTS = T::PARTS;
-  constexpr int HALF = 4 * 3 * PARTS;       // fragments of one side (A or B)
-  constexpr int NFRAG = 2 * HALF;           // per chunk and workgroup
-  constexpr int PER_WAVE = NFRAG / 4;       // fetched by each wave
-  __shared__ u32x4 lds[2][NFRAG][64];
-
-  const int w = blockIdx.x;
-  const int t = (w % args.nxcd) * args.per_xcd + (w / args.nxcd);
-  if (t >= args.ntiles) return;
-  const int2 tile = args.tiles[t];
-  const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  const int wr = wave >> 1, wc = wave & 1;
-  const int64_t fbA0 = (int64_t)tile.x * 4, fbB0 = (int64_t)tile.y * 4;
-  const u32x4* A = reinterpret_cast<const u32x4*>(args.A);
-  const u32x4* B = reinterpret_cast<const u32x4*>(args.B);
-  const int KB = args.KB;
-
-  // Fetch plan: waves 0-1 fetch the A side, waves 2-3 the B side; each wave
-  // takes two frame blocks, whose 3 * PARTS fragments of one chunk are
-  // contiguous (CS pieces per chunk), so a chunk's fetch is two pointers plus
-  // immediate offsets.
-  constexpr int BF = 3 * PARTS;             // fragments per block and chunk
-  constexpr int64_t CS = BF * 64;           // 16-byte pieces per block and chunk
-  const int side = wave >> 1;
-  const int blk0 = (wave & 1) * 2;
-  const int64_t fb0 = (side ? fbB0 : fbA0) + blk0;
-  const u32x4* src0 = (side ? B : A) + fb0 * KB * CS + lane;
-  const u32x4* src1 = src0 + (int64_t)KB * CS;
-  const int dst0 = side * HALF + blk0 * BF;  // first LDS fragment of this wave
-  u32x4 stg[2][PER_WAVE];
-  // Register stage S holds the chunks of parity S; LDS stage S likewise.
-  auto fetch = [&](auto S_, int kb) {
-    constexpr int S = decltype(S_)::value;
-    const u32x4* p0 = src0 + (int64_t)kb * CS;
-    const u32x4* p1 = src1 + (int64_t)kb * CS;
-#pragma unroll
-    for (int e = 0; e < BF; ++e) {
-      stg[S][e] = ld16(p0 + 64 * e);
-      stg[S][BF + e] = ld16(p1 + 64 * e);
-    }
-  };
-  auto stash = [&](auto S_) {
-    constexpr int S = decltype(S_)::value;
-#pragma unroll
-    for (int e = 0; e < PER_WAVE; ++e) lds[S][dst0 + e][lane] = stg[S][e];
-  };
-
-  acc_t acc[2][2][3][3];
-#pragma unroll
-  for (int r = 0; r < 2; ++r)
-#pragma unroll
-    for (int q = 0; q < 2; ++q)
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) acc[r][q][c][d] = acc_t{0, 0, 0, 0};
-
-  auto compute = [&](auto S_) {
-    constexpr int S = decltype(S_)::value;
-    u32x4 fbv[2][3][PARTS];
-#pragma unroll
-    for (int q = 0; q < 2; ++q)
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int p = 0; p < PARTS; ++p)
-          fbv[q][c][p] = lds[S][HALF + ((2 * wc + q) * 3 + c) * PARTS + p][lane];
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      u32x4 fav[3][PARTS];
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int p = 0; p < PARTS; ++p) fav[c][p] = lds[S][((2 * wr + r) * 3 + c) * PARTS + p][lane];
-#pragma unroll
-      for (int q = 0; q < 2; ++q)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-          for (int d = 0; d < 3; ++d) mma<PREC>(acc[r][q][c][d], fav[c], fbv[q][d]);
-    }
-  };
-  // One chunk: fetch chunk kb + 2 into the register stage that held chunk kb
-  // (already in LDS), contract chunk kb from LDS stage S, move chunk kb + 1
-  // (fetched one chunk ago) into the other LDS stage, which every wave
-  // finished reading before the previous barrier.
-  // The fetch and the stash are issued unconditionally (past the last chunk
-  // they re-read chunk KB - 1 and fill a stage nobody reads again), so the
-  // load counter waits stay exact: a stash waits only for the loads of the
-  // chunk it stores, never for the chunk fetched in the same step.
-  auto step = [&](auto S_, int kb) {
-    constexpr int S = decltype(S_)::value;
-    std::integral_constant<int, 1 - S> O;
-    fetch(S_, kb + 2 < KB ? kb + 2 : KB - 1);
-    compute(S_);
-    stash(O);
-    __syncthreads();
-  };
-  std::integral_constant<int, 0> S0;
-  std::integral_constant<int, 1> S1;
-  fetch(S0, 0);
-  stash(S0);
-  fetch(S1, KB > 1 ? 1 : 0);
-  __syncthreads();
-  for (int kb = 0; kb < KB; kb += 2) {
-    step(S0, kb);
-    if (kb + 1 < KB) step(S1, kb + 1);
-  }
-
-  // Per-frame FP64 terms of the tile's 64 row and 64 column frames, staged in
-  // LDS so that the epilogue does not wait on global loads pair by pair.
-  __shared__ double eg[2][64];
-  __shared__ double eu[2][64][9];
-  const int64_t r0 = fbA0 * 16, c0 = fbB0 * 16;
-  for (int e = threadIdx.x; e < 2 * 64; e += blockDim.x) {
-    const int side = e >> 6, f = e & 63;
-    const int64_t fr = (side ? c0 : r0) + f;
-    const bool ok = fr < (side ? args.FB : args.FA);
-    eg[side][f] = ok ? (side ? args.gB : args.gA)[fr] : 0.0;
-    const double* u = (side ? args.uB : args.uA) + 9 * fr;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) eu[side][f][k] = ok ? u[k] : 0.0;
-  }
-  __syncthreads();
-  pair_epilogue<PREC, 2, 2>(args, acc, fbA0 + 2 * wr, fbB0 + 2 * wc, lane, eg[0], &eu[0][0][0], r0,
-                            eg[1], &eu[1][0][0], c0);
-}
-
-// ---- register-pipelined contraction ---------------------------------------------
-// Same wave tile as k_pairwise (WR x WC blocks of 16 x 16 pairs, all nine
-// (c, d) blocks), no LDS and no barriers: every wave streams its own fragments
-// through NS register stages, refilling a stage right after the chunk in it has
-// been contracted, so each load has NS - 1 chunks of MFMA work to land in.
-template <int PREC, int WR, int WC, int NWR, int NWC, int NS>
-__global__ __launch_bounds__(64 * NWR * NWC) void k_pairwise_rs(PairArgs args) {
-  typedef PrecTraits<PREC> T;
-  typedef typename T::acc_t acc_t;
-  constexpr int PARTS = T::PARTS;
-  constexpr int TILE_R = 16 * WR * NWR;
-  constexpr int TILE_C = 16 * WC * NWC;
-  const int w = blockIdx.x;
-  const int t = (w % args.nxcd) * args.per_xcd + (w / args.nxcd);
-  if (t >= args.ntiles) return;
-  const int2 tile = args.tiles[t];
-  const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  const int wr = wave / NWC, wc = wave % NWC;
-  const int64_t fbA = (int64_t)tile.x * (TILE_R / 16) + wr * WR;
-  const int64_t fbB = (int64_t)tile.y * (TILE_C / 16) + wc * WC;
-  const int KB = args.KB;
-  constexpr int64_t CS = 3 * PARTS * 64;  // pieces per block and chunk
-  const u32x4* pA = reinterpret_cast<const u32x4*>(args.A) + fbA * KB * CS + lane;
-  const u32x4* pB = reinterpret_cast<const u32x4*>(args.B) + fbB * KB * CS + lane;
-  const int64_t BS = (int64_t)KB * CS;    // pieces per frame block
-
-  acc_t acc[WR][WC][3][3];
-#pragma unroll
-  for (int r = 0; r < WR; ++r)
-#pragma unroll
-    for (int q = 0; q < WC; ++q)
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) acc[r][q][c][d] = acc_t{0, 0, 0, 0};
-
-  u32x4 fa[NS][WR][3][PARTS], fb[NS][WC][3][PARTS];
-  auto load = [&](auto S_, int kb) {
-    constexpr int S = decltype(S_)::value;
-    const int64_t o = (int64_t)kb * CS;
-#pragma unroll
-    for (int r = 0; r < WR; ++r)
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int p = 0; p < PARTS; ++p) fa[S][r][c][p] = ld16(pA + r * BS + o + (c * PARTS + p) * 64);
-#pragma unroll
-    for (int q = 0; q < WC; ++q)
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int p = 0; p < PARTS; ++p) fb[S][q][c][p] = ld16(pB + q * BS + o + (c * PARTS + p) * 64);
-  };
-  auto compute = [&](auto S_) {
-    constexpr int S = decltype(S_)::value;
-#pragma unroll
-    for (int r = 0; r < WR; ++r)
-#pragma unroll
-      for (int q = 0; q < WC; ++q)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-          for (int d = 0; d < 3; ++d) mma<PREC>(acc[r][q][c][d], fa[S][r][c], fb[S][q][d]);
-  };
-  // Loads past the last chunk re-read chunk KB - 1 (in bounds, unused) so that
-  // every path issues the same loads and the counter waits stay exact.
-  auto clampk = [&](int kb) { return kb < KB ? kb : KB - 1; };
-  std::integral_constant<int, 0> S0;
-  std::integral_constant<int, 1> S1;
-  std::integral_constant<int, 2> S2;
-  load(S0, 0);
-  if constexpr (NS > 1) load(S1, clampk(1));
-  if constexpr (NS > 2) load(S2, clampk(2));
-  for (int kb = 0; kb < KB; kb += NS) {
-    compute(S0);
-    load(S0, clampk(kb + NS));
-    if constexpr (NS > 1) {
-      if (kb + 1 < KB) {
-        compute(S1);
-        load(S1, clampk(kb + 1 + NS));
-      }
-    }
-    if constexpr (NS > 2) {
-      if (kb + 2 < KB) {
-        compute(S2);
-        load(S2, clampk(kb + 2 + NS));
-      }
-    }
-  }
-  pair_epilogue<PREC, WR, WC>(args, acc, fbA, fbB, lane, args.gA, args.uA, 0, args.gB, args.uB, 0);
-}
-
 }  // namespace
-
-// ---- LDS-DMA contraction for the MFMA-reduced forms --------------------------------
-// A 256-thread workgroup owns 64 x 64 frame pairs (four 32 x 32 waves).  Per
-// chunk of atoms the workgroup needs NF fragments of 1 KiB (4 row and 4 column
-// frame blocks x 3 coordinates x PARTS).  They travel from HBM/L2 straight into
-// an NS-stage LDS ring with global_load_lds_dwordx4 (64 lanes x 16 B = one
-// fragment per instruction, no register staging); each wave issues NF / 4 of
-// them NS - 1 chunks ahead, so every fragment is fetched once per workgroup
-// and the vector memory path carries half the bytes of the register-tile
-// kernel.  One barrier per chunk: after it, the chunk's fragments are in LDS
-// and the stage of the previous chunk is free for the next DMA.
-template <int PREC, int NS>
-__global__ __launch_bounds__(256) void k_pairwise_dma(PairArgs args) {
-  typedef PrecTraits<PREC> T;
-  typedef typename T::acc_t acc_t;
-  constexpr int PARTS = T::PARTS;
-  constexpr int WR = 2, WC = 2;
-  constexpr int NF = 2 * 4 * 3 * PARTS;  // fragments per chunk and workgroup
-  constexpr int LPW = NF / 4;            // DMA instructions per wave and chunk
-  __shared__ u32x4 lds[NS][NF][64];
-
-  const int w = blockIdx.x;
-  const int t = (w % args.nxcd) * args.per_xcd + (w / args.nxcd);
-  if (t >= args.ntiles) return;
-  const int2 tile = args.tiles[t];
-  const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  const int wr = wave >> 1, wc = wave & 1;
-  const int64_t fb0A = (int64_t)tile.x * 4, fb0B = (int64_t)tile.y * 4;
-  const int64_t fbA = fb0A + wr * WR, fbB = fb0B + wc * WC;
-  const u32x4* A = reinterpret_cast<const u32x4*>(args.A);
-  const u32x4* B = reinterpret_cast<const u32x4*>(args.B);
-  const int KB = args.KB;
-
-  // This wave's DMA fragments: f = wave * LPW + i, f = ((side * 4 + blk) * 3 + c) * PARTS + p.
-  const u32x4* src[LPW];
-#pragma unroll
-  for (int i = 0; i < LPW; ++i) {
-    const int f = wave * LPW + i;
-    const int p = f % PARTS, c = (f / PARTS) % 3, blk = (f / (3 * PARTS)) % 4;
-    const int side = f / (12 * PARTS);
-    src[i] = frag_ptr(side ? B : A, (side ? fb0B : fb0A) + blk, 0, KB, c, p, PARTS) + lane;
-  }
-  // Chunk stride of one fragment sequence: 3 * PARTS fragments of 64 lanes.
-  const int64_t kstride = (int64_t)3 * PARTS * 64;
-  auto issue = [&](int kb, int stage) {
-    const int k = kb < KB ? kb : KB - 1;  // past the end: reload the last chunk (never read)
-#pragma unroll
-    for (int i = 0; i < LPW; ++i)
-      __builtin_amdgcn_global_load_lds(
-          (const void*)(src[i] + k * kstride),
-          (__attribute__((address_space(3))) void*)&lds[stage][wave * LPW + i][0], 16, 0, 0);
-  };
-
-  acc_t acc[WR][WC][3][3];
-#pragma unroll
-  for (int r = 0; r < WR; ++r)
-#pragma unroll
-    for (int q = 0; q < WC; ++q)
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) acc[r][q][c][d] = acc_t{0, 0, 0, 0};
-
-#pragma unroll
-  for (int s = 0; s < NS - 1; ++s) issue(s, s);
-  int stage = 0;
-  for (int kb = 0; kb < KB; ++kb) {
-    // Own DMA of chunk kb done (NS - 2 later chunks may still be in flight),
-    // then every wave's: a bare s_barrier (a __syncthreads fence would wait for
-    // every DMA in flight).  The asm's memory clobber keeps the compiler from
-    // moving LDS reads across it.  vmcnt in the gfx9 encoding; expcnt and
-    // lgkmcnt not waited.
-    constexpr int VM = (NS - 2) * LPW;
-    __builtin_amdgcn_s_waitcnt((VM & 15) | ((VM >> 4) << 14) | (7 << 4) | (15 << 8));
-    asm volatile("s_barrier" ::: "memory");
-    int nstage = stage + NS - 1;
-    if (nstage >= NS) nstage -= NS;
-    issue(kb + NS - 1, nstage);
-    u32x4 ca[WR][3][PARTS], cb[WC][3][PARTS];
-#pragma unroll
-    for (int r = 0; r < WR; ++r)
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int p = 0; p < PARTS; ++p)
-          ca[r][c][p] = lds[stage][((0 * 4 + wr * WR + r) * 3 + c) * PARTS + p][lane];
-#pragma unroll
-    for (int q = 0; q < WC; ++q)
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int p = 0; p < PARTS; ++p)
-          cb[q][c][p] = lds[stage][((1 * 4 + wc * WC + q) * 3 + c) * PARTS + p][lane];
-#pragma unroll
-    for (int r = 0; r < WR; ++r)
-#pragma unroll
-      for (int q = 0; q < WC; ++q)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-          for (int d = 0; d < 3; ++d) mma<PREC>(acc[r][q][c][d], ca[r][c], cb[q][d]);
-    stage = stage + 1 == NS ? 0 : stage + 1;
-  }
-  // Drain the tail DMA before the workgroup's LDS can be reused.
-  __builtin_amdgcn_s_waitcnt((7 << 4) | (15 << 8));
-  pair_epilogue<PREC, WR, WC>(args, acc, fbA, fbB, lane, args.gA, args.uA, 0, args.gB, args.uB, 0);
-}
 
 // ---------------------------------------------------------------------------
 // Launchers.  Tile shapes: FP64 waves own 16x32 pairs (72 FP64 accumulators),
-// the FP32 / BF16 waves 32x32 pairs (144 FP32 accumulators); every workgroup
-// covers a 64x64 block of frame pairs.
+// FP32 waves 32x32 pairs (144 FP32 accumulators), split-BF16 waves 32x48 pairs
+// (216); a workgroup covers 64 x 64 (64 x 96 for split-BF16) frame pairs.
 template <int PREC, int WR, int WC, int NWR, int NWC>
 static hipError_t launch_tiles(const PairArgs& a, hipStream_t s) {
   const int nblocks = a.per_xcd * a.nxcd;
@@ -668,49 +325,18 @@ static hipError_t launch_tiles(const PairArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int PREC, int NS>
-static hipError_t launch_rs(const PairArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_pairwise_rs<PREC, 2, 2, 2, 2, NS>), dim3(a.per_xcd * a.nxcd), dim3(256), 0,
-                     s, a);
-  return hipGetLastError();
-}
-
-template <int PREC, int NS>
-static hipError_t launch_dma(const PairArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_pairwise_dma<PREC, NS>), dim3(a.per_xcd * a.nxcd), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-
-template <int PREC>
-static hipError_t launch_lds(const PairArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_pairwise_lds<PREC>), dim3(a.per_xcd * a.nxcd), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
+// Pair tile of the single-matrix kernel: 64 row frames x pairwise_tile_cols(prec)
+// column frames.  Split-BF16 waves own 32 x 48 pairs (fewer operand bytes per
+// MFMA than 32 x 32; measured 3 % faster on the 50k-frame matrix).
+int rmsd_pairwise_tile_cols(int prec) { return prec == PREC_BF16X3 ? 96 : 64; }
 
 hipError_t rmsd_launch_pairwise(int prec, const PairArgs& a, hipStream_t s) {
   switch (prec) {
     case PREC_F64: return launch_tiles<PREC_F64, 1, 2, 4, 2>(a, s);
     case PREC_F32: return launch_tiles<PREC_F32, 2, 2, 2, 2>(a, s);
-    case PREC_BF16X3: return launch_tiles<PREC_BF16X3, 2, 2, 2, 2>(a, s);
+    case PREC_BF16X3: return launch_tiles<PREC_BF16X3, 2, 3, 2, 2>(a, s);
     default: return hipErrorInvalidValue;
   }
-}
-
-// Variants of the alternative single-matrix kernels, for timing studies and
-// the tuner: 0 register tiles (default), 1 register pipeline (three chunks in
-// flight), 2 LDS-staged operands, 3 LDS-DMA ring (three stages), 4 LDS-DMA
-// ring (two stages).
-hipError_t rmsd_launch_pairwise_variant(int prec, int variant, const PairArgs& a, hipStream_t s) {
-  if (variant == 0 || prec == PREC_F64) return rmsd_launch_pairwise(prec, a, s);
-  if (variant == 1)
-    return prec == PREC_F32 ? launch_rs<PREC_F32, 3>(a, s) : launch_rs<PREC_BF16X3, 3>(a, s);
-  if (variant == 2)
-    return prec == PREC_F32 ? launch_lds<PREC_F32>(a, s) : launch_lds<PREC_BF16X3>(a, s);
-  if (variant == 3)
-    return prec == PREC_F32 ? launch_dma<PREC_F32, 3>(a, s) : launch_dma<PREC_BF16X3, 3>(a, s);
-  if (variant == 4)
-    return prec == PREC_F32 ? launch_dma<PREC_F32, 2>(a, s) : launch_dma<PREC_BF16X3, 2>(a, s);
-  return hipErrorInvalidValue;
 }
 
 // Batched launches.  Variant v: tile shape v / 2 (shape 0: 64 x 64 pair tiles,

@@ -139,9 +139,10 @@ hipError_t rmsd_launch_setup_batch(const SetupArgs* sets, int nsets, hipStream_t
 hipError_t rmsd_launch_prep_batch(int prec, const PrepArgs* sets, const int* fbase, int nsets,
                                   int total_frames, hipStream_t s);
 hipError_t rmsd_launch_pairwise(int prec, const PairArgs& a, hipStream_t s);
+// Column frames per pair tile of rmsd_launch_pairwise (rows: 64).
+int rmsd_pairwise_tile_cols(int prec);
 int rmsd_autocorr_chunks(int n, int* rows);
 hipError_t rmsd_launch_autocorr(const float* tri, int n, double* part, double* ct, hipStream_t s);
-hipError_t rmsd_launch_pairwise_variant(int prec, int variant, const PairArgs& a, hipStream_t s);
 int rmsd_batch_variant_count();
 int rmsd_batch_tile_side(int variant);
 hipError_t rmsd_launch_batch_reset(const BatchArgs& b, hipStream_t s);
