@@ -315,8 +315,8 @@ __global__ void k_fill_int(int* p, int n, int v) {
 
 // ---------------------------------------------------------------------------
 // Launchers.  Tile shapes: FP64 waves own 16x32 pairs (72 FP64 accumulators),
-// FP32 waves 32x32 pairs (144 FP32 accumulators), split-BF16 waves 32x48 pairs
-// (216); a workgroup covers 64 x 64 (64 x 96 for split-BF16) frame pairs.
+// FP32 and split-BF16 waves 32x48 pairs (216 FP32 accumulators); a workgroup
+// covers 64 x 64 (FP64) or 64 x 96 frame pairs.
 template <int PREC, int WR, int WC, int NWR, int NWC>
 static hipError_t launch_tiles(const PairArgs& a, hipStream_t s) {
   const int nblocks = a.per_xcd * a.nxcd;
@@ -326,14 +326,15 @@ static hipError_t launch_tiles(const PairArgs& a, hipStream_t s) {
 }
 
 // Pair tile of the single-matrix kernel: 64 row frames x pairwise_tile_cols(prec)
-// column frames.  Split-BF16 waves own 32 x 48 pairs (fewer operand bytes per
-// MFMA than 32 x 32; measured 3 % faster on the 50k-frame matrix).
-int rmsd_pairwise_tile_cols(int prec) { return prec == PREC_BF16X3 ? 96 : 64; }
+// column frames.  FP32 and split-BF16 waves own 32 x 48 pairs (fewer operand
+// bytes per MFMA than 32 x 32; measured 3 % (split-BF16) and 1 % (FP32) faster
+// on the 50k-frame matrix).
+int rmsd_pairwise_tile_cols(int prec) { return prec == PREC_F64 ? 64 : 96; }
 
 hipError_t rmsd_launch_pairwise(int prec, const PairArgs& a, hipStream_t s) {
   switch (prec) {
     case PREC_F64: return launch_tiles<PREC_F64, 1, 2, 4, 2>(a, s);
-    case PREC_F32: return launch_tiles<PREC_F32, 2, 2, 2, 2>(a, s);
+    case PREC_F32: return launch_tiles<PREC_F32, 2, 3, 2, 2>(a, s);
     case PREC_BF16X3: return launch_tiles<PREC_BF16X3, 2, 3, 2, 2>(a, s);
     default: return hipErrorInvalidValue;
   }

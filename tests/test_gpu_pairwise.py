@@ -157,3 +157,26 @@ def test_mixed_precision_bound(oracle, ctx, precision, use_mass):
     exp = oracle.rms2d(X, None, m, 0, use_mass)
     k = rows * F - rows * (rows + 1) // 2
     assert np.abs(ref64[:k] - exp[:k]).max() <= 1e-6
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_triangle_past_2_31_elements(oracle, ctx, precision):
+    """70,000 frames: the packed triangle holds 2,449,965,000 floats, so its
+    index passes 2^31 (Matrix<float>::calcTriIndex on size_t); rows at the
+    start, middle and end (the last pairs sit past the 32-bit range) against
+    the oracle.  Eight atoms keep the work small."""
+    import torch
+    import cpptraj_amd as ca
+    F, n = 70000, 8
+    X = synth.noisy(71, F, n, 0.7).reshape(F, -1)
+    out = torch.empty(F * (F - 1) // 2, dtype=torch.float32, device="cuda")
+    ctx.pairwise_desc(ca.frames_desc(dev(X)), None, 0, False, ca.PRECISIONS[precision], out)
+    rows = np.array([0, 35000, 60000, F - 2], np.int32)
+    exp = oracle.rms2d_rows(X, rows)
+    starts = [int(r) * F - int(r) * (int(r) + 1) // 2 for r in rows]
+    assert starts[-1] > 2 ** 31
+    got = np.concatenate([out[s:s + F - 1 - int(r)].cpu().numpy() for s, r in zip(starts, rows)])
+    tol = 1e-6 if precision == "fp64" else 1e-4
+    assert np.abs(got.astype(np.float64) - exp).max() <= tol
+    del out
+    torch.cuda.empty_cache()
