@@ -126,7 +126,6 @@ __global__ __launch_bounds__(PREP_THREADS) void k_setup_batch(const SetupArgs* s
 template <int PREC>
 __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
   __shared__ double red[16 * (PREP_THREADS / 64)];
-  __shared__ double Ush[9];
   const int Kp = a.Kp;
 
   if (f >= a.fs.nframes) {
@@ -174,19 +173,16 @@ __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
     v[9] += x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
   }
   rmsd_dev::block_sum<10>(v, red);
-  if (threadIdx.x == 0) {
-    a.g[f] = v[9];
+  // Every thread holds the block totals and solves the same FP64 QCP (same
+  // inputs, same operations), so the rotation needs no broadcast or barrier.
+  if (threadIdx.x == 0) a.g[f] = v[9];
+  double U[9];
+  {
     const double E0 = 0.5 * (v[9] + a.gT[0]);
     const rmsd_dev::QcpCoef co = rmsd_dev::qcp_coefficients(v);
     const double lam = rmsd_dev::qcp_lambda(co, E0);
-    double U[9];
     rmsd_dev::qcp_rotation(v, lam, U);
-    for (int k = 0; k < 9; ++k) Ush[k] = U[k];
   }
-  __syncthreads();
-  double U[9];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) U[k] = Ush[k];
 
   double u[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int k = threadIdx.x; k < Kp; k += blockDim.x) {
