@@ -9,7 +9,8 @@
 // src/Frame.cpp:1141-1171) and the E0 half-sums (src/Frame.cpp:1194).  These
 // depend only on the frame, so they run once per frame here.
 //
-// One workgroup per frame.  All sums are FP64.  With weights the coordinates
+// One wave per frame (64-thread workgroups: 14 % faster than 256 on the
+// 50k-frame matrix).  All sums are FP64.  With weights the coordinates
 // are stored pre-multiplied by sqrt(w), so sum_k w_k t_k r_k^T becomes a plain
 // dot product of the stored rows.
 //
@@ -25,7 +26,7 @@
 
 namespace {
 
-constexpr int PREP_THREADS = 256;
+constexpr int PREP_THREADS = 64;
 
 __device__ __forceinline__ float load_coord(const FrameSetDev& fs, int64_t f, int k, int c) {
   const int a = fs.sel ? fs.sel[k] : k;
