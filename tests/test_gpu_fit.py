@@ -115,3 +115,22 @@ def test_large_offsets(oracle, ctx):
     r_o, _, _, _ = oracle_fit(oracle, X, None, None, False)
     r, _, _ = ca.rmsd_fit(dev(X), None, 0, modify="nomod", ctx=ctx)
     assert_rmsd_close(r.cpu().numpy(), r_o, X)
+
+
+@pytest.mark.parametrize("F", [1, 7, 601])
+@pytest.mark.parametrize("use_mass", [False, True])
+def test_fit_soa_ragged(oracle, ctx, F, use_mass):
+    """SoA rows with an atom count that is not a multiple of four (the vector
+    path's scalar tail) and frame counts that leave a wave with one frame."""
+    import cpptraj_amd as ca
+    n = 2999
+    X3 = synth.noisy(5, F, n, 1.0)
+    m = synth.masses(np.random.default_rng(5), n)
+    X = X3.reshape(F, -1)
+    r_o, U_o, tv_o, _ = oracle_fit(oracle, X, None, m, use_mass)
+    Xd = dev(synth.to_soa(X3, ld=3008))
+    r, U, tv = ca.rmsd_fit(Xd, None, 0, mass=dev(m), use_mass=use_mass, modify="nomod",
+                           save_matrices=True, save_vectors=True, soa=True, natom=n, ctx=ctx)
+    assert_rmsd_close(r.cpu().numpy(), r_o, X)
+    assert np.abs(U.cpu().numpy() - U_o).max() <= 1e-9
+    assert np.abs(tv.cpu().numpy() - tv_o).max() <= 1e-9
