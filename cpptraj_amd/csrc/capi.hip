@@ -703,8 +703,8 @@ double tile_cost(int prec, int KB) {
 // Tile queues for tile side S: sets sorted by cost per tile (descending, ties
 // by set index), each set's triangle tiles in grouped order, the concatenated
 // list cut into nq contiguous runs of equal total cost.
-void build_queues(const rmsd_batch& b, int S, std::vector<int4>* tiles, std::vector<int>* qbeg,
-                  int* maxq) {
+void build_queues(const rmsd_batch& b, int tr, int tc, std::vector<int4>* tiles,
+                  std::vector<int>* qbeg, int* maxq) {
   const int ns = (int)b.sets.size();
   const int CH = chunk_atoms(b.prec);
   std::vector<int> order(ns);
@@ -717,9 +717,9 @@ void build_queues(const rmsd_batch& b, int S, std::vector<int4>* tiles, std::vec
   for (int s : order) {
     const int F = b.sets[s].nframes;
     if (F < 2) continue;
-    const int nb = (F + S - 1) / S;
+    const int nbr = (F + tr - 1) / tr, nbc = (F + tc - 1) / tc;
     const double c = tile_cost(b.prec, b.Kp[s] / CH);
-    for (int2 t : grouped_tiles(nb, nb, 1)) {
+    for (int2 t : grouped_tiles(nbr, nbc, 1, tr, tc)) {
       tiles->push_back(make_int4(s, t.x, t.y, 0));
       cost.push_back(c);
     }
@@ -879,6 +879,9 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
   b->sets.assign(sets, sets + nsets);
   b->outs.assign(outs, outs + nsets);
   const int CH = chunk_atoms(b->prec);
+  // Frames padded to whole tiles of every batch shape (rows 64, columns up to
+  // rmsd_pairwise_tile_cols).
+  const int pad = std::lcm(TILE, rmsd_pairwise_tile_cols(b->prec));
   const size_t esz = b->prec == PREC_F64 ? 8 : (b->prec == PREC_F32 ? 4 : 2);
   // Layout of the single allocation.
   auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
@@ -888,7 +891,7 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
   b->Kp.resize(nsets);
   for (int s = 0; s < nsets; ++s) {
     const int F = sets[s].nframes;
-    const int Fp = (int)round_up(F, TILE);
+    const int Fp = (int)round_up(F, pad);
     b->Kp[s] = (int)round_up(sets[s].nsel, CH);
     o_c[s] = off;
     off += al(sizeof(double) * 16);
@@ -923,7 +926,9 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
     o_fbase = off;
     off += al(sizeof(int) * (nactive + 1));
     for (int sh = 0; sh < 2; ++sh) {
-      build_queues(*b, sh == 0 ? 64 : 32, &th[sh], &qh[sh], &b->maxq[sh]);
+      int tr = 0, tc = 0;
+      rmsd_batch_tile_shape(b->prec, 2 * sh, &tr, &tc);
+      build_queues(*b, tr, tc, &th[sh], &qh[sh], &b->maxq[sh]);
       b->ntiles[sh] = (int)th[sh].size();
       o_tiles[sh] = off;
       off += al(sizeof(int4) * std::max<size_t>(1, th[sh].size()));
@@ -1012,7 +1017,7 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
       a.prec = b->prec;
       a.Kp = b->Kp[s];
       a.KB = b->Kp[s] / CH;
-      a.Fp = (int)round_up(f.nframes, TILE);
+      a.Fp = (int)round_up(f.nframes, pad);
       a.packed = b->packed[s];
       a.g = b->g[s];
       a.u = b->u[s];
@@ -1120,7 +1125,9 @@ int rmsd_batch_info(const rmsd_batch* plan, int64_t* info, double* tune_ms) {
   if (info) {
     const int sh = plan->variant >> 1;
     info[0] = plan->variant;
-    info[1] = plan->mode == RMSD_MODE_NOFIT ? TILE : rmsd_batch_tile_side(plan->variant);
+    int tr = TILE, tc = TILE;
+    if (plan->mode != RMSD_MODE_NOFIT) rmsd_batch_tile_shape(plan->prec, plan->variant, &tr, &tc);
+    info[1] = tr;
     int64_t nt = 0;
     if (plan->mode == RMSD_MODE_NOFIT)
       for (int k : plan->nofit_ntiles) nt += k;

@@ -340,10 +340,10 @@ hipError_t rmsd_launch_pairwise(int prec, const PairArgs& a, hipStream_t s) {
   }
 }
 
-// Batched launches.  Variant v: tile shape v / 2 (shape 0: 64 x 64 pair tiles,
-// shape 1: 32 x 32 pair tiles, with the same per-wave tile: FP64 waves own
-// 16 x 32 pairs, FP32 / BF16 waves 32 x 32), grid form for even v, persistent
-// form for odd v.  The tile list must be built for the shape's tile side.
+// Batched launches.  Variant v: tile shape v / 2 (shape 0: the single-matrix
+// tiles, 64 x 64 for FP64 and 64 x 96 for FP32 / BF16; shape 1: 32 x 32 pair
+// tiles of one or two waves), grid form for even v, persistent form for odd v.
+// The tile list must be built for the shape (rmsd_batch_tile_shape).
 template <int PREC, int WR, int WC, int NWR, int NWC>
 static hipError_t launch_batch_shape(const BatchArgs& b, bool persist, hipStream_t s) {
   constexpr int NT = 64 * NWR * NWC;
@@ -374,7 +374,14 @@ static hipError_t launch_batch_shape(const BatchArgs& b, bool persist, hipStream
 
 int rmsd_batch_variant_count() { return 4; }
 
-int rmsd_batch_tile_side(int variant) { return (variant >> 1) == 0 ? 64 : 32; }
+void rmsd_batch_tile_shape(int prec, int variant, int* rows, int* cols) {
+  if ((variant >> 1) == 0) {
+    *rows = 64;
+    *cols = rmsd_pairwise_tile_cols(prec);
+  } else {
+    *rows = *cols = 32;
+  }
+}
 
 hipError_t rmsd_launch_batch_reset(const BatchArgs& b, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, b.counters, b.nq, 0);
@@ -390,10 +397,10 @@ hipError_t rmsd_launch_pairwise_batch(int prec, int variant, const BatchArgs& b,
       return shape == 0 ? launch_batch_shape<PREC_F64, 1, 2, 4, 2>(b, persist, s)
                         : launch_batch_shape<PREC_F64, 1, 2, 2, 1>(b, persist, s);
     case PREC_F32:
-      return shape == 0 ? launch_batch_shape<PREC_F32, 2, 2, 2, 2>(b, persist, s)
+      return shape == 0 ? launch_batch_shape<PREC_F32, 2, 3, 2, 2>(b, persist, s)
                         : launch_batch_shape<PREC_F32, 2, 2, 1, 1>(b, persist, s);
     case PREC_BF16X3:
-      return shape == 0 ? launch_batch_shape<PREC_BF16X3, 2, 2, 2, 2>(b, persist, s)
+      return shape == 0 ? launch_batch_shape<PREC_BF16X3, 2, 3, 2, 2>(b, persist, s)
                         : launch_batch_shape<PREC_BF16X3, 2, 2, 1, 1>(b, persist, s);
     default: return hipErrorInvalidValue;
   }

@@ -144,7 +144,7 @@ int rmsd_pairwise_tile_cols(int prec);
 int rmsd_autocorr_chunks(int n, int* rows);
 hipError_t rmsd_launch_autocorr(const float* tri, int n, double* part, double* ct, hipStream_t s);
 int rmsd_batch_variant_count();
-int rmsd_batch_tile_side(int variant);
+void rmsd_batch_tile_shape(int prec, int variant, int* rows, int* cols);
 hipError_t rmsd_launch_batch_reset(const BatchArgs& b, hipStream_t s);
 hipError_t rmsd_launch_pairwise_batch(int prec, int variant, const BatchArgs& b, hipStream_t s);
 hipError_t rmsd_launch_pair_nofit_tiles(const FrameSetDev& A, const FrameSetDev& B,

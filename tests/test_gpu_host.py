@@ -57,6 +57,16 @@ def test_2drms_outputs(hs, golden, tmp_path, cmd, save):
     assert text(out) == expected(save)
 
 
+# Test_2DRMS test 3: the same frames as a reference trajectory give the full
+# matrix, printed like the triangle.
+def test_2drms_reftraj(hs, golden, tmp_path):
+    g = load_tz2(hs, golden)
+    hs.add_coords("ref", g["frames"].copy(), 223, g["mass"])
+    out = str(tmp_path / "rmsd1.dat")
+    hs.run(f"2drms crdset tz2 :3-7 rmsout {out} reftraj ref")
+    assert text(out) == expected("Test_2DRMS__rmsd.dat.save")
+
+
 # Test_2DRMS test 6: 'rms first :2-12@CA' rewrites the frames, then a no-fit matrix.
 def test_2drms_nofit_after_fit(hs, golden, tmp_path):
     load_tz2(hs, golden)
