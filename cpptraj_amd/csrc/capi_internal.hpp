@@ -1,0 +1,75 @@
+// capi_internal.hpp -- state and helpers shared by the C ABI translation
+// units (capi.hip: contexts, matrices, fit; capi_grad.hip: gradients;
+// capi_batch.hip: batched plans).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "cpptraj_rmsd.h"
+#include "rmsd_kernels.hpp"
+
+namespace rmsd_capi {
+
+constexpr int TILE = 64;      // frames per tile side (all pairwise kernels)
+constexpr int GROUP = 8;      // tile rows per band of the grouped schedule
+constexpr int NXCD = 8;       // MI355X: 8 XCDs, workgroups dealt round-robin
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct TileSchedule {
+  int nbr = -1, nbc = -1, tri = -1, tc = -1;
+  int2* dev = nullptr;
+  int ntiles = 0;
+};
+
+}  // namespace rmsd_capi
+
+struct rmsd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::map<std::string, rmsd_capi::DevBuf> bufs;
+  rmsd_capi::TileSchedule sched;
+  // Stage timing: events recorded on the context's stream around the per-frame
+  // stage, the main kernel and the epilogue stage of the last call.
+  bool profiling = false;
+  bool events_made = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  int marks = 0;
+  // Batch plans bound to this context; freed with it if still alive.
+  std::set<rmsd_batch*> plans;
+};
+
+namespace rmsd_capi {
+
+// Records an error message on the context and returns `code`.
+int fail(rmsd_ctx* c, int code, const std::string& msg);
+int hip_fail(rmsd_ctx* c, hipError_t e, const char* where);
+
+#define HIP_TRY(ctx, expr)                                   \
+  do {                                                       \
+    hipError_t e_ = (expr);                                  \
+    if (e_ != hipSuccess) return rmsd_capi::hip_fail((ctx), e_, #expr); \
+  } while (0)
+
+// Stage-timing event k on the context's stream (when profiling is on).
+void mark(rmsd_ctx* c, int k);
+// Grow-only named scratch buffer of the context.
+void* scratch(rmsd_ctx* c, const char* name, size_t bytes, int* status);
+int check_frames(rmsd_ctx* c, const rmsd_frames* f, const char* what);
+FrameSetDev to_dev(const rmsd_frames& f);
+std::vector<int2> grouped_tiles(int nbr, int nbc, int tri, int tr = TILE, int tc = TILE);
+int64_t round_up(int64_t x, int64_t m);
+// Frees a batch plan (capi_batch.hip); rmsd_ctx_destroy frees leftovers.
+void batch_free(rmsd_batch* b);
+
+}  // namespace rmsd_capi
