@@ -121,6 +121,12 @@ def _optional(L):
     if hasattr(L, "rmsd_pairwise_backward"):
         L.rmsd_pairwise_backward.argtypes = [P, FP, FP, C.c_int, P, P, P]
         L.rmsd_pairwise_backward.restype = C.c_int
+    if hasattr(L, "rmsd_perres_nofit"):
+        IP = C.POINTER(C.c_int32)
+        DP = C.POINTER(C.c_double)
+        L.rmsd_perres_nofit.argtypes = [P, FP, FP, C.c_int, C.c_int, IP, IP, IP, DP, DP,
+                                        C.c_int, C.c_int, C.c_int, P, P, P]
+        L.rmsd_perres_nofit.restype = C.c_int
 
 
 def _ptr(t):
@@ -129,6 +135,10 @@ def _ptr(t):
 
 def frames_desc(xyz, sel=None, mass=None, fidx=None, soa=False, nsel=None, natom=None):
     """Builds an rmsd_frames for a CUDA tensor.
+
+    The descriptor holds raw device pointers, not references: the caller keeps
+    xyz, sel, mass and fidx alive until the library call that uses it has been
+    enqueued (a freed tensor's block can be handed to the next allocation).
 
     xyz: AoS (F, natom*3) or (F, natom, 3) float32, or SoA (F, 3, ld) with soa=True
          (natom atoms used of each padded row; default ld).
@@ -316,6 +326,44 @@ def rmsd_fit(frames, ref_frames=None, ref_index=0, mask=None, mass=None, ref_mas
     tv = torch.empty((F, 6), dtype=torch.float64, device=dev) if save_vectors else None
     ctx.fit_desc(d, r, ref_index, fit, use_mass, MODIFY[modify], rmsd, U, tv)
     return rmsd, U, tv
+
+
+def rmsd_perres(frames, res_off, tgt_atoms, ref_frames=None, ref_index=0, ref_atoms=None,
+                tmass=None, rmass=None, use_mass=False, center=False, modify="nomod", U=None,
+                tvec=None, soa=False, natom=None, ctx: Context | None = None):
+    """Per-residue no-fit RMSD of the rmsd action ('perres').
+
+    res_off (nres+1), tgt_atoms / ref_atoms (res_off[-1] atom indices) and the
+    per-entry masses tmass / rmass are host arrays (numpy or lists).  modify
+    'rottrans' / 'norotate' applies the fit's U and tvec (CUDA float64 tensors
+    from rmsd_fit with modify='nomod') to the target in FP64 first.  Returns a
+    (nframes, nres) float64 CUDA tensor."""
+    import numpy as np
+    import torch
+
+    ctx = ctx or _ctx(frames.device.index or 0)
+    d = frames_desc(frames, soa=soa, natom=natom)
+    r = frames_desc(frames if ref_frames is None else ref_frames, soa=soa, natom=natom)
+    off = np.ascontiguousarray(res_off, dtype=np.int32)
+    ta = np.ascontiguousarray(tgt_atoms, dtype=np.int32)
+    ra = ta if ref_atoms is None else np.ascontiguousarray(ref_atoms, dtype=np.int32)
+    tm = None if tmass is None else np.ascontiguousarray(tmass, dtype=np.float64)
+    rm = tm if rmass is None else np.ascontiguousarray(rmass, dtype=np.float64)
+    nres = len(off) - 1
+    out = torch.empty((d.nframes, max(nres, 0)), dtype=torch.float64, device=frames.device)
+    IP = C.POINTER(C.c_int32)
+    DP = C.POINTER(C.c_double)
+
+    def cp(a, t):
+        return None if a is None else a.ctypes.data_as(t)
+
+    Ut = None if U is None else U.contiguous()
+    tvt = None if tvec is None else tvec.contiguous()
+    ctx.check(ctx._L.rmsd_perres_nofit(ctx.handle, C.byref(d), C.byref(r), ref_index, nres,
+                                       cp(off, IP), cp(ta, IP), cp(ra, IP), cp(tm, DP),
+                                       cp(rm, DP), int(use_mass), int(center), MODIFY[modify],
+                                       _ptr(Ut), _ptr(tvt), _ptr(out)))
+    return out
 
 
 class Batch:

@@ -32,8 +32,10 @@ def fit_backward(frames, ref, grad_rmsd, mass=None, use_mass=False, ctx: Context
     ctx = ctx or _ctx(frames.device)
     L = load_library()
     F, n = frames.shape[0], frames.shape[1]
-    d = frames_desc(frames.contiguous(), None, mass)
-    r = frames_desc(ref.contiguous(), None, mass)
+    # The descriptors hold raw pointers: keep the contiguous copies alive.
+    fc, rc = frames.contiguous(), ref.contiguous()
+    d = frames_desc(fc, None, mass)
+    r = frames_desc(rc, None, mass)
     g = torch.empty((F, n, 3), dtype=torch.float64, device=frames.device)
     gr = grad_rmsd.to(torch.float64).contiguous()
     ctx.check(L.rmsd_fit_backward(ctx.handle, C.byref(d), C.byref(r), 0, int(use_mass),
@@ -45,8 +47,9 @@ def pairwise_backward(A, B, grad_out, mass=None, use_mass=False, ctx: Context | 
     """FP64 gradients of sum_ij G_ij RMSD(A_i, B_j) with respect to A and B."""
     ctx = ctx or _ctx(A.device)
     L = load_library()
-    da = frames_desc(A.contiguous(), None, mass)
-    db = frames_desc(B.contiguous(), None, mass)
+    Ac, Bc = A.contiguous(), B.contiguous()
+    da = frames_desc(Ac, None, mass)
+    db = frames_desc(Bc, None, mass)
     ga = torch.empty((A.shape[0], A.shape[1], 3), dtype=torch.float64, device=A.device)
     gb = torch.empty((B.shape[0], B.shape[1], 3), dtype=torch.float64, device=B.device)
     G = grad_out.to(torch.float64).contiguous()
@@ -63,8 +66,9 @@ class RmsdToReference(torch.autograd.Function):
         if frames.dim() != 3 or ref.dim() != 3 or ref.shape[0] != 1:
             raise RmsdError("frames (F, n, 3) and ref (1, n, 3) expected")
         ctx = _ctx(frames.device)
-        d = frames_desc(frames.contiguous(), None, mass)
-        r = frames_desc(ref.contiguous(), None, mass)
+        fc, rc = frames.contiguous(), ref.contiguous()
+        d = frames_desc(fc, None, mass)
+        r = frames_desc(rc, None, mass)
         out = torch.empty(frames.shape[0], dtype=torch.float64, device=frames.device)
         ctx.fit_desc(d, r, 0, True, use_mass, ca.MODIFY["nomod"], out)
         fctx.save_for_backward(frames, ref)
@@ -84,8 +88,9 @@ class PairwiseRmsd(torch.autograd.Function):
     def forward(fctx, A, B, mass=None, use_mass=False):
         import cpptraj_amd as ca
         ctx = _ctx(A.device)
-        da = frames_desc(A.contiguous(), None, mass)
-        db = frames_desc(B.contiguous(), None, mass)
+        Ac, Bc = A.contiguous(), B.contiguous()
+        da = frames_desc(Ac, None, mass)
+        db = frames_desc(Bc, None, mass)
         out = torch.empty((A.shape[0], B.shape[0]), dtype=torch.float32, device=A.device)
         ctx.pairwise_desc(da, db, ca.MODES["fit"], use_mass, ca.PRECISIONS["fp64"], out)
         fctx.save_for_backward(A, B)

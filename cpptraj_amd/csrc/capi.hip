@@ -507,4 +507,152 @@ int rmsd_transform(rmsd_ctx* ctx, const rmsd_frames* frames, const int32_t* atom
   return RMSD_OK;
 }
 
+// Per-residue no-fit RMSD.  The residue table is validated here, cut into
+// pieces whose LDS image fits one workgroup, and uploaded as one blob:
+//   res_off (nres+1 int) | tgt (L int) | ref (L int) | pieces (P int4) |
+//   tmass (L double) | rmass (L double)
+int rmsd_perres_nofit(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* ref,
+                      int ref_frame, int nres, const int32_t* res_off, const int32_t* tgt_atoms,
+                      const int32_t* ref_atoms, const double* tmass, const double* rmass,
+                      int use_mass, int center, int modify, const double* U, const double* tvec,
+                      double* out) {
+  if (!ctx) return RMSD_ERR_ARG;
+  int st;
+  if (!frames || !ref) return fail(ctx, RMSD_ERR_ARG, "frames or ref is NULL");
+  if (frames->nframes < 0 || frames->natom < 1 || (!frames->xyz && frames->nframes > 0))
+    return fail(ctx, RMSD_ERR_ARG, "frames: need nframes >= 0, natom >= 1 and xyz");
+  if (ref->natom < 1 || !ref->xyz) return fail(ctx, RMSD_ERR_ARG, "ref: need natom >= 1 and xyz");
+  for (const rmsd_frames* f : {frames, ref}) {
+    if (f->ld && f->ld < f->natom) return fail(ctx, RMSD_ERR_ARG, "ld < natom");
+    const int64_t need = f->ld ? (2 * f->ld + f->natom) : 3 * (int64_t)f->natom;
+    if (f->nframes > 1 && f->stride < need)
+      return fail(ctx, RMSD_ERR_ARG, "frame stride smaller than one frame");
+  }
+  if (ref_frame < 0 || ref_frame >= ref->nframes)
+    return fail(ctx, RMSD_ERR_ARG, "reference frame out of range");
+  if (nres < 0 || (nres > 0 && (!res_off || !tgt_atoms || !ref_atoms)))
+    return fail(ctx, RMSD_ERR_ARG, "residue table incomplete");
+  if (use_mass && (!tmass || !rmass))
+    return fail(ctx, RMSD_ERR_ARG, "use_mass needs target and reference masses");
+  if (modify < RMSD_MODIFY_NONE || modify > RMSD_MODIFY_TRANS)
+    return fail(ctx, RMSD_ERR_ARG, "unknown modify mode");
+  if (modify != RMSD_MODIFY_NONE && (!tvec || (modify == RMSD_MODIFY_ROT_TRANS && !U)))
+    return fail(ctx, RMSD_ERR_ARG, "modify needs tvec (and U for ROT_TRANS)");
+  if (nres == 0 || frames->nframes == 0) return RMSD_OK;
+  if (!out) return fail(ctx, RMSD_ERR_ARG, "out is NULL");
+  if (res_off[0] != 0) return fail(ctx, RMSD_ERR_ARG, "res_off[0] must be 0");
+  for (int r = 0; r < nres; ++r)
+    if (res_off[r + 1] < res_off[r]) return fail(ctx, RMSD_ERR_ARG, "res_off is decreasing");
+  const int L = res_off[nres];
+  for (int k = 0; k < L; ++k) {
+    if (tgt_atoms[k] < 0 || tgt_atoms[k] >= frames->natom)
+      return fail(ctx, RMSD_ERR_ARG, "target atom index out of range");
+    if (ref_atoms[k] < 0 || ref_atoms[k] >= ref->natom)
+      return fail(ctx, RMSD_ERR_ARG, "reference atom index out of range");
+  }
+  // Pieces: runs of whole residues that fit PERRES_LDS_PIECE at one frame.  A
+  // piece stages its target atom span, or gathers its own atoms when the span
+  // is more than twice their number (lo = -1); a residue that would switch a
+  // run from one form to the other starts a new piece.
+  std::vector<int4> pieces;
+  auto staged = [](int span, int ne) { return span <= 2 * ne + 64 ? span : -1; };
+  for (int r = 0; r < nres;) {
+    int lo = INT32_MAX, hi = -1, r1 = r;
+    bool span_form = true;
+    while (r1 < nres) {
+      int l2 = lo, h2 = hi;
+      for (int k = res_off[r1]; k < res_off[r1 + 1]; ++k) {
+        l2 = std::min(l2, tgt_atoms[k]);
+        h2 = std::max(h2, tgt_atoms[k] + 1);
+      }
+      const int ne = res_off[r1 + 1] - res_off[r];
+      const int span = h2 > l2 ? h2 - l2 : 0;
+      const bool sf = staged(span, ne) >= 0;
+      const size_t b = rmsd_perres_lds_bytes(r1 + 1 - r, ne, sf ? span : ne, 1);
+      if (r1 > r && (b > PERRES_LDS_PIECE || sf != span_form)) break;
+      if (r1 == r && b > PERRES_LDS_MAX)
+        return fail(ctx, RMSD_ERR_ARG, "residue " + std::to_string(r) + " has too many atoms (" +
+                                           std::to_string(ne) + ") for per-residue RMSD");
+      lo = l2;
+      hi = h2;
+      span_form = sf;
+      ++r1;
+    }
+    if (hi < 0) lo = hi = 0;  // residues without atoms
+    pieces.push_back(span_form ? make_int4(r, r1, lo, hi) : make_int4(r, r1, -1, -1));
+    r = r1;
+  }
+  // Frames per batch: enough (frame, residue) items for the workgroup, within
+  // the piece budget (one frame always fits).
+  int maxnr = 1;
+  for (const int4& p : pieces) maxnr = std::max(maxnr, p.y - p.x);
+  int nfb = std::max(1, PERRES_THREADS / maxnr);
+  auto lds_for = [&](int nf) {
+    size_t m = 0;
+    for (const int4& p : pieces)
+      m = std::max(m, rmsd_perres_lds_bytes(p.y - p.x, res_off[p.y] - res_off[p.x],
+                                            p.z < 0 ? res_off[p.y] - res_off[p.x] : p.w - p.z, nf));
+    return m;
+  };
+  if (pieces.size() > 65535) return fail(ctx, RMSD_ERR_ARG, "residue table too large");
+  while (nfb > 1 && lds_for(nfb) > PERRES_LDS_PIECE) --nfb;
+  const size_t lds = lds_for(nfb);
+  // Table blob (16-byte aligned sections).
+  const size_t o_off = 0;
+  const size_t o_tgt = round_up(o_off + sizeof(int32_t) * (nres + 1), 16);
+  const size_t o_ref = round_up(o_tgt + sizeof(int32_t) * L, 16);
+  const size_t o_pc = round_up(o_ref + sizeof(int32_t) * L, 16);
+  const size_t o_tm = round_up(o_pc + sizeof(int4) * pieces.size(), 16);
+  const size_t o_rm = round_up(o_tm + (use_mass ? sizeof(double) * L : 0), 16);
+  const size_t bytes = round_up(o_rm + (use_mass ? sizeof(double) * L : 0), 16);
+  std::vector<unsigned char> blob(bytes, 0);
+  memcpy(blob.data() + o_off, res_off, sizeof(int32_t) * (nres + 1));
+  memcpy(blob.data() + o_tgt, tgt_atoms, sizeof(int32_t) * L);
+  memcpy(blob.data() + o_ref, ref_atoms, sizeof(int32_t) * L);
+  memcpy(blob.data() + o_pc, pieces.data(), sizeof(int4) * pieces.size());
+  if (use_mass) {
+    memcpy(blob.data() + o_tm, tmass, sizeof(double) * L);
+    memcpy(blob.data() + o_rm, rmass, sizeof(double) * L);
+  }
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  unsigned char* dtab = (unsigned char*)scratch(ctx, "perres.table", bytes, &st);
+  if (!dtab) return st;
+  if (blob != ctx->perres_blob) {
+    // The previous table may still be in flight from the host image.
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->perres_blob.swap(blob);
+    HIP_TRY(ctx, hipMemcpyAsync(dtab, ctx->perres_blob.data(), bytes, hipMemcpyHostToDevice,
+                                ctx->stream));
+  }
+  PerresArgs a;
+  a.xyz = frames->xyz;
+  a.stride = frames->stride;
+  a.ld = frames->ld;
+  a.fidx = frames->fidx;
+  a.nframes = frames->nframes;
+  a.ref = to_dev(*ref);
+  a.ref_frame = ref_frame;
+  a.nres = nres;
+  a.res_off = (const int32_t*)(dtab + o_off);
+  a.tgt_atoms = (const int32_t*)(dtab + o_tgt);
+  a.ref_atoms = (const int32_t*)(dtab + o_ref);
+  a.tmass = use_mass ? (const double*)(dtab + o_tm) : nullptr;
+  a.rmass = use_mass ? (const double*)(dtab + o_rm) : nullptr;
+  a.use_mass = use_mass ? 1 : 0;
+  a.center = center ? 1 : 0;
+  a.U = modify == RMSD_MODIFY_ROT_TRANS ? U : nullptr;
+  a.tvec = modify != RMSD_MODIFY_NONE ? tvec : nullptr;
+  a.modify = modify == RMSD_MODIFY_ROT_TRANS ? 1 : (modify == RMSD_MODIFY_TRANS ? 2 : 0);
+  a.pieces = (const int4*)(dtab + o_pc);
+  a.nfb = nfb;
+  a.out = out;
+  ctx->marks = 0;
+  mark(ctx, 0);
+  mark(ctx, 1);
+  HIP_TRY(ctx, rmsd_launch_perres(a, (int)pieces.size(), lds, ctx->stream));
+  mark(ctx, 2);
+  mark(ctx, 3);
+  return RMSD_OK;
+}
+
 }  // extern "C"

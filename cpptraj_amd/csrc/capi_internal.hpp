@@ -47,6 +47,9 @@ struct rmsd_ctx {
   int marks = 0;
   // Batch plans bound to this context; freed with it if still alive.
   std::set<rmsd_batch*> plans;
+  // Per-residue tables of the last rmsd_perres_nofit call (host image of the
+  // device copy in scratch "perres.table"), re-uploaded only when they change.
+  std::vector<unsigned char> perres_blob;
 };
 
 namespace rmsd_capi {

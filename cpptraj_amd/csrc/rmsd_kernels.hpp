@@ -132,6 +132,38 @@ struct ApplyArgs {
   const double* tvec = nullptr;    // ... and t1, t2 (6 per frame)
 };
 
+// Per-residue no-fit RMSD (k_perres.hip).  Residue r owns entries
+// res_off[r] .. res_off[r+1] of tgt_atoms / ref_atoms / tmass / rmass.  pieces:
+// {first residue, end residue, lowest target atom, highest target atom + 1},
+// or {first, end, -1, -1} for a piece that gathers its own atoms; a workgroup
+// stages `nfb` frames' atoms at a time.
+struct PerresArgs {
+  const float* xyz;
+  int64_t stride, ld;
+  const int32_t* fidx;
+  int64_t nframes;
+  FrameSetDev ref;        // reference set (xyz, stride, ld, fidx used) ...
+  int ref_frame;          // ... and its frame
+  int nres;
+  const int32_t* res_off;
+  const int32_t* tgt_atoms;
+  const int32_t* ref_atoms;
+  const double* tmass;
+  const double* rmass;
+  int use_mass, center;
+  const double* U;        // 9 per frame (modify 1)
+  const double* tvec;     // 6 per frame (modify 1, 2)
+  int modify;             // 0: frames as stored; 1: x' = U (x + t) + r; 2: x + (t + r)
+  const int4* pieces;
+  int nfb;
+  double* out;            // out[f * nres + r]
+};
+constexpr int PERRES_THREADS = 256;
+constexpr size_t PERRES_LDS_PIECE = size_t(64) << 10;  // piece budget (LDS bytes at nfb = 1)
+constexpr size_t PERRES_LDS_MAX = size_t(160) << 10;   // one oversized residue alone
+size_t rmsd_perres_lds_bytes(int nres_piece, int nentries, int span, int nfb);
+hipError_t rmsd_launch_perres(const PerresArgs& a, int npieces, size_t lds_bytes, hipStream_t s);
+
 hipError_t rmsd_launch_setup(const SetupArgs& a, hipStream_t s);
 hipError_t rmsd_launch_prep(const PrepArgs& a, hipStream_t s);
 // Batched forms: one workgroup per set / per padded frame of every set.

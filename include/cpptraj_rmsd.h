@@ -171,6 +171,33 @@ int rmsd_fit_to_ref(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames*
 int rmsd_transform(rmsd_ctx* ctx, const rmsd_frames* frames, const int32_t* atoms, int natoms,
                    const double* U, const double* tvec, int mode);
 
+/* Per-residue no-fit RMSD ('rmsd ... perres').
+ * Replaces the per-residue loop of Action_Rmsd::DoAction (src/Action_Rmsd.cpp:
+ * 398-413) over the residue masks of Action_Rmsd::perResSetup (:197-315),
+ * which the caller resolves: residue r compares the target atoms
+ * tgt_atoms[res_off[r] .. res_off[r+1]) of every frame f of `frames` (fidx
+ * honoured; sel and mass ignored) with the reference atoms ref_atoms[...] of
+ * frame ref_frame of `ref`, uncentred (ReferenceAction::CurrentReference,
+ * src/ReferenceAction.h:36), by Frame::RMSD_NoFit (src/Frame.cpp:1279-1307)
+ * with the target masses.
+ *   res_off, tgt_atoms, ref_atoms, tmass, rmass   HOST arrays (nres+1 ints;
+ *            res_off[nres] ints or doubles), res_off[0] = 0; masses are needed
+ *            with use_mass only.  Copied to the device, once per new table.
+ *   center   'perrescenter': target and reference residue each moved to its
+ *            own centre first (Frame::CenterOnOrigin, src/Frame.cpp:1043-1055)
+ *   modify   the target as the fit's coordinate update leaves it, in FP64,
+ *            before any rounding to FP32: RMSD_MODIFY_ROT_TRANS with U and tvec
+ *            of rmsd_fit_to_ref (Frame::Trans_Rot_Trans), RMSD_MODIFY_TRANS
+ *            with tvec, RMSD_MODIFY_NONE for the frames as stored.  A caller
+ *            fits with RMSD_MODIFY_NONE, calls this, then rmsd_transform.
+ *   out      device, nframes * nres doubles: out[f*nres + r]; -1 for a residue
+ *            of zero total mass (Constants::SMALL). */
+int rmsd_perres_nofit(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* ref,
+                      int ref_frame, int nres, const int32_t* res_off, const int32_t* tgt_atoms,
+                      const int32_t* ref_atoms, const double* tmass, const double* rmass,
+                      int use_mass, int center, int modify, const double* U, const double* tvec,
+                      double* out);
+
 /* Gradient of the fit: no counterpart in the reference (cpptraj has no RMSD
  * derivative).  For loss = sum_f grad_rmsd[f] * RMSD_f of rmsd_fit_to_ref
  * (fit, same frames / reference / masses), writes d loss / d x for the selected

@@ -94,6 +94,9 @@ class Oracle:
         L.orc_rms2d_rows.argtypes = [_fp, C.c_int64, C.c_int, _ip, C.c_int, _dp, C.c_int,
                                      C.c_int, _ip, C.c_int, _fp, C.c_int]
         L.orc_rms2d_rows.restype = C.c_int
+        L.orc_perres.argtypes = [_fp, C.c_int64, C.c_int, _fp, C.c_int, _ip, _ip, _ip, _dp,
+                                 _dp, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp]
+        L.orc_perres.restype = C.c_int
 
     # -- per-frame routines -------------------------------------------------
     def center_on_origin(self, xyz, mass=None, use_mass=False):
@@ -230,6 +233,27 @@ class Oracle:
             raise MemoryError("orc_fit_frames failed")
         return rmsd, U.reshape(nf, 3, 3), tv, fo
 
+    def perres(self, frames, ref, res_off, tgt_atoms, ref_atoms=None, tmass=None,
+               rmass=None, use_mass=False, center=False, U=None, tvec=None, modify=0):
+        """Per-residue no-fit RMSD (Action_Rmsd 'perres'): (nframes, nres) doubles."""
+        fr = _c32(frames)
+        nf = fr.shape[0]
+        off = _ci(res_off)
+        nres = len(off) - 1
+        ta = _ci(tgt_atoms)
+        ra = _ci(ref_atoms) if ref_atoms is not None else ta
+        tm = _c64(tmass) if tmass is not None else None
+        rm = _c64(rmass) if rmass is not None else tm
+        Uc = _c64(U).reshape(nf, 9) if U is not None else None
+        tv = _c64(tvec).reshape(nf, 6) if tvec is not None else None
+        out = np.zeros((nf, nres))
+        err = self.L.orc_perres(_f(fr), fr.shape[1], nf, _f(_c32(ref).reshape(-1)), nres,
+                                _i(off), _i(ta), _i(ra), _d(tm), _d(rm), int(use_mass),
+                                int(center), _d(Uc), _d(tv), modify, _d(out))
+        if err:
+            raise MemoryError("orc_perres failed")
+        return out
+
 
 class RefLib:
     """The reference's own sources compiled in place (oracle/_ref)."""
@@ -263,6 +287,9 @@ class RefLib:
                                       _dp, _dp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp,
                                       _fp, C.c_int]
         L.cref_fit_frames.restype = C.c_int
+        L.cref_perres.argtypes = [_fp, C.c_int64, C.c_int, C.c_int, _fp, C.c_int, C.c_int, _ip,
+                                  _ip, _ip, _dp, _dp, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp]
+        L.cref_perres.restype = C.c_int
 
     def rmsd_centered_ref(self, tgt, ref, mass=None, ref_mass=None, use_mass=False):
         t = _c64(tgt).copy()
@@ -373,6 +400,25 @@ class RefLib:
                                int(use_mass), modify, _d(rmsd), _d(U), _d(tv), _f(fo),
                                nthreads)
         return rmsd, U.reshape(nf, 3, 3), tv, fo
+
+    def perres(self, frames, ref, res_off, tgt_atoms, ref_atoms=None, tmass=None,
+               rmass=None, use_mass=False, center=False, U=None, tvec=None, modify=0):
+        fr = _c32(frames)
+        nf = fr.shape[0]
+        rf = _c32(ref).reshape(-1)
+        off = _ci(res_off)
+        nres = len(off) - 1
+        ta = _ci(tgt_atoms)
+        ra = _ci(ref_atoms) if ref_atoms is not None else ta
+        tm = _c64(tmass) if tmass is not None else None
+        rm = _c64(rmass) if rmass is not None else tm
+        Uc = _c64(U).reshape(nf, 9) if U is not None else None
+        tv = _c64(tvec).reshape(nf, 6) if tvec is not None else None
+        out = np.zeros((nf, nres))
+        self.L.cref_perres(_f(fr), fr.shape[1], nf, fr.shape[1] // 3, _f(rf), len(rf) // 3,
+                           nres, _i(off), _i(ta), _i(ra), _d(tm), _d(rm), int(use_mass),
+                           int(center), _d(Uc), _d(tv), modify, _d(out))
+        return out
 
 
 def ref_available() -> bool:

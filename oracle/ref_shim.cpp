@@ -261,4 +261,50 @@ int cref_fit_frames(const float* frames, int64_t stride, int nframes, int natom,
   return 0;
 }
 
+// Per-residue loop of Action_Rmsd::DoAction (src/Action_Rmsd.cpp:398-413):
+// the frame after the fit's update (Frame::Trans_Rot_Trans / Translate in
+// double), each residue copied out with its own masses, optionally centred
+// (Frame::CenterOnOrigin on both), then Frame::RMSD_NoFit.  out[f*nres + r].
+int cref_perres(const float* frames, int64_t stride, int nframes, int natom, const float* ref,
+                int ref_natom, int nres, const int* res_off, const int* tgt_atoms,
+                const int* ref_atoms, const double* tmass, const double* rmass, int use_mass,
+                int center, const double* U, const double* tvec, int modify, double* out) {
+  std::vector<double> all(3 * (size_t)natom, 0.0);
+  Frame Full = make_frame(all.data(), 0, natom);
+  std::vector<double> rall(3 * (size_t)ref_natom, 0.0);
+  Frame RefFull = make_frame(rall.data(), 0, ref_natom);
+  frame_from_floats(RefFull, ref, 0, ref_natom);
+  for (int f = 0; f < nframes; ++f) {
+    frame_from_floats(Full, frames + (int64_t)f * stride, 0, natom);
+    if (modify && tvec) {
+      const double* tv = tvec + 6 * (int64_t)f;
+      Vec3 t1(tv[0], tv[1], tv[2]), t2(tv[3], tv[4], tv[5]);
+      if (modify == 1) {
+        Matrix_3x3 u(U + 9 * (int64_t)f);
+        Full.Trans_Rot_Trans(t1, u, t2);
+      } else {
+        t1 += t2;
+        Full.Translate(t1);
+      }
+    }
+    for (int r = 0; r < nres; ++r) {
+      const int b = res_off[r], n = res_off[r + 1] - b;
+      std::vector<double> tx(3 * (size_t)n), rx(3 * (size_t)n);
+      for (int k = 0; k < n; ++k)
+        for (int c = 0; c < 3; ++c) {
+          tx[3 * k + c] = Full.xAddress()[3 * tgt_atoms[b + k] + c];
+          rx[3 * k + c] = RefFull.xAddress()[3 * ref_atoms[b + k] + c];
+        }
+      Frame T = make_frame(tx.data(), tmass ? tmass + b : 0, n);
+      Frame R = make_frame(rx.data(), rmass ? rmass + b : 0, n);
+      if (center) {
+        T.CenterOnOrigin(use_mass != 0);
+        R.CenterOnOrigin(use_mass != 0);
+      }
+      out[(int64_t)f * nres + r] = T.RMSD_NoFit(R, use_mass != 0);
+    }
+  }
+  return 0;
+}
+
 }  // extern "C"

@@ -767,3 +767,69 @@ int orc_rms2d_rows(const float *frames, int64_t stride, int nframes, const int *
   free(off);
   return err;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Per-residue no-fit RMSD of Action_Rmsd (src/Action_Rmsd.cpp:398-413, masks
+ * from perResSetup :197-315).  For frame f and residue r the target atoms
+ * tgt_atoms[res_off[r] .. res_off[r+1]) of the frame, as the fit's coordinate
+ * update leaves it in double precision (modify 1: Frame::Trans_Rot_Trans,
+ * src/Frame.h:572-581, with U and tvec of the fit; modify 2: translate by
+ * Trans + refTrans; 0 or U == NULL: the frame as stored), are compared with
+ * the reference atoms ref_atoms[...] of the uncentred reference frame
+ * (ReferenceAction::CurrentReference, src/ReferenceAction.h:36): both sides
+ * optionally centred (perrescenter: Frame::CenterOnOrigin with the target's
+ * and the reference's own masses), then Frame::RMSD_NoFit with the target
+ * masses.  out[f * nres + r].                                                 */
+int orc_perres(const float *frames, int64_t stride, int nframes, const float *ref,
+               int nres, const int *res_off, const int *tgt_atoms, const int *ref_atoms,
+               const double *tmass, const double *rmass, int use_mass, int center,
+               const double *U, const double *tvec, int modify, double *out)
+{
+  int maxn = 0;
+  for (int r = 0; r < nres; ++r)
+    if (res_off[r + 1] - res_off[r] > maxn) maxn = res_off[r + 1] - res_off[r];
+  double *T = (double *)malloc(sizeof(double) * 3 * (size_t)(maxn > 0 ? maxn : 1));
+  double *R = (double *)malloc(sizeof(double) * 3 * (size_t)(maxn > 0 ? maxn : 1));
+  if (!T || !R) {
+    free(T);
+    free(R);
+    return 1;
+  }
+  for (int f = 0; f < nframes; ++f) {
+    const float *fr = frames + (int64_t)f * stride;
+    const double *Uf = U ? U + 9 * (int64_t)f : NULL;
+    const double *tv = tvec ? tvec + 6 * (int64_t)f : NULL;
+    for (int r = 0; r < nres; ++r) {
+      const int b = res_off[r], n = res_off[r + 1] - b;
+      for (int k = 0; k < n; ++k) {
+        const int a = tgt_atoms[b + k], q = ref_atoms[b + k];
+        double x = fr[3 * a], y = fr[3 * a + 1], z = fr[3 * a + 2];
+        if (Uf && tv && modify == 1) {
+          x += tv[0]; y += tv[1]; z += tv[2];
+          double rx = Uf[0] * x + Uf[1] * y + Uf[2] * z;
+          double ry = Uf[3] * x + Uf[4] * y + Uf[5] * z;
+          double rz = Uf[6] * x + Uf[7] * y + Uf[8] * z;
+          x = rx + tv[3];
+          y = ry + tv[4];
+          z = rz + tv[5];
+        } else if (tv && modify == 2) {
+          x += tv[0] + tv[3];
+          y += tv[1] + tv[4];
+          z += tv[2] + tv[5];
+        }
+        T[3 * k] = x; T[3 * k + 1] = y; T[3 * k + 2] = z;
+        R[3 * k] = ref[3 * q]; R[3 * k + 1] = ref[3 * q + 1]; R[3 * k + 2] = ref[3 * q + 2];
+      }
+      if (center) {
+        double c[3];
+        orc_center_on_origin(T, tmass ? tmass + b : NULL, n, use_mass, c);
+        orc_center_on_origin(R, rmass ? rmass + b : NULL, n, use_mass, c);
+      }
+      out[(int64_t)f * nres + r] =
+          n > 0 ? orc_rmsd_nofit(T, R, tmass ? tmass + b : NULL, n, use_mass) : -1.0;
+    }
+  }
+  free(T);
+  free(R);
+  return 0;
+}

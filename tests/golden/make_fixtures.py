@@ -23,9 +23,13 @@ Outputs (numpy .npz, no pickles):
                    'previous' RMSD series
   rgacc.npz        rGACC.full.nc frame 1, :1-4&!@H= atoms (9 identical frames ->
                    all-zero matrix)
-  text/            the expected text outputs themselves (Test_2DRMS and
-                   Test_RMSD .save files), for byte-level checks of the host
-                   layer's DataIO_Std writers
+  tz2_perres.npz   tz2.truncoct.nc solute atoms (13 residues, 223 atoms) of all
+                   10 frames, masses, residue first atoms and names, and the
+                   Test_PerResRMSD expected per-residue series, averages and
+                   the centred residue-1 series
+  text/            the expected text outputs themselves (Test_2DRMS,
+                   Test_RMSD and Test_PerResRMSD .save files), for byte-level
+                   checks of the host layer's DataIO_Std / DataIO_Grace writers
 
 The atom masks used by these tests are resolved here with a few lines of
 parm7 parsing (residue ranges, atom names, hydrogen exclusion); the general
@@ -132,6 +136,20 @@ def read_series(path, ncols=None):
     return np.array(rows, dtype=np.float64)
 
 
+def read_agr(path):
+    """Grace xy sets -> (legends, array (nsets, npoints))."""
+    legends, sets = [], []
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("@  s") and "legend" in line:
+                legends.append(line.split('"')[1])
+            elif line.startswith("@target"):
+                sets.append([])
+            elif not line.startswith("@") and line.strip():
+                sets[-1].append(float(line.split()[1]))
+    return legends, np.array(sets, dtype=np.float64)
+
+
 def read_crd(path, natom, nbox=0):
     """Amber ASCII trajectory (%8.3f, 10 per line); nbox trailing box values
     per frame are dropped."""
@@ -231,6 +249,25 @@ def main():
         frame=xg[0, sg, :].reshape(-1),
         expected=np.array(vals),
     )
+    # ---- Test_PerResRMSD (truncoct; rmsd :2-11 refindex 0 perres ...) ----------
+    P = os.path.join(T, "Test_PerResRMSD")
+    legends, perres = read_agr(os.path.join(P, "PerResRMSD.agr.save"))
+    _, center = read_agr(os.path.join(P, "center.agr.save"))
+    nsolute = len(solute)
+    ptr = [int(v) - 1 for v in read_parm7(os.path.join(T, "tz2.truncoct.parm7"))["RESIDUE_POINTER"]]
+    nres = int(np.count_nonzero(np.array(ptr) < nsolute))
+    np.savez_compressed(
+        os.path.join(OUT, "tz2_perres.npz"),
+        frames=xt[:, solute, :].reshape(xt.shape[0], -1),
+        mass=pt.mass[solute],
+        res_first=np.array(ptr[:nres], dtype=np.int32),
+        res_names=np.array(pt.resnames[:nres]),
+        legends=np.array(legends),
+        perres=perres,
+        perresavg=read_series(os.path.join(P, "perresavg.dat.save")),
+        center=center,
+    )
+
     # ---- expected text outputs, verbatim -------------------------------------------
     txt = os.path.join(OUT, "text")
     os.makedirs(txt, exist_ok=True)
@@ -239,7 +276,9 @@ def main():
                        ("Test_RMSD", ["rmsd.dat.save", "rmsd.mass.dat.save",
                                       "rmsd.reftraj.dat.save", "vecs.dat.save",
                                       "rmatrices.dat.save", "NoMod.dat.save",
-                                      "Previous.dat.save"])):
+                                      "Previous.dat.save"]),
+                      ("Test_PerResRMSD", ["PerResRMSD.agr.save", "perresavg.dat.save",
+                                           "center.agr.save"])):
         for n in names:
             with open(os.path.join(T, sub, n), "rb") as fi:
                 data = fi.read()

@@ -40,6 +40,13 @@ class OracleBackend:
                                          modify=modify, want_frames=True)
         return r, U, tv, fo
 
+    def perres(self, frames, fit_sel, res_off, atoms, center=False):
+        """rmsd <fit_sel> perres against frame 0: fit, then per-residue no-fit
+        RMSD of the frames as the fit's update leaves them."""
+        _, U, tv, _ = self.fit(frames, sel=fit_sel, modify=0)
+        return self.o.perres(frames, frames[0], res_off, atoms, U=U, tvec=tv, modify=1,
+                             center=center)
+
 
 class GpuBackend:
     def __init__(self, ctx):
@@ -79,6 +86,16 @@ class GpuBackend:
                                modify={0: "nomod", 1: "rottrans", 2: "norotate"}[modify],
                                save_matrices=True, save_vectors=True, ctx=self.ctx)
         return (r.cpu().numpy(), U.cpu().numpy(), tv.cpu().numpy(), X.cpu().numpy())
+
+    def perres(self, frames, fit_sel, res_off, atoms, center=False):
+        import cpptraj_amd as ca
+        t = self.t
+        X = self._dev(frames, t.float32)
+        _, U, tv = ca.rmsd_fit(X, None, 0, mask=self._dev(fit_sel, t.int32), modify="nomod",
+                               save_matrices=True, save_vectors=True, ctx=self.ctx)
+        out = ca.rmsd_perres(X, res_off, atoms, U=U, tvec=tv, modify="rottrans", center=center,
+                             ctx=self.ctx)
+        return out.cpu().numpy()
 
 
 @pytest.fixture(params=["oracle", pytest.param("gpu", marks=pytest.mark.gpu)])
@@ -185,3 +202,41 @@ def test_rmsd_nomod_series(golden, backend):
     r, _, _, f = backend.fit(d["frames"], modify=0)
     assert np.abs(r - d["nomod"]).max() <= HALF4
     assert np.array_equal(f, d["frames"])  # nomod leaves coordinates alone
+
+
+# ---------------------------------------------------------------------------------
+# Test_PerResRMSD: 'rmsd :2-11 refindex 0 perres' on tz2.truncoct (reference =
+# frame 1); per-residue no-fit RMSD of all 13 solute residues.
+def _perres_setup(d):
+    n = d["frames"].shape[1] // 3
+    off = np.append(d["res_first"], n).astype(np.int32)
+    fit_sel = np.arange(off[1], off[11], dtype=np.int32)  # :2-11
+    return off, fit_sel, np.arange(n, dtype=np.int32)
+
+
+def test_perres_series(golden, backend):
+    d = golden("tz2_perres")
+    off, fit_sel, atoms = _perres_setup(d)
+    P = backend.perres(d["frames"], fit_sel, off, atoms)
+    assert P.shape == (10, 13)
+    assert np.abs(P.T - d["perres"]).max() <= HALF4
+
+
+def test_perres_average_and_stdev(golden, backend):
+    # 'trajin ... 2 4' + 'perresavg': frames 2-4 against frame 1.
+    d = golden("tz2_perres")
+    off, fit_sel, atoms = _perres_setup(d)
+    X = np.concatenate([d["frames"][:1], d["frames"][1:4]])
+    P = backend.perres(X, fit_sel, off, atoms)[1:]
+    avg = P.mean(axis=0)
+    sd = np.sqrt(((P - avg) ** 2).mean(axis=0))  # DataSet_1D::Avg: population
+    assert np.abs(avg - d["perresavg"][:, 0]).max() <= HALF4
+    assert np.abs(sd - d["perresavg"][:, 1]).max() <= HALF4
+
+
+def test_perres_centered_residue_1(golden, backend):
+    # 'range 1 perrescenter'
+    d = golden("tz2_perres")
+    off, fit_sel, atoms = _perres_setup(d)
+    P = backend.perres(d["frames"], fit_sel, off[:2], atoms, center=True)
+    assert np.abs(P[:, 0] - d["center"][0]).max() <= HALF4

@@ -170,7 +170,8 @@ def test_triangle_past_2_31_elements(oracle, ctx, precision):
     F, n = 70000, 8
     X = synth.noisy(71, F, n, 0.7).reshape(F, -1)
     out = torch.empty(F * (F - 1) // 2, dtype=torch.float32, device="cuda")
-    ctx.pairwise_desc(ca.frames_desc(dev(X)), None, 0, False, ca.PRECISIONS[precision], out)
+    Xd = dev(X)  # the descriptor holds a raw pointer: keep the tensor alive
+    ctx.pairwise_desc(ca.frames_desc(Xd), None, 0, False, ca.PRECISIONS[precision], out)
     rows = np.array([0, 35000, 60000, F - 2], np.int32)
     exp = oracle.rms2d_rows(X, rows)
     starts = [int(r) * F - int(r) * (int(r) + 1) // 2 for r in rows]

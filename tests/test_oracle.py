@@ -232,3 +232,75 @@ def test_gradient_central_differences(oracle, use_mass):
     np.testing.assert_allclose(g, fd, atol=1e-7, rtol=1e-6)
     # translation invariance: the gradient sums to zero
     assert np.abs(g.sum(0)).max() < 1e-12
+
+
+# -- per-residue no-fit RMSD ('rmsd ... perres') ----------------------------------
+def _ragged_residues(rng, natom, nres, maxsize=30):
+    """Random residue table over atoms 0..natom-1: contiguous runs plus one
+    residue with scattered atoms (a perresmask-style selection)."""
+    sizes = rng.integers(1, maxsize, size=nres)
+    cuts = np.minimum(np.cumsum(sizes), natom)
+    first = np.concatenate([[0], cuts[:-1]])
+    atoms = [np.arange(a, b) for a, b in zip(first, cuts) if b > a]
+    atoms.append(np.sort(rng.choice(natom, size=7, replace=False)))
+    off = np.concatenate([[0], np.cumsum([len(a) for a in atoms])]).astype(np.int32)
+    return off, np.concatenate(atoms).astype(np.int32)
+
+
+@pytest.mark.parametrize("use_mass", [False, True])
+@pytest.mark.parametrize("center", [False, True])
+@pytest.mark.parametrize("modify", [0, 1, 2])
+def test_perres_bitexact(oracle, reflib, use_mass, center, modify):
+    rng = np.random.default_rng(13 + 4 * modify + 2 * center + use_mass)
+    natom, F = 150, 6
+    X = synth.noisy(int(rng.integers(1000)), F, natom, 0.6).reshape(F, -1).astype(np.float32)
+    ref = X[0] + rng.normal(size=3 * natom).astype(np.float32)
+    off, atoms = _ragged_residues(rng, natom, 12)
+    ref_atoms = rng.permutation(natom)[atoms].astype(np.int32)  # a different reference order
+    m = synth.masses(rng, natom)
+    tm, rm = m[atoms], synth.masses(rng, natom)[ref_atoms]
+    U = synth.rotations(rng, F).reshape(F, 9) if modify else None
+    tv = rng.uniform(-10, 10, size=(F, 6)) if modify else None
+    kw = dict(ref_atoms=ref_atoms, tmass=tm, rmass=rm, use_mass=use_mass, center=center, U=U,
+              tvec=tv, modify=modify)
+    a = oracle.perres(X, ref, off, atoms, **kw)
+    b = reflib.perres(X, ref, off, atoms, **kw)
+    assert same(a, b)
+
+
+@pytest.mark.parametrize("use_mass", [False, True])
+def test_perres_closed_forms(oracle, use_mass):
+    """One residue that is the fit mask itself: after the fit's update the
+    no-fit RMSD against the uncentred reference is the best-fit RMSD (the fit
+    superposes the target onto the reference where it lies); with centring
+    too.  Translate-only ('norotate') equals the centred no-fit value."""
+    rng = np.random.default_rng(21)
+    n, F = 60, 8
+    X = synth.noisy(5, F, n, 0.7).reshape(F, -1).astype(np.float32)
+    m = synth.masses(rng, n)
+    ref = X[0].reshape(-1, 3).astype(np.float64)
+    rc, rt = oracle.center_on_origin(ref, m, use_mass)
+    r, U, tv, _ = oracle.fit_frames(X, rc, rt, mass=m, use_mass=use_mass)
+    off = np.array([0, n], np.int32)
+    atoms = np.arange(n, dtype=np.int32)
+    kw = dict(tmass=m, rmass=m, use_mass=use_mass)
+    # Frame 0 is the reference itself: the direct sum gives ~0 where the
+    # Kabsch eigenvalue formula bottoms out near sqrt(eps * E0 / M) ~ 1e-7.
+    P = oracle.perres(X, X[0], off, atoms, U=U, tvec=tv, modify=1, **kw)[:, 0]
+    assert np.abs(P[1:] - r[1:]).max() < 1e-10 and P[0] < 1e-12 and r[0] < 1e-6
+    Pc = oracle.perres(X, X[0], off, atoms, U=U, tvec=tv, modify=1, center=True, **kw)[:, 0]
+    assert np.abs(Pc[1:] - r[1:]).max() < 1e-10 and Pc[0] < 1e-12
+    Pt = oracle.perres(X, X[0], off, atoms, U=U, tvec=tv, modify=2, **kw)[:, 0]
+    P0c = oracle.perres(X, X[0], off, atoms, center=True, **kw)[:, 0]
+    assert np.abs(Pt - P0c).max() < 1e-10
+    # a rigid translation of a residue: |d| without centring, 0 with it
+    d = np.array([1.5, -2.0, 0.5])
+    Y = (X[:1].reshape(-1, 3) + d).reshape(1, -1).astype(np.float32)
+    Q = oracle.perres(Y, X[0], off, atoms, **kw)[0, 0]
+    dd = Y[0].reshape(-1, 3).astype(np.float64) - X[0].reshape(-1, 3)
+    w = m if use_mass else np.ones(n)
+    assert abs(Q - np.sqrt((w * (dd ** 2).sum(1)).sum() / w.sum())) < 1e-12
+    assert oracle.perres(Y, X[0], off, atoms, center=True, **kw)[0, 0] < 1e-5
+    # zero total mass -> -1 (RMSD_NoFit's divide-by-zero sentinel)
+    z = oracle.perres(X, X[0], off, atoms, tmass=np.zeros(n), rmass=m, use_mass=True)
+    assert np.all(z == -1.0)
