@@ -69,6 +69,11 @@ def load_library():
         return _lib
     if not os.path.exists(_LIB_PATH):
         raise RmsdError(f"{_LIB_PATH} is missing: build it with python -m cpptraj_amd.build")
+    # One HIP runtime per process: torch ships its own libamdhip64 and asks for
+    # it by file name, so if this library were loaded first (binding the ROCm
+    # install's copy by soname) torch would load a second runtime next to it and
+    # find no device.  Loading torch first makes this library bind to torch's.
+    import torch  # noqa: F401
     L = C.CDLL(_LIB_PATH)
     P = C.c_void_p
     L.rmsd_ctx_create.argtypes = [C.POINTER(P), C.c_int, P]

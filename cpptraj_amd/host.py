@@ -53,11 +53,16 @@ def load_host():
     L.chost_write_series.argtypes = [C.c_char_p, I, C.POINTER(C.c_char_p), IP, IP, IP,
                                      C.POINTER(DP), I64]
     L.chost_write_cmatrix.argtypes = [C.c_char_p, FP, I64, I64, I, IP]
-    for f in ("chost_write_matrix", "chost_write_series", "chost_write_cmatrix"):
+    L.chost_set_residues.argtypes = [P, C.c_char_p, I, IP, C.POINTER(C.c_char_p), I, IP]
+    L.chost_write_grace.argtypes = [C.c_char_p, I, C.POINTER(C.c_char_p), C.POINTER(DP), I64, I]
+    L.chost_write_mesh.argtypes = [C.c_char_p, C.c_char_p, DP, I64, I, C.POINTER(C.c_char_p),
+                                   C.POINTER(DP)]
+    for f in ("chost_write_matrix", "chost_write_series", "chost_write_cmatrix",
+              "chost_write_grace", "chost_write_mesh"):
         getattr(L, f).restype = I
     for f in ("chost_create", "chost_add_coords", "chost_add_reference", "chost_add_mask",
               "chost_run", "chost_get_matrix", "chost_get_series", "chost_get_coords",
-              "chost_read_cmatrix"):
+              "chost_read_cmatrix", "chost_set_residues"):
         getattr(L, f).restype = I
     _hl = L
     return L
@@ -108,6 +113,18 @@ class HostSession:
         x = np.ascontiguousarray(frame, dtype=np.float32).reshape(-1)
         m = None if mass is None else np.ascontiguousarray(mass, dtype=np.float64)
         self._check(self._L.chost_add_reference(self._h, name.encode(), _fp(x), natom, _dp(m)))
+
+    def set_residues(self, name, first, names, solute=None):
+        """Residues of a set's topology: first atoms (0-based), names, and the
+        solute residues (0-based; default every residue)."""
+        f = np.ascontiguousarray(first, dtype=np.int32)
+        nm = (C.c_char_p * max(len(f), 1))(*[str(n).encode() for n in names])
+        sol = None if solute is None else np.ascontiguousarray(solute, dtype=np.int32)
+        IP = C.POINTER(C.c_int)
+        self._check(self._L.chost_set_residues(self._h, name.encode(), len(f),
+                                               f.ctypes.data_as(IP), nm,
+                                               0 if sol is None else len(sol),
+                                               None if sol is None else sol.ctypes.data_as(IP)))
 
     def add_mask(self, expr, sel, set_name=None):
         s = np.ascontiguousarray(sel, dtype=np.int32)
@@ -190,4 +207,27 @@ def write_cmatrix(path, tri, nrows, actual_nrows, sieve, frames):
     fr = np.ascontiguousarray(frames, dtype=np.int32)
     if L.chost_write_cmatrix(path.encode(), _fp(t), nrows, actual_nrows, sieve,
                              fr.ctypes.data_as(C.POINTER(C.c_int))):
+        raise RmsdError(f"cannot write {path}")
+
+
+def write_grace(path, legends, values, invert=False):
+    """DataIO_Grace xy text of per-frame series (values: (nsets, nframes))."""
+    L = load_host()
+    v = [np.ascontiguousarray(x, dtype=np.float64) for x in values]
+    n = len(v)
+    lg = (C.c_char_p * n)(*[s.encode() for s in legends])
+    ptrs = (C.POINTER(C.c_double) * n)(*[_dp(x) for x in v])
+    if L.chost_write_grace(path.encode(), n, lg, ptrs, len(v[0]) if v else 0, int(invert)):
+        raise RmsdError(f"cannot write {path}")
+
+
+def write_mesh(path, xlabel, x, names, values):
+    """DataIO_Std text of XY mesh sets sharing the x column."""
+    L = load_host()
+    xa = np.ascontiguousarray(x, dtype=np.float64)
+    v = [np.ascontiguousarray(y, dtype=np.float64) for y in values]
+    n = len(v)
+    nm = (C.c_char_p * max(n, 1))(*[s.encode() for s in names])
+    ptrs = (C.POINTER(C.c_double) * max(n, 1))(*[_dp(y) for y in v])
+    if L.chost_write_mesh(path.encode(), xlabel.encode(), _dp(xa), len(xa), n, nm, ptrs):
         raise RmsdError(f"cannot write {path}")

@@ -71,10 +71,15 @@ struct PinnedAllocator {
 using PinnedFloats = std::vector<float, PinnedAllocator<float>>;
 using PinnedDoubles = std::vector<double, PinnedAllocator<double>>;
 
-// Topology facts the path reads: atom count and masses.
+// Topology facts the path reads: atom count, masses and residues.
 struct Topology {
   int natom = 0;
   std::vector<double> mass;
+  std::vector<int> resFirst;         // first atom of each residue (0-based)
+  std::vector<std::string> resName;  // residue names, trailing blanks removed
+  std::vector<int> soluteRes;        // Topology::SoluteResidues() (0-based); empty: all
+  int Nres() const { return (int)resFirst.size(); }
+  int ResEnd(int r) const { return r + 1 < Nres() ? resFirst[r + 1] : natom; }
 };
 
 // A COORDS set: nframes FP32 frames of `stride` floats each, the 3*natom
@@ -123,6 +128,7 @@ class MatrixFlt {
 // ncomp values per frame, written with width w and precision p.
 struct Series {
   std::string name;
+  std::string legend;  // plot legend (Grace); the name when empty
   int ncomp = 1;
   int width = 12, prec = 4;
   std::vector<double> v;
@@ -135,6 +141,7 @@ class DataSetList {
   CoordsSet* AddCoords(const std::string& name);
   CoordsSet* FindCoordsSet(const std::string& name) const;  // "" -> the default (first) set
   CoordsSet* AddReference(const std::string& name);         // reference frames ('ref <name>')
+  CoordsSet* ReferenceAt(int idx) const;                    // 'refindex <#>'
   CoordsSet* FindReference(const std::string& name) const;  // "" -> the first reference
   MatrixFlt* AddMatrix(const std::string& name);
   Series* AddSeries(const std::string& name);
@@ -154,6 +161,13 @@ class DataSetList {
 namespace dataio {
 int WriteMatrix(const std::string& path, const MatrixFlt& m, bool square2d);
 int WriteSeries(const std::string& path, const std::vector<const Series*>& sets);
+// Grace xy sets (DataIO_Grace::WriteDataNormal / WriteDataInverted,
+// src/DataIO_Grace.cpp:179-224, :276-313), x = frame number.
+int WriteGrace(const std::string& path, const std::vector<const Series*>& sets, bool invert);
+// XY mesh sets sharing one x column (DataSet_Mesh through DataIO_Std):
+// "#<xlabel>" header, x as %8.3f, then each set's values.
+int WriteMesh(const std::string& path, const std::string& xlabel, const std::vector<double>& x,
+              const std::vector<const Series*>& sets);
 }  // namespace dataio
 
 // The library context plus device staging for host COORDS sets.
@@ -234,13 +248,32 @@ class Action_Rmsd {
   // unless 'nofit' or 'nomod').  Frames go to the library in batches when the
   // reference is fixed (first / ref), one at a time otherwise.
   RetType DoFrames(CoordsSet& crd);
-  int WriteOutput() const;
+  // Output files; with 'perres', also Action_Rmsd::Print (src/Action_Rmsd.cpp:
+  // 423-470): the per-residue series (perresout, Grace) and their average and
+  // standard deviation (perresavg).
+  int WriteOutput();
   const Series* Rmsd() const { return rmsd_; }
 
  private:
   RetType InitRef(ArgList& args, DataSetList& dsl, GpuSession& gpu);
   RetType RunBatch(CoordsSet& crd, const AtomMask& tsel, int f0, int nf, const float* refxyz,
                    const Topology& reftop, const std::string& refname);
+  // Action_Rmsd::perResSetup (src/Action_Rmsd.cpp:197-315): residue masks of
+  // the target and reference topologies, one data set per residue.
+  RetType PerResSetup(const Topology& tgt, const CoordsSet& tgtset, const Topology& ref,
+                      const CoordsSet& refset);
+  // Per-residue table (library layout) of the active residues.
+  struct PerRes {
+    int tgtRes = 0;  // 1-based residue number (the data set's index)
+    Series* data = nullptr;
+    bool active = false;
+  };
+  bool perres_ = false, perresCenter_ = false, perresInvert_ = false, perresReady_ = false;
+  std::string perresOut_, perresAvg_, perresMask_, tgtRange_, refRange_;
+  std::vector<PerRes> resRMS_;
+  std::vector<int32_t> prOff_, prTgt_, prRef_;  // active residues only
+  std::vector<double> prTmass_, prRmass_;
+  std::vector<int> prSeries_;                   // active residue -> resRMS_ index
   bool fit_ = true, useMass_ = false;
   ModeType mode_ = ROT_AND_TRANS;
   TvecType tvecType_ = NO_TVEC;
@@ -254,6 +287,7 @@ class Action_Rmsd {
   Series* tvecs_ = nullptr;
   MaskResolver resolve_;
   GpuSession* gpu_ = nullptr;
+  DataSetList* dsl_ = nullptr;
 };
 
 // Cluster::Metric_RMS with the pairwise cache of Cluster::MetricArray:

@@ -61,6 +61,38 @@ int chost_add_reference(chost_session* s, const char* name, const float* xyz, in
   return 0;
 }
 
+int chost_set_residues(chost_session* s, const char* set, int nres, const int* first,
+                       const char* const* names, int nsolute, const int* solute) {
+  if (!s || !set || nres < 0 || (nres > 0 && !first) || nsolute < 0 ||
+      (nsolute > 0 && !solute))
+    return 1;
+  CoordsSet* c = s->s.dsl.FindCoordsSet(set);
+  if (!c) c = s->s.dsl.FindReference(set);
+  if (!c || (nres > 0 && *set == 0)) {
+    s->err = std::string("chost_set_residues: no set '") + set + "'";
+    return 1;
+  }
+  for (int r = 0; r < nres; ++r)
+    if (first[r] < 0 || first[r] >= c->top.natom || (r > 0 && first[r] <= first[r - 1])) {
+      s->err = "chost_set_residues: residue first atoms must increase within the topology";
+      return 1;
+    }
+  for (int k = 0; k < nsolute; ++k)
+    if (solute[k] < 0 || solute[k] >= nres) {
+      s->err = "chost_set_residues: solute residue out of range";
+      return 1;
+    }
+  c->top.resFirst.assign(first, first + nres);
+  c->top.resName.clear();
+  for (int r = 0; r < nres; ++r) {
+    std::string n = names && names[r] ? names[r] : "";
+    while (!n.empty() && n.back() == ' ') n.pop_back();
+    c->top.resName.push_back(n);
+  }
+  c->top.soluteRes.assign(solute, solute + nsolute);
+  return 0;
+}
+
 int chost_add_mask(chost_session* s, const char* set, const char* expr, const int* sel, int nsel) {
   if (!s || !expr || nsel < 0 || (nsel > 0 && !sel)) return 1;
   const std::string key = (set && *set) ? std::string(set) + "|" + expr : std::string(expr);
@@ -169,6 +201,35 @@ int chost_read_cmatrix(const char* path, float* tri, int64_t* nrows, int64_t* ac
   if (tri && m.size()) memcpy(tri, m.data(), sizeof(float) * m.size());
   if (frames && !fr.empty()) memcpy(frames, fr.data(), sizeof(int) * fr.size());
   return 0;
+}
+
+int chost_write_grace(const char* path, int nsets, const char* const* legends,
+                      const double* const* values, int64_t nframes, int invert) {
+  if (!path || nsets < 1 || nframes < 0 || !legends || (nframes > 0 && !values)) return 1;
+  std::vector<Series> sets(nsets);
+  std::vector<const Series*> ptrs;
+  for (int k = 0; k < nsets; ++k) {
+    Series& s = sets[k];
+    s.legend = legends[k] ? legends[k] : "";
+    s.name = s.legend;
+    if (nframes > 0) s.v.assign(values[k], values[k] + nframes);
+    ptrs.push_back(&s);
+  }
+  return dataio::WriteGrace(path, ptrs, invert != 0);
+}
+
+int chost_write_mesh(const char* path, const char* xlabel, const double* x, int64_t n, int nsets,
+                     const char* const* names, const double* const* values) {
+  if (!path || !xlabel || n < 0 || nsets < 0 || (n > 0 && (!x || (nsets > 0 && !values))))
+    return 1;
+  std::vector<Series> sets(nsets);
+  std::vector<const Series*> ptrs;
+  for (int k = 0; k < nsets; ++k) {
+    sets[k].name = names && names[k] ? names[k] : "";
+    if (n > 0) sets[k].v.assign(values[k], values[k] + n);
+    ptrs.push_back(&sets[k]);
+  }
+  return dataio::WriteMesh(path, xlabel, std::vector<double>(x, x + n), ptrs);
 }
 
 }  // extern "C"

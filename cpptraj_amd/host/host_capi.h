@@ -27,6 +27,11 @@ int chost_add_coords(chost_session* s, const char* name, const float* xyz, int64
 /* Reference frame ('ref <name>' / 'reference'): one frame of 3*natom floats. */
 int chost_add_reference(chost_session* s, const char* name, const float* xyz, int natom,
                         const double* mass);
+/* Residues of a set's topology (COORDS set or reference): nres first atoms
+ * (0-based, increasing), names, and the solute residues (0-based; NULL / 0:
+ * every residue) that Topology::SoluteResidues would list. */
+int chost_set_residues(chost_session* s, const char* set, int nres, const int* first,
+                       const char* const* names, int nsolute, const int* solute);
 /* Mask expression -> selected atom indices, for the set (COORDS or reference)
  * named `set`, or for every set when `set` is NULL or empty. */
 int chost_add_mask(chost_session* s, const char* set, const char* expr, const int* sel, int nsel);
@@ -50,6 +55,14 @@ int chost_write_series(const char* path, int nsets, const char* const* names, co
                        int64_t nframes);
 int chost_write_cmatrix(const char* path, const float* tri, int64_t nrows, int64_t actual_nrows,
                         int sieve, const int* frames);
+/* Grace xy file of nsets per-frame series (one value per frame, %12.4f),
+ * normal or inverted (DataIO_Grace). */
+int chost_write_grace(const char* path, int nsets, const char* const* legends,
+                      const double* const* values, int64_t nframes, int invert);
+/* XY mesh text (x %8.3f, then each set %12.4f) with header "#<xlabel>" and
+ * the set names. */
+int chost_write_mesh(const char* path, const char* xlabel, const double* x, int64_t n, int nsets,
+                     const char* const* names, const double* const* values);
 /* Cmatrix file reader: fills sizes, then (second call) the triangle and the
  * cached frame list. */
 int chost_read_cmatrix(const char* path, float* tri, int64_t* nrows, int64_t* actual_nrows,

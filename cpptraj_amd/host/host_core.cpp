@@ -138,6 +138,10 @@ CoordsSet* DataSetList::FindReference(const std::string& name) const {
   return nullptr;
 }
 
+CoordsSet* DataSetList::ReferenceAt(int idx) const {
+  return idx >= 0 && idx < (int)refs_.size() ? refs_[idx].get() : nullptr;
+}
+
 MatrixFlt* DataSetList::AddMatrix(const std::string& name) {
   mats_.emplace_back(new MatrixFlt());
   mats_.back()->name = name;
@@ -174,8 +178,8 @@ namespace dataio {
 // Width of a frame-number column (TextFormat::SetCoordFormat,
 // src/TextFormat.cpp:52-82): the digits of the largest coordinate
 // (step * n + min), widened for the precision, at least the default width.
-static int coord_width(size_t n, int def_width, int prec) {
-  long maxc = (long)n + 1;
+static int coord_width(size_t n, int def_width, int prec, long minc = 1) {
+  long maxc = (long)n + minc;
   int w = 1;
   while (maxc >= 10) {
     maxc /= 10;
@@ -240,6 +244,77 @@ int WriteSeries(const std::string& path, const std::vector<const Series*>& sets)
         const double v = i < s->nframes() ? s->v[i * s->ncomp + c] : 0.0;
         fprintf(f, " %*.*f", s->width, s->prec, v);
       }
+    fprintf(f, "\n");
+  }
+  return fclose(f) != 0;
+}
+
+static const char* legend_of(const Series* s) {
+  return s->legend.empty() ? s->name.c_str() : s->legend.c_str();
+}
+
+int WriteGrace(const std::string& path, const std::vector<const Series*>& sets, bool invert) {
+  if (sets.empty()) return 1;
+  FILE* f = fopen(path.c_str(), "w");
+  if (!f) return 1;
+  // Header: axis labels from the first set's dimension ("Frame"), swapped
+  // when inverted.
+  fprintf(f, "@with g0\n@  xaxis label \"%s\"\n@  yaxis label \"%s\"\n"
+             "@  legend 0.2, 0.995\n@  legend char size 0.60\n",
+          invert ? "" : "Frame", invert ? "Frame" : "");
+  if (!invert) {
+    // One xy set per data set: x = frame number (1-based).
+    for (size_t k = 0; k < sets.size(); ++k) {
+      const Series* s = sets[k];
+      fprintf(f, "@  s%zu legend \"%s\"\n@target G0.S%zu\n@type xy\n", k, legend_of(s), k);
+      const size_t n = s->nframes();
+      const int xw = coord_width(n, 8, 3);
+      for (size_t i = 0; i < n; ++i) {
+        fprintf(f, "%*.3f", xw, (double)(i + 1));
+        for (int c = 0; c < s->ncomp; ++c)
+          fprintf(f, " %*.*f", s->width, s->prec, s->v[i * s->ncomp + c]);
+        fprintf(f, "\n");
+      }
+    }
+  } else {
+    // One xy set per frame: x = data set number (0-based), then the legend.
+    size_t n = 0;
+    for (const Series* s : sets) n = std::max(n, s->nframes());
+    const int xw = coord_width(sets.size(), 8, 3, 0);
+    for (size_t i = 0; i < n; ++i) {
+      fprintf(f, "@target G0.S%zu\n@type xy\n", i);
+      for (size_t k = 0; k < sets.size(); ++k) {
+        const Series* s = sets[k];
+        fprintf(f, "%*.3f", xw, (double)k);
+        for (int c = 0; c < s->ncomp; ++c)
+          fprintf(f, " %*.*f", s->width, s->prec,
+                  i < s->nframes() ? s->v[i * s->ncomp + c] : 0.0);
+        fprintf(f, " \"%s\"\n", legend_of(s));
+      }
+    }
+  }
+  return fclose(f) != 0;
+}
+
+int WriteMesh(const std::string& path, const std::string& xlabel, const std::vector<double>& x,
+              const std::vector<const Series*>& sets) {
+  FILE* f = fopen(path.c_str(), "w");
+  if (!f) return 1;
+  // x column: %8.3f, widened for coordinates of more than four digits.
+  double xmax = 0.0;
+  for (double v : x) xmax = std::max(xmax, std::fabs(v));
+  const int xw = coord_width(0, 8, 3, (long)xmax);
+  fprintf(f, "%-*s", xw, ("#" + xlabel).c_str());
+  for (const Series* s : sets) {
+    std::string nm = s->name;
+    std::replace(nm.begin(), nm.end(), ' ', '_');
+    fprintf(f, "%*s", s->width + 1, nm.c_str());
+  }
+  fprintf(f, "\n");
+  for (size_t i = 0; i < x.size(); ++i) {
+    fprintf(f, "%*.3f", xw, x[i]);
+    for (const Series* s : sets)
+      fprintf(f, " %*.*f", s->width, s->prec, i < s->v.size() ? s->v[i] : 0.0);
     fprintf(f, "\n");
   }
   return fclose(f) != 0;

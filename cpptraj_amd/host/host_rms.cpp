@@ -39,6 +39,41 @@ bool resolve_or_all(const MaskResolver& r, const std::string& expr, const Coords
   return r && r(expr, set, sel);
 }
 
+// Range::SetRange (src/Range.cpp:49-103): comma-separated numbers or
+// "a-b" spans (inclusive), sorted, duplicates removed; empty is an error.
+bool parse_range(const std::string& arg, std::vector<int>* out, std::string* err) {
+  out->clear();
+  if (arg.find_first_of(":@*") != std::string::npos) {
+    *err = "Using a mask expression for range (" + arg + ")";
+    return false;
+  }
+  size_t p = 0;
+  while (p <= arg.size()) {
+    size_t q = arg.find(',', p);
+    if (q == std::string::npos) q = arg.size();
+    const std::string tok = arg.substr(p, q - p);
+    p = q + 1;
+    if (tok.empty()) continue;
+    const size_t dash = tok.find('-', 1);
+    char* end = nullptr;
+    const long a = strtol(tok.c_str(), &end, 10);
+    if (end == tok.c_str()) {
+      *err = "Range::SetRange(" + arg + "): bad number in " + tok;
+      return false;
+    }
+    long b = a;
+    if (dash != std::string::npos) b = strtol(tok.c_str() + dash + 1, nullptr, 10);
+    for (long v = a; v <= b; ++v) out->push_back((int)v);
+  }
+  std::sort(out->begin(), out->end());
+  out->erase(std::unique(out->begin(), out->end()), out->end());
+  if (out->empty()) {
+    *err = "empty range '" + arg + "'";
+    return false;
+  }
+  return true;
+}
+
 }  // namespace
 
 // ---- Analysis_Rms2d ------------------------------------------------------------
@@ -176,6 +211,20 @@ RetType Action_Rmsd::Init(ArgList& args, DataSetList& dsl, MaskResolver resolve,
                           GpuSession& gpu) {
   gpu_ = &gpu;
   resolve_ = resolve;
+  dsl_ = &dsl;
+  // Per-residue options first (src/Action_Rmsd.cpp:35-50), so that their
+  // values are not taken for masks.
+  perres_ = args.hasKey("perres");
+  if (perres_) {
+    perresOut_ = args.GetStringKey("perresout");
+    perresInvert_ = args.hasKey("perresinvert");
+    tgtRange_ = args.GetStringKey("range");
+    refRange_ = args.GetStringKey("refrange");
+    perresMask_ = args.GetStringKey("perresmask");
+    if (!perresMask_.empty() && perresMask_[0] == '&') perresMask_.erase(0, 1);
+    perresCenter_ = args.hasKey("perrescenter");
+    perresAvg_ = args.GetStringKey("perresavg");
+  }
   fit_ = !args.hasKey("nofit");
   if (fit_) {
     if (args.hasKey("norotate")) mode_ = TRANS_ONLY;
@@ -206,10 +255,6 @@ RetType Action_Rmsd::Init(ArgList& args, DataSetList& dsl, MaskResolver resolve,
   }
   if (tvecType_ != NO_TVEC) vecsout_ = args.GetStringKey("vecsout");
   if (InitRef(args, dsl, gpu) != OK) return ERR;
-  if (args.hasKey("perres")) {
-    gpu.SetError("rms: 'perres' is not on the GPU RMSD path");
-    return ERR;
-  }
   tgtMaskExpr_ = args.GetMaskNext();
   refMaskExpr_ = args.GetMaskNext();
   if (refMaskExpr_.empty()) refMaskExpr_ = tgtMaskExpr_;
@@ -272,9 +317,113 @@ RetType Action_Rmsd::InitRef(ArgList& args, DataSetList& dsl, GpuSession& gpu) {
           return ERR;
         }
         refMode_ = FRAME;
+      } else if (args.Contains("refindex")) {
+        // 'refindex <#>': the reference frame at that position
+        // (DataSetList::GetReferenceSet, src/DataSetList.cpp:1011-1014).
+        const int idx = args.getKeyInt("refindex", -1);
+        refCrd_ = dsl.ReferenceAt(idx);
+        if (!refCrd_) {
+          gpu.SetError("no reference at index " + std::to_string(idx));
+          return ERR;
+        }
+        refMode_ = FRAME;
       }
     }
   }
+  return OK;
+}
+
+// Action_Rmsd::perResSetup (src/Action_Rmsd.cpp:197-315).  Target residues:
+// 'range' or every solute residue; reference residues: 'refrange' or the same
+// numbers.  Residue masks are ":<res>" intersected with 'perresmask'.  A
+// residue out of range is skipped; one whose masks select nothing or differ in
+// size keeps an (empty) data set but is not computed.
+RetType Action_Rmsd::PerResSetup(const Topology& tgt, const CoordsSet& tgtset,
+                                 const Topology& ref, const CoordsSet& refset) {
+  GpuSession& g = *gpu_;
+  std::string err;
+  if (tgt.Nres() == 0 || ref.Nres() == 0) {
+    g.SetError("perres: the topology has no residue information");
+    return ERR;
+  }
+  std::vector<int> tr, rr;
+  if (tgtRange_.empty()) {
+    if (tgt.soluteRes.empty())
+      for (int r = 0; r < tgt.Nres(); ++r) tr.push_back(r + 1);
+    else
+      for (int r : tgt.soluteRes) tr.push_back(r + 1);  // user numbering starts at 1
+  } else if (!parse_range(tgtRange_, &tr, &err)) {
+    g.SetError(err);
+    return ERR;
+  }
+  if (refRange_.empty()) rr = tr;
+  else if (!parse_range(refRange_, &rr, &err)) {
+    g.SetError(err);
+    return ERR;
+  }
+  if (tr.size() != rr.size()) {
+    g.SetError("Number of target residues " + std::to_string(tr.size()) +
+               " does not match number of reference residues " + std::to_string(rr.size()));
+    return ERR;
+  }
+  // 'perresmask': one selection per topology, intersected with each residue.
+  std::vector<char> tpick(tgt.natom, 1), rpick(ref.natom, 1);
+  if (!perresMask_.empty()) {
+    AtomMask ts, rs;
+    if (!resolve_ || !resolve_(perresMask_, tgtset, &ts) || !resolve_(perresMask_, refset, &rs)) {
+      g.SetError("perresmask [" + perresMask_ + "] cannot be resolved");
+      return ERR;
+    }
+    std::fill(tpick.begin(), tpick.end(), 0);
+    std::fill(rpick.begin(), rpick.end(), 0);
+    for (int a : ts) tpick[a] = 1;
+    for (int a : rs) rpick[a] = 1;
+  }
+  prOff_.assign(1, 0);
+  prTgt_.clear();
+  prRef_.clear();
+  prTmass_.clear();
+  prRmass_.clear();
+  prSeries_.clear();
+  int maxNatom = 0;
+  for (size_t i = 0; i < tr.size(); ++i) {
+    const int tres = tr[i], rres = rr[i];
+    if (tres < 1 || tres > tgt.Nres() || rres < 1 || rres > ref.Nres()) continue;
+    size_t p = 0;
+    while (p < resRMS_.size() && resRMS_[p].tgtRes != tres) ++p;
+    if (p == resRMS_.size()) {
+      PerRes pr;
+      pr.tgtRes = tres;
+      pr.data = dsl_->AddSeries(rmsd_->name + "[res]:" + std::to_string(tres));
+      const std::string& rn = (size_t)(tres - 1) < tgt.resName.size() ? tgt.resName[tres - 1] : "";
+      pr.data->legend = rn + ":" + std::to_string(tres);
+      pr.data->width = 12;
+      pr.data->prec = 4;
+      resRMS_.push_back(pr);
+    }
+    resRMS_[p].active = false;
+    AtomMask ta, ra;
+    for (int a = tgt.resFirst[tres - 1]; a < tgt.ResEnd(tres - 1); ++a)
+      if (tpick[a]) ta.push_back(a);
+    for (int a = ref.resFirst[rres - 1]; a < ref.ResEnd(rres - 1); ++a)
+      if (rpick[a]) ra.push_back(a);
+    if (ta.empty() || ra.empty() || ta.size() != ra.size()) continue;
+    maxNatom = std::max(maxNatom, (int)ta.size());
+    resRMS_[p].active = true;
+    for (size_t k = 0; k < ta.size(); ++k) {
+      prTgt_.push_back(ta[k]);
+      prRef_.push_back(ra[k]);
+      prTmass_.push_back((size_t)ta[k] < tgt.mass.size() ? tgt.mass[ta[k]] : 1.0);
+      prRmass_.push_back((size_t)ra[k] < ref.mass.size() ? ref.mass[ra[k]] : 1.0);
+    }
+    prOff_.push_back((int32_t)prTgt_.size());
+    prSeries_.push_back((int)p);
+  }
+  if (maxNatom == 0) {
+    g.SetError("No residues selected for per-residue calculation.");
+    return ERR;
+  }
+  perresReady_ = true;
   return OK;
 }
 
@@ -343,12 +492,20 @@ RetType Action_Rmsd::RunBatch(CoordsSet& crd, const AtomMask& tsel, int f0, int 
   const size_t base_t = tvecs_ ? tvecs_->v.size() : 0;
   if (tvecs_) tvecs_->v.resize(base_t + 6 * (size_t)nf);
 
+  // Per-residue values: the fit runs without its coordinate update, the
+  // residues are measured on the frames as that update leaves them (FP64),
+  // then the update is applied.
+  const bool pr = perresReady_;
+  const int prn = pr ? (int)prSeries_.size() : 0;
+  const bool needUT = rmatrices_ || moving || (pr && fit_);
+  if (pr)
+    for (int j = 0; j < prn; ++j) resRMS_[prSeries_[j]].data->v.resize(base_r + nf);
   struct SlotBufs {
     rmsd_ctx* ctx = nullptr;
     hipStream_t st = nullptr;
     float* dx = nullptr;
-    double *dr = nullptr, *dU = nullptr, *dT = nullptr;
-    PinnedDoubles hr, hU, hT;
+    double *dr = nullptr, *dU = nullptr, *dT = nullptr, *dpr = nullptr;
+    PinnedDoubles hr, hU, hT, hpr;
     int c = -1;  // chunk in flight
   } sl[2];
   for (int k = 0; k < 2 && k < nchunks; ++k) {
@@ -358,12 +515,14 @@ RetType Action_Rmsd::RunBatch(CoordsSet& crd, const AtomMask& tsel, int f0, int 
     const std::string t = "rms.slot" + std::to_string(k);
     sl[k].dx = (float*)g.Buffer(t + ".xyz", fbytes * chunk);
     sl[k].dr = (double*)g.Buffer(t + ".rmsd", sizeof(double) * chunk);
-    if (rmatrices_ || moving) sl[k].dU = (double*)g.Buffer(t + ".U", sizeof(double) * 9 * chunk);
-    if (tvecs_ || moving) sl[k].dT = (double*)g.Buffer(t + ".T", sizeof(double) * 6 * chunk);
-    if (!sl[k].dx || !sl[k].dr || ((rmatrices_ || moving) && !sl[k].dU) ||
-        ((tvecs_ || moving) && !sl[k].dT))
+    if (needUT) sl[k].dU = (double*)g.Buffer(t + ".U", sizeof(double) * 9 * chunk);
+    if (tvecs_ || needUT) sl[k].dT = (double*)g.Buffer(t + ".T", sizeof(double) * 6 * chunk);
+    if (pr) sl[k].dpr = (double*)g.Buffer(t + ".perres", sizeof(double) * (size_t)prn * chunk);
+    if (!sl[k].dx || !sl[k].dr || (needUT && !sl[k].dU) || ((tvecs_ || needUT) && !sl[k].dT) ||
+        (pr && !sl[k].dpr))
       return ERR;
     sl[k].hr.resize(chunk);
+    if (pr) sl[k].hpr.resize((size_t)prn * chunk);
     if (rmatrices_) sl[k].hU.resize(9 * (size_t)chunk);
     if (tvecs_) sl[k].hT.resize(6 * (size_t)chunk);
   }
@@ -378,6 +537,10 @@ RetType Action_Rmsd::RunBatch(CoordsSet& crd, const AtomMask& tsel, int f0, int 
     if (!hip_ok(hipStreamSynchronize(b.st), "rms stream")) return false;
     const int c0 = b.c * chunk, n = std::min(chunk, nf - c0);
     std::copy(b.hr.begin(), b.hr.begin() + n, rmsd_->v.begin() + base_r + c0);
+    for (int j = 0; j < prn; ++j) {
+      std::vector<double>& v = resRMS_[prSeries_[j]].data->v;
+      for (int f = 0; f < n; ++f) v[base_r + c0 + f] = b.hpr[(size_t)f * prn + j];
+    }
     if (rmatrices_)
       std::copy(b.hU.begin(), b.hU.begin() + 9 * (size_t)n,
                 rmatrices_->v.begin() + base_u + 9 * (size_t)c0);
@@ -410,12 +573,26 @@ RetType Action_Rmsd::RunBatch(CoordsSet& crd, const AtomMask& tsel, int f0, int 
     F.xyz = b.dx;
     F.nframes = n;
     if (g.Check(rmsd_fit_to_ref(b.ctx, &F, &R, 0, fit_, useMass_,
-                                moving ? RMSD_MODIFY_NONE : modify, b.dr, b.dU, b.dT),
+                                (moving || pr) ? RMSD_MODIFY_NONE : modify, b.dr, b.dU, b.dT),
                 "rms"))
       return ERR;
+    if (pr) {
+      if (g.Check(rmsd_perres_nofit(b.ctx, &F, &R, 0, prn, prOff_.data(), prTgt_.data(),
+                                    prRef_.data(), prTmass_.data(), prRmass_.data(), useMass_,
+                                    perresCenter_, modify, b.dU, b.dT, b.dpr),
+                  "rms perres"))
+        return ERR;
+      if (modify != RMSD_MODIFY_NONE &&
+          g.Check(rmsd_transform(b.ctx, &F, nullptr, 0, b.dU, b.dT, modify), "rms"))
+        return ERR;
+    }
     if (moving && g.Check(rmsd_transform(b.ctx, &F, dmove, nmove, b.dU, b.dT,
                                          RMSD_MODIFY_ROT_TRANS),
                           "align"))
+      return ERR;
+    if (pr && !hip_ok(hipMemcpyAsync(b.hpr.data(), b.dpr, sizeof(double) * (size_t)prn * n,
+                                     hipMemcpyDeviceToHost, b.st),
+                      "download"))
       return ERR;
     if (!hip_ok(hipMemcpyAsync(b.hr.data(), b.dr, sizeof(double) * n, hipMemcpyDeviceToHost, b.st),
                 "download") ||
@@ -463,6 +640,10 @@ RetType Action_Rmsd::DoFrames(CoordsSet& crd) {
     g.SetError("No atoms selected for [" + tgtMaskExpr_ + "]");
     return ERR;
   }
+  if (perres_ && !perresReady_) {
+    const CoordsSet& rs = (refMode_ == FRAME || refMode_ == TRAJ) ? *refCrd_ : crd;
+    if (PerResSetup(crd.top, crd, rs.top, rs) != OK) return ERR;
+  }
   const int nf = crd.nframes();
   if (nf == 0) return OK;
   switch (refMode_) {
@@ -498,11 +679,36 @@ RetType Action_Rmsd::DoFrames(CoordsSet& crd) {
   return ERR;
 }
 
-int Action_Rmsd::WriteOutput() const {
+int Action_Rmsd::WriteOutput() {
   if (!outfile_.empty() && dataio::WriteSeries(outfile_, {rmsd_})) return 1;
   if (rmatrices_ && !matricesout_.empty() && dataio::WriteSeries(matricesout_, {rmatrices_}))
     return 1;
   if (tvecs_ && !vecsout_.empty() && dataio::WriteSeries(vecsout_, {tvecs_})) return 1;
+  if (!perres_ || resRMS_.empty()) return 0;
+  std::vector<const Series*> sets;
+  for (const PerRes& p : resRMS_) sets.push_back(p.data);
+  if (!perresOut_.empty() && dataio::WriteGrace(perresOut_, sets, perresInvert_)) return 1;
+  if (!perresAvg_.empty()) {
+    // DataSet_1D::Avg (src/DataSet_1D.cpp:13-61): mean and population
+    // standard deviation; x = the residue number.
+    Series* avg = dsl_->AddSeries(rmsd_->name + "[Avg]");
+    Series* sd = dsl_->AddSeries(rmsd_->name + "[Stdev]");
+    std::vector<double> x;
+    for (const PerRes& p : resRMS_) {
+      const std::vector<double>& v = p.data->v;
+      double a = 0.0, d = 0.0;
+      if (!v.empty()) {
+        for (double y : v) a += y;
+        a /= (double)v.size();
+        for (double y : v) d += (a - y) * (a - y);
+        d = std::sqrt(d / (double)v.size());
+      }
+      x.push_back((double)p.tgtRes);
+      avg->v.push_back(a);
+      sd->v.push_back(d);
+    }
+    if (dataio::WriteMesh(perresAvg_, "Residue", x, {avg, sd})) return 1;
+  }
   return 0;
 }
 

@@ -223,3 +223,69 @@ def test_align_move_mask(hs):
     assert np.array_equal(a, r)
     assert np.array_equal(m[:, 60:], r[:, 60:])
     assert np.array_equal(m[:, :60], X.reshape(50, 120, 3)[:, :60])
+
+
+# -- Test_PerResRMSD (rmsd ... perres) ---------------------------------------------
+def load_perres(hs, golden, frames=None):
+    """tz2.truncoct solute frames as COORDS 'trunc' and frame 1 as reference
+    'tz2.truncoct.nc' (refindex 0), both with residues; masks :2-11."""
+    g = golden("tz2_perres")
+    X = g["frames"] if frames is None else g["frames"][frames]
+    n = g["frames"].shape[1] // 3
+    names = [str(r) for r in g["res_names"]]
+    hs.add_coords("trunc", X, n, g["mass"])
+    hs.set_residues("trunc", g["res_first"], names)
+    hs.add_reference("tz2.truncoct.nc", g["frames"][0], n, g["mass"])
+    hs.set_residues("tz2.truncoct.nc", g["res_first"], names)
+    hs.add_mask(":2-11", np.arange(g["res_first"][1], g["res_first"][11]))
+    return g
+
+
+def test_perres_grace_output(hs, golden, tmp_path):
+    load_perres(hs, golden)
+    out = str(tmp_path / "PerResRMSD.agr")
+    hs.run(f"rmsd :2-11 refindex 0 perres perresout {out} crdset trunc")
+    assert text(out) == expected("Test_PerResRMSD__PerResRMSD.agr.save")
+
+
+def test_perres_average(hs, golden, tmp_path):
+    # 'trajin ../tz2.truncoct.nc 2 4': frames 2-4 only.
+    load_perres(hs, golden, frames=slice(1, 4))
+    out = str(tmp_path / "perresavg.dat")
+    hs.run(f"rmsd R2-11 :2-11 refindex 0 perres perresavg {out} crdset trunc")
+    assert text(out) == expected("Test_PerResRMSD__perresavg.dat.save")
+
+
+def test_perres_centered_range(hs, golden, tmp_path):
+    g = load_perres(hs, golden)
+    out = str(tmp_path / "center.agr")
+    hs.run(f"rmsd :2-11 refindex 0 perres perresout {out} range 1 perrescenter crdset trunc")
+    assert text(out) == expected("Test_PerResRMSD__center.agr.save")
+    # The fit still moved every frame; the per-residue set is a data set too.
+    assert np.abs(hs.series("RMSD_00000[res]:1") - g["center"][0]).max() <= 0.00005 + 1e-7
+
+
+def test_perres_mask_ranges_and_nofit(hs, golden, oracle, tmp_path):
+    """perresmask (heavy atoms picked by the caller's mask table), refrange
+    onto other residues of the reference, nofit, and residues with atom
+    count mismatches (kept as empty sets), against the oracle."""
+    g = load_perres(hs, golden)
+    X = g["frames"]
+    n = X.shape[1] // 3
+    first = np.append(g["res_first"], n)
+    heavy = np.arange(0, n, 2)  # a stand-in selection for '&!@H='
+    hs.add_mask("!@H=", heavy)
+    hs.run("rmsd P :2-11 refindex 0 nofit perres range 2-4,6 refrange 3-6 perresmask !@H= "
+           "perrescenter mass crdset trunc")
+    pick = np.zeros(n, bool)
+    pick[heavy] = True
+    for t, r in zip([2, 3, 4, 6], [3, 4, 5, 6]):
+        ta = np.arange(first[t - 1], first[t])[pick[first[t - 1]:first[t]]]
+        ra = np.arange(first[r - 1], first[r])[pick[first[r - 1]:first[r]]]
+        got = hs.series(f"P[res]:{t}")
+        if len(ta) != len(ra) or len(ta) == 0:
+            assert got.size == 0
+            continue
+        exp = oracle.perres(X, X[0], [0, len(ta)], ta, ref_atoms=ra, tmass=g["mass"][ta],
+                            rmass=g["mass"][ra], use_mass=True, center=True)[:, 0]
+        assert np.abs(got - exp).max() <= 1e-12

@@ -85,3 +85,56 @@ def test_cmatrix_roundtrip_and_layout(tmp_path):
     # no sieve: no presence flags
     write_cmatrix(p, tri, 5, 5, 1, np.arange(5))
     assert len(open(p, "rb").read()) == 28 + 40
+
+
+def _agr_sets(text):
+    legends, sets = [], []
+    for line in text.splitlines():
+        if line.startswith("@  s") and "legend" in line:
+            legends.append(line.split('"')[1])
+        elif line.startswith("@target"):
+            sets.append([])
+        elif line.strip() and not line.startswith("@"):
+            sets[-1].append(float(line.split()[1]))
+    return legends, sets
+
+
+@pytest.mark.parametrize("save", ["Test_PerResRMSD__PerResRMSD.agr.save",
+                                  "Test_PerResRMSD__center.agr.save"])
+def test_grace_roundtrip(tmp_path, save):
+    """Per-residue series in DataIO_Grace's normal layout (perresout)."""
+    from cpptraj_amd.host import write_grace
+    legends, sets = _agr_sets(read(save))
+    out = str(tmp_path / "p.agr")
+    write_grace(out, legends, sets)
+    assert open(out).read() == read(save)
+
+
+def test_grace_inverted_layout(tmp_path):
+    """perresinvert: one xy set per frame, x = data set number from 0, each
+    line ending with the set's legend (DataIO_Grace::WriteDataInverted)."""
+    from cpptraj_amd.host import write_grace
+    legends, sets = _agr_sets(read("Test_PerResRMSD__PerResRMSD.agr.save"))
+    out = str(tmp_path / "i.agr")
+    write_grace(out, legends, sets, invert=True)
+    lines = open(out).read().splitlines()
+    assert lines[:5] == ['@with g0', '@  xaxis label ""', '@  yaxis label "Frame"',
+                         '@  legend 0.2, 0.995', '@  legend char size 0.60']
+    body = lines[5:]
+    nsets, nframes = len(sets), len(sets[0])
+    assert len(body) == nframes * (2 + nsets)
+    for f in range(nframes):
+        blk = body[f * (2 + nsets):(f + 1) * (2 + nsets)]
+        assert blk[0] == f"@target G0.S{f}" and blk[1] == "@type xy"
+        for k, line in enumerate(blk[2:]):
+            assert line == "%8.3f %12.4f \"%s\"" % (k, sets[k][f], legends[k])
+
+
+def test_mesh_roundtrip(tmp_path):
+    """perresavg: residue number, average and standard deviation."""
+    from cpptraj_amd.host import write_mesh
+    text = read("Test_PerResRMSD__perresavg.dat.save")
+    R = rows(text)
+    out = str(tmp_path / "avg.dat")
+    write_mesh(out, "Residue", R[:, 0], ["R2-11[Avg]", "R2-11[Stdev]"], [R[:, 1], R[:, 2]])
+    assert open(out).read() == text
