@@ -164,9 +164,55 @@ def fit_stream(ctx, F, n, steps, warmup):
     err[0] = 0.0 if abs(got[0]) < 1e-5 else err[0]  # frame 0 fitted to itself: RMSD ~ 0
     out["max_err_vs_oracle_64_frames"] = float(err.max())
     out["workload"] = f"{F} frames x {n} atoms FP32 SoA, fit to frame 0, FP64 QCP, nomod"
+    out["perres"] = perres_bench(ctx, X, d, m, steps, warmup)
     del X
     torch.cuda.empty_cache()
     return out
+
+
+def perres_bench(ctx, X, d, m, steps, warmup, res_atoms=15):
+    """Per-residue no-fit RMSD ('rmsd ... perres mass') over the fit workload:
+    residues of `res_atoms` consecutive atoms, the fit's U and translations
+    applied to the target in FP64 (the rotate + translate update), one value
+    per frame and residue.  Algorithmic bytes: 12 per residue atom and frame
+    read + 8 per result written."""
+    import ctypes as C
+    import torch
+    F, n = d.nframes, d.natom
+    nres = n // res_atoms
+    off = np.arange(0, nres * res_atoms + 1, res_atoms, dtype=np.int32)
+    atoms = np.arange(nres * res_atoms, dtype=np.int32)
+    mh = m.cpu().numpy()[atoms].copy()
+    U = torch.empty((F, 9), dtype=torch.float64, device="cuda")
+    tv = torch.empty((F, 6), dtype=torch.float64, device="cuda")
+    rmsd = torch.empty(F, dtype=torch.float64, device="cuda")
+    ctx.fit_desc(d, d, 0, True, True, 0, rmsd, U, tv)
+    res = torch.empty((F, nres), dtype=torch.float64, device="cuda")
+    IP, DP = C.POINTER(C.c_int32), C.POINTER(C.c_double)
+    L = ctx._L
+
+    def fn():
+        ctx.check(L.rmsd_perres_nofit(ctx.handle, C.byref(d), C.byref(d), 0, nres,
+                                      off.ctypes.data_as(IP), atoms.ctypes.data_as(IP),
+                                      atoms.ctypes.data_as(IP), mh.ctypes.data_as(DP),
+                                      mh.ctypes.data_as(DP), 1, 0, 1,
+                                      C.c_void_p(U.data_ptr()), C.c_void_p(tv.data_ptr()),
+                                      C.c_void_p(res.data_ptr())))
+    ms = timed_steps(fn, steps, warmup)
+    _, main = profiled_main_ms(ctx, fn, 3)
+    algo = F * (12.0 * nres * res_atoms + 8.0 * nres)
+    # parity on sampled frames against the oracle (AoS copies of the frames)
+    from oracle.oracle import Oracle
+    idx = np.linspace(0, F - 1, 32).astype(np.int64)
+    aos = np.transpose(X[idx].cpu().numpy(), (0, 2, 1)).reshape(len(idx), -1)
+    ref0 = np.transpose(X[0].cpu().numpy(), (1, 0)).reshape(-1)
+    exp = Oracle().perres(aos, ref0, off, atoms, tmass=mh, use_mass=True,
+                          U=U[idx].cpu().numpy(), tvec=tv[idx].cpu().numpy(), modify=1)
+    err = float(np.abs(res[idx].cpu().numpy() - exp).max())
+    return {"workload": f"{F} frames x {n} atoms FP32 SoA, {nres} residues of {res_atoms} atoms, "
+                        "mass-weighted, fit update applied in FP64",
+            "frames_per_s": F / (ms / 1e3), "ms_per_step": ms, "kernel_ms": main,
+            "hbm_gbs": algo / (main / 1e3) / 1e9, "max_err_vs_oracle_32_frames": err}
 
 
 def batch_bench(ctx, seed=3, nsets=64, precision="bf16x3"):

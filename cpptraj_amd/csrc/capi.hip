@@ -569,7 +569,8 @@ int rmsd_perres_nofit(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frame
       const int span = h2 > l2 ? h2 - l2 : 0;
       const bool sf = staged(span, ne) >= 0;
       const size_t b = rmsd_perres_lds_bytes(r1 + 1 - r, ne, sf ? span : ne, 1);
-      if (r1 > r && (b > PERRES_LDS_PIECE || sf != span_form)) break;
+      if (r1 > r && (b > PERRES_LDS_PIECE || sf != span_form || r1 - r >= PERRES_RES_PIECE))
+        break;
       if (r1 == r && b > PERRES_LDS_MAX)
         return fail(ctx, RMSD_ERR_ARG, "residue " + std::to_string(r) + " has too many atoms (" +
                                            std::to_string(ne) + ") for per-residue RMSD");
@@ -582,8 +583,8 @@ int rmsd_perres_nofit(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frame
     pieces.push_back(span_form ? make_int4(r, r1, lo, hi) : make_int4(r, r1, -1, -1));
     r = r1;
   }
-  // Frames per batch: enough (frame, residue) items for the workgroup, within
-  // the piece budget (one frame always fits).
+  // Frames per batch: enough (frame, residue) items for the workgroup's lanes,
+  // within PERRES_LDS_BATCH (one frame always runs).
   int maxnr = 1;
   for (const int4& p : pieces) maxnr = std::max(maxnr, p.y - p.x);
   int nfb = std::max(1, PERRES_THREADS / maxnr);
@@ -595,7 +596,7 @@ int rmsd_perres_nofit(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frame
     return m;
   };
   if (pieces.size() > 65535) return fail(ctx, RMSD_ERR_ARG, "residue table too large");
-  while (nfb > 1 && lds_for(nfb) > PERRES_LDS_PIECE) --nfb;
+  while (nfb > 1 && lds_for(nfb) > PERRES_LDS_BATCH) --nfb;
   const size_t lds = lds_for(nfb);
   // Table blob (16-byte aligned sections).
   const size_t o_off = 0;

@@ -158,8 +158,10 @@ struct PerresArgs {
   int nfb;
   double* out;            // out[f * nres + r]
 };
-constexpr int PERRES_THREADS = 256;
-constexpr size_t PERRES_LDS_PIECE = size_t(64) << 10;  // piece budget (LDS bytes at nfb = 1)
+constexpr int PERRES_THREADS = 64;
+constexpr int PERRES_RES_PIECE = 16;                   // residues per piece
+constexpr size_t PERRES_LDS_PIECE = size_t(12) << 10;   // piece budget (LDS bytes at nfb = 1)
+constexpr size_t PERRES_LDS_BATCH = size_t(24) << 10;  // budget when batching frames
 constexpr size_t PERRES_LDS_MAX = size_t(160) << 10;   // one oversized residue alone
 size_t rmsd_perres_lds_bytes(int nres_piece, int nentries, int span, int nfb);
 hipError_t rmsd_launch_perres(const PerresArgs& a, int npieces, size_t lds_bytes, hipStream_t s);
