@@ -126,6 +126,12 @@ def _optional(L):
     if hasattr(L, "rmsd_pairwise_backward"):
         L.rmsd_pairwise_backward.argtypes = [P, FP, FP, C.c_int, P, P, P]
         L.rmsd_pairwise_backward.restype = C.c_int
+    if hasattr(L, "rmsd_avgcorr"):
+        L.rmsd_avgcorr_windows.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int32),
+                                           C.c_int]
+        L.rmsd_avgcorr_windows.restype = C.c_int
+        L.rmsd_avgcorr.argtypes = [P, FP, FP, C.c_int, C.c_int, C.c_int, C.c_int, P, P]
+        L.rmsd_avgcorr.restype = C.c_int
     if hasattr(L, "rmsd_perres_nofit"):
         IP = C.POINTER(C.c_int32)
         DP = C.POINTER(C.c_double)
@@ -369,6 +375,40 @@ def rmsd_perres(frames, res_off, tgt_atoms, ref_frames=None, ref_index=0, ref_at
                                        cp(rm, DP), int(use_mass), int(center), MODIFY[modify],
                                        _ptr(Ut), _ptr(tvt), _ptr(out)))
     return out
+
+
+def avgcorr_windows(nframes, maxwindow=-1, offset=1):
+    """Window sizes of the RMS average correlation (rmsd_avgcorr_windows)."""
+    import numpy as np
+    L = load_library()
+    n = L.rmsd_avgcorr_windows(nframes, maxwindow, offset, None, 0)
+    if n < 1:
+        raise RmsdError("Not enough frames to perform calculation.")
+    w = np.zeros(n, np.int32)
+    L.rmsd_avgcorr_windows(nframes, maxwindow, offset, w.ctypes.data_as(C.POINTER(C.c_int32)), n)
+    return w
+
+
+def rms_avgcorr(frames, mask=None, mass=None, ref_frames=None, ref_index=0, ref_mask=None,
+                ref_mass=None, use_mass=False, maxwindow=-1, offset=1, soa=False, natom=None,
+                ctx: Context | None = None):
+    """RMS average correlation ('rmsavgcorr').  ref_frames None = 'first'.
+    Returns (window sizes, mean RMSD, SD), the last two float64 CUDA tensors."""
+    import torch
+
+    ctx = ctx or _ctx(frames.device.index or 0)
+    d = frames_desc(frames, mask, mass, soa=soa, natom=natom)
+    r = None
+    if ref_frames is not None:
+        r = frames_desc(ref_frames, ref_mask if ref_mask is not None else mask,
+                        ref_mass if ref_mass is not None else mass, soa=soa, natom=natom)
+    w = avgcorr_windows(d.nframes, maxwindow, offset)
+    ct = torch.empty(len(w), dtype=torch.float64, device=frames.device)
+    sd = torch.empty(len(w), dtype=torch.float64, device=frames.device)
+    ctx.check(ctx._L.rmsd_avgcorr(ctx.handle, C.byref(d), None if r is None else C.byref(r),
+                                  ref_index, int(use_mass), maxwindow, offset, _ptr(ct),
+                                  _ptr(sd)))
+    return w, ct, sd
 
 
 class Batch:

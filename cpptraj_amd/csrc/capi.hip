@@ -656,4 +656,86 @@ int rmsd_perres_nofit(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frame
   return RMSD_OK;
 }
 
+int rmsd_avgcorr_windows(int nframes, int maxwindow, int offset, int32_t* windows, int cap) {
+  if (offset < 1) offset = 1;
+  int wmax = nframes;
+  if (maxwindow != -1) {
+    wmax = maxwindow + 1;
+    if (wmax > nframes) wmax = nframes;
+  }
+  const int start = 1 + offset;
+  int total = (wmax - start) / offset;
+  if (total % offset > 0) ++total;
+  if (total < 0) return -1;  // not enough frames
+  int n = 0;
+  if (windows && n < cap) windows[n] = 1;
+  ++n;
+  for (int ws = start; ws < wmax; ws += offset) {
+    if (windows && n < cap) windows[n] = ws;
+    ++n;
+  }
+  return n;
+}
+
+int rmsd_avgcorr(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* ref, int ref_frame,
+                 int use_mass, int maxwindow, int offset, double* ct, double* sd) {
+  if (!ctx) return RMSD_ERR_ARG;
+  int st;
+  if ((st = check_frames(ctx, frames, "frames")) != RMSD_OK) return st;
+  if (ref) {
+    if ((st = check_frames(ctx, ref, "ref")) != RMSD_OK) return st;
+    if (ref->nsel != frames->nsel)
+      return fail(ctx, RMSD_ERR_ARG, "reference and target select different atom counts");
+    if (ref_frame < 0 || ref_frame >= ref->nframes)
+      return fail(ctx, RMSD_ERR_ARG, "reference frame out of range");
+    if (use_mass && !ref->mass) return fail(ctx, RMSD_ERR_ARG, "use_mass needs reference masses");
+  }
+  if (use_mass && !frames->mass) return fail(ctx, RMSD_ERR_ARG, "use_mass needs target masses");
+  const int nw = rmsd_avgcorr_windows(frames->nframes, maxwindow, offset, nullptr, 0);
+  if (nw < 1) return fail(ctx, RMSD_ERR_ARG, "Not enough frames to perform calculation.");
+  if (!ct || !sd) return fail(ctx, RMSD_ERR_ARG, "ct or sd is NULL");
+  const size_t lds = sizeof(double) * 6 * (size_t)frames->nsel;
+  if (lds > AVGCORR_LDS_MAX)
+    return fail(ctx, RMSD_ERR_ARG, "rmsavgcorr: at most " + std::to_string(AVGCORR_LDS_MAX / 48) +
+                                       " selected atoms");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  std::vector<int32_t> ws(nw);
+  rmsd_avgcorr_windows(frames->nframes, maxwindow, offset, ws.data(), nw);
+  int32_t* dws = (int32_t*)scratch(ctx, "avgcorr.windows", sizeof(int32_t) * nw, &st);
+  if (!dws) return st;
+  HIP_TRY(ctx, hipMemcpyAsync(dws, ws.data(), sizeof(int32_t) * nw, hipMemcpyHostToDevice,
+                              ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // ws is a local buffer
+  AvgCorrArgs a;
+  a.fs = to_dev(*frames);
+  a.mass = use_mass ? frames->mass : nullptr;
+  a.ref = nullptr;
+  a.consts = nullptr;
+  a.ldr = 0;
+  a.windows = dws;
+  a.nwin = nw;
+  a.ct = ct;
+  a.sd = sd;
+  ctx->marks = 0;
+  mark(ctx, 0);
+  if (ref) {
+    const int ldr = (int)round_up(frames->nsel, 4);
+    double* refbuf = (double*)scratch(ctx, "avgcorr.ref", sizeof(double) * (3 * ldr + 8), &st);
+    if (!refbuf) return st;
+    double* consts = (double*)scratch(ctx, "avgcorr.consts", sizeof(double) * 8, &st);
+    if (!consts) return st;
+    HIP_TRY(ctx, rmsd_launch_ref_prep(to_dev(*ref), ref_frame, use_mass ? ref->mass : nullptr,
+                                      use_mass ? frames->mass : nullptr, 1, refbuf,
+                                      refbuf + 3 * ldr, consts, ctx->stream));
+    a.ref = refbuf;
+    a.consts = consts;
+    a.ldr = ldr;
+  }
+  mark(ctx, 1);
+  HIP_TRY(ctx, rmsd_launch_avgcorr(a, ref == nullptr, ctx->stream));
+  mark(ctx, 2);
+  mark(ctx, 3);
+  return RMSD_OK;
+}
+
 }  // extern "C"

@@ -166,6 +166,21 @@ constexpr size_t PERRES_LDS_MAX = size_t(160) << 10;   // one oversized residue 
 size_t rmsd_perres_lds_bytes(int nres_piece, int nentries, int span, int nfb);
 hipError_t rmsd_launch_perres(const PerresArgs& a, int npieces, size_t lds_bytes, hipStream_t s);
 
+// RMS average correlation (k_avgcorr.hip): one wavefront per window size.
+struct AvgCorrArgs {
+  FrameSetDev fs;         // frames (nsel selected atoms), in order
+  const double* mass;     // nsel target masses, or null (unweighted)
+  const double* ref;      // centred reference SoA [3][ldr] (null: 'first')
+  const double* consts;   // reference constants of rmsd_launch_ref_prep
+  int ldr;
+  const int* windows;     // nwin window sizes
+  int nwin;
+  double* ct;             // nwin mean RMSD
+  double* sd;             // nwin standard deviation
+};
+constexpr size_t AVGCORR_LDS_MAX = size_t(160) << 10;  // running sum + reference
+hipError_t rmsd_launch_avgcorr(const AvgCorrArgs& a, bool first, hipStream_t s);
+
 hipError_t rmsd_launch_setup(const SetupArgs& a, hipStream_t s);
 hipError_t rmsd_launch_prep(const PrepArgs& a, hipStream_t s);
 // Batched forms: one workgroup per set / per padded frame of every set.

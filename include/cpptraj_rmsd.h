@@ -192,6 +192,29 @@ int rmsd_transform(rmsd_ctx* ctx, const rmsd_frames* frames, const int32_t* atom
  *            fits with RMSD_MODIFY_NONE, calls this, then rmsd_transform.
  *   out      device, nframes * nres doubles: out[f*nres + r]; -1 for a residue
  *            of zero total mass (Constants::SMALL). */
+/* Window sizes of the RMS average correlation (Analysis_RmsAvgCorr::Analyze,
+ * src/Analysis_RmsAvgCorr.cpp:154-209): 1, then 1 + offset, 1 + 2 offset, ...
+ * below min(maxwindow + 1, nframes) (maxwindow -1: nframes).  Host function:
+ * writes up to cap sizes into windows (may be NULL) and returns their number,
+ * or -1 when there are not enough frames. */
+int rmsd_avgcorr_windows(int nframes, int maxwindow, int offset, int32_t* windows, int cap);
+
+/* RMS average correlation ('rmsavgcorr').
+ * Replaces Analysis_RmsAvgCorr::Analyze (src/Analysis_RmsAvgCorr.cpp:119-276):
+ * for each window size w (rmsd_avgcorr_windows), the running average of the
+ * selected atoms over w consecutive frames (FP64 running sum, as the
+ * reference's operator+= / operator-= / Divide), best-fit RMSD of every
+ * averaged frame (Frame::RMSD_CenteredRef) to the reference, and the mean and
+ * standard deviation of those F - w + 1 values.
+ *   ref      reference set and frame ('ref' / 'reference': its selected atoms,
+ *            centred with its masses), or NULL for 'first' (the window's first
+ *            averaged frame, centred)
+ *   maxwindow, offset   'stop' (-1: every frame) and 'offset' (< 1 -> 1)
+ *   ct, sd   device, one double per window size (DataSet Ct and Ct[SD])
+ * At most 3413 selected atoms (running sum and reference in LDS). */
+int rmsd_avgcorr(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* ref, int ref_frame,
+                 int use_mass, int maxwindow, int offset, double* ct, double* sd);
+
 int rmsd_perres_nofit(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* ref,
                       int ref_frame, int nres, const int32_t* res_off, const int32_t* tgt_atoms,
                       const int32_t* ref_atoms, const double* tmass, const double* rmass,

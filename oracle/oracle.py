@@ -97,6 +97,11 @@ class Oracle:
         L.orc_perres.argtypes = [_fp, C.c_int64, C.c_int, _fp, C.c_int, _ip, _ip, _ip, _dp,
                                  _dp, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp]
         L.orc_perres.restype = C.c_int
+        L.orc_avgcorr_windows.argtypes = [C.c_int, C.c_int, C.c_int, _ip, C.c_int]
+        L.orc_avgcorr_windows.restype = C.c_int
+        L.orc_avgcorr.argtypes = [_fp, C.c_int64, C.c_int, _ip, C.c_int, _dp, _dp, C.c_int,
+                                  C.c_int, C.c_int, _dp, _dp]
+        L.orc_avgcorr.restype = C.c_int
 
     # -- per-frame routines -------------------------------------------------
     def center_on_origin(self, xyz, mass=None, use_mass=False):
@@ -254,6 +259,33 @@ class Oracle:
             raise MemoryError("orc_perres failed")
         return out
 
+    def avgcorr_windows(self, nframes, maxwindow=-1, offset=1):
+        n = self.L.orc_avgcorr_windows(nframes, maxwindow, offset, None, 0)
+        if n < 1:
+            return np.zeros(0, np.int32)
+        w = np.zeros(n, np.int32)
+        self.L.orc_avgcorr_windows(nframes, maxwindow, offset, _i(w), n)
+        return w
+
+    def avgcorr(self, frames, sel=None, mass=None, ref_sel_centered=None, use_mass=False,
+                maxwindow=-1, offset=1):
+        """Analysis_RmsAvgCorr: (window sizes, mean RMSD, SD) per window size;
+        ref_sel_centered None = 'first'."""
+        fr = _c32(frames)
+        s = _ci(sel)
+        nsel = len(s) if s is not None else fr.shape[1] // 3
+        m = _c64(mass) if mass is not None else np.ones(nsel)
+        w = self.avgcorr_windows(fr.shape[0], maxwindow, offset)
+        if len(w) == 0:
+            raise ValueError("not enough frames")
+        ct, sd = np.zeros(len(w)), np.zeros(len(w))
+        rs = None if ref_sel_centered is None else _c64(ref_sel_centered)
+        n = self.L.orc_avgcorr(_f(fr), fr.shape[1], fr.shape[0], _i(s), nsel, _d(m), _d(rs),
+                               int(use_mass), maxwindow, offset, _d(ct), _d(sd))
+        if n != len(w):
+            raise MemoryError("orc_avgcorr failed")
+        return w, ct, sd
+
 
 class RefLib:
     """The reference's own sources compiled in place (oracle/_ref)."""
@@ -290,6 +322,9 @@ class RefLib:
         L.cref_perres.argtypes = [_fp, C.c_int64, C.c_int, C.c_int, _fp, C.c_int, C.c_int, _ip,
                                   _ip, _ip, _dp, _dp, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp]
         L.cref_perres.restype = C.c_int
+        L.cref_avgcorr.argtypes = [_fp, C.c_int64, C.c_int, _ip, C.c_int, _dp, _dp, C.c_int,
+                                   _ip, C.c_int, _dp, _dp]
+        L.cref_avgcorr.restype = C.c_int
 
     def rmsd_centered_ref(self, tgt, ref, mass=None, ref_mass=None, use_mass=False):
         t = _c64(tgt).copy()
@@ -419,6 +454,18 @@ class RefLib:
                            nres, _i(off), _i(ta), _i(ra), _d(tm), _d(rm), int(use_mass),
                            int(center), _d(Uc), _d(tv), modify, _d(out))
         return out
+
+    def avgcorr(self, frames, ws, sel=None, mass=None, ref_sel_centered=None, use_mass=False):
+        fr = _c32(frames)
+        s = _ci(sel)
+        nsel = len(s) if s is not None else fr.shape[1] // 3
+        m = _c64(mass) if mass is not None else np.ones(nsel)
+        w = _ci(ws)
+        ct, sd = np.zeros(len(w)), np.zeros(len(w))
+        rs = None if ref_sel_centered is None else _c64(ref_sel_centered)
+        self.L.cref_avgcorr(_f(fr), fr.shape[1], fr.shape[0], _i(s), nsel, _d(m), _d(rs),
+                            int(use_mass), _i(w), len(w), _d(ct), _d(sd))
+        return ct, sd
 
 
 def ref_available() -> bool:

@@ -10,6 +10,7 @@
 // calls.
 #include <vector>
 #include <cstring>
+#include <cmath>
 #include <cstdint>
 #ifdef _OPENMP
 #include <omp.h>
@@ -303,6 +304,52 @@ int cref_perres(const float* frames, int64_t stride, int nframes, int natom, con
       }
       out[(int64_t)f * nres + r] = T.RMSD_NoFit(R, use_mass != 0);
     }
+  }
+  return 0;
+}
+
+// Window loop of Analysis_RmsAvgCorr::Analyze (src/Analysis_RmsAvgCorr.cpp:
+// 208-257) on the reference's Frame: operator+= / operator-= for the running
+// sum, Divide(Frame, double), CenterOnOrigin for the 'first' reference and
+// RMSD_CenteredRef.  ws: the window sizes; ref_sel (centred) or NULL.
+int cref_avgcorr(const float* frames, int64_t stride, int nframes, const int* sel, int nsel,
+                 const double* mass, const double* ref_sel, int use_mass, const int* ws, int nw,
+                 double* ct, double* sd) {
+  std::vector<double> zero(3 * (size_t)nsel, 0.0);
+  Frame Tgt = make_frame(zero.data(), mass, nsel);
+  Frame Sum = make_frame(zero.data(), mass, nsel);
+  Frame Ref = make_frame(ref_sel ? ref_sel : zero.data(), mass, nsel);
+  for (int wi = 0; wi < nw; ++wi) {
+    const int w = ws[wi];
+    const double dw = (double)w;
+    Sum.ZeroCoords();
+    bool first = ref_sel == 0;
+    double avg = 0.0, sq = 0.0;
+    int sub = 0;
+    for (int f = 0; f < nframes; ++f) {
+      frame_from_floats(Tgt, frames + (int64_t)f * stride, sel, nsel);
+      Sum += Tgt;
+      if (f > w - 2) {
+        Tgt.Divide(Sum, dw);
+        if (first) {
+          Ref.SetCoordinates(Tgt);
+          Ref.CenterOnOrigin(use_mass != 0);
+          first = false;
+        }
+        const double r = Tgt.RMSD_CenteredRef(Ref, use_mass != 0);
+        avg += r;
+        sq += r * r;
+        frame_from_floats(Tgt, frames + (int64_t)sub * stride, sel, nsel);
+        Sum -= Tgt;
+        ++sub;
+      }
+    }
+    const double d = 1.0 / ((double)nframes - dw + 1.0);
+    avg *= d;
+    sq *= d;
+    sq -= avg * avg;
+    ct[wi] = avg;
+    sd[wi] = sq > 0.0 ? sqrt(sq) : 0.0;
   }
   return 0;
 }

@@ -833,3 +833,93 @@ int orc_perres(const float *frames, int64_t stride, int nframes, const float *re
   free(R);
   return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Window sizes of Analysis_RmsAvgCorr::Analyze (src/Analysis_RmsAvgCorr.cpp:
+ * 154-209): 1, then 1 + offset, 1 + 2 offset, ... below WindowMax (nframes,
+ * or stop + 1 capped at nframes).  Writes up to cap sizes into w and returns
+ * their number, or -1 when there are not enough frames.                      */
+int orc_avgcorr_windows(int nframes, int maxwindow, int offset, int *w, int cap)
+{
+  if (offset < 1) offset = 1;
+  int wmax = nframes;
+  if (maxwindow != -1) {
+    wmax = maxwindow + 1;
+    if (wmax > nframes) wmax = nframes;
+  }
+  const int start = 1 + offset;
+  int total = (wmax - start) / offset;
+  if (total % offset > 0) ++total;
+  if (total < 0) return -1;
+  int n = 0;
+  if (n < cap && w) w[n] = 1;
+  ++n;
+  for (int ws = start; ws < wmax; ws += offset) {
+    if (n < cap && w) w[n] = ws;
+    ++n;
+  }
+  return n;
+}
+
+/* Analysis_RmsAvgCorr::Analyze (src/Analysis_RmsAvgCorr.cpp:119-276): for
+ * each window size, the running sum of the selected atoms (Frame operator+=
+ * / operator-=, FP64), divided by the window (Frame::Divide), fitted to the
+ * reference (Frame::RMSD_CenteredRef): the centred reference ref_sel, or
+ * with ref_sel == NULL ('first') the window's first averaged frame, centred.
+ * ct / sd receive the mean and standard deviation over the averaged frames,
+ * one per window size (orc_avgcorr_windows).  Returns the number of window
+ * sizes or -1.                                                                */
+int orc_avgcorr(const float *frames, int64_t stride, int nframes, const int *sel, int nsel,
+                const double *mass, const double *ref_sel, int use_mass, int maxwindow,
+                int offset, double *ct, double *sd)
+{
+  const int nw = orc_avgcorr_windows(nframes, maxwindow, offset, NULL, 0);
+  if (nw < 1) return -1;
+  int *ws = (int *)malloc(sizeof(int) * (size_t)nw);
+  double *sum = (double *)malloc(sizeof(double) * 3 * (size_t)nsel);
+  double *tgt = (double *)malloc(sizeof(double) * 3 * (size_t)nsel);
+  double *ref = (double *)malloc(sizeof(double) * 3 * (size_t)nsel);
+  double *x = (double *)malloc(sizeof(double) * 3 * (size_t)nsel);
+  if (!ws || !sum || !tgt || !ref || !x) {
+    free(ws); free(sum); free(tgt); free(ref); free(x);
+    return -1;
+  }
+  orc_avgcorr_windows(nframes, maxwindow, offset, ws, nw);
+  for (int wi = 0; wi < nw; ++wi) {
+    const int w = ws[wi];
+    const double dw = (double)w;
+    int first = ref_sel == NULL;
+    if (ref_sel) memcpy(ref, ref_sel, sizeof(double) * 3 * (size_t)nsel);
+    for (int k = 0; k < 3 * nsel; ++k) sum[k] = 0.0;
+    double avg = 0.0, sq = 0.0;
+    int sub = 0;
+    for (int f = 0; f < nframes; ++f) {
+      orc_gather(frames + (int64_t)f * stride, sel, nsel, x);
+      for (int k = 0; k < 3 * nsel; ++k) sum[k] += x[k];
+      if (f > w - 2) {
+        for (int k = 0; k < 3 * nsel; ++k) tgt[k] = sum[k] / dw;
+        if (first) {
+          double c[3];
+          memcpy(ref, tgt, sizeof(double) * 3 * (size_t)nsel);
+          orc_center_on_origin(ref, mass, nsel, use_mass, c);
+          first = 0;
+        }
+        double U[9], tr[3];
+        const double r = orc_rmsd_centered_ref(tgt, ref, mass, nsel, use_mass, U, tr);
+        avg += r;
+        sq += r * r;
+        orc_gather(frames + (int64_t)sub * stride, sel, nsel, x);
+        for (int k = 0; k < 3 * nsel; ++k) sum[k] -= x[k];
+        ++sub;
+      }
+    }
+    const double d = 1.0 / ((double)nframes - dw + 1.0);
+    avg *= d;
+    sq *= d;
+    sq -= avg * avg;
+    ct[wi] = avg;
+    sd[wi] = sq > 0.0 ? sqrt(sq) : 0.0;
+  }
+  free(ws); free(sum); free(tgt); free(ref); free(x);
+  return nw;
+}

@@ -27,9 +27,14 @@ Outputs (numpy .npz, no pickles):
                    10 frames, masses, residue first atoms and names, and the
                    Test_PerResRMSD expected per-residue series, averages and
                    the centred residue-1 series
+  tz2_avgcorr.npz  Test_RmsAvgCorr: the averaged CA reference (avg.CA.rst7,
+                   11 atoms of :2-12@CA) and the expected RMS average
+                   correlation series (reference, offset 10, first); the
+                   frames are those of tz2_camask.npz
   text/            the expected text outputs themselves (Test_2DRMS,
-                   Test_RMSD and Test_PerResRMSD .save files), for byte-level
-                   checks of the host layer's DataIO_Std / DataIO_Grace writers
+                   Test_RMSD, Test_PerResRMSD and Test_RmsAvgCorr .save
+                   files), for byte-level checks of the host layer's
+                   DataIO_Std / DataIO_Grace writers
 
 The atom masks used by these tests are resolved here with a few lines of
 parm7 parsing (residue ranges, atom names, hydrogen exclusion); the general
@@ -150,6 +155,19 @@ def read_agr(path):
     return legends, np.array(sets, dtype=np.float64)
 
 
+def read_rst7(path):
+    """Amber ASCII restart: title, natom line, then 6 coordinates per line."""
+    with open(path) as fh:
+        fh.readline()
+        natom = int(fh.readline().split()[0])
+        vals = []
+        for line in fh:
+            vals.extend(float(t) for t in line.split())
+            if len(vals) >= 3 * natom:
+                break
+    return np.array(vals[:3 * natom], dtype=np.float64).reshape(natom, 3)
+
+
 def read_crd(path, natom, nbox=0):
     """Amber ASCII trajectory (%8.3f, 10 per line); nbox trailing box values
     per frame are dropped."""
@@ -268,6 +286,16 @@ def main():
         center=center,
     )
 
+    # ---- Test_RmsAvgCorr (strip !(:2-12@CA); reference avg.CA.rst7) -----------
+    A = os.path.join(T, "Test_RmsAvgCorr")
+    np.savez_compressed(
+        os.path.join(OUT, "tz2_avgcorr.npz"),
+        ref=read_rst7(os.path.join(A, "avg.CA.rst7")),
+        ref_all=read_series(os.path.join(A, "rmscorr.dat.save")),
+        off10=read_series(os.path.join(A, "rmscorr.10.dat.save")),
+        first=read_series(os.path.join(A, "rmscorr.first.dat.save")),
+    )
+
     # ---- expected text outputs, verbatim -------------------------------------------
     txt = os.path.join(OUT, "text")
     os.makedirs(txt, exist_ok=True)
@@ -278,7 +306,9 @@ def main():
                                       "rmatrices.dat.save", "NoMod.dat.save",
                                       "Previous.dat.save"]),
                       ("Test_PerResRMSD", ["PerResRMSD.agr.save", "perresavg.dat.save",
-                                           "center.agr.save"])):
+                                           "center.agr.save"]),
+                      ("Test_RmsAvgCorr", ["rmscorr.dat.save", "rmscorr.10.dat.save",
+                                           "rmscorr.first.dat.save"])):
         for n in names:
             with open(os.path.join(T, sub, n), "rb") as fi:
                 data = fi.read()

@@ -47,6 +47,11 @@ class OracleBackend:
         return self.o.perres(frames, frames[0], res_off, atoms, U=U, tvec=tv, modify=1,
                              center=center)
 
+    def avgcorr(self, frames, ref=None, offset=1):
+        rc = None if ref is None else self.o.center_on_origin(ref)[0]
+        _, ct, sd = self.o.avgcorr(frames, ref_sel_centered=rc, offset=offset)
+        return ct, sd
+
 
 class GpuBackend:
     def __init__(self, ctx):
@@ -96,6 +101,13 @@ class GpuBackend:
         out = ca.rmsd_perres(X, res_off, atoms, U=U, tvec=tv, modify="rottrans", center=center,
                              ctx=self.ctx)
         return out.cpu().numpy()
+
+    def avgcorr(self, frames, ref=None, offset=1):
+        import cpptraj_amd as ca
+        X = self._dev(frames, self.t.float32)
+        R = None if ref is None else self._dev(ref.reshape(1, -1), self.t.float32)
+        _, ct, sd = ca.rms_avgcorr(X, ref_frames=R, offset=offset, ctx=self.ctx)
+        return ct.cpu().numpy(), sd.cpu().numpy()
 
 
 @pytest.fixture(params=["oracle", pytest.param("gpu", marks=pytest.mark.gpu)])
@@ -240,3 +252,17 @@ def test_perres_centered_residue_1(golden, backend):
     off, fit_sel, atoms = _perres_setup(d)
     P = backend.perres(d["frames"], fit_sel, off[:2], atoms, center=True)
     assert np.abs(P[:, 0] - d["center"][0]).max() <= HALF4
+
+
+# ---------------------------------------------------------------------------------
+# Test_RmsAvgCorr: 101 frames of :2-12@CA; 'ref avg.CA.rst7', 'offset 10
+# reference' and 'first'.
+@pytest.mark.parametrize("case", ["ref_all", "off10", "first"])
+def test_avgcorr_series(golden, backend, case):
+    c, a = golden("tz2_camask"), golden("tz2_avgcorr")
+    ref = None if case == "first" else a["ref"]
+    ct, sd = backend.avgcorr(c["frames"], ref, offset=10 if case == "off10" else 1)
+    exp = a[case]
+    assert ct.shape == (len(exp),)
+    assert np.abs(ct - exp[:, 0]).max() <= HALF4
+    assert np.abs(sd - exp[:, 1]).max() <= HALF4

@@ -304,3 +304,42 @@ def test_perres_closed_forms(oracle, use_mass):
     # zero total mass -> -1 (RMSD_NoFit's divide-by-zero sentinel)
     z = oracle.perres(X, X[0], off, atoms, tmass=np.zeros(n), rmass=m, use_mass=True)
     assert np.all(z == -1.0)
+
+
+# -- RMS average correlation (rmsavgcorr) -------------------------------------------
+@pytest.mark.parametrize("use_mass", [False, True])
+@pytest.mark.parametrize("first", [False, True])
+@pytest.mark.parametrize("stop_offset", [(-1, 1), (-1, 4), (17, 3), (60, 1)])
+def test_avgcorr_bitexact(oracle, reflib, use_mass, first, stop_offset):
+    stop, offset = stop_offset
+    rng = np.random.default_rng(31 + 2 * use_mass + first)
+    F, n = 41, 23
+    X = synth.noisy(7, F, n, 0.8).reshape(F, -1).astype(np.float32)
+    sel = np.sort(rng.choice(n, 17, replace=False)).astype(np.int32)
+    m = synth.masses(rng, len(sel))
+    rc = None
+    if not first:
+        ref = X[5].reshape(-1, 3)[sel].astype(np.float64) + rng.normal(size=(len(sel), 3))
+        rc, _ = oracle.center_on_origin(ref, m, use_mass)
+    w, ct, sd = oracle.avgcorr(X, sel, m, rc, use_mass, stop, offset)
+    ct2, sd2 = reflib.avgcorr(X, w, sel, m, rc, use_mass)
+    assert same(ct, ct2) and same(sd, sd2)
+
+
+def test_avgcorr_windows_and_closed_forms(oracle):
+    # window sizes: 1, then 1 + offset ... below min(stop + 1, nframes)
+    assert list(oracle.avgcorr_windows(101, -1, 10)) == [1, 11, 21, 31, 41, 51, 61, 71, 81, 91]
+    assert list(oracle.avgcorr_windows(10, 3, 1)) == [1, 2, 3]
+    assert list(oracle.avgcorr_windows(5, -1, 1)) == [1, 2, 3, 4]
+    assert len(oracle.avgcorr_windows(1, -1, 1)) == 0  # not enough frames
+    # Frames that are translated copies of one structure: every running
+    # average is a translated copy too, so every window's mean and SD are 0.
+    rng = np.random.default_rng(3)
+    base = rng.normal(size=(12, 3)) * 4
+    X = np.stack([(base + rng.integers(-8, 8, size=3)).reshape(-1) for _ in range(20)])
+    X = X.astype(np.float32)
+    w, ct, sd = oracle.avgcorr(X, ref_sel_centered=oracle.center_on_origin(base)[0])
+    assert len(w) == 19 and np.abs(ct).max() < 1e-6 and np.abs(sd).max() < 1e-6
+    # 'first' with a constant trajectory: every averaged frame is the first
+    w, ct, sd = oracle.avgcorr(np.repeat(X[:1], 9, axis=0))
+    assert np.abs(ct).max() < 1e-6
