@@ -129,6 +129,7 @@ class MatrixFlt {
 struct Series {
   std::string name;
   std::string legend;  // plot legend (Grace); the name when empty
+  double xmin = 1.0, xstep = 1.0;  // x of value i: xmin + i * xstep (DataSet dimension)
   int ncomp = 1;
   int width = 12, prec = 4;
   std::vector<double> v;
@@ -288,6 +289,29 @@ class Action_Rmsd {
   MaskResolver resolve_;
   GpuSession* gpu_ = nullptr;
   DataSetList* dsl_ = nullptr;
+};
+
+// Analysis_RmsAvgCorr (src/Analysis_RmsAvgCorr.cpp): 'rmsavgcorr [crdset <set>]
+// [<name>] [<mask>] [out <file>] [output <file>] [mass] [stop <maxwindow>]
+// [offset <offset>] {ref <name> | reference | refindex <#> [<refmask>] | first}'.
+// One library call for every window size.
+class Analysis_RmsAvgCorr {
+ public:
+  RetType Setup(ArgList& args, DataSetList& dsl, MaskResolver resolve, GpuSession& gpu);
+  RetType Analyze();
+  int WriteOutput() const;
+
+ private:
+  CoordsSet* coords_ = nullptr;
+  CoordsSet* ref_ = nullptr;  // null: 'first'
+  bool useMass_ = false;
+  int maxwindow_ = -1, lagOffset_ = 1;
+  std::string outfile_, separate_, tgtMaskExpr_, refMaskExpr_;
+  Series* Ct_ = nullptr;
+  Series* Csd_ = nullptr;
+  std::vector<int32_t> windows_;
+  MaskResolver resolve_;
+  GpuSession* gpu_ = nullptr;
 };
 
 // Cluster::Metric_RMS with the pairwise cache of Cluster::MetricArray:

@@ -178,8 +178,8 @@ namespace dataio {
 // Width of a frame-number column (TextFormat::SetCoordFormat,
 // src/TextFormat.cpp:52-82): the digits of the largest coordinate
 // (step * n + min), widened for the precision, at least the default width.
-static int coord_width(size_t n, int def_width, int prec, long minc = 1) {
-  long maxc = (long)n + minc;
+static int coord_width(size_t n, int def_width, int prec, long minc = 1, double step = 1.0) {
+  long maxc = (long)(step * (double)n + (double)minc);
   int w = 1;
   while (maxc >= 10) {
     maxc /= 10;
@@ -226,7 +226,9 @@ int WriteSeries(const std::string& path, const std::vector<const Series*>& sets)
   if (!f) return 1;
   size_t n = 0;
   for (const Series* s : sets) n = std::max(n, s->nframes());
-  const int fw = coord_width(n, 8, 0);
+  // x column from the first set's dimension (min, step), as integers
+  const double x0 = sets[0]->xmin, dx = sets[0]->xstep;
+  const int fw = coord_width(n, 8, 0, (long)x0, dx);
   // Header: "#Frame", then each set's legend right-aligned over its columns
   // (ncomp x (1 + width)); spaces in names become underscores.
   fprintf(f, "%-*s", fw, "#Frame");
@@ -238,7 +240,7 @@ int WriteSeries(const std::string& path, const std::vector<const Series*>& sets)
   }
   fprintf(f, "\n");
   for (size_t i = 0; i < n; ++i) {
-    fprintf(f, "%*.0f", fw, (double)(i + 1));
+    fprintf(f, "%*.0f", fw, x0 + (double)i * dx);
     for (const Series* s : sets)
       for (int c = 0; c < s->ncomp; ++c) {
         const double v = i < s->nframes() ? s->v[i * s->ncomp + c] : 0.0;

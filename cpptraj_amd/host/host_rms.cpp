@@ -712,6 +712,125 @@ int Action_Rmsd::WriteOutput() {
   return 0;
 }
 
+// ---- Analysis_RmsAvgCorr ---------------------------------------------------------------
+// Setup follows src/Analysis_RmsAvgCorr.cpp:33-116.
+RetType Analysis_RmsAvgCorr::Setup(ArgList& args, DataSetList& dsl, MaskResolver resolve,
+                                   GpuSession& gpu) {
+  gpu_ = &gpu;
+  resolve_ = resolve;
+  const std::string setname = args.GetStringKey("crdset");
+  coords_ = dsl.FindCoordsSet(setname);
+  if (!coords_) {
+    gpu.SetError("rmsavgcorr: Could not locate COORDS set corresponding to " + setname);
+    return ERR;
+  }
+  lagOffset_ = std::max(1, args.getKeyInt("offset", 1));
+  outfile_ = args.GetStringKey("out");
+  separate_ = args.GetStringKey("output");
+  useMass_ = args.hasKey("mass");
+  maxwindow_ = args.getKeyInt("stop", -1);
+  bool useFirst = args.hasKey("first");
+  // DataSetList::GetReferenceFrame: 'ref <name>', 'reference', 'refindex <#>'.
+  if (args.Contains("ref")) {
+    const std::string rn = args.GetStringKey("ref");
+    ref_ = dsl.FindReference(rn);
+    if (!ref_) {
+      gpu.SetError("reference '" + rn + "' not found");
+      return ERR;
+    }
+  } else if (args.hasKey("reference")) {
+    ref_ = dsl.FindReference("");
+    if (!ref_) {
+      gpu.SetError("no reference frame loaded");
+      return ERR;
+    }
+  } else if (args.Contains("refindex")) {
+    const int idx = args.getKeyInt("refindex", -1);
+    ref_ = dsl.ReferenceAt(idx);
+    if (!ref_) {
+      gpu.SetError("no reference at index " + std::to_string(idx));
+      return ERR;
+    }
+  }
+  // Without a reference: the first running-averaged frame of each window;
+  // 'first' cannot be combined with a reference (the reference wins).
+  if (ref_) useFirst = false;
+  (void)useFirst;
+  tgtMaskExpr_ = args.GetMaskNext();
+  if (ref_) {
+    refMaskExpr_ = args.GetMaskNext();
+    if (refMaskExpr_.empty()) refMaskExpr_ = tgtMaskExpr_;
+  }
+  std::string name = args.GetStringNext();
+  if (name.empty()) name = dsl.GenerateDefaultName("RACorr");
+  Ct_ = dsl.AddSeries(name);
+  Csd_ = dsl.AddSeries(name + "[SD]");
+  for (Series* s : {Ct_, Csd_}) {
+    s->xmin = 1.0;
+    s->xstep = (double)lagOffset_;
+    s->width = 12;
+    s->prec = 4;
+  }
+  return OK;
+}
+
+// Analyze (src/Analysis_RmsAvgCorr.cpp:119-276): every window size in one
+// library call; window 1 (no averaging) first, then 1 + offset, ...
+RetType Analysis_RmsAvgCorr::Analyze() {
+  GpuSession& g = *gpu_;
+  AtomMask tsel, rsel;
+  if (!resolve_or_all(resolve_, tgtMaskExpr_, *coords_, &tsel) || tsel.empty()) {
+    g.SetError("No atoms selected for [" + tgtMaskExpr_ + "]");
+    return ERR;
+  }
+  rmsd_frames T, R;
+  if (g.FramesOnDevice("avgcorr.tgt", *coords_, tsel, &T)) return ERR;
+  if (ref_) {
+    if (!resolve_or_all(resolve_, refMaskExpr_, *ref_, &rsel) || rsel.empty()) {
+      g.SetError("Could not set up reference mask [" + refMaskExpr_ + "]");
+      return ERR;
+    }
+    if (rsel.size() != tsel.size()) {
+      g.SetError("Target mask " + tgtMaskExpr_ + " (" + std::to_string(tsel.size()) +
+                 ") does not correspond to reference mask (" + std::to_string(rsel.size()) + ")");
+      return ERR;
+    }
+    if (g.FramesOnDevice("avgcorr.ref", *ref_, rsel, &R, ref_->frame(0), 1)) return ERR;
+  }
+  const int nw = rmsd_avgcorr_windows(T.nframes, maxwindow_, lagOffset_, nullptr, 0);
+  if (nw < 1) {
+    g.SetError("Not enough frames to perform calculation.");
+    return ERR;
+  }
+  windows_.resize(nw);
+  rmsd_avgcorr_windows(T.nframes, maxwindow_, lagOffset_, windows_.data(), nw);
+  double* d = (double*)g.Buffer("avgcorr.out", sizeof(double) * 2 * nw);
+  if (!d) return ERR;
+  if (g.Check(rmsd_avgcorr(g.ctx(), &T, ref_ ? &R : nullptr, 0, useMass_, maxwindow_, lagOffset_,
+                           d, d + nw),
+              "rmsavgcorr"))
+    return ERR;
+  std::vector<double> h(2 * (size_t)nw);
+  if (g.Download(h.data(), d, sizeof(double) * h.size())) return ERR;
+  Ct_->v.assign(h.begin(), h.begin() + nw);
+  Csd_->v.assign(h.begin() + nw, h.end());
+  return WriteOutput() ? ERR : OK;
+}
+
+int Analysis_RmsAvgCorr::WriteOutput() const {
+  if (!outfile_.empty() && dataio::WriteSeries(outfile_, {Ct_, Csd_})) return 1;
+  if (!separate_.empty()) {
+    // The 'output' file: window size, mean, SD (src/Analysis_RmsAvgCorr.cpp:
+    // 189-190, 259-260).
+    FILE* f = fopen(separate_.c_str(), "w");
+    if (!f) return 1;
+    for (size_t i = 0; i < windows_.size(); ++i)
+      fprintf(f, "%8i %f %f\n", windows_[i], Ct_->v[i], Csd_->v[i]);
+    if (fclose(f) != 0) return 1;
+  }
+  return 0;
+}
+
 // ---- Metric_RMS + pairwise cache -------------------------------------------------------
 RetType Metric_RMS::Setup(ArgList& args, DataSetList& dsl, MaskResolver resolve,
                           GpuSession& gpu) {
@@ -848,6 +967,9 @@ int Session::Run(const std::string& line) {
       if (!crd) gpu.SetError("align: no COORDS set");
       else if (a.DoFrames(*crd) == OK) rc = 0;
     }
+  } else if (cmd == "rmsavgcorr") {
+    Analysis_RmsAvgCorr a;
+    rc = a.Setup(args, dsl, resolve, gpu) == OK && a.Analyze() == OK ? 0 : 1;
   } else if (cmd == "cluster") {
     Metric_RMS m;
     rc = m.Setup(args, dsl, resolve, gpu) == OK && m.CacheDistances() == OK ? 0 : 1;

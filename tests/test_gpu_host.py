@@ -289,3 +289,39 @@ def test_perres_mask_ranges_and_nofit(hs, golden, oracle, tmp_path):
         exp = oracle.perres(X, X[0], [0, len(ta)], ta, ref_atoms=ra, tmass=g["mass"][ta],
                             rmass=g["mass"][ra], use_mass=True, center=True)[:, 0]
         assert np.abs(got - exp).max() <= 1e-12
+
+
+# -- Test_RmsAvgCorr (rmsavgcorr) ---------------------------------------------------
+@pytest.mark.parametrize("cmd,save", [
+    ("rmsavgcorr crdset ca out {out} R2-12 ref avg.CA.rst7", "Test_RmsAvgCorr__rmscorr.dat.save"),
+    ("rmsavgcorr crdset ca out {out} Off10 offset 10 reference",
+     "Test_RmsAvgCorr__rmscorr.10.dat.save"),
+    ("rmsavgcorr crdset ca out {out} First first", "Test_RmsAvgCorr__rmscorr.first.dat.save"),
+])
+def test_rmsavgcorr_outputs(hs, golden, tmp_path, cmd, save):
+    """The frames are the 101 tz2 frames stripped to :2-12@CA (11 atoms); the
+    reference is avg.CA.rst7 on the stripped topology."""
+    c, a = golden("tz2_camask"), golden("tz2_avgcorr")
+    hs.add_coords("ca", c["frames"], 11, c["mass"])
+    hs.add_reference("avg.CA.rst7", a["ref"].reshape(-1), 11)
+    out = str(tmp_path / "rmscorr.dat")
+    hs.run(cmd.format(out=out))
+    assert text(out) == expected(save)
+
+
+def test_rmsavgcorr_stop_mass_and_output_file(hs, golden, oracle, tmp_path):
+    """'stop', 'mass', a mask and the 'output' file (window, mean, SD)."""
+    c = golden("tz2_camask")
+    X = c["frames"]
+    hs.add_coords("ca", X, 11, c["mass"])
+    sel = np.array([0, 2, 3, 5, 7, 8, 10], np.int32)
+    hs.add_mask("@CA&!:5", sel)
+    sep = str(tmp_path / "sep.txt")
+    hs.run(f"rmsavgcorr crdset ca S @CA&!:5 mass stop 30 offset 4 first output {sep}")
+    w, ct, sd = oracle.avgcorr(X, sel, c["mass"][sel], None, True, 30, 4)
+    got = hs.series("S")
+    assert got.shape == ct.shape and np.abs(got - ct).max() <= 5e-7
+    assert np.abs(hs.series("S[SD]") - sd).max() <= 5e-7
+    lines = text(sep).splitlines()
+    assert [int(l.split()[0]) for l in lines] == list(w)
+    assert all(abs(float(l.split()[1]) - v) <= 5e-7 for l, v in zip(lines, ct))
