@@ -215,6 +215,40 @@ def perres_bench(ctx, X, d, m, steps, warmup, res_atoms=15):
             "hbm_gbs": algo / (main / 1e3) / 1e9, "max_err_vs_oracle_32_frames": err}
 
 
+def avgcorr_bench(ctx, F=10000, n=300, steps=3, warmup=1):
+    """RMS average correlation ('rmsavgcorr') over F frames x n atoms (AoS
+    FP32, a CA-sized selection) against a fixed reference, every window size
+    1 .. F-1: sum_w (F - w + 1) averaged frames, each one running-sum update,
+    an FP64 fit and a QCP solve.  Parity: the first 20 window sizes against the
+    oracle (stop 20)."""
+    import torch
+    import cpptraj_amd as ca
+    from cpptraj_amd import synth
+    X = synth.noisy(5, F, n, 0.6).reshape(F, -1).astype(np.float32)
+    rng = np.random.default_rng(5)
+    ref = (X[0].reshape(-1, 3) + rng.normal(size=(n, 3)) * 0.3).astype(np.float32)
+    Xd = torch.as_tensor(X).cuda()
+    R = torch.as_tensor(ref.reshape(1, -1)).cuda()
+    w = ca.avgcorr_windows(F)
+    out = {}
+
+    def fn():
+        out["r"] = ca.rms_avgcorr(Xd, ref_frames=R, ctx=ctx)
+    ms = timed_steps(fn, steps, warmup)
+    fits = float(np.sum(F - w + 1))
+    _, ct, sd = out["r"]
+    from oracle.oracle import Oracle
+    o = Oracle()
+    rc, _ = o.center_on_origin(ref.astype(np.float64))
+    w20, ct20, sd20 = o.avgcorr(X, ref_sel_centered=rc, maxwindow=20)
+    err = max(float(np.abs(ct[:len(w20)].cpu().numpy() - ct20).max()),
+              float(np.abs(sd[:len(w20)].cpu().numpy() - sd20).max()))
+    return {"workload": f"{F} frames x {n} atoms, fixed reference, window sizes 1..{F - 1}",
+            "windows": int(len(w)), "averaged_frames": fits, "ms_per_step": ms,
+            "averaged_frames_per_s": fits / (ms / 1e3),
+            "max_err_vs_oracle_first_20_windows": err}
+
+
 def batch_bench(ctx, seed=3, nsets=64, precision="bf16x3"):
     """64 trajectories of uneven size (frames log-uniform in 500-20000, atoms
     log-uniform in 50-5000, each its own chain, sigma 0.5 A, rigid motion):
@@ -374,6 +408,7 @@ def main():
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-grad", action="store_true")
     ap.add_argument("--skip-host", action="store_true")
+    ap.add_argument("--skip-avgcorr", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -493,6 +528,11 @@ def main():
     if not args.skip_host:
         log("host-resident fit stream")
         line["host_stream"] = host_stream_bench(ctx)
+    if not args.skip_avgcorr:
+        log("rms average correlation")
+        line["avgcorr"] = avgcorr_bench(ctx)
+        log(f"avgcorr {line['avgcorr']['ms_per_step']:.1f} ms, "
+            f"{line['avgcorr']['averaged_frames_per_s']:.3g} averaged frames/s")
     ctx.close()
     print(json.dumps(line), flush=True)
 

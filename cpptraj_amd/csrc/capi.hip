@@ -694,10 +694,12 @@ int rmsd_avgcorr(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* re
   const int nw = rmsd_avgcorr_windows(frames->nframes, maxwindow, offset, nullptr, 0);
   if (nw < 1) return fail(ctx, RMSD_ERR_ARG, "Not enough frames to perform calculation.");
   if (!ct || !sd) return fail(ctx, RMSD_ERR_ARG, "ct or sd is NULL");
-  const size_t lds = sizeof(double) * 6 * (size_t)frames->nsel;
-  if (lds > AVGCORR_LDS_MAX)
-    return fail(ctx, RMSD_ERR_ARG, "rmsavgcorr: at most " + std::to_string(AVGCORR_LDS_MAX / 48) +
-                                       " selected atoms");
+  // LDS: the running sum, and the reference in 'first' mode (FP64 [3][nsel]).
+  const size_t per_atom = sizeof(double) * (ref ? 3 : 6);
+  if (per_atom * (size_t)frames->nsel > AVGCORR_LDS_MAX)
+    return fail(ctx, RMSD_ERR_ARG, "rmsavgcorr: at most " +
+                                       std::to_string(AVGCORR_LDS_MAX / per_atom) +
+                                       " selected atoms" + (ref ? "" : " with 'first'"));
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   std::vector<int32_t> ws(nw);
   rmsd_avgcorr_windows(frames->nframes, maxwindow, offset, ws.data(), nw);

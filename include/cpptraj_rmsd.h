@@ -211,7 +211,8 @@ int rmsd_avgcorr_windows(int nframes, int maxwindow, int offset, int32_t* window
  *            averaged frame, centred)
  *   maxwindow, offset   'stop' (-1: every frame) and 'offset' (< 1 -> 1)
  *   ct, sd   device, one double per window size (DataSet Ct and Ct[SD])
- * At most 3413 selected atoms (running sum and reference in LDS). */
+ * At most 6826 selected atoms with a reference, 3413 with 'first' (the running
+ * sum, and the 'first' reference, are kept in LDS). */
 int rmsd_avgcorr(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* ref, int ref_frame,
                  int use_mass, int maxwindow, int offset, double* ct, double* sd);
 

@@ -83,5 +83,8 @@ def test_avgcorr_errors(ctx):
     with pytest.raises(ca.RmsdError, match="Not enough frames"):
         ca.rms_avgcorr(X, ctx=ctx)
     Y = torch.zeros((4, 3 * 4000), dtype=torch.float32, device="cuda")
-    with pytest.raises(ca.RmsdError, match="selected atoms"):
+    with pytest.raises(ca.RmsdError, match="selected atoms with 'first'"):
         ca.rms_avgcorr(Y, ctx=ctx)
+    Z = torch.zeros((4, 3 * 7000), dtype=torch.float32, device="cuda")
+    with pytest.raises(ca.RmsdError, match="selected atoms"):
+        ca.rms_avgcorr(Z, ref_frames=Z[:1], ctx=ctx)
