@@ -209,10 +209,24 @@ def perres_bench(ctx, X, d, m, steps, warmup, res_atoms=15):
     exp = Oracle().perres(aos, ref0, off, atoms, tmass=mh, use_mass=True,
                           U=U[idx].cpu().numpy(), tvec=tv[idx].cpu().numpy(), modify=1)
     err = float(np.abs(res[idx].cpu().numpy() - exp).max())
-    return {"workload": f"{F} frames x {n} atoms FP32 SoA, {nres} residues of {res_atoms} atoms, "
-                        "mass-weighted, fit update applied in FP64",
-            "frames_per_s": F / (ms / 1e3), "ms_per_step": ms, "kernel_ms": main,
-            "hbm_gbs": algo / (main / 1e3) / 1e9, "max_err_vs_oracle_32_frames": err}
+    out = {"workload": f"{F} frames x {n} atoms FP32 SoA, {nres} residues of {res_atoms} atoms, "
+                       "mass-weighted, fit update applied in FP64",
+           "frames_per_s": F / (ms / 1e3), "ms_per_step": ms, "kernel_ms": main,
+           "hbm_gbs": algo / (main / 1e3) / 1e9, "max_err_vs_oracle_32_frames": err}
+    # The reference's own per-frame loop (compiled Frame::Trans_Rot_Trans,
+    # CenterOnOrigin, RMSD_NoFit; serial, as the action runs) on a sample.
+    from oracle.oracle import RefLib, ref_available
+    if ref_available():
+        ns = min(F, 40000)
+        sample = np.transpose(X[:ns].cpu().numpy(), (0, 2, 1)).reshape(ns, -1)
+        Us, ts = U[:ns].cpu().numpy(), tv[:ns].cpu().numpy()
+        t0 = time.perf_counter()
+        RefLib().perres(sample, ref0, off, atoms, tmass=mh, use_mass=True, U=Us, tvec=ts,
+                        modify=1)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": ns / dt, "unit": "frames/s", "cores": 1,
+                               "kind": "reference", "sample": f"first {ns} frames ({dt:.1f} s)"}
+    return out
 
 
 def avgcorr_bench(ctx, F=10000, n=300, steps=3, warmup=1):
@@ -243,10 +257,23 @@ def avgcorr_bench(ctx, F=10000, n=300, steps=3, warmup=1):
     w20, ct20, sd20 = o.avgcorr(X, ref_sel_centered=rc, maxwindow=20)
     err = max(float(np.abs(ct[:len(w20)].cpu().numpy() - ct20).max()),
               float(np.abs(sd[:len(w20)].cpu().numpy() - sd20).max()))
-    return {"workload": f"{F} frames x {n} atoms, fixed reference, window sizes 1..{F - 1}",
-            "windows": int(len(w)), "averaged_frames": fits, "ms_per_step": ms,
-            "averaged_frames_per_s": fits / (ms / 1e3),
-            "max_err_vs_oracle_first_20_windows": err}
+    res = {"workload": f"{F} frames x {n} atoms, fixed reference, window sizes 1..{F - 1}",
+           "windows": int(len(w)), "averaged_frames": fits, "ms_per_step": ms,
+           "averaged_frames_per_s": fits / (ms / 1e3),
+           "max_err_vs_oracle_first_20_windows": err}
+    # The reference's window loop (compiled Frame operations, serial) on the
+    # first 64 window sizes.
+    from oracle.oracle import RefLib, ref_available
+    if ref_available():
+        ws = w[:64]
+        t0 = time.perf_counter()
+        RefLib().avgcorr(X, ws, ref_sel_centered=rc)
+        dt = time.perf_counter() - t0
+        nfit = float(np.sum(F - ws + 1))
+        res["cpu_baseline"] = {"value": nfit / dt, "unit": "averaged frames/s", "cores": 1,
+                               "kind": "reference",
+                               "sample": f"window sizes {int(ws[0])}..{int(ws[-1])} ({dt:.1f} s)"}
+    return res
 
 
 def batch_bench(ctx, seed=3, nsets=64, precision="bf16x3"):
