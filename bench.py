@@ -232,9 +232,11 @@ def perres_bench(ctx, X, d, m, steps, warmup, res_atoms=15):
 def avgcorr_bench(ctx, F=10000, n=300, steps=3, warmup=1):
     """RMS average correlation ('rmsavgcorr') over F frames x n atoms (AoS
     FP32, a CA-sized selection) against a fixed reference, every window size
-    1 .. F-1: sum_w (F - w + 1) averaged frames, each one running-sum update,
-    an FP64 fit and a QCP solve.  Parity: the first 20 window sizes against the
-    oracle (stop 20)."""
+    1 .. F-1: sum_w (F - w + 1) averaged frames, each an FP64 fit and a QCP
+    solve.  Timed by both methods: the prefix form (the default for a fixed
+    reference) and the running sum.  Parity: the first 20 window sizes
+    against the oracle (stop 20), and the two methods against each other on
+    every window."""
     import torch
     import cpptraj_amd as ca
     from cpptraj_amd import synth
@@ -244,23 +246,42 @@ def avgcorr_bench(ctx, F=10000, n=300, steps=3, warmup=1):
     Xd = torch.as_tensor(X).cuda()
     R = torch.as_tensor(ref.reshape(1, -1)).cuda()
     w = ca.avgcorr_windows(F)
-    out = {}
-
-    def fn():
-        out["r"] = ca.rms_avgcorr(Xd, ref_frames=R, ctx=ctx)
-    ms = timed_steps(fn, steps, warmup)
     fits = float(np.sum(F - w + 1))
-    _, ct, sd = out["r"]
+    out, ms, stages = {}, {}, {}
+    for method in ("prefix", "running"):
+        def fn():
+            out[method] = ca.rms_avgcorr(Xd, ref_frames=R, method=method, ctx=ctx)
+        ms[method] = timed_steps(fn, steps if method == "prefix" else 1, warmup)
+        ctx.set_profiling(True)
+        fn()
+        stages[method] = ctx.stage_ms()
+        ctx.set_profiling(False)
+    ca.rms_avgcorr(Xd[:2], ref_frames=R, method="auto", ctx=ctx)  # back to the default
+    _, ct, sd = out["prefix"]
     from oracle.oracle import Oracle
     o = Oracle()
     rc, _ = o.center_on_origin(ref.astype(np.float64))
     w20, ct20, sd20 = o.avgcorr(X, ref_sel_centered=rc, maxwindow=20)
     err = max(float(np.abs(ct[:len(w20)].cpu().numpy() - ct20).max()),
               float(np.abs(sd[:len(w20)].cpu().numpy() - sd20).max()))
+    _, ct_r, sd_r = out["running"]
+    err_methods = max(float((ct - ct_r).abs().max()), float((sd - sd_r).abs().max()))
+    # Tile kernel: per (row, column) prefix-row pair 3 k FP64 flops (difference
+    # and fused multiply-add per coordinate) over 64 x 64 pairs per tile.
+    T = 64
+    nt = (F + 1 + T - 1) // T
+    tile_pairs = T * T * nt * (nt + 1) / 2
+    tile_flops = 3.0 * 3 * n * tile_pairs
     res = {"workload": f"{F} frames x {n} atoms, fixed reference, window sizes 1..{F - 1}",
-           "windows": int(len(w)), "averaged_frames": fits, "ms_per_step": ms,
-           "averaged_frames_per_s": fits / (ms / 1e3),
-           "max_err_vs_oracle_first_20_windows": err}
+           "windows": int(len(w)), "averaged_frames": fits, "ms_per_step": ms["prefix"],
+           "averaged_frames_per_s": fits / (ms["prefix"] / 1e3),
+           "prefix_stages_ms": {"prefix_and_row_moments": stages["prefix"][0],
+                                "tiles": stages["prefix"][1], "final": stages["prefix"][2]},
+           "tiles_fp64_tflops": tile_flops / (stages["prefix"][1] / 1e3) / 1e12,
+           "running_ms": ms["running"],
+           "running_averaged_frames_per_s": fits / (ms["running"] / 1e3),
+           "max_err_vs_oracle_first_20_windows": err,
+           "max_diff_prefix_vs_running": err_methods}
     # The reference's window loop (compiled Frame operations, serial) on the
     # first 64 window sizes.
     from oracle.oracle import RefLib, ref_available
@@ -558,8 +579,9 @@ def main():
     if not args.skip_avgcorr:
         log("rms average correlation")
         line["avgcorr"] = avgcorr_bench(ctx)
-        log(f"avgcorr {line['avgcorr']['ms_per_step']:.1f} ms, "
-            f"{line['avgcorr']['averaged_frames_per_s']:.3g} averaged frames/s")
+        log(f"avgcorr {line['avgcorr']['ms_per_step']:.1f} ms prefix form "
+            f"({line['avgcorr']['averaged_frames_per_s']:.3g} averaged frames/s), "
+            f"{line['avgcorr']['running_ms']:.1f} ms running sum")
     ctx.close()
     print(json.dumps(line), flush=True)
 

@@ -132,6 +132,8 @@ def _optional(L):
         L.rmsd_avgcorr_windows.restype = C.c_int
         L.rmsd_avgcorr.argtypes = [P, FP, FP, C.c_int, C.c_int, C.c_int, C.c_int, P, P]
         L.rmsd_avgcorr.restype = C.c_int
+        L.rmsd_avgcorr_set_method.argtypes = [P, C.c_int]
+        L.rmsd_avgcorr_set_method.restype = C.c_int
     if hasattr(L, "rmsd_perres_nofit"):
         IP = C.POINTER(C.c_int32)
         DP = C.POINTER(C.c_double)
@@ -389,10 +391,15 @@ def avgcorr_windows(nframes, maxwindow=-1, offset=1):
     return w
 
 
+AVGCORR_METHODS = {"auto": 0, "running": 1, "prefix": 2}
+
+
 def rms_avgcorr(frames, mask=None, mass=None, ref_frames=None, ref_index=0, ref_mask=None,
                 ref_mass=None, use_mass=False, maxwindow=-1, offset=1, soa=False, natom=None,
-                ctx: Context | None = None):
+                method=None, ctx: Context | None = None):
     """RMS average correlation ('rmsavgcorr').  ref_frames None = 'first'.
+    method ("auto", "running", "prefix"; rmsd_avgcorr_set_method), when given,
+    is set on the context first and stays there.
     Returns (window sizes, mean RMSD, SD), the last two float64 CUDA tensors."""
     import torch
 
@@ -402,6 +409,10 @@ def rms_avgcorr(frames, mask=None, mass=None, ref_frames=None, ref_index=0, ref_
     if ref_frames is not None:
         r = frames_desc(ref_frames, ref_mask if ref_mask is not None else mask,
                         ref_mass if ref_mass is not None else mass, soa=soa, natom=natom)
+    if method is not None:
+        if method not in AVGCORR_METHODS:
+            raise RmsdError(f"unknown rmsavgcorr method {method!r}")
+        ctx.check(ctx._L.rmsd_avgcorr_set_method(ctx.handle, AVGCORR_METHODS[method]))
     w = avgcorr_windows(d.nframes, maxwindow, offset)
     ct = torch.empty(len(w), dtype=torch.float64, device=frames.device)
     sd = torch.empty(len(w), dtype=torch.float64, device=frames.device)

@@ -677,6 +677,15 @@ int rmsd_avgcorr_windows(int nframes, int maxwindow, int offset, int32_t* window
   return n;
 }
 
+int rmsd_avgcorr_set_method(rmsd_ctx* ctx, int method) {
+  if (!ctx) return RMSD_ERR_ARG;
+  if (method != RMSD_AVGCORR_AUTO && method != RMSD_AVGCORR_RUNNING &&
+      method != RMSD_AVGCORR_PREFIX)
+    return fail(ctx, RMSD_ERR_ARG, "unknown rmsavgcorr method " + std::to_string(method));
+  ctx->avgcorr_method = method;
+  return RMSD_OK;
+}
+
 int rmsd_avgcorr(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* ref, int ref_frame,
                  int use_mass, int maxwindow, int offset, double* ct, double* sd) {
   if (!ctx) return RMSD_ERR_ARG;
@@ -694,12 +703,30 @@ int rmsd_avgcorr(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* re
   const int nw = rmsd_avgcorr_windows(frames->nframes, maxwindow, offset, nullptr, 0);
   if (nw < 1) return fail(ctx, RMSD_ERR_ARG, "Not enough frames to perform calculation.");
   if (!ct || !sd) return fail(ctx, RMSD_ERR_ARG, "ct or sd is NULL");
-  // LDS: the running sum, and the reference in 'first' mode (FP64 [3][nsel]).
-  const size_t per_atom = sizeof(double) * (ref ? 3 : 6);
-  if (per_atom * (size_t)frames->nsel > AVGCORR_LDS_MAX)
-    return fail(ctx, RMSD_ERR_ARG, "rmsavgcorr: at most " +
-                                       std::to_string(AVGCORR_LDS_MAX / per_atom) +
-                                       " selected atoms" + (ref ? "" : " with 'first'"));
+  // Method: the prefix form needs a fixed reference and its memory.
+  const int64_t F = frames->nframes;
+  const int k = 3 * frames->nsel;
+  const int ntile = (int)((F + 1 + rmsd_avgcorr_tile() - 1) / rmsd_avgcorr_tile());
+  const size_t prefix_bytes = sizeof(double) * ((size_t)(F + 1) * k +
+                                                (size_t)rmsd_avgcorr_chunks(F) * k + 2 * k +
+                                                (size_t)(F + 2) * 13 + (size_t)nw * ntile * 4);
+  int method = ctx->avgcorr_method;
+  if (method == RMSD_AVGCORR_AUTO)
+    method = ref && prefix_bytes <= AVGCORR_PREFIX_MAX ? RMSD_AVGCORR_PREFIX : RMSD_AVGCORR_RUNNING;
+  if (method == RMSD_AVGCORR_PREFIX) {
+    if (!ref) return fail(ctx, RMSD_ERR_ARG, "rmsavgcorr: the prefix method needs a fixed reference");
+    if (prefix_bytes > AVGCORR_PREFIX_MAX)
+      return fail(ctx, RMSD_ERR_ARG, "rmsavgcorr: the prefix method needs " +
+                                         std::to_string(prefix_bytes) + " bytes, over " +
+                                         std::to_string(AVGCORR_PREFIX_MAX));
+  } else {
+    // LDS: the running sum, and the reference in 'first' mode (FP64 [3][nsel]).
+    const size_t per_atom = sizeof(double) * (ref ? 3 : 6);
+    if (per_atom * (size_t)frames->nsel > AVGCORR_LDS_MAX)
+      return fail(ctx, RMSD_ERR_ARG, "rmsavgcorr: at most " +
+                                         std::to_string(AVGCORR_LDS_MAX / per_atom) +
+                                         " selected atoms" + (ref ? "" : " with 'first'"));
+  }
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   std::vector<int32_t> ws(nw);
   rmsd_avgcorr_windows(frames->nframes, maxwindow, offset, ws.data(), nw);
@@ -708,34 +735,90 @@ int rmsd_avgcorr(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* re
   HIP_TRY(ctx, hipMemcpyAsync(dws, ws.data(), sizeof(int32_t) * nw, hipMemcpyHostToDevice,
                               ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // ws is a local buffer
-  AvgCorrArgs a;
-  a.fs = to_dev(*frames);
-  a.mass = use_mass ? frames->mass : nullptr;
-  a.ref = nullptr;
-  a.consts = nullptr;
-  a.ldr = 0;
-  a.windows = dws;
-  a.nwin = nw;
-  a.ct = ct;
-  a.sd = sd;
   ctx->marks = 0;
   mark(ctx, 0);
+  double* refbuf = nullptr;
+  double* consts = nullptr;
+  const int ldr = (int)round_up(frames->nsel, 4);
   if (ref) {
-    const int ldr = (int)round_up(frames->nsel, 4);
-    double* refbuf = (double*)scratch(ctx, "avgcorr.ref", sizeof(double) * (3 * ldr + 8), &st);
+    refbuf = (double*)scratch(ctx, "avgcorr.ref", sizeof(double) * (3 * ldr + 8), &st);
     if (!refbuf) return st;
-    double* consts = (double*)scratch(ctx, "avgcorr.consts", sizeof(double) * 8, &st);
+    consts = (double*)scratch(ctx, "avgcorr.consts", sizeof(double) * 8, &st);
     if (!consts) return st;
     HIP_TRY(ctx, rmsd_launch_ref_prep(to_dev(*ref), ref_frame, use_mass ? ref->mass : nullptr,
                                       use_mass ? frames->mass : nullptr, 1, refbuf,
                                       refbuf + 3 * ldr, consts, ctx->stream));
+  }
+  if (method == RMSD_AVGCORR_RUNNING) {
+    AvgCorrArgs a;
+    a.fs = to_dev(*frames);
+    a.mass = use_mass ? frames->mass : nullptr;
     a.ref = refbuf;
     a.consts = consts;
-    a.ldr = ldr;
+    a.ldr = ref ? ldr : 0;
+    a.windows = dws;
+    a.nwin = nw;
+    a.ct = ct;
+    a.sd = sd;
+    mark(ctx, 1);
+    HIP_TRY(ctx, rmsd_launch_avgcorr(a, ref == nullptr, ctx->stream));
+    mark(ctx, 2);
+    mark(ctx, 3);
+    return RMSD_OK;
   }
+  // Prefix form: window map and the (row tile, column tile) pairs that hold
+  // a window diagonal.
+  const int T = rmsd_avgcorr_tile();
+  std::vector<int32_t> wmap((size_t)F + 1, -1);
+  for (int i = 0; i < nw; ++i) wmap[ws[i]] = i;
+  std::vector<int32_t> cnt((size_t)F + 2, 0);  // cnt[w + 1] = windows <= w
+  for (int64_t w = 0; w <= F; ++w) cnt[w + 1] = cnt[w] + (wmap[w] >= 0 ? 1 : 0);
+  std::vector<int2> tiles;
+  for (int ti = 0; ti < ntile; ++ti)
+    for (int tj = 0; tj <= ti; ++tj) {
+      const int64_t base = (int64_t)T * (ti - tj);
+      const int64_t lo = std::max<int64_t>(1, base - (T - 1)), hi = std::min<int64_t>(F, base + T - 1);
+      if (lo <= hi && cnt[hi + 1] > cnt[lo]) tiles.push_back(make_int2(ti, tj));
+    }
+  AvgPrefixArgs a;
+  a.fs = to_dev(*frames);
+  a.mass = use_mass ? frames->mass : nullptr;
+  a.ref = refbuf;
+  a.consts = consts;
+  a.ldr = ldr;
+  a.k = k;
+  a.chunk = (double*)scratch(ctx, "avgcorr.chunk", sizeof(double) * rmsd_avgcorr_chunks(F) * k, &st);
+  if (!a.chunk) return st;
+  a.mu = (double*)scratch(ctx, "avgcorr.mu", sizeof(double) * k, &st);
+  if (!a.mu) return st;
+  a.P = (double*)scratch(ctx, "avgcorr.prefix", sizeof(double) * (size_t)(F + 1) * k, &st);
+  if (!a.P) return st;
+  a.rows = (double*)scratch(ctx, "avgcorr.rows", sizeof(double) * (size_t)(F + 2) * 13, &st);
+  if (!a.rows) return st;
+  int32_t* dmap = (int32_t*)scratch(ctx, "avgcorr.wmap", sizeof(int32_t) * (size_t)(F + 1), &st);
+  if (!dmap) return st;
+  int2* dtiles = (int2*)scratch(ctx, "avgcorr.tiles", sizeof(int2) * tiles.size(), &st);
+  if (!dtiles) return st;
+  a.part = (double*)scratch(ctx, "avgcorr.part", sizeof(double) * (size_t)nw * ntile * 4, &st);
+  if (!a.part) return st;
+  HIP_TRY(ctx, hipMemcpyAsync(dmap, wmap.data(), sizeof(int32_t) * wmap.size(),
+                              hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(dtiles, tiles.data(), sizeof(int2) * tiles.size(),
+                              hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // host vectors are local
+  a.wmap = dmap;
+  a.tiles = dtiles;
+  a.ntiles = (int)tiles.size();
+  a.ntile = ntile;
+  a.windows = dws;
+  a.nwin = nw;
+  a.ct = ct;
+  a.sd = sd;
+  HIP_TRY(ctx, rmsd_launch_avgcorr_prefix(a, ctx->stream));
   mark(ctx, 1);
-  HIP_TRY(ctx, rmsd_launch_avgcorr(a, ref == nullptr, ctx->stream));
+  HIP_TRY(ctx, rmsd_launch_avgcorr_tiles(a, ctx->stream));
   mark(ctx, 2);
+  HIP_TRY(ctx, rmsd_launch_avgcorr_final(a, ctx->stream));
   mark(ctx, 3);
   return RMSD_OK;
 }

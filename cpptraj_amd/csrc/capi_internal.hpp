@@ -42,6 +42,7 @@ struct rmsd_ctx {
   // Stage timing: events recorded on the context's stream around the per-frame
   // stage, the main kernel and the epilogue stage of the last call.
   bool profiling = false;
+  int avgcorr_method = 0;  // RMSD_AVGCORR_*
   bool events_made = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int marks = 0;

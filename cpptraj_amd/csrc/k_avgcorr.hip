@@ -212,3 +212,313 @@ hipError_t rmsd_launch_avgcorr(const AvgCorrArgs& a, bool first, hipStream_t s) 
   void* args[] = {const_cast<AvgCorrArgs*>(&a)};
   return hipLaunchKernel(fn, dim3(a.nwin), dim3(AVG_THREADS), args, lds, s);
 }
+
+// ---- prefix-sum form (fixed reference) --------------------------------------------
+// With the selected coordinates x_f and their per-coordinate mean mu, the
+// prefix sums P_i = sum_{f < i} (x_f - mu) (i = 0..F, P_0 = 0) give the
+// window-w average ending at frame t (i = t + 1, j = i - w) as
+// a = (P_i - P_j) / w + mu.  With d = P_i - P_j, v = mu - mu_c (mu_c the
+// weighted centroid of mu) and the fit centroid c = (sum m d) / (w W) + mu_c:
+//   sum m a y^T       = (Q_i - Q_j) / w + sum m mu y^T
+//   sum m |a - c|^2   = sum m |d|^2 / w^2 + 2 (L_i - L_j) / w + sum m |v|^2
+//                       - W |(C_i - C_j) / (w W)|^2
+// with per-prefix moments C_i = sum m P_i, Q_i = sum m P_i y^T and
+// L_i = sum m v . P_i.  Only sum m |d|^2 needs all coordinates of a pair:
+// it is a squared distance between two prefix rows, computed from the
+// difference itself (no |P_i|^2 + |P_j|^2 - 2 P_i . P_j cancellation), in
+// 64 x 64 tiles of (i, j) pairs.  Each tile fits the pairs whose i - j is a
+// window size and sums their RMSDs per diagonal; a last pass adds the
+// per-tile sums of every window in a fixed order.  Subtracting mu first
+// keeps the prefix sums, and the digits their differences lose, small.
+namespace {
+
+constexpr int AC_CHUNK = 128;  // frames per scan chunk
+constexpr int AC_THREADS = 256;
+constexpr int AC_TILE = 64;    // prefix rows per tile side
+constexpr int AC_KC = 32;      // coordinates per LDS stage
+constexpr int AC_LD = AC_TILE + 1;
+
+__global__ __launch_bounds__(AC_THREADS) void k_ac_chunksum(AvgPrefixArgs a) {
+  const int j = blockIdx.x * AC_THREADS + threadIdx.x;
+  const int q = blockIdx.y;
+  if (j >= a.k) return;
+  const int kk = j / 3, c = j - 3 * kk;
+  const int64_t f0 = (int64_t)q * AC_CHUNK;
+  const int64_t f1 = f0 + AC_CHUNK < a.fs.nframes ? f0 + AC_CHUNK : a.fs.nframes;
+  double s = 0.0;
+  for (int64_t f = f0; f < f1; ++f) s += (double)coord(a.fs, f, kk, c);
+  a.chunk[(int64_t)q * a.k + j] = s;
+}
+
+// Coordinate means, and the chunk sums turned into exclusive prefix sums of
+// the mean-subtracted coordinates.
+__global__ __launch_bounds__(AC_THREADS) void k_ac_scan(AvgPrefixArgs a) {
+  const int j = blockIdx.x * AC_THREADS + threadIdx.x;
+  if (j >= a.k) return;
+  const int nq = (int)((a.fs.nframes + AC_CHUNK - 1) / AC_CHUNK);
+  double tot = 0.0;
+  for (int q = 0; q < nq; ++q) tot += a.chunk[(int64_t)q * a.k + j];
+  const double mu = tot / (double)a.fs.nframes;
+  a.mu[j] = mu;
+  double off = 0.0;
+  for (int q = 0; q < nq; ++q) {
+    const int64_t f0 = (int64_t)q * AC_CHUNK;
+    const int64_t cnt = f0 + AC_CHUNK < a.fs.nframes ? AC_CHUNK : a.fs.nframes - f0;
+    const double s = a.chunk[(int64_t)q * a.k + j];
+    a.chunk[(int64_t)q * a.k + j] = off;
+    off += s - (double)cnt * mu;
+  }
+  a.P[j] = 0.0;  // P_0
+}
+
+// P rows 1..F, each coordinate times sqrt(mass) when weighted.
+__global__ __launch_bounds__(AC_THREADS) void k_ac_prefix(AvgPrefixArgs a) {
+  const int j = blockIdx.x * AC_THREADS + threadIdx.x;
+  const int q = blockIdx.y;
+  if (j >= a.k) return;
+  const int kk = j / 3, c = j - 3 * kk;
+  const double mu = a.mu[j];
+  const double sw = a.mass ? sqrt(a.mass[kk]) : 1.0;
+  double run = a.chunk[(int64_t)q * a.k + j];
+  const int64_t f0 = (int64_t)q * AC_CHUNK;
+  const int64_t f1 = f0 + AC_CHUNK < a.fs.nframes ? f0 + AC_CHUNK : a.fs.nframes;
+  for (int64_t f = f0; f < f1; ++f) {
+    run += (double)coord(a.fs, f, kk, c) - mu;
+    a.P[(f + 1) * a.k + j] = run * sw;
+  }
+}
+
+// Constants (row F + 1 of rows): sum m mu (3), sum m mu y^T (9),
+// sum m |v|^2 (1).  One wave.
+__global__ __launch_bounds__(64) void k_ac_consts(AvgPrefixArgs a) {
+  const int n = a.fs.nsel;
+  const double W = a.consts[4];
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int kk = threadIdx.x; kk < n; kk += 64) {
+    const double m = a.mass ? a.mass[kk] : 1.0;
+    for (int c = 0; c < 3; ++c) s[c] += m * a.mu[3 * kk + c];
+  }
+  double mc[3];
+  for (int c = 0; c < 3; ++c) {
+    s[c] = rmsd_dev::wave_sum(s[c]);
+    mc[c] = W > 0.0 ? s[c] / W : 0.0;
+  }
+  double v[13];
+#pragma unroll
+  for (int q = 0; q < 13; ++q) v[q] = 0.0;
+  for (int kk = threadIdx.x; kk < n; kk += 64) {
+    const double m = a.mass ? a.mass[kk] : 1.0;
+    const double y0 = a.ref[kk], y1 = a.ref[a.ldr + kk], y2 = a.ref[2 * a.ldr + kk];
+    for (int c = 0; c < 3; ++c) {
+      const double mm = m * a.mu[3 * kk + c];
+      const double vc = a.mu[3 * kk + c] - mc[c];
+      v[3 + 3 * c] += mm * y0;
+      v[4 + 3 * c] += mm * y1;
+      v[5 + 3 * c] += mm * y2;
+      v[12] += m * vc * vc;
+    }
+  }
+  double r[13];
+  wave_moments(v, r);
+  if (threadIdx.x == 0) {
+    double* o = a.rows + (a.fs.nframes + 1) * 13;
+    for (int c = 0; c < 3; ++c) o[c] = s[c];
+    for (int q = 3; q < 13; ++q) o[q] = r[q];
+  }
+}
+
+// Per prefix row i (0..F): C_i (3), Q_i (9), L_i (1).  One wave per row.
+__global__ __launch_bounds__(64) void k_ac_rows(AvgPrefixArgs a) {
+  const int64_t i = blockIdx.x;
+  const int n = a.fs.nsel;
+  const double W = a.consts[4];
+  const double* cr = a.rows + (a.fs.nframes + 1) * 13;
+  double mc[3];
+  for (int c = 0; c < 3; ++c) mc[c] = W > 0.0 ? cr[c] / W : 0.0;
+  const double* Pi = a.P + i * a.k;
+  double v[13];
+#pragma unroll
+  for (int q = 0; q < 13; ++q) v[q] = 0.0;
+  for (int kk = threadIdx.x; kk < n; kk += 64) {
+    const double sw = a.mass ? sqrt(a.mass[kk]) : 1.0;
+    const double y0 = a.ref[kk], y1 = a.ref[a.ldr + kk], y2 = a.ref[2 * a.ldr + kk];
+    for (int c = 0; c < 3; ++c) {
+      const double mp = sw * Pi[3 * kk + c];  // m P
+      v[c] += mp;
+      v[3 + 3 * c] += mp * y0;
+      v[4 + 3 * c] += mp * y1;
+      v[5 + 3 * c] += mp * y2;
+      v[12] += mp * (a.mu[3 * kk + c] - mc[c]);
+    }
+  }
+  double r[13];
+  wave_moments(v, r);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 13; ++q) a.rows[i * 13 + q] = r[q];
+}
+
+// One 64 x 64 tile of (i, j) prefix-row pairs.  Thread (ty, tx) = (t / 16,
+// t % 16) owns rows i0 + ty + 16 a and j0 + tx + 16 b (a, b < 4).
+__global__ __launch_bounds__(AC_THREADS) void k_ac_tiles(AvgPrefixArgs a) {
+  // [2][KC][65] staged coordinates (odd row stride: conflict-free transposed
+  // stores), later the RMSDs [64][65].
+  __shared__ double sm[2 * AC_KC * AC_LD];
+  static_assert(2 * AC_KC * AC_LD >= AC_TILE * AC_LD, "RMSD tile does not fit");
+  double* si = sm;
+  double* sj = sm + AC_KC * AC_LD;
+  const int2 tile = a.tiles[blockIdx.x];
+  const int ti = tile.x, tj = tile.y;
+  const int64_t i0 = (int64_t)ti * AC_TILE, j0 = (int64_t)tj * AC_TILE;
+  const int64_t nrow = a.fs.nframes + 1;
+  const int t = threadIdx.x, ty = t >> 4, tx = t & 15;
+  double acc[4][4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[p][q] = 0.0;
+  // Staging: thread t loads coordinate (t % 32) of rows (t / 32) + 8 r.
+  const int lc = t & (AC_KC - 1), lr = t >> 5;
+  for (int k0 = 0; k0 < a.k; k0 += AC_KC) {
+    const int kc = k0 + lc;
+#pragma unroll
+    for (int r = 0; r < AC_TILE / 8; ++r) {
+      const int row = lr + 8 * r;
+      const int64_t gi = i0 + row, gj = j0 + row;
+      si[lc * AC_LD + row] = (kc < a.k && gi < nrow) ? a.P[gi * a.k + kc] : 0.0;
+      sj[lc * AC_LD + row] = (kc < a.k && gj < nrow) ? a.P[gj * a.k + kc] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int c = 0; c < AC_KC; ++c) {
+      double pi[4], pj[4];
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        pi[p] = si[c * AC_LD + ty + 16 * p];
+        pj[p] = sj[c * AC_LD + tx + 16 * p];
+      }
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double d = pi[p] - pj[q];
+          acc[p][q] = fma(d, d, acc[p][q]);
+        }
+    }
+    __syncthreads();
+  }
+  // Fits of the pairs on window diagonals.
+  const double* K = a.consts;  // sum w y (3), sum w |y|^2, W
+  const double W = K[4];
+  const double* cm = a.rows + nrow * 13;
+  double* rr = sm;
+  for (int p = 0; p < 4; ++p)
+    for (int q = 0; q < 4; ++q) {
+      const int il = ty + 16 * p, jl = tx + 16 * q;
+      const int64_t i = i0 + il, j = j0 + jl;
+      const int64_t w = i - j;
+      double r = 0.0;
+      if (w >= 1 && i < nrow && a.wmap[w] >= 0) {
+        if (W < RMSD_SMALL) {
+          r = -1.0;
+        } else {
+          const double dw = (double)w;
+          const double* ri = a.rows + i * 13;
+          const double* rj = a.rows + j * 13;
+          double dc[3], c[3];
+          for (int e = 0; e < 3; ++e) {
+            dc[e] = (ri[e] - rj[e]) / (dw * W);
+            c[e] = dc[e] + cm[e] / W;
+          }
+          double R[9];
+          for (int e = 0; e < 9; ++e)
+            R[e] = (ri[3 + e] - rj[3 + e]) / dw + cm[3 + e] - c[e / 3] * K[e % 3];
+          const double sx = acc[p][q] / (dw * dw) + 2.0 * (ri[12] - rj[12]) / dw + cm[12] -
+                            W * (dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2]);
+          const double E0 = 0.5 * (sx + K[3]);
+          r = rmsd_dev::qcp_rmsd(R, E0, W, 1);
+        }
+      }
+      rr[il * AC_LD + jl] = r;
+    }
+  __syncthreads();
+  // Diagonal sums: thread dd + 63 takes local diagonal dd = il - jl.
+  if (t < 2 * AC_TILE - 1) {
+    const int dd = t - (AC_TILE - 1);
+    const int64_t w = (i0 - j0) + dd;
+    if (w >= 1 && w < nrow && a.wmap[w] >= 0) {
+      double s1 = 0.0, s2 = 0.0;
+      const int il0 = dd > 0 ? dd : 0, il1 = dd < 0 ? AC_TILE + dd : AC_TILE;
+      for (int il = il0; il < il1; ++il) {
+        const double r = rr[il * AC_LD + il - dd];
+        s1 += r;
+        s2 += r * r;
+      }
+      // Slot of this tile among the two j tiles that diagonal w crosses in
+      // row tile ti.
+      const int half = tj - (ti - (int)((w + AC_TILE - 1) / AC_TILE));
+      double* o = a.part + (((int64_t)a.wmap[w] * a.ntile + ti) * 2 + half) * 2;
+      o[0] = s1;
+      o[1] = s2;
+    }
+  }
+}
+
+// Mean and standard deviation per window from the per-tile sums, in a fixed
+// order.  One wave per window.
+__global__ __launch_bounds__(64) void k_ac_final(AvgPrefixArgs a) {
+  const int wi = blockIdx.x;
+  const int w = a.windows[wi];
+  double s1 = 0.0, s2 = 0.0;
+  for (int slot = threadIdx.x; slot < 2 * a.ntile; slot += 64) {
+    const int ti = slot >> 1, half = slot & 1;
+    // The tile (ti, tj) that holds this slot; written when it exists and
+    // diagonal w crosses it (a multiple of 64 crosses one tile per row tile).
+    const int tj = ti - (w + AC_TILE - 1) / AC_TILE + half;
+    if (tj < 0 || tj > ti) continue;
+    const int off = w - AC_TILE * (ti - tj);
+    if (off < 1 - AC_TILE || off > AC_TILE - 1) continue;
+    const double* o = a.part + (((int64_t)wi * a.ntile + ti) * 2 + half) * 2;
+    s1 += o[0];
+    s2 += o[1];
+  }
+  s1 = rmsd_dev::wave_sum(s1);
+  s2 = rmsd_dev::wave_sum(s2);
+  if (threadIdx.x == 0) {
+    const double dw = (double)w;
+    const double d = 1.0 / ((double)a.fs.nframes - dw + 1.0);
+    const double avg = s1 * d;
+    double sd = s2 * d;
+    sd -= avg * avg;
+    a.ct[wi] = avg;
+    a.sd[wi] = sd > 0.0 ? sqrt(sd) : 0.0;
+  }
+}
+
+}  // namespace
+
+int rmsd_avgcorr_chunks(int64_t nframes) { return (int)((nframes + AC_CHUNK - 1) / AC_CHUNK); }
+int rmsd_avgcorr_tile() { return AC_TILE; }
+
+hipError_t rmsd_launch_avgcorr_prefix(const AvgPrefixArgs& a, hipStream_t s) {
+  if (a.nwin == 0 || a.ntiles == 0) return hipSuccess;
+  const int nq = rmsd_avgcorr_chunks(a.fs.nframes);
+  const dim3 gj((a.k + AC_THREADS - 1) / AC_THREADS, nq);
+  hipLaunchKernelGGL(k_ac_chunksum, gj, dim3(AC_THREADS), 0, s, a);
+  hipLaunchKernelGGL(k_ac_scan, dim3(gj.x), dim3(AC_THREADS), 0, s, a);
+  hipLaunchKernelGGL(k_ac_prefix, gj, dim3(AC_THREADS), 0, s, a);
+  hipLaunchKernelGGL(k_ac_consts, dim3(1), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(k_ac_rows, dim3((unsigned)(a.fs.nframes + 1)), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t rmsd_launch_avgcorr_tiles(const AvgPrefixArgs& a, hipStream_t s) {
+  if (a.nwin == 0 || a.ntiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ac_tiles, dim3(a.ntiles), dim3(AC_THREADS), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t rmsd_launch_avgcorr_final(const AvgPrefixArgs& a, hipStream_t s) {
+  if (a.nwin == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ac_final, dim3(a.nwin), dim3(64), 0, s, a);
+  return hipGetLastError();
+}

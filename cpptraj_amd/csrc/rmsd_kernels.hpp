@@ -180,6 +180,38 @@ struct AvgCorrArgs {
 };
 constexpr size_t AVGCORR_LDS_MAX = size_t(160) << 10;  // running sum + reference
 hipError_t rmsd_launch_avgcorr(const AvgCorrArgs& a, bool first, hipStream_t s);
+// Prefix-sum form of the RMS average correlation with a fixed reference
+// (k_avgcorr.hip): prefix sums of the mean-subtracted coordinates and their
+// per-row moments, then 64 x 64 tiles of prefix-row pairs (i, j) whose
+// difference i - j is a window size, fitted and summed per diagonal.
+struct AvgPrefixArgs {
+  FrameSetDev fs;
+  const double* mass;     // nsel or null
+  const double* ref;      // centred reference SoA [3][ldr]
+  const double* consts;   // reference constants (sum w y, sum w |y|^2, W)
+  int ldr;
+  int k;                  // 3 nsel
+  double* chunk;          // [nchunks][k] scan scratch
+  double* mu;             // [k] coordinate means
+  double* P;              // [F+1][k] prefix sums (row 0 = 0), times sqrt(mass) when weighted
+  double* rows;           // [F+2][13] per-row moments; row F+1 = constants
+  const int* wmap;        // [F+1] window index of a window size, else -1
+  const int2* tiles;      // (row tile, column tile) pairs to run
+  int ntiles;
+  int ntile;              // row tiles: ceil((F+1) / 64)
+  double* part;           // [nwin][ntile][2][2] per-tile sums of RMSD and RMSD^2
+  const int* windows;
+  int nwin;
+  double* ct;
+  double* sd;
+};
+int rmsd_avgcorr_chunks(int64_t nframes);
+int rmsd_avgcorr_tile();
+hipError_t rmsd_launch_avgcorr_prefix(const AvgPrefixArgs& a, hipStream_t s);
+hipError_t rmsd_launch_avgcorr_tiles(const AvgPrefixArgs& a, hipStream_t s);
+hipError_t rmsd_launch_avgcorr_final(const AvgPrefixArgs& a, hipStream_t s);
+// Device bytes the prefix form may take (P, moments and per-tile sums).
+constexpr size_t AVGCORR_PREFIX_MAX = size_t(16) << 30;
 
 hipError_t rmsd_launch_setup(const SetupArgs& a, hipStream_t s);
 hipError_t rmsd_launch_prep(const PrepArgs& a, hipStream_t s);

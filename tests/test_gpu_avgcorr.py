@@ -10,6 +10,14 @@ formula returns rounding noise up to ~sqrt(eps E0 / M) ~ 1e-7 A and QCP
 the bar there is 5e-7 A.  Cases:
 fixed reference and 'first', mass weighting, 'stop' and 'offset', a
 selection, SoA frames, and the argument errors.
+
+A fixed reference runs by either method of rmsd_avgcorr_set_method: the
+running sum (one wavefront per window) or the prefix form (window sums as
+differences of prefix sums of the mean-subtracted coordinates, every
+averaged frame fitted independently in 64 x 64 tiles of prefix rows).  Both
+meet the same bar; the prefix form also past the running method's LDS atom
+limit, across many tiles, on a drifting trajectory, and against the running
+method on 3,000 frames.
 """
 import numpy as np
 import pytest
@@ -21,10 +29,19 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-9
 
 
+def _avgcorr(ca, ctx, *args, method="auto", **kw):
+    """rms_avgcorr with the method set for this call only (the setting stays
+    with the shared context otherwise)."""
+    try:
+        return ca.rms_avgcorr(*args, method=method, ctx=ctx, **kw)
+    finally:
+        ctx.check(ctx._L.rmsd_avgcorr_set_method(ctx.handle, 0))
+
+
 @pytest.mark.parametrize("use_mass", [False, True])
-@pytest.mark.parametrize("first", [False, True])
+@pytest.mark.parametrize("first,method", [(False, "running"), (False, "prefix"), (True, "running")])
 @pytest.mark.parametrize("stop_offset", [(-1, 1), (40, 3), (-1, 7)])
-def test_avgcorr_matches_oracle(ctx, oracle, use_mass, first, stop_offset):
+def test_avgcorr_matches_oracle(ctx, oracle, use_mass, first, method, stop_offset):
     import torch
 
     import cpptraj_amd as ca
@@ -43,8 +60,8 @@ def test_avgcorr_matches_oracle(ctx, oracle, use_mass, first, stop_offset):
         ref = (X[3].reshape(-1, 3) + rng.normal(size=(n, 3)) * 0.3).astype(np.float32)
         R = torch.as_tensor(ref.reshape(1, -1)).cuda()
         rc, _ = oracle.center_on_origin(ref[sel].astype(np.float64), m[sel], use_mass)
-    w, ct, sd = ca.rms_avgcorr(Xd, seld, md, ref_frames=R, use_mass=use_mass, maxwindow=stop,
-                               offset=offset, ctx=ctx)
+    w, ct, sd = _avgcorr(ca, ctx, Xd, seld, md, ref_frames=R, use_mass=use_mass, maxwindow=stop,
+                         offset=offset, method=method)
     w_o, ct_o, sd_o = oracle.avgcorr(X, sel, m[sel], rc, use_mass, stop, offset)
     assert np.array_equal(w, w_o)
     tol = 5e-7 if first else TOL
@@ -52,7 +69,8 @@ def test_avgcorr_matches_oracle(ctx, oracle, use_mass, first, stop_offset):
     assert np.abs(sd.cpu().numpy() - sd_o).max() <= tol
 
 
-def test_avgcorr_soa_frames(ctx, oracle):
+@pytest.mark.parametrize("method", ["running", "prefix"])
+def test_avgcorr_soa_frames(ctx, oracle, method):
     import torch
 
     import cpptraj_amd as ca
@@ -67,12 +85,75 @@ def test_avgcorr_soa_frames(ctx, oracle):
     RX = np.stack([X[k] + rng.normal(size=(n, 3)).astype(np.float32) * 0.4 for k in (1, 4, 7)])
     RS = np.zeros((3, 3, ld), np.float32)
     RS[:, :, :n] = RX.transpose(0, 2, 1)
-    w, ct, sd = ca.rms_avgcorr(Sd, ref_frames=torch.as_tensor(RS).cuda(), ref_index=2, soa=True,
-                               natom=n, offset=2, ctx=ctx)
+    w, ct, sd = _avgcorr(ca, ctx, Sd, ref_frames=torch.as_tensor(RS).cuda(), ref_index=2,
+                         soa=True, natom=n, offset=2, method=method)
     rc, _ = oracle.center_on_origin(RX[2].astype(np.float64))
     _, ct_o, sd_o = oracle.avgcorr(X.reshape(F, -1), ref_sel_centered=rc, offset=2)
     assert np.abs(ct.cpu().numpy() - ct_o).max() <= TOL
     assert np.abs(sd.cpu().numpy() - sd_o).max() <= TOL
+
+
+@pytest.mark.parametrize("use_mass", [False, True])
+def test_avgcorr_prefix_many_tiles_with_drift(ctx, oracle, use_mass):
+    # 300 frames (5 x 5 row tiles, windows on every diagonal class) of a
+    # trajectory that drifts 8 A, so that the prefix sums are large.
+    import torch
+
+    import cpptraj_amd as ca
+    rng = np.random.default_rng(77)
+    F, n = 300, 60
+    X = synth.noisy(11, F, n, 0.4).reshape(F, n, 3).astype(np.float64)
+    X += np.linspace(0.0, 8.0, F)[:, None, None] * rng.normal(size=(1, n, 3)) + 40.0
+    X = X.reshape(F, -1).astype(np.float32)
+    m = synth.masses(rng, n)
+    ref = (X[5].reshape(n, 3) + rng.normal(size=(n, 3)) * 0.2).astype(np.float32)
+    rc, _ = oracle.center_on_origin(ref.astype(np.float64), m, use_mass)
+    for stop, offset in ((-1, 1), (200, 64), (130, 5)):
+        w, ct, sd = _avgcorr(ca, ctx, torch.as_tensor(X).cuda(), mass=torch.as_tensor(m).cuda(),
+                             ref_frames=torch.as_tensor(ref.reshape(1, -1)).cuda(),
+                             use_mass=use_mass, maxwindow=stop, offset=offset, method="prefix")
+        w_o, ct_o, sd_o = oracle.avgcorr(X, None, m, rc, use_mass, stop, offset)
+        assert np.array_equal(w, w_o)
+        assert np.abs(ct.cpu().numpy() - ct_o).max() <= TOL
+        assert np.abs(sd.cpu().numpy() - sd_o).max() <= TOL
+
+
+def test_avgcorr_prefix_beyond_running_atom_limit(ctx, oracle):
+    # 7,000 selected atoms: over the running method's LDS limit; the default
+    # (auto) method takes the prefix form.
+    import torch
+
+    import cpptraj_amd as ca
+    rng = np.random.default_rng(78)
+    F, n = 24, 7000
+    X = synth.noisy(12, F, n, 0.5).reshape(F, -1).astype(np.float32)
+    ref = (X[2].reshape(n, 3) + rng.normal(size=(n, 3)) * 0.3).astype(np.float32)
+    rc, _ = oracle.center_on_origin(ref.astype(np.float64))
+    w, ct, sd = _avgcorr(ca, ctx, torch.as_tensor(X).cuda(),
+                         ref_frames=torch.as_tensor(ref.reshape(1, -1)).cuda())
+    _, ct_o, sd_o = oracle.avgcorr(X, ref_sel_centered=rc)
+    assert np.abs(ct.cpu().numpy() - ct_o).max() <= TOL
+    assert np.abs(sd.cpu().numpy() - sd_o).max() <= TOL
+
+
+def test_avgcorr_prefix_matches_running_3000_frames(ctx):
+    # Every window of 3,000 frames x 200 atoms: the two methods agree.
+    import torch
+
+    import cpptraj_amd as ca
+    rng = np.random.default_rng(79)
+    F, n = 3000, 200
+    X = synth.noisy(13, F, n, 0.6).reshape(F, -1).astype(np.float32)
+    m = synth.masses(rng, n)
+    Xd, md = torch.as_tensor(X).cuda(), torch.as_tensor(m).cuda()
+    R = torch.as_tensor(X[7:8] + rng.normal(size=(1, 3 * n)).astype(np.float32) * 0.3).cuda()
+    out = {}
+    for method in ("running", "prefix"):
+        w, ct, sd = _avgcorr(ca, ctx, Xd, mass=md, ref_frames=R, use_mass=True, method=method)
+        out[method] = (ct.cpu().numpy(), sd.cpu().numpy())
+    assert len(w) == F - 1
+    assert np.abs(out["running"][0] - out["prefix"][0]).max() <= TOL
+    assert np.abs(out["running"][1] - out["prefix"][1]).max() <= TOL
 
 
 def test_avgcorr_errors(ctx):
@@ -87,4 +168,8 @@ def test_avgcorr_errors(ctx):
         ca.rms_avgcorr(Y, ctx=ctx)
     Z = torch.zeros((4, 3 * 7000), dtype=torch.float32, device="cuda")
     with pytest.raises(ca.RmsdError, match="selected atoms"):
-        ca.rms_avgcorr(Z, ref_frames=Z[:1], ctx=ctx)
+        _avgcorr(ca, ctx, Z, ref_frames=Z[:1], method="running")
+    with pytest.raises(ca.RmsdError, match="needs a fixed reference"):
+        _avgcorr(ca, ctx, Y, method="prefix")
+    with pytest.raises(ca.RmsdError, match="unknown rmsavgcorr method"):
+        ctx.check(ctx._L.rmsd_avgcorr_set_method(ctx.handle, 7))
