@@ -140,6 +140,11 @@ def _optional(L):
         L.rmsd_perres_nofit.argtypes = [P, FP, FP, C.c_int, C.c_int, IP, IP, IP, DP, DP,
                                         C.c_int, C.c_int, C.c_int, P, P, P]
         L.rmsd_perres_nofit.restype = C.c_int
+    if hasattr(L, "rmsd_symm"):
+        IP = C.POINTER(C.c_int32)
+        L.rmsd_symm.argtypes = [P, FP, FP, C.c_int, C.c_int, IP, IP, C.c_int, C.c_int, P, P, P,
+                                P]
+        L.rmsd_symm.restype = C.c_int
 
 
 def _ptr(t):
@@ -339,6 +344,52 @@ def rmsd_fit(frames, ref_frames=None, ref_index=0, mask=None, mass=None, ref_mas
     tv = torch.empty((F, 6), dtype=torch.float64, device=dev) if save_vectors else None
     ctx.fit_desc(d, r, ref_index, fit, use_mass, MODIFY[modify], rmsd, U, tv)
     return rmsd, U, tv
+
+
+def symm_groups(groups):
+    """List of index lists -> (group_off, group_atoms) int32 host arrays."""
+    import numpy as np
+
+    off = np.zeros(len(groups) + 1, np.int32)
+    for g, a in enumerate(groups):
+        off[g + 1] = off[g] + len(a)
+    flat = (np.concatenate([np.asarray(a, np.int32) for a in groups]) if len(groups)
+            else np.zeros(0, np.int32))
+    return off, np.ascontiguousarray(flat, dtype=np.int32)
+
+
+def rmsd_symm(frames, groups, ref_frames=None, ref_index=0, mask=None, mass=None,
+              ref_mask=None, ref_mass=None, fit=True, use_mass=False, want_map=False,
+              want_xform=False, soa=False, natom=None, ctx: Context | None = None):
+    """Symmetry-corrected RMSD ('symmrmsd', rmsd_symm) of every frame against
+    one reference frame.  groups: lists of selection indices (or a
+    (group_off, group_atoms) pair).  Returns (rmsd, amap or None, U or None,
+    tvec or None) as CUDA tensors; amap is (F, nsel) int32."""
+    import numpy as np
+    import torch
+
+    ctx = ctx or _ctx(frames.device.index or 0)
+    d = frames_desc(frames, mask, mass, soa=soa, natom=natom)
+    rf = frames if ref_frames is None else ref_frames
+    r = frames_desc(rf, ref_mask if ref_mask is not None else mask,
+                    ref_mass if ref_mass is not None else mass, soa=soa, natom=natom)
+    if isinstance(groups, tuple) and len(groups) == 2 and isinstance(groups[0], np.ndarray):
+        off = np.ascontiguousarray(groups[0], dtype=np.int32)
+        flat = np.ascontiguousarray(groups[1], dtype=np.int32)
+    else:
+        off, flat = symm_groups(groups)
+    F, dev = d.nframes, frames.device
+    rmsd = torch.empty(F, dtype=torch.float64, device=dev)
+    amap = torch.empty((F, d.nsel), dtype=torch.int32, device=dev) if want_map else None
+    U = torch.empty((F, 3, 3), dtype=torch.float64, device=dev) if want_xform else None
+    tv = torch.empty((F, 6), dtype=torch.float64, device=dev) if want_xform else None
+    IP = C.POINTER(C.c_int32)
+    ng = len(off) - 1
+    ctx.check(ctx._L.rmsd_symm(ctx.handle, C.byref(d), C.byref(r), ref_index, ng,
+                               off.ctypes.data_as(IP), (flat if len(flat) else
+                                                        np.zeros(1, np.int32)).ctypes.data_as(IP),
+                               int(fit), int(use_mass), _ptr(rmsd), _ptr(amap), _ptr(U), _ptr(tv)))
+    return rmsd, amap, U, tv
 
 
 def rmsd_perres(frames, res_off, tgt_atoms, ref_frames=None, ref_index=0, ref_atoms=None,

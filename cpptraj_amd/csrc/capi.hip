@@ -823,4 +823,147 @@ int rmsd_avgcorr(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* re
   return RMSD_OK;
 }
 
+int rmsd_symm(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* ref, int ref_frame,
+              int ngroups, const int32_t* group_off, const int32_t* group_atoms, int fit,
+              int use_mass, double* rmsd, int32_t* amap, double* U, double* tvec) {
+  if (!ctx) return RMSD_ERR_ARG;
+  int st;
+  if ((st = check_frames(ctx, frames, "frames")) != RMSD_OK) return st;
+  if ((st = check_frames(ctx, ref, "ref")) != RMSD_OK) return st;
+  if (ref->nsel != frames->nsel)
+    return fail(ctx, RMSD_ERR_ARG, "reference and target select different atom counts");
+  if (ref_frame < 0 || ref_frame >= ref->nframes)
+    return fail(ctx, RMSD_ERR_ARG, "reference frame out of range");
+  if (use_mass && (!frames->mass || !ref->mass))
+    return fail(ctx, RMSD_ERR_ARG, "use_mass needs target and reference masses");
+  if (ngroups < 0 || (ngroups > 0 && (!group_off || !group_atoms)))
+    return fail(ctx, RMSD_ERR_ARG, "symmetric groups: need group_off and group_atoms");
+  if (!rmsd && frames->nframes > 0) return fail(ctx, RMSD_ERR_ARG, "rmsd is NULL");
+  const int n = frames->nsel;
+  // Group table: offsets, atoms, each atom's entry, and the scratch offsets
+  // of groups too large for LDS.
+  std::vector<int32_t> pos((size_t)n, -1), bigoff((size_t)std::max(ngroups, 1), -1);
+  int64_t bigstride = 0;
+  if (ngroups > 0 && group_off[0] != 0)
+    return fail(ctx, RMSD_ERR_ARG, "symmetric groups: group_off[0] must be 0");
+  for (int g = 0; g < ngroups; ++g) {
+    const int b = group_off[g], e = group_off[g + 1];
+    if (e - b < 1 || e - b > SYMM_GROUP_MAX)
+      return fail(ctx, RMSD_ERR_ARG, "symmetric group " + std::to_string(g) + " has " +
+                                         std::to_string(e - b) + " atoms (1.." +
+                                         std::to_string(SYMM_GROUP_MAX) + " allowed)");
+    for (int k = b; k < e; ++k) {
+      const int a = group_atoms[k];
+      if (a < 0 || a >= n)
+        return fail(ctx, RMSD_ERR_ARG, "symmetric group atom out of range of the selection");
+      if (pos[a] >= 0)
+        return fail(ctx, RMSD_ERR_ARG, "selected atom " + std::to_string(a) +
+                                           " is in more than one symmetric group");
+      pos[a] = k;
+    }
+    if (e - b > SYMM_GROUP_LDS) {
+      bigoff[g] = (int32_t)bigstride;
+      bigstride += (int64_t)(e - b) * (e - b) + (6 * (e - b) + 7) / 8;
+    }
+  }
+  const int S = ngroups > 0 ? group_off[ngroups] : 0;
+  if (frames->nframes == 0) return RMSD_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const int64_t nf = frames->nframes;
+  const int ldr = (int)round_up(n, 4);
+  std::vector<int32_t> blob;
+  blob.reserve((size_t)(ngroups + 1) + S + n + ngroups);
+  for (int g = 0; g <= ngroups; ++g) blob.push_back(ngroups > 0 ? group_off[g] : 0);
+  for (int k = 0; k < S; ++k) blob.push_back(group_atoms[k]);
+  blob.insert(blob.end(), pos.begin(), pos.end());
+  blob.insert(blob.end(), bigoff.begin(), bigoff.begin() + ngroups);
+  int32_t* dblob = (int32_t*)scratch(ctx, "symm.table", sizeof(int32_t) * blob.size(), &st);
+  if (!dblob) return st;
+  HIP_TRY(ctx, hipMemcpyAsync(dblob, blob.data(), sizeof(int32_t) * blob.size(),
+                              hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // blob is a local buffer
+  double* refbuf = (double*)scratch(ctx, "symm.ref", sizeof(double) * (3 * ldr + 8), &st);
+  if (!refbuf) return st;
+  double* ref_trans = refbuf + 3 * ldr;
+  double* consts = (double*)scratch(ctx, "symm.consts", sizeof(double) * 8, &st);
+  if (!consts) return st;
+  double* wpad = nullptr;
+  if (use_mass) {
+    wpad = (double*)scratch(ctx, "symm.mass", sizeof(double) * ldr, &st);
+    if (!wpad) return st;
+    HIP_TRY(ctx, hipMemsetAsync(wpad, 0, sizeof(double) * ldr, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(wpad, frames->mass, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                                ctx->stream));
+  }
+  int* gmap = (int*)scratch(ctx, "symm.gmap", sizeof(int) * (size_t)(nf * std::max(S, 1)), &st);
+  if (!gmap) return st;
+  double* big = nullptr;
+  if (bigstride > 0) {
+    big = (double*)scratch(ctx, "symm.big", sizeof(double) * (size_t)(nf * bigstride), &st);
+    if (!big) return st;
+  }
+  const FrameSetDev F = to_dev(*frames);
+  ctx->marks = 0;
+  mark(ctx, 0);
+  HIP_TRY(ctx, rmsd_launch_ref_prep(to_dev(*ref), ref_frame, use_mass ? ref->mass : nullptr, wpad,
+                                    fit ? 1 : 0, refbuf, ref_trans, consts, ctx->stream));
+  double* U0 = nullptr;
+  double* t0 = nullptr;
+  if (fit) {
+    // Initial fit of the frames as given: U and Trans for the cost matrices.
+    double* moments = (double*)scratch(ctx, "symm.moments", sizeof(double) * 16 * nf, &st);
+    if (!moments) return st;
+    double* fit0 = (double*)scratch(ctx, "symm.fit0", sizeof(double) * 16 * nf, &st);
+    if (!fit0) return st;
+    U0 = fit0;
+    t0 = fit0 + 9 * nf;
+    FitArgs fa;
+    fa.fs = F;
+    fa.mass = wpad;
+    fa.ref = refbuf;
+    fa.consts = consts;
+    fa.moments = moments;
+    HIP_TRY(ctx, rmsd_launch_fit_moments(fa, ctx->stream));
+    SolveArgs sa;
+    sa.moments = moments;
+    sa.consts = consts;
+    sa.ref_trans = ref_trans;
+    sa.nframes = (int)nf;
+    sa.fit = 1;
+    sa.clamp = 1;
+    sa.rmsd = fit0 + 15 * nf;
+    sa.U = U0;
+    sa.tvec = t0;
+    sa.xform = nullptr;
+    HIP_TRY(ctx, rmsd_launch_fit_solve(sa, ctx->stream));
+  }
+  mark(ctx, 1);
+  SymmArgs a;
+  a.fs = F;
+  a.mass = use_mass ? frames->mass : nullptr;
+  a.ref = refbuf;
+  a.ldr = ldr;
+  a.ref_trans = ref_trans;
+  a.fit = fit ? 1 : 0;
+  a.U0 = U0;
+  a.t0 = t0;
+  a.ngroups = ngroups;
+  a.goff = dblob;
+  a.gatoms = dblob + (ngroups + 1);
+  a.ngatoms = S;
+  a.pos = dblob + (ngroups + 1) + S;
+  a.bigoff = dblob + (ngroups + 1) + S + n;
+  a.bigstride = bigstride;
+  a.big = big;
+  a.gmap = gmap;
+  a.rmsd = rmsd;
+  a.amap = amap;
+  a.U = fit ? U : nullptr;
+  a.tvec = fit ? tvec : nullptr;
+  HIP_TRY(ctx, rmsd_launch_symm(a, ctx->stream));
+  mark(ctx, 2);
+  mark(ctx, 3);
+  return RMSD_OK;
+}
+
 }  // extern "C"

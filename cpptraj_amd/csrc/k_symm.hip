@@ -1,0 +1,336 @@
+// k_symm.hip -- symmetry-corrected RMSD ('symmrmsd').
+//
+//   k_symm_assign  SymmetricRmsdCalc::SymmRMSD_CenteredRef
+//                  (src/SymmetricRmsdCalc.cpp:125-176): per (frame, group of
+//                  symmetric atoms) the squared distances between the
+//                  reference atoms and the target atoms as the initial fit
+//                  leaves them (translated by Trans, rotated by U), and the
+//                  reference's Hungarian heuristic on them
+//                  (Hungarian::Optimize, src/Hungarian.cpp:21-337); the
+//                  assignment goes to the frame's map.
+//   k_symm_final   the rest of SymmRMSD_CenteredRef (:178-192): the target
+//                  remapped (coordinates and masses, Frame::
+//                  SetCoordinatesByMap) and fitted again (FP64 moments, QCP,
+//                  rotation) or compared without fit.
+//
+// The initial fit is the frame-to-reference fit (k_fit.hip) run first.  One
+// lane per (frame, group) for the assignment: groups are small (2-3 atoms in
+// proteins), the heuristic is sequential and branchy, and lanes of one wave
+// take consecutive groups of one frame (nearby atoms).  Its working matrix
+// stays in LDS for groups of up to SYMM_GROUP_LDS atoms, in global scratch
+// for larger ones.  The final pass is one wavefront per frame, lanes over
+// atoms, one wave reduction of 18 sums.
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <stdint.h>
+#include "rmsd_device.hpp"
+#include "rmsd_kernels.hpp"
+
+namespace {
+
+constexpr int SYM_THREADS = 64;
+constexpr int SYM_WS_INTS = 6;  // per-row/column byte arrays of the heuristic
+
+__device__ __forceinline__ double scoord(const FrameSetDev& fs, int64_t f, int k, int c) {
+  const int a = fs.sel ? fs.sel[k] : k;
+  const float* fr = fs.xyz + frame_of(fs, f) * fs.stride;
+  return (double)(fs.ld ? fr[c * fs.ld + a] : fr[3 * a + c]);
+}
+
+// Hungarian::Optimize: row and column reduction, then at most n * n rounds of
+// greedy zero assignment, greedy line cover and update by the smallest
+// uncovered element.  M is n x n row-major (row = target atom, column =
+// reference atom); map[col] receives the assigned row or -1.  ws holds
+// 6 n bytes.  Lines are bit masks (n <= 32).
+__device__ void hungarian(double* M, int n, signed char* map, signed char* ws) {
+#pragma clang fp contract(off)
+  signed char* a_cr = ws;           // row -> column
+  signed char* rz = ws + n;         // zeros per row / column, non-zeros per row / column
+  signed char* cz = ws + 2 * n;
+  signed char* rnz = ws + 3 * n;
+  signed char* cnz = ws + 4 * n;
+  for (int r = 0; r < n; ++r) {
+    double mn = DBL_MAX;
+    for (int c = 0; c < n; ++c) mn = M[r * n + c] < mn ? M[r * n + c] : mn;
+    for (int c = 0; c < n; ++c) M[r * n + c] -= mn;
+  }
+  for (int c = 0; c < n; ++c) {
+    double mn = DBL_MAX;
+    for (int r = 0; r < n; ++r) mn = M[r * n + c] < mn ? M[r * n + c] : mn;
+    for (int r = 0; r < n; ++r) M[r * n + c] -= mn;
+  }
+  for (int it = 0; it < n * n; ++it) {
+    for (int k = 0; k < n; ++k) {
+      map[k] = -1;
+      a_cr[k] = -1;
+    }
+    int nassigned = 0;
+    for (;;) {
+      int min_row = -1, min_row_z = n + 1;
+      for (int r = 0; r < n; ++r) {
+        if (a_cr[r] != -1) continue;
+        int z = 0;
+        for (int c = 0; c < n; ++c) z += (map[c] == -1 && M[r * n + c] < RMSD_SMALL) ? 1 : 0;
+        if (z != 0 && z < min_row_z) {
+          min_row_z = z;
+          min_row = r;
+        }
+      }
+      int min_col = -1, min_col_z = n + 1;
+      for (int c = 0; c < n; ++c) {
+        if (map[c] != -1) continue;
+        int z = 0;
+        for (int r = 0; r < n; ++r) z += (a_cr[r] == -1 && M[r * n + c] < RMSD_SMALL) ? 1 : 0;
+        if (z != 0 && z < min_col_z) {
+          min_col_z = z;
+          min_col = c;
+        }
+      }
+      if (min_row == -1 && min_col == -1) break;
+      bool done = false;
+      if (min_col_z < min_row_z) {
+        for (int r = 0; r < n && !done; ++r)
+          if (M[r * n + min_col] < RMSD_SMALL && a_cr[r] == -1) {
+            map[min_col] = (signed char)r;
+            a_cr[r] = (signed char)min_col;
+            done = true;
+          }
+      } else {
+        for (int c = 0; c < n && !done; ++c)
+          if (M[min_row * n + c] < RMSD_SMALL && map[c] == -1) {
+            map[c] = (signed char)min_row;
+            a_cr[min_row] = (signed char)c;
+            done = true;
+          }
+      }
+      if (!done) break;
+      ++nassigned;
+    }
+    if (nassigned == n) break;
+    // Cover the zeros: first by most zeros, then by zeros minus non-zeros;
+    // rows win ties.
+    uint32_t lr = 0, lc = 0;
+    int total = 0;
+    for (int k = 0; k < n; ++k) rz[k] = cz[k] = rnz[k] = cnz[k] = 0;
+    for (int r = 0; r < n; ++r)
+      for (int c = 0; c < n; ++c) {
+        if (M[r * n + c] < RMSD_SMALL) {
+          ++rz[r];
+          ++cz[c];
+          ++total;
+        } else {
+          ++rnz[r];
+          ++cnz[c];
+        }
+      }
+    for (int step = 0; total > 0 && step < 2 * n; ++step) {
+      int max_diff = -1, max_zero = -1, max_idx = -1;
+      bool is_row = true;
+      for (int r = 0; r < n; ++r) {
+        if (lr >> r & 1u) continue;
+        const int d = step > 0 ? rz[r] - rnz[r] : rz[r];
+        if (max_zero == -1 || d > max_diff) {
+          max_diff = d;
+          max_zero = rz[r];
+          max_idx = r;
+        }
+      }
+      for (int c = 0; c < n; ++c) {
+        if (lc >> c & 1u) continue;
+        const int d = step > 0 ? cz[c] - cnz[c] : cz[c];
+        if (d > max_diff) {
+          max_diff = d;
+          max_zero = cz[c];
+          max_idx = c;
+          is_row = false;
+        }
+      }
+      if (max_idx < 0) break;
+      if (is_row) {
+        lr |= 1u << max_idx;
+        for (int c = 0; c < n; ++c) {
+          if (M[max_idx * n + c] < RMSD_SMALL)
+            --cz[c];
+          else
+            --cnz[c];
+        }
+      } else {
+        lc |= 1u << max_idx;
+        for (int r = 0; r < n; ++r) {
+          if (M[r * n + max_idx] < RMSD_SMALL)
+            --rz[r];
+          else
+            --rnz[r];
+        }
+      }
+      total -= max_zero;
+    }
+    double mu = DBL_MAX;
+    for (int r = 0; r < n; ++r) {
+      if (lr >> r & 1u) continue;
+      for (int c = 0; c < n; ++c)
+        if (!(lc >> c & 1u) && M[r * n + c] < mu) mu = M[r * n + c];
+    }
+    for (int r = 0; r < n; ++r)
+      for (int c = 0; c < n; ++c) {
+        const bool br = lr >> r & 1u, bc = lc >> c & 1u;
+        if (br && bc)
+          M[r * n + c] += mu;
+        else if (!br && !bc)
+          M[r * n + c] -= mu;
+      }
+  }
+}
+
+__global__ __launch_bounds__(SYM_THREADS) void k_symm_assign(SymmArgs a) {
+#pragma clang fp contract(off)
+  __shared__ double lds_m[SYM_THREADS * SYMM_GROUP_LDS * SYMM_GROUP_LDS];
+  __shared__ signed char lds_i[SYM_THREADS * (SYM_WS_INTS * SYMM_GROUP_LDS)];
+  const int64_t pair = (int64_t)blockIdx.x * SYM_THREADS + threadIdx.x;
+  if (pair >= (int64_t)a.fs.nframes * a.ngroups) return;
+  const int64_t f = pair / a.ngroups;
+  const int g = (int)(pair - f * a.ngroups);
+  const int g0 = a.goff[g];
+  const int n = a.goff[g + 1] - g0;
+  const int* ga = a.gatoms + g0;
+  int* out = a.gmap + f * a.ngatoms + g0;
+  if (n == 1) {
+    out[0] = ga[0];
+    return;
+  }
+  double* M;
+  signed char* ws;
+  if (n <= SYMM_GROUP_LDS) {
+    M = lds_m + threadIdx.x * (SYMM_GROUP_LDS * SYMM_GROUP_LDS);
+    ws = lds_i + threadIdx.x * (SYM_WS_INTS * SYMM_GROUP_LDS);
+  } else {
+    M = a.big + f * a.bigstride + a.bigoff[g];
+    ws = reinterpret_cast<signed char*>(M + n * n);
+  }
+  double U[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, t[3] = {0, 0, 0};
+  if (a.U0) {
+    for (int k = 0; k < 9; ++k) U[k] = a.U0[9 * f + k];
+    for (int k = 0; k < 3; ++k) t[k] = a.t0[6 * f + k];
+  }
+  for (int ta = 0; ta < n; ++ta) {
+    double x = scoord(a.fs, f, ga[ta], 0), y = scoord(a.fs, f, ga[ta], 1),
+           z = scoord(a.fs, f, ga[ta], 2);
+    if (a.U0) {
+      // Translated to the origin, then Frame::Rotate.
+      x += t[0];
+      y += t[1];
+      z += t[2];
+      const double rx = (x * U[0]) + (y * U[1]) + (z * U[2]);
+      const double ry = (x * U[3]) + (y * U[4]) + (z * U[5]);
+      const double rz = (x * U[6]) + (y * U[7]) + (z * U[8]);
+      x = rx;
+      y = ry;
+      z = rz;
+    }
+    for (int ra = 0; ra < n; ++ra) {
+      const int k = ga[ra];
+      const double dx = a.ref[k] - x, dy = a.ref[a.ldr + k] - y, dz = a.ref[2 * a.ldr + k] - z;
+      M[ta * n + ra] = (dx * dx + dy * dy + dz * dz);
+    }
+  }
+  signed char* map = ws + 5 * n;
+  hungarian(M, n, map, ws);
+  for (int c = 0; c < n; ++c) out[c] = ga[map[c] >= 0 ? map[c] : c];
+}
+
+__global__ __launch_bounds__(SYM_THREADS) void k_symm_final(SymmArgs a) {
+#pragma clang fp contract(off)
+  const int64_t f = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int n = a.fs.nsel;
+  const int* gm = a.gmap + f * a.ngatoms;
+  auto target = [&](int i) {
+    const int p = a.pos[i];
+    return p >= 0 ? gm[p] : i;
+  };
+  double v[18];
+#pragma unroll
+  for (int q = 0; q < 18; ++q) v[q] = 0.0;
+  const int t0 = target(0);
+  const double s0 = a.fit ? scoord(a.fs, f, t0, 0) : 0.0;
+  const double s1 = a.fit ? scoord(a.fs, f, t0, 1) : 0.0;
+  const double s2 = a.fit ? scoord(a.fs, f, t0, 2) : 0.0;
+  for (int i = lane; i < n; i += SYM_THREADS) {
+    const int t = target(i);
+    if (a.amap) a.amap[f * n + i] = t;
+    const double w = a.mass ? a.mass[t] : 1.0;
+    const double y0 = a.ref[i], y1 = a.ref[a.ldr + i], y2 = a.ref[2 * a.ldr + i];
+    const double x0 = scoord(a.fs, f, t, 0), x1 = scoord(a.fs, f, t, 1),
+                 x2 = scoord(a.fs, f, t, 2);
+    v[17] += w;
+    if (a.fit) {
+      const double d0 = x0 - s0, d1 = x1 - s1, d2 = x2 - s2;
+      const double w0 = w * d0, w1 = w * d1, w2 = w * d2;
+      v[0] += w0;
+      v[1] += w1;
+      v[2] += w2;
+      v[3] += w0 * y0; v[4] += w0 * y1; v[5] += w0 * y2;
+      v[6] += w1 * y0; v[7] += w1 * y1; v[8] += w1 * y2;
+      v[9] += w2 * y0; v[10] += w2 * y1; v[11] += w2 * y2;
+      v[12] += w0 * d0 + w1 * d1 + w2 * d2;
+      v[13] += w * y0;
+      v[14] += w * y1;
+      v[15] += w * y2;
+      v[16] += w * (y0 * y0 + y1 * y1 + y2 * y2);
+    } else {
+      const double dx = y0 - x0, dy = y1 - x1, dz = y2 - x2;
+      v[12] += (w * (dx * dx + dy * dy + dz * dz));
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 18; ++q) v[q] = rmsd_dev::wave_sum(v[q]);
+  if (lane != 0) return;
+  const double W = v[17];
+  double U[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  double t1[3] = {0, 0, 0};
+  double r;
+  if (W < RMSD_SMALL) {
+    r = -1.0;
+  } else if (!a.fit) {
+    r = v[12] < 0.0 ? 0.0 : sqrt(v[12] / W);
+  } else {
+    const double c0 = v[0] / W, c1 = v[1] / W, c2 = v[2] / W;
+    const double K0 = v[13], K1 = v[14], K2 = v[15];
+    double R[9];
+    R[0] = v[3] - c0 * K0; R[1] = v[4] - c0 * K1; R[2] = v[5] - c0 * K2;
+    R[3] = v[6] - c1 * K0; R[4] = v[7] - c1 * K1; R[5] = v[8] - c1 * K2;
+    R[6] = v[9] - c2 * K0; R[7] = v[10] - c2 * K1; R[8] = v[11] - c2 * K2;
+    const double sx = v[12] - W * (c0 * c0 + c1 * c1 + c2 * c2);
+    const double E0 = 0.5 * (sx + v[16]);
+    const rmsd_dev::QcpCoef co = rmsd_dev::qcp_coefficients(R);
+    const double lam = rmsd_dev::qcp_lambda(co, E0);
+    r = rmsd_dev::rmsd_from_lambda(E0, lam, W, 1);
+    rmsd_dev::qcp_rotation(R, lam, U);
+    t1[0] = -(c0 + s0);
+    t1[1] = -(c1 + s1);
+    t1[2] = -(c2 + s2);
+  }
+  a.rmsd[f] = r;
+  if (a.U)
+    for (int k = 0; k < 9; ++k) a.U[9 * f + k] = U[k];
+  if (a.tvec)
+    for (int k = 0; k < 3; ++k) {
+      a.tvec[6 * f + k] = t1[k];
+      a.tvec[6 * f + 3 + k] = a.ref_trans[k];
+    }
+}
+
+}  // namespace
+
+hipError_t rmsd_launch_symm(const SymmArgs& a, hipStream_t s) {
+  const int64_t F = a.fs.nframes;
+  if (F == 0) return hipSuccess;
+  if (a.ngroups > 0) {
+    const int64_t pairs = F * a.ngroups;
+    const int64_t blocks = (pairs + SYM_THREADS - 1) / SYM_THREADS;
+    if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_symm_assign, dim3((unsigned)blocks), dim3(SYM_THREADS), 0, s, a);
+  }
+  hipLaunchKernelGGL(k_symm_final, dim3((unsigned)F), dim3(SYM_THREADS), 0, s, a);
+  return hipGetLastError();
+}

@@ -251,3 +251,32 @@ hipError_t rmsd_launch_pair_grad(const double* XA, const double* XB, const doubl
                                  const double* gB, const double* w, int n, int FA, int FB,
                                  const double* M, const double* G, double* cw, double* gradA,
                                  double* gradB, hipStream_t s);
+
+// Symmetry-corrected RMSD (k_symm.hip).  Group g owns entries goff[g] ..
+// goff[g+1] of gatoms (selection indices); pos[i] is atom i's entry or -1.
+struct SymmArgs {
+  FrameSetDev fs;
+  const double* mass;     // selected target masses (null: unit)
+  const double* ref;      // reference SoA [3][ldr]: centred (fit) or as stored
+  int ldr;
+  const double* ref_trans;
+  int fit;
+  const double* U0;       // initial fit: U (9 per frame) and Trans (tvec layout,
+  const double* t0;       // 6 per frame); null without fit
+  int ngroups;
+  const int* goff;
+  const int* gatoms;
+  int ngatoms;            // entries of gatoms
+  const int* pos;         // nsel
+  const int* bigoff;      // per group: offset (doubles) of its working matrix
+  int64_t bigstride;      // doubles per frame of big-group scratch
+  double* big;            // groups over SYMM_GROUP_LDS atoms
+  int* gmap;              // F x ngatoms: target atom for each group entry
+  double* rmsd;
+  int32_t* amap;          // F x nsel or null
+  double* U;              // 9 per frame or null
+  double* tvec;           // 6 per frame or null
+};
+constexpr int SYMM_GROUP_LDS = 8;    // largest group kept in LDS
+constexpr int SYMM_GROUP_MAX = 32;   // largest group (bit-mask lines, byte indices)
+hipError_t rmsd_launch_symm(const SymmArgs& a, hipStream_t s);

@@ -130,3 +130,42 @@ def noisy_soa_torch(seed: int, F: int, n: int, sigma: float, ld: int, device="cu
         X = ref_t.unsqueeze(0) + sigma * torch.randn((k, 3, n), generator=g, device=device)
         out[s:e, :, :n] = torch.bmm(R, X) + t
     return out, ref
+
+
+def symm_case(seed: int, F: int, n: int, sizes=(2, 3, 2, 4, 3, 2), ngroups: int = 40,
+              sigma: float = 0.3, rigid: bool = True, compact: bool = True):
+    """Frames for the symmetry-corrected RMSD: a reference chain of n atoms
+    with `ngroups` disjoint groups (sizes cycled from `sizes`, atoms picked
+    over the whole chain; with `compact` each group's atoms are then moved to
+    1.5 A from its first atom, as symmetric atoms sit in a molecule), masses
+    equal within each group, and F frames that are
+    the reference plus N(0, sigma^2) noise with the atoms of every group
+    permuted at random (a different permutation per frame), then (rigid)
+    rotated and translated.  Returns (frames (F, 3n) float32, ref (n, 3),
+    groups (lists of atom indices), masses (n), perms (F, n): frame atom k
+    holds the reference's atom perms[f, k])."""
+    rng = np.random.default_rng(seed)
+    ref = chain(rng, n)
+    m = masses(rng, n)
+    order = rng.permutation(n)
+    groups, p = [], 0
+    for g in range(ngroups):
+        k = int(sizes[g % len(sizes)])
+        if p + k > n:
+            break
+        grp = sorted(int(a) for a in order[p:p + k])
+        groups.append(grp)
+        m[grp] = m[grp[0]]
+        if compact:
+            for a in grp[1:]:
+                u = rng.normal(size=3)
+                ref[a] = ref[grp[0]] + 1.5 * u / np.linalg.norm(u)
+        p += k
+    X = np.empty((F, n, 3))
+    perms = np.tile(np.arange(n), (F, 1))
+    for f in range(F):
+        for grp in groups:
+            perms[f, grp] = np.asarray(grp)[rng.permutation(len(grp))]
+        Y = ref[perms[f]] + rng.normal(size=(n, 3)) * sigma
+        X[f] = _rigid(rng, Y[None])[0] if rigid else Y
+    return X.reshape(F, -1).astype(np.float32), ref, groups, m, perms

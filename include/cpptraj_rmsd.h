@@ -243,6 +243,35 @@ int rmsd_perres_nofit(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frame
                       int use_mass, int center, int modify, const double* U, const double* tvec,
                       double* out);
 
+/* Symmetry-corrected RMSD ('symmrmsd').
+ * Replaces SymmetricRmsdCalc::SymmRMSD_CenteredRef (src/SymmetricRmsdCalc.cpp:
+ * 125-193) in Action_SymmetricRmsd::DoAction (src/Action_SymmetricRmsd.cpp:
+ * 88-112), for every frame: with fit, the frame's best fit to the reference
+ * (translated, rotated); per group of symmetric atoms, the squared distances
+ * between its reference and target atoms and the reference's Hungarian
+ * heuristic (Hungarian::Optimize, src/Hungarian.cpp:21-337) to pair them; the
+ * target remapped (coordinates and masses) and fitted again, or compared
+ * without fit (Frame::RMSD_NoFit).
+ *   ref, ref_frame   the reference set and frame (selected atoms): centred with
+ *            its masses for a fit, as stored without (ReferenceAction)
+ *   ngroups, group_off, group_atoms   HOST arrays: group g is group_atoms
+ *            [group_off[g] .. group_off[g+1]), selection indices, 1..32 atoms,
+ *            disjoint -- SymmetricRmsdCalc::SetupSymmRMSD's groups (derived by
+ *            the caller from the topology, src/SymmetricRmsdCalc.cpp:28-119)
+ *   fit, use_mass    'nofit', 'mass'
+ *   rmsd     device, one double per frame (-1 for zero total mass)
+ *   amap     device, nframes * nsel int32 or NULL: amap[f*nsel + i] = the
+ *            selected target atom that reference atom i is paired with
+ *            (SymmetricRmsdCalc::AMap)
+ *   U, tvec  device or NULL (fit only): the final fit's rotation and
+ *            translations in rmsd_fit_to_ref's layout, for rmsd_transform of
+ *            the remapped frame (the action's Trans_Rot_Trans)
+ * A column the heuristic leaves unassigned keeps its own atom (the reference
+ * would index the group at -1). */
+int rmsd_symm(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* ref, int ref_frame,
+              int ngroups, const int32_t* group_off, const int32_t* group_atoms, int fit,
+              int use_mass, double* rmsd, int32_t* amap, double* U, double* tvec);
+
 /* Gradient of the fit: no counterpart in the reference (cpptraj has no RMSD
  * derivative).  For loss = sum_f grad_rmsd[f] * RMSD_f of rmsd_fit_to_ref
  * (fit, same frames / reference / masses), writes d loss / d x for the selected

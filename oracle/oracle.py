@@ -53,6 +53,43 @@ def _ci(a):
     return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
 
 
+
+def _groups(groups):
+    """List of index lists -> (offsets, flat atoms) int32."""
+    off = np.zeros(len(groups) + 1, np.int32)
+    for g, a in enumerate(groups):
+        off[g + 1] = off[g] + len(a)
+    flat = np.concatenate([np.asarray(a, np.int32) for a in groups]) if groups else \
+        np.zeros(0, np.int32)
+    return off, np.ascontiguousarray(flat, np.int32)
+
+
+def _hungarian(fn, M):
+    M = np.array(M, dtype=np.float64, order="C")
+    n = M.shape[0]
+    m = np.full(max(n, 1), -1, np.int32)
+    fn(_d(M), n, _i(m))
+    return m[:n]
+
+
+def _symm(fn, frames, sel, mass, ref_sel, use_mass, fit, groups):
+    fr = _c32(frames)
+    s = _ci(sel)
+    nsel = len(s) if s is not None else fr.shape[1] // 3
+    F = fr.shape[0]
+    m = _c64(mass) if mass is not None else None
+    rs = _c64(ref_sel)
+    off, flat = _groups(groups)
+    r = np.zeros(F)
+    amap = np.zeros((F, nsel), np.int32)
+    U = np.zeros((F, 9))
+    tr = np.zeros((F, 3))
+    err = fn(_f(fr), fr.shape[1], F, _i(s), nsel, _d(m), _d(rs), int(use_mass), int(fit),
+             len(groups), _i(off), _i(flat), _d(r), _i(amap), _d(U), _d(tr))
+    if err:
+        raise MemoryError("symmetric RMSD failed")
+    return r, amap, U, tr
+
 class Oracle:
     """The C restatement (rmsd_oracle.c)."""
 
@@ -102,6 +139,11 @@ class Oracle:
         L.orc_avgcorr.argtypes = [_fp, C.c_int64, C.c_int, _ip, C.c_int, _dp, _dp, C.c_int,
                                   C.c_int, C.c_int, _dp, _dp]
         L.orc_avgcorr.restype = C.c_int
+        L.orc_hungarian.argtypes = [_dp, C.c_int, _ip]
+        L.orc_hungarian.restype = C.c_int
+        L.orc_symm_frames.argtypes = [_fp, C.c_int64, C.c_int, _ip, C.c_int, _dp, _dp, C.c_int,
+                                      C.c_int, C.c_int, _ip, _ip, _dp, _ip, _dp, _dp]
+        L.orc_symm_frames.restype = C.c_int
 
     # -- per-frame routines -------------------------------------------------
     def center_on_origin(self, xyz, mass=None, use_mass=False):
@@ -286,6 +328,16 @@ class Oracle:
             raise MemoryError("orc_avgcorr failed")
         return w, ct, sd
 
+    def hungarian(self, M):
+        """Hungarian::Optimize restated: map[col] = row."""
+        return _hungarian(self.L.orc_hungarian, M)
+
+    def symm(self, frames, sel=None, mass=None, ref_sel=None, use_mass=False, fit=True,
+             groups=()):
+        """Symmetry-corrected RMSD per frame: (rmsd, amap, U, trans)."""
+        return _symm(self.L.orc_symm_frames, frames, sel, mass, ref_sel, use_mass, fit, groups)
+
+
 
 class RefLib:
     """The reference's own sources compiled in place (oracle/_ref)."""
@@ -325,6 +377,11 @@ class RefLib:
         L.cref_avgcorr.argtypes = [_fp, C.c_int64, C.c_int, _ip, C.c_int, _dp, _dp, C.c_int,
                                    _ip, C.c_int, _dp, _dp]
         L.cref_avgcorr.restype = C.c_int
+        L.cref_hungarian.argtypes = [_dp, C.c_int, _ip]
+        L.cref_hungarian.restype = C.c_int
+        L.cref_symm_frames.argtypes = [_fp, C.c_int64, C.c_int, _ip, C.c_int, _dp, _dp, C.c_int,
+                                       C.c_int, C.c_int, _ip, _ip, _dp, _ip, _dp, _dp]
+        L.cref_symm_frames.restype = C.c_int
 
     def rmsd_centered_ref(self, tgt, ref, mass=None, ref_mass=None, use_mass=False):
         t = _c64(tgt).copy()
@@ -467,6 +524,16 @@ class RefLib:
                             int(use_mass), _i(w), len(w), _d(ct), _d(sd))
         return ct, sd
 
+    def hungarian(self, M):
+        """The reference's Hungarian::Optimize: map[col] = row."""
+        return _hungarian(self.L.cref_hungarian, M)
+
+    def symm(self, frames, sel=None, mass=None, ref_sel=None, use_mass=False, fit=True,
+             groups=()):
+        """SymmRMSD_CenteredRef's steps on the reference's classes."""
+        return _symm(self.L.cref_symm_frames, frames, sel, mass, ref_sel, use_mass, fit, groups)
+
 
 def ref_available() -> bool:
     return os.path.exists(REF_SO)
+

@@ -20,6 +20,8 @@
 #include "Vec3.h"
 #include "QuaternionRMSD.h"
 #include "qcprot.h"
+#include "Hungarian.h"
+#include "DistRoutines.h"
 
 namespace {
 
@@ -350,6 +352,73 @@ int cref_avgcorr(const float* frames, int64_t stride, int nframes, const int* se
     sq -= avg * avg;
     ct[wi] = avg;
     sd[wi] = sq > 0.0 ? sqrt(sq) : 0.0;
+  }
+  return 0;
+}
+
+
+// Hungarian::Initialize / AddElement / Optimize (src/Hungarian.cpp:9-66) on an
+// n x n matrix given row by row; map[col] = row, as Optimize returns it.
+int cref_hungarian(const double* M, int n, int* map) {
+  Hungarian h;
+  if (h.Initialize((size_t)n)) return -1;
+  for (int k = 0; k < n * n; ++k) h.AddElement(M[k]);
+  std::vector<int> r = h.Optimize();
+  for (int c = 0; c < n; ++c) map[c] = r[c];
+  return 0;
+}
+
+// The steps of SymmetricRmsdCalc::SymmRMSD_CenteredRef (src/SymmetricRmsdCalc.
+// cpp:125-193) on the reference's Frame, Hungarian and DIST2_NoImage, for each
+// frame: SetCoordinates, RMSD_CenteredRef + Rotate (fit), cost matrices per
+// group, Optimize, SetCoordinatesByMap, RMSD_CenteredRef or RMSD_NoFit.  The
+// groups are given (SymmetricRmsdCalc::SetupSymmRMSD derives them from a
+// topology).  amap (F x nsel), U (F x 9), trans (F x 3) may be NULL.
+int cref_symm_frames(const float* frames, int64_t stride, int nframes, const int* sel, int nsel,
+                     const double* mass, const double* ref_sel, int use_mass, int fit,
+                     int ngroups, const int* goff, const int* gatoms, double* rmsd, int* amap,
+                     double* U, double* trans) {
+  std::vector<double> zero(3 * (size_t)nsel, 0.0);
+  Frame Sel = make_frame(zero.data(), mass, nsel);
+  Frame Remap = make_frame(zero.data(), mass, nsel);
+  Frame Ref = make_frame(ref_sel, 0, nsel);
+  std::vector<int> AMap((size_t)nsel);
+  Matrix_3x3 rot;
+  Vec3 tt;
+  Hungarian cost;
+  for (int f = 0; f < nframes; ++f) {
+    const float* fr = frames + (int64_t)f * stride;
+    double* x = Sel.xAddress();
+    for (int j = 0; j < nsel; ++j) {
+      const int a = sel ? sel[j] : j;
+      for (int c = 0; c < 3; ++c) x[3 * j + c] = (double)fr[3 * a + c];
+    }
+    for (int i = 0; i < nsel; ++i) AMap[i] = i;
+    Remap.SetCoordinates(Sel);
+    if (fit) {
+      Remap.RMSD_CenteredRef(Ref, rot, tt, use_mass != 0);
+      Remap.Rotate(rot);
+    }
+    for (int g = 0; g < ngroups; ++g) {
+      const int* ga = gatoms + goff[g];
+      const int m = goff[g + 1] - goff[g];
+      cost.Initialize((size_t)m);
+      for (int ta = 0; ta < m; ++ta)
+        for (int ra = 0; ra < m; ++ra)
+          cost.AddElement(DIST2_NoImage(Ref.XYZ(ga[ra]), Remap.XYZ(ga[ta])));
+      std::vector<int> rm = cost.Optimize();
+      for (int c = 0; c < m; ++c) AMap[ga[c]] = rm[c] >= 0 ? ga[rm[c]] : ga[c];
+    }
+    Remap.SetCoordinatesByMap(Sel, AMap);
+    double r;
+    if (fit)
+      r = Remap.RMSD_CenteredRef(Ref, rot, tt, use_mass != 0);
+    else
+      r = Remap.RMSD_NoFit(Ref, use_mass != 0);
+    rmsd[f] = r;
+    if (amap) std::memcpy(amap + (int64_t)f * nsel, AMap.data(), sizeof(int) * (size_t)nsel);
+    if (U) for (int k = 0; k < 9; ++k) U[9 * (int64_t)f + k] = fit ? rot[k] : (k % 4 == 0);
+    if (trans) for (int k = 0; k < 3; ++k) trans[3 * (int64_t)f + k] = fit ? tt[k] : 0.0;
   }
   return 0;
 }

@@ -923,3 +923,227 @@ int orc_avgcorr(const float *frames, int64_t stride, int nframes, const int *sel
   free(ws); free(sum); free(tgt); free(ref); free(x);
   return nw;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Hungarian::Optimize (src/Hungarian.cpp:21-66) with AssignRowsToColumns
+ * (:75-159), CoverZeroElements (:240-337) and UpdateMatrix (:165-190): the
+ * reference's own heuristic form of the assignment, step for step (row then
+ * column reduction; greedy zero assignment from the row or column with the
+ * fewest free zeros, rows preferred on ties; greedy line cover by the most
+ * zeros, later by zeros minus non-zeros, rows preferred on ties; add the
+ * smallest uncovered value where lines cross and subtract it where none runs),
+ * at most n*n rounds.  M (n x n, row-major, row = target atom, column =
+ * reference atom) is modified.  map[col] receives the row assigned to column
+ * col (-1 where none).  Returns the number of assignments.                   */
+int orc_hungarian(double *M, int n, int *map)
+{
+  int *a_cr = (int *)malloc(sizeof(int) * (size_t)n);          /* row -> col */
+  int *cnt = (int *)malloc(sizeof(int) * 4 * (size_t)n);
+  unsigned char *lr = (unsigned char *)malloc((size_t)2 * n);
+  if (!a_cr || !cnt || !lr) { free(a_cr); free(cnt); free(lr); return -1; }
+  unsigned char *lc = lr + n;
+  int *rz = cnt, *cz = cnt + n, *rnz = cnt + 2 * n, *cnz = cnt + 3 * n;
+  for (int r = 0; r < n; ++r) {
+    double mn = 1.7976931348623157e308;
+    for (int c = 0; c < n; ++c) if (M[r * n + c] < mn) mn = M[r * n + c];
+    for (int c = 0; c < n; ++c) M[r * n + c] -= mn;
+  }
+  for (int c = 0; c < n; ++c) {
+    double mn = 1.7976931348623157e308;
+    for (int r = 0; r < n; ++r) if (M[r * n + c] < mn) mn = M[r * n + c];
+    for (int r = 0; r < n; ++r) M[r * n + c] -= mn;
+  }
+  int nassigned = 0;
+  for (int it = 0; it < n * n; ++it) {
+    /* greedy assignment of zeros */
+    for (int k = 0; k < n; ++k) { map[k] = -1; a_cr[k] = -1; }
+    nassigned = 0;
+    for (;;) {
+      int min_row = -1, min_row_z = n + 1;
+      for (int r = 0; r < n; ++r) {
+        if (a_cr[r] != -1) continue;
+        int z = 0;
+        for (int c = 0; c < n; ++c) if (map[c] == -1 && M[r * n + c] < ORC_SMALL) ++z;
+        if (z != 0 && z < min_row_z) { min_row_z = z; min_row = r; }
+      }
+      int min_col = -1, min_col_z = n + 1;
+      for (int c = 0; c < n; ++c) {
+        if (map[c] != -1) continue;
+        int z = 0;
+        for (int r = 0; r < n; ++r) if (a_cr[r] == -1 && M[r * n + c] < ORC_SMALL) ++z;
+        if (z != 0 && z < min_col_z) { min_col_z = z; min_col = c; }
+      }
+      if (min_row == -1 && min_col == -1) break;
+      int done = 0;
+      if (min_col_z < min_row_z) {
+        for (int r = 0; r < n; ++r)
+          if (M[r * n + min_col] < ORC_SMALL && a_cr[r] == -1) {
+            map[min_col] = r; a_cr[r] = min_col; done = 1; break;
+          }
+      } else {
+        for (int c = 0; c < n; ++c)
+          if (M[min_row * n + c] < ORC_SMALL && map[c] == -1) {
+            map[c] = min_row; a_cr[min_row] = c; done = 1; break;
+          }
+      }
+      if (!done) break;
+      ++nassigned;
+    }
+    if (nassigned == n) break;
+    /* greedy line cover of the zeros */
+    for (int k = 0; k < n; ++k) { lr[k] = 0; lc[k] = 0; rz[k] = cz[k] = rnz[k] = cnz[k] = 0; }
+    int total = 0;
+    for (int r = 0; r < n; ++r)
+      for (int c = 0; c < n; ++c) {
+        if (M[r * n + c] < ORC_SMALL) { rz[r]++; cz[c]++; ++total; }
+        else { rnz[r]++; cnz[c]++; }
+      }
+    for (int step = 0; total > 0; ++step) {
+      int max_diff = -1, max_zero = -1, max_idx = -1, is_row = 1;
+      for (int r = 0; r < n; ++r) {
+        if (lr[r]) continue;
+        const int d = step > 0 ? rz[r] - rnz[r] : rz[r];
+        if (max_zero == -1 || d > max_diff) { max_diff = d; max_zero = rz[r]; max_idx = r; }
+      }
+      for (int c = 0; c < n; ++c) {
+        if (lc[c]) continue;
+        const int d = step > 0 ? cz[c] - cnz[c] : cz[c];
+        if (d > max_diff) { max_diff = d; max_zero = cz[c]; max_idx = c; is_row = 0; }
+      }
+      if (max_idx < 0) break; /* every line drawn */
+      if (is_row) {
+        lr[max_idx] = 1;
+        for (int c = 0; c < n; ++c) {
+          if (M[max_idx * n + c] < ORC_SMALL) cz[c]--; else cnz[c]--;
+        }
+      } else {
+        lc[max_idx] = 1;
+        for (int r = 0; r < n; ++r) {
+          if (M[r * n + max_idx] < ORC_SMALL) rz[r]--; else rnz[r]--;
+        }
+      }
+      total -= max_zero;
+    }
+    /* update by the smallest uncovered element */
+    double mu = 1.7976931348623157e308;
+    for (int r = 0; r < n; ++r) {
+      if (lr[r]) continue;
+      for (int c = 0; c < n; ++c) if (!lc[c] && M[r * n + c] < mu) mu = M[r * n + c];
+    }
+    for (int r = 0; r < n; ++r)
+      for (int c = 0; c < n; ++c) {
+        if (lr[r] && lc[c]) M[r * n + c] += mu;
+        else if (!lr[r] && !lc[c]) M[r * n + c] -= mu;
+      }
+  }
+  free(a_cr); free(cnt); free(lr);
+  return nassigned;
+}
+
+/* SymmetricRmsdCalc::SymmRMSD_CenteredRef (src/SymmetricRmsdCalc.cpp:125-193)
+ * for one frame.  tgt: the selected target atoms (n x 3); ref: the selected
+ * reference (centred when fitting, as ReferenceAction leaves it); mass: the
+ * selected target masses.  Groups: atoms gatoms[goff[g] .. goff[g+1]) (indices
+ * into the selection), as SymmetricRmsdCalc::SetupSymmRMSD leaves them.  With
+ * fit: best fit of the target as given, rotated onto the reference (Frame::
+ * Rotate, src/Frame.h:508-517); per group the squared distances
+ * DIST2_NoImage(ref[ra], tgt'[ta]) (src/DistRoutines.cpp:411-419) into a
+ * Hungarian cost matrix (row ta, column ra); amap[ra] = the target atom of
+ * the assigned row; the target remapped (coordinates and masses,
+ * Frame::SetCoordinatesByMap, src/Frame.cpp:765-797) and fitted again (or
+ * RMSD_NoFit).  amap (n), U and trans (the final fit's, may be NULL) out.
+ * A column the heuristic leaves unassigned keeps its own atom (the reference
+ * would index the group at -1).                                               */
+double orc_symm_rmsd(const double *tgt, const double *ref, const double *mass, int n,
+                     int use_mass, int fit, int ngroups, const int *goff, const int *gatoms,
+                     int *amap, double *U, double *trans)
+{
+  int gmax = 0;
+  for (int g = 0; g < ngroups; ++g)
+    if (goff[g + 1] - goff[g] > gmax) gmax = goff[g + 1] - goff[g];
+  double *T = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+  double *mr = (double *)malloc(sizeof(double) * (size_t)n);
+  double *M = (double *)malloc(sizeof(double) * ((size_t)gmax * gmax + 1));
+  int *map = (int *)malloc(sizeof(int) * ((size_t)gmax + 1));
+  if (!T || !mr || !M || !map) { free(T); free(mr); free(M); free(map); return -2.0; }
+  for (int i = 0; i < n; ++i) amap[i] = i;
+  memcpy(T, tgt, sizeof(double) * 3 * (size_t)n);
+  double Uw[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, tw[3] = {0, 0, 0};
+  if (fit) {
+    orc_rmsd_centered_ref(T, ref, mass, n, use_mass, Uw, tw);
+    for (int i = 0; i < n; ++i) {
+      const double x = T[3 * i], y = T[3 * i + 1], z = T[3 * i + 2];
+      T[3 * i] = (x * Uw[0]) + (y * Uw[1]) + (z * Uw[2]);
+      T[3 * i + 1] = (x * Uw[3]) + (y * Uw[4]) + (z * Uw[5]);
+      T[3 * i + 2] = (x * Uw[6]) + (y * Uw[7]) + (z * Uw[8]);
+    }
+  }
+  for (int g = 0; g < ngroups; ++g) {
+    const int *ga = gatoms + goff[g];
+    const int m = goff[g + 1] - goff[g];
+    int e = 0;
+    for (int ta = 0; ta < m; ++ta)
+      for (int ra = 0; ra < m; ++ra) {
+        const double *a1 = ref + 3 * ga[ra], *a2 = T + 3 * ga[ta];
+        const double x = a1[0] - a2[0], y = a1[1] - a2[1], z = a1[2] - a2[2];
+        M[e++] = (x * x + y * y + z * z);
+      }
+    orc_hungarian(M, m, map);
+    for (int c = 0; c < m; ++c) amap[ga[c]] = map[c] >= 0 ? ga[map[c]] : ga[c];
+  }
+  for (int i = 0; i < n; ++i) {
+    memcpy(T + 3 * i, tgt + 3 * amap[i], sizeof(double) * 3);
+    mr[i] = mass ? mass[amap[i]] : 1.0;
+  }
+  double r;
+  if (fit) {
+    r = orc_rmsd_centered_ref(T, ref, mr, n, use_mass, Uw, tw);
+  } else {
+    r = orc_rmsd_nofit(T, ref, mr, n, use_mass);
+  }
+  if (U) memcpy(U, Uw, sizeof(Uw));
+  if (trans) memcpy(trans, tw, sizeof(tw));
+  free(T); free(mr); free(M); free(map);
+  return r;
+}
+
+/* Action_SymmetricRmsd::DoAction (src/Action_SymmetricRmsd.cpp:88-112) over a
+ * frame set against one prepared reference: gather, symmetric RMSD, and per
+ * frame the atom map (amap, F x nsel, may be NULL), U (F x 9) and the
+ * target translation (F x 3), both may be NULL.                              */
+int orc_symm_frames(const float *frames, int64_t stride, int nframes, const int *sel, int nsel,
+                    const double *mass, const double *ref_sel, int use_mass, int fit,
+                    int ngroups, const int *goff, const int *gatoms, double *rmsd, int *amap,
+                    double *U, double *trans)
+{
+  int err = 0;
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+  {
+    double *T = (double *)malloc(sizeof(double) * 3 * (size_t)nsel);
+    int *am = (int *)malloc(sizeof(int) * (size_t)nsel);
+    if (!T || !am) {
+#ifdef _OPENMP
+#pragma omp atomic write
+#endif
+      err = 1;
+    }
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (int f = 0; f < nframes; ++f) {
+      if (!T || !am) continue;
+      orc_gather(frames + (int64_t)f * stride, sel, nsel, T);
+      double Uf[9], tf[3];
+      rmsd[f] = orc_symm_rmsd(T, ref_sel, mass, nsel, use_mass, fit, ngroups, goff, gatoms, am,
+                              Uf, tf);
+      if (amap) memcpy(amap + (int64_t)f * nsel, am, sizeof(int) * (size_t)nsel);
+      if (U) memcpy(U + 9 * (int64_t)f, Uf, sizeof(Uf));
+      if (trans) memcpy(trans + 3 * (int64_t)f, tf, sizeof(tf));
+    }
+    free(T);
+    free(am);
+  }
+  return err;
+}

@@ -10,6 +10,7 @@ Inputs (read as data, never executed):
   test/tz2.truncoct.parm7, test/tz2.truncoct.nc (5827 atoms, 10 frames)
   test/rGACC.full.parm7, test/rGACC.full.nc     (7622 atoms, 1 frame)
   test/Test_2DRMS/*.save, test/Test_RMSD/*.save (expected outputs)
+  test/AFV.parm7, test/AFV.nc, test/Test_SymmRmsd/* (symmetric RMSD)
 
 Outputs (numpy .npz, no pickles):
   tz2_2drms.npz    frames 1-10 of tz2.nc, masses, masks :3-7 / :2 / :11 /
@@ -31,6 +32,13 @@ Outputs (numpy .npz, no pickles):
                    11 atoms of :2-12@CA) and the expected RMS average
                    correlation series (reference, offset 10, first); the
                    frames are those of tz2_camask.npz
+  symm_afv.npz     Test_SymmRmsd, AFV heavy atoms: reference AFV.rst7, the
+                   three rotated structures and AFV.nc (150 frames), the
+                   symmetric groups, masses, expected NoSymm / Symm RMSDs
+  symm_asp.npz, symm_glu.npz, symm_tyr.npz
+                   Test_SymmRmsd per residue: all frames and atoms, groups,
+                   masses, expected NOSYMM / SYMM RMSDs and the remapped,
+                   fitted coordinates (remap.crd.save)
   text/            the expected text outputs themselves (Test_2DRMS,
                    Test_RMSD, Test_PerResRMSD and Test_RmsAvgCorr .save
                    files), for byte-level checks of the host layer's
@@ -184,6 +192,97 @@ def read_crd(path, natom, nbox=0):
     return a[:, :3 * natom].reshape(-1, natom, 3)
 
 
+def symmetric_groups(path, sel):
+    """Groups of potentially symmetric selected atoms, as cpptraj's
+    SymmetricRmsdCalc::SetupSymmRMSD (src/SymmetricRmsdCalc.cpp:28-119) forms
+    them through AtomMap::SymmetricAtoms (src/AtomMap.cpp:374-439), written
+    here from that behaviour to produce the inputs of the symmetric RMSD
+    fixtures (the library takes the groups from its caller):
+      * per residue up to the last selected one, bonds within the residue,
+        neighbour lists in atom order;
+      * a 'chiral' atom: 4 bonds and fewer than 3 bonded terminal atoms
+        (AtomMap::CheckBonds, :269-298);
+      * atom ID: element + sorted neighbour elements; unique ID: that plus the
+        atom IDs one bond out, two bonds out for residues over 9 atoms and
+        three for residues over 19, all but the first character sorted
+        (AtomMap::AssignUniqueAtomIDs, :132-187);
+      * a group: the atoms with one unique ID reached from its first atom
+        without entering a chiral atom (FindSymmetricAtoms, :336-367); groups
+        of one atom are dropped, the rest kept in visiting order;
+      * then only selected atoms, as selection indices; groups left with at
+        least one atom are kept.
+    Elements come from ATOMIC_NUMBER (any one-to-one character map gives the
+    same equalities)."""
+    f = read_parm7(path)
+    natom = len(f["ATOM_NAME"])
+    z = [int(v) for v in f["ATOMIC_NUMBER"]]
+    ch = [chr(0x100 + v) for v in z]
+    nb = [[] for _ in range(natom)]
+    for key in ("BONDS_INC_HYDROGEN", "BONDS_WITHOUT_HYDROGEN"):
+        v = [int(t) for t in f.get(key, [])]
+        for k in range(0, len(v), 3):
+            a, b = v[k] // 3, v[k + 1] // 3
+            nb[a].append(b)
+            nb[b].append(a)
+    ptr = [int(v) - 1 for v in f["RESIDUE_POINTER"]] + [natom]
+    sel = [int(a) for a in sel]
+    sel_idx = {a: i for i, a in enumerate(sel)}
+    res_of = np.zeros(natom, np.int64)
+    for r in range(len(ptr) - 1):
+        res_of[ptr[r]:ptr[r + 1]] = r
+    last_res = int(res_of[sel[-1]]) + 1
+    groups = []
+    for r in range(last_res):
+        a0, a1 = ptr[r], ptr[r + 1]
+        n = a1 - a0
+        bl = [sorted(b - a0 for b in nb[a] if a0 <= b < a1) for a in range(a0, a1)]
+        c = [ch[a] for a in range(a0, a1)]
+        chiral = [len(bl[i]) == 4 and sum(len(bl[j]) == 1 for j in bl[i]) < 3 for i in range(n)]
+        aid = [c[i] + "".join(sorted(c[j] for j in bl[i])) for i in range(n)]
+        uid = []
+        for i in range(n):
+            u = aid[i]
+            if n > 9:
+                for b in bl[i]:
+                    u += aid[b]
+                    for c2 in bl[b]:
+                        if c2 != i:
+                            u += aid[c2]
+                            if n > 19:
+                                for d in bl[c2]:
+                                    if d != i:
+                                        u += aid[d]
+            uid.append(u[0] + "".join(sorted(u[1:])))
+        dup = [sum(uid[j] == uid[i] for j in range(n)) > 1 for i in range(n)]
+        status = [0] * n  # 0 unselected, 1 nonsymm, 2 symm
+        for at in range(n):
+            if not dup[at]:
+                status[at] = 1
+            elif status[at] != 2:
+                seen = [False] * n
+                found = []
+
+                def visit(x, want):
+                    if seen[x]:
+                        return
+                    seen[x] = True
+                    if uid[x] == want:
+                        found.append(x)
+                    for y in bl[x]:
+                        if not chiral[y]:
+                            visit(y, want)
+                visit(at, uid[at])
+                if len(found) == 1:
+                    status[found[0]] = 1
+                elif len(found) > 1:
+                    for x in found:
+                        status[x] = 2
+                    g = [sel_idx[x + a0] for x in found if x + a0 in sel_idx]
+                    if g:
+                        groups.append(g)
+    return groups
+
+
 # --------------------------------------------------------------------------
 def main():
     ap = argparse.ArgumentParser()
@@ -308,12 +407,51 @@ def main():
                       ("Test_PerResRMSD", ["PerResRMSD.agr.save", "perresavg.dat.save",
                                            "center.agr.save"]),
                       ("Test_RmsAvgCorr", ["rmscorr.dat.save", "rmscorr.10.dat.save",
-                                           "rmscorr.first.dat.save"])):
+                                           "rmscorr.first.dat.save"]),
+                      ("Test_SymmRmsd", ["rmsd.dat.save", "afv.rmsd.dat.save",
+                                         "ASP.rmsd.dat.save", "GLU.rmsd.dat.save",
+                                         "TYR.rmsd.dat.save"])):
         for n in names:
             with open(os.path.join(T, sub, n), "rb") as fi:
                 data = fi.read()
             with open(os.path.join(txt, sub + "__" + n), "wb") as fo:
                 fo.write(data)
+    # ---- Test_SymmRmsd -----------------------------------------------------
+    S = os.path.join(T, "Test_SymmRmsd")
+
+    def pack_groups(groups):
+        off = np.zeros(len(groups) + 1, np.int32)
+        for g, a in enumerate(groups):
+            off[g + 1] = off[g] + len(a)
+        return off, np.concatenate([np.asarray(a, np.int32) for a in groups])
+
+    # AFV, heavy atoms (!@H=): reference AFV.rst7, three rotated structures
+    # (rmsd.dat.save: NoSymm, Symm) and 150 frames of AFV.nc against its first
+    # frame (afv.rmsd.dat.save).
+    pa = Parm(os.path.join(T, "AFV.parm7"))
+    sel = pa.select(no_h=True)
+    off, flat = pack_groups(symmetric_groups(os.path.join(T, "AFV.parm7"), sel))
+    rot = np.stack([read_rst7(os.path.join(S, n)) for n in
+                    ("AFV.rotate2.rst7", "AFV.rotate3.rst7", "AFV.rotate23.rst7")])
+    afv_nc = read_nc(os.path.join(T, "AFV.nc"))
+    np.savez_compressed(
+        os.path.join(OUT, "symm_afv.npz"),
+        sel=sel, goff=off, gatoms=flat, mass=pa.mass,
+        ref=read_rst7(os.path.join(S, "AFV.rst7")), rotated=rot,
+        expect_rotated=read_series(os.path.join(S, "rmsd.dat.save")),
+        traj=afv_nc, expect_traj=read_series(os.path.join(S, "afv.rmsd.dat.save"))[:, 0])
+    # ASP / GLU / TYR, all atoms, reference = first frame, with 'remap':
+    # NOSYMM / SYMM columns and the remapped, fitted coordinates.
+    for res in ("ASP", "GLU", "TYR"):
+        pp = os.path.join(S, res + ".parm7")
+        pr = Parm(pp)
+        allsel = np.arange(pr.natom, dtype=np.int32)
+        off, flat = pack_groups(symmetric_groups(pp, allsel))
+        np.savez_compressed(
+            os.path.join(OUT, f"symm_{res.lower()}.npz"),
+            goff=off, gatoms=flat, mass=pr.mass, frames=read_nc(os.path.join(S, res + ".nc")),
+            expect=read_series(os.path.join(S, res + ".rmsd.dat.save")),
+            remap=read_crd(os.path.join(S, res + ".remap.crd.save"), pr.natom))
     for fn in sorted(os.listdir(OUT)):
         if fn.endswith(".npz"):
             print(fn, os.path.getsize(os.path.join(OUT, fn)))

@@ -1,5 +1,6 @@
-"""The reference's own regression outputs (test/Test_2DRMS, test/Test_RMSD),
-reproduced by the oracle on CPU and by the HIP library on the GPU.
+"""The reference's own regression outputs (test/Test_2DRMS, test/Test_RMSD,
+test/Test_PerResRMSD, test/Test_RmsAvgCorr, test/Test_SymmRmsd), reproduced
+by the oracle on CPU and by the HIP library on the GPU.
 
 The .save files are printed at 3-4 decimals, so they pin semantics (masks,
 weighting, orientation, layout, chaining of coordinate updates), not
@@ -51,6 +52,21 @@ class OracleBackend:
         rc = None if ref is None else self.o.center_on_origin(ref)[0]
         _, ct, sd = self.o.avgcorr(frames, ref_sel_centered=rc, offset=offset)
         return ct, sd
+
+    def symm(self, frames, ref, groups, sel=None, mass=None, fit=True, use_mass=False):
+        """Symmetric RMSD against one reference structure (all atoms, AoS):
+        (rmsd, amap, U, tvec) with tvec = (Trans, refTrans) per frame."""
+        r = ref.reshape(-1, 3).astype(np.float64)
+        if sel is not None:
+            r = r[sel]
+        msel = None if mass is None else (mass[sel] if sel is not None else mass)
+        rt = np.zeros(3)
+        if fit:
+            r, rt = self.o.center_on_origin(r, msel, use_mass)
+        rm, amap, U, tr = self.o.symm(frames.reshape(len(frames), -1), sel, msel, r, use_mass,
+                                      fit, groups)
+        tv = np.concatenate([tr, np.tile(rt, (len(frames), 1))], axis=1)
+        return rm, amap, U, tv
 
 
 class GpuBackend:
@@ -108,6 +124,21 @@ class GpuBackend:
         R = None if ref is None else self._dev(ref.reshape(1, -1), self.t.float32)
         _, ct, sd = ca.rms_avgcorr(X, ref_frames=R, offset=offset, ctx=self.ctx)
         return ct.cpu().numpy(), sd.cpu().numpy()
+
+    def symm(self, frames, ref, groups, sel=None, mass=None, fit=True, use_mass=False):
+        import cpptraj_amd as ca
+        t = self.t
+        X = self._dev(frames.reshape(len(frames), -1), t.float32)
+        R = self._dev(ref.reshape(1, -1), t.float32)
+        s = self._dev(sel, t.int32)
+        msel = None if mass is None else (mass[sel] if sel is not None else mass)
+        m = self._dev(msel, t.float64)
+        rm, amap, U, tv = ca.rmsd_symm(X, groups, ref_frames=R, mask=s, mass=m, fit=fit,
+                                       use_mass=use_mass, want_map=True, want_xform=fit,
+                                       ctx=self.ctx)
+        return (rm.cpu().numpy(), amap.cpu().numpy(),
+                None if U is None else U.cpu().numpy().reshape(-1, 9),
+                None if tv is None else tv.cpu().numpy())
 
 
 @pytest.fixture(params=["oracle", pytest.param("gpu", marks=pytest.mark.gpu)])
@@ -266,3 +297,42 @@ def test_avgcorr_series(golden, backend, case):
     assert ct.shape == (len(exp),)
     assert np.abs(ct - exp[:, 0]).max() <= HALF4
     assert np.abs(sd - exp[:, 1]).max() <= HALF4
+
+
+# ---------------------------------------------------------------------------------
+# Test_SymmRmsd.  Symmetric groups from the topologies are part of the
+# fixtures (tests/golden/make_fixtures.py, symmetric_groups).
+def _groups(d):
+    off, flat = d["goff"], d["gatoms"]
+    return [flat[off[g]:off[g + 1]].tolist() for g in range(len(off) - 1)]
+
+
+def test_symm_afv_rotated(golden, backend):
+    # 'symmrmsd Symm !@H= reference': a PHE ring flip (frame 1) is undone, a
+    # VAL rotation (frame 2) is not (its CB counts as chiral).
+    d = golden("symm_afv")
+    rm, _, _, _ = backend.symm(d["rotated"], d["ref"], _groups(d), sel=d["sel"])
+    assert np.abs(rm - d["expect_rotated"][:, 1]).max() <= HALF4
+
+
+def test_symm_afv_trajectory(golden, backend):
+    # 'symmrmsd AFV !@H=' over 150 frames of AFV.nc, reference = first frame.
+    d = golden("symm_afv")
+    traj = d["traj"]
+    rm, _, _, _ = backend.symm(traj, traj[0], _groups(d), sel=d["sel"])
+    assert np.abs(rm - d["expect_traj"]).max() <= HALF4
+
+
+@pytest.mark.parametrize("res", ["asp", "glu", "tyr"])
+def test_symm_remap(golden, backend, res):
+    # 'symmrmsd first SYMM remap' on all atoms: RMSDs, and the frames
+    # remapped and fitted (trajout of the remapped, Trans_Rot_Trans'd frame).
+    d = golden(f"symm_{res}")
+    fr = d["frames"]
+    rm, amap, U, tv = backend.symm(fr, fr[0], _groups(d))
+    assert np.abs(rm - d["expect"][:, 1]).max() <= HALF4
+    out = np.empty(fr.shape)
+    for f in range(len(fr)):
+        x = fr[f].astype(np.float64)[amap[f]] + tv[f, :3]
+        out[f] = x @ U[f].reshape(3, 3).T + tv[f, 3:]
+    assert np.abs(out - d["remap"]).max() <= HALF3

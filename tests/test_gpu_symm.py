@@ -1,0 +1,190 @@
+"""Symmetry-corrected RMSD (rmsd_symm) against the oracle.
+
+The assignment follows the reference's Hungarian heuristic step for step, on
+cost matrices from the initial fit (QCP here, Kabsch in the oracle: the same
+rotation to ~1e-15), so the atom maps must be identical.  The heuristic
+tests reduced costs for zero against an absolute 1e-14, so where a group
+has near-equal alternative assignments (many atoms crowded together, or
+costs in the thousands of A^2) a last-bit difference in the rotation can
+change its pairing.  Hence: with a fit, compact groups of up to 12 atoms
+against the oracle; 32-atom groups without fit (costs computed identically
+on both sides); and crowded 32-atom groups with a fit checked against the
+oracle's Hungarian run on the cost matrices of the library's own initial
+fit.  RMSD, U and the
+translations then come from the final fit: 1e-9 A / 1e-9 against the oracle
+with fit, 1e-12 A without.  Cases: groups of 1 atom, 2-8 atoms (working
+matrix in LDS) and 12 and 32 atoms (global scratch); fit and no fit, with
+and without masses; SoA frames, a selection and a frame list; a reference
+frame from another set; 20,000 frames; argument errors.
+"""
+import numpy as np
+import pytest
+
+from cpptraj_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+SIZES = (2, 3, 2, 4, 3, 2, 1, 8, 12, 32)
+SIZES_FIT = (2, 3, 2, 4, 3, 2, 1, 8, 12)
+
+
+def _oracle(oracle, X, ref, groups, m, fit, use_mass, sel=None):
+    r = ref if sel is None else ref[sel]
+    msel = m if sel is None else m[sel]
+    rt = np.zeros(3)
+    if fit:
+        r, rt = oracle.center_on_origin(r, msel, use_mass)
+    rm, amap, U, tr = oracle.symm(X, sel, msel, r, use_mass, fit, groups)
+    return rm, amap, U, tr, rt
+
+
+@pytest.mark.parametrize("fit", [True, False])
+@pytest.mark.parametrize("use_mass", [False, True])
+def test_symm_matches_oracle(ctx, oracle, fit, use_mass):
+    import torch
+
+    import cpptraj_amd as ca
+    X, ref, groups, m, _ = synth.symm_case(71, 200, 300, sizes=SIZES_FIT if fit else SIZES,
+                                           rigid=fit, compact=fit)
+    assert max(len(g) for g in groups) == (12 if fit else 32)
+    assert min(len(g) for g in groups) == 1
+    Xd = torch.as_tensor(X).cuda()
+    R = torch.as_tensor(ref.reshape(1, -1).astype(np.float32)).cuda()
+    md = torch.as_tensor(m).cuda()
+    rm, amap, U, tv = ca.rmsd_symm(Xd, groups, ref_frames=R, mass=md, fit=fit, use_mass=use_mass,
+                                   want_map=True, want_xform=fit, ctx=ctx)
+    # the oracle's reference is the FP32-rounded one the GPU sees
+    eo, ao, Uo, to, rt = _oracle(oracle, X, ref.astype(np.float32).astype(np.float64), groups, m,
+                                 fit, use_mass)
+    assert np.array_equal(amap.cpu().numpy(), ao)
+    tol = 1e-9 if fit else 1e-12
+    assert np.abs(rm.cpu().numpy() - eo).max() <= tol
+    if fit:
+        assert np.abs(U.cpu().numpy().reshape(-1, 9) - Uo).max() <= 1e-9
+        tv = tv.cpu().numpy()
+        assert np.abs(tv[:, :3] - to).max() <= 1e-9
+        assert np.abs(tv[:, 3:] - rt).max() <= 1e-12
+
+
+def test_symm_device_hungarian_on_crowded_groups(ctx, oracle):
+    # 32 atoms within 1.5 A of one another: the pairing depends on the last
+    # bits of the initial rotation.  The library's maps must be what the
+    # oracle's Hungarian makes of the library's own cost matrices (initial fit
+    # U and Trans from rmsd_fit, the same kernels rmsd_symm runs first).
+    import torch
+
+    import cpptraj_amd as ca
+    X, ref, groups, m, _ = synth.symm_case(75, 40, 300, sizes=SIZES)
+    Xd = torch.as_tensor(X).cuda()
+    R = torch.as_tensor(ref.reshape(1, -1).astype(np.float32)).cuda()
+    _, amap, _, _ = ca.rmsd_symm(Xd, groups, ref_frames=R, want_map=True, ctx=ctx)
+    _, U0, t0 = ca.rmsd_fit(Xd.clone(), R, modify="nomod", save_matrices=True, save_vectors=True,
+                            ctx=ctx)
+    amap, U0, t0 = amap.cpu().numpy(), U0.cpu().numpy().reshape(-1, 9), t0.cpu().numpy()
+    rc, _ = oracle.center_on_origin(ref.astype(np.float32).astype(np.float64))
+    for f in range(len(X)):
+        x = X[f].reshape(-1, 3).astype(np.float64) + t0[f, :3]
+        u = U0[f]
+        xr = np.stack([(x[:, 0] * u[0]) + (x[:, 1] * u[1]) + (x[:, 2] * u[2]),
+                       (x[:, 0] * u[3]) + (x[:, 1] * u[4]) + (x[:, 2] * u[5]),
+                       (x[:, 0] * u[6]) + (x[:, 1] * u[7]) + (x[:, 2] * u[8])], axis=1)
+        for g in groups:
+            g = np.asarray(g)
+            d = rc[g][None, :, :] - xr[g][:, None, :]  # [ta, ra]
+            M = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+            mp = oracle.hungarian(M)
+            exp = np.where(mp >= 0, g[np.maximum(mp, 0)], g)
+            assert np.array_equal(amap[f, g], exp), (f, len(g))
+
+
+def test_symm_soa_selection_frame_list_and_reference_set(ctx, oracle):
+    import torch
+
+    import cpptraj_amd as ca
+    X, ref, groups, m, _ = synth.symm_case(72, 60, 150, sizes=(2, 3, 4))
+    F, n, ld = X.shape[0], 150, 160
+    # select every atom but 10 (groups re-indexed into the selection)
+    drop = set(range(0, 150, 15))
+    sel = np.array([a for a in range(n) if a not in drop], np.int32)
+    idx = {a: i for i, a in enumerate(sel)}
+    gsel = [[idx[a] for a in g if a in idx] for g in groups]
+    gsel = [g for g in gsel if g]
+    S = np.zeros((F, 3, ld), np.float32)
+    S[:, :, :n] = X.reshape(F, n, 3).transpose(0, 2, 1)
+    Sd = torch.as_tensor(S).cuda()
+    fl = np.arange(F)[::2][::-1].copy()
+    fl_d = torch.as_tensor(fl.astype(np.int32)).cuda()
+    sel_d = torch.as_tensor(sel).cuda()
+    m_d = torch.as_tensor(m[sel]).cuda()
+    d = ca.frames_desc(Sd, sel_d, m_d, fidx=fl_d, soa=True, natom=n)
+    # reference: stored frame 3 of a separate AoS set of 5 structures
+    rset = np.stack([ref + np.random.default_rng(k).normal(size=ref.shape) * 0.1
+                     for k in range(5)]).astype(np.float32)
+    Rd = torch.as_tensor(rset.reshape(5, -1)).cuda()
+    r = ca.frames_desc(Rd, sel_d, m_d)
+    off, flat = ca.symm_groups(gsel)
+    nf = len(fl)
+    rm = torch.empty(nf, dtype=torch.float64, device="cuda")
+    amap = torch.empty((nf, len(sel)), dtype=torch.int32, device="cuda")
+    import ctypes as C
+    IP = C.POINTER(C.c_int32)
+    ctx.check(ctx._L.rmsd_symm(ctx.handle, C.byref(d), C.byref(r), 3, len(gsel),
+                               off.ctypes.data_as(IP), flat.ctypes.data_as(IP), 1, 1,
+                               C.c_void_p(rm.data_ptr()), C.c_void_p(amap.data_ptr()), None, None))
+    got_r, got_a = rm.cpu().numpy(), amap.cpu().numpy()
+    del fl_d, sel_d, m_d
+    eo, ao, _, _, _ = _oracle(oracle, X[fl], rset[3].astype(np.float64).reshape(-1, 3), gsel, m,
+                              True, True, sel=sel)
+    assert np.array_equal(got_a, ao)
+    assert np.abs(got_r - eo).max() <= 1e-9
+
+
+def test_symm_20000_frames(ctx, oracle):
+    import torch
+
+    import cpptraj_amd as ca
+    X, ref, groups, m, _ = synth.symm_case(73, 20000, 200, ngroups=60)
+    Xd = torch.as_tensor(X).cuda()
+    R = torch.as_tensor(X[:1]).cuda()
+    rm, amap, _, _ = ca.rmsd_symm(Xd, groups, ref_frames=R, want_map=True, ctx=ctx)
+    eo, ao, _, _, _ = _oracle(oracle, X, X[0].astype(np.float64).reshape(-1, 3), groups, m,
+                              True, False)
+    assert np.array_equal(amap.cpu().numpy(), ao)
+    # frame 0 against itself: both at their near-zero floors
+    assert np.abs(rm.cpu().numpy()[1:] - eo[1:]).max() <= 1e-9
+    assert rm.cpu().numpy()[0] < 1e-6
+
+
+def test_symm_no_groups_is_the_fit(ctx):
+    import torch
+
+    import cpptraj_amd as ca
+    X, ref, _, m, _ = synth.symm_case(74, 50, 120)
+    Xd = torch.as_tensor(X).cuda()
+    R = torch.as_tensor(ref.reshape(1, -1).astype(np.float32)).cuda()
+    rm, _, U, tv = ca.rmsd_symm(Xd, [], ref_frames=R, want_xform=True, ctx=ctx)
+    r2, U2, tv2 = ca.rmsd_fit(Xd.clone(), R, modify="nomod", save_matrices=True,
+                              save_vectors=True, ctx=ctx)
+    assert (rm - r2).abs().max().item() <= 1e-12
+    assert (U - U2).abs().max().item() <= 1e-12
+    assert (tv - tv2).abs().max().item() <= 1e-12
+
+
+def test_symm_argument_errors(ctx):
+    import torch
+
+    import cpptraj_amd as ca
+    X = torch.zeros((3, 30), dtype=torch.float32, device="cuda")
+    with pytest.raises(ca.RmsdError, match="out of range"):
+        ca.rmsd_symm(X, [[0, 10]], ctx=ctx)
+    with pytest.raises(ca.RmsdError, match="more than one symmetric group"):
+        ca.rmsd_symm(X, [[0, 1], [1, 2]], ctx=ctx)
+    Y = torch.zeros((2, 3 * 40), dtype=torch.float32, device="cuda")
+    with pytest.raises(ca.RmsdError, match="33 atoms"):
+        ca.rmsd_symm(Y, [list(range(33))], ctx=ctx)
+    with pytest.raises(ca.RmsdError, match="group_off"):
+        ca.rmsd_symm(X, (np.array([1, 3], np.int32), np.array([0, 1, 2], np.int32)), ctx=ctx)
+    with pytest.raises(ca.RmsdError, match="use_mass"):
+        ca.rmsd_symm(X, [[0, 1]], use_mass=True, ctx=ctx)
+    rm, _, _, _ = ca.rmsd_symm(X[:0], [[0, 1]], ref_frames=X, ctx=ctx)
+    assert rm.shape == (0,)

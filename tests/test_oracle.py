@@ -343,3 +343,71 @@ def test_avgcorr_windows_and_closed_forms(oracle):
     # 'first' with a constant trajectory: every averaged frame is the first
     w, ct, sd = oracle.avgcorr(np.repeat(X[:1], 9, axis=0))
     assert np.abs(ct).max() < 1e-6
+
+
+# -- symmetry-corrected RMSD ----------------------------------------------------
+def test_hungarian_matches_reference(oracle, reflib):
+    # Random costs, and small integers / one-decimal values full of ties that
+    # steer the heuristic's tie rules: the same map as the compiled reference.
+    rng = np.random.default_rng(61)
+    for t in range(6000):
+        n = int(rng.integers(1, 11))
+        if t % 3 == 0:
+            M = rng.uniform(0, 10, (n, n))
+        elif t % 3 == 1:
+            M = rng.integers(0, 4, (n, n)).astype(float)
+        else:
+            M = np.round(rng.uniform(0, 3, (n, n)), 1)
+        assert np.array_equal(oracle.hungarian(M), reflib.hungarian(M)), (t, M)
+
+
+def test_hungarian_recovers_planted_assignment(oracle):
+    # One zero per row and column on a permutation, positive costs elsewhere:
+    # the assignment is that permutation (map[col] = row).
+    rng = np.random.default_rng(62)
+    for n in range(1, 17):
+        perm = rng.permutation(n)
+        M = rng.uniform(1.0, 9.0, (n, n))
+        M[perm, np.arange(n)] = 0.0
+        assert np.array_equal(oracle.hungarian(M), perm)
+
+
+@pytest.mark.parametrize("fit", [True, False])
+@pytest.mark.parametrize("use_mass", [False, True])
+def test_symm_matches_reference(oracle, reflib, fit, use_mass):
+    X, ref, groups, m, _ = synth.symm_case(63, 30, 120, rigid=fit)
+    r = ref.copy()
+    if fit:
+        r, _ = oracle.center_on_origin(r, m, use_mass)
+    a = oracle.symm(X, None, m, r, use_mass, fit, groups)
+    b = reflib.symm(X, None, m, r, use_mass, fit, groups)
+    for x, y in zip(a, b):
+        assert same(x, y)
+
+
+def test_symm_undoes_planted_permutations(oracle):
+    # Frames = rigid motions of the reference with atoms permuted inside their
+    # groups: the map is the inverse permutation and the RMSD that of the
+    # un-permuted frame (0 without noise, to the Kabsch floor).
+    for sigma in (0.0, 0.05):
+        X, ref, groups, m, perms = synth.symm_case(64, 12, 90, sigma=sigma)
+        rc, _ = oracle.center_on_origin(ref)
+        rm, amap, _, _ = oracle.symm(X, None, None, rc, False, True, groups)
+        inv = np.argsort(perms, axis=1)
+        assert np.array_equal(amap, inv)
+        fixed = np.stack([X[f].reshape(-1, 3)[inv[f]] for f in range(len(X))])
+        r0, _, _, _ = oracle.symm(fixed.reshape(len(X), -1), None, None, rc, False, True, [])
+        assert np.abs(rm - r0).max() <= 1e-12
+        if sigma == 0.0:
+            # Kabsch's eigenvalue formula floor, ~sqrt(eps E0 / M)
+            assert rm.max() < 5e-6
+
+
+def test_symm_without_groups_is_the_plain_fit(oracle):
+    X, ref, _, m, _ = synth.symm_case(65, 8, 50)
+    rc, rt = oracle.center_on_origin(ref, m, True)
+    rm, amap, U, tr = oracle.symm(X, None, m, rc, True, True, [])
+    r2, U2, tv2, _ = oracle.fit_frames(X, rc, rt, mass=m, use_mass=True, modify=0,
+                                       want_frames=True)
+    assert same(rm, r2) and same(U, U2.reshape(-1, 9)) and same(tr, tv2[:, :3])
+    assert np.array_equal(amap, np.tile(np.arange(50), (8, 1)))
