@@ -861,7 +861,7 @@ int rmsd_symm(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* ref, 
                                            " is in more than one symmetric group");
       pos[a] = k;
     }
-    if (e - b > SYMM_GROUP_LDS) {
+    if (e - b > SYMM_GROUP_REG) {
       bigoff[g] = (int32_t)bigstride;
       bigstride += (int64_t)(e - b) * (e - b) + (6 * (e - b) + 7) / 8;
     }

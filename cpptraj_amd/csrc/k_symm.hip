@@ -15,10 +15,11 @@
 //
 // The initial fit is the frame-to-reference fit (k_fit.hip) run first.  One
 // lane per (frame, group) for the assignment: groups are small (2-3 atoms in
-// proteins), the heuristic is sequential and branchy, and lanes of one wave
-// take consecutive groups of one frame (nearby atoms).  Its working matrix
-// stays in LDS for groups of up to SYMM_GROUP_LDS atoms, in global scratch
-// for larger ones.  The final pass is one wavefront per frame, lanes over
+// proteins) and the heuristic is sequential and branchy, so lanes of one
+// wave take one group in 64 frames (one size per wave).  Groups of up to
+// SYMM_GROUP_REG atoms run a compile-time-size form with every array in
+// registers (no LDS: occupancy is set by registers alone); larger groups keep
+// their arrays in a slab of global scratch per (frame, group).  The final pass is one wavefront per frame, lanes over
 // atoms, one wave reduction of 18 sums.
 #include <hip/hip_runtime.h>
 #include <cfloat>
@@ -29,7 +30,6 @@
 namespace {
 
 constexpr int SYM_THREADS = 64;
-constexpr int SYM_WS_INTS = 6;  // per-row/column byte arrays of the heuristic
 
 __device__ __forceinline__ double scoord(const FrameSetDev& fs, int64_t f, int k, int c) {
   const int a = fs.sel ? fs.sel[k] : k;
@@ -40,15 +40,23 @@ __device__ __forceinline__ double scoord(const FrameSetDev& fs, int64_t f, int k
 // Hungarian::Optimize: row and column reduction, then at most n * n rounds of
 // greedy zero assignment, greedy line cover and update by the smallest
 // uncovered element.  M is n x n row-major (row = target atom, column =
-// reference atom); map[col] receives the assigned row or -1.  ws holds
-// 6 n bytes.  Lines are bit masks (n <= 32).
-__device__ void hungarian(double* M, int n, signed char* map, signed char* ws) {
+// reference atom); map[col] receives the assigned row or -1.  ws holds 5 n
+// entries.  Lines are bit masks (n <= 32).  Element k of every array sits at
+// [k * S] (S = 1: a private slab).
+template <int S, typename IT>
+struct Strided {
+  IT* p;
+  __device__ IT& operator[](int k) const { return p[k * S]; }
+};
+
+template <int S, typename IT>
+__device__ void hungarian(Strided<S, double> M, int n, Strided<S, IT> map, IT* wsp) {
 #pragma clang fp contract(off)
-  signed char* a_cr = ws;           // row -> column
-  signed char* rz = ws + n;         // zeros per row / column, non-zeros per row / column
-  signed char* cz = ws + 2 * n;
-  signed char* rnz = ws + 3 * n;
-  signed char* cnz = ws + 4 * n;
+  const Strided<S, IT> a_cr{wsp};  // row -> column
+  const Strided<S, IT> rz{wsp + n * S};   // zeros per row / column, non-zeros per row / column
+  const Strided<S, IT> cz{wsp + 2 * n * S};
+  const Strided<S, IT> rnz{wsp + 3 * n * S};
+  const Strided<S, IT> cnz{wsp + 4 * n * S};
   for (int r = 0; r < n; ++r) {
     double mn = DBL_MAX;
     for (int c = 0; c < n; ++c) mn = M[r * n + c] < mn ? M[r * n + c] : mn;
@@ -91,15 +99,15 @@ __device__ void hungarian(double* M, int n, signed char* map, signed char* ws) {
       if (min_col_z < min_row_z) {
         for (int r = 0; r < n && !done; ++r)
           if (M[r * n + min_col] < RMSD_SMALL && a_cr[r] == -1) {
-            map[min_col] = (signed char)r;
-            a_cr[r] = (signed char)min_col;
+            map[min_col] = (IT)r;
+            a_cr[r] = (IT)min_col;
             done = true;
           }
       } else {
         for (int c = 0; c < n && !done; ++c)
           if (M[min_row * n + c] < RMSD_SMALL && map[c] == -1) {
-            map[c] = (signed char)min_row;
-            a_cr[min_row] = (signed char)c;
+            map[c] = (IT)min_row;
+            a_cr[min_row] = (IT)c;
             done = true;
           }
       }
@@ -182,41 +190,185 @@ __device__ void hungarian(double* M, int n, signed char* map, signed char* ws) {
   }
 }
 
-__global__ __launch_bounds__(SYM_THREADS) void k_symm_assign(SymmArgs a) {
+// The same heuristic for a compile-time size N, every array in registers:
+// all loops unroll, and the few data-dependent indices (the chosen row or
+// column) are resolved by selects over the N candidates.
+template <int N>
+__device__ __forceinline__ double pick_col(const double (&M)[N * N], int r, int c_dyn) {
+  double v = 0.0;
+#pragma unroll
+  for (int c = 0; c < N; ++c) v = c == c_dyn ? M[r * N + c] : v;
+  return v;
+}
+
+template <int N>
+__device__ __forceinline__ double pick_row(const double (&M)[N * N], int r_dyn, int c) {
+  double v = 0.0;
+#pragma unroll
+  for (int r = 0; r < N; ++r) v = r == r_dyn ? M[r * N + c] : v;
+  return v;
+}
+
+template <int N>
+__device__ void hungarian_n(double (&M)[N * N], int (&map)[N]) {
 #pragma clang fp contract(off)
-  __shared__ double lds_m[SYM_THREADS * SYMM_GROUP_LDS * SYMM_GROUP_LDS];
-  __shared__ signed char lds_i[SYM_THREADS * (SYM_WS_INTS * SYMM_GROUP_LDS)];
-  const int64_t pair = (int64_t)blockIdx.x * SYM_THREADS + threadIdx.x;
-  if (pair >= (int64_t)a.fs.nframes * a.ngroups) return;
-  const int64_t f = pair / a.ngroups;
-  const int g = (int)(pair - f * a.ngroups);
-  const int g0 = a.goff[g];
-  const int n = a.goff[g + 1] - g0;
-  const int* ga = a.gatoms + g0;
-  int* out = a.gmap + f * a.ngatoms + g0;
-  if (n == 1) {
-    out[0] = ga[0];
-    return;
+  int a_cr[N], rz[N], cz[N], rnz[N], cnz[N];
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    double mn = DBL_MAX;
+#pragma unroll
+    for (int c = 0; c < N; ++c) mn = M[r * N + c] < mn ? M[r * N + c] : mn;
+#pragma unroll
+    for (int c = 0; c < N; ++c) M[r * N + c] -= mn;
   }
-  double* M;
-  signed char* ws;
-  if (n <= SYMM_GROUP_LDS) {
-    M = lds_m + threadIdx.x * (SYMM_GROUP_LDS * SYMM_GROUP_LDS);
-    ws = lds_i + threadIdx.x * (SYM_WS_INTS * SYMM_GROUP_LDS);
-  } else {
-    M = a.big + f * a.bigstride + a.bigoff[g];
-    ws = reinterpret_cast<signed char*>(M + n * n);
+#pragma unroll
+  for (int c = 0; c < N; ++c) {
+    double mn = DBL_MAX;
+#pragma unroll
+    for (int r = 0; r < N; ++r) mn = M[r * N + c] < mn ? M[r * N + c] : mn;
+#pragma unroll
+    for (int r = 0; r < N; ++r) M[r * N + c] -= mn;
   }
-  double U[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, t[3] = {0, 0, 0};
-  if (a.U0) {
-    for (int k = 0; k < 9; ++k) U[k] = a.U0[9 * f + k];
-    for (int k = 0; k < 3; ++k) t[k] = a.t0[6 * f + k];
+  for (int it = 0; it < N * N; ++it) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      map[k] = -1;
+      a_cr[k] = -1;
+    }
+    int nassigned = 0;
+    for (int round = 0; round < N; ++round) {
+      int min_row = -1, min_row_z = N + 1;
+#pragma unroll
+      for (int r = 0; r < N; ++r) {
+        int z = 0;
+#pragma unroll
+        for (int c = 0; c < N; ++c) z += (map[c] == -1 && M[r * N + c] < RMSD_SMALL) ? 1 : 0;
+        if (a_cr[r] == -1 && z != 0 && z < min_row_z) {
+          min_row_z = z;
+          min_row = r;
+        }
+      }
+      int min_col = -1, min_col_z = N + 1;
+#pragma unroll
+      for (int c = 0; c < N; ++c) {
+        int z = 0;
+#pragma unroll
+        for (int r = 0; r < N; ++r) z += (a_cr[r] == -1 && M[r * N + c] < RMSD_SMALL) ? 1 : 0;
+        if (map[c] == -1 && z != 0 && z < min_col_z) {
+          min_col_z = z;
+          min_col = c;
+        }
+      }
+      if (min_row == -1 && min_col == -1) break;
+      int ar = -1, ac = -1;  // the assignment made this round
+      if (min_col_z < min_row_z) {
+#pragma unroll
+        for (int r = N - 1; r >= 0; --r)
+          if (a_cr[r] == -1 && pick_col<N>(M, r, min_col) < RMSD_SMALL) ar = r;
+        ac = min_col;
+      } else {
+#pragma unroll
+        for (int c = N - 1; c >= 0; --c)
+          if (map[c] == -1 && pick_row<N>(M, min_row, c) < RMSD_SMALL) ac = c;
+        ar = min_row;
+      }
+      if (ar < 0 || ac < 0) break;
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        if (k == ac) map[k] = ar;
+        if (k == ar) a_cr[k] = ac;
+      }
+      ++nassigned;
+    }
+    if (nassigned == N) break;
+    uint32_t lr = 0, lc = 0;
+    int total = 0;
+#pragma unroll
+    for (int k = 0; k < N; ++k) rz[k] = cz[k] = rnz[k] = cnz[k] = 0;
+#pragma unroll
+    for (int r = 0; r < N; ++r)
+#pragma unroll
+      for (int c = 0; c < N; ++c) {
+        const bool z = M[r * N + c] < RMSD_SMALL;
+        rz[r] += z ? 1 : 0;
+        cz[c] += z ? 1 : 0;
+        rnz[r] += z ? 0 : 1;
+        cnz[c] += z ? 0 : 1;
+        total += z ? 1 : 0;
+      }
+    for (int step = 0; total > 0 && step < 2 * N; ++step) {
+      int max_diff = -1, max_zero = -1, max_idx = -1;
+      bool is_row = true;
+#pragma unroll
+      for (int r = 0; r < N; ++r) {
+        const int d = step > 0 ? rz[r] - rnz[r] : rz[r];
+        if (!(lr >> r & 1u) && (max_zero == -1 || d > max_diff)) {
+          max_diff = d;
+          max_zero = rz[r];
+          max_idx = r;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < N; ++c) {
+        const int d = step > 0 ? cz[c] - cnz[c] : cz[c];
+        if (!(lc >> c & 1u) && d > max_diff) {
+          max_diff = d;
+          max_zero = cz[c];
+          max_idx = c;
+          is_row = false;
+        }
+      }
+      if (max_idx < 0) break;
+      if (is_row) {
+        lr |= 1u << max_idx;
+#pragma unroll
+        for (int c = 0; c < N; ++c) {
+          const bool z = pick_row<N>(M, max_idx, c) < RMSD_SMALL;
+          cz[c] -= z ? 1 : 0;
+          cnz[c] -= z ? 0 : 1;
+        }
+      } else {
+        lc |= 1u << max_idx;
+#pragma unroll
+        for (int r = 0; r < N; ++r) {
+          const bool z = pick_col<N>(M, r, max_idx) < RMSD_SMALL;
+          rz[r] -= z ? 1 : 0;
+          rnz[r] -= z ? 0 : 1;
+        }
+      }
+      total -= max_zero;
+    }
+    double mu = DBL_MAX;
+#pragma unroll
+    for (int r = 0; r < N; ++r)
+#pragma unroll
+      for (int c = 0; c < N; ++c)
+        if (!(lr >> r & 1u) && !(lc >> c & 1u) && M[r * N + c] < mu) mu = M[r * N + c];
+#pragma unroll
+    for (int r = 0; r < N; ++r)
+#pragma unroll
+      for (int c = 0; c < N; ++c) {
+        const bool br = lr >> r & 1u, bc = lc >> c & 1u;
+        if (br && bc)
+          M[r * N + c] += mu;
+        else if (!br && !bc)
+          M[r * N + c] -= mu;
+      }
   }
+}
+
+// Cost matrix of one group: target atoms (rows) as the initial fit leaves
+// them (translated by Trans, Frame::Rotate by U) against the reference atoms
+// (columns), DIST2_NoImage.
+template <int S>
+__device__ __forceinline__ void group_costs(const SymmArgs& a, int64_t f, const int* ga, int n,
+                                            const double (&U)[9], const double (&t)[3],
+                                            Strided<S, double> M) {
+#pragma clang fp contract(off)
   for (int ta = 0; ta < n; ++ta) {
     double x = scoord(a.fs, f, ga[ta], 0), y = scoord(a.fs, f, ga[ta], 1),
            z = scoord(a.fs, f, ga[ta], 2);
     if (a.U0) {
-      // Translated to the origin, then Frame::Rotate.
       x += t[0];
       y += t[1];
       z += t[2];
@@ -233,11 +385,58 @@ __global__ __launch_bounds__(SYM_THREADS) void k_symm_assign(SymmArgs a) {
       M[ta * n + ra] = (dx * dx + dy * dy + dz * dz);
     }
   }
-  signed char* map = ws + 5 * n;
-  hungarian(M, n, map, ws);
-  for (int c = 0; c < n; ++c) out[c] = ga[map[c] >= 0 ? map[c] : c];
 }
 
+template <int N>
+__device__ __forceinline__ void assign_reg(const SymmArgs& a, int64_t f, const int* ga,
+                                           const double (&U)[9], const double (&t)[3], int* out) {
+  double M[N * N];
+  int map[N];
+  group_costs(a, f, ga, N, U, t, Strided<1, double>{M});
+  hungarian_n<N>(M, map);
+#pragma unroll
+  for (int c = 0; c < N; ++c) out[c] = ga[map[c] >= 0 ? map[c] : c];
+}
+
+// Lanes: consecutive frames of one group (pair = group * F + frame), so a
+// wave's groups share one size.
+__global__ __launch_bounds__(SYM_THREADS) void k_symm_assign(SymmArgs a) {
+  const int64_t F = a.fs.nframes;
+  const int64_t pair = (int64_t)blockIdx.x * SYM_THREADS + threadIdx.x;
+  if (pair >= F * a.ngroups) return;
+  const int g = (int)(pair / F);
+  const int64_t f = pair - (int64_t)g * F;
+  const int g0 = a.goff[g];
+  const int n = a.goff[g + 1] - g0;
+  const int* ga = a.gatoms + g0;
+  int* out = a.gmap + f * a.ngatoms + g0;
+  if (n == 1) {
+    out[0] = ga[0];
+    return;
+  }
+  double U[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, t[3] = {0, 0, 0};
+  if (a.U0) {
+    for (int k = 0; k < 9; ++k) U[k] = a.U0[9 * f + k];
+    for (int k = 0; k < 3; ++k) t[k] = a.t0[6 * f + k];
+  }
+  if (n == 2) {
+    assign_reg<2>(a, f, ga, U, t, out);
+  } else if (n == 3) {
+    assign_reg<3>(a, f, ga, U, t, out);
+  } else if (n == 4) {
+    assign_reg<4>(a, f, ga, U, t, out);
+  } else {
+    double* slab = a.big + f * a.bigstride + a.bigoff[g];
+    const Strided<1, double> M{slab};
+    signed char* ws = reinterpret_cast<signed char*>(slab + n * n);
+    const Strided<1, signed char> map{ws + 5 * n};
+    group_costs(a, f, ga, n, U, t, M);
+    hungarian(M, n, map, ws);
+    for (int c = 0; c < n; ++c) out[c] = ga[map[c] >= 0 ? map[c] : c];
+  }
+}
+
+template <bool FIT, bool WEIGHTED>
 __global__ __launch_bounds__(SYM_THREADS) void k_symm_final(SymmArgs a) {
 #pragma clang fp contract(off)
   const int64_t f = blockIdx.x;
@@ -252,18 +451,19 @@ __global__ __launch_bounds__(SYM_THREADS) void k_symm_final(SymmArgs a) {
 #pragma unroll
   for (int q = 0; q < 18; ++q) v[q] = 0.0;
   const int t0 = target(0);
-  const double s0 = a.fit ? scoord(a.fs, f, t0, 0) : 0.0;
-  const double s1 = a.fit ? scoord(a.fs, f, t0, 1) : 0.0;
-  const double s2 = a.fit ? scoord(a.fs, f, t0, 2) : 0.0;
+  const double s0 = FIT ? scoord(a.fs, f, t0, 0) : 0.0;
+  const double s1 = FIT ? scoord(a.fs, f, t0, 1) : 0.0;
+  const double s2 = FIT ? scoord(a.fs, f, t0, 2) : 0.0;
+#pragma unroll 4
   for (int i = lane; i < n; i += SYM_THREADS) {
     const int t = target(i);
     if (a.amap) a.amap[f * n + i] = t;
-    const double w = a.mass ? a.mass[t] : 1.0;
+    const double w = WEIGHTED ? a.mass[t] : 1.0;
     const double y0 = a.ref[i], y1 = a.ref[a.ldr + i], y2 = a.ref[2 * a.ldr + i];
     const double x0 = scoord(a.fs, f, t, 0), x1 = scoord(a.fs, f, t, 1),
                  x2 = scoord(a.fs, f, t, 2);
     v[17] += w;
-    if (a.fit) {
+    if (FIT) {
       const double d0 = x0 - s0, d1 = x1 - s1, d2 = x2 - s2;
       const double w0 = w * d0, w1 = w * d1, w2 = w * d2;
       v[0] += w0;
@@ -291,7 +491,7 @@ __global__ __launch_bounds__(SYM_THREADS) void k_symm_final(SymmArgs a) {
   double r;
   if (W < RMSD_SMALL) {
     r = -1.0;
-  } else if (!a.fit) {
+  } else if (!FIT) {
     r = v[12] < 0.0 ? 0.0 : sqrt(v[12] / W);
   } else {
     const double c0 = v[0] / W, c1 = v[1] / W, c2 = v[2] / W;
@@ -331,6 +531,14 @@ hipError_t rmsd_launch_symm(const SymmArgs& a, hipStream_t s) {
     if (blocks > 0x7fffffff) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_symm_assign, dim3((unsigned)blocks), dim3(SYM_THREADS), 0, s, a);
   }
-  hipLaunchKernelGGL(k_symm_final, dim3((unsigned)F), dim3(SYM_THREADS), 0, s, a);
+  const dim3 grid((unsigned)F), block(SYM_THREADS);
+  if (a.fit && a.mass)
+    hipLaunchKernelGGL((k_symm_final<true, true>), grid, block, 0, s, a);
+  else if (a.fit)
+    hipLaunchKernelGGL((k_symm_final<true, false>), grid, block, 0, s, a);
+  else if (a.mass)
+    hipLaunchKernelGGL((k_symm_final<false, true>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_symm_final<false, false>), grid, block, 0, s, a);
   return hipGetLastError();
 }

@@ -270,13 +270,13 @@ struct SymmArgs {
   const int* pos;         // nsel
   const int* bigoff;      // per group: offset (doubles) of its working matrix
   int64_t bigstride;      // doubles per frame of big-group scratch
-  double* big;            // groups over SYMM_GROUP_LDS atoms
+  double* big;            // groups over SYMM_GROUP_REG atoms
   int* gmap;              // F x ngatoms: target atom for each group entry
   double* rmsd;
   int32_t* amap;          // F x nsel or null
   double* U;              // 9 per frame or null
   double* tvec;           // 6 per frame or null
 };
-constexpr int SYMM_GROUP_LDS = 8;    // largest group kept in LDS
+constexpr int SYMM_GROUP_REG = 4;    // largest group kept in registers
 constexpr int SYMM_GROUP_MAX = 32;   // largest group (bit-mask lines, byte indices)
 hipError_t rmsd_launch_symm(const SymmArgs& a, hipStream_t s);

@@ -54,6 +54,7 @@ def load_host():
                                      C.POINTER(DP), I64]
     L.chost_write_cmatrix.argtypes = [C.c_char_p, FP, I64, I64, I, IP]
     L.chost_set_residues.argtypes = [P, C.c_char_p, I, IP, C.POINTER(C.c_char_p), I, IP]
+    L.chost_set_symmetric_groups.argtypes = [P, C.c_char_p, I, IP, IP]
     L.chost_write_grace.argtypes = [C.c_char_p, I, C.POINTER(C.c_char_p), C.POINTER(DP), I64, I]
     L.chost_write_mesh.argtypes = [C.c_char_p, C.c_char_p, DP, I64, I, C.POINTER(C.c_char_p),
                                    C.POINTER(DP)]
@@ -62,7 +63,7 @@ def load_host():
         getattr(L, f).restype = I
     for f in ("chost_create", "chost_add_coords", "chost_add_reference", "chost_add_mask",
               "chost_run", "chost_get_matrix", "chost_get_series", "chost_get_coords",
-              "chost_read_cmatrix", "chost_set_residues"):
+              "chost_read_cmatrix", "chost_set_residues", "chost_set_symmetric_groups"):
         getattr(L, f).restype = I
     _hl = L
     return L
@@ -125,6 +126,20 @@ class HostSession:
                                                f.ctypes.data_as(IP), nm,
                                                0 if sol is None else len(sol),
                                                None if sol is None else sol.ctypes.data_as(IP)))
+
+    def set_symmetric_groups(self, name, groups):
+        """Symmetric atom groups of a set's topology (lists of 0-based atom
+        indices, residue by residue)."""
+        off = np.zeros(len(groups) + 1, np.int32)
+        for k, g in enumerate(groups):
+            off[k + 1] = off[k] + len(g)
+        flat = np.ascontiguousarray(
+            np.concatenate([np.asarray(g, np.int32) for g in groups]) if len(groups)
+            else np.zeros(1, np.int32), dtype=np.int32)
+        IP = C.POINTER(C.c_int)
+        self._check(self._L.chost_set_symmetric_groups(self._h, name.encode(), len(groups),
+                                                       off.ctypes.data_as(IP),
+                                                       flat.ctypes.data_as(IP)))
 
     def add_mask(self, expr, sel, set_name=None):
         s = np.ascontiguousarray(sel, dtype=np.int32)

@@ -78,6 +78,10 @@ struct Topology {
   std::vector<int> resFirst;         // first atom of each residue (0-based)
   std::vector<std::string> resName;  // residue names, trailing blanks removed
   std::vector<int> soluteRes;        // Topology::SoluteResidues() (0-based); empty: all
+  // Groups of potentially symmetric atoms (atom indices), residue by residue,
+  // as AtomMap::SymmetricAtoms (src/AtomMap.cpp:374-439) finds them from the
+  // bonds and elements; given by the caller (the host layer reads no bonds).
+  std::vector<std::vector<int>> symmGroups;
   int Nres() const { return (int)resFirst.size(); }
   int ResEnd(int r) const { return r + 1 < Nres() ? resFirst[r + 1] : natom; }
 };
@@ -291,6 +295,35 @@ class Action_Rmsd {
   DataSetList* dsl_ = nullptr;
 };
 
+// Action_SymmetricRmsd (src/Action_SymmetricRmsd.cpp): 'symmrmsd [<name>]
+// <mask> [<refmask>] [out <file>] [nofit] [mass] [remap]' with the reference
+// modes of rms (first, previous, reftraj <set>, ref <name> / reference /
+// refindex <#>).  Symmetric groups: the target topology's symmGroups, kept for
+// residues up to the last selected one and reduced to selected atoms
+// (SymmetricRmsdCalc::SetupSymmRMSD, src/SymmetricRmsdCalc.cpp:28-119).  With
+// a fixed reference every frame goes to the library in one call; 'remap' and
+// the fit's coordinate update (Trans_Rot_Trans) are then applied to the
+// frames of the set.
+class Action_SymmetricRmsd {
+ public:
+  RetType Init(ArgList& args, DataSetList& dsl, MaskResolver resolve, GpuSession& gpu);
+  RetType DoFrames(CoordsSet& crd);
+  int WriteOutput() const;
+  const Series* Rmsd() const { return rmsd_; }
+
+ private:
+  RetType Run(CoordsSet& crd, const AtomMask& tsel, int f0, int nf, const float* refxyz,
+              const Topology& reftop, const std::string& refname);
+  bool fit_ = true, useMass_ = false, remap_ = false;
+  Action_Rmsd::RefModeType refMode_ = Action_Rmsd::FIRST;
+  std::string outfile_, tgtMaskExpr_, refMaskExpr_;
+  CoordsSet* refCrd_ = nullptr;
+  Series* rmsd_ = nullptr;
+  std::vector<int32_t> goff_, gatoms_;
+  MaskResolver resolve_;
+  GpuSession* gpu_ = nullptr;
+};
+
 // Analysis_RmsAvgCorr (src/Analysis_RmsAvgCorr.cpp): 'rmsavgcorr [crdset <set>]
 // [<name>] [<mask>] [out <file>] [output <file>] [mass] [stop <maxwindow>]
 // [offset <offset>] {ref <name> | reference | refindex <#> [<refmask>] | first}'.
@@ -352,7 +385,8 @@ int Read(const std::string& path, MatrixFlt* tri, size_t* actual_nrows, int* sie
 }  // namespace cmatrix
 
 // Dispatches one command line ("rms2d ...", "2drms ...", "rms ...",
-// "rmsd ...", "cluster ...") on a session's data.
+// "rmsd ...", "align ...", "symmrmsd ...", "rmsavgcorr ...", "cluster ...")
+// on a session's data.
 class Session {
  public:
   explicit Session(int device = 0) : gpu(device) {}

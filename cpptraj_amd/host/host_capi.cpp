@@ -93,6 +93,37 @@ int chost_set_residues(chost_session* s, const char* set, int nres, const int* f
   return 0;
 }
 
+int chost_set_symmetric_groups(chost_session* s, const char* set, int ngroups, const int* off,
+                               const int* atoms) {
+  if (!s || !set || ngroups < 0 || (ngroups > 0 && (!off || !atoms))) return 1;
+  CoordsSet* c = s->s.dsl.FindCoordsSet(set);
+  if (!c) c = s->s.dsl.FindReference(set);
+  if (!c) {
+    s->err = std::string("chost_set_symmetric_groups: no set '") + set + "'";
+    return 1;
+  }
+  if (ngroups > 0 && off[0] != 0) {
+    s->err = "chost_set_symmetric_groups: off[0] must be 0";
+    return 1;
+  }
+  std::vector<std::vector<int>> g((size_t)ngroups);
+  for (int k = 0; k < ngroups; ++k) {
+    if (off[k + 1] < off[k]) {
+      s->err = "chost_set_symmetric_groups: decreasing offsets";
+      return 1;
+    }
+    for (int j = off[k]; j < off[k + 1]; ++j) {
+      if (atoms[j] < 0 || atoms[j] >= c->top.natom) {
+        s->err = "chost_set_symmetric_groups: atom out of range";
+        return 1;
+      }
+      g[k].push_back(atoms[j]);
+    }
+  }
+  c->top.symmGroups = std::move(g);
+  return 0;
+}
+
 int chost_add_mask(chost_session* s, const char* set, const char* expr, const int* sel, int nsel) {
   if (!s || !expr || nsel < 0 || (nsel > 0 && !sel)) return 1;
   const std::string key = (set && *set) ? std::string(set) + "|" + expr : std::string(expr);

@@ -4,7 +4,8 @@
  * Python tests).  A session holds named COORDS sets, reference frames, a mask
  * table (expression -> resolved selection) and the data sets that commands
  * produce; chost_run executes one command line of the reference's syntax
- * ("rms2d ...", "2drms ...", "rms ...", "rmsd ...", "align ...", "cluster ...").
+ * ("rms2d ...", "2drms ...", "rms ...", "rmsd ...", "align ...", "symmrmsd ...",
+ * "rmsavgcorr ...", "cluster ...").
  */
 #ifndef CPPTRAJ_RMSD_HOST_CAPI_H
 #define CPPTRAJ_RMSD_HOST_CAPI_H
@@ -32,6 +33,11 @@ int chost_add_reference(chost_session* s, const char* name, const float* xyz, in
  * every residue) that Topology::SoluteResidues would list. */
 int chost_set_residues(chost_session* s, const char* set, int nres, const int* first,
                        const char* const* names, int nsolute, const int* solute);
+/* Symmetric atom groups of a set's topology: group k is atoms[off[k] ..
+ * off[k+1]) (0-based atom indices), as AtomMap::SymmetricAtoms finds them
+ * residue by residue ('symmrmsd' reduces them to its selection). */
+int chost_set_symmetric_groups(chost_session* s, const char* set, int ngroups, const int* off,
+                               const int* atoms);
 /* Mask expression -> selected atom indices, for the set (COORDS or reference)
  * named `set`, or for every set when `set` is NULL or empty. */
 int chost_add_mask(chost_session* s, const char* set, const char* expr, const int* sel, int nsel);

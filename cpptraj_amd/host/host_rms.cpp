@@ -288,19 +288,22 @@ RetType Action_Rmsd::Init(ArgList& args, DataSetList& dsl, MaskResolver resolve,
 
 // ReferenceAction::InitRef (src/ReferenceAction.cpp:72-130): first (default),
 // previous, reftraj <COORDS set>, ref <name> / reference.
-RetType Action_Rmsd::InitRef(ArgList& args, DataSetList& dsl, GpuSession& gpu) {
-  refMode_ = FIRST;
+// ReferenceAction::InitRef's modes (src/ReferenceAction.cpp:73-130): first
+// (default), previous, reftraj <set>, ref <name> / reference, refindex <#>.
+static RetType init_reference(ArgList& args, DataSetList& dsl, GpuSession& gpu,
+                              Action_Rmsd::RefModeType* mode, CoordsSet** refCrd) {
+  *mode = Action_Rmsd::FIRST;
   if (args.hasKey("previous")) {
-    refMode_ = PREVIOUS;
+    *mode = Action_Rmsd::PREVIOUS;
   } else if (!args.hasKey("first")) {
     if (args.Contains("reftraj")) {
       const std::string rt = args.GetStringKey("reftraj");
-      refCrd_ = dsl.FindCoordsSet(rt);
-      if (!refCrd_) {
+      *refCrd = dsl.FindCoordsSet(rt);
+      if (!*refCrd) {
         gpu.SetError("reference trajectory '" + rt + "' is not a loaded COORDS set");
         return ERR;
       }
-      refMode_ = TRAJ;
+      *mode = Action_Rmsd::TRAJ;
     } else {
       std::string rn;
       bool want = false;
@@ -311,26 +314,30 @@ RetType Action_Rmsd::InitRef(ArgList& args, DataSetList& dsl, GpuSession& gpu) {
         want = true;
       }
       if (want) {
-        refCrd_ = dsl.FindReference(rn);
-        if (!refCrd_) {
+        *refCrd = dsl.FindReference(rn);
+        if (!*refCrd) {
           gpu.SetError("reference '" + rn + "' not found");
           return ERR;
         }
-        refMode_ = FRAME;
+        *mode = Action_Rmsd::FRAME;
       } else if (args.Contains("refindex")) {
         // 'refindex <#>': the reference frame at that position
         // (DataSetList::GetReferenceSet, src/DataSetList.cpp:1011-1014).
         const int idx = args.getKeyInt("refindex", -1);
-        refCrd_ = dsl.ReferenceAt(idx);
-        if (!refCrd_) {
+        *refCrd = dsl.ReferenceAt(idx);
+        if (!*refCrd) {
           gpu.SetError("no reference at index " + std::to_string(idx));
           return ERR;
         }
-        refMode_ = FRAME;
+        *mode = Action_Rmsd::FRAME;
       }
     }
   }
   return OK;
+}
+
+RetType Action_Rmsd::InitRef(ArgList& args, DataSetList& dsl, GpuSession& gpu) {
+  return init_reference(args, dsl, gpu, &refMode_, &refCrd_);
 }
 
 // Action_Rmsd::perResSetup (src/Action_Rmsd.cpp:197-315).  Target residues:
@@ -712,6 +719,171 @@ int Action_Rmsd::WriteOutput() {
   return 0;
 }
 
+// ---- Action_SymmetricRmsd ------------------------------------------------------------
+// Init follows src/Action_SymmetricRmsd.cpp:17-58, DoAction :88-112.
+RetType Action_SymmetricRmsd::Init(ArgList& args, DataSetList& dsl, MaskResolver resolve,
+                                   GpuSession& gpu) {
+  gpu_ = &gpu;
+  resolve_ = resolve;
+  fit_ = !args.hasKey("nofit");
+  useMass_ = args.hasKey("mass");
+  outfile_ = args.GetStringKey("out");
+  remap_ = args.hasKey("remap");
+  if (init_reference(args, dsl, gpu, &refMode_, &refCrd_) != OK) return ERR;
+  tgtMaskExpr_ = args.GetMaskNext();
+  refMaskExpr_ = args.GetMaskNext();
+  if (refMaskExpr_.empty()) refMaskExpr_ = tgtMaskExpr_;
+  std::string name = args.GetStringNext();
+  if (name.empty()) name = dsl.GenerateDefaultName("RMSD");
+  rmsd_ = dsl.AddSeries(name);
+  rmsd_->width = 12;
+  rmsd_->prec = 4;
+  return OK;
+}
+
+RetType Action_SymmetricRmsd::DoFrames(CoordsSet& crd) {
+  GpuSession& g = *gpu_;
+  AtomMask tsel;
+  if (!resolve_or_all(resolve_, tgtMaskExpr_, crd, &tsel) || tsel.empty()) {
+    g.SetError("No atoms selected for [" + tgtMaskExpr_ + "]");
+    return ERR;
+  }
+  // SymmetricRmsdCalc::SetupSymmRMSD: groups of residues up to the last
+  // selected atom's residue, as selection indices; groups with no selected
+  // atom are dropped.
+  const Topology& top = crd.top;
+  std::vector<int> selIdx((size_t)top.natom, -1);
+  for (size_t k = 0; k < tsel.size(); ++k) selIdx[tsel[k]] = (int)k;
+  int lastRes = top.Nres();
+  if (top.Nres() > 0) {
+    const int lastAtom = *std::max_element(tsel.begin(), tsel.end());
+    lastRes = (int)(std::upper_bound(top.resFirst.begin(), top.resFirst.end(), lastAtom) -
+                    top.resFirst.begin());
+  }
+  goff_.assign(1, 0);
+  gatoms_.clear();
+  for (const std::vector<int>& grp : top.symmGroups) {
+    if (grp.empty()) continue;
+    if (top.Nres() > 0) {
+      const int r = (int)(std::upper_bound(top.resFirst.begin(), top.resFirst.end(), grp[0]) -
+                          top.resFirst.begin()) - 1;
+      if (r >= lastRes) continue;
+    }
+    size_t before = gatoms_.size();
+    for (int a : grp) {
+      if (a < 0 || a >= top.natom) {
+        g.SetError("symmetric group atom out of range of the topology");
+        return ERR;
+      }
+      if (selIdx[a] >= 0) gatoms_.push_back(selIdx[a]);
+    }
+    if (gatoms_.size() > before) goff_.push_back((int32_t)gatoms_.size());
+  }
+  const int nf = crd.nframes();
+  if (nf == 0) return OK;
+  switch (refMode_) {
+    case Action_Rmsd::FIRST: {
+      std::vector<float> first(crd.frame(0), crd.frame(0) + crd.stride);
+      return Run(crd, tsel, 0, nf, first.data(), crd.top, crd.name);
+    }
+    case Action_Rmsd::FRAME:
+      return Run(crd, tsel, 0, nf, refCrd_->frame(0), refCrd_->top, refCrd_->name);
+    case Action_Rmsd::TRAJ:
+      if (refCrd_->nframes() < nf) {
+        g.SetError("reference trajectory has fewer frames than the target");
+        return ERR;
+      }
+      for (int f = 0; f < nf; ++f)
+        if (Run(crd, tsel, f, 1, refCrd_->frame(f), refCrd_->top, refCrd_->name) != OK)
+          return ERR;
+      return OK;
+    case Action_Rmsd::PREVIOUS: {
+      std::vector<float> prev(crd.frame(0), crd.frame(0) + crd.stride);
+      for (int f = 0; f < nf; ++f) {
+        if (Run(crd, tsel, f, 1, prev.data(), crd.top, crd.name) != OK) return ERR;
+        std::copy(crd.frame(f), crd.frame(f) + crd.stride, prev.begin());
+      }
+      return OK;
+    }
+  }
+  return ERR;
+}
+
+RetType Action_SymmetricRmsd::Run(CoordsSet& crd, const AtomMask& tsel, int f0, int nf,
+                                  const float* refxyz, const Topology& reftop,
+                                  const std::string& refname) {
+  GpuSession& g = *gpu_;
+  AtomMask rsel;
+  CoordsSet refset;
+  refset.name = refname;
+  refset.top = reftop;
+  refset.stride = 3 * (int64_t)reftop.natom;
+  refset.xyz.assign(refxyz, refxyz + refset.stride);
+  if (!resolve_or_all(resolve_, refMaskExpr_, refset, &rsel) || rsel.empty()) {
+    g.SetError("No reference atoms selected for [" + refMaskExpr_ + "]");
+    return ERR;
+  }
+  if (rsel.size() != tsel.size()) {
+    g.SetError("Number of atoms in target mask does not equal number of atoms in reference mask");
+    return ERR;
+  }
+  rmsd_frames R, T;
+  if (g.FramesOnDevice("symm.ref", refset, rsel, &R)) return ERR;
+  if (g.FramesOnDevice("symm.tgt", crd, tsel, &T, crd.frame(f0), nf)) return ERR;
+  const int n = (int)tsel.size();
+  double* dr = (double*)g.Buffer("symm.rmsd", sizeof(double) * nf);
+  int32_t* dm = remap_ ? (int32_t*)g.Buffer("symm.amap", sizeof(int32_t) * (size_t)nf * n) : nullptr;
+  double* dU = fit_ ? (double*)g.Buffer("symm.U", sizeof(double) * 9 * nf) : nullptr;
+  double* dT = fit_ ? (double*)g.Buffer("symm.T", sizeof(double) * 6 * nf) : nullptr;
+  if (!dr || (remap_ && !dm) || (fit_ && (!dU || !dT))) return ERR;
+  const int ng = (int)goff_.size() - 1;
+  if (g.Check(rmsd_symm(g.ctx(), &T, &R, 0, ng, goff_.data(), gatoms_.empty() ? nullptr
+                                                                                : gatoms_.data(),
+                        fit_ ? 1 : 0, useMass_ ? 1 : 0, dr, dm, dU, dT),
+              "rmsd_symm"))
+    return ERR;
+  std::vector<double> hr(nf), hU(fit_ ? 9 * (size_t)nf : 0), hT(fit_ ? 6 * (size_t)nf : 0);
+  std::vector<int32_t> hm(remap_ ? (size_t)nf * n : 0);
+  if (g.Download(hr.data(), dr, sizeof(double) * nf) ||
+      (remap_ && g.Download(hm.data(), dm, sizeof(int32_t) * hm.size())) ||
+      (fit_ && (g.Download(hU.data(), dU, sizeof(double) * hU.size()) ||
+                g.Download(hT.data(), dT, sizeof(double) * hT.size()))))
+    return ERR;
+  rmsd_->v.insert(rmsd_->v.end(), hr.begin(), hr.end());
+  if (!remap_ && !fit_) return OK;
+  // The frame as the action leaves it: remapped (targetMap over all atoms,
+  // Frame::SetCoordinatesByMap), then Trans_Rot_Trans(Trans, U, refTrans)
+  // in FP64 (src/Action_SymmetricRmsd.cpp:96-108).
+  const int natom = crd.top.natom;
+  std::vector<double> x(3 * (size_t)natom);
+  std::vector<int> tmap((size_t)natom);
+  for (int f = 0; f < nf; ++f) {
+    float* fr = crd.frame(f0 + f);
+    for (int a = 0; a < natom; ++a) tmap[a] = a;
+    if (remap_)
+      for (int i = 0; i < n; ++i) tmap[tsel[i]] = tsel[hm[(size_t)f * n + i]];
+    for (int a = 0; a < natom; ++a)
+      for (int c = 0; c < 3; ++c) x[3 * a + c] = (double)fr[3 * tmap[a] + c];
+    if (fit_) {
+      const double* u = &hU[9 * (size_t)f];
+      const double* t = &hT[6 * (size_t)f];
+      for (int a = 0; a < natom; ++a) {
+        const double px = x[3 * a] + t[0], py = x[3 * a + 1] + t[1], pz = x[3 * a + 2] + t[2];
+        x[3 * a] = (px * u[0] + py * u[1] + pz * u[2]) + t[3];
+        x[3 * a + 1] = (px * u[3] + py * u[4] + pz * u[5]) + t[4];
+        x[3 * a + 2] = (px * u[6] + py * u[7] + pz * u[8]) + t[5];
+      }
+    }
+    for (size_t k = 0; k < x.size(); ++k) fr[k] = (float)x[k];
+  }
+  return OK;
+}
+
+int Action_SymmetricRmsd::WriteOutput() const {
+  if (!outfile_.empty() && dataio::WriteSeries(outfile_, {rmsd_})) return 1;
+  return 0;
+}
+
 // ---- Analysis_RmsAvgCorr ---------------------------------------------------------------
 // Setup follows src/Analysis_RmsAvgCorr.cpp:33-116.
 RetType Analysis_RmsAvgCorr::Setup(ArgList& args, DataSetList& dsl, MaskResolver resolve,
@@ -966,6 +1138,16 @@ int Session::Run(const std::string& line) {
       CoordsSet* crd = dsl.FindCoordsSet(crdset);
       if (!crd) gpu.SetError("align: no COORDS set");
       else if (a.DoFrames(*crd) == OK) rc = 0;
+    }
+  } else if (cmd == "symmrmsd") {
+    Action_SymmetricRmsd a;
+    const std::string crdset = args.GetStringKey("crdset");
+    rc = 1;
+    if (a.Init(args, dsl, resolve, gpu) == OK) {
+      CoordsSet* crd = dsl.FindCoordsSet(crdset);
+      if (!crd) gpu.SetError("symmrmsd: no COORDS set");
+      else if (a.DoFrames(*crd) == OK && a.WriteOutput() == 0) rc = 0;
+      else if (gpu.error().empty()) gpu.SetError("symmrmsd: cannot write output");
     }
   } else if (cmd == "rmsavgcorr") {
     Analysis_RmsAvgCorr a;

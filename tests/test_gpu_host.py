@@ -325,3 +325,61 @@ def test_rmsavgcorr_stop_mass_and_output_file(hs, golden, oracle, tmp_path):
     lines = text(sep).splitlines()
     assert [int(l.split()[0]) for l in lines] == list(w)
     assert all(abs(float(l.split()[1]) - v) <= 5e-7 for l, v in zip(lines, ct))
+
+
+# Test_SymmRmsd through the host layer: 'symmrmsd' with the topology's
+# symmetric groups (fixture data), reference modes and 'remap'.
+def _topology_groups(d, sel=None):
+    off, flat = d["goff"], d["gatoms"]
+    g = [flat[off[k]:off[k + 1]] for k in range(len(off) - 1)]
+    return [(np.asarray(sel)[x] if sel is not None else x).tolist() for x in g]
+
+
+def test_symmrmsd_reference_and_output(hs, golden, tmp_path):
+    d = golden("symm_afv")
+    n = len(d["mass"])
+    hs.add_coords("afv", d["rotated"].reshape(3, -1).astype(np.float32), n, d["mass"])
+    hs.add_reference("AFV.rst7", d["ref"].reshape(-1).astype(np.float32), n, d["mass"])
+    hs.set_symmetric_groups("afv", _topology_groups(d, d["sel"]))
+    hs.add_mask("!@H=", d["sel"])
+    out = str(tmp_path / "rmsd.dat")
+    hs.run(f"symmrmsd crdset afv Symm !@H= reference out {out}")
+    assert np.abs(hs.series("Symm") - d["expect_rotated"][:, 1]).max() <= 0.00005 + 1e-7
+    lines = [l.split() for l in text(out).splitlines()]
+    assert lines[0] == ["#Frame", "Symm"]
+    assert [l[1] for l in lines[1:]] == ["%.4f" % v for v in d["expect_rotated"][:, 1]]
+
+
+@pytest.mark.parametrize("res", ["asp", "tyr"])
+def test_symmrmsd_first_remap_coordinates(hs, golden, res):
+    d = golden(f"symm_{res}")
+    fr = d["frames"]
+    F, n = fr.shape[:2]
+    hs.add_coords(res, fr.reshape(F, -1), n, d["mass"])
+    hs.set_symmetric_groups(res, _topology_groups(d))
+    hs.run(f"symmrmsd crdset {res} first SYMM remap")
+    assert np.abs(hs.series("SYMM") - d["expect"][:, 1]).max() <= 0.00005 + 1e-7
+    got = hs.coords(res).reshape(F, n, 3)
+    assert np.abs(got - d["remap"]).max() <= 0.0005 + 1e-6
+
+
+def test_symmrmsd_previous_and_nofit(hs, golden, oracle):
+    # 'previous': each frame against the previous one as the action left it;
+    # 'nofit' leaves the frames untouched.
+    d = golden("symm_afv")
+    traj = d["traj"][:20]
+    F, n = traj.shape[:2]
+    hs.add_coords("t", traj.reshape(F, -1), n, d["mass"])
+    hs.set_symmetric_groups("t", _topology_groups(d, d["sel"]))
+    hs.add_mask("!@H=", d["sel"])
+    hs.run("symmrmsd crdset t P !@H= nofit previous")
+    sel = d["sel"]
+    groups = [g for g in _topology_groups(d)]
+    exp = []
+    for f in range(F):
+        prev = traj[max(f - 1, 0)].astype(np.float64)[sel]
+        r, _, _, _ = oracle.symm(traj[f:f + 1].reshape(1, -1), sel, None, prev, False, False,
+                                 groups)
+        exp.append(r[0])
+    assert np.abs(hs.series("P") - np.array(exp)).max() <= 1e-12
+    assert np.array_equal(hs.coords("t").reshape(F, n, 3), traj)
