@@ -346,7 +346,7 @@ def symm_bench(ctx, F=50000, n=2000, ngroups=250, steps=3, warmup=1):
            "max_err_vs_oracle_256_frames": err, "maps_equal_256_frames": maps_equal}
     from oracle.oracle import RefLib, ref_available
     if ref_available():
-        ns = 400
+        ns = 5000
         Xc = Xd[:ns].cpu().numpy()
         t0 = time.perf_counter()
         RefLib().symm(Xc, None, m, rc, False, True, groups)

@@ -909,9 +909,11 @@ int rmsd_symm(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* ref, 
                                     fit ? 1 : 0, refbuf, ref_trans, consts, ctx->stream));
   double* U0 = nullptr;
   double* t0 = nullptr;
+  double* moments = nullptr;
   if (fit) {
-    // Initial fit of the frames as given: U and Trans for the cost matrices.
-    double* moments = (double*)scratch(ctx, "symm.moments", sizeof(double) * 16 * nf, &st);
+    // Initial fit of the frames as given: U and Trans for the cost matrices,
+    // moments for the remapped fit.
+    moments = (double*)scratch(ctx, "symm.moments", sizeof(double) * 16 * nf, &st);
     if (!moments) return st;
     double* fit0 = (double*)scratch(ctx, "symm.fit0", sizeof(double) * 16 * nf, &st);
     if (!fit0) return st;
@@ -947,6 +949,8 @@ int rmsd_symm(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* ref, 
   a.fit = fit ? 1 : 0;
   a.U0 = U0;
   a.t0 = t0;
+  a.moments = moments;
+  a.consts = consts;
   a.ngroups = ngroups;
   a.goff = dblob;
   a.gatoms = dblob + (ngroups + 1);

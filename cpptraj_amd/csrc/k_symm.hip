@@ -8,10 +8,12 @@
 //                  reference's Hungarian heuristic on them
 //                  (Hungarian::Optimize, src/Hungarian.cpp:21-337); the
 //                  assignment goes to the frame's map.
-//   k_symm_final   the rest of SymmRMSD_CenteredRef (:178-192): the target
+//   k_symm_final_fit / k_symm_final_nofit
+//                  the rest of SymmRMSD_CenteredRef (:178-192): the target
 //                  remapped (coordinates and masses, Frame::
 //                  SetCoordinatesByMap) and fitted again (FP64 moments, QCP,
 //                  rotation) or compared without fit.
+//   k_symm_amap    the per-frame map of every selected atom, when asked for.
 //
 // The initial fit is the frame-to-reference fit (k_fit.hip) run first.  One
 // lane per (frame, group) for the assignment: groups are small (2-3 atoms in
@@ -19,8 +21,9 @@
 // wave take one group in 64 frames (one size per wave).  Groups of up to
 // SYMM_GROUP_REG atoms run a compile-time-size form with every array in
 // registers (no LDS: occupancy is set by registers alone); larger groups keep
-// their arrays in a slab of global scratch per (frame, group).  The final pass is one wavefront per frame, lanes over
-// atoms, one wave reduction of 18 sums.
+// their arrays in a slab of global scratch per (frame, group).  The final pass is one wavefront per frame: with a fit it
+// corrects the initial fit's moments for the atoms whose pairing changed;
+// without, it sums over every atom.
 #include <hip/hip_runtime.h>
 #include <cfloat>
 #include <stdint.h>
@@ -436,88 +439,131 @@ __global__ __launch_bounds__(SYM_THREADS) void k_symm_assign(SymmArgs a) {
   }
 }
 
-template <bool FIT, bool WEIGHTED>
-__global__ __launch_bounds__(SYM_THREADS) void k_symm_final(SymmArgs a) {
+// The remapped fit.  A map permutes atoms inside groups, so the remapped
+// frame's moments are the initial fit's (k_fit.hip: 13 per frame against the
+// same reference and shift) plus, for each atom i whose target t(i) != i, the
+// difference of its terms: w_t (x_t - s) - w_i (x_i - s) in the sums of
+// d, d y_i^T and |d|^2, (w_t - w_i) y_i and (w_t - w_i) |y_i|^2 in the
+// reference sums and w_t - w_i in the total weight.  Every term is corrected,
+// so the result is the remapped frame's for any map.  Only the frame's group
+// entries and the atoms whose pairing changed are read.
+template <bool WEIGHTED>
+__global__ __launch_bounds__(SYM_THREADS) void k_symm_final_fit(SymmArgs a) {
 #pragma clang fp contract(off)
   const int64_t f = blockIdx.x;
+  const int64_t F = a.fs.nframes;
   const int lane = threadIdx.x;
-  const int n = a.fs.nsel;
   const int* gm = a.gmap + f * a.ngatoms;
-  auto target = [&](int i) {
-    const int p = a.pos[i];
-    return p >= 0 ? gm[p] : i;
-  };
+  const double* sh = a.moments + 13 * F + 3 * f;
+  const double s0 = sh[0], s1 = sh[1], s2 = sh[2];
   double v[18];
 #pragma unroll
   for (int q = 0; q < 18; ++q) v[q] = 0.0;
-  const int t0 = target(0);
-  const double s0 = FIT ? scoord(a.fs, f, t0, 0) : 0.0;
-  const double s1 = FIT ? scoord(a.fs, f, t0, 1) : 0.0;
-  const double s2 = FIT ? scoord(a.fs, f, t0, 2) : 0.0;
-#pragma unroll 4
-  for (int i = lane; i < n; i += SYM_THREADS) {
-    const int t = target(i);
-    if (a.amap) a.amap[f * n + i] = t;
-    const double w = WEIGHTED ? a.mass[t] : 1.0;
+  for (int p = lane; p < a.ngatoms; p += SYM_THREADS) {
+    const int i = a.gatoms[p];
+    const int t = gm[p];
+    if (t == i) continue;
+    const double wt = WEIGHTED ? a.mass[t] : 1.0;
+    const double wi = WEIGHTED ? a.mass[i] : 1.0;
     const double y0 = a.ref[i], y1 = a.ref[a.ldr + i], y2 = a.ref[2 * a.ldr + i];
-    const double x0 = scoord(a.fs, f, t, 0), x1 = scoord(a.fs, f, t, 1),
-                 x2 = scoord(a.fs, f, t, 2);
-    v[17] += w;
-    if (FIT) {
-      const double d0 = x0 - s0, d1 = x1 - s1, d2 = x2 - s2;
-      const double w0 = w * d0, w1 = w * d1, w2 = w * d2;
-      v[0] += w0;
-      v[1] += w1;
-      v[2] += w2;
-      v[3] += w0 * y0; v[4] += w0 * y1; v[5] += w0 * y2;
-      v[6] += w1 * y0; v[7] += w1 * y1; v[8] += w1 * y2;
-      v[9] += w2 * y0; v[10] += w2 * y1; v[11] += w2 * y2;
-      v[12] += w0 * d0 + w1 * d1 + w2 * d2;
-      v[13] += w * y0;
-      v[14] += w * y1;
-      v[15] += w * y2;
-      v[16] += w * (y0 * y0 + y1 * y1 + y2 * y2);
-    } else {
-      const double dx = y0 - x0, dy = y1 - x1, dz = y2 - x2;
-      v[12] += (w * (dx * dx + dy * dy + dz * dz));
+    const double t0 = scoord(a.fs, f, t, 0) - s0, t1 = scoord(a.fs, f, t, 1) - s1,
+                 t2 = scoord(a.fs, f, t, 2) - s2;
+    const double i0 = scoord(a.fs, f, i, 0) - s0, i1 = scoord(a.fs, f, i, 1) - s1,
+                 i2 = scoord(a.fs, f, i, 2) - s2;
+    const double e0 = wt * t0 - wi * i0, e1 = wt * t1 - wi * i1, e2 = wt * t2 - wi * i2;
+    v[0] += e0;
+    v[1] += e1;
+    v[2] += e2;
+    v[3] += e0 * y0; v[4] += e0 * y1; v[5] += e0 * y2;
+    v[6] += e1 * y0; v[7] += e1 * y1; v[8] += e1 * y2;
+    v[9] += e2 * y0; v[10] += e2 * y1; v[11] += e2 * y2;
+    v[12] += (wt * t0 * t0 + wt * t1 * t1 + wt * t2 * t2) -
+             (wi * i0 * i0 + wi * i1 * i1 + wi * i2 * i2);
+    if (WEIGHTED) {
+      const double dw = wt - wi;
+      v[13] += dw * y0;
+      v[14] += dw * y1;
+      v[15] += dw * y2;
+      v[16] += dw * (y0 * y0 + y1 * y1 + y2 * y2);
+      v[17] += dw;
     }
   }
 #pragma unroll
   for (int q = 0; q < 18; ++q) v[q] = rmsd_dev::wave_sum(v[q]);
   if (lane != 0) return;
-  const double W = v[17];
+  const double* mo = a.moments + 13 * f;
+  const double* K = a.consts;
+  const double W = K[4] + v[17];
   double U[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-  double t1[3] = {0, 0, 0};
+  double tr[3] = {0, 0, 0};
   double r;
   if (W < RMSD_SMALL) {
     r = -1.0;
-  } else if (!FIT) {
-    r = v[12] < 0.0 ? 0.0 : sqrt(v[12] / W);
   } else {
-    const double c0 = v[0] / W, c1 = v[1] / W, c2 = v[2] / W;
-    const double K0 = v[13], K1 = v[14], K2 = v[15];
+    double m[13];
+#pragma unroll
+    for (int q = 0; q < 13; ++q) m[q] = mo[q] + v[q];
+    const double K0 = K[0] + v[13], K1 = K[1] + v[14], K2 = K[2] + v[15], K3 = K[3] + v[16];
+    const double c0 = m[0] / W, c1 = m[1] / W, c2 = m[2] / W;
     double R[9];
-    R[0] = v[3] - c0 * K0; R[1] = v[4] - c0 * K1; R[2] = v[5] - c0 * K2;
-    R[3] = v[6] - c1 * K0; R[4] = v[7] - c1 * K1; R[5] = v[8] - c1 * K2;
-    R[6] = v[9] - c2 * K0; R[7] = v[10] - c2 * K1; R[8] = v[11] - c2 * K2;
-    const double sx = v[12] - W * (c0 * c0 + c1 * c1 + c2 * c2);
-    const double E0 = 0.5 * (sx + v[16]);
+    R[0] = m[3] - c0 * K0; R[1] = m[4] - c0 * K1; R[2] = m[5] - c0 * K2;
+    R[3] = m[6] - c1 * K0; R[4] = m[7] - c1 * K1; R[5] = m[8] - c1 * K2;
+    R[6] = m[9] - c2 * K0; R[7] = m[10] - c2 * K1; R[8] = m[11] - c2 * K2;
+    const double sx = m[12] - W * (c0 * c0 + c1 * c1 + c2 * c2);
+    const double E0 = 0.5 * (sx + K3);
     const rmsd_dev::QcpCoef co = rmsd_dev::qcp_coefficients(R);
     const double lam = rmsd_dev::qcp_lambda(co, E0);
     r = rmsd_dev::rmsd_from_lambda(E0, lam, W, 1);
     rmsd_dev::qcp_rotation(R, lam, U);
-    t1[0] = -(c0 + s0);
-    t1[1] = -(c1 + s1);
-    t1[2] = -(c2 + s2);
+    tr[0] = -(c0 + s0);
+    tr[1] = -(c1 + s1);
+    tr[2] = -(c2 + s2);
   }
   a.rmsd[f] = r;
   if (a.U)
     for (int k = 0; k < 9; ++k) a.U[9 * f + k] = U[k];
   if (a.tvec)
     for (int k = 0; k < 3; ++k) {
-      a.tvec[6 * f + k] = t1[k];
+      a.tvec[6 * f + k] = tr[k];
       a.tvec[6 * f + 3 + k] = a.ref_trans[k];
     }
+}
+
+// Without fit: the remapped frame against the reference as stored,
+// RMSD_NoFit over every atom with the remapped masses.
+template <bool WEIGHTED>
+__global__ __launch_bounds__(SYM_THREADS) void k_symm_final_nofit(SymmArgs a) {
+#pragma clang fp contract(off)
+  const int64_t f = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int n = a.fs.nsel;
+  const int* gm = a.gmap + f * a.ngatoms;
+  double acc = 0.0, W = 0.0;
+#pragma unroll 4
+  for (int i = lane; i < n; i += SYM_THREADS) {
+    const int p = a.pos[i];
+    const int t = p >= 0 ? gm[p] : i;
+    const double w = WEIGHTED ? a.mass[t] : 1.0;
+    const double dx = a.ref[i] - scoord(a.fs, f, t, 0);
+    const double dy = a.ref[a.ldr + i] - scoord(a.fs, f, t, 1);
+    const double dz = a.ref[2 * a.ldr + i] - scoord(a.fs, f, t, 2);
+    W += w;
+    acc += (w * (dx * dx + dy * dy + dz * dz));
+  }
+  acc = rmsd_dev::wave_sum(acc);
+  W = rmsd_dev::wave_sum(W);
+  if (lane == 0) a.rmsd[f] = W < RMSD_SMALL ? -1.0 : (acc < 0.0 ? 0.0 : sqrt(acc / W));
+}
+
+// The per-frame map of every selected atom (SymmetricRmsdCalc::AMap).
+__global__ __launch_bounds__(256) void k_symm_amap(SymmArgs a) {
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int n = a.fs.nsel;
+  if (idx >= (int64_t)a.fs.nframes * n) return;
+  const int64_t f = idx / n;
+  const int i = (int)(idx - f * n);
+  const int p = a.pos[i];
+  a.amap[idx] = p >= 0 ? a.gmap[f * a.ngatoms + p] : i;
 }
 
 }  // namespace
@@ -533,12 +579,18 @@ hipError_t rmsd_launch_symm(const SymmArgs& a, hipStream_t s) {
   }
   const dim3 grid((unsigned)F), block(SYM_THREADS);
   if (a.fit && a.mass)
-    hipLaunchKernelGGL((k_symm_final<true, true>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_symm_final_fit<true>), grid, block, 0, s, a);
   else if (a.fit)
-    hipLaunchKernelGGL((k_symm_final<true, false>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_symm_final_fit<false>), grid, block, 0, s, a);
   else if (a.mass)
-    hipLaunchKernelGGL((k_symm_final<false, true>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_symm_final_nofit<true>), grid, block, 0, s, a);
   else
-    hipLaunchKernelGGL((k_symm_final<false, false>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_symm_final_nofit<false>), grid, block, 0, s, a);
+  if (a.amap) {
+    const int64_t total = F * a.fs.nsel;
+    const int64_t blocks = (total + 255) / 256;
+    if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_symm_amap, dim3((unsigned)blocks), dim3(256), 0, s, a);
+  }
   return hipGetLastError();
 }

@@ -263,6 +263,8 @@ struct SymmArgs {
   int fit;
   const double* U0;       // initial fit: U (9 per frame) and Trans (tvec layout,
   const double* t0;       // 6 per frame); null without fit
+  const double* moments;  // initial fit's moments (13 per frame, then 3 of shift)
+  const double* consts;   // reference constants of rmsd_launch_ref_prep
   int ngroups;
   const int* goff;
   const int* gatoms;
