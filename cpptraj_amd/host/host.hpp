@@ -350,8 +350,10 @@ class Analysis_RmsAvgCorr {
 // Cluster::Metric_RMS with the pairwise cache of Cluster::MetricArray:
 // 'cluster' arguments rms [<mask>] [mass] [nofit] [sieve <#>]
 // [pairdist <file>] [savepairdist]; builds and optionally saves the cache
-// (Cmatrix binary, src/Cluster/Cmatrix_Binary.cpp).  The clustering
-// algorithms that consume the cache are not part of this path.
+// (Cmatrix binary, src/Cluster/Cmatrix_Binary.cpp).  'srmsd [<mask>] [mass]'
+// caches Cluster::Metric_SRMSD's distances instead (symmetric groups from the
+// set's topology).  The clustering algorithms that consume the cache are not
+// part of this path.
 class Metric_RMS {
  public:
   RetType Setup(ArgList& args, DataSetList& dsl, MaskResolver resolve, GpuSession& gpu);
@@ -362,7 +364,8 @@ class Metric_RMS {
   const std::vector<int>& FramesToCache() const { return frames_; }
 
  private:
-  bool useMass_ = false, nofit_ = false, save_ = false;
+  bool useMass_ = false, nofit_ = false, save_ = false, symm_ = false;
+  std::vector<int32_t> goff_, gatoms_;  // symmetric groups (selection indices)
   int sieve_ = 1;
   int precision_ = RMSD_PREC_FP64;
   std::string maskExpr_, pairdist_;

@@ -76,9 +76,116 @@ bool parse_range(const std::string& arg, std::vector<int>* out, std::string* err
 
 }  // namespace
 
+// ---- symmetric groups and rows of symmetry-corrected RMSDs -----------------------------
+// SymmetricRmsdCalc::SetupSymmRMSD (src/SymmetricRmsdCalc.cpp:28-119): the
+// topology's groups for residues up to the last selected atom's residue, as
+// selection indices; groups with no selected atom are dropped.
+static bool symm_groups_for(const Topology& top, const AtomMask& tsel, std::vector<int32_t>* off,
+                            std::vector<int32_t>* atoms, std::string* err) {
+  std::vector<int> selIdx((size_t)top.natom, -1);
+  for (size_t k = 0; k < tsel.size(); ++k) selIdx[tsel[k]] = (int)k;
+  int lastRes = top.Nres();
+  if (top.Nres() > 0 && !tsel.empty()) {
+    const int lastAtom = *std::max_element(tsel.begin(), tsel.end());
+    lastRes = (int)(std::upper_bound(top.resFirst.begin(), top.resFirst.end(), lastAtom) -
+                    top.resFirst.begin());
+  }
+  off->assign(1, 0);
+  atoms->clear();
+  for (const std::vector<int>& grp : top.symmGroups) {
+    if (grp.empty()) continue;
+    if (top.Nres() > 0) {
+      const int r = (int)(std::upper_bound(top.resFirst.begin(), top.resFirst.end(), grp[0]) -
+                          top.resFirst.begin()) - 1;
+      if (r >= lastRes) continue;
+    }
+    const size_t before = atoms->size();
+    for (int a : grp) {
+      if (a < 0 || a >= top.natom) {
+        *err = "symmetric group atom out of range of the topology";
+        return false;
+      }
+      if (selIdx[a] >= 0) atoms->push_back(selIdx[a]);
+    }
+    if (atoms->size() > before) off->push_back((int32_t)atoms->size());
+  }
+  return true;
+}
+
+// A matrix of symmetry-corrected RMSDs, one rmsd_symm call per reference
+// frame (the SRMSD case of Analysis_Rms2d::Calculate_2D, src/Analysis_Rms2d.cpp:
+// 244-287, and of the cluster cache's symmetric metric).  tri: references are
+// the frames of T, each against the frames after it (packed triangle over T);
+// otherwise each frame of R against every frame of T (full, x = reference).
+// ref_is_column (triangle only): element (i, j), i < j, is frame i against
+// frame j as the reference (the cluster cache's FrameDist(f1, f2)) instead of
+// frame j against frame i (rms2d).  T may carry a frame list (fidx); a device
+// copy of it is indexed by row.
+static RetType symm_rows(GpuSession& g, const rmsd_frames& T, const rmsd_frames* R, bool tri,
+                         int use_mass, const std::vector<int32_t>& goff,
+                         const std::vector<int32_t>& gatoms, MatrixFlt* out,
+                         bool ref_is_column = false) {
+  const int nt = T.nframes;
+  const int nr = tri ? nt : R->nframes;
+  if (tri) out->AllocateTriangle(nt);
+  else out->Allocate2D(nr, nt);
+  if (nt == 0 || nr == 0) return OK;
+  // Frame list of T on the device (identity when T has none), so that each
+  // row's targets are a contiguous slice of it.
+  std::vector<int32_t> ids((size_t)nt);
+  for (int k = 0; k < nt; ++k) ids[k] = k;
+  int32_t* dids = (int32_t*)g.Buffer("symm.rows.ids", sizeof(int32_t) * nt);
+  double* drow = (double*)g.Buffer("symm.rows.out", sizeof(double) * nt);
+  if (!dids || !drow) return ERR;
+  if (T.fidx) {
+    if (hipMemcpy(dids, T.fidx, sizeof(int32_t) * nt, hipMemcpyDeviceToDevice) != hipSuccess) {
+      g.SetError("symm rows: frame list copy failed");
+      return ERR;
+    }
+  } else if (g.Upload(dids, ids.data(), sizeof(int32_t) * nt)) {
+    return ERR;
+  }
+  std::vector<int32_t> hostList((size_t)nt);
+  if (g.Download(hostList.data(), dids, sizeof(int32_t) * nt)) return ERR;
+  const int stored = *std::max_element(hostList.begin(), hostList.end()) + 1;
+  std::vector<double> row((size_t)nt);
+  const int ng = (int)goff.size() - 1;
+  const int32_t* ga = gatoms.empty() ? nullptr : gatoms.data();
+  float* o = out->data();
+  for (int r = 0; r < nr; ++r) {
+    rmsd_frames tg = T;
+    tg.fidx = dids;
+    rmsd_frames rf = tri ? T : *R;
+    int ref_frame = r;
+    if (tri) {
+      if (ref_is_column ? r == 0 : r == nt - 1) continue;
+      tg.fidx = ref_is_column ? dids : dids + r + 1;
+      tg.nframes = ref_is_column ? r : nt - r - 1;
+      rf.fidx = nullptr;  // the reference frame by its storage index
+      rf.nframes = stored;
+      ref_frame = hostList[r];
+    }
+    if (g.Check(rmsd_symm(g.ctx(), &tg, &rf, ref_frame, ng, goff.data(), ga, 1, use_mass, drow,
+                          nullptr, nullptr, nullptr),
+                "rmsd_symm"))
+      return ERR;
+    if (g.Download(row.data(), drow, sizeof(double) * tg.nframes)) return ERR;
+    if (tri && ref_is_column) {
+      for (int i = 0; i < r; ++i)  // element (i, r)
+        o[(size_t)nt * i - ((size_t)i * (i + 1)) / 2 + r - i - 1] = (float)row[i];
+    } else if (tri) {
+      const size_t base = (size_t)nt * r - ((size_t)r * (r + 1)) / 2;  // element (r, r+1)
+      for (int k = 0; k < tg.nframes; ++k) o[base + k] = (float)row[k];
+    } else {
+      for (int t = 0; t < nt; ++t) o[(size_t)t * nr + r] = (float)row[t];
+    }
+  }
+  return OK;
+}
+
 // ---- Analysis_Rms2d ------------------------------------------------------------
-// Setup follows src/Analysis_Rms2d.cpp:38-148; 'dme' and 'srmsd' are other
-// metrics and are rejected here.
+// Setup follows src/Analysis_Rms2d.cpp:38-148; 'dme' (another metric) is
+// rejected here; 'srmsd' uses the target topology's symmetric groups.
 RetType Analysis_Rms2d::Setup(ArgList& args, DataSetList& dsl, MaskResolver resolve,
                               GpuSession& gpu) {
   gpu_ = &gpu;
@@ -94,8 +201,8 @@ RetType Analysis_Rms2d::Setup(ArgList& args, DataSetList& dsl, MaskResolver reso
   else if (args.hasKey("srmsd")) mode_ = SRMSD;
   else if (args.hasKey("qrmsd")) mode_ = QUAT;
   else mode_ = RMS_FIT;
-  if (mode_ == DME || mode_ == SRMSD) {
-    gpu.SetError("rms2d: 'dme' and 'srmsd' are not on the GPU RMSD path");
+  if (mode_ == DME) {
+    gpu.SetError("rms2d: 'dme' is not on the GPU RMSD path");
     return ERR;
   }
   useMass_ = args.hasKey("mass");
@@ -168,7 +275,31 @@ RetType Analysis_Rms2d::Analyze() {
   rmsd_frames T, R;
   if (g.FramesOnDevice("rms2d.tgt", *TgtTraj_, tsel, &T)) return ERR;
   const int ntgt = TgtTraj_->nframes();
-  if (tri) {
+  if (mode_ == SRMSD) {
+    if (precision_ != RMSD_PREC_FP64) {
+      g.SetError("rms2d: 'srmsd' is FP64 only");
+      return ERR;
+    }
+    std::vector<int32_t> goff, gatoms;
+    std::string err;
+    if (!symm_groups_for(TgtTraj_->top, tsel, &goff, &gatoms, &err)) {
+      g.SetError("rms2d: " + err);
+      return ERR;
+    }
+    if (!tri && g.FramesOnDevice("rms2d.ref", *ref, rsel, &R)) return ERR;
+    if (symm_rows(g, T, tri ? nullptr : &R, tri, useMass_ ? 1 : 0, goff, gatoms,
+                  rmsdataset_) != OK)
+      return ERR;
+    if (tri && Ct_) {
+      float* d = (float*)g.Buffer("rms2d.out", sizeof(float) * std::max<size_t>(1, rmsdataset_->size()));
+      double* ct = (double*)g.Buffer("rms2d.corr", sizeof(double) * std::max(ntgt, 1));
+      if (!d || !ct || g.Upload(d, rmsdataset_->data(), sizeof(float) * rmsdataset_->size()))
+        return ERR;
+      if (g.Check(rmsd_matrix_autocorr(g.ctx(), d, ntgt, ct), "rms2d corr")) return ERR;
+      Ct_->v.resize(ntgt);
+      if (g.Download(Ct_->v.data(), ct, sizeof(double) * ntgt)) return ERR;
+    }
+  } else if (tri) {
     rmsdataset_->AllocateTriangle(ntgt);
     float* d = (float*)g.Buffer("rms2d.out", sizeof(float) * std::max<size_t>(1, rmsdataset_->size()));
     if (!d) return ERR;
@@ -1013,13 +1144,20 @@ RetType Metric_RMS::Setup(ArgList& args, DataSetList& dsl, MaskResolver resolve,
     gpu.SetError("cluster: no COORDS set");
     return ERR;
   }
-  if (args.hasKey("dme") || args.hasKey("srmsd") || args.hasKey("data")) {
-    gpu.SetError("cluster: only the 'rms' metric is on the GPU RMSD path");
+  if (args.hasKey("dme") || args.hasKey("data")) {
+    gpu.SetError("cluster: only the 'rms' and 'srmsd' metrics are on the GPU RMSD path");
     return ERR;
   }
+  symm_ = args.hasKey("srmsd");
   (void)args.hasKey("rms");
   useMass_ = args.hasKey("mass");
   nofit_ = args.hasKey("nofit");
+  if (symm_ && nofit_) {
+    // Metric_SRMSD without fit centres the reference frame but not the target
+    // (SymmetricRmsdCalc::SymmRMSD), a combination rmsd_symm does not offer.
+    gpu.SetError("cluster: 'srmsd' with 'nofit' is not on the GPU RMSD path");
+    return ERR;
+  }
   sieve_ = args.getKeyInt("sieve", 1);
   if (sieve_ < 1) {
     gpu.SetError("cluster: only regular sieves (sieve >= 1) are supported");
@@ -1040,6 +1178,13 @@ RetType Metric_RMS::Setup(ArgList& args, DataSetList& dsl, MaskResolver resolve,
   if (!resolve_or_all(resolve_, maskExpr_, *crd_, &sel_) || sel_.empty()) {
     gpu.SetError("cluster: no atoms selected for [" + maskExpr_ + "]");
     return ERR;
+  }
+  if (symm_) {
+    std::string err;
+    if (!symm_groups_for(crd_->top, sel_, &goff_, &gatoms_, &err)) {
+      gpu.SetError("cluster: " + err);
+      return ERR;
+    }
   }
   std::string name = args.GetStringNext();
   if (name.empty()) name = dsl.GenerateDefaultName("CpptrajPairDist");
@@ -1063,12 +1208,23 @@ RetType Metric_RMS::CacheDistances() {
   if (g.Upload(fidx, f32.data(), sizeof(int32_t) * n)) return ERR;
   F.fidx = fidx;
   F.nframes = n;
-  cache_->AllocateTriangle(n);
-  float* d = (float*)g.Buffer("cluster.out", sizeof(float) * std::max<size_t>(1, cache_->size()));
-  if (!d) return ERR;
-  if (g.Check(rmsd_cluster_cache(g.ctx(), &F, nofit_, useMass_, precision_, 1.0, d), "cluster"))
-    return ERR;
-  if (g.Download(cache_->data(), d, sizeof(float) * cache_->size())) return ERR;
+  if (symm_) {
+    // Metric_SRMSD::FrameDist(f1, f2): frame f1 against frame f2 as the
+    // reference (src/Cluster/Metric_SRMSD.cpp:34-38).
+    if (precision_ != RMSD_PREC_FP64) {
+      g.SetError("cluster: 'srmsd' is FP64 only");
+      return ERR;
+    }
+    if (symm_rows(g, F, nullptr, true, useMass_ ? 1 : 0, goff_, gatoms_, cache_, true) != OK)
+      return ERR;
+  } else {
+    cache_->AllocateTriangle(n);
+    float* d = (float*)g.Buffer("cluster.out", sizeof(float) * std::max<size_t>(1, cache_->size()));
+    if (!d) return ERR;
+    if (g.Check(rmsd_cluster_cache(g.ctx(), &F, nofit_, useMass_, precision_, 1.0, d), "cluster"))
+      return ERR;
+    if (g.Download(cache_->data(), d, sizeof(float) * cache_->size())) return ERR;
+  }
   if (save_ || !pairdist_.empty()) {
     const std::string path = pairdist_.empty() ? "CpptrajPairDist" : pairdist_;
     if (cmatrix::Write(path, *cache_, crd_->nframes(), sieve_, frames_)) {
@@ -1085,6 +1241,18 @@ double Metric_RMS::FrameDist(int f1, int f2) {
   if (g.FramesOnDevice("metric.a", *crd_, sel_, &A, crd_->frame(f1), 1) ||
       g.FramesOnDevice("metric.b", *crd_, sel_, &B, crd_->frame(f2), 1))
     return -1.0;
+  if (symm_) {
+    // Metric_SRMSD::FrameDist: f1 against f2 as the (centred) reference.
+    double* dr = (double*)g.Buffer("metric.symm", sizeof(double));
+    if (!dr || g.Check(rmsd_symm(g.ctx(), &A, &B, 0, (int)goff_.size() - 1, goff_.data(),
+                                 gatoms_.empty() ? nullptr : gatoms_.data(), 1, useMass_ ? 1 : 0,
+                                 dr, nullptr, nullptr, nullptr),
+                       "FrameDist"))
+      return -1.0;
+    double v = 0.0;
+    if (g.Download(&v, dr, sizeof(double))) return -1.0;
+    return v;
+  }
   float* d = (float*)g.Buffer("metric.out", sizeof(float));
   if (!d) return -1.0;
   // Full 1 x 1 block: row = f1 as the target, column = f2 as the reference

@@ -34,7 +34,8 @@ Outputs (numpy .npz, no pickles):
                    frames are those of tz2_camask.npz
   symm_afv.npz     Test_SymmRmsd, AFV heavy atoms: reference AFV.rst7, the
                    three rotated structures and AFV.nc (150 frames), the
-                   symmetric groups, masses, expected NoSymm / Symm RMSDs
+                   symmetric groups, masses, expected NoSymm / Symm RMSDs, and
+                   Test_Cluster_SymmRMSD's '2drms srmsd' matrix (2drms.gnu.save)
   symm_asp.npz, symm_glu.npz, symm_tyr.npz
                    Test_SymmRmsd per residue: all frames and atoms, groups,
                    masses, expected NOSYMM / SYMM RMSDs and the remapped,
@@ -434,12 +435,25 @@ def main():
     rot = np.stack([read_rst7(os.path.join(S, n)) for n in
                     ("AFV.rotate2.rst7", "AFV.rotate3.rst7", "AFV.rotate23.rst7")])
     afv_nc = read_nc(os.path.join(T, "AFV.nc"))
+    # Test_Cluster_SymmRMSD: '2drms !@H= out 2drms.gnu srmsd' over AFV.nc, the
+    # gnuplot grid (x, y 1-based, padded with a 151st row and column of 0).
+    nfr = afv_nc.shape[0]
+    gnu = np.full((nfr, nfr), np.nan)
+    with open(os.path.join(T, "Test_Cluster_SymmRMSD", "2drms.gnu.save")) as fh:
+        for line in fh:
+            tok = line.split()
+            if len(tok) == 3 and tok[0][0].isdigit():
+                x, y = int(float(tok[0])), int(float(tok[1]))
+                if x <= nfr and y <= nfr:
+                    gnu[x - 1, y - 1] = float(tok[2])
+    assert not np.isnan(gnu).any()
     np.savez_compressed(
         os.path.join(OUT, "symm_afv.npz"),
         sel=sel, goff=off, gatoms=flat, mass=pa.mass,
         ref=read_rst7(os.path.join(S, "AFV.rst7")), rotated=rot,
         expect_rotated=read_series(os.path.join(S, "rmsd.dat.save")),
-        traj=afv_nc, expect_traj=read_series(os.path.join(S, "afv.rmsd.dat.save"))[:, 0])
+        traj=afv_nc, expect_traj=read_series(os.path.join(S, "afv.rmsd.dat.save"))[:, 0],
+        expect_2drms=gnu)
     # ASP / GLU / TYR, all atoms, reference = first frame, with 'remap':
     # NOSYMM / SYMM columns and the remapped, fitted coordinates.
     for res in ("ASP", "GLU", "TYR"):

@@ -68,6 +68,18 @@ class OracleBackend:
         tv = np.concatenate([tr, np.tile(rt, (len(frames), 1))], axis=1)
         return rm, amap, U, tv
 
+    def symm_matrix(self, frames, groups, sel):
+        """rms2d srmsd: element (i, j), i < j, is frame j against frame i
+        (centred reference); returned as the symmetric square matrix."""
+        F = len(frames)
+        X = frames.reshape(F, -1)
+        M = np.zeros((F, F))
+        for i in range(F - 1):
+            r, _ = self.o.center_on_origin(frames[i].reshape(-1, 3).astype(np.float64)[sel])
+            rm, _, _, _ = self.o.symm(X[i + 1:], sel, None, r, False, True, groups)
+            M[i, i + 1:] = M[i + 1:, i] = rm.astype(np.float32)
+        return M
+
 
 class GpuBackend:
     def __init__(self, ctx):
@@ -139,6 +151,19 @@ class GpuBackend:
         return (rm.cpu().numpy(), amap.cpu().numpy(),
                 None if U is None else U.cpu().numpy().reshape(-1, 9),
                 None if tv is None else tv.cpu().numpy())
+
+    def symm_matrix(self, frames, groups, sel):
+        import cpptraj_amd as ca
+        t = self.t
+        F = len(frames)
+        X = self._dev(frames.reshape(F, -1), t.float32)
+        s = self._dev(sel, t.int32)
+        M = np.zeros((F, F))
+        for i in range(F - 1):
+            rm, _, _, _ = ca.rmsd_symm(X[i + 1:], groups, ref_frames=X, ref_index=i, mask=s,
+                                       ctx=self.ctx)
+            M[i, i + 1:] = M[i + 1:, i] = rm.cpu().numpy().astype(np.float32)
+        return M
 
 
 @pytest.fixture(params=["oracle", pytest.param("gpu", marks=pytest.mark.gpu)])
@@ -321,6 +346,13 @@ def test_symm_afv_trajectory(golden, backend):
     traj = d["traj"]
     rm, _, _, _ = backend.symm(traj, traj[0], _groups(d), sel=d["sel"])
     assert np.abs(rm - d["expect_traj"]).max() <= HALF4
+
+
+def test_symm_2drms_matrix(golden, backend):
+    # Test_Cluster_SymmRMSD: '2drms !@H= srmsd' over the 150 frames of AFV.nc.
+    d = golden("symm_afv")
+    M = backend.symm_matrix(d["traj"], _groups(d), d["sel"])
+    assert np.abs(M - d["expect_2drms"]).max() <= HALF3
 
 
 @pytest.mark.parametrize("res", ["asp", "glu", "tyr"])

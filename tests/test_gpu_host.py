@@ -383,3 +383,41 @@ def test_symmrmsd_previous_and_nofit(hs, golden, oracle):
         exp.append(r[0])
     assert np.abs(hs.series("P") - np.array(exp)).max() <= 1e-12
     assert np.array_equal(hs.coords("t").reshape(F, n, 3), traj)
+
+
+def test_rms2d_srmsd_matrix(hs, golden):
+    # Test_Cluster_SymmRMSD's '2drms !@H= out 2drms.gnu srmsd' (150 frames).
+    from conftest import tri_to_square
+    d = golden("symm_afv")
+    traj = d["traj"]
+    F, n = traj.shape[:2]
+    hs.add_coords("afv", traj.reshape(F, -1), n, d["mass"])
+    hs.set_symmetric_groups("afv", _topology_groups(d, d["sel"]))
+    hs.add_mask("!@H=", d["sel"])
+    hs.run("rms2d crdset afv S2 !@H= srmsd")
+    M = tri_to_square(hs.matrix("S2"), F)
+    assert np.abs(M - d["expect_2drms"]).max() <= 0.0005 + 1e-6
+
+
+def test_cluster_cache_srmsd_metric(hs, golden, oracle):
+    # 'cluster ... srmsd !@H= sieve 3': element (i, j) of the cache is
+    # Metric_SRMSD::FrameDist(f_i, f_j), frame f_i against f_j as the reference.
+    d = golden("symm_afv")
+    traj = d["traj"][:60]
+    F, n = traj.shape[:2]
+    hs.add_coords("afv", traj.reshape(F, -1), n, d["mass"])
+    hs.set_symmetric_groups("afv", _topology_groups(d, d["sel"]))
+    hs.add_mask("!@H=", d["sel"])
+    hs.run("cluster crdset afv C srmsd !@H= sieve 3")
+    got = hs.matrix("C")
+    frames = list(range(0, F, 3))
+    sel, groups = d["sel"], _topology_groups(d)
+    k = 0
+    for a in range(len(frames)):
+        for b in range(a + 1, len(frames)):
+            r, _ = oracle.center_on_origin(traj[frames[b]].astype(np.float64)[sel])
+            e, _, _, _ = oracle.symm(traj[frames[a]].reshape(1, -1), sel, None, r, False, True,
+                                     groups)
+            assert abs(got[k] - np.float32(e[0])) <= 1e-6, (a, b)
+            k += 1
+    assert k == len(got)
