@@ -256,6 +256,11 @@ def avgcorr_bench(ctx, F=10000, n=300, steps=3, warmup=1):
         fn()
         stages[method] = ctx.stage_ms()
         ctx.set_profiling(False)
+    # 'first': every window against its own first averaged frame.
+    for method in ("prefix", "running"):
+        def fn():
+            out["first_" + method] = ca.rms_avgcorr(Xd, method=method, ctx=ctx)
+        ms["first_" + method] = timed_steps(fn, steps if method == "prefix" else 1, warmup)
     ca.rms_avgcorr(Xd[:2], ref_frames=R, method="auto", ctx=ctx)  # back to the default
     _, ct, sd = out["prefix"]
     from oracle.oracle import Oracle
@@ -281,7 +286,11 @@ def avgcorr_bench(ctx, F=10000, n=300, steps=3, warmup=1):
            "running_ms": ms["running"],
            "running_averaged_frames_per_s": fits / (ms["running"] / 1e3),
            "max_err_vs_oracle_first_20_windows": err,
-           "max_diff_prefix_vs_running": err_methods}
+           "max_diff_prefix_vs_running": err_methods,
+           "first_prefix_ms": ms["first_prefix"], "first_running_ms": ms["first_running"],
+           "first_max_diff_prefix_vs_running": max(
+               float((out["first_prefix"][1] - out["first_running"][1]).abs().max()),
+               float((out["first_prefix"][2] - out["first_running"][2]).abs().max()))}
     # The reference's window loop (compiled Frame operations, serial) on the
     # first 64 window sizes.
     from oracle.oracle import RefLib, ref_available

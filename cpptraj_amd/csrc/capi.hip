@@ -703,18 +703,21 @@ int rmsd_avgcorr(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* re
   const int nw = rmsd_avgcorr_windows(frames->nframes, maxwindow, offset, nullptr, 0);
   if (nw < 1) return fail(ctx, RMSD_ERR_ARG, "Not enough frames to perform calculation.");
   if (!ct || !sd) return fail(ctx, RMSD_ERR_ARG, "ct or sd is NULL");
-  // Method: the prefix form needs a fixed reference and its memory.
+  // Method: the prefix form needs its memory ('first' also one reference
+  // per window).
   const int64_t F = frames->nframes;
   const int k = 3 * frames->nsel;
-  const int ntile = (int)((F + 1 + rmsd_avgcorr_tile() - 1) / rmsd_avgcorr_tile());
-  const size_t prefix_bytes = sizeof(double) * ((size_t)(F + 1) * k +
-                                                (size_t)rmsd_avgcorr_chunks(F) * k + 2 * k +
-                                                (size_t)(F + 2) * 13 + (size_t)nw * ntile * 4);
+  const bool first = ref == nullptr;
+  const int T = rmsd_avgcorr_tile(first);
+  const int ntile = (int)((F + 1 + T - 1) / T);
+  const size_t prefix_bytes =
+      sizeof(double) * ((size_t)(F + 1) * k + (size_t)rmsd_avgcorr_chunks(F) * k + 2 * k +
+                        (size_t)(F + 2) * 13 + (size_t)nw * ntile * 4 +
+                        (first ? (size_t)nw * (k + 13) : 0));
   int method = ctx->avgcorr_method;
   if (method == RMSD_AVGCORR_AUTO)
-    method = ref && prefix_bytes <= AVGCORR_PREFIX_MAX ? RMSD_AVGCORR_PREFIX : RMSD_AVGCORR_RUNNING;
+    method = prefix_bytes <= AVGCORR_PREFIX_MAX ? RMSD_AVGCORR_PREFIX : RMSD_AVGCORR_RUNNING;
   if (method == RMSD_AVGCORR_PREFIX) {
-    if (!ref) return fail(ctx, RMSD_ERR_ARG, "rmsavgcorr: the prefix method needs a fixed reference");
     if (prefix_bytes > AVGCORR_PREFIX_MAX)
       return fail(ctx, RMSD_ERR_ARG, "rmsavgcorr: the prefix method needs " +
                                          std::to_string(prefix_bytes) + " bytes, over " +
@@ -768,7 +771,6 @@ int rmsd_avgcorr(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* re
   }
   // Prefix form: window map and the (row tile, column tile) pairs that hold
   // a window diagonal.
-  const int T = rmsd_avgcorr_tile();
   std::vector<int32_t> wmap((size_t)F + 1, -1);
   for (int i = 0; i < nw; ++i) wmap[ws[i]] = i;
   std::vector<int32_t> cnt((size_t)F + 2, 0);  // cnt[w + 1] = windows <= w
@@ -810,6 +812,19 @@ int rmsd_avgcorr(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* re
   a.tiles = dtiles;
   a.ntiles = (int)tiles.size();
   a.ntile = ntile;
+  a.tile = T;
+  a.yref = nullptr;
+  a.wconst = nullptr;
+  if (first) {
+    double* wc = (double*)scratch(ctx, "avgcorr.consts", sizeof(double) * 8, &st);
+    if (!wc) return st;
+    a.consts = wc;
+    HIP_TRY(ctx, rmsd_launch_avgcorr_wsum(a, wc, ctx->stream));
+    a.yref = (double*)scratch(ctx, "avgcorr.yref", sizeof(double) * (size_t)nw * k, &st);
+    if (!a.yref) return st;
+    a.wconst = (double*)scratch(ctx, "avgcorr.wconst", sizeof(double) * (size_t)nw * 13, &st);
+    if (!a.wconst) return st;
+  }
   a.windows = dws;
   a.nwin = nw;
   a.ct = ct;

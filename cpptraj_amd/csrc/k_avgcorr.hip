@@ -308,7 +308,8 @@ __global__ __launch_bounds__(64) void k_ac_consts(AvgPrefixArgs a) {
   for (int q = 0; q < 13; ++q) v[q] = 0.0;
   for (int kk = threadIdx.x; kk < n; kk += 64) {
     const double m = a.mass ? a.mass[kk] : 1.0;
-    const double y0 = a.ref[kk], y1 = a.ref[a.ldr + kk], y2 = a.ref[2 * a.ldr + kk];
+    const double y0 = a.ref ? a.ref[kk] : 0.0, y1 = a.ref ? a.ref[a.ldr + kk] : 0.0,
+                 y2 = a.ref ? a.ref[2 * a.ldr + kk] : 0.0;
     for (int c = 0; c < 3; ++c) {
       const double mm = m * a.mu[3 * kk + c];
       const double vc = a.mu[3 * kk + c] - mc[c];
@@ -341,7 +342,8 @@ __global__ __launch_bounds__(64) void k_ac_rows(AvgPrefixArgs a) {
   for (int q = 0; q < 13; ++q) v[q] = 0.0;
   for (int kk = threadIdx.x; kk < n; kk += 64) {
     const double sw = a.mass ? sqrt(a.mass[kk]) : 1.0;
-    const double y0 = a.ref[kk], y1 = a.ref[a.ldr + kk], y2 = a.ref[2 * a.ldr + kk];
+    const double y0 = a.ref ? a.ref[kk] : 0.0, y1 = a.ref ? a.ref[a.ldr + kk] : 0.0,
+                 y2 = a.ref ? a.ref[2 * a.ldr + kk] : 0.0;
     for (int c = 0; c < 3; ++c) {
       const double mp = sw * Pi[3 * kk + c];  // m P
       v[c] += mp;
@@ -473,10 +475,11 @@ __global__ __launch_bounds__(64) void k_ac_final(AvgPrefixArgs a) {
     const int ti = slot >> 1, half = slot & 1;
     // The tile (ti, tj) that holds this slot; written when it exists and
     // diagonal w crosses it (a multiple of 64 crosses one tile per row tile).
-    const int tj = ti - (w + AC_TILE - 1) / AC_TILE + half;
+    const int T = a.tile;
+    const int tj = ti - (w + T - 1) / T + half;
     if (tj < 0 || tj > ti) continue;
-    const int off = w - AC_TILE * (ti - tj);
-    if (off < 1 - AC_TILE || off > AC_TILE - 1) continue;
+    const int off = w - T * (ti - tj);
+    if (off < 1 - T || off > T - 1) continue;
     const double* o = a.part + (((int64_t)wi * a.ntile + ti) * 2 + half) * 2;
     s1 += o[0];
     s2 += o[1];
@@ -494,10 +497,205 @@ __global__ __launch_bounds__(64) void k_ac_final(AvgPrefixArgs a) {
   }
 }
 
+// ---- prefix form, 'first' reference --------------------------------------------------
+// Window w's reference is its first averaged frame, centred:
+// y_w = P_w / w + mu - c_w.  An averaged frame's cross moment is then
+//   sum m a y_w^T = (1/w) sum m (P_i - P_j) y_w^T + sum m mu y_w^T,
+// linear in the prefix difference (its rounding stays relative to the
+// difference, as for the fixed reference).  k_ac_firstref writes sqrt(m) y_w
+// per window and its constants (sum m y_w, sum m |y_w|^2, sum m mu y_w^T);
+// k_ac_tiles_first runs 32 x 32 tiles of prefix-row pairs with the window
+// references of the tile's 63 diagonals staged beside the rows, 2 x 2 pairs
+// per thread (a squared distance and nine cross sums each).
+constexpr int ACF_TILE = 32;
+constexpr int ACF_KA = 8;                // atoms per LDS stage
+constexpr int ACF_KC = 3 * ACF_KA;       // coordinates per stage
+constexpr int ACF_LD = ACF_TILE + 1;
+constexpr int ACF_NDIAG = 2 * ACF_TILE - 1;
+
+__global__ __launch_bounds__(64) void k_ac_firstref(AvgPrefixArgs a) {
+  const int wi = blockIdx.x;
+  const int w = a.windows[wi];
+  const double dw = (double)w;
+  const int n = a.fs.nsel;
+  const double W = a.consts[4];
+  const double* cm = a.rows + (a.fs.nframes + 1) * 13;
+  const double* Cw = a.rows + (int64_t)w * 13;
+  double cw[3];
+  for (int e = 0; e < 3; ++e) cw[e] = W > 0.0 ? (Cw[e] / dw + cm[e]) / W : 0.0;
+  const double* Pw = a.P + (int64_t)w * a.k;
+  double* ys = a.yref + (int64_t)wi * a.k;
+  double v[13];
+#pragma unroll
+  for (int q = 0; q < 13; ++q) v[q] = 0.0;
+  for (int kk = threadIdx.x; kk < n; kk += 64) {
+    const double m = a.mass ? a.mass[kk] : 1.0;
+    const double sw = a.mass ? sqrt(m) : 1.0;
+    double y[3];
+    for (int e = 0; e < 3; ++e) {
+      const double pw = sw > 0.0 ? Pw[3 * kk + e] / sw : 0.0;
+      y[e] = pw / dw + a.mu[3 * kk + e] - cw[e];
+      ys[3 * kk + e] = sw * y[e];
+      v[e] += m * y[e];
+      v[3] += m * y[e] * y[e];
+    }
+    for (int c = 0; c < 3; ++c) {
+      const double mm = m * a.mu[3 * kk + c];
+      for (int e = 0; e < 3; ++e) v[4 + 3 * c + e] += mm * y[e];
+    }
+  }
+  double r[13];
+  wave_moments(v, r);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 13; ++q) a.wconst[(int64_t)wi * 13 + q] = r[q];
+}
+
+__global__ __launch_bounds__(AC_THREADS) void k_ac_tiles_first(AvgPrefixArgs a) {
+  // [2][KC][33] staged rows (later the RMSDs [32][33]) and [63][KC] window
+  // references.
+  __shared__ double sij[2 * ACF_KC * ACF_LD];
+  __shared__ double sy[ACF_NDIAG * ACF_KC];
+  static_assert(2 * ACF_KC * ACF_LD >= ACF_TILE * ACF_LD, "RMSD tile does not fit");
+  double* si = sij;
+  double* sj = sij + ACF_KC * ACF_LD;
+  const int2 tile = a.tiles[blockIdx.x];
+  const int ti = tile.x, tj = tile.y;
+  const int64_t i0 = (int64_t)ti * ACF_TILE, j0 = (int64_t)tj * ACF_TILE;
+  const int64_t base = i0 - j0;
+  const int64_t nrow = a.fs.nframes + 1;
+  const int t = threadIdx.x, ty = t >> 4, tx = t & 15;
+  double D[2][2], X[2][2][9];
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      D[p][q] = 0.0;
+#pragma unroll
+      for (int e = 0; e < 9; ++e) X[p][q][e] = 0.0;
+    }
+  // Diagonal of each of the thread's pairs (index into sy).
+  int dg[2][2];
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) dg[p][q] = (ty + 16 * p) - (tx + 16 * q) + (ACF_TILE - 1);
+  for (int k0 = 0; k0 < a.k; k0 += ACF_KC) {
+    // Rows: 32 x 24 coordinates each for i and j.
+    for (int e = t; e < ACF_TILE * ACF_KC; e += AC_THREADS) {
+      const int row = e / ACF_KC, kc = e - row * ACF_KC;
+      const int64_t gi = i0 + row, gj = j0 + row;
+      const int kg = k0 + kc;
+      si[kc * ACF_LD + row] = (kg < a.k && gi < nrow) ? a.P[gi * a.k + kg] : 0.0;
+      sj[kc * ACF_LD + row] = (kg < a.k && gj < nrow) ? a.P[gj * a.k + kg] : 0.0;
+    }
+    // Window references of the 63 diagonals.
+    for (int e = t; e < ACF_NDIAG * ACF_KC; e += AC_THREADS) {
+      const int d = e / ACF_KC, kc = e - d * ACF_KC;
+      const int64_t w = base + d - (ACF_TILE - 1);
+      const int kg = k0 + kc;
+      double v = 0.0;
+      if (w >= 1 && w < nrow && kg < a.k) {
+        const int wi = a.wmap[w];
+        if (wi >= 0) v = a.yref[(int64_t)wi * a.k + kg];
+      }
+      sy[d * ACF_KC + kc] = v;
+    }
+    __syncthreads();
+    for (int at = 0; at < ACF_KA; ++at) {
+      double pi[2][3], pj[2][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+          pi[p][c] = si[(3 * at + c) * ACF_LD + ty + 16 * p];
+          pj[p][c] = sj[(3 * at + c) * ACF_LD + tx + 16 * p];
+        }
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const double* y = sy + dg[p][q] * ACF_KC + 3 * at;
+          const double y0 = y[0], y1 = y[1], y2 = y[2];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const double d = pi[p][c] - pj[q][c];
+            D[p][q] = fma(d, d, D[p][q]);
+            X[p][q][3 * c] = fma(d, y0, X[p][q][3 * c]);
+            X[p][q][3 * c + 1] = fma(d, y1, X[p][q][3 * c + 1]);
+            X[p][q][3 * c + 2] = fma(d, y2, X[p][q][3 * c + 2]);
+          }
+        }
+    }
+    __syncthreads();
+  }
+  const double W = a.consts[4];
+  const double* cm = a.rows + nrow * 13;
+  double* rr = sij;  // [32][33] RMSDs, over the staging
+  for (int p = 0; p < 2; ++p)
+    for (int q = 0; q < 2; ++q) {
+      const int il = ty + 16 * p, jl = tx + 16 * q;
+      const int64_t i = i0 + il, j = j0 + jl;
+      const int64_t w = i - j;
+      double r = 0.0;
+      if (w >= 1 && i < nrow && a.wmap[w] >= 0) {
+        if (W < RMSD_SMALL) {
+          r = -1.0;
+        } else {
+          const int wi = a.wmap[w];
+          const double dw = (double)w;
+          const double* ri = a.rows + i * 13;
+          const double* rj = a.rows + j * 13;
+          const double* K = a.wconst + (int64_t)wi * 13;  // sum m y (3), sum m |y|^2, M (9)
+          double dc[3], c[3];
+          for (int e = 0; e < 3; ++e) {
+            dc[e] = (ri[e] - rj[e]) / (dw * W);
+            c[e] = dc[e] + cm[e] / W;
+          }
+          double R[9];
+          for (int e = 0; e < 9; ++e) R[e] = X[p][q][e] / dw + K[4 + e] - c[e / 3] * K[e % 3];
+          const double sx = D[p][q] / (dw * dw) + 2.0 * (ri[12] - rj[12]) / dw + cm[12] -
+                            W * (dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2]);
+          const double E0 = 0.5 * (sx + K[3]);
+          r = rmsd_dev::qcp_rmsd(R, E0, W, 1);
+        }
+      }
+      rr[il * ACF_LD + jl] = r;
+    }
+  __syncthreads();
+  if (t < ACF_NDIAG) {
+    const int dd = t - (ACF_TILE - 1);
+    const int64_t w = base + dd;
+    if (w >= 1 && w < nrow && a.wmap[w] >= 0) {
+      double s1 = 0.0, s2 = 0.0;
+      const int il0 = dd > 0 ? dd : 0, il1 = dd < 0 ? ACF_TILE + dd : ACF_TILE;
+      for (int il = il0; il < il1; ++il) {
+        const double r = rr[il * ACF_LD + il - dd];
+        s1 += r;
+        s2 += r * r;
+      }
+      const int half = tj - (ti - (int)((w + ACF_TILE - 1) / ACF_TILE));
+      double* o = a.part + (((int64_t)a.wmap[w] * a.ntile + ti) * 2 + half) * 2;
+      o[0] = s1;
+      o[1] = s2;
+    }
+  }
+}
+
+// Total weight for the 'first' form (consts[4]; the rest unused): one wave.
+__global__ __launch_bounds__(64) void k_ac_wsum(AvgPrefixArgs a, double* consts) {
+  double s = 0.0;
+  for (int kk = threadIdx.x; kk < a.fs.nsel; kk += 64) s += a.mass ? a.mass[kk] : 1.0;
+  s = rmsd_dev::wave_sum(s);
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < 4; ++q) consts[q] = 0.0;
+    consts[4] = s;
+  }
+}
+
 }  // namespace
 
 int rmsd_avgcorr_chunks(int64_t nframes) { return (int)((nframes + AC_CHUNK - 1) / AC_CHUNK); }
-int rmsd_avgcorr_tile() { return AC_TILE; }
+int rmsd_avgcorr_tile(bool first) { return first ? ACF_TILE : AC_TILE; }
 
 hipError_t rmsd_launch_avgcorr_prefix(const AvgPrefixArgs& a, hipStream_t s) {
   if (a.nwin == 0 || a.ntiles == 0) return hipSuccess;
@@ -513,7 +711,17 @@ hipError_t rmsd_launch_avgcorr_prefix(const AvgPrefixArgs& a, hipStream_t s) {
 
 hipError_t rmsd_launch_avgcorr_tiles(const AvgPrefixArgs& a, hipStream_t s) {
   if (a.nwin == 0 || a.ntiles == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_ac_tiles, dim3(a.ntiles), dim3(AC_THREADS), 0, s, a);
+  if (a.yref) {
+    hipLaunchKernelGGL(k_ac_firstref, dim3(a.nwin), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_ac_tiles_first, dim3(a.ntiles), dim3(AC_THREADS), 0, s, a);
+  } else {
+    hipLaunchKernelGGL(k_ac_tiles, dim3(a.ntiles), dim3(AC_THREADS), 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t rmsd_launch_avgcorr_wsum(const AvgPrefixArgs& a, double* consts, hipStream_t s) {
+  hipLaunchKernelGGL(k_ac_wsum, dim3(1), dim3(64), 0, s, a, consts);
   return hipGetLastError();
 }
 

@@ -187,8 +187,8 @@ hipError_t rmsd_launch_avgcorr(const AvgCorrArgs& a, bool first, hipStream_t s);
 struct AvgPrefixArgs {
   FrameSetDev fs;
   const double* mass;     // nsel or null
-  const double* ref;      // centred reference SoA [3][ldr]
-  const double* consts;   // reference constants (sum w y, sum w |y|^2, W)
+  const double* ref;      // centred reference SoA [3][ldr], null for 'first'
+  const double* consts;   // reference constants (sum w y, sum w |y|^2, W); [4] = W
   int ldr;
   int k;                  // 3 nsel
   double* chunk;          // [nchunks][k] scan scratch
@@ -200,13 +200,17 @@ struct AvgPrefixArgs {
   int ntiles;
   int ntile;              // row tiles: ceil((F+1) / 64)
   double* part;           // [nwin][ntile][2][2] per-tile sums of RMSD and RMSD^2
+  int tile;               // frames per tile side (64 fixed reference, 32 'first')
+  double* yref;           // 'first': [nwin][k] sqrt(m) y_w per window, else null
+  double* wconst;         // 'first': [nwin][13] sum m y_w, sum m |y_w|^2, sum m mu y_w^T
   const int* windows;
   int nwin;
   double* ct;
   double* sd;
 };
 int rmsd_avgcorr_chunks(int64_t nframes);
-int rmsd_avgcorr_tile();
+int rmsd_avgcorr_tile(bool first);
+hipError_t rmsd_launch_avgcorr_wsum(const AvgPrefixArgs& a, double* consts, hipStream_t s);
 hipError_t rmsd_launch_avgcorr_prefix(const AvgPrefixArgs& a, hipStream_t s);
 hipError_t rmsd_launch_avgcorr_tiles(const AvgPrefixArgs& a, hipStream_t s);
 hipError_t rmsd_launch_avgcorr_final(const AvgPrefixArgs& a, hipStream_t s);

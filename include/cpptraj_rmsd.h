@@ -217,20 +217,21 @@ int rmsd_avgcorr_windows(int nframes, int maxwindow, int offset, int32_t* window
 int rmsd_avgcorr(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* ref, int ref_frame,
                  int use_mass, int maxwindow, int offset, double* ct, double* sd);
 
-/* How rmsd_avgcorr computes a fixed-reference run; no counterpart in the
- * reference (its running sum is the RUNNING method).
- *   RMSD_AVGCORR_AUTO     PREFIX when the reference is fixed and its device
- *                         memory (8 (F+1) 3 nsel + 32 nwindows ceil((F+1)/64)
- *                         bytes, about) stays within 16 GiB, else RUNNING
+/* How rmsd_avgcorr computes; no counterpart in the reference (its running sum
+ * is the RUNNING method).
+ *   RMSD_AVGCORR_AUTO     PREFIX when its device memory (about
+ *                         8 (F+1) 3 nsel + 32 nwindows ceil((F+1)/T) bytes, T = 64
+ *                         with a reference, 32 with 'first', which also keeps
+ *                         8 (3 nsel + 13) bytes per window) stays within
+ *                         16 GiB, else RUNNING
  *   RMSD_AVGCORR_RUNNING  one wavefront per window size, FP64 running sum
- *                         (the only method for 'first'; atom limits above)
+ *                         (atom limits above)
  *   RMSD_AVGCORR_PREFIX   prefix sums of the mean-subtracted coordinates: every
  *                         averaged frame of every window is fitted
- *                         independently, 64 x 64 (frame, frame) tiles at a time;
+ *                         independently, T x T (frame, frame) tiles at a time;
  *                         agrees with RUNNING to rounding (the window sums are
  *                         differences of prefix sums, not running sums); no
- *                         atom limit; an error for 'first' or over the memory
- *                         limit.
+ *                         atom limit; an error over the memory limit.
  * The setting stays with the context. */
 #define RMSD_AVGCORR_AUTO 0
 #define RMSD_AVGCORR_RUNNING 1

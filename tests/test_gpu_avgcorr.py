@@ -11,13 +11,14 @@ the bar there is 5e-7 A.  Cases:
 fixed reference and 'first', mass weighting, 'stop' and 'offset', a
 selection, SoA frames, and the argument errors.
 
-A fixed reference runs by either method of rmsd_avgcorr_set_method: the
-running sum (one wavefront per window) or the prefix form (window sums as
-differences of prefix sums of the mean-subtracted coordinates, every
-averaged frame fitted independently in 64 x 64 tiles of prefix rows).  Both
-meet the same bar; the prefix form also past the running method's LDS atom
-limit, across many tiles, on a drifting trajectory, and against the running
-method on 3,000 frames.
+Both methods of rmsd_avgcorr_set_method run every case: the running sum (one
+wavefront per window) and the prefix form (window sums as differences of
+prefix sums of the mean-subtracted coordinates, every averaged frame fitted
+independently in tiles of prefix rows: 64 x 64 with a fixed reference,
+32 x 32 with 'first', whose window references are staged per diagonal).
+Both meet the same bars; the prefix form also past the running method's LDS
+atom limit, across many tiles, on a drifting trajectory, and against the
+running method on 3,000 frames.
 """
 import numpy as np
 import pytest
@@ -39,7 +40,8 @@ def _avgcorr(ca, ctx, *args, method="auto", **kw):
 
 
 @pytest.mark.parametrize("use_mass", [False, True])
-@pytest.mark.parametrize("first,method", [(False, "running"), (False, "prefix"), (True, "running")])
+@pytest.mark.parametrize("first,method", [(False, "running"), (False, "prefix"), (True, "running"),
+                                          (True, "prefix")])
 @pytest.mark.parametrize("stop_offset", [(-1, 1), (40, 3), (-1, 7)])
 def test_avgcorr_matches_oracle(ctx, oracle, use_mass, first, method, stop_offset):
     import torch
@@ -156,6 +158,25 @@ def test_avgcorr_prefix_matches_running_3000_frames(ctx):
     assert np.abs(out["running"][1] - out["prefix"][1]).max() <= TOL
 
 
+@pytest.mark.parametrize("use_mass", [False, True])
+def test_avgcorr_first_prefix_matches_running_3000_frames(ctx, use_mass):
+    # 'first': every window's reference is its own first averaged frame.
+    import torch
+
+    import cpptraj_amd as ca
+    rng = np.random.default_rng(80)
+    F, n = 3000, 150
+    X = synth.noisy(14, F, n, 0.6).reshape(F, -1).astype(np.float32)
+    m = synth.masses(rng, n)
+    Xd, md = torch.as_tensor(X).cuda(), torch.as_tensor(m).cuda()
+    out = {}
+    for method in ("running", "prefix"):
+        w, ct, sd = _avgcorr(ca, ctx, Xd, mass=md, use_mass=use_mass, method=method)
+        out[method] = (ct.cpu().numpy(), sd.cpu().numpy())
+    assert np.abs(out["running"][0] - out["prefix"][0]).max() <= 5e-7
+    assert np.abs(out["running"][1] - out["prefix"][1]).max() <= 5e-7
+
+
 def test_avgcorr_errors(ctx):
     import torch
 
@@ -165,11 +186,9 @@ def test_avgcorr_errors(ctx):
         ca.rms_avgcorr(X, ctx=ctx)
     Y = torch.zeros((4, 3 * 4000), dtype=torch.float32, device="cuda")
     with pytest.raises(ca.RmsdError, match="selected atoms with 'first'"):
-        ca.rms_avgcorr(Y, ctx=ctx)
+        _avgcorr(ca, ctx, Y, method="running")
     Z = torch.zeros((4, 3 * 7000), dtype=torch.float32, device="cuda")
     with pytest.raises(ca.RmsdError, match="selected atoms"):
         _avgcorr(ca, ctx, Z, ref_frames=Z[:1], method="running")
-    with pytest.raises(ca.RmsdError, match="needs a fixed reference"):
-        _avgcorr(ca, ctx, Y, method="prefix")
     with pytest.raises(ca.RmsdError, match="unknown rmsavgcorr method"):
         ctx.check(ctx._L.rmsd_avgcorr_set_method(ctx.handle, 7))
