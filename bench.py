@@ -529,8 +529,8 @@ def main():
     ap.add_argument("--skip-symm", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(see DESIGN.md for how the pair tiles would shard).", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (the pair tiles are "
+              "independent; split across GPUs they would need no reduction).", file=sys.stderr)
         sys.exit(2)
 
     import torch
