@@ -601,6 +601,7 @@ __global__ __launch_bounds__(AC_THREADS) void k_ac_tiles_first(AvgPrefixArgs a) 
       sy[d * ACF_KC + kc] = v;
     }
     __syncthreads();
+#pragma unroll
     for (int at = 0; at < ACF_KA; ++at) {
       double pi[2][3], pj[2][3];
 #pragma unroll
@@ -610,12 +611,21 @@ __global__ __launch_bounds__(AC_THREADS) void k_ac_tiles_first(AvgPrefixArgs a) 
           pi[p][c] = si[(3 * at + c) * ACF_LD + ty + 16 * p];
           pj[p][c] = sj[(3 * at + c) * ACF_LD + tx + 16 * p];
         }
+      // The pairs (0, 0) and (1, 1) lie on one diagonal: three references.
+      double yv[3][3];
+#pragma unroll
+      for (int u = 0; u < 3; ++u) {
+        const double* y = sy + (dg[0][0] + 16 * (u - 1)) * ACF_KC + 3 * at;
+        yv[u][0] = y[0];
+        yv[u][1] = y[1];
+        yv[u][2] = y[2];
+      }
 #pragma unroll
       for (int p = 0; p < 2; ++p)
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-          const double* y = sy + dg[p][q] * ACF_KC + 3 * at;
-          const double y0 = y[0], y1 = y[1], y2 = y[2];
+          const int u = p - q + 1;
+          const double y0 = yv[u][0], y1 = yv[u][1], y2 = yv[u][2];
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
             const double d = pi[p][c] - pj[q][c];
