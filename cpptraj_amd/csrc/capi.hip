@@ -886,12 +886,32 @@ int rmsd_symm(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* ref, 
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   const int64_t nf = frames->nframes;
   const int ldr = (int)round_up(n, 4);
+  // Groups in the device table are ordered by size (stable): a wave takes
+  // consecutive groups of one frame, which then share one size.  Every group
+  // is assigned on its own, so the order changes no result.
+  std::vector<int> order((size_t)ngroups);
+  for (int g = 0; g < ngroups; ++g) order[g] = g;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    return group_off[x + 1] - group_off[x] < group_off[y + 1] - group_off[y];
+  });
+  std::vector<int32_t> goff((size_t)ngroups + 1, 0), gat;
+  std::vector<int32_t> bigo((size_t)std::max(ngroups, 1), -1);
+  gat.reserve(S);
+  for (int g = 0; g < ngroups; ++g) {
+    const int src = order[g];
+    for (int k = group_off[src]; k < group_off[src + 1]; ++k) {
+      pos[group_atoms[k]] = (int32_t)gat.size();
+      gat.push_back(group_atoms[k]);
+    }
+    goff[g + 1] = (int32_t)gat.size();
+    bigo[g] = bigoff[src];
+  }
   std::vector<int32_t> blob;
   blob.reserve((size_t)(ngroups + 1) + S + n + ngroups);
-  for (int g = 0; g <= ngroups; ++g) blob.push_back(ngroups > 0 ? group_off[g] : 0);
-  for (int k = 0; k < S; ++k) blob.push_back(group_atoms[k]);
+  blob.insert(blob.end(), goff.begin(), goff.end());
+  blob.insert(blob.end(), gat.begin(), gat.end());
   blob.insert(blob.end(), pos.begin(), pos.end());
-  blob.insert(blob.end(), bigoff.begin(), bigoff.begin() + ngroups);
+  blob.insert(blob.end(), bigo.begin(), bigo.begin() + ngroups);
   int32_t* dblob = (int32_t*)scratch(ctx, "symm.table", sizeof(int32_t) * blob.size(), &st);
   if (!dblob) return st;
   HIP_TRY(ctx, hipMemcpyAsync(dblob, blob.data(), sizeof(int32_t) * blob.size(),

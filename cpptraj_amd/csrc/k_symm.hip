@@ -17,8 +17,9 @@
 //
 // The initial fit is the frame-to-reference fit (k_fit.hip) run first.  One
 // lane per (frame, group) for the assignment: groups are small (2-3 atoms in
-// proteins) and the heuristic is sequential and branchy, so lanes of one
-// wave take one group in 64 frames (one size per wave).  Groups of up to
+// proteins) and the heuristic is sequential and branchy; the groups are
+// ordered by size and lanes of one wave take consecutive groups of one frame
+// (one size per wave but at size boundaries, loads within one frame).  Groups of up to
 // SYMM_GROUP_REG atoms run a compile-time-size form with every array in
 // registers (no LDS: occupancy is set by registers alone); larger groups keep
 // their arrays in a slab of global scratch per (frame, group).  The final pass is one wavefront per frame: with a fit it
@@ -401,14 +402,16 @@ __device__ __forceinline__ void assign_reg(const SymmArgs& a, int64_t f, const i
   for (int c = 0; c < N; ++c) out[c] = ga[map[c] >= 0 ? map[c] : c];
 }
 
-// Lanes: consecutive frames of one group (pair = group * F + frame), so a
-// wave's groups share one size.
+// Lanes: consecutive groups of one frame (pair = frame * G + group); the
+// groups are ordered by size, so a wave's groups share one size (but at the
+// few size boundaries), its loads stay within one frame and the frame's
+// initial fit is one broadcast load.
 __global__ __launch_bounds__(SYM_THREADS) void k_symm_assign(SymmArgs a) {
   const int64_t F = a.fs.nframes;
   const int64_t pair = (int64_t)blockIdx.x * SYM_THREADS + threadIdx.x;
   if (pair >= F * a.ngroups) return;
-  const int g = (int)(pair / F);
-  const int64_t f = pair - (int64_t)g * F;
+  const int64_t f = pair / a.ngroups;
+  const int g = (int)(pair - f * a.ngroups);
   const int g0 = a.goff[g];
   const int n = a.goff[g + 1] - g0;
   const int* ga = a.gatoms + g0;
