@@ -559,14 +559,20 @@ __global__ __launch_bounds__(SYM_THREADS) void k_symm_final_nofit(SymmArgs a) {
 }
 
 // The per-frame map of every selected atom (SymmetricRmsdCalc::AMap).
+// Grid: x over frames (AMAP_FRAMES per block), y over atoms (256 per block).
+constexpr int AMAP_FRAMES = 8;
 __global__ __launch_bounds__(256) void k_symm_amap(SymmArgs a) {
-  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int n = a.fs.nsel;
-  if (idx >= (int64_t)a.fs.nframes * n) return;
-  const int64_t f = idx / n;
-  const int i = (int)(idx - f * n);
+  const int i = blockIdx.y * 256 + threadIdx.x;
+  if (i >= n) return;
   const int p = a.pos[i];
-  a.amap[idx] = p >= 0 ? a.gmap[f * a.ngatoms + p] : i;
+  const int64_t f0 = (int64_t)blockIdx.x * AMAP_FRAMES;
+#pragma unroll
+  for (int k = 0; k < AMAP_FRAMES; ++k) {
+    const int64_t f = f0 + k;
+    if (f >= a.fs.nframes) break;
+    a.amap[f * n + i] = p >= 0 ? a.gmap[f * a.ngatoms + p] : i;
+  }
 }
 
 }  // namespace
@@ -590,10 +596,11 @@ hipError_t rmsd_launch_symm(const SymmArgs& a, hipStream_t s) {
   else
     hipLaunchKernelGGL((k_symm_final_nofit<false>), grid, block, 0, s, a);
   if (a.amap) {
-    const int64_t total = F * a.fs.nsel;
-    const int64_t blocks = (total + 255) / 256;
-    if (blocks > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_symm_amap, dim3((unsigned)blocks), dim3(256), 0, s, a);
+    const int64_t gx = (F + AMAP_FRAMES - 1) / AMAP_FRAMES;
+    const int gy = (a.fs.nsel + 255) / 256;
+    if (gy > 65535) return hipErrorInvalidValue;
+    const dim3 g((unsigned)gx, (unsigned)gy);
+    hipLaunchKernelGGL(k_symm_amap, g, dim3(256), 0, s, a);
   }
   return hipGetLastError();
 }
