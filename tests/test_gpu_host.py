@@ -421,3 +421,25 @@ def test_cluster_cache_srmsd_metric(hs, golden, oracle):
             assert abs(got[k] - np.float32(e[0])) <= 1e-6, (a, b)
             k += 1
     assert k == len(got)
+
+
+def test_symmrmsd_reftraj_and_mass(hs, golden, oracle):
+    # 'reftraj': frame f against frame f of another COORDS set (centred with
+    # masses for the fit); 'mass' weights both fits.
+    d = golden("symm_afv")
+    traj = d["traj"][:30]
+    F, n = traj.shape[:2]
+    rng = np.random.default_rng(90)
+    rtraj = (traj[::-1] + rng.normal(size=traj.shape) * 0.2).astype(np.float32)
+    hs.add_coords("t", traj.reshape(F, -1), n, d["mass"])
+    hs.add_coords("r", rtraj.reshape(F, -1), n, d["mass"])
+    hs.set_symmetric_groups("t", _topology_groups(d, d["sel"]))
+    hs.add_mask("!@H=", d["sel"])
+    hs.run("symmrmsd crdset t RT !@H= reftraj r mass")
+    sel, groups, m = d["sel"], _topology_groups(d), d["mass"][d["sel"]]
+    exp = []
+    for f in range(F):
+        rc, _ = oracle.center_on_origin(rtraj[f].astype(np.float64)[sel], m, True)
+        e, _, _, _ = oracle.symm(traj[f:f + 1].reshape(1, -1), sel, m, rc, True, True, groups)
+        exp.append(e[0])
+    assert np.abs(hs.series("RT") - np.array(exp)).max() <= 1e-9
