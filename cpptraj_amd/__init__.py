@@ -158,8 +158,9 @@ def frames_desc(xyz, sel=None, mass=None, fidx=None, soa=False, nsel=None, natom
     xyz, sel, mass and fidx alive until the library call that uses it has been
     enqueued (a freed tensor's block can be handed to the next allocation).
 
-    xyz: AoS (F, natom*3) or (F, natom, 3) float32, or SoA (F, 3, ld) with soa=True
-         (natom atoms used of each padded row; default ld).
+    xyz: AoS (F, natom*3) or (F, natom, 3) float32, or SoA (F, 3, ld) with soa=True;
+         natom atoms are used of each (padded) row (default: the row length / 3
+         for AoS, ld for SoA).
     sel: int32 CUDA tensor of selected atoms; mass: float64 CUDA tensor (nsel)."""
     import torch
 
@@ -179,7 +180,10 @@ def frames_desc(xyz, sel=None, mass=None, fidx=None, soa=False, nsel=None, natom
         if x.stride(-1) != 1 and x.numel() > 0:
             raise RmsdError("AoS frames need unit coordinate stride")
         ld = 0
-        natom = x.shape[1] // 3
+        if natom is None:
+            natom = x.shape[1] // 3
+        elif 3 * natom > x.shape[1]:
+            raise RmsdError(f"{natom} atoms do not fit AoS rows of {x.shape[1]} floats")
         stride = x.stride(0) if F > 1 else 3 * natom
     for t, dt in ((sel, torch.int32), (mass, torch.float64), (fidx, torch.int32)):
         if t is not None and (not t.is_cuda or t.dtype != dt or not t.is_contiguous()):

@@ -269,6 +269,63 @@ __global__ __launch_bounds__(WAVE_FIT_THREADS) void k_fit_moments_wave(FitArgs a
 // in flight and one LDS read of the reference serves both frames.
 constexpr int WAVE2_FIT_THREADS = 512;
 
+// One frame per wave for AoS frames without a selection (cpptraj's own
+// layout): a lane takes four consecutive atoms, twelve floats, as three
+// 16-byte loads, so consecutive lanes read consecutive 48 bytes.
+template <bool WEIGHTED>
+__global__ __launch_bounds__(WAVE_FIT_THREADS) void k_fit_moments_aos4(FitArgs a, int ldr,
+                                                                       double* shift) {
+  extern __shared__ double sref[];
+  const int n = a.fs.nsel;
+  for (int i = threadIdx.x; i < 3 * ldr; i += blockDim.x) sref[i] = a.ref[i];
+  if (WEIGHTED)
+    for (int i = threadIdx.x; i < ldr; i += blockDim.x) sref[3 * ldr + i] = i < n ? a.mass[i] : 0.0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+  const double* Y0 = sref;
+  const double* Y1 = sref + ldr;
+  const double* Y2 = sref + 2 * ldr;
+  const double* Wt = sref + 3 * ldr;
+  const int n4 = n >> 2;
+  for (int64_t f = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+       f < a.fs.nframes; f += nw) {
+    const float* fr = a.fs.xyz + frame_of(a.fs, f) * a.fs.stride;
+    const double s0 = fr[0], s1 = fr[1], s2 = fr[2];
+    double v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = 0.0;
+    double (&m)[13] = *reinterpret_cast<double(*)[13]>(v);
+    const f32x4* V = reinterpret_cast<const f32x4*>(fr);
+#pragma unroll 2
+    for (int q = lane; q < n4; q += 64) {
+      const f32x4 p0 = __builtin_nontemporal_load(V + 3 * q);
+      const f32x4 p1 = __builtin_nontemporal_load(V + 3 * q + 1);
+      const f32x4 p2 = __builtin_nontemporal_load(V + 3 * q + 2);
+      const int k = 4 * q;
+      accum<WEIGHTED>(m, p0.x - s0, p0.y - s1, p0.z - s2, WEIGHTED ? Wt[k] : 1.0, Y0[k], Y1[k],
+                      Y2[k]);
+      accum<WEIGHTED>(m, p0.w - s0, p1.x - s1, p1.y - s2, WEIGHTED ? Wt[k + 1] : 1.0, Y0[k + 1],
+                      Y1[k + 1], Y2[k + 1]);
+      accum<WEIGHTED>(m, p1.z - s0, p1.w - s1, p2.x - s2, WEIGHTED ? Wt[k + 2] : 1.0, Y0[k + 2],
+                      Y1[k + 2], Y2[k + 2]);
+      accum<WEIGHTED>(m, p2.y - s0, p2.z - s1, p2.w - s2, WEIGHTED ? Wt[k + 3] : 1.0, Y0[k + 3],
+                      Y1[k + 3], Y2[k + 3]);
+    }
+    for (int k = 4 * n4 + lane; k < n; k += 64)
+      accum<WEIGHTED>(m, (double)fr[3 * k] - s0, (double)fr[3 * k + 1] - s1,
+                      (double)fr[3 * k + 2] - s2, WEIGHTED ? Wt[k] : 1.0, Y0[k], Y1[k], Y2[k]);
+    const double t = rmsd_dev::wave_reduce16(v);
+    const int idx = lane >> 2;
+    if ((lane & 3) == 0 && idx < 13) a.moments[13 * f + idx] = t;
+    if (lane == 0) {
+      shift[3 * f] = s0;
+      shift[3 * f + 1] = s1;
+      shift[3 * f + 2] = s2;
+    }
+  }
+}
+
 template <bool WEIGHTED>
 __global__ __launch_bounds__(WAVE2_FIT_THREADS) void k_fit_moments_wave2(FitArgs a, int ldr,
                                                                          double* shift) {
@@ -499,6 +556,8 @@ hipError_t rmsd_launch_fit_moments(const FitArgs& a, hipStream_t s) {
   if (nf == 0) return hipSuccess;
   const bool vec = a.fs.sel == nullptr && a.fs.ld > 0 && (a.fs.ld % 4) == 0 &&
                    (a.fs.stride % 4) == 0 && ((uintptr_t)a.fs.xyz % 16) == 0;
+  const bool aos4 = a.fs.sel == nullptr && a.fs.ld == 0 && (a.fs.stride % 4) == 0 &&
+                    ((uintptr_t)a.fs.xyz % 16) == 0;
   const bool w = a.mass != nullptr;
   // Wave-per-frame path whenever the reference fits in LDS (up to 5120
   // weighted / 6826 unweighted atoms); one-block-per-frame kernels otherwise.
@@ -537,8 +596,23 @@ hipError_t rmsd_launch_fit_moments(const FitArgs& a, hipStream_t s) {
     hipLaunchKernelGGL((k_fit_moments_wave2<W>), dim3(grid), dim3(WAVE2_FIT_THREADS), lds, s, a, \
                        ldr, shift);                                                          \
   }
+#define RMSD_FIT_AOS4(W)                                                                      \
+  {                                                                                          \
+    const void* fn = reinterpret_cast<const void*>(&k_fit_moments_aos4<W>);                \
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+    if (e != hipSuccess) return e;                                                           \
+    int bpc = 1;                                                                             \
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, fn, WAVE_FIT_THREADS, lds);       \
+    if (e != hipSuccess) return e;                                                           \
+    if (bpc < 1) bpc = 1;                                                                    \
+    const int grid = (int)(want < (int64_t)ncu * bpc ? want : (int64_t)ncu * bpc);           \
+    hipLaunchKernelGGL((k_fit_moments_aos4<W>), dim3(grid), dim3(WAVE_FIT_THREADS), lds, s, a, \
+                       ldr, shift);                                                          \
+  }
     if (vec && kFitFramesPerWave == 2) {
       if (w) RMSD_FIT_WAVE2(true) else RMSD_FIT_WAVE2(false)
+    } else if (aos4) {
+      if (w) RMSD_FIT_AOS4(true) else RMSD_FIT_AOS4(false)
     } else if (vec) {
       if (w) RMSD_FIT_WAVE(true, true) else RMSD_FIT_WAVE(true, false)
     } else {
@@ -546,6 +620,7 @@ hipError_t rmsd_launch_fit_moments(const FitArgs& a, hipStream_t s) {
     }
 #undef RMSD_FIT_WAVE
 #undef RMSD_FIT_WAVE2
+#undef RMSD_FIT_AOS4
     return hipGetLastError();
   }
   hipLaunchKernelGGL(k_fit_shift, dim3((nf + 255) / 256), dim3(256), 0, s, a.fs, shift);

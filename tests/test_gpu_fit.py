@@ -134,3 +134,23 @@ def test_fit_soa_ragged(oracle, ctx, F, use_mass):
     assert_rmsd_close(r.cpu().numpy(), r_o, X)
     assert np.abs(U.cpu().numpy() - U_o).max() <= 1e-9
     assert np.abs(tv.cpu().numpy() - tv_o).max() <= 1e-9
+
+
+@pytest.mark.parametrize("use_mass", [False, True])
+def test_fit_aos_padded_stride_tail(oracle, ctx, use_mass):
+    # AoS frames of 301 atoms stored 908 floats apart (a 16-byte-aligned stride
+    # with trailing values, as box data would be): the vector AoS path takes
+    # 300 atoms four at a time and the last one alone.
+    import cpptraj_amd as ca
+    F, n, stride = 200, 301, 908
+    X3 = synth.noisy(5, F, n, 0.8)
+    m = synth.masses(np.random.default_rng(5), n)
+    Xp = np.full((F, stride), 7.5, np.float32)
+    Xp[:, :3 * n] = X3.reshape(F, -1)
+    r_o, U_o, tv_o, _ = oracle_fit(oracle, X3.reshape(F, -1), None, m, use_mass)
+    r, U, tv = ca.rmsd_fit(dev(Xp), None, 0, mass=dev(m), use_mass=use_mass, modify="nomod",
+                           save_matrices=True, save_vectors=True, natom=n, ctx=ctx)
+    r, U, tv = r.cpu().numpy(), U.cpu().numpy(), tv.cpu().numpy()
+    assert_rmsd_close(r, r_o, X3.reshape(F, -1))
+    assert np.abs(U - U_o).max() <= 1e-9
+    assert np.abs(tv - tv_o).max() <= 1e-9
