@@ -514,13 +514,17 @@ int GpuSession::FramesOnDevice(const std::string& tag, const CoordsSet& set, con
   if (Upload(dx, src, fbytes) || Upload(ds, s32.data(), sizeof(int32_t) * s32.size()) ||
       Upload(dm, m.data(), sizeof(double) * m.size()))
     return 1;
+  // Every atom in order: no selection array, which the library reads as the
+  // same atoms and which lets it use its contiguous-frame loads.
+  bool identity = (int)sel.size() == set.top.natom;
+  for (size_t k = 0; identity && k < sel.size(); ++k) identity = sel[k] == (int)k;
   memset(out, 0, sizeof(*out));
   out->xyz = dx;
   out->stride = set.stride;
   out->ld = 0;
   out->nframes = nf;
   out->natom = set.top.natom;
-  out->sel = ds;
+  out->sel = identity ? nullptr : ds;
   out->nsel = (int32_t)sel.size();
   out->mass = dm;
   out->fidx = nullptr;
