@@ -934,7 +934,10 @@ int rmsd_symm(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* ref, 
   if (!gmap) return st;
   double* big = nullptr;
   if (bigstride > 0) {
-    big = (double*)scratch(ctx, "symm.big", sizeof(double) * (size_t)(nf * bigstride), &st);
+    // Frames of one assignment launch: as many as SYMM_BIG_MAX bytes hold.
+    const int64_t chunk =
+        std::min<int64_t>(nf, std::max<int64_t>(1, (int64_t)(SYMM_BIG_MAX / (sizeof(double) * bigstride))));
+    big = (double*)scratch(ctx, "symm.big", sizeof(double) * (size_t)(chunk * bigstride), &st);
     if (!big) return st;
   }
   const FrameSetDev F = to_dev(*frames);
@@ -994,6 +997,8 @@ int rmsd_symm(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* ref, 
   a.bigoff = dblob + (ngroups + 1) + S + n;
   a.bigstride = bigstride;
   a.big = big;
+  a.f0 = 0;
+  a.f1 = nf;
   a.gmap = gmap;
   a.rmsd = rmsd;
   a.amap = amap;

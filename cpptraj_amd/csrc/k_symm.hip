@@ -26,6 +26,7 @@
 // corrects the initial fit's moments for the atoms whose pairing changed;
 // without, it sums over every atom.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cfloat>
 #include <stdint.h>
 #include "rmsd_device.hpp"
@@ -407,11 +408,11 @@ __device__ __forceinline__ void assign_reg(const SymmArgs& a, int64_t f, const i
 // few size boundaries), its loads stay within one frame and the frame's
 // initial fit is one broadcast load.
 __global__ __launch_bounds__(SYM_THREADS) void k_symm_assign(SymmArgs a) {
-  const int64_t F = a.fs.nframes;
   const int64_t pair = (int64_t)blockIdx.x * SYM_THREADS + threadIdx.x;
-  if (pair >= F * a.ngroups) return;
-  const int64_t f = pair / a.ngroups;
-  const int g = (int)(pair - f * a.ngroups);
+  if (pair >= (a.f1 - a.f0) * a.ngroups) return;
+  const int64_t fl = pair / a.ngroups;  // frame within this launch
+  const int64_t f = a.f0 + fl;
+  const int g = (int)(pair - fl * a.ngroups);
   const int g0 = a.goff[g];
   const int n = a.goff[g + 1] - g0;
   const int* ga = a.gatoms + g0;
@@ -432,7 +433,7 @@ __global__ __launch_bounds__(SYM_THREADS) void k_symm_assign(SymmArgs a) {
   } else if (n == 4) {
     assign_reg<4>(a, f, ga, U, t, out);
   } else {
-    double* slab = a.big + f * a.bigstride + a.bigoff[g];
+    double* slab = a.big + fl * a.bigstride + a.bigoff[g];
     const Strided<1, double> M{slab};
     signed char* ws = reinterpret_cast<signed char*>(slab + n * n);
     const Strided<1, signed char> map{ws + 5 * n};
@@ -581,10 +582,20 @@ hipError_t rmsd_launch_symm(const SymmArgs& a, hipStream_t s) {
   const int64_t F = a.fs.nframes;
   if (F == 0) return hipSuccess;
   if (a.ngroups > 0) {
-    const int64_t pairs = F * a.ngroups;
-    const int64_t blocks = (pairs + SYM_THREADS - 1) / SYM_THREADS;
-    if (blocks > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_symm_assign, dim3((unsigned)blocks), dim3(SYM_THREADS), 0, s, a);
+    // Frames in chunks whose big-group scratch fits SYMM_BIG_MAX (one chunk
+    // when no group needs it); launches on one stream reuse it in order.
+    int64_t chunk = F;
+    if (a.bigstride > 0)
+      chunk = std::max<int64_t>(1, (int64_t)(SYMM_BIG_MAX / (sizeof(double) * a.bigstride)));
+    for (int64_t f0 = 0; f0 < F; f0 += chunk) {
+      SymmArgs b = a;
+      b.f0 = f0;
+      b.f1 = std::min<int64_t>(F, f0 + chunk);
+      const int64_t pairs = (b.f1 - b.f0) * a.ngroups;
+      const int64_t blocks = (pairs + SYM_THREADS - 1) / SYM_THREADS;
+      if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+      hipLaunchKernelGGL(k_symm_assign, dim3((unsigned)blocks), dim3(SYM_THREADS), 0, s, b);
+    }
   }
   const dim3 grid((unsigned)F), block(SYM_THREADS);
   if (a.fit && a.mass)

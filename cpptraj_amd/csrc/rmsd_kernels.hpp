@@ -276,7 +276,8 @@ struct SymmArgs {
   const int* pos;         // nsel
   const int* bigoff;      // per group: offset (doubles) of its working matrix
   int64_t bigstride;      // doubles per frame of big-group scratch
-  double* big;            // groups over SYMM_GROUP_REG atoms
+  double* big;            // groups over SYMM_GROUP_REG atoms, frames f0 .. f1
+  int64_t f0, f1;         // frames of this assignment launch
   int* gmap;              // F x ngatoms: target atom for each group entry
   double* rmsd;
   int32_t* amap;          // F x nsel or null
@@ -285,4 +286,5 @@ struct SymmArgs {
 };
 constexpr int SYMM_GROUP_REG = 4;    // largest group kept in registers
 constexpr int SYMM_GROUP_MAX = 32;   // largest group (bit-mask lines, byte indices)
+constexpr size_t SYMM_BIG_MAX = size_t(1) << 30;  // bytes of big-group scratch per launch
 hipError_t rmsd_launch_symm(const SymmArgs& a, hipStream_t s);

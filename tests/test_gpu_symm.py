@@ -188,3 +188,31 @@ def test_symm_argument_errors(ctx):
         ca.rmsd_symm(X, [[0, 1]], use_mass=True, ctx=ctx)
     rm, _, _, _ = ca.rmsd_symm(X[:0], [[0, 1]], ref_frames=X, ctx=ctx)
     assert rm.shape == (0,)
+
+
+def test_symm_big_group_scratch_in_frame_chunks(ctx):
+    # A 32-atom group needs 8.4 KB of working space per frame; 140,000 frames
+    # run the assignment in two chunks of bounded scratch.
+    import torch
+
+    import cpptraj_amd as ca
+    F, n = 140000, 40
+    X, ref, groups, m, _ = synth.symm_case(76, 1, n, sizes=(32,), ngroups=1)
+    rng = np.random.default_rng(76)
+    g = np.asarray(groups[0])
+    base = ref[None] + rng.normal(scale=0.2, size=(F, n, 3))
+    order = np.argsort(rng.random((F, len(g))), axis=1)
+    base[:, g] = base[np.arange(F)[:, None], g[order]]
+    Xf = base.reshape(F, -1).astype(np.float32)
+    Xd = torch.as_tensor(Xf).cuda()
+    R = torch.as_tensor(ref.reshape(1, -1).astype(np.float32)).cuda()
+    rm, amap, _, _ = ca.rmsd_symm(Xd, groups, ref_frames=R, want_map=True, ctx=ctx)
+    # Every frame's work is its own: 300 frames across both chunks, run alone
+    # (one chunk), give the same maps and RMSDs bit for bit.  (Against the
+    # oracle, a 32-atom crowded group pairs by the last bits of the rotation:
+    # test_symm_device_hungarian_on_crowded_groups checks that case.)
+    idx = np.linspace(0, F - 1, 300).astype(int)
+    rm1, amap1, _, _ = ca.rmsd_symm(torch.as_tensor(Xf[idx]).cuda(), groups, ref_frames=R,
+                                    want_map=True, ctx=ctx)
+    assert np.array_equal(amap.cpu().numpy()[idx], amap1.cpu().numpy())
+    assert np.array_equal(rm.cpu().numpy()[idx], rm1.cpu().numpy())
