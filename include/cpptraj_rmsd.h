@@ -268,7 +268,10 @@ int rmsd_perres_nofit(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frame
  *            translations in rmsd_fit_to_ref's layout, for rmsd_transform of
  *            the remapped frame (the action's Trans_Rot_Trans)
  * A column the heuristic leaves unassigned keeps its own atom (the reference
- * would index the group at -1). */
+ * would index the group at -1).  Groups of up to 4 atoms are solved in
+ * registers; a larger group of n atoms takes n*n + ceil(6n/8) doubles of
+ * context scratch per frame, and the frames are assigned in chunks that keep
+ * that scratch within 1 GiB. */
 int rmsd_symm(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* ref, int ref_frame,
               int ngroups, const int32_t* group_off, const int32_t* group_atoms, int fit,
               int use_mass, double* rmsd, int32_t* amap, double* U, double* tvec);
