@@ -856,7 +856,7 @@ int rmsd_symm(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* ref, 
   if (!rmsd && frames->nframes > 0) return fail(ctx, RMSD_ERR_ARG, "rmsd is NULL");
   const int n = frames->nsel;
   // Group table: offsets, atoms, each atom's entry, and the scratch offsets
-  // of groups too large for LDS.
+  // of groups too large for the register solver.
   std::vector<int32_t> pos((size_t)n, -1), bigoff((size_t)std::max(ngroups, 1), -1);
   int64_t bigstride = 0;
   if (ngroups > 0 && group_off[0] != 0)
