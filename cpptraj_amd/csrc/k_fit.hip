@@ -271,23 +271,33 @@ constexpr int WAVE2_FIT_THREADS = 512;
 
 // One frame per wave for AoS frames without a selection (cpptraj's own
 // layout): a lane takes four consecutive atoms, twelve floats, as three
-// 16-byte loads, so consecutive lanes read consecutive 48 bytes.
+// 16-byte loads, so consecutive lanes read consecutive 48 bytes.  The
+// reference sits in LDS as twelve arrays over atom quads (component d of atom
+// 4q + i at [(3i + d) Q + q]), masses as four, so each LDS read is contiguous
+// across the wave; a lane reading Y_d[4q + i] from the plain [3][ldr] table
+// would stride 32 bytes and conflict on the banks.  The last n mod 4 atoms
+// read the reference from global memory.
 template <bool WEIGHTED>
 __global__ __launch_bounds__(WAVE_FIT_THREADS) void k_fit_moments_aos4(FitArgs a, int ldr,
                                                                        double* shift) {
   extern __shared__ double sref[];
   const int n = a.fs.nsel;
-  for (int i = threadIdx.x; i < 3 * ldr; i += blockDim.x) sref[i] = a.ref[i];
+  const int Q = n >> 2;
+  for (int t = threadIdx.x; t < 12 * Q; t += blockDim.x) {
+    const int r = t / Q, q = t - r * Q;
+    const int i = r / 3, d = r - 3 * i;
+    sref[t] = a.ref[(size_t)d * ldr + 4 * q + i];
+  }
   if (WEIGHTED)
-    for (int i = threadIdx.x; i < ldr; i += blockDim.x) sref[3 * ldr + i] = i < n ? a.mass[i] : 0.0;
+    for (int t = threadIdx.x; t < 4 * Q; t += blockDim.x) {
+      const int i = t / Q, q = t - i * Q;
+      sref[12 * Q + t] = a.mass[4 * q + i];
+    }
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
-  const double* Y0 = sref;
-  const double* Y1 = sref + ldr;
-  const double* Y2 = sref + 2 * ldr;
-  const double* Wt = sref + 3 * ldr;
-  const int n4 = n >> 2;
+  const double* Yq = sref;
+  const double* Wq = sref + 12 * Q;
   for (int64_t f = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
        f < a.fs.nframes; f += nw) {
     const float* fr = a.fs.xyz + frame_of(a.fs, f) * a.fs.stride;
@@ -298,23 +308,23 @@ __global__ __launch_bounds__(WAVE_FIT_THREADS) void k_fit_moments_aos4(FitArgs a
     double (&m)[13] = *reinterpret_cast<double(*)[13]>(v);
     const f32x4* V = reinterpret_cast<const f32x4*>(fr);
 #pragma unroll 2
-    for (int q = lane; q < n4; q += 64) {
+    for (int q = lane; q < Q; q += 64) {
       const f32x4 p0 = __builtin_nontemporal_load(V + 3 * q);
       const f32x4 p1 = __builtin_nontemporal_load(V + 3 * q + 1);
       const f32x4 p2 = __builtin_nontemporal_load(V + 3 * q + 2);
-      const int k = 4 * q;
-      accum<WEIGHTED>(m, p0.x - s0, p0.y - s1, p0.z - s2, WEIGHTED ? Wt[k] : 1.0, Y0[k], Y1[k],
-                      Y2[k]);
-      accum<WEIGHTED>(m, p0.w - s0, p1.x - s1, p1.y - s2, WEIGHTED ? Wt[k + 1] : 1.0, Y0[k + 1],
-                      Y1[k + 1], Y2[k + 1]);
-      accum<WEIGHTED>(m, p1.z - s0, p1.w - s1, p2.x - s2, WEIGHTED ? Wt[k + 2] : 1.0, Y0[k + 2],
-                      Y1[k + 2], Y2[k + 2]);
-      accum<WEIGHTED>(m, p2.y - s0, p2.z - s1, p2.w - s2, WEIGHTED ? Wt[k + 3] : 1.0, Y0[k + 3],
-                      Y1[k + 3], Y2[k + 3]);
+      accum<WEIGHTED>(m, p0.x - s0, p0.y - s1, p0.z - s2, WEIGHTED ? Wq[q] : 1.0, Yq[q],
+                      Yq[Q + q], Yq[2 * Q + q]);
+      accum<WEIGHTED>(m, p0.w - s0, p1.x - s1, p1.y - s2, WEIGHTED ? Wq[Q + q] : 1.0,
+                      Yq[3 * Q + q], Yq[4 * Q + q], Yq[5 * Q + q]);
+      accum<WEIGHTED>(m, p1.z - s0, p1.w - s1, p2.x - s2, WEIGHTED ? Wq[2 * Q + q] : 1.0,
+                      Yq[6 * Q + q], Yq[7 * Q + q], Yq[8 * Q + q]);
+      accum<WEIGHTED>(m, p2.y - s0, p2.z - s1, p2.w - s2, WEIGHTED ? Wq[3 * Q + q] : 1.0,
+                      Yq[9 * Q + q], Yq[10 * Q + q], Yq[11 * Q + q]);
     }
-    for (int k = 4 * n4 + lane; k < n; k += 64)
+    for (int k = 4 * Q + lane; k < n; k += 64)
       accum<WEIGHTED>(m, (double)fr[3 * k] - s0, (double)fr[3 * k + 1] - s1,
-                      (double)fr[3 * k + 2] - s2, WEIGHTED ? Wt[k] : 1.0, Y0[k], Y1[k], Y2[k]);
+                      (double)fr[3 * k + 2] - s2, WEIGHTED ? a.mass[k] : 1.0, a.ref[k],
+                      a.ref[ldr + k], a.ref[2 * ldr + k]);
     const double t = rmsd_dev::wave_reduce16(v);
     const int idx = lane >> 2;
     if ((lane & 3) == 0 && idx < 13) a.moments[13 * f + idx] = t;
