@@ -36,10 +36,24 @@ namespace {
 
 constexpr int SYM_THREADS = 64;
 
-__device__ __forceinline__ double scoord(const FrameSetDev& fs, int64_t f, int k, int c) {
+// All three coordinates of selected atom k of frame f.  An AoS atom is read
+// as one 12-byte load: three 4-byte gathers would each look up the same
+// cache lines again.
+typedef float xyz3 __attribute__((ext_vector_type(3), aligned(4)));
+__device__ __forceinline__ void satom(const FrameSetDev& fs, int64_t f, int k, double& x,
+                                      double& y, double& z) {
   const int a = fs.sel ? fs.sel[k] : k;
   const float* fr = fs.xyz + frame_of(fs, f) * fs.stride;
-  return (double)(fs.ld ? fr[c * fs.ld + a] : fr[3 * a + c]);
+  if (fs.ld) {
+    x = fr[a];
+    y = fr[fs.ld + a];
+    z = fr[2 * fs.ld + a];
+  } else {
+    const xyz3 p = *reinterpret_cast<const xyz3*>(fr + 3 * a);
+    x = p.x;
+    y = p.y;
+    z = p.z;
+  }
 }
 
 // Hungarian::Optimize: row and column reduction, then at most n * n rounds of
@@ -371,8 +385,8 @@ __device__ __forceinline__ void group_costs(const SymmArgs& a, int64_t f, const 
                                             Strided<S, double> M) {
 #pragma clang fp contract(off)
   for (int ta = 0; ta < n; ++ta) {
-    double x = scoord(a.fs, f, ga[ta], 0), y = scoord(a.fs, f, ga[ta], 1),
-           z = scoord(a.fs, f, ga[ta], 2);
+    double x, y, z;
+    satom(a.fs, f, ga[ta], x, y, z);
     if (a.U0) {
       x += t[0];
       y += t[1];
@@ -470,10 +484,11 @@ __global__ __launch_bounds__(SYM_THREADS) void k_symm_final_fit(SymmArgs a) {
     const double wt = WEIGHTED ? a.mass[t] : 1.0;
     const double wi = WEIGHTED ? a.mass[i] : 1.0;
     const double y0 = a.ref[i], y1 = a.ref[a.ldr + i], y2 = a.ref[2 * a.ldr + i];
-    const double t0 = scoord(a.fs, f, t, 0) - s0, t1 = scoord(a.fs, f, t, 1) - s1,
-                 t2 = scoord(a.fs, f, t, 2) - s2;
-    const double i0 = scoord(a.fs, f, i, 0) - s0, i1 = scoord(a.fs, f, i, 1) - s1,
-                 i2 = scoord(a.fs, f, i, 2) - s2;
+    double xt0, xt1, xt2, xi0, xi1, xi2;
+    satom(a.fs, f, t, xt0, xt1, xt2);
+    satom(a.fs, f, i, xi0, xi1, xi2);
+    const double t0 = xt0 - s0, t1 = xt1 - s1, t2 = xt2 - s2;
+    const double i0 = xi0 - s0, i1 = xi1 - s1, i2 = xi2 - s2;
     const double e0 = wt * t0 - wi * i0, e1 = wt * t1 - wi * i1, e2 = wt * t2 - wi * i2;
     v[0] += e0;
     v[1] += e1;
@@ -548,9 +563,11 @@ __global__ __launch_bounds__(SYM_THREADS) void k_symm_final_nofit(SymmArgs a) {
     const int p = a.pos[i];
     const int t = p >= 0 ? gm[p] : i;
     const double w = WEIGHTED ? a.mass[t] : 1.0;
-    const double dx = a.ref[i] - scoord(a.fs, f, t, 0);
-    const double dy = a.ref[a.ldr + i] - scoord(a.fs, f, t, 1);
-    const double dz = a.ref[2 * a.ldr + i] - scoord(a.fs, f, t, 2);
+    double xt0, xt1, xt2;
+    satom(a.fs, f, t, xt0, xt1, xt2);
+    const double dx = a.ref[i] - xt0;
+    const double dy = a.ref[a.ldr + i] - xt1;
+    const double dz = a.ref[2 * a.ldr + i] - xt2;
     W += w;
     acc += (w * (dx * dx + dy * dy + dz * dz));
   }
