@@ -180,9 +180,10 @@ __global__ __launch_bounds__(FIT_THREADS) void k_fit_moments_any(FitArgs a, int 
   for (int i = 0; i < 13; ++i) v[i] = 0.0;
   for (int k = threadIdx.x; k < a.fs.nsel; k += blockDim.x) {
     const double w = WEIGHTED ? a.mass[k] : 1.0;
-    accum<WEIGHTED>(v, (double)fcoord(a.fs, f, k, 0) - s0, (double)fcoord(a.fs, f, k, 1) - s1,
-                    (double)fcoord(a.fs, f, k, 2) - s2, w, a.ref[k], a.ref[ldr + k],
-                    a.ref[2 * ldr + k]);
+    float p[3];
+    load_atom(a.fs, f, k, p);
+    accum<WEIGHTED>(v, (double)p[0] - s0, (double)p[1] - s1, (double)p[2] - s2, w, a.ref[k],
+                    a.ref[ldr + k], a.ref[2 * ldr + k]);
   }
   rmsd_dev::block_sum<13>(v, red);
   if (threadIdx.x == 0)
@@ -249,9 +250,10 @@ __global__ __launch_bounds__(WAVE_FIT_THREADS) void k_fit_moments_wave(FitArgs a
     }
 #pragma unroll 4
     for (int k = k0 + lane; k < n; k += 64) {
-      accum<WEIGHTED>(m, (double)fcoord(a.fs, f, k, 0) - s0, (double)fcoord(a.fs, f, k, 1) - s1,
-                      (double)fcoord(a.fs, f, k, 2) - s2, WEIGHTED ? Wt[k] : 1.0, Y0[k], Y1[k],
-                      Y2[k]);
+      float p[3];
+      load_atom(a.fs, f, k, p);
+      accum<WEIGHTED>(m, (double)p[0] - s0, (double)p[1] - s1, (double)p[2] - s2,
+                      WEIGHTED ? Wt[k] : 1.0, Y0[k], Y1[k], Y2[k]);
     }
     const double t = rmsd_dev::wave_reduce16(v);
     const int idx = lane >> 2;
@@ -497,9 +499,11 @@ __global__ __launch_bounds__(FIT_THREADS) void k_nofit_frames(NofitArgs a, int l
   const int64_t f = blockIdx.x;
   double v[1] = {0.0};
   for (int k = threadIdx.x; k < a.fs.nsel; k += blockDim.x) {
-    const double dx = a.ref[k] - (double)fcoord(a.fs, f, k, 0);
-    const double dy = a.ref[ldr + k] - (double)fcoord(a.fs, f, k, 1);
-    const double dz = a.ref[2 * ldr + k] - (double)fcoord(a.fs, f, k, 2);
+    float p[3];
+    load_atom(a.fs, f, k, p);
+    const double dx = a.ref[k] - (double)p[0];
+    const double dy = a.ref[ldr + k] - (double)p[1];
+    const double dz = a.ref[2 * ldr + k] - (double)p[2];
     const double w = WEIGHTED ? a.mass[k] : 1.0;
     v[0] += (w * (dx * dx + dy * dy + dz * dz));
   }

@@ -28,12 +28,6 @@ namespace {
 
 constexpr int PREP_THREADS = 64;
 
-__device__ __forceinline__ float load_coord(const FrameSetDev& fs, int64_t f, int k, int c) {
-  const int a = fs.sel ? fs.sel[k] : k;
-  const float* fr = fs.xyz + frame_of(fs, f) * fs.stride;
-  return fs.ld ? fr[c * fs.ld + a] : fr[3 * a + c];
-}
-
 template <int PREC>
 __device__ __forceinline__ int64_t packed_elem(int64_t f, int k, int c, int p, int KB) {
   constexpr int CH = chunk_atoms(PREC);
@@ -54,9 +48,11 @@ __device__ __forceinline__ void centroid(const FrameSetDev& fs, int64_t f, const
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   for (int k = threadIdx.x; k < fs.nsel; k += blockDim.x) {
     const double m = w ? w[k] : 1.0;
-    v[0] += m * (double)load_coord(fs, f, k, 0);
-    v[1] += m * (double)load_coord(fs, f, k, 1);
-    v[2] += m * (double)load_coord(fs, f, k, 2);
+    float p[3];
+    load_atom(fs, f, k, p);
+    v[0] += m * (double)p[0];
+    v[1] += m * (double)p[1];
+    v[2] += m * (double)p[2];
     v[3] += m;
   }
   rmsd_dev::block_sum<4>(v, red);
@@ -71,9 +67,11 @@ __device__ __forceinline__ void centred(const FrameSetDev& fs, int64_t f, int k,
                                         const double* sw, double x[3]) {
   if (k < fs.nsel) {
     const double s = sw ? sqrt(sw[k]) : 1.0;
-    x[0] = ((double)load_coord(fs, f, k, 0) - cen[0]) * s;
-    x[1] = ((double)load_coord(fs, f, k, 1) - cen[1]) * s;
-    x[2] = ((double)load_coord(fs, f, k, 2) - cen[2]) * s;
+    float p[3];
+    load_atom(fs, f, k, p);
+    x[0] = ((double)p[0] - cen[0]) * s;
+    x[1] = ((double)p[1] - cen[1]) * s;
+    x[2] = ((double)p[2] - cen[2]) * s;
   } else {
     x[0] = x[1] = x[2] = 0.0;
   }

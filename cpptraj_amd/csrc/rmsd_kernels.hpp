@@ -32,6 +32,24 @@ __host__ __device__ inline int64_t frame_of(const FrameSetDev& fs, int64_t f) {
   return fs.fidx ? (int64_t)fs.fidx[f] : f;
 }
 
+// All three coordinates of selected atom k of frame f.  An AoS atom is one
+// 12-byte load: three 4-byte gathers would look its cache lines up three times.
+typedef float f32x3_a4 __attribute__((ext_vector_type(3), aligned(4)));
+__device__ __forceinline__ void load_atom(const FrameSetDev& fs, int64_t f, int k, float (&p)[3]) {
+  const int a = fs.sel ? fs.sel[k] : k;
+  const float* fr = fs.xyz + frame_of(fs, f) * fs.stride;
+  if (fs.ld) {
+    p[0] = fr[a];
+    p[1] = fr[fs.ld + a];
+    p[2] = fr[2 * fs.ld + a];
+  } else {
+    const f32x3_a4 q = *reinterpret_cast<const f32x3_a4*>(fr + 3 * a);
+    p[0] = q.x;
+    p[1] = q.y;
+    p[2] = q.z;
+  }
+}
+
 // Per-frame preparation for the pairwise contraction (k_prep.hip).
 struct PrepArgs {
   FrameSetDev fs;
