@@ -24,12 +24,6 @@
 
 namespace {
 
-__device__ __forceinline__ float gcoord(const FrameSetDev& fs, int64_t f, int k, int c) {
-  const int a = fs.sel ? fs.sel[k] : k;
-  const float* fr = fs.xyz + frame_of(fs, f) * fs.stride;
-  return fs.ld ? fr[c * fs.ld + a] : fr[3 * a + c];
-}
-
 // xform: 15 doubles per frame (U, t1 = -centroid, t2) from k_fit_solve.
 __global__ __launch_bounds__(256) void k_fit_grad(FrameSetDev fs, const double* mass, const double* ref,
                                                   int ldr, const double* xform, const double* rmsd,
@@ -47,9 +41,11 @@ __global__ __launch_bounds__(256) void k_fit_grad(FrameSetDev fs, const double* 
     return;
   }
   const double s = gout[f] * (mass ? mass[k] : 1.0) / (W * r);
-  const double x0 = (double)gcoord(fs, f, k, 0) + X[9];
-  const double x1 = (double)gcoord(fs, f, k, 1) + X[10];
-  const double x2 = (double)gcoord(fs, f, k, 2) + X[11];
+  float p[3];
+  load_atom(fs, f, k, p);
+  const double x0 = (double)p[0] + X[9];
+  const double x1 = (double)p[1] + X[10];
+  const double x2 = (double)p[2] + X[11];
   const double y0 = ref[k], y1 = ref[ldr + k], y2 = ref[2 * ldr + k];
   // U^T y
   const double u0 = X[0] * y0 + X[3] * y1 + X[6] * y2;
@@ -69,9 +65,11 @@ __global__ __launch_bounds__(256) void k_center(FrameSetDev fs, const double* w,
   double v[4] = {0, 0, 0, 0};
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
     const double m = w ? w[k] : 1.0;
-    v[0] += m * (double)gcoord(fs, f, k, 0);
-    v[1] += m * (double)gcoord(fs, f, k, 1);
-    v[2] += m * (double)gcoord(fs, f, k, 2);
+    float p[3];
+    load_atom(fs, f, k, p);
+    v[0] += m * (double)p[0];
+    v[1] += m * (double)p[1];
+    v[2] += m * (double)p[2];
     v[3] += m;
   }
   rmsd_dev::block_sum<4>(v, red);
@@ -81,9 +79,11 @@ __global__ __launch_bounds__(256) void k_center(FrameSetDev fs, const double* w,
   double e[1] = {0.0};
   double* out = xc + f * 3 * n;
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
-    const double x0 = (double)gcoord(fs, f, k, 0) - c0;
-    const double x1 = (double)gcoord(fs, f, k, 1) - c1;
-    const double x2 = (double)gcoord(fs, f, k, 2) - c2;
+    float p[3];
+    load_atom(fs, f, k, p);
+    const double x0 = (double)p[0] - c0;
+    const double x1 = (double)p[1] - c1;
+    const double x2 = (double)p[2] - c2;
     out[k] = x0;
     out[n + k] = x1;
     out[2 * n + k] = x2;
