@@ -19,12 +19,6 @@ namespace {
 constexpr int NF_TILE = 64;
 constexpr int NF_KS = 16;
 
-__device__ __forceinline__ float ncoord(const FrameSetDev& fs, int64_t f, int k, int c) {
-  const int a = fs.sel ? fs.sel[k] : k;
-  const float* fr = fs.xyz + frame_of(fs, f) * fs.stride;
-  return fs.ld ? fr[c * fs.ld + a] : fr[3 * a + c];
-}
-
 __global__ __launch_bounds__(256) void k_pair_nofit(FrameSetDev A, FrameSetDev B,
                                                     const double* mass, const double* total_mass,
                                                     const int2* tiles, int ntiles, int per_xcd,
@@ -53,10 +47,13 @@ __global__ __launch_bounds__(256) void k_pair_nofit(FrameSetDev A, FrameSetDev B
       const int k = k0 + kk;
       const int64_t fi = i0 + fr, fj = j0 + fr;
       const bool kin = k < n;
+      float pa[3] = {0.0f, 0.0f, 0.0f}, pb[3] = {0.0f, 0.0f, 0.0f};
+      if (kin && fi < A.nframes) load_atom(A, fi, k, pa);
+      if (kin && fj < B.nframes) load_atom(B, fj, k, pb);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        sA[kk][c][fr] = (kin && fi < A.nframes) ? (double)ncoord(A, fi, k, c) : 0.0;
-        sB[kk][c][fr] = (kin && fj < B.nframes) ? (double)ncoord(B, fj, k, c) : 0.0;
+        sA[kk][c][fr] = (double)pa[c];
+        sB[kk][c][fr] = (double)pb[c];
       }
     }
     if (threadIdx.x < NF_KS) {
