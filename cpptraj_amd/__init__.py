@@ -21,13 +21,12 @@ import ctypes as C
 import os
 import weakref
 
-__all__ = ["RmsdError", "Context", "Batch", "frames_desc", "rms2d", "rms2d_batch",
-           "cluster_cache", "rmsd_fit", "library_path", "load_library", "MODES", "PRECISIONS",
+__all__ = ["RmsdError", "Context", "Batch", "frames_desc", "frame_layout", "rms2d",
+           "rms2d_batch", "cluster_cache", "rmsd_fit", "library_path", "load_library", "MODES", "PRECISIONS",
            "MODIFY", "BATCH_GRAPH", "BATCH_AUTOTUNE"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-# CPPTRAJ_RMSD_LIB selects another build of the same library (e.g. a tuned variant).
-_LIB_PATH = os.environ.get("CPPTRAJ_RMSD_LIB") or os.path.join(_HERE, "_lib", "libcpptraj_rmsd.so")
+_LIB_PATH = os.path.join(_HERE, "_lib", "libcpptraj_rmsd.so")
 
 MODES = {"fit": 0, "nofit": 1, "qrmsd": 2}
 PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2}
@@ -166,35 +165,58 @@ def frames_desc(xyz, sel=None, mass=None, fidx=None, soa=False, nsel=None, natom
 
     if not xyz.is_cuda or xyz.dtype != torch.float32:
         raise RmsdError("frames must be a float32 CUDA tensor")
-    F = xyz.shape[0]
-    if soa:
-        empty = xyz.numel() == 0
-        if xyz.dim() != 3 or xyz.shape[1] != 3 or (
-                not empty and (xyz.stride(2) != 1 or xyz.stride(1) < 1)):
-            raise RmsdError("SoA frames must be (F, 3, ld) with unit atom stride")
-        ld = xyz.stride(1) if not empty else max(xyz.shape[2], 1)
-        natom = xyz.shape[2] if natom is None else natom
-        stride = xyz.stride(0) if F > 1 else 3 * ld
-    else:
-        x = xyz.reshape(F, -1) if xyz.dim() == 3 else xyz
-        if x.stride(-1) != 1 and x.numel() > 0:
-            raise RmsdError("AoS frames need unit coordinate stride")
-        ld = 0
-        if natom is None:
-            natom = x.shape[1] // 3
-        elif 3 * natom > x.shape[1]:
-            raise RmsdError(f"{natom} atoms do not fit AoS rows of {x.shape[1]} floats")
-        stride = x.stride(0) if F > 1 else 3 * natom
+    stride, ld, natom = frame_layout(xyz, soa, natom)
     for t, dt in ((sel, torch.int32), (mass, torch.float64), (fidx, torch.int32)):
         if t is not None and (not t.is_cuda or t.dtype != dt or not t.is_contiguous()):
             raise RmsdError("sel/fidx must be contiguous int32 and mass float64 CUDA tensors")
     if nsel is None:
         nsel = sel.numel() if sel is not None else natom
-    nfr = fidx.numel() if fidx is not None else F
+    nfr = fidx.numel() if fidx is not None else xyz.shape[0]
     d = Frames(xyz.data_ptr(), stride, ld, nfr, natom, None if sel is None else sel.data_ptr(),
                nsel, None if mass is None else mass.data_ptr(),
                None if fidx is None else fidx.data_ptr())
     return d
+
+
+def frame_layout(xyz, soa=False, natom=None):
+    """(frame stride, ld, natom) of a frame tensor as the library will read it.
+
+    The library reads the tensor's own storage through (data_ptr, stride, ld),
+    so the layout is checked on xyz itself: a view whose rows overlap or
+    whose atoms are not packed is rejected, never silently copied."""
+    F = xyz.shape[0]
+    empty = xyz.numel() == 0
+    if soa:
+        if xyz.dim() != 3 or xyz.shape[1] != 3 or (
+                not empty and (xyz.stride(2) != 1 or xyz.stride(1) < xyz.shape[2]
+                               or (F > 1 and xyz.stride(0) < 3 * xyz.stride(1)))):
+            raise RmsdError("SoA frames must be (F, 3, ld) with unit atom stride, "
+                            "ld >= atoms and non-overlapping frames")
+        ld = xyz.stride(1) if not empty else max(xyz.shape[2], 1)
+        natom = xyz.shape[2] if natom is None else natom
+        stride = xyz.stride(0) if F > 1 else 3 * ld
+    else:
+        if xyz.dim() == 3:
+            if xyz.shape[2] != 3:
+                raise RmsdError("AoS frames must be (F, natom*3) or (F, natom, 3)")
+            if not empty and (xyz.stride(2) != 1 or (xyz.shape[1] > 1 and xyz.stride(1) != 3)):
+                raise RmsdError("AoS frames (F, natom, 3) need packed atoms (strides (., 3, 1))")
+            row = 3 * xyz.shape[1]
+        elif xyz.dim() == 2:
+            if not empty and xyz.shape[1] > 1 and xyz.stride(1) != 1:
+                raise RmsdError("AoS frames need unit coordinate stride")
+            row = xyz.shape[1]
+        else:
+            raise RmsdError("AoS frames must be (F, natom*3) or (F, natom, 3)")
+        ld = 0
+        if natom is None:
+            natom = row // 3
+        elif 3 * natom > row:
+            raise RmsdError(f"{natom} atoms do not fit AoS rows of {row} floats")
+        if not empty and F > 1 and xyz.stride(0) < row:
+            raise RmsdError("AoS frames overlap (frame stride below the row length)")
+        stride = xyz.stride(0) if F > 1 else 3 * natom
+    return stride, ld, natom
 
 
 class Context:
@@ -279,10 +301,20 @@ _default_ctx = {}
 
 
 def _ctx(device=0) -> Context:
-    c = _default_ctx.get(device)
+    """The default context for `device` on torch's current stream there.
+
+    A context enqueues its work on the stream it was made with, so the cache is
+    keyed on (device, stream): a call made under another torch stream gets a
+    context on that stream and stays ordered with the work that produced its
+    inputs and will consume its outputs."""
+    import torch
+
+    stream = torch.cuda.current_stream(device)
+    key = (device, stream.cuda_stream)
+    c = _default_ctx.get(key)
     if c is None:
-        c = Context(device)
-        _default_ctx[device] = c
+        c = Context(device, stream)
+        _default_ctx[key] = c
     return c
 
 

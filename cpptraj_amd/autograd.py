@@ -16,15 +16,12 @@ import ctypes as C
 import torch
 
 from . import Context, RmsdError, frames_desc, load_library
-
-_ctx_cache = {}
+from . import _ctx as _device_ctx
 
 
 def _ctx(dev) -> Context:
-    i = dev.index or 0
-    if i not in _ctx_cache:
-        _ctx_cache[i] = Context(i)
-    return _ctx_cache[i]
+    """The default context of `dev` on torch's current stream there."""
+    return _device_ctx(dev.index or 0)
 
 
 def fit_backward(frames, ref, grad_rmsd, mass=None, use_mass=False, ctx: Context | None = None):
@@ -65,6 +62,11 @@ class RmsdToReference(torch.autograd.Function):
         import cpptraj_amd as ca
         if frames.dim() != 3 or ref.dim() != 3 or ref.shape[0] != 1:
             raise RmsdError("frames (F, n, 3) and ref (1, n, 3) expected")
+        if ref.requires_grad:
+            # rmsd_fit_backward differentiates the frames only; a silent None
+            # for the reference would read as a zero gradient.
+            raise RmsdError("RmsdToReference has no gradient for the reference: "
+                            "pass ref.detach(), or use PairwiseRmsd for both sides")
         ctx = _ctx(frames.device)
         fc, rc = frames.contiguous(), ref.contiguous()
         d = frames_desc(fc, None, mass)

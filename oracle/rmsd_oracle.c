@@ -318,7 +318,32 @@ double orc_rmsd_centered_ref(double *tgt, const double *ref, const double *mass,
  * eigenvector from adjugate columns with the reference's fallbacks (norm^2 <
  * 1e-6) down to the identity.  rot (if not NULL) receives the rotation in the
  * reference's QCP layout, which is the transpose of the Kabsch U.
- * Returns -1 when rot is NULL, 0 for the identity fallback, 1 otherwise.     */
+ * Returns -1 when rot is NULL, 0 for the identity fallback, 1 otherwise.
+ *
+ * qcprot (version 1.5) is distributed under this notice, which applies to the
+ * restated algorithm:
+ *   Copyright (c) 2009-2016 Pu Liu and Douglas L. Theobald.  All rights reserved.
+ *   Redistribution and use in source and binary forms, with or without
+ *   modification, are permitted provided that the following conditions are met:
+ *   * Redistributions of source code must retain the above copyright notice,
+ *     this list of conditions and the following disclaimer.
+ *   * Redistributions in binary form must reproduce the above copyright notice,
+ *     this list of conditions and the following disclaimer in the documentation
+ *     and/or other materials provided with the distribution.
+ *   * Neither the name of the <ORGANIZATION> nor the names of its contributors
+ *     may be used to endorse or promote products derived from this software
+ *     without specific prior written permission.
+ *   THIS SOFTWARE IS PROVIDED BY THE COPYRIGHT HOLDERS AND CONTRIBUTORS "AS IS"
+ *   AND ANY EXPRESS OR IMPLIED WARRANTIES, INCLUDING, BUT NOT LIMITED TO, THE
+ *   IMPLIED WARRANTIES OF MERCHANTABILITY AND FITNESS FOR A PARTICULAR PURPOSE
+ *   ARE DISCLAIMED. IN NO EVENT SHALL THE COPYRIGHT HOLDER OR CONTRIBUTORS BE
+ *   LIABLE FOR ANY DIRECT, INDIRECT, INCIDENTAL, SPECIAL, EXEMPLARY, OR
+ *   CONSEQUENTIAL DAMAGES (INCLUDING, BUT NOT LIMITED TO, PROCUREMENT OF
+ *   SUBSTITUTE GOODS OR SERVICES; LOSS OF USE, DATA, OR PROFITS; OR BUSINESS
+ *   INTERRUPTION) HOWEVER CAUSED AND ON ANY THEORY OF LIABILITY, WHETHER IN
+ *   CONTRACT, STRICT LIABILITY, OR TORT (INCLUDING NEGLIGENCE OR OTHERWISE)
+ *   ARISING IN ANY WAY OUT OF THE USE OF THIS SOFTWARE, EVEN IF ADVISED OF THE
+ *   POSSIBILITY OF SUCH DAMAGE.                                                */
 int orc_qcp(double *rot, const double A[9], double *rmsd, double E0, double len)
 {
   const double Sxx = A[0], Sxy = A[1], Sxz = A[2];

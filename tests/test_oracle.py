@@ -122,6 +122,75 @@ def test_rms2d_loop_bitexact(oracle, reflib):
     assert a.shape == (1600,) and same(a, b)
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("use_mass", [False, True])
+def test_rms2d_rows_bitexact(oracle, reflib, mode, use_mass):
+    """orc_rms2d_rows against the compiled row loop of Calculate_2D
+    (src/Analysis_Rms2d.cpp:244-287; cref_rms2d_rows) on random rows, a random
+    selection and masses; and against the same rows of the full triangle."""
+    rng = np.random.default_rng(40 + mode + 3 * use_mass)
+    F, n = 37, 45
+    X = synth.noisy(11 + mode, F, n, 0.8).reshape(F, -1)
+    sel = np.array(sorted(rng.choice(n, 29, replace=False)), dtype=np.int32)
+    m = synth.masses(rng, 29)
+    rows = np.array(sorted(rng.choice(F - 1, 9, replace=False)) + [F - 1], dtype=np.int32)
+    rng.shuffle(rows)
+    a = oracle.rms2d_rows(X, rows, sel, m, mode, use_mass)
+    b = reflib.rms2d_rows(X, rows, sel, m, mode, use_mass)
+    assert a.shape == b.shape == (sum(F - 1 - int(r) for r in rows),)
+    assert same(a, b)
+    tri = oracle.rms2d(X, sel, m, mode, use_mass)
+    pieces = [tri[oracle.tri_index(F, int(r), int(r) + 1):
+                  oracle.tri_index(F, int(r), int(r) + 1) + F - 1 - int(r)]
+              if r < F - 1 else np.zeros(0, np.float32) for r in rows]
+    assert same(a, np.concatenate(pieces))
+
+
+@pytest.mark.parametrize("nofit", [False, True])
+@pytest.mark.parametrize("use_mass", [False, True])
+def test_cluster_cache_bitexact(oracle, reflib, nofit, use_mass):
+    """orc_cluster_cache against a cache built pair by pair from the compiled
+    Frame::RMSD / Frame::RMSD_NoFit the way MetricArray::calcFrameDistances
+    (src/Cluster/MetricArray.cpp:766-801) calls Metric_RMS::FrameDist
+    (src/Cluster/Metric_RMS.cpp:44-63): frame framesToCache[f1] is the target,
+    framesToCache[f2] the re-centred reference, times the metric weight
+    (Dist_Manhattan, :539-547), stored as float (Matrix<float>::setElement)."""
+    rng = np.random.default_rng(50 + 2 * nofit + use_mass)
+    F, n = 30, 40
+    X = synth.noisy(21, F, n, 0.9).reshape(F, -1)
+    sel = np.array(sorted(rng.choice(n, 31, replace=False)), dtype=np.int32)
+    m = synth.masses(rng, 31)
+    fidx = np.arange(1, F, 3, dtype=np.int32)         # a regular sieve, offset 1
+    weight = 0.75
+    K = len(fidx)
+    exp = np.zeros(K * (K - 1) // 2, np.float32)
+    sel_xyz = X.reshape(F, n, 3)[:, sel].astype(np.float64)
+    for f1 in range(K - 1):
+        for f2 in range(f1 + 1, K):
+            A, B = sel_xyz[fidx[f1]], sel_xyz[fidx[f2]]
+            if nofit:
+                v = reflib.rmsd_nofit(A, B, m, use_mass)
+            else:
+                v = reflib.rmsd(A, B, m, m, use_mass)[0]
+            exp[oracle.tri_index(K, f1, f2)] = np.float32(weight * v)
+    got = oracle.cluster_cache(X, fidx, sel, m, nofit, use_mass, weight)
+    assert same(got, exp)
+
+
+def test_cluster_cache_identity_list_is_rms2d_triangle(oracle):
+    """With every frame cached and weight 1 the cache is the rms2d triangle
+    with target and reference swapped: the same RMSD to FP64 rounding, so the
+    stored floats differ by at most one unit in the last place."""
+    F, n = 33, 50
+    X = synth.noisy(23, F, n, 0.6).reshape(F, -1)
+    for nofit in (False, True):
+        c = oracle.cluster_cache(X, None, nofit=nofit).astype(np.float64)
+        t = oracle.rms2d(X, mode=1 if nofit else 0).astype(np.float64)
+        ulp = np.spacing(np.maximum(np.abs(c), np.abs(t)).astype(np.float32))
+        assert np.all(np.abs(c - t) <= ulp)
+        assert np.abs(c - t).max() < 1e-6
+
+
 def test_fit_frames_bitexact(oracle, reflib):
     X = synth.noisy(6, 20, 60, 1.0).reshape(20, -1)
     rng = np.random.default_rng(15)
