@@ -26,6 +26,11 @@ import os
 import sys
 import time
 
+# The OpenMP CPU baselines run with their threads bound to consecutive cores;
+# the runtime reads this when it starts, so it is set before anything loads it.
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -57,17 +62,55 @@ def timed_steps(fn, steps, warmup):
     return start.elapsed_time(stop) / steps  # ms per step
 
 
-def profiled_main_ms(ctx, fn, reps):
-    """Average (per-frame stage, main kernel) ms over `reps` profiled calls."""
+def profiled_main_ms(ctx, fn, reps, want_post=False):
+    """Average (per-frame stage, main kernel[, last stage]) ms over `reps`
+    profiled calls."""
     ctx.set_profiling(True)
-    pre, main = [], []
+    pre, main, post = [], [], []
     for _ in range(reps):
         fn()
         s = ctx.stage_ms()
         pre.append(s[0])
         main.append(s[1])
+        post.append(s[2])
     ctx.set_profiling(False)
+    if want_post:
+        return float(np.mean(pre)), float(np.mean(main)), float(np.mean(post))
     return float(np.mean(pre)), float(np.mean(main))
+
+
+def host_cpu_info():
+    """CPU model, the cores this process may run on (affinity) and the cgroup
+    CPU quota in cores (None when unlimited)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    quota = None
+    for path in ("/sys/fs/cgroup/cpu.max",):
+        try:
+            q, per = open(path).read().split()[:2]
+            if q != "max":
+                quota = int(q) / int(per)
+        except (OSError, ValueError):
+            pass
+    return {"model": model, "affinity_cores": len(os.sched_getaffinity(0)),
+            "cgroup_quota_cores": quota}
+
+
+def baseline_threads():
+    """Threads for the OpenMP CPU baselines: every core this process may use,
+    capped by the cgroup quota (a GPU box grants a share of its host)."""
+    info = host_cpu_info()
+    n = info["affinity_cores"]
+    if info["cgroup_quota_cores"]:
+        n = min(n, max(1, int(info["cgroup_quota_cores"])))
+    return n
 
 
 def profiled_traffic(kernel, workload):
@@ -164,10 +207,48 @@ def fit_stream(ctx, F, n, steps, warmup):
     err[0] = 0.0 if abs(got[0]) < 1e-5 else err[0]  # frame 0 fitted to itself: RMSD ~ 0
     out["max_err_vs_oracle_64_frames"] = float(err.max())
     out["workload"] = f"{F} frames x {n} atoms FP32 SoA, fit to frame 0, FP64 QCP, nomod"
+    out["cpu_baseline"] = fit_cpu_baseline(X, rc, rt)
     out["perres"] = perres_bench(ctx, X, d, m, steps, warmup)
     del X
     torch.cuda.empty_cache()
     return out
+
+
+def fit_cpu_baseline(X, ref_centered, ref_trans, target_s=6.0):
+    """The reference's own per-frame body of Action_Rmsd::DoAction (compiled
+    Frame::RMSD_CenteredRef, unweighted, nomod) over a sample of the fit
+    workload's frames: serially, as the action runs, and with OpenMP over
+    frames (a stronger baseline the reference does not have)."""
+    from oracle.oracle import RefLib, ref_available
+    if not ref_available():
+        return None
+    lib = RefLib()
+    F = X.shape[0]
+
+    def sample(k):
+        idx = np.linspace(0, F - 1, k).astype(np.int64)
+        return np.transpose(X[idx].cpu().numpy(), (0, 2, 1)).reshape(k, -1)
+
+    probe = sample(256)
+    t0 = time.perf_counter()
+    lib.fit_frames(probe, ref_centered, ref_trans, nthreads=1)
+    rate = 256 / (time.perf_counter() - t0)
+    k = int(min(F, max(256, target_s * rate)))
+    S = sample(k)
+    t0 = time.perf_counter()
+    lib.fit_frames(S, ref_centered, ref_trans, nthreads=1)
+    serial = k / (time.perf_counter() - t0)
+    nt = baseline_threads()
+    t0 = time.perf_counter()
+    lib.fit_frames(S, ref_centered, ref_trans, nthreads=nt)
+    dt = time.perf_counter() - t0
+    info = host_cpu_info()
+    return {"value": serial, "unit": "frames/s", "cores": 1, "kind": "reference",
+            "sample": f"{k} evenly spaced frames of the workload, serial per frame as "
+                      "Action_Rmsd::DoAction runs",
+            "openmp_over_frames": {"value": k / dt, "unit": "frames/s", "cores": nt,
+                                   "omp_proc_bind": os.environ.get("OMP_PROC_BIND")},
+            "model": info["model"]}
 
 
 def perres_bench(ctx, X, d, m, steps, warmup, res_atoms=15):
@@ -295,14 +376,18 @@ def avgcorr_bench(ctx, F=10000, n=300, steps=3, warmup=1):
     # first 64 window sizes.
     from oracle.oracle import RefLib, ref_available
     if ref_available():
-        ws = w[:64]
+        # Evenly spaced window sizes, so that the sample's cost per window
+        # matches the whole run's; OpenMP over windows as the reference builds it.
+        nt = baseline_threads()
+        ws = w[np.linspace(0, len(w) - 1, 16 * nt).astype(np.int64)]
         t0 = time.perf_counter()
-        RefLib().avgcorr(X, ws, ref_sel_centered=rc)
+        RefLib().avgcorr(X, ws, ref_sel_centered=rc, nthreads=nt)
         dt = time.perf_counter() - t0
         nfit = float(np.sum(F - ws + 1))
-        res["cpu_baseline"] = {"value": nfit / dt, "unit": "averaged frames/s", "cores": 1,
+        res["cpu_baseline"] = {"value": nfit / dt, "unit": "averaged frames/s", "cores": nt,
                                "kind": "reference",
-                               "sample": f"window sizes {int(ws[0])}..{int(ws[-1])} ({dt:.1f} s)"}
+                               "sample": f"{len(ws)} evenly spaced window sizes, OpenMP dynamic "
+                                         f"over windows ({dt:.1f} s)"}
     return res
 
 
@@ -563,11 +648,14 @@ def main():
         steps = args.fp64_steps if prec == "fp64" else args.steps
         warm = 1 if prec == "fp64" else args.warmup
         ms = timed_steps(fn, steps, warm)
-        pre, main = profiled_main_ms(ctx, fn, 2 if prec == "fp64" else 3)
+        pre, main, post = profiled_main_ms(ctx, fn, 2 if prec == "fp64" else 3, want_post=True)
+        bound, nfp64 = ctx.error_report()
         res[prec] = {"ms_per_step": ms, "pairs_per_s": npairs / (ms / 1e3),
-                     "prep_ms": pre, "contraction_ms": main}
+                     "prep_ms": pre, "contraction_ms": main, "fp64_recompute_ms": post,
+                     "stated_max_bound": bound, "pairs_recomputed_fp64": nfp64}
         log(f"{prec}: {ms:.2f} ms/step, {npairs / (ms / 1e3):.4g} pairs/s "
-            f"(prep {pre:.2f} ms, contraction {main:.2f} ms)")
+            f"(prep {pre:.2f} ms, contraction {main:.2f} ms, FP64 recompute {post:.2f} ms of "
+            f"{nfp64} pairs; largest stated bound {bound:.3e} A)")
 
     head = args.only or "bf16x3"
     flops_pair = 18.0 * n
@@ -616,16 +704,18 @@ def main():
         o = Oracle()
         rows = np.linspace(0, F - 2, 8).astype(np.int32)
         Xh = X.cpu().numpy()
-        exp = o.rms2d_rows(Xh, rows, nthreads=min(16, os.cpu_count() or 1))
+        exp = o.rms2d_rows(Xh, rows, nthreads=baseline_threads())
         starts = [int(r) * F - int(r) * (int(r) + 1) // 2 for r in rows]
         got = np.concatenate([outs["fp64"][s:s + F - 1 - int(r)].cpu().numpy()
                               for s, r in zip(starts, rows)])
         line["fp64_max_err_vs_oracle_sampled_rows"] = float(np.abs(got.astype(np.float64) - exp).max())
         line["fp64_oracle_sample"] = f"rows {rows.tolist()} ({len(exp)} pairs)"
     if not args.skip_cpu:
-        n_threads = min(16, os.cpu_count() or 1)
+        n_threads = baseline_threads()
         log(f"cpu baseline on {n_threads} threads")
         line["cpu_baseline"] = cpu_baseline(X.cpu().numpy(), n_threads)
+        line["cpu_baseline"].update(host_cpu_info())
+        line["cpu_baseline"]["omp_proc_bind"] = os.environ.get("OMP_PROC_BIND")
         log(f"cpu baseline {line['cpu_baseline']['value']:.4g} pairs/s")
     for o_ in list(outs):
         del outs[o_]

@@ -89,6 +89,10 @@ def load_library():
     L.rmsd_ctx_set_profiling.restype = C.c_int
     L.rmsd_ctx_stage_ms.argtypes = [P, C.POINTER(C.c_double)]
     L.rmsd_ctx_stage_ms.restype = C.c_int
+    L.rmsd_ctx_set_tolerance.argtypes = [P, C.c_double]
+    L.rmsd_ctx_set_tolerance.restype = C.c_int
+    L.rmsd_ctx_error_report.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    L.rmsd_ctx_error_report.restype = C.c_int
     FP = C.POINTER(Frames)
     L.rmsd_pairwise.argtypes = [P, FP, FP, C.c_int, C.c_int, C.c_int, P]
     L.rmsd_pairwise.restype = C.c_int
@@ -271,6 +275,20 @@ class Context:
 
     def set_profiling(self, on: bool = True):
         self.check(self._L.rmsd_ctx_set_profiling(self._h, int(on)))
+
+    def set_tolerance(self, tol: float):
+        """Error tolerance of the mixed-precision pairwise forms (Angstrom): pairs
+        whose stated bound exceeds it are recomputed in FP64; 0 switches the
+        bound off, a negative value reports the bound without recomputing."""
+        self.check(self._L.rmsd_ctx_set_tolerance(self._h, float(tol)))
+
+    def error_report(self):
+        """(largest stated bound of the results kept in mixed precision,
+        pairs recomputed in FP64) for the last pairwise / cluster-cache call."""
+        b = C.c_double()
+        n = C.c_int64()
+        self.check(self._L.rmsd_ctx_error_report(self._h, C.byref(b), C.byref(n)))
+        return b.value, n.value
 
     def stage_ms(self):
         """(per-frame stage, main kernel, epilogue) milliseconds of the last call."""

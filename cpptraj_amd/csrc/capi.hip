@@ -7,6 +7,7 @@
 // point enqueues work on the context's stream and returns without waiting
 // (except rmsd_ctx_synchronize).
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <string.h>
@@ -105,6 +106,42 @@ std::vector<int2> grouped_tiles(int nbr, int nbc, int tri, int tr, int tc) {
 
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
+BoundCoef rmsd_bound_coef(int prec, int KB) {
+  BoundCoef k{};
+  const double sqrt3 = 1.7320508075688772;
+  // FP64 reassembly S = P + u_j + u_i^T + C and the QCP solve: a few roundings
+  // of numbers up to E0 in size.
+  k.e0 = 64.0 * 0x1p-53;
+  if (prec == PREC_BF16X3) {
+    // D = hi + lo + e with hi, lo bfloat16 (8-bit significands): |lo| <= 2^-8 |D|,
+    // |e| <= 2^-16 |D| (and the float step, 2^-24).  The contraction drops
+    // lo_i lo_j: bounded deterministically by Cauchy-Schwarz on the per-frame
+    // ||lo|| (this term is biased, lo_i = lo_j for identical frames).
+    k.drop = 1.0;
+    // The residual terms e_i D_j + D_i e_j: independent zero-mean roundings,
+    // each |term| <= 2^-16 |D_ik| |D_jk| (1 + 2^-8), standard deviation
+    // <= that / sqrt(3); summed over atoms the deviation is at most
+    // 2^-16 / sqrt(3) ||D_i,c||_4 ||D_j,d||_4 per element.
+    k.rep = BOUND_LAMBDA * 2.0 * 0x1p-16 * (1.0 + 0x1p-8) / sqrt3;
+    // FP32 accumulation: 3 MFMAs per 32-atom chunk.  Measured on gfx950
+    // (v_mfma_f32_16x16x32_bf16): each product is rounded to nearest, ties
+    // symmetric, on a grid 2^-26 of the accumulator's binade, and the sum is
+    // rounded to float.  Per MFMA at most 32 such errors of 2^-26 m (m bounds
+    // |acc| by Cauchy-Schwarz, ||D_i,c|| ||D_j,d||) and one of 2^-24 m.
+    const double per = sqrt(33.0) * 0x1p-26 / sqrt3 + 0x1p-24 / sqrt3;
+    k.acc = BOUND_LAMBDA * sqrt(3.0 * KB) * per;
+  } else if (prec == PREC_F32) {
+    // D rounded to float: |D - float(D)| <= 2^-24 |D|, bounded deterministically
+    // with the per-frame residual norm l: |e_i^T D_j + D_i^T e_j| <= l_i ||D_j|| + ||D_i|| l_j.
+    k.mix = 1.0;
+    // FP32 accumulation: 4 MFMAs per 16-atom chunk, each summing 4 products
+    // (rounded to float) and the accumulator: 5 roundings of 2^-24 m.
+    const double per = sqrt(5.0) * 0x1p-24 / sqrt3;
+    k.acc = BOUND_LAMBDA * sqrt(4.0 * KB) * per;
+  }
+  return k;
+}
+
 }  // namespace rmsd_capi
 
 using namespace rmsd_capi;
@@ -141,11 +178,13 @@ struct Packed {
   void* data;
   double* g;
   double* u;
+  double* nrm = nullptr;  // MFMA forms: per-frame bound norms (4 doubles)
+  double* cen = nullptr;  // MFMA forms: per-frame centroids (4 doubles)
 };
 
 int prepare_set(rmsd_ctx* c, const FrameSetDev& fs, const double* cmass, const double* smass,
-                int prec, int Kp, const double* T, const double* gT, const char* tag,
-                Packed* out) {
+                int prec, int Kp, const double* T, const double* gT, const double* M,
+                const char* tag, Packed* out) {
   const int CH = chunk_atoms(prec);
   // Padded to whole row and column tiles: 64 and rmsd_pairwise_tile_cols.
   const int tc = rmsd_pairwise_tile_cols(prec);
@@ -160,6 +199,12 @@ int prepare_set(rmsd_ctx* c, const FrameSetDev& fs, const double* cmass, const d
   if (!out->g) return st;
   out->u = (double*)scratch(c, (n + ".u").c_str(), sizeof(double) * 9 * Fp, &st);
   if (!out->u) return st;
+  if (prec != PREC_F64) {
+    out->nrm = (double*)scratch(c, (n + ".nrm").c_str(), sizeof(double) * 4 * Fp, &st);
+    if (!out->nrm) return st;
+    out->cen = (double*)scratch(c, (n + ".cen").c_str(), sizeof(double) * 4 * Fp, &st);
+    if (!out->cen) return st;
+  }
   PrepArgs a;
   a.fs = fs;
   a.cmass = cmass;
@@ -173,6 +218,10 @@ int prepare_set(rmsd_ctx* c, const FrameSetDev& fs, const double* cmass, const d
   a.u = out->u;
   a.T = T;
   a.gT = gT;
+  a.nrm = out->nrm;
+  a.cen = out->cen;
+  a.total_mass = M;
+  if (prec != PREC_F64) a.bound = rmsd_bound_coef(prec, a.KB);
   HIP_TRY(c, rmsd_launch_prep(a, c->stream));
   return RMSD_OK;
 }
@@ -198,6 +247,7 @@ int pairwise_impl(rmsd_ctx* c, const FrameSetDev& A, const FrameSetDev& B, int t
   double* gT = consts + 10;  // template norm
 
   c->marks = 0;
+  c->report = {0, nullptr};
   mark(c, 0);
   if (mode == RMSD_MODE_NOFIT) {
     SetupArgs s{};
@@ -215,8 +265,9 @@ int pairwise_impl(rmsd_ctx* c, const FrameSetDev& A, const FrameSetDev& B, int t
 
   const int CH = chunk_atoms(prec);
   const int Kp = (int)round_up(A.nsel, CH);
+  const bool mixed = prec != PREC_F64;
   double* T = nullptr;
-  if (prec != PREC_F64) {
+  if (mixed) {
     T = (double*)scratch(c, "pair.template", sizeof(double) * 3 * Kp, &st);
     if (!T) return st;
   }
@@ -231,12 +282,19 @@ int pairwise_impl(rmsd_ctx* c, const FrameSetDev& A, const FrameSetDev& B, int t
   s.gT = gT;
   s.total_mass = M;
   HIP_TRY(c, rmsd_launch_setup(s, c->stream));
+  if (mixed) {
+    // Mean structure of evenly spaced reference frames as the template.
+    const int nsamp = std::min(TEMPLATE_SAMPLES, B.nframes);
+    double* part = (double*)scratch(c, "pair.tmplpart", sizeof(double) * 3 * Kp * nsamp, &st);
+    if (!part) return st;
+    HIP_TRY(c, rmsd_launch_template_mean(s, nsamp, part, c->stream));
+  }
 
   Packed pa, pb;
-  if ((st = prepare_set(c, A, amass, wmass, prec, Kp, T, gT, "A", &pa)) != RMSD_OK) return st;
+  if ((st = prepare_set(c, A, amass, wmass, prec, Kp, T, gT, M, "A", &pa)) != RMSD_OK) return st;
   if (tri) {
     pb = pa;
-  } else if ((st = prepare_set(c, B, bmass, wmass, prec, Kp, T, gT, "B", &pb)) != RMSD_OK) {
+  } else if ((st = prepare_set(c, B, bmass, wmass, prec, Kp, T, gT, M, "B", &pb)) != RMSD_OK) {
     return st;
   }
   PairArgs p;
@@ -259,9 +317,55 @@ int pairwise_impl(rmsd_ctx* c, const FrameSetDev& A, const FrameSetDev& B, int t
   p.clamp = mode == RMSD_MODE_FIT ? 1 : 0;
   p.out = out;
   (void)rows_are_ref;
+  const bool bounded = mixed && c->tol != 0.0;
+  unsigned long long* stats = nullptr;
+  int* flags = nullptr;
+  if (bounded) {
+    flags = (int*)scratch(c, "pair.flags", sizeof(int) * (size_t)ntiles, &st);
+    if (!flags) return st;
+    stats = (unsigned long long*)scratch(c, "pair.stats", 2 * sizeof(unsigned long long), &st);
+    if (!stats) return st;
+    // Fill kernels, not memsets: the sequence stays valid under graph capture.
+    HIP_TRY(c, rmsd_launch_fill_int(flags, ntiles, 0, c->stream));
+    HIP_TRY(c, rmsd_launch_fill_int((int*)stats, 4, 0, c->stream));
+    p.nrmA = pa.nrm;
+    p.nrmB = pb.nrm;
+    p.ke0 = rmsd_bound_coef(prec, p.KB).e0;
+    p.tol = c->tol > 0.0 ? c->tol : INFINITY;  // < 0: report only
+    p.flags = flags;
+    p.stats = stats;
+  }
+  c->report = {mixed ? (bounded ? 1 : 2) : 0, stats};
   mark(c, 1);
   HIP_TRY(c, rmsd_launch_pairwise(prec, p, c->stream));
   mark(c, 2);
+  if (bounded && c->tol > 0.0) {
+    FallbackArgs f;
+    f.A = A;
+    f.B = B;
+    f.smass = wmass;
+    f.cenA = pa.cen;
+    f.cenB = pb.cen;
+    f.gA = pa.g;
+    f.gB = pb.g;
+    f.nrmA = pa.nrm;
+    f.nrmB = pb.nrm;
+    f.total_mass = M;
+    f.ke0 = p.ke0;
+    f.tol = p.tol;
+    f.tiles = tiles;
+    f.ntiles = ntiles;
+    f.tile_r = TILE;
+    f.tile_c = tc;
+    f.flags = flags;
+    f.FA = FA;
+    f.FB = FB;
+    f.tri = tri;
+    f.clamp = p.clamp;
+    f.out = out;
+    f.stats = stats;
+    HIP_TRY(c, rmsd_launch_pair_fallback(f, c->stream));
+  }
   mark(c, 3);
   return RMSD_OK;
 }
@@ -335,6 +439,34 @@ int64_t rmsd_ctx_scratch_bytes(const rmsd_ctx* ctx) {
   int64_t s = 0;
   for (const auto& kv : ctx->bufs) s += (int64_t)kv.second.bytes;
   return s;
+}
+
+int rmsd_ctx_set_tolerance(rmsd_ctx* ctx, double tol) {
+  if (!ctx) return RMSD_ERR_ARG;
+  if (tol != tol) return fail(ctx, RMSD_ERR_ARG, "tolerance is NaN");
+  ctx->tol = tol;
+  return RMSD_OK;
+}
+
+int rmsd_ctx_error_report(rmsd_ctx* ctx, double* max_bound, int64_t* n_fp64) {
+  if (!ctx || !max_bound || !n_fp64) return RMSD_ERR_ARG;
+  *max_bound = 0.0;
+  *n_fp64 = 0;
+  if (ctx->report.kind == 2) {
+    *max_bound = -1.0;  // mixed precision with the bound switched off: unknown
+    return RMSD_OK;
+  }
+  if (ctx->report.kind != 1) return RMSD_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  unsigned long long h[2] = {0, 0};
+  HIP_TRY(ctx, hipMemcpyAsync(h, ctx->report.stats, sizeof(h), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  double b;
+  memcpy(&b, &h[0], sizeof(b));
+  *max_bound = b;
+  *n_fp64 = (int64_t)h[1];
+  return RMSD_OK;
 }
 
 int rmsd_pairwise(rmsd_ctx* ctx, const rmsd_frames* tgt, const rmsd_frames* ref, int mode,

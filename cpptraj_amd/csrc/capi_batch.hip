@@ -1,5 +1,6 @@
 // capi_batch.hip -- C ABI of batched plans (rmsd_batch_*): many frame sets in
 // one graph-captured pass of the setup, prep and batched contraction kernels.
+#include <math.h>
 #include <string.h>
 
 #include <algorithm>
@@ -18,6 +19,7 @@ struct rmsd_batch {
   // per set (device pointers into mem)
   std::vector<void*> packed;
   std::vector<double*> g, u, consts, T;
+  std::vector<double*> nrm, cen, part;  // MFMA forms: bound norms, centroids, template samples
   std::vector<int> Kp;
   std::vector<int2*> nofit_tiles;
   std::vector<int> nofit_ntiles;
@@ -26,6 +28,17 @@ struct rmsd_batch {
   PrepArgs* dprep = nullptr;
   int* dfbase = nullptr;
   int nactive = 0, total_frames = 0;
+  // mean-structure templates (MFMA forms): sample prefix over the active sets
+  int* dsbase = nullptr;
+  int total_samples = 0, maxk = 0;
+  // error bound (MFMA forms, context tolerance != 0 at creation): one flag per
+  // tile of either shape's list, {largest kept bound, pairs recomputed}, and
+  // the per-set arguments of the FP64 recomputation
+  bool bounded = false;
+  double tol = 0.0;
+  int* tflags = nullptr;
+  unsigned long long* stats = nullptr;
+  FallbackArgs* dfall = nullptr;
   // batched kernel
   PairArgs* dargs = nullptr;
   int4* dtiles[2] = {nullptr, nullptr};  // per tile side (64, 32)
@@ -87,6 +100,8 @@ void build_queues(const rmsd_batch& b, int tr, int tc, std::vector<int4>* tiles,
   }
   while (q <= b.nq) (*qbeg)[q++] = (int)cost.size();
   (*qbeg)[b.nq] = (int)cost.size();
+  // w: the tile's position in the list (its flag of the error bound).
+  for (size_t t = 0; t < tiles->size(); ++t) (*tiles)[t].w = (int)t;
   *maxq = 0;
   for (int k = 0; k < b.nq; ++k) *maxq = std::max(*maxq, (*qbeg)[k + 1] - (*qbeg)[k]);
 }
@@ -114,8 +129,12 @@ int batch_enqueue(rmsd_batch* b, int v, hipStream_t st) {
     }
     return RMSD_OK;
   }
-  // Every set's template / total weight, then every set's frames: two launches.
+  // Every set's template / total weight (the mean structure of its samples
+  // for the MFMA forms), then every set's frames.
   HIP_TRY(c, rmsd_launch_setup_batch(b->dsetup, b->nactive, st));
+  if (b->prec != PREC_F64)
+    HIP_TRY(c, rmsd_launch_template_mean_batch(b->dsetup, b->dsbase, b->nactive,
+                                               b->total_samples, b->maxk, st));
   HIP_TRY(c, rmsd_launch_prep_batch(b->prec, b->dprep, b->dfbase, b->nactive, b->total_frames,
                                     st));
   mark(c, 1);
@@ -128,8 +147,19 @@ int batch_enqueue(rmsd_batch* b, int v, hipStream_t st) {
   ba.maxq = b->maxq[sh];
   ba.counters = b->counters;
   if (b->ntiles[sh] == 0) return RMSD_OK;
+  if (b->bounded) {
+    // Fill kernels (no memset nodes in the captured graph).
+    HIP_TRY(c, rmsd_launch_fill_int(b->tflags, b->ntiles[sh], 0, st));
+    HIP_TRY(c, rmsd_launch_fill_int((int*)b->stats, 4, 0, st));
+  }
   if (v & 1) HIP_TRY(c, rmsd_launch_batch_reset(ba, st));
   HIP_TRY(c, rmsd_launch_pairwise_batch(b->prec, v, ba, st));
+  if (b->bounded && b->tol > 0.0) {
+    int tr = 0, tc = 0;
+    rmsd_batch_tile_shape(b->prec, v, &tr, &tc);
+    HIP_TRY(c, rmsd_launch_pair_fallback_batch(b->dfall, b->dtiles[sh], b->ntiles[sh], tr, tc,
+                                               b->tflags, st));
+  }
   return RMSD_OK;
 }
 
@@ -162,6 +192,7 @@ int batch_capture(rmsd_batch* b) {
 
 int batch_run_on(rmsd_batch* b, int v) {
   rmsd_ctx* c = b->ctx;
+  c->report = {b->prec == PREC_F64 ? 0 : (b->bounded ? 1 : 2), b->stats};
   c->marks = 0;
   mark(c, 0);
   if (b->exec && v == b->variant) {
@@ -239,6 +270,9 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
   auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
   size_t off = 0;
   std::vector<size_t> o_packed(nsets), o_g(nsets), o_u(nsets), o_c(nsets), o_T(nsets), o_nt(nsets);
+  std::vector<size_t> o_nrm(nsets), o_cen(nsets), o_part(nsets);
+  std::vector<int> nsamp(nsets, 0);
+  const bool mixed = mode != RMSD_MODE_NOFIT && b->prec != PREC_F64;
   std::vector<std::vector<int2>> nt_host(nsets);
   b->Kp.resize(nsets);
   for (int s = 0; s < nsets; ++s) {
@@ -263,11 +297,20 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
     off += al(sizeof(double) * 9 * Fp);
     o_T[s] = off;
     off += al(sizeof(double) * 3 * b->Kp[s]);
+    if (mixed) {
+      nsamp[s] = std::min(TEMPLATE_SAMPLES, F);
+      o_nrm[s] = off;
+      off += al(sizeof(double) * 4 * Fp);
+      o_cen[s] = off;
+      off += al(sizeof(double) * 4 * Fp);
+      o_part[s] = off;
+      off += al(sizeof(double) * 3 * b->Kp[s] * nsamp[s]);
+    }
   }
   std::vector<int4> th[2];
   std::vector<int> qh[2];
   size_t o_args = 0, o_tiles[2] = {0, 0}, o_q[2] = {0, 0}, o_cnt = 0;
-  size_t o_setup = 0, o_prep = 0, o_fbase = 0;
+  size_t o_setup = 0, o_prep = 0, o_fbase = 0, o_sbase = 0, o_flags = 0, o_stats = 0, o_fall = 0;
   int nactive = 0;
   for (int s = 0; s < nsets; ++s) nactive += (mode != RMSD_MODE_NOFIT && sets[s].nframes >= 2);
   if (mode != RMSD_MODE_NOFIT) {
@@ -291,6 +334,16 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
     off += al(sizeof(PairArgs) * std::max(1, nsets));
     o_cnt = off;
     off += al(sizeof(int) * b->nq);
+    if (mixed) {
+      o_sbase = off;
+      off += al(sizeof(int) * (nactive + 1));
+      o_flags = off;
+      off += al(sizeof(int) * std::max(1, std::max(b->ntiles[0], b->ntiles[1])));
+      o_stats = off;
+      off += al(2 * sizeof(unsigned long long));
+      o_fall = off;
+      off += al(sizeof(FallbackArgs) * std::max(1, nsets));
+    }
   }
   if (hipMalloc(&b->mem, std::max<size_t>(off, 256)) != hipSuccess) {
     b->mem = nullptr;
@@ -303,6 +356,17 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
   b->u.assign(nsets, nullptr);
   b->consts.assign(nsets, nullptr);
   b->T.assign(nsets, nullptr);
+  b->nrm.assign(nsets, nullptr);
+  b->cen.assign(nsets, nullptr);
+  b->part.assign(nsets, nullptr);
+  b->bounded = mixed && ctx->tol != 0.0;
+  b->tol = ctx->tol > 0.0 ? ctx->tol : INFINITY;  // < 0: report only
+  if (mixed) {
+    b->tflags = (int*)(base + o_flags);
+    b->stats = (unsigned long long*)(base + o_stats);
+    b->dfall = (FallbackArgs*)(base + o_fall);
+  }
+  std::vector<FallbackArgs> fall(nsets);
   b->nofit_tiles.assign(nsets, nullptr);
   b->nofit_ntiles.assign(nsets, 0);
   std::vector<PairArgs> args(nsets);
@@ -318,6 +382,11 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
     b->g[s] = (double*)(base + o_g[s]);
     b->u[s] = (double*)(base + o_u[s]);
     b->T[s] = b->prec == PREC_F64 ? nullptr : (double*)(base + o_T[s]);
+    if (mixed) {
+      b->nrm[s] = (double*)(base + o_nrm[s]);
+      b->cen[s] = (double*)(base + o_cen[s]);
+      b->part[s] = (double*)(base + o_part[s]);
+    }
     PairArgs& p = args[s];
     memset(&p, 0, sizeof(p));
     p.A = p.B = b->packed[s];
@@ -330,6 +399,28 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
     p.tri = 1;
     p.clamp = mode == RMSD_MODE_FIT ? 1 : 0;
     p.out = outs[s];
+    if (b->bounded) {
+      p.nrmA = p.nrmB = b->nrm[s];
+      p.ke0 = rmsd_bound_coef(b->prec, p.KB).e0;
+      p.tol = b->tol;
+      p.flags = b->tflags;
+      p.stats = b->stats;
+      FallbackArgs& f = fall[s];
+      memset(&f, 0, sizeof(f));
+      f.A = f.B = to_dev(sets[s]);
+      f.smass = use_mass ? sets[s].mass : nullptr;
+      f.cenA = f.cenB = b->cen[s];
+      f.gA = f.gB = b->g[s];
+      f.nrmA = f.nrmB = b->nrm[s];
+      f.total_mass = b->consts[s];
+      f.ke0 = p.ke0;
+      f.tol = p.tol;
+      f.FA = f.FB = sets[s].nframes;
+      f.tri = 1;
+      f.clamp = p.clamp;
+      f.out = outs[s];
+      f.stats = b->stats;
+    }
   }
   int rc = RMSD_OK;
   auto up = [&](void* dst, const void* src, size_t bytes) {
@@ -344,7 +435,7 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
     // their padded frame counts (one prep workgroup per padded frame).
     std::vector<SetupArgs> su;
     std::vector<PrepArgs> pa;
-    std::vector<int> fbase(1, 0);
+    std::vector<int> fbase(1, 0), sbase(1, 0);
     for (int s = 0; s < nsets; ++s) {
       const rmsd_frames& f = sets[s];
       if (f.nframes < 2) continue;
@@ -360,6 +451,8 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
       x.P = M + 1;
       x.gT = M + 10;
       x.total_mass = M;
+      x.nsamp = nsamp[s];
+      x.part = b->part[s];
       su.push_back(x);
       PrepArgs a;
       memset(&a, 0, sizeof(a));
@@ -375,8 +468,14 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
       a.u = b->u[s];
       a.T = b->T[s];
       a.gT = M + 10;
+      a.nrm = b->nrm[s];
+      a.cen = b->cen[s];
+      a.total_mass = M;
+      a.bound = rmsd_bound_coef(b->prec, a.KB);
       pa.push_back(a);
       fbase.push_back(fbase.back() + a.Fp);
+      sbase.push_back(sbase.back() + nsamp[s]);
+      b->maxk = std::max(b->maxk, 3 * b->Kp[s]);
     }
     b->nactive = (int)su.size();
     b->total_frames = fbase.back();
@@ -386,6 +485,12 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
     up(b->dsetup, su.data(), sizeof(SetupArgs) * su.size());
     up(b->dprep, pa.data(), sizeof(PrepArgs) * pa.size());
     up(b->dfbase, fbase.data(), sizeof(int) * fbase.size());
+    if (mixed) {
+      b->total_samples = sbase.back();
+      b->dsbase = (int*)(base + o_sbase);
+      up(b->dsbase, sbase.data(), sizeof(int) * sbase.size());
+      if (b->bounded) up(b->dfall, fall.data(), sizeof(FallbackArgs) * nsets);
+    }
   }
   if (mode != RMSD_MODE_NOFIT) {
     b->dargs = (PairArgs*)(base + o_args);

@@ -19,6 +19,20 @@ namespace rmsd_capi {
 constexpr int TILE = 64;      // frames per tile side (all pairwise kernels)
 constexpr int GROUP = 8;      // tile rows per band of the grouped schedule
 constexpr int NXCD = 8;       // MI355X: 8 XCDs, workgroups dealt round-robin
+// Frames averaged into the mean-structure template of the MFMA-reduced forms.
+constexpr int TEMPLATE_SAMPLES = 256;
+// Standard deviations covered by the probabilistic terms of the error bound.
+constexpr double BOUND_LAMBDA = 6.0;
+constexpr double DEFAULT_TOLERANCE = 1e-4;  // Angstrom
+
+// What rmsd_ctx_error_report can say about the last pairwise call:
+// kind 0 = FP64 (or no-fit) arithmetic, 1 = bounded mixed precision (stats:
+// {largest kept bound as double bits, pairs recomputed in FP64}), 2 = mixed
+// precision with the bound switched off.
+struct ErrorReport {
+  int kind = 0;
+  unsigned long long* stats = nullptr;
+};
 
 struct DevBuf {
   void* p = nullptr;
@@ -51,6 +65,10 @@ struct rmsd_ctx {
   // Per-residue tables of the last rmsd_perres_nofit call (host image of the
   // device copy in scratch "perres.table"), re-uploaded only when they change.
   std::vector<unsigned char> perres_blob;
+  // Error bound of the mixed-precision pairwise forms: tolerance (Angstrom;
+  // 0 = off, < 0 = report only) and what the last call reported.
+  double tol = rmsd_capi::DEFAULT_TOLERANCE;
+  rmsd_capi::ErrorReport report;
 };
 
 namespace rmsd_capi {
@@ -73,6 +91,8 @@ int check_frames(rmsd_ctx* c, const rmsd_frames* f, const char* what);
 FrameSetDev to_dev(const rmsd_frames& f);
 std::vector<int2> grouped_tiles(int nbr, int nbc, int tri, int tr = TILE, int tc = TILE);
 int64_t round_up(int64_t x, int64_t m);
+// Coefficients of the per-pair error bound for a precision and chunk count.
+BoundCoef rmsd_bound_coef(int prec, int KB);
 // Frees a batch plan (capi_batch.hip); rmsd_ctx_destroy frees leftovers.
 void batch_free(rmsd_batch* b);
 

@@ -116,8 +116,18 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
                                               typename PrecTraits<PREC>::acc_t (&acc)[WR][WC][3][3],
                                               int64_t fbA, int64_t fbB, int lane,
                                               const double* gA, const double* uA, int64_t rbase,
-                                              const double* gB, const double* uB, int64_t cbase) {
+                                              const double* gB, const double* uB, int64_t cbase,
+                                              int tile_id) {
   const double M = args.total_mass[0];
+  // Error bound (MFMA-reduced forms with a flag array): per lane, whether a
+  // pair went over the tolerance, the kept pair with the largest eps / r
+  // (compared by cross-multiplication: no division per pair) and the largest
+  // kept r; for tol = infinity (report only) also the bounds of the pairs
+  // that fail r^2 > eps.
+  const bool bounded = PREC != PREC_F64 && args.flags != nullptr && M >= RMSD_SMALL;
+  const double ke0M = bounded ? args.ke0 / M : 0.0;
+  bool over = false;
+  double qe = 0.0, qr = 1.0, rmax = 0.0, bover = 0.0;
   double Pm[9];
   if (PREC != PREC_F64) {
 #pragma unroll
@@ -126,13 +136,14 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
 #pragma unroll
   for (int q = 0; q < WC; ++q) {
     const int64_t j = (fbB + q) * 16 + (lane & 15);  // column frame
-    double gj = 0.0, uj[9];
+    double gj = 0.0, uj[9], csj = 0.0;
     const bool jok = j < args.FB;
     if (jok) {
       gj = gB[j - cbase];
       if (PREC != PREC_F64) {
 #pragma unroll
         for (int k = 0; k < 9; ++k) uj[k] = uB[9 * (j - cbase) + k];
+        if (bounded) csj = args.nrmB[4 * j];
       }
     }
 #pragma unroll
@@ -173,16 +184,128 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
         }
         const double E0 = 0.5 * (gA[i - rbase] + gj);
         const int64_t idx = args.tri ? rmsd_dev::tri_index(args.FA, i, j) : i * args.FB + j;
-        args.out[idx] = M < RMSD_SMALL ? -1.0f : (float)rmsd_dev::qcp_rmsd(R, E0, M, args.clamp);
+        const float r = M < RMSD_SMALL ? -1.0f : (float)rmsd_dev::qcp_rmsd(R, E0, M, args.clamp);
+        args.out[idx] = r;
+        if (bounded) {
+          const double rr = (double)r;
+          const double eps = args.nrmA[4 * i] * csj + ke0M * E0;
+          if (rmsd_dev::bound_keep(eps, rr, args.tol)) {
+            if (eps * qr > qe * rr) {
+              qe = eps;
+              qr = rr;
+            }
+            rmax = fmax(rmax, rr);
+          } else {
+            over = true;  // NaN included: recomputed in FP64
+            if (args.tol == INFINITY) bover = fmax(bover, rmsd_dev::bound_value(eps, rr));
+          }
+        }
       }
     }
   }
+  if (bounded) {
+    // One flag store and one atomic per wave.
+    if (__ballot(over) && lane == 0) args.flags[tile_id] = 1;
+    double bmax = fmax(qe / qr * (1.0 + 0x1p-23) + rmax * 0x1p-23, bover);
+    for (int off = 32; off > 0; off >>= 1) bmax = fmax(bmax, __shfl_xor(bmax, off, 64));
+    if (lane == 0 && bmax > 0.0)
+      atomicMax(args.stats, (unsigned long long)__double_as_longlong(bmax));
+  }
+}
+
+// ---- FP64 recomputation of flagged pairs -------------------------------------------
+// One workgroup per tile of the schedule; tiles without a flag leave at once.
+// The bound is evaluated again from the stored float RMSD (the epilogue
+// decided on the same float with the same test, so both agree) and the pairs
+// over the tolerance are listed in LDS; then each thread recomputes whole
+// pairs, one at a time, summing over the atoms in FP64 from the frames as
+// stored, centred with the per-frame centroids of k_prep, and solves them with
+// the same FP64 QCP.  Consecutive threads take consecutive pairs of a tile row:
+// the row frame's atoms are one broadcast load, the column frames' lines stay
+// in the vector cache across the atom loop.
+constexpr int FALLBACK_THREADS = 256;
+constexpr int FALLBACK_MAX_PAIRS = 64 * 96;  // largest tile of any schedule
+
+__device__ __forceinline__ void centred_atom(const FrameSetDev& fs, int64_t f, int k,
+                                             const double* cen, const double* w, double x[3]) {
+  float p[3];
+  load_atom(fs, f, k, p);
+  const double s = w ? sqrt(w[k]) : 1.0;
+  x[0] = ((double)p[0] - cen[0]) * s;
+  x[1] = ((double)p[1] - cen[1]) * s;
+  x[2] = ((double)p[2] - cen[2]) * s;
+}
+
+__device__ __forceinline__ void fallback_tile(const FallbackArgs& a, int2 tile, int tile_r,
+                                              int tile_c) {
+  __shared__ unsigned short list[FALLBACK_MAX_PAIRS];
+  __shared__ int nlist;
+  const double M = a.total_mass[0];
+  const double ke0M = a.ke0 / M;
+  const int npairs = tile_r * tile_c;
+  if (threadIdx.x == 0) nlist = 0;
+  __syncthreads();
+  for (int p = threadIdx.x; p < npairs; p += FALLBACK_THREADS) {
+    const int64_t i = (int64_t)tile.x * tile_r + p / tile_c;
+    const int64_t j = (int64_t)tile.y * tile_c + p % tile_c;
+    if (i < a.FA && j < a.FB && (!a.tri || i < j)) {
+      const int64_t idx = a.tri ? rmsd_dev::tri_index(a.FA, i, j) : i * a.FB + j;
+      const double eps = a.nrmA[4 * i] * a.nrmB[4 * j] + ke0M * 0.5 * (a.gA[i] + a.gB[j]);
+      if (!rmsd_dev::bound_keep(eps, (double)a.out[idx], a.tol))
+        list[atomicAdd(&nlist, 1)] = (unsigned short)p;
+    }
+  }
+  __syncthreads();
+  const int n = nlist;
+  for (int q = threadIdx.x; q < n; q += FALLBACK_THREADS) {
+    const int p = list[q];
+    const int64_t i = (int64_t)tile.x * tile_r + p / tile_c;
+    const int64_t j = (int64_t)tile.y * tile_c + p % tile_c;
+    const double* ci = a.cenA + 4 * i;
+    const double* cj = a.cenB + 4 * j;
+    double S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < a.A.nsel; ++k) {
+      double x[3], y[3];
+      centred_atom(a.A, i, k, ci, a.smass, x);
+      centred_atom(a.B, j, k, cj, a.smass, y);
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) S[3 * c + d] += x[c] * y[d];
+    }
+    double R[9];
+    // As in the epilogue: rows are the reference in the triangle layout.
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) R[3 * c + d] = a.tri ? S[3 * d + c] : S[3 * c + d];
+    const double E0 = 0.5 * (a.gA[i] + a.gB[j]);
+    const int64_t idx = a.tri ? rmsd_dev::tri_index(a.FA, i, j) : i * a.FB + j;
+    a.out[idx] = (float)rmsd_dev::qcp_rmsd(R, E0, M, a.clamp);
+  }
+  if (threadIdx.x == 0 && n) atomicAdd(a.stats + 1, (unsigned long long)n);
+}
+
+__global__ __launch_bounds__(FALLBACK_THREADS) void k_pair_fallback(FallbackArgs a) {
+  const int t = blockIdx.x;
+  if (t >= a.ntiles || a.flags[t] == 0) return;
+  fallback_tile(a, a.tiles[t], a.tile_r, a.tile_c);
+}
+
+__global__ __launch_bounds__(FALLBACK_THREADS) void k_pair_fallback_batch(const FallbackArgs* fsets,
+                                                             const int4* tiles, int ntiles,
+                                                             int tile_r, int tile_c,
+                                                             const int* flags) {
+  const int t = blockIdx.x;
+  if (t >= ntiles || flags[t] == 0) return;
+  const int4 tl = tiles[t];
+  fallback_tile(fsets[tl.x], make_int2(tl.y, tl.z), tile_r, tile_c);
 }
 
 // One 64 x 64 tile of frame pairs: contraction over all atom chunks, then
 // the FP64 epilogue.  Shared by the single-matrix and the batched kernels.
 template <int PREC, int WR, int WC, int NWR, int NWC>
-__device__ __forceinline__ void pair_tile(const PairArgs& args, int2 tile) {
+__device__ __forceinline__ void pair_tile(const PairArgs& args, int2 tile, int tile_id) {
   typedef PrecTraits<PREC> T;
   typedef typename T::acc_t acc_t;
   constexpr int PARTS = T::PARTS;
@@ -252,7 +375,8 @@ __device__ __forceinline__ void pair_tile(const PairArgs& args, int2 tile) {
           for (int d = 0; d < 3; ++d) mma<PREC>(acc[r][q][c][d], ca[r][c], cb[q][d]);
   }
 
-  pair_epilogue<PREC, WR, WC>(args, acc, fbA, fbB, lane, args.gA, args.uA, 0, args.gB, args.uB, 0);
+  pair_epilogue<PREC, WR, WC>(args, acc, fbA, fbB, lane, args.gA, args.uA, 0, args.gB, args.uB, 0,
+                              tile_id);
 }
 
 template <int PREC, int WR, int WC, int NWR, int NWC>
@@ -264,7 +388,7 @@ __global__ __launch_bounds__(64 * NWR * NWC) void k_pairwise(PairArgs args) {
   const int w = blockIdx.x;
   const int t = (w % args.nxcd) * args.per_xcd + (w / args.nxcd);
   if (t >= args.ntiles) return;
-  pair_tile<PREC, WR, WC, NWR, NWC>(args, args.tiles[t]);
+  pair_tile<PREC, WR, WC, NWR, NWC>(args, args.tiles[t], t);
 }
 
 // ---- batched matrices ---------------------------------------------------------------
@@ -279,7 +403,7 @@ __global__ __launch_bounds__(64 * NWR * NWC) void k_pairwise_batch(BatchArgs b) 
   if (t >= b.qbeg[q + 1]) return;
   const int4 tl = b.tiles[t];
   const PairArgs args = b.sets[tl.x];
-  pair_tile<PREC, WR, WC, NWR, NWC>(args, make_int2(tl.y, tl.z));
+  pair_tile<PREC, WR, WC, NWR, NWC>(args, make_int2(tl.y, tl.z), tl.w);
 }
 
 // Persistent form: as many workgroups as are resident; each drains its own
@@ -301,7 +425,7 @@ __global__ __launch_bounds__(64 * NWR * NWC) void k_pairwise_batch_persist(Batch
       if (pos >= len) break;
       const int4 tl = b.tiles[beg + pos];
       const PairArgs args = b.sets[tl.x];
-      pair_tile<PREC, WR, WC, NWR, NWC>(args, make_int2(tl.y, tl.z));
+      pair_tile<PREC, WR, WC, NWR, NWC>(args, make_int2(tl.y, tl.z), tl.w);
     }
   }
 }
@@ -369,6 +493,29 @@ static hipError_t launch_batch_shape(const BatchArgs& b, bool persist, hipStream
   if (grid < b.nq) grid = b.nq;
   hipLaunchKernelGGL((k_pairwise_batch_persist<PREC, WR, WC, NWR, NWC>), dim3((unsigned)grid),
                      dim3(NT), 0, s, b);
+  return hipGetLastError();
+}
+
+hipError_t rmsd_launch_pair_fallback(const FallbackArgs& a, hipStream_t s) {
+  if (a.ntiles <= 0) return hipSuccess;
+  if (a.tile_r * a.tile_c > FALLBACK_MAX_PAIRS) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_pair_fallback, dim3(a.ntiles), dim3(FALLBACK_THREADS), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t rmsd_launch_pair_fallback_batch(const FallbackArgs* fsets, const int4* tiles,
+                                           int ntiles, int tile_r, int tile_c, const int* flags,
+                                           hipStream_t s) {
+  if (ntiles <= 0) return hipSuccess;
+  if (tile_r * tile_c > FALLBACK_MAX_PAIRS) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_pair_fallback_batch, dim3(ntiles), dim3(FALLBACK_THREADS), 0, s, fsets,
+                     tiles, ntiles, tile_r, tile_c, flags);
+  return hipGetLastError();
+}
+
+hipError_t rmsd_launch_fill_int(int* p, int n, int v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
   return hipGetLastError();
 }
 

@@ -115,6 +115,112 @@ __device__ __forceinline__ void setup_body(const SetupArgs& a) {
 
 __global__ __launch_bounds__(PREP_THREADS) void k_setup(SetupArgs a) { setup_body(a); }
 
+// ---- mean-structure template ---------------------------------------------------
+// The MFMA-reduced forms contract deviations D = U x~ - T, and their error
+// grows with ||D_i|| ||D_j||: a template near the middle of the trajectory
+// keeps D small.  Sample s (evenly spaced frame of the set) is rotated onto the
+// first template (FP64 QCP, as in k_prep) and written to part[s][3][Kp];
+// k_tmpl_mean averages the samples in a fixed order (deterministic) and
+// k_tmpl_consts recomputes P = T^T T and sum |T|^2.
+__device__ __forceinline__ void tmpl_sample_body(const SetupArgs& a, int s, int nsamp,
+                                                 double* part) {
+  __shared__ double red[16 * (PREP_THREADS / 64)];
+  const int64_t f = ((int64_t)s * a.fs.nframes) / nsamp;
+  const int Kp = a.Kp;
+  double cen[3];
+  centroid(a.fs, f, a.cmass, red, cen);
+  double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int k = threadIdx.x; k < a.fs.nsel; k += blockDim.x) {
+    double x[3];
+    centred(a.fs, f, k, cen, a.smass, x);
+    const double t0 = a.T[k], t1 = a.T[Kp + k], t2 = a.T[2 * Kp + k];
+    v[0] += x[0] * t0; v[1] += x[0] * t1; v[2] += x[0] * t2;
+    v[3] += x[1] * t0; v[4] += x[1] * t1; v[5] += x[1] * t2;
+    v[6] += x[2] * t0; v[7] += x[2] * t1; v[8] += x[2] * t2;
+    v[9] += x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+  }
+  rmsd_dev::block_sum<10>(v, red);
+  double U[9];
+  {
+    const double E0 = 0.5 * (v[9] + a.gT[0]);
+    const rmsd_dev::QcpCoef co = rmsd_dev::qcp_coefficients(v);
+    const double lam = rmsd_dev::qcp_lambda(co, E0);
+    rmsd_dev::qcp_rotation(v, lam, U);
+  }
+  double* out = part + (int64_t)s * 3 * Kp;
+  for (int k = threadIdx.x; k < Kp; k += blockDim.x) {
+    double x[3];
+    centred(a.fs, f, k, cen, a.smass, x);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      out[c * Kp + k] = U[3 * c] * x[0] + U[3 * c + 1] * x[1] + U[3 * c + 2] * x[2];
+  }
+}
+
+__global__ __launch_bounds__(PREP_THREADS) void k_tmpl_sample(SetupArgs a, int nsamp,
+                                                              double* part) {
+  tmpl_sample_body(a, blockIdx.x, nsamp, part);
+}
+
+// Batched: workgroup b is sample b - sbase[k] of set k, sbase[k] <= b < sbase[k + 1].
+__global__ __launch_bounds__(PREP_THREADS) void k_tmpl_sample_batch(const SetupArgs* sets,
+                                                                    const int* sbase, int nsets) {
+  const int b = blockIdx.x;
+  int lo = 0, hi = nsets - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (sbase[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const SetupArgs a = sets[lo];
+  tmpl_sample_body(a, b - sbase[lo], a.nsamp, a.part);
+}
+
+__device__ __forceinline__ void tmpl_mean_elem(const double* part, int nsamp, int n, int e,
+                                               double* T) {
+  if (e >= n) return;
+  double acc = 0.0;
+  for (int s = 0; s < nsamp; ++s) acc += part[(int64_t)s * n + e];
+  T[e] = acc / (double)nsamp;
+}
+
+__global__ __launch_bounds__(256) void k_tmpl_mean(const double* part, int nsamp, int n,
+                                                   double* T) {
+  tmpl_mean_elem(part, nsamp, n, blockIdx.x * blockDim.x + threadIdx.x, T);
+}
+
+// Batched: blockIdx.y is the set.
+__global__ __launch_bounds__(256) void k_tmpl_mean_batch(const SetupArgs* sets) {
+  const SetupArgs& a = sets[blockIdx.y];
+  tmpl_mean_elem(a.part, a.nsamp, 3 * a.Kp, blockIdx.x * blockDim.x + threadIdx.x, a.T);
+}
+
+__device__ __forceinline__ void tmpl_consts_body(const SetupArgs& a) {
+  __shared__ double red[16 * (PREP_THREADS / 64)];
+  const int Kp = a.Kp;
+  double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int k = threadIdx.x; k < Kp; k += blockDim.x) {
+    const double x[3] = {a.T[k], a.T[Kp + k], a.T[2 * Kp + k]};
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) v[3 * c + d] += x[c] * x[d];
+    v[9] += x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+  }
+  rmsd_dev::block_sum<10>(v, red);
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 9; ++k) a.P[k] = v[k];
+    a.gT[0] = v[9];
+  }
+}
+
+__global__ __launch_bounds__(PREP_THREADS) void k_tmpl_consts(SetupArgs a) { tmpl_consts_body(a); }
+
+__global__ __launch_bounds__(PREP_THREADS) void k_tmpl_consts_batch(const SetupArgs* sets) {
+  const SetupArgs a = sets[blockIdx.x];
+  tmpl_consts_body(a);
+}
+
 // One workgroup per set (batched matrices).
 __global__ __launch_bounds__(PREP_THREADS) void k_setup_batch(const SetupArgs* sets) {
   const SetupArgs a = sets[blockIdx.x];
@@ -130,6 +236,8 @@ __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
   if (f >= a.fs.nframes) {
     // Padding frames: zero operands so that padded rows and columns of a tile
     // contribute nothing.
+    if (a.nrm && threadIdx.x < 4) a.nrm[4 * f + threadIdx.x] = 0.0;
+    if (a.cen && threadIdx.x < 4) a.cen[4 * f + threadIdx.x] = 0.0;
     for (int k = threadIdx.x; k < Kp; k += blockDim.x)
       for (int c = 0; c < 3; ++c)
         for (int p = 0; p < chunk_parts(PREC); ++p) {
@@ -143,6 +251,12 @@ __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
 
   double cen[3];
   centroid(a.fs, f, a.cmass, red, cen);
+  if (a.cen && threadIdx.x == 0) {
+    a.cen[4 * f] = cen[0];
+    a.cen[4 * f + 1] = cen[1];
+    a.cen[4 * f + 2] = cen[2];
+    a.cen[4 * f + 3] = 0.0;
+  }
 
   if (PREC == PREC_F64) {
     double g[1] = {0.0};
@@ -183,7 +297,10 @@ __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
     rmsd_dev::qcp_rotation(v, lam, U);
   }
 
-  double u[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // u = T^T D (9), then the sums behind the error bound's norms: ||D||^2,
+  // sum_k D_c^4 per coordinate, and ||lo||^2 (the split's low parts; for FP32
+  // the rounding residual D - float(D)).
+  double u[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int k = threadIdx.x; k < Kp; k += blockDim.x) {
     double x[3];
     centred(a.fs, f, k, cen, a.smass, x);
@@ -198,20 +315,42 @@ __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
       for (int d = 0; d < 3; ++d) u[3 * c + d] += t[c] * D[d];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
+      const double d2 = D[c] * D[c];
+      u[9] += d2;
+      u[10 + c] += d2 * d2;
       if (PREC == PREC_F32) {
-        reinterpret_cast<float*>(a.packed)[packed_elem<PREC>(f, k, c, 0, a.KB)] = (float)D[c];
+        const float df = (float)D[c];
+        reinterpret_cast<float*>(a.packed)[packed_elem<PREC>(f, k, c, 0, a.KB)] = df;
+        const double r = D[c] - (double)df;
+        u[13] += r * r;
       } else {
         const __bf16 hi = (__bf16)(float)D[c];
         const __bf16 lo = (__bf16)(float)(D[c] - (double)(float)hi);
         __bf16* out = reinterpret_cast<__bf16*>(a.packed);
         out[packed_elem<PREC>(f, k, c, 0, a.KB)] = hi;
         out[packed_elem<PREC>(f, k, c, 1, a.KB)] = lo;
+        const double l = (double)(float)lo;
+        u[13] += l * l;
       }
     }
   }
-  rmsd_dev::block_sum<9>(u, red);
-  if (threadIdx.x == 0)
+  rmsd_dev::block_sum<14>(u, red);
+  if (threadIdx.x == 0) {
     for (int k = 0; k < 9; ++k) a.u[9 * f + k] = u[k];
+    if (a.nrm) {
+      // The error bound's per-frame coefficient (rmsd_dev::bound_keep).
+      const double d2 = u[9];                                  // ||D||^2
+      const double b2 = sqrt(u[10]) + sqrt(u[11]) + sqrt(u[12]);  // sum_c ||D_c||_4^2
+      const double l2 = u[13];                                 // ||l||^2
+      const BoundCoef& k = a.bound;
+      const double c2 = k.drop * l2 + k.mix * (l2 + d2) + k.rep * b2 + k.acc * d2;
+      const double M = a.total_mass ? a.total_mass[0] : 0.0;
+      a.nrm[4 * f] = M >= RMSD_SMALL ? sqrt(c2 * (2.0 * 1.7320508075688772 / M)) : 0.0;
+      a.nrm[4 * f + 1] = sqrt(d2);
+      a.nrm[4 * f + 2] = sqrt(b2);
+      a.nrm[4 * f + 3] = sqrt(l2);
+    }
+  }
 }
 
 template <int PREC>
@@ -239,6 +378,26 @@ __global__ __launch_bounds__(PREP_THREADS) void k_prep_batch(const PrepArgs* set
 
 hipError_t rmsd_launch_setup(const SetupArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_setup, dim3(1), dim3(PREP_THREADS), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t rmsd_launch_template_mean(const SetupArgs& a, int nsamp, double* part,
+                                     hipStream_t s) {
+  if (!a.T || nsamp < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_tmpl_sample, dim3(nsamp), dim3(PREP_THREADS), 0, s, a, nsamp, part);
+  const int n = 3 * a.Kp;
+  hipLaunchKernelGGL(k_tmpl_mean, dim3((n + 255) / 256), dim3(256), 0, s, part, nsamp, n, a.T);
+  hipLaunchKernelGGL(k_tmpl_consts, dim3(1), dim3(PREP_THREADS), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t rmsd_launch_template_mean_batch(const SetupArgs* sets, const int* sbase, int nsets,
+                                           int total_samples, int maxk, hipStream_t s) {
+  if (nsets <= 0 || total_samples <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tmpl_sample_batch, dim3(total_samples), dim3(PREP_THREADS), 0, s, sets,
+                     sbase, nsets);
+  hipLaunchKernelGGL(k_tmpl_mean_batch, dim3((maxk + 255) / 256, nsets), dim3(256), 0, s, sets);
+  hipLaunchKernelGGL(k_tmpl_consts_batch, dim3(nsets), dim3(PREP_THREADS), 0, s, sets);
   return hipGetLastError();
 }
 

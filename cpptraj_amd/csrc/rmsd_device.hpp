@@ -110,6 +110,35 @@ __device__ __forceinline__ double qcp_rmsd(const double* S, double E0, double M,
   return rmsd_from_lambda(E0, qcp_lambda(k, E0), M, clamp);
 }
 
+// Error bound of one pair of the MFMA-reduced forms against the FP64
+// computation.  The model (coefficients k*, per-frame norms ||D||, b, l from
+// k_prep) bounds the covariance error
+//   ||dS||_F <= kdrop l_i l_j + kmix (l_i ||D_j|| + ||D_i|| l_j)
+//             + krep b_i b_j + kacc ||D_i|| ||D_j||  =  u_i . v_j
+// with u_f, v_f vectors of equal norm, so ||dS||_F <= c_f(i) c_f(j),
+// c_f^2 = kdrop l^2 + kmix (l^2 + ||D||^2) + krep b^2 + kacc ||D||^2 (Cauchy-
+// Schwarz: one number per frame).  dS moves the largest QCP eigenvalue by at
+// most sqrt(3) ||dS||_F (lambda = max over rotations U of <U, S>,
+// ||U||_F = sqrt(3)), so r^2 = 2 (E0 - lambda) / M moves by at most
+//   eps = cs_i cs_j + ke0 E0 / M,   cs_f = c_f sqrt(2 sqrt(3) / M)
+// (k_prep stores cs_f; ke0 E0 covers the FP64 reassembly and solve).  For the
+// stored float r with r^2 > eps the FP64 result lies within
+// eps / (r + sqrt(r^2 - eps)) <= eps / r, and the float roundings of both
+// outputs add at most (r + eps / r) 2^-23, so
+//   B = eps / r (1 + 2^-23) + r 2^-23
+// is the stated bound.  bound_keep: B <= tol, without a division.
+__device__ __forceinline__ bool bound_keep(double eps, double r, double tol) {
+  return r * r > eps && eps * (1.0 + 0x1p-23) <= (tol - r * 0x1p-23) * r;
+}
+
+// B of a kept pair (r^2 > eps); for the others the bound is sqrt(r^2 + eps)
+// plus the float term.
+__device__ __forceinline__ double bound_value(double eps, double r) {
+  if (r * r > eps) return eps / r * (1.0 + 0x1p-23) + r * 0x1p-23;
+  const double b = sqrt(r * r + eps);
+  return b + (r + b) * 0x1p-23;
+}
+
 // Optimal rotation for covariance S at eigenvalue lam, written as the Kabsch
 // U (x' = U x for the centred target).  Returns 0 when every adjugate column
 // is degenerate and the identity is returned (QCP fallback), 1 otherwise.

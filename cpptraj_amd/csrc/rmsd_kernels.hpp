@@ -50,6 +50,16 @@ __device__ __forceinline__ void load_atom(const FrameSetDev& fs, int64_t f, int 
   }
 }
 
+// Coefficients of the per-pair error bound of the MFMA-reduced forms (the
+// model and its constants are set on the host: rmsd_bound_coef).
+struct BoundCoef {
+  double drop;  // times l_i l_j: the split's dropped lo*lo products
+  double mix;   // times (l_i ||D_j|| + ||D_i|| l_j): FP32 operand rounding
+  double rep;   // times b_i b_j: residual of the two-term split
+  double acc;   // times ||D_i|| ||D_j||: FP32 accumulation in the MFMAs
+  double e0;    // times E0: FP64 reassembly and QCP
+};
+
 // Per-frame preparation for the pairwise contraction (k_prep.hip).
 struct PrepArgs {
   FrameSetDev fs;
@@ -64,7 +74,16 @@ struct PrepArgs {
   const double* T;      // template, SoA [3][Kp] FP64 (MFMA forms)
   int Kp;
   const double* gT;     // sum |T|^2 (device scalar)
+  // MFMA forms, per frame (4 doubles each, may be null): the error bound's
+  // {cs, ||D||_2, (sum_c ||D_c||_4^2)^(1/2), ||l||_2} (cs: the per-frame
+  // coefficient of rmsd_dev::bound_keep, from `bound` and the total weight),
+  // and the centroid {x, y, z, 0} that the FP64 recomputation of a pair needs.
+  double* nrm = nullptr;
+  double* cen = nullptr;
+  const double* total_mass = nullptr;
+  BoundCoef bound{};
 };
+
 
 struct SetupArgs {
   FrameSetDev fs;       // template source set
@@ -76,6 +95,9 @@ struct SetupArgs {
   double* P;            // out: T^T T (9)
   double* gT;           // out: sum |T|^2
   double* total_mass;   // out: sum w (or nsel)
+  // Mean-structure template (batched form): samples and their scratch.
+  int nsamp = 0;
+  double* part = nullptr;  // nsamp * 3 * Kp doubles
 };
 
 struct PairArgs {
@@ -96,6 +118,45 @@ struct PairArgs {
   int tri;              // packed upper triangle (A == B); else full FA x FB
   int clamp;            // 1: Kabsch-style clamp at 0; 0: QCP-style fabs
   float* out;
+  // Error bound of the MFMA-reduced forms (off when flags is null): per-frame
+  // norms (PrepArgs::nrm, first entry cs), the FP64 term's coefficient, the
+  // tolerance; a tile holding a pair whose bound exceeds tol gets
+  // flags[tile] = 1 (rmsd_launch_pair_fallback recomputes those pairs in FP64);
+  // stats[0] collects the largest bound of the pairs that stay (as the bits
+  // of a non-negative double).
+  const double* nrmA = nullptr;
+  const double* nrmB = nullptr;
+  double ke0 = 0.0;
+  double tol = 0.0;
+  int* flags = nullptr;
+  unsigned long long* stats = nullptr;
+};
+
+// FP64 recomputation of the pairs whose bound exceeds the tolerance
+// (k_pairwise.hip, k_pair_fallback): one workgroup per flagged tile of the
+// schedule, the bound re-evaluated from the stored output, each flagged pair
+// summed over the atoms in FP64 by one wave.
+struct FallbackArgs {
+  FrameSetDev A, B;
+  const double* smass;   // covariance weights (null: unit)
+  const double* cenA;    // per-frame centroids (PrepArgs::cen)
+  const double* cenB;
+  const double* gA;
+  const double* gB;
+  const double* nrmA;
+  const double* nrmB;
+  const double* total_mass;
+  double ke0;
+  double tol;
+  const int2* tiles;
+  int ntiles;
+  int tile_r, tile_c;    // frames per tile side (rows, columns)
+  const int* flags;
+  int FA, FB;
+  int tri;
+  int clamp;
+  float* out;
+  unsigned long long* stats;  // [1]: pairs recomputed
 };
 
 // Tiles of many matrices (k_pairwise.hip, batched kernels).
@@ -242,6 +303,22 @@ hipError_t rmsd_launch_setup_batch(const SetupArgs* sets, int nsets, hipStream_t
 hipError_t rmsd_launch_prep_batch(int prec, const PrepArgs* sets, const int* fbase, int nsets,
                                   int total_frames, hipStream_t s);
 hipError_t rmsd_launch_pairwise(int prec, const PairArgs& a, hipStream_t s);
+hipError_t rmsd_launch_pair_fallback(const FallbackArgs& a, hipStream_t s);
+hipError_t rmsd_launch_fill_int(int* p, int n, int v, hipStream_t s);
+// Mean-structure template (k_prep.hip): the first template frame's T (from
+// rmsd_launch_setup) is replaced by the mean of `nsamp` evenly spaced frames
+// of the set, each rotated onto it (FP64 QCP); P and gT are recomputed.
+// part: nsamp * 3 * Kp doubles of scratch.
+hipError_t rmsd_launch_template_mean(const SetupArgs& a, int nsamp, double* part, hipStream_t s);
+// Batched form over every set (SetupArgs::nsamp / part per set); sbase: prefix
+// of the sample counts (nsets + 1); maxk: the largest 3 * Kp.
+hipError_t rmsd_launch_template_mean_batch(const SetupArgs* sets, const int* sbase, int nsets,
+                                           int total_samples, int maxk, hipStream_t s);
+// FP64 recomputation over a batch's tile list: tile t = (set, row tile,
+// column tile, t) of tiles (tile_r x tile_c frames), fsets one per set.
+hipError_t rmsd_launch_pair_fallback_batch(const FallbackArgs* fsets, const int4* tiles,
+                                           int ntiles, int tile_r, int tile_c, const int* flags,
+                                           hipStream_t s);
 // Column frames per pair tile of rmsd_launch_pairwise (rows: 64).
 int rmsd_pairwise_tile_cols(int prec);
 int rmsd_autocorr_chunks(int n, int* rows);

@@ -14,6 +14,8 @@ Recipes (one per workload):
 * drift(F, n)     frame_k = rigid(ref + d_k), d_k = 0.99 d_{k-1} + N(0, 0.3^2)
                   per coordinate, d_0 drawn from the stationary law
                   N(0, 0.3^2 / (1 - 0.99^2)).
+* two_state(F, n) two conformations 20 A apart (a hinge), sigma 0.1 A noise,
+                  alternating frames, rigid motion (precision stress case).
 * masses(n)       i.i.d. draws from {1, 12, 14, 16, 32} with probabilities
                   0.50 / 0.30 / 0.08 / 0.10 / 0.02 (H / C / N / O / S-like).
 """
@@ -88,6 +90,47 @@ def drift(seed: int, F: int, n: int, rho: float = 0.99, sigma: float = 0.3) -> n
     for s in range(0, F, blk):
         out[s:s + blk] = _rigid(rng, X[s:s + blk])
     return out
+
+
+def two_state(seed: int, F: int, n: int, separation: float = 20.0, sigma: float = 0.1):
+    """(F, n, 3) float32 frames of two conformations `separation` Angstrom apart
+    (best-fit RMSD): state A is a chain, state B the same chain with its second
+    half rotated about the middle atom (a hinge), the angle found by bisection.
+    Frame k is in state k % 2, plus N(0, sigma^2) noise, then a rigid motion.
+    Returns (frames, state per frame)."""
+    rng = np.random.default_rng(seed)
+    A = chain(rng, n)
+    mid = n // 2
+    axis = rng.normal(size=3)
+    axis /= np.linalg.norm(axis)
+
+    def hinge(theta):
+        K = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+        R = np.eye(3) + np.sin(theta) * K + (1 - np.cos(theta)) * (K @ K)
+        B = A.copy()
+        B[mid:] = (A[mid:] - A[mid]) @ R.T + A[mid]
+        return B
+
+    def fit_rmsd(X, Y):
+        x = X - X.mean(0)
+        y = Y - Y.mean(0)
+        V, S, Wt = np.linalg.svd(x.T @ y)
+        d = np.sign(np.linalg.det(V @ Wt))
+        S[-1] *= d
+        return np.sqrt(max((x * x).sum() + (y * y).sum() - 2 * S.sum(), 0.0) / len(X))
+
+    lo, hi = 0.0, np.pi
+    for _ in range(60):
+        th = 0.5 * (lo + hi)
+        if fit_rmsd(A, hinge(th)) < separation:
+            lo = th
+        else:
+            hi = th
+    B = hinge(hi)
+    state = np.arange(F) % 2
+    Y = np.where(state[:, None, None] == 0, A[None], B[None]) + rng.normal(scale=sigma,
+                                                                            size=(F, n, 3))
+    return _rigid(rng, Y).astype(np.float32), state
 
 
 def batch_sizes(seed: int, ntraj: int = 64, f_range=(500, 20000), n_range=(50, 5000)):

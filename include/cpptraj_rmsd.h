@@ -125,6 +125,23 @@ int rmsd_ctx_stage_ms(rmsd_ctx* ctx, double* ms);
 int rmsd_pairwise(rmsd_ctx* ctx, const rmsd_frames* tgt, const rmsd_frames* ref, int mode,
                   int use_mass, int precision, float* out);
 
+/* Error bound of the mixed-precision forms (RMSD_PREC_FP32, RMSD_PREC_BF16X3)
+ * of rmsd_pairwise and rmsd_cluster_cache.  No reference counterpart: the
+ * reference computes every pair in FP64.  Each pair's result carries a stated
+ * bound on its distance from the FP64 computation (Angstrom), from per-frame
+ * norms and a rounding model of the contraction; a pair whose bound exceeds the
+ * context's tolerance is recomputed in FP64 in the same call, so every result
+ * is within the tolerance of FP64.
+ *   tol > 0   recompute above tol (default 1e-4 Angstrom);
+ *   tol == 0  no bound (pure mixed precision; the report says -1);
+ *   tol < 0   bound reported, nothing recomputed.
+ * rmsd_ctx_error_report waits for the context's stream and returns, for the
+ * last rmsd_pairwise / rmsd_cluster_cache call, the largest bound among the
+ * results kept in mixed precision (0 for FP64 and no-fit calls) and how many
+ * pairs were recomputed in FP64. */
+int rmsd_ctx_set_tolerance(rmsd_ctx* ctx, double tol);
+int rmsd_ctx_error_report(rmsd_ctx* ctx, double* max_bound, int64_t* n_fp64);
+
 /* RMSD auto-correlation of a packed triangle (rms2d 'corr').
  * Replaces Analysis_Rms2d::CalcAutoCorr (src/Analysis_Rms2d.cpp:301-319):
  * ct[0] = 1 and ct[lag] = sum_j exp(-R(j, j+lag)) / (n - lag) for

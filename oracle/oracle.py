@@ -375,7 +375,7 @@ class RefLib:
                                   _ip, _ip, _dp, _dp, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp]
         L.cref_perres.restype = C.c_int
         L.cref_avgcorr.argtypes = [_fp, C.c_int64, C.c_int, _ip, C.c_int, _dp, _dp, C.c_int,
-                                   _ip, C.c_int, _dp, _dp]
+                                   _ip, C.c_int, _dp, _dp, C.c_int]
         L.cref_avgcorr.restype = C.c_int
         L.cref_hungarian.argtypes = [_dp, C.c_int, _ip]
         L.cref_hungarian.restype = C.c_int
@@ -512,7 +512,8 @@ class RefLib:
                            int(center), _d(Uc), _d(tv), modify, _d(out))
         return out
 
-    def avgcorr(self, frames, ws, sel=None, mass=None, ref_sel_centered=None, use_mass=False):
+    def avgcorr(self, frames, ws, sel=None, mass=None, ref_sel_centered=None, use_mass=False,
+                nthreads=1):
         fr = _c32(frames)
         s = _ci(sel)
         nsel = len(s) if s is not None else fr.shape[1] // 3
@@ -521,7 +522,7 @@ class RefLib:
         ct, sd = np.zeros(len(w)), np.zeros(len(w))
         rs = None if ref_sel_centered is None else _c64(ref_sel_centered)
         self.L.cref_avgcorr(_f(fr), fr.shape[1], fr.shape[0], _i(s), nsel, _d(m), _d(rs),
-                            int(use_mass), _i(w), len(w), _d(ct), _d(sd))
+                            int(use_mass), _i(w), len(w), _d(ct), _d(sd), nthreads)
         return ct, sd
 
     def hungarian(self, M):

@@ -313,14 +313,25 @@ int cref_perres(const float* frames, int64_t stride, int nframes, int natom, con
 // Window loop of Analysis_RmsAvgCorr::Analyze (src/Analysis_RmsAvgCorr.cpp:
 // 208-257) on the reference's Frame: operator+= / operator-= for the running
 // sum, Divide(Frame, double), CenterOnOrigin for the 'first' reference and
-// RMSD_CenteredRef.  ws: the window sizes; ref_sel (centred) or NULL.
+// RMSD_CenteredRef.  ws: the window sizes; ref_sel (centred) or NULL.  As in
+// the reference's OpenMP build (:222-233), threads take window sizes with a
+// dynamic schedule, each with private frames (nthreads 0: the runtime's
+// default; the results do not depend on it).
 int cref_avgcorr(const float* frames, int64_t stride, int nframes, const int* sel, int nsel,
                  const double* mass, const double* ref_sel, int use_mass, const int* ws, int nw,
-                 double* ct, double* sd) {
+                 double* ct, double* sd, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+#endif
+  {
   std::vector<double> zero(3 * (size_t)nsel, 0.0);
   Frame Tgt = make_frame(zero.data(), mass, nsel);
   Frame Sum = make_frame(zero.data(), mass, nsel);
   Frame Ref = make_frame(ref_sel ? ref_sel : zero.data(), mass, nsel);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic)
+#endif
   for (int wi = 0; wi < nw; ++wi) {
     const int w = ws[wi];
     const double dw = (double)w;
@@ -352,6 +363,7 @@ int cref_avgcorr(const float* frames, int64_t stride, int nframes, const int* se
     sq -= avg * avg;
     ct[wi] = avg;
     sd[wi] = sq > 0.0 ? sqrt(sq) : 0.0;
+  }
   }
   return 0;
 }
