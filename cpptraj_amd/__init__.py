@@ -143,6 +143,9 @@ def _optional(L):
         L.rmsd_perres_nofit.argtypes = [P, FP, FP, C.c_int, C.c_int, IP, IP, IP, DP, DP,
                                         C.c_int, C.c_int, C.c_int, P, P, P]
         L.rmsd_perres_nofit.restype = C.c_int
+    if hasattr(L, "rmsd_rotate_ucell"):
+        L.rmsd_rotate_ucell.argtypes = [P, P, C.c_int, P]
+        L.rmsd_rotate_ucell.restype = C.c_int
     if hasattr(L, "rmsd_symm"):
         IP = C.POINTER(C.c_int32)
         L.rmsd_symm.argtypes = [P, FP, FP, C.c_int, C.c_int, IP, IP, C.c_int, C.c_int, P, P, P,
@@ -398,6 +401,21 @@ def rmsd_fit(frames, ref_frames=None, ref_index=0, mask=None, mass=None, ref_mas
     tv = torch.empty((F, 6), dtype=torch.float64, device=dev) if save_vectors else None
     ctx.fit_desc(d, r, ref_index, fit, use_mass, MODIFY[modify], rmsd, U, tv)
     return rmsd, U, tv
+
+
+def rotate_ucell(U, ucell, ctx: Context | None = None):
+    """Box::RotateUcell of the fit's update, in place: ucell (F, 3, 3) float64
+    CUDA tensor of unit cells (rows = cell vectors), U (F, 3, 3) the fit's
+    Kabsch rotations (rmsd_fit(..., save_matrices=True))."""
+    import torch
+
+    ctx = ctx or _ctx(ucell.device.index or 0)
+    if (ucell.dtype != torch.float64 or U.dtype != torch.float64 or not ucell.is_contiguous()
+            or ucell.shape[0] != U.shape[0]):
+        raise RmsdError("U and ucell must be (F, 3, 3) float64, ucell contiguous")
+    Uc = U.contiguous()
+    ctx.check(ctx._L.rmsd_rotate_ucell(ctx.handle, _ptr(Uc), int(ucell.shape[0]), _ptr(ucell)))
+    return ucell
 
 
 def symm_groups(groups):

@@ -106,6 +106,41 @@ std::vector<int2> grouped_tiles(int nbr, int nbc, int tri, int tr, int tc) {
 
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
+namespace {
+float f_ru(double x) {
+  float f = (float)x;
+  return (double)f < x ? nextafterf(f, INFINITY) : f;
+}
+float f_rd(double x) {
+  float f = (float)x;
+  return (double)f > x ? nextafterf(f, -INFINITY) : f;
+}
+}  // namespace
+
+rmsd_dev::KeepSplit rmsd_keep_split(double tol) {
+  rmsd_dev::KeepSplit k{};
+  if (!(tol > 0.0) || tol == INFINITY) {  // report only: keep sqrt(eps)+ and up
+    k.inv4t = 0.0f;
+    k.c8 = 0.0f;
+    k.h1 = k.h2 = k.e16 = k.tol = INFINITY;
+    return k;
+  }
+  const double a = 1.0 + 0x1p-23;
+  // h1 = 16 A (h1 2^-23 = 1.9e-6 taken from tol below it); for tiny tol at
+  // most half of tol.
+  k.h1 = f_rd(fmin(16.0, tol * 0x1p22));
+  const double tol1 = tol - (double)k.h1 * 0x1p-23;
+  k.inv4t = f_ru(1.0 / (4.0 * tol1));
+  k.c8 = f_ru(8.0 * tol1 * tol1);
+  // Above h1 up to h2 = 256 A when eps a h1 / (2 h1^2 - eps) <= tol - h2 2^-23.
+  k.h2 = 256.0f;
+  const double tol2 = tol - 256.0 * 0x1p-23;
+  const double h1 = k.h1;
+  k.e16 = tol2 > 0.0 ? f_rd(2.0 * tol2 * h1 * h1 / (a * h1 + tol2) * (1.0 - 0x1p-40)) : -1.0f;
+  k.tol = f_ru(tol);
+  return k;
+}
+
 BoundCoef rmsd_bound_coef(int prec, int KB) {
   BoundCoef k{};
   const double sqrt3 = 1.7320508075688772;
@@ -125,10 +160,11 @@ BoundCoef rmsd_bound_coef(int prec, int KB) {
     k.rep = BOUND_LAMBDA * 2.0 * 0x1p-16 * (1.0 + 0x1p-8) / sqrt3;
     // FP32 accumulation: 3 MFMAs per 32-atom chunk.  Measured on gfx950
     // (v_mfma_f32_16x16x32_bf16): each product is rounded to nearest, ties
-    // symmetric, on a grid 2^-26 of the accumulator's binade, and the sum is
-    // rounded to float.  Per MFMA at most 32 such errors of 2^-26 m (m bounds
-    // |acc| by Cauchy-Schwarz, ||D_i,c|| ||D_j,d||) and one of 2^-24 m.
-    const double per = sqrt(33.0) * 0x1p-26 / sqrt3 + 0x1p-24 / sqrt3;
+    // symmetric, on a grid of 2^-26 times the accumulator's binade (at most
+    // 2^-26 |acc|), and the sum is rounded to float.  Per MFMA 32 such errors
+    // of at most half the grid, 2^-27 m (m bounds |acc| by Cauchy-Schwarz,
+    // ||D_i,c|| ||D_j,d||), and one of half a float ulp, 2^-24 m.
+    const double per = sqrt(32.0) * 0x1p-27 / sqrt3 + 0x1p-24 / sqrt3;
     k.acc = BOUND_LAMBDA * sqrt(3.0 * KB) * per;
   } else if (prec == PREC_F32) {
     // D rounded to float: |D - float(D)| <= 2^-24 |D|, bounded deterministically
@@ -318,21 +354,29 @@ int pairwise_impl(rmsd_ctx* c, const FrameSetDev& A, const FrameSetDev& B, int t
   p.out = out;
   (void)rows_are_ref;
   const bool bounded = mixed && c->tol != 0.0;
+  const WaveLayout lay = rmsd_pairwise_layout(prec);
+  const int nw = lay.NWR * lay.NWC;
   unsigned long long* stats = nullptr;
   int* flags = nullptr;
+  unsigned int* masks = nullptr;
   if (bounded) {
-    flags = (int*)scratch(c, "pair.flags", sizeof(int) * (size_t)ntiles, &st);
+    flags = (int*)scratch(c, "pair.flags", sizeof(int) * (size_t)ntiles * nw, &st);
     if (!flags) return st;
+    // Lane masks are written (and read) only for flagged waves: no clearing.
+    masks = (unsigned int*)scratch(c, "pair.masks",
+                                   sizeof(unsigned int) * (size_t)ntiles * nw * 64, &st);
+    if (!masks) return st;
     stats = (unsigned long long*)scratch(c, "pair.stats", 2 * sizeof(unsigned long long), &st);
     if (!stats) return st;
     // Fill kernels, not memsets: the sequence stays valid under graph capture.
-    HIP_TRY(c, rmsd_launch_fill_int(flags, ntiles, 0, c->stream));
+    HIP_TRY(c, rmsd_launch_fill_int(flags, ntiles * nw, 0, c->stream));
     HIP_TRY(c, rmsd_launch_fill_int((int*)stats, 4, 0, c->stream));
     p.nrmA = pa.nrm;
     p.nrmB = pb.nrm;
     p.ke0 = rmsd_bound_coef(prec, p.KB).e0;
-    p.tol = c->tol > 0.0 ? c->tol : INFINITY;  // < 0: report only
+    p.keep = rmsd_keep_split(c->tol);  // < 0: report only
     p.flags = flags;
+    p.masks = masks;
     p.stats = stats;
   }
   c->report = {mixed ? (bounded ? 1 : 2) : 0, stats};
@@ -348,23 +392,18 @@ int pairwise_impl(rmsd_ctx* c, const FrameSetDev& A, const FrameSetDev& B, int t
     f.cenB = pb.cen;
     f.gA = pa.g;
     f.gB = pb.g;
-    f.nrmA = pa.nrm;
-    f.nrmB = pb.nrm;
     f.total_mass = M;
-    f.ke0 = p.ke0;
-    f.tol = p.tol;
     f.tiles = tiles;
     f.ntiles = ntiles;
-    f.tile_r = TILE;
-    f.tile_c = tc;
     f.flags = flags;
+    f.masks = masks;
     f.FA = FA;
     f.FB = FB;
     f.tri = tri;
     f.clamp = p.clamp;
     f.out = out;
     f.stats = stats;
-    HIP_TRY(c, rmsd_launch_pair_fallback(f, c->stream));
+    HIP_TRY(c, rmsd_launch_pair_fallback(f, lay, c->stream));
   }
   mark(c, 3);
   return RMSD_OK;
@@ -609,6 +648,16 @@ int rmsd_fit_to_ref(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames*
     HIP_TRY(ctx, rmsd_launch_apply(ap, modify == RMSD_MODIFY_ROT_TRANS ? 1 : 2, ctx->stream));
   }
   mark(ctx, 3);
+  return RMSD_OK;
+}
+
+int rmsd_rotate_ucell(rmsd_ctx* ctx, const double* U, int nframes, double* ucell) {
+  if (!ctx) return RMSD_ERR_ARG;
+  if (nframes < 0 || (nframes > 0 && (!U || !ucell)))
+    return fail(ctx, RMSD_ERR_ARG, "rotate_ucell: need nframes >= 0 and non-NULL U and ucell");
+  if (nframes == 0) return RMSD_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, rmsd_launch_rotate_ucell(U, nframes, ucell, ctx->stream));
   return RMSD_OK;
 }
 

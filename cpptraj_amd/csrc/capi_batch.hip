@@ -32,11 +32,13 @@ struct rmsd_batch {
   int* dsbase = nullptr;
   int total_samples = 0, maxk = 0;
   // error bound (MFMA forms, context tolerance != 0 at creation): one flag per
-  // tile of either shape's list, {largest kept bound, pairs recomputed}, and
-  // the per-set arguments of the FP64 recomputation
+  // wave and 64 lane masks per wave of either shape's tile list, {largest kept
+  // bound, pairs recomputed}, and the per-set arguments of the FP64
+  // recomputation
   bool bounded = false;
   double tol = 0.0;
   int* tflags = nullptr;
+  unsigned int* tmasks = nullptr;
   unsigned long long* stats = nullptr;
   FallbackArgs* dfall = nullptr;
   // batched kernel
@@ -147,19 +149,18 @@ int batch_enqueue(rmsd_batch* b, int v, hipStream_t st) {
   ba.maxq = b->maxq[sh];
   ba.counters = b->counters;
   if (b->ntiles[sh] == 0) return RMSD_OK;
+  const WaveLayout lay = rmsd_batch_layout(b->prec, v);
   if (b->bounded) {
-    // Fill kernels (no memset nodes in the captured graph).
-    HIP_TRY(c, rmsd_launch_fill_int(b->tflags, b->ntiles[sh], 0, st));
+    // Fill kernels (no memset nodes in the captured graph); lane masks are
+    // written and read only for flagged waves.
+    HIP_TRY(c, rmsd_launch_fill_int(b->tflags, b->ntiles[sh] * lay.NWR * lay.NWC, 0, st));
     HIP_TRY(c, rmsd_launch_fill_int((int*)b->stats, 4, 0, st));
   }
   if (v & 1) HIP_TRY(c, rmsd_launch_batch_reset(ba, st));
   HIP_TRY(c, rmsd_launch_pairwise_batch(b->prec, v, ba, st));
-  if (b->bounded && b->tol > 0.0) {
-    int tr = 0, tc = 0;
-    rmsd_batch_tile_shape(b->prec, v, &tr, &tc);
-    HIP_TRY(c, rmsd_launch_pair_fallback_batch(b->dfall, b->dtiles[sh], b->ntiles[sh], tr, tc,
-                                               b->tflags, st));
-  }
+  if (b->bounded && b->tol > 0.0)  // tol < 0: report only
+    HIP_TRY(c, rmsd_launch_pair_fallback_batch(b->dfall, b->dtiles[sh], b->ntiles[sh], lay,
+                                               b->tflags, b->tmasks, st));
   return RMSD_OK;
 }
 
@@ -310,7 +311,9 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
   std::vector<int4> th[2];
   std::vector<int> qh[2];
   size_t o_args = 0, o_tiles[2] = {0, 0}, o_q[2] = {0, 0}, o_cnt = 0;
-  size_t o_setup = 0, o_prep = 0, o_fbase = 0, o_sbase = 0, o_flags = 0, o_stats = 0, o_fall = 0;
+  size_t o_setup = 0, o_prep = 0, o_fbase = 0, o_sbase = 0, o_flags = 0, o_stats = 0;
+  size_t o_fall = 0, o_masks = 0;
+  size_t nwaves = 0;
   int nactive = 0;
   for (int s = 0; s < nsets; ++s) nactive += (mode != RMSD_MODE_NOFIT && sets[s].nframes >= 2);
   if (mode != RMSD_MODE_NOFIT) {
@@ -337,8 +340,15 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
     if (mixed) {
       o_sbase = off;
       off += al(sizeof(int) * (nactive + 1));
+      for (int sh = 0; sh < 2; ++sh) {
+        const WaveLayout L = rmsd_batch_layout(b->prec, 2 * sh);
+        nwaves = std::max<size_t>(nwaves, (size_t)b->ntiles[sh] * L.NWR * L.NWC);
+      }
+      nwaves = std::max<size_t>(nwaves, 1);
       o_flags = off;
-      off += al(sizeof(int) * std::max(1, std::max(b->ntiles[0], b->ntiles[1])));
+      off += al(sizeof(int) * nwaves);
+      o_masks = off;
+      off += al(sizeof(unsigned int) * 64 * nwaves);
       o_stats = off;
       off += al(2 * sizeof(unsigned long long));
       o_fall = off;
@@ -360,9 +370,10 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
   b->cen.assign(nsets, nullptr);
   b->part.assign(nsets, nullptr);
   b->bounded = mixed && ctx->tol != 0.0;
-  b->tol = ctx->tol > 0.0 ? ctx->tol : INFINITY;  // < 0: report only
+  b->tol = ctx->tol;
   if (mixed) {
     b->tflags = (int*)(base + o_flags);
+    b->tmasks = (unsigned int*)(base + o_masks);
     b->stats = (unsigned long long*)(base + o_stats);
     b->dfall = (FallbackArgs*)(base + o_fall);
   }
@@ -402,8 +413,9 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
     if (b->bounded) {
       p.nrmA = p.nrmB = b->nrm[s];
       p.ke0 = rmsd_bound_coef(b->prec, p.KB).e0;
-      p.tol = b->tol;
+      p.keep = rmsd_keep_split(b->tol);
       p.flags = b->tflags;
+      p.masks = b->tmasks;
       p.stats = b->stats;
       FallbackArgs& f = fall[s];
       memset(&f, 0, sizeof(f));
@@ -411,10 +423,7 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
       f.smass = use_mass ? sets[s].mass : nullptr;
       f.cenA = f.cenB = b->cen[s];
       f.gA = f.gB = b->g[s];
-      f.nrmA = f.nrmB = b->nrm[s];
       f.total_mass = b->consts[s];
-      f.ke0 = p.ke0;
-      f.tol = p.tol;
       f.FA = f.FB = sets[s].nframes;
       f.tri = 1;
       f.clamp = p.clamp;

@@ -93,6 +93,8 @@ std::vector<int2> grouped_tiles(int nbr, int nbc, int tri, int tr = TILE, int tc
 int64_t round_up(int64_t x, int64_t m);
 // Coefficients of the per-pair error bound for a precision and chunk count.
 BoundCoef rmsd_bound_coef(int prec, int KB);
+// Host split of the tolerance for keep_limits (tol < 0: report only).
+rmsd_dev::KeepSplit rmsd_keep_split(double tol);
 // Frees a batch plan (capi_batch.hip); rmsd_ctx_destroy frees leftovers.
 void batch_free(rmsd_batch* b);
 

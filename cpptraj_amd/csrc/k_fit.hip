@@ -567,6 +567,37 @@ hipError_t rmsd_launch_ref_prep(const FrameSetDev& fs, int f, const double* rmas
 }
 
 // moments must hold 13 doubles per frame followed by 3 per frame of shift.
+// Box::RotateUcell after the rotate + translate update (src/Action_Rmsd.cpp:
+// 381, src/Box.cpp:732-745): the three unit-cell vectors of frame f (rows of
+// its 3x3 cell) rotated by the frame's Kabsch U, row k' = U row k, in the
+// reference's operation order.
+__global__ __launch_bounds__(256) void k_rotate_ucell(const double* U, int64_t nframes,
+                                                      double* ucell) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nframes) return;
+  double u[9], b[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    u[k] = U[9 * f + k];
+    b[k] = ucell[9 * f + k];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      ucell[9 * f + 3 * k + c] = __dadd_rn(__dadd_rn(__dmul_rn(u[3 * c], b[3 * k]),
+                                                     __dmul_rn(u[3 * c + 1], b[3 * k + 1])),
+                                           __dmul_rn(u[3 * c + 2], b[3 * k + 2]));
+}
+
+hipError_t rmsd_launch_rotate_ucell(const double* U, int64_t nframes, double* ucell,
+                                    hipStream_t s) {
+  if (nframes <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rotate_ucell, dim3((unsigned)((nframes + 255) / 256)), dim3(256), 0, s, U,
+                     nframes, ucell);
+  return hipGetLastError();
+}
+
 hipError_t rmsd_launch_fit_moments(const FitArgs& a, hipStream_t s) {
   const int ldr = (a.fs.nsel + 3) & ~3;
   const int nf = a.fs.nframes;

@@ -117,17 +117,51 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
                                               int64_t fbA, int64_t fbB, int lane,
                                               const double* gA, const double* uA, int64_t rbase,
                                               const double* gB, const double* uB, int64_t cbase,
-                                              int tile_id) {
+                                              int tile_id, int wave, int nwaves) {
   const double M = args.total_mass[0];
-  // Error bound (MFMA-reduced forms with a flag array): per lane, whether a
-  // pair went over the tolerance, the kept pair with the largest eps / r
-  // (compared by cross-multiplication: no division per pair) and the largest
-  // kept r; for tol = infinity (report only) also the bounds of the pairs
-  // that fail r^2 > eps.
+  // Error bound (MFMA-reduced forms with a flag array).  One eps bounds the
+  // pairs of a lane's column j with its 8 rows (the largest row coefficient
+  // times j's, plus the FP64 term at the largest E0), and keep_limits turns it
+  // into a float interval of RMSDs whose stated bound is within tol: per pair
+  // two compares.  Pairs outside it are recorded in the lane's mask.  The 8
+  // rows (4 (lane / 16) + v of two 16-frame blocks starting on a multiple of
+  // 32) are the same in every tile shape with WR = 2, so which pairs are
+  // recomputed -- and every result -- does not depend on the shape.
+  static_assert(PREC == PREC_F64 || WR == 2, "the bound's row groups assume WR = 2");
   const bool bounded = PREC != PREC_F64 && args.flags != nullptr && M >= RMSD_SMALL;
-  const double ke0M = bounded ? args.ke0 / M : 0.0;
-  bool over = false;
-  double qe = 0.0, qr = 1.0, rmax = 0.0, bover = 0.0;
+  float klo[WC], khi[WC], eps[WC];
+  if (bounded) {
+    double csr = 0.0, gr = 0.0;
+#pragma unroll
+    for (int r = 0; r < WR; ++r)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t i = (fbA + r) * 16 + 4 * (lane >> 4) + v;
+        if (i < args.FA) {
+          csr = fmax(csr, args.nrmA[4 * i]);
+          gr = fmax(gr, gA[i - rbase]);
+        }
+      }
+#pragma unroll
+    for (int q = 0; q < WC; ++q) {
+      const int64_t j = (fbB + q) * 16 + (lane & 15);
+      double csc = 0.0, gc = 0.0;
+      if (j < args.FB) {
+        csc = args.nrmB[4 * j];
+        gc = gB[j - cbase];
+      }
+      eps[q] = __double2float_ru(csr * csc + args.ke0 / M * 0.5 * (gr + gc));
+      rmsd_dev::keep_limits(eps[q], args.keep, &klo[q], &khi[q]);
+    }
+  }
+  unsigned int bits = 0;
+  float kmin[WC], kmax[WC], omax[WC];
+#pragma unroll
+  for (int q = 0; q < WC; ++q) {
+    kmin[q] = INFINITY;
+    kmax[q] = 0.0f;
+    omax[q] = -1.0f;  // >= 0 once a pair of the column is outside [klo, khi]
+  }
   double Pm[9];
   if (PREC != PREC_F64) {
 #pragma unroll
@@ -136,14 +170,13 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
 #pragma unroll
   for (int q = 0; q < WC; ++q) {
     const int64_t j = (fbB + q) * 16 + (lane & 15);  // column frame
-    double gj = 0.0, uj[9], csj = 0.0;
+    double gj = 0.0, uj[9];
     const bool jok = j < args.FB;
     if (jok) {
       gj = gB[j - cbase];
       if (PREC != PREC_F64) {
 #pragma unroll
         for (int k = 0; k < 9; ++k) uj[k] = uB[9 * (j - cbase) + k];
-        if (bounded) csj = args.nrmB[4 * j];
       }
     }
 #pragma unroll
@@ -184,122 +217,219 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
         }
         const double E0 = 0.5 * (gA[i - rbase] + gj);
         const int64_t idx = args.tri ? rmsd_dev::tri_index(args.FA, i, j) : i * args.FB + j;
-        const float r = M < RMSD_SMALL ? -1.0f : (float)rmsd_dev::qcp_rmsd(R, E0, M, args.clamp);
-        args.out[idx] = r;
+        const float res = M < RMSD_SMALL ? -1.0f : (float)rmsd_dev::qcp_rmsd(R, E0, M, args.clamp);
+        args.out[idx] = res;
         if (bounded) {
-          const double rr = (double)r;
-          const double eps = args.nrmA[4 * i] * csj + ke0M * E0;
-          if (rmsd_dev::bound_keep(eps, rr, args.tol)) {
-            if (eps * qr > qe * rr) {
-              qe = eps;
-              qr = rr;
-            }
-            rmax = fmax(rmax, rr);
+          if (res >= klo[q] && res <= khi[q]) {  // NaN fails: recomputed in FP64
+            kmin[q] = fminf(kmin[q], res);
+            kmax[q] = fmaxf(kmax[q], res);
           } else {
-            over = true;  // NaN included: recomputed in FP64
-            if (args.tol == INFINITY) bover = fmax(bover, rmsd_dev::bound_value(eps, rr));
+            bits |= 1u << ((q * WR + r) * 4 + v);
+            omax[q] = fmaxf(omax[q], res);
           }
         }
       }
     }
   }
   if (bounded) {
-    // One flag store and one atomic per wave.
-    if (__ballot(over) && lane == 0) args.flags[tile_id] = 1;
-    double bmax = fmax(qe / qr * (1.0 + 0x1p-23) + rmax * 0x1p-23, bover);
-    for (int off = 32; off > 0; off >>= 1) bmax = fmax(bmax, __shfl_xor(bmax, off, 64));
-    if (lane == 0 && bmax > 0.0)
-      atomicMax(args.stats, (unsigned long long)__double_as_longlong(bmax));
+    // Flag and masks: one store per lane of a wave with a pair over tol.
+    const int64_t w = (int64_t)tile_id * nwaves + wave;
+    if (__ballot(bits != 0)) {
+      args.masks[w * 64 + lane] = bits;
+      if (lane == 0) args.flags[w] = 1;
+    }
+    // The largest stated bound of the kept pairs (<= tol by construction).
+    // Report only (tol = infinity): also the pairs below sqrt(eps)+.
+    float b = 0.0f;
+#pragma unroll
+    for (int q = 0; q < WC; ++q) {
+      if (kmax[q] > 0.0f)
+        b = fmaxf(b, fminf(rmsd_dev::bound_kept(eps[q], kmin[q], kmax[q]), args.keep.tol));
+      if (omax[q] >= 0.0f && args.keep.tol == INFINITY)
+        b = fmaxf(b, rmsd_dev::bound_any(eps[q], omax[q]));
+    }
+    for (int off = 32; off > 0; off >>= 1) b = fmaxf(b, __shfl_xor(b, off, 64));
+    if (lane == 0 && b > 0.0f)
+      atomicMax(args.stats, (unsigned long long)__double_as_longlong((double)b));
   }
 }
 
 // ---- FP64 recomputation of flagged pairs -------------------------------------------
-// One workgroup per tile of the schedule; tiles without a flag leave at once.
-// The bound is evaluated again from the stored float RMSD (the epilogue
-// decided on the same float with the same test, so both agree) and the pairs
-// over the tolerance are listed in LDS; then each thread recomputes whole
-// pairs, one at a time, summing over the atoms in FP64 from the frames as
-// stored, centred with the per-frame centroids of k_prep, and solves them with
-// the same FP64 QCP.  Consecutive threads take consecutive pairs of a tile row:
-// the row frame's atoms are one broadcast load, the column frames' lines stay
-// in the vector cache across the atom loop.
+// One workgroup per tile of the schedule; tiles without a flagged wave leave at
+// once.  The pairs come from the contraction's lane masks (so both sides
+// agree on them by construction) and are listed in LDS with the tile's frames
+// they use.  Those frames are then staged chunk by chunk in LDS as centred
+// (and sqrt(mass)-scaled) FP64 coordinates, from the frames as stored and the
+// per-frame centroids of k_prep, and each thread sums FALLBACK_PPT listed
+// pairs over the chunk; the pairs are solved with the same FP64 QCP.  (The
+// tile's frames do not stay in L2 -- an XCD's concurrent tiles hold far more --
+// so per-pair loads from memory were latency-bound: 18 ms for 2.3e6 pairs.)
 constexpr int FALLBACK_THREADS = 256;
-constexpr int FALLBACK_MAX_PAIRS = 64 * 96;  // largest tile of any schedule
+constexpr int FALLBACK_MAX_PAIRS = 64 * 96;   // largest tile of any schedule
+constexpr int FALLBACK_MAX_FRAMES = 64 + 96;  // its row + column frames
+constexpr int FALLBACK_KCH = 14;              // atoms per staged chunk
+constexpr int FALLBACK_PPT = 4;               // pairs per thread per round
+// staging elements per thread per chunk (all loads issued before any store)
+constexpr int FALLBACK_STAGE = (FALLBACK_MAX_FRAMES * FALLBACK_KCH + FALLBACK_THREADS - 1) /
+                               FALLBACK_THREADS;
 
-__device__ __forceinline__ void centred_atom(const FrameSetDev& fs, int64_t f, int k,
-                                             const double* cen, const double* w, double x[3]) {
-  float p[3];
-  load_atom(fs, f, k, p);
-  const double s = w ? sqrt(w[k]) : 1.0;
-  x[0] = ((double)p[0] - cen[0]) * s;
-  x[1] = ((double)p[1] - cen[1]) * s;
-  x[2] = ((double)p[2] - cen[2]) * s;
-}
-
-__device__ __forceinline__ void fallback_tile(const FallbackArgs& a, int2 tile, int tile_r,
-                                              int tile_c) {
+__device__ __forceinline__ void fallback_tile(const FallbackArgs& a, int2 tile, int64_t tile_id,
+                                              WaveLayout L) {
   __shared__ unsigned short list[FALLBACK_MAX_PAIRS];
-  __shared__ int nlist;
+  __shared__ int nlist, nused;
+  __shared__ short cidx[FALLBACK_MAX_FRAMES];   // tile frame slot -> staged index (-1: unused)
+  __shared__ short cslot[FALLBACK_MAX_FRAMES];  // staged index -> tile frame slot
+  __shared__ double cen[FALLBACK_MAX_FRAMES][3];
+  __shared__ double stage[FALLBACK_MAX_FRAMES][3][FALLBACK_KCH + 1];
+  const int nw = L.NWR * L.NWC;
+  const int tile_r = 16 * L.WR * L.NWR, tile_c = 16 * L.WC * L.NWC;
+  const int nslots = tile_r + tile_c;
   const double M = a.total_mass[0];
-  const double ke0M = a.ke0 / M;
-  const int npairs = tile_r * tile_c;
   if (threadIdx.x == 0) nlist = 0;
+  for (int s = threadIdx.x; s < nslots; s += FALLBACK_THREADS) cidx[s] = -1;
   __syncthreads();
-  for (int p = threadIdx.x; p < npairs; p += FALLBACK_THREADS) {
-    const int64_t i = (int64_t)tile.x * tile_r + p / tile_c;
-    const int64_t j = (int64_t)tile.y * tile_c + p % tile_c;
-    if (i < a.FA && j < a.FB && (!a.tri || i < j)) {
-      const int64_t idx = a.tri ? rmsd_dev::tri_index(a.FA, i, j) : i * a.FB + j;
-      const double eps = a.nrmA[4 * i] * a.nrmB[4 * j] + ke0M * 0.5 * (a.gA[i] + a.gB[j]);
-      if (!rmsd_dev::bound_keep(eps, (double)a.out[idx], a.tol))
-        list[atomicAdd(&nlist, 1)] = (unsigned short)p;
+  // Decode the lane masks: bit (q WR + r) 4 + v of lane l in wave w is the
+  // pair at row 16 (wr WR + r) + 4 (l / 16) + v, column 16 (wc WC + q) + l % 16
+  // of the tile (wr = w / NWC, wc = w % NWC).
+  if (threadIdx.x < nw * 64) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (a.flags[tile_id * nw + w]) {
+      unsigned int m = a.masks[(tile_id * nw + w) * 64 + lane];
+      const int wr = w / L.NWC, wc = w % L.NWC;
+      while (m) {
+        const int b = __builtin_ctz(m);
+        m &= m - 1;
+        const int v = b & 3, rq = b >> 2, r = rq % L.WR, q = rq / L.WR;
+        const int il = 16 * (wr * L.WR + r) + 4 * (lane >> 4) + v;
+        const int jl = 16 * (wc * L.WC + q) + (lane & 15);
+        list[atomicAdd(&nlist, 1)] = (unsigned short)(il * tile_c + jl);
+        cidx[il] = 0;  // marks: slot used
+        cidx[tile_r + jl] = 0;
+      }
     }
   }
   __syncthreads();
   const int n = nlist;
-  for (int q = threadIdx.x; q < n; q += FALLBACK_THREADS) {
-    const int p = list[q];
-    const int64_t i = (int64_t)tile.x * tile_r + p / tile_c;
-    const int64_t j = (int64_t)tile.y * tile_c + p % tile_c;
-    const double* ci = a.cenA + 4 * i;
-    const double* cj = a.cenB + 4 * j;
-    double S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int k = 0; k < a.A.nsel; ++k) {
-      double x[3], y[3];
-      centred_atom(a.A, i, k, ci, a.smass, x);
-      centred_atom(a.B, j, k, cj, a.smass, y);
+  if (n == 0) return;
+  if (threadIdx.x == 0) {
+    int c = 0;
+    for (int s = 0; s < nslots; ++s)
+      if (cidx[s] == 0) {
+        cidx[s] = (short)c;
+        cslot[c++] = (short)s;
+      }
+    nused = c;
+  }
+  __syncthreads();
+  const int nu = nused;
+  for (int c = threadIdx.x; c < nu; c += FALLBACK_THREADS) {
+    const int s = cslot[c];
+    const double* src = s < tile_r ? a.cenA + 4 * ((int64_t)tile.x * tile_r + s)
+                                   : a.cenB + 4 * ((int64_t)tile.y * tile_c + (s - tile_r));
+    cen[c][0] = src[0];
+    cen[c][1] = src[1];
+    cen[c][2] = src[2];
+  }
+  const int nsel = a.A.nsel;
+  for (int base = 0; base < n; base += FALLBACK_THREADS * FALLBACK_PPT) {
+    int si[FALLBACK_PPT], sj[FALLBACK_PPT];
+    double S[FALLBACK_PPT][9];
+#pragma unroll
+    for (int u = 0; u < FALLBACK_PPT; ++u) {
+      const int q = base + threadIdx.x + FALLBACK_THREADS * u;
+      si[u] = sj[u] = -1;
+      if (q < n) {
+        const int p = list[q];
+        si[u] = cidx[p / tile_c];
+        sj[u] = cidx[tile_r + p % tile_c];
+      }
+#pragma unroll
+      for (int e = 0; e < 9; ++e) S[u][e] = 0.0;
+    }
+    for (int k0 = 0; k0 < nsel; k0 += FALLBACK_KCH) {
+      __syncthreads();  // the previous chunk's reads (and cen) are done
+      float px[FALLBACK_STAGE][3];
+#pragma unroll
+      for (int t = 0; t < FALLBACK_STAGE; ++t) {
+        const int e = threadIdx.x + FALLBACK_THREADS * t;
+        const int c = e / FALLBACK_KCH, k = k0 + e % FALLBACK_KCH;
+        px[t][0] = px[t][1] = px[t][2] = 0.0f;
+        if (c < nu && k < nsel) {
+          const int s = cslot[c];
+          if (s < tile_r) load_atom(a.A, (int64_t)tile.x * tile_r + s, k, px[t]);
+          else load_atom(a.B, (int64_t)tile.y * tile_c + (s - tile_r), k, px[t]);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < FALLBACK_STAGE; ++t) {
+        const int e = threadIdx.x + FALLBACK_THREADS * t;
+        const int c = e / FALLBACK_KCH, kk = e % FALLBACK_KCH, k = k0 + kk;
+        if (c < nu) {
+          const double sw = (k < nsel && a.smass) ? sqrt(a.smass[k]) : 1.0;
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            stage[c][d][kk] = k < nsel ? ((double)px[t][d] - cen[c][d]) * sw : 0.0;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < FALLBACK_PPT; ++u) {
+        if (si[u] < 0) continue;
+#pragma unroll
+        for (int kk = 0; kk < FALLBACK_KCH; ++kk) {
+          const double x0 = stage[si[u]][0][kk], x1 = stage[si[u]][1][kk], x2 = stage[si[u]][2][kk];
+          const double y0 = stage[sj[u]][0][kk], y1 = stage[sj[u]][1][kk], y2 = stage[sj[u]][2][kk];
+          S[u][0] += x0 * y0; S[u][1] += x0 * y1; S[u][2] += x0 * y2;
+          S[u][3] += x1 * y0; S[u][4] += x1 * y1; S[u][5] += x1 * y2;
+          S[u][6] += x2 * y0; S[u][7] += x2 * y1; S[u][8] += x2 * y2;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < FALLBACK_PPT; ++u) {
+      const int q = base + threadIdx.x + FALLBACK_THREADS * u;
+      if (q >= n) continue;
+      const int p = list[q];
+      const int64_t i = (int64_t)tile.x * tile_r + p / tile_c;
+      const int64_t j = (int64_t)tile.y * tile_c + p % tile_c;
+      double R[9];
+      // As in the epilogue: rows are the reference in the triangle layout.
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) S[3 * c + d] += x[c] * y[d];
+        for (int d = 0; d < 3; ++d) R[3 * c + d] = a.tri ? S[u][3 * d + c] : S[u][3 * c + d];
+      const double E0 = 0.5 * (a.gA[i] + a.gB[j]);
+      const int64_t idx = a.tri ? rmsd_dev::tri_index(a.FA, i, j) : i * a.FB + j;
+      a.out[idx] = (float)rmsd_dev::qcp_rmsd(R, E0, M, a.clamp);
     }
-    double R[9];
-    // As in the epilogue: rows are the reference in the triangle layout.
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int d = 0; d < 3; ++d) R[3 * c + d] = a.tri ? S[3 * d + c] : S[3 * c + d];
-    const double E0 = 0.5 * (a.gA[i] + a.gB[j]);
-    const int64_t idx = a.tri ? rmsd_dev::tri_index(a.FA, i, j) : i * a.FB + j;
-    a.out[idx] = (float)rmsd_dev::qcp_rmsd(R, E0, M, a.clamp);
   }
-  if (threadIdx.x == 0 && n) atomicAdd(a.stats + 1, (unsigned long long)n);
+  if (threadIdx.x == 0) atomicAdd(a.stats + 1, (unsigned long long)n);
 }
 
-__global__ __launch_bounds__(FALLBACK_THREADS) void k_pair_fallback(FallbackArgs a) {
+// A tile with no flagged wave leaves before touching LDS.
+__device__ __forceinline__ bool tile_flagged(const int* flags, int64_t t, int nw) {
+  int any = 0;
+  for (int w = 0; w < nw; ++w) any |= flags[t * nw + w];
+  return any != 0;
+}
+
+__global__ __launch_bounds__(FALLBACK_THREADS) void k_pair_fallback(FallbackArgs a, WaveLayout L) {
   const int t = blockIdx.x;
-  if (t >= a.ntiles || a.flags[t] == 0) return;
-  fallback_tile(a, a.tiles[t], a.tile_r, a.tile_c);
+  if (t >= a.ntiles || !tile_flagged(a.flags, t, L.NWR * L.NWC)) return;
+  fallback_tile(a, a.tiles[t], t, L);
 }
 
 __global__ __launch_bounds__(FALLBACK_THREADS) void k_pair_fallback_batch(const FallbackArgs* fsets,
                                                              const int4* tiles, int ntiles,
-                                                             int tile_r, int tile_c,
-                                                             const int* flags) {
+                                                             WaveLayout L, const int* flags,
+                                                             const unsigned int* masks) {
   const int t = blockIdx.x;
-  if (t >= ntiles || flags[t] == 0) return;
+  if (t >= ntiles || !tile_flagged(flags, t, L.NWR * L.NWC)) return;
   const int4 tl = tiles[t];
-  fallback_tile(fsets[tl.x], make_int2(tl.y, tl.z), tile_r, tile_c);
+  FallbackArgs a = fsets[tl.x];
+  a.flags = flags;
+  a.masks = masks;
+  fallback_tile(a, make_int2(tl.y, tl.z), t, L);
 }
 
 // One 64 x 64 tile of frame pairs: contraction over all atom chunks, then
@@ -376,7 +506,7 @@ __device__ __forceinline__ void pair_tile(const PairArgs& args, int2 tile, int t
   }
 
   pair_epilogue<PREC, WR, WC>(args, acc, fbA, fbB, lane, args.gA, args.uA, 0, args.gB, args.uB, 0,
-                              tile_id);
+                              tile_id, wave, NWR * NWC);
 }
 
 template <int PREC, int WR, int WC, int NWR, int NWC>
@@ -455,6 +585,16 @@ static hipError_t launch_tiles(const PairArgs& a, hipStream_t s) {
 // on the 50k-frame matrix).
 int rmsd_pairwise_tile_cols(int prec) { return prec == PREC_F64 ? 64 : 96; }
 
+// The <WR, WC, NWR, NWC> of the launches below.
+WaveLayout rmsd_pairwise_layout(int prec) {
+  return prec == PREC_F64 ? WaveLayout{1, 2, 4, 2} : WaveLayout{2, 3, 2, 2};
+}
+
+WaveLayout rmsd_batch_layout(int prec, int variant) {
+  if ((variant >> 1) == 0) return rmsd_pairwise_layout(prec);
+  return prec == PREC_F64 ? WaveLayout{1, 2, 2, 1} : WaveLayout{2, 2, 1, 1};
+}
+
 hipError_t rmsd_launch_pairwise(int prec, const PairArgs& a, hipStream_t s) {
   switch (prec) {
     case PREC_F64: return launch_tiles<PREC_F64, 1, 2, 4, 2>(a, s);
@@ -496,20 +636,26 @@ static hipError_t launch_batch_shape(const BatchArgs& b, bool persist, hipStream
   return hipGetLastError();
 }
 
-hipError_t rmsd_launch_pair_fallback(const FallbackArgs& a, hipStream_t s) {
+static bool fallback_fits(WaveLayout L) {
+  const int tr = 16 * L.WR * L.NWR, tc = 16 * L.WC * L.NWC;
+  return tr * tc <= FALLBACK_MAX_PAIRS && tr + tc <= FALLBACK_MAX_FRAMES &&
+         L.NWR * L.NWC * 64 <= FALLBACK_THREADS && L.WR * L.WC * 4 <= 32;
+}
+
+hipError_t rmsd_launch_pair_fallback(const FallbackArgs& a, WaveLayout L, hipStream_t s) {
   if (a.ntiles <= 0) return hipSuccess;
-  if (a.tile_r * a.tile_c > FALLBACK_MAX_PAIRS) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_pair_fallback, dim3(a.ntiles), dim3(FALLBACK_THREADS), 0, s, a);
+  if (!fallback_fits(L)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_pair_fallback, dim3(a.ntiles), dim3(FALLBACK_THREADS), 0, s, a, L);
   return hipGetLastError();
 }
 
 hipError_t rmsd_launch_pair_fallback_batch(const FallbackArgs* fsets, const int4* tiles,
-                                           int ntiles, int tile_r, int tile_c, const int* flags,
-                                           hipStream_t s) {
+                                           int ntiles, WaveLayout L, const int* flags,
+                                           const unsigned int* masks, hipStream_t s) {
   if (ntiles <= 0) return hipSuccess;
-  if (tile_r * tile_c > FALLBACK_MAX_PAIRS) return hipErrorInvalidValue;
+  if (!fallback_fits(L)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_pair_fallback_batch, dim3(ntiles), dim3(FALLBACK_THREADS), 0, s, fsets,
-                     tiles, ntiles, tile_r, tile_c, flags);
+                     tiles, ntiles, L, flags, masks);
   return hipGetLastError();
 }
 

@@ -343,7 +343,11 @@ __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
       const double b2 = sqrt(u[10]) + sqrt(u[11]) + sqrt(u[12]);  // sum_c ||D_c||_4^2
       const double l2 = u[13];                                 // ||l||^2
       const BoundCoef& k = a.bound;
-      const double c2 = k.drop * l2 + k.mix * (l2 + d2) + k.rep * b2 + k.acc * d2;
+      // c^2 = kdrop l^2 + krep b^2 + (sqrt(kacc) ||D|| + kmix / sqrt(kacc) l)^2
+      // (rmsd_dev::bound_keep; kacc > 0 for every MFMA form).
+      const double sa = sqrt(k.acc);
+      const double t = sa * sqrt(d2) + (k.mix / sa) * sqrt(l2);
+      const double c2 = k.drop * l2 + k.rep * b2 + t * t;
       const double M = a.total_mass ? a.total_mass[0] : 0.0;
       a.nrm[4 * f] = M >= RMSD_SMALL ? sqrt(c2 * (2.0 * 1.7320508075688772 / M)) : 0.0;
       a.nrm[4 * f + 1] = sqrt(d2);

@@ -114,29 +114,66 @@ __device__ __forceinline__ double qcp_rmsd(const double* S, double E0, double M,
 // computation.  The model (coefficients k*, per-frame norms ||D||, b, l from
 // k_prep) bounds the covariance error
 //   ||dS||_F <= kdrop l_i l_j + kmix (l_i ||D_j|| + ||D_i|| l_j)
-//             + krep b_i b_j + kacc ||D_i|| ||D_j||  =  u_i . v_j
-// with u_f, v_f vectors of equal norm, so ||dS||_F <= c_f(i) c_f(j),
-// c_f^2 = kdrop l^2 + kmix (l^2 + ||D||^2) + krep b^2 + kacc ||D||^2 (Cauchy-
-// Schwarz: one number per frame).  dS moves the largest QCP eigenvalue by at
-// most sqrt(3) ||dS||_F (lambda = max over rotations U of <U, S>,
-// ||U||_F = sqrt(3)), so r^2 = 2 (E0 - lambda) / M moves by at most
+//             + krep b_i b_j + kacc ||D_i|| ||D_j||
+//           <= u_i . u_j,  u_f = (sqrt(kdrop) l, sqrt(krep) b,
+//                                 sqrt(kacc) ||D|| + kmix / sqrt(kacc) l)
+// (the last components' product adds kmix^2 / kacc l_i l_j >= 0), and by
+// Cauchy-Schwarz ||dS||_F <= c_i c_j with c_f = |u_f|: one number per frame;
+// the u_f of different frames are nearly parallel, so this loses little.
+// dS moves the largest QCP eigenvalue by at most sqrt(3) ||dS||_F
+// (lambda = max over rotations U of <U, S>, ||U||_F = sqrt(3)), so
+// r^2 = 2 (E0 - lambda) / M moves by at most
 //   eps = cs_i cs_j + ke0 E0 / M,   cs_f = c_f sqrt(2 sqrt(3) / M)
 // (k_prep stores cs_f; ke0 E0 covers the FP64 reassembly and solve).  For the
 // stored float r with r^2 > eps the FP64 result lies within
-// eps / (r + sqrt(r^2 - eps)) <= eps / r, and the float roundings of both
-// outputs add at most (r + eps / r) 2^-23, so
-//   B = eps / r (1 + 2^-23) + r 2^-23
-// is the stated bound.  bound_keep: B <= tol, without a division.
-__device__ __forceinline__ bool bound_keep(double eps, double r, double tol) {
-  return r * r > eps && eps * (1.0 + 0x1p-23) <= (tol - r * 0x1p-23) * r;
+// eps / (r + sqrt(r^2 - eps)) <= eps r / (2 r^2 - eps) = b (as
+// sqrt(r^2 - eps) >= r - eps / r), and the float roundings of both outputs add
+// at most (r + b) 2^-23, so
+//   B = b (1 + 2^-23) + r 2^-23
+// is the stated bound, valid for any float r with r^2 > eps; for any r the
+// FP64 result also lies in [0, sqrt(r^2 + eps)], so within sqrt(r^2 + eps) of r.
+//
+// Keep test (keep_limits): the host splits tol once per launch (KeepSplit).
+// For r <= h1 the rounding term r 2^-23 is at most h1 2^-23, so with
+// tol1 = tol - h1 2^-23 and a = 1 + 2^-23
+//   B <= tol  <=  eps a r <= tol1 (2 r^2 - eps)  <=>  r >= R0,
+//   R0 = (a eps + sqrt(a^2 eps^2 + 8 tol1^2 eps)) / (4 tol1),
+// the larger root of that quadratic.  b decreases above sqrt(eps / 2), so when
+// eps a h1 / (2 h1^2 - eps) <= tol - h2 2^-23 (eps <= e16) every r in [h1, h2]
+// is kept as well.  keep_limits evaluates R0 and sqrt(eps) in float with every
+// rounding (and the hardware square root's 1 ulp) covered by a 2^-18 margin
+// upwards, so every float r in [lo, hi] has r^2 > eps and B <= tol: one eps
+// serves many pairs with two float compares each.  Report only: lo =
+// sqrt(eps)+, hi = infinity.
+struct KeepSplit {
+  float inv4t;  // 1 / (4 tol1), rounded up (0: report only)
+  float c8;     // 8 tol1^2, rounded up
+  float h1, h2; // kept ranges end at h1, or h2 when eps <= e16
+  float e16;
+  float tol;    // tol rounded up (infinity: report only)
+};
+
+__device__ __forceinline__ void keep_limits(float e, const KeepSplit& k, float* lo, float* hi) {
+  const float t1 = e * (1.0f + 0x1p-22f);  // >= a e after rounding
+  const float s = __builtin_amdgcn_sqrtf(fmaf(t1, t1, k.c8 * e));
+  const float r0 = (t1 + s) * k.inv4t * (1.0f + 0x1p-18f);
+  const float rs = __builtin_amdgcn_sqrtf(e) * (1.0f + 0x1p-18f);
+  *lo = fmaxf(r0, rs);  // NaN eps: lo NaN, every compare fails (recomputed)
+  *hi = e <= k.e16 ? k.h2 : k.h1;
 }
 
-// B of a kept pair (r^2 > eps); for the others the bound is sqrt(r^2 + eps)
-// plus the float term.
-__device__ __forceinline__ double bound_value(double eps, double r) {
-  if (r * r > eps) return eps / r * (1.0 + 0x1p-23) + r * 0x1p-23;
-  const double b = sqrt(r * r + eps);
-  return b + (r + b) * 0x1p-23;
+// Upper bounds on B, in float with margins: kept pairs of one eps (e >= eps)
+// with RMSDs in [kmin, kmax] (kmin >= sqrt(e)+: b is largest at kmin, the
+// rounding term at kmax), and pairs up to r = rmax by the sqrt(r^2 + eps) form.
+__device__ __forceinline__ float bound_kept(float e, float kmin, float kmax) {
+  const float den = fmaf(2.0f * kmin, kmin, -e) * (1.0f - 0x1p-20f);
+  const float b = e * (1.0f + 0x1p-22f) * kmin * __builtin_amdgcn_rcpf(den) * (1.0f + 0x1p-18f);
+  return b + kmax * (0x1p-23f * (1.0f + 0x1p-20f));
+}
+
+__device__ __forceinline__ float bound_any(float e, float rmax) {
+  const float b = __builtin_amdgcn_sqrtf(fmaf(rmax, rmax, e)) * (1.0f + 0x1p-18f);
+  return (b + (rmax + b) * 0x1p-23f) * (1.0f + 0x1p-20f);
 }
 
 // Optimal rotation for covariance S at eigenvalue lam, written as the Kabsch

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "rmsd_device.hpp"
+
 enum { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16X3 = 2 };
 
 // Atoms per fragment chunk for each precision: one 16-byte piece per lane
@@ -120,22 +122,31 @@ struct PairArgs {
   float* out;
   // Error bound of the MFMA-reduced forms (off when flags is null): per-frame
   // norms (PrepArgs::nrm, first entry cs), the FP64 term's coefficient, the
-  // tolerance; a tile holding a pair whose bound exceeds tol gets
-  // flags[tile] = 1 (rmsd_launch_pair_fallback recomputes those pairs in FP64);
-  // stats[0] collects the largest bound of the pairs that stay (as the bits
-  // of a non-negative double).
+  // tolerance.  Wave w of tile t with a pair whose bound exceeds tol sets
+  // flags[t * waves + w] = 1 and stores each lane's pairs over it as a bit
+  // mask, masks[(t * waves + w) * 64 + lane] (bit (q WR + r) 4 + v: its
+  // accumulator block (r, q), item v); rmsd_launch_pair_fallback recomputes
+  // those pairs in FP64.  stats[0] collects the largest bound of the pairs
+  // that stay (as the bits of a non-negative double).
   const double* nrmA = nullptr;
   const double* nrmB = nullptr;
   double ke0 = 0.0;
-  double tol = 0.0;
+  rmsd_dev::KeepSplit keep{};
   int* flags = nullptr;
+  unsigned int* masks = nullptr;
   unsigned long long* stats = nullptr;
 };
 
+// Wave layout of a contraction variant: each wave owns (16 WR) x (16 WC) frame
+// pairs of an NWR x NWC grid of waves.
+struct WaveLayout {
+  int WR, WC, NWR, NWC;
+};
+
 // FP64 recomputation of the pairs whose bound exceeds the tolerance
-// (k_pairwise.hip, k_pair_fallback): one workgroup per flagged tile of the
-// schedule, the bound re-evaluated from the stored output, each flagged pair
-// summed over the atoms in FP64 by one wave.
+// (k_pairwise.hip, k_pair_fallback): one workgroup per tile of the schedule,
+// the pairs read from the contraction's wave flags and lane masks
+// (PairArgs), summed over the atoms in FP64 from LDS-staged frames.
 struct FallbackArgs {
   FrameSetDev A, B;
   const double* smass;   // covariance weights (null: unit)
@@ -143,15 +154,11 @@ struct FallbackArgs {
   const double* cenB;
   const double* gA;
   const double* gB;
-  const double* nrmA;
-  const double* nrmB;
   const double* total_mass;
-  double ke0;
-  double tol;
   const int2* tiles;
   int ntiles;
-  int tile_r, tile_c;    // frames per tile side (rows, columns)
   const int* flags;
+  const unsigned int* masks;
   int FA, FB;
   int tri;
   int clamp;
@@ -303,7 +310,11 @@ hipError_t rmsd_launch_setup_batch(const SetupArgs* sets, int nsets, hipStream_t
 hipError_t rmsd_launch_prep_batch(int prec, const PrepArgs* sets, const int* fbase, int nsets,
                                   int total_frames, hipStream_t s);
 hipError_t rmsd_launch_pairwise(int prec, const PairArgs& a, hipStream_t s);
-hipError_t rmsd_launch_pair_fallback(const FallbackArgs& a, hipStream_t s);
+// Wave layout of rmsd_launch_pairwise / rmsd_launch_pairwise_batch variants
+// (flags and masks hold NWR * NWC waves per tile).
+WaveLayout rmsd_pairwise_layout(int prec);
+WaveLayout rmsd_batch_layout(int prec, int variant);
+hipError_t rmsd_launch_pair_fallback(const FallbackArgs& a, WaveLayout w, hipStream_t s);
 hipError_t rmsd_launch_fill_int(int* p, int n, int v, hipStream_t s);
 // Mean-structure template (k_prep.hip): the first template frame's T (from
 // rmsd_launch_setup) is replaced by the mean of `nsamp` evenly spaced frames
@@ -315,10 +326,10 @@ hipError_t rmsd_launch_template_mean(const SetupArgs& a, int nsamp, double* part
 hipError_t rmsd_launch_template_mean_batch(const SetupArgs* sets, const int* sbase, int nsets,
                                            int total_samples, int maxk, hipStream_t s);
 // FP64 recomputation over a batch's tile list: tile t = (set, row tile,
-// column tile, t) of tiles (tile_r x tile_c frames), fsets one per set.
+// column tile, t), fsets one per set (their flags / masks are the plan's).
 hipError_t rmsd_launch_pair_fallback_batch(const FallbackArgs* fsets, const int4* tiles,
-                                           int ntiles, int tile_r, int tile_c, const int* flags,
-                                           hipStream_t s);
+                                           int ntiles, WaveLayout w, const int* flags,
+                                           const unsigned int* masks, hipStream_t s);
 // Column frames per pair tile of rmsd_launch_pairwise (rows: 64).
 int rmsd_pairwise_tile_cols(int prec);
 int rmsd_autocorr_chunks(int n, int* rows);
@@ -336,6 +347,8 @@ hipError_t rmsd_launch_fit_solve(const SolveArgs& a, hipStream_t s);
 hipError_t rmsd_launch_nofit_frames(const NofitArgs& a, const double* consts, double* rmsd,
                                     hipStream_t s);
 hipError_t rmsd_launch_apply(const ApplyArgs& a, int mode, hipStream_t s);
+hipError_t rmsd_launch_rotate_ucell(const double* U, int64_t nframes, double* ucell,
+                                    hipStream_t s);
 hipError_t rmsd_launch_ref_prep(const FrameSetDev& fs, int f, const double* rmass,
                                 const double* tmass, int center, double* ref_out,
                                 double* ref_trans, double* consts, hipStream_t s);

@@ -48,6 +48,8 @@ def load_host():
     L.chost_get_matrix.argtypes = [P, C.c_char_p, FP, C.POINTER(I64), C.POINTER(I64), IP]
     L.chost_get_series.argtypes = [P, C.c_char_p, DP, C.POINTER(I64), IP]
     L.chost_get_coords.argtypes = [P, C.c_char_p, FP, C.POINTER(I64)]
+    L.chost_set_box.argtypes = [P, C.c_char_p, DP, I64]
+    L.chost_get_box.argtypes = [P, C.c_char_p, DP, C.POINTER(I64)]
     L.chost_read_cmatrix.argtypes = [C.c_char_p, FP, C.POINTER(I64), C.POINTER(I64), IP, IP]
     L.chost_write_matrix.argtypes = [C.c_char_p, FP, I64, I64, I, I, C.c_char_p]
     L.chost_write_series.argtypes = [C.c_char_p, I, C.POINTER(C.c_char_p), IP, IP, IP,
@@ -63,7 +65,8 @@ def load_host():
         getattr(L, f).restype = I
     for f in ("chost_create", "chost_add_coords", "chost_add_reference", "chost_add_mask",
               "chost_run", "chost_get_matrix", "chost_get_series", "chost_get_coords",
-              "chost_read_cmatrix", "chost_set_residues", "chost_set_symmetric_groups"):
+              "chost_read_cmatrix", "chost_set_residues", "chost_set_symmetric_groups",
+              "chost_set_box", "chost_get_box"):
         getattr(L, f).restype = I
     _hl = L
     return L
@@ -109,6 +112,18 @@ class HostSession:
         m = None if mass is None else np.ascontiguousarray(mass, dtype=np.float64)
         self._check(self._L.chost_add_coords(self._h, name.encode(), _fp(x), x.shape[0], st,
                                              natom, _dp(m)))
+
+    def set_box(self, name, ucell):
+        """Unit cells of a COORDS set's frames: (nframes, 3, 3), rows = cell vectors."""
+        u = np.ascontiguousarray(ucell, dtype=np.float64).reshape(-1, 9)
+        self._check(self._L.chost_set_box(self._h, name.encode(), _dp(u), u.shape[0]))
+
+    def box(self, name):
+        n = C.c_int64()
+        self._check(self._L.chost_get_box(self._h, name.encode(), None, C.byref(n)))
+        out = np.zeros((max(n.value, 1), 9))
+        self._check(self._L.chost_get_box(self._h, name.encode(), _dp(out), None))
+        return out[: n.value].reshape(-1, 3, 3)
 
     def add_reference(self, name, frame, natom, mass=None):
         x = np.ascontiguousarray(frame, dtype=np.float32).reshape(-1)

@@ -93,6 +93,9 @@ struct CoordsSet {
   Topology top;
   int64_t stride = 0;
   PinnedFloats xyz;
+  // Unit cell of every frame (9 doubles, rows = cell vectors, Box::UnitCell),
+  // or empty when the set has no box.
+  PinnedDoubles box;
   int nframes() const { return stride ? (int)(xyz.size() / (size_t)stride) : 0; }
   const float* frame(int f) const { return xyz.data() + (int64_t)f * stride; }
   float* frame(int f) { return xyz.data() + (int64_t)f * stride; }

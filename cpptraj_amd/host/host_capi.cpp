@@ -46,6 +46,33 @@ int chost_add_coords(chost_session* s, const char* name, const float* xyz, int64
   return 0;
 }
 
+int chost_set_box(chost_session* s, const char* name, const double* ucell, int64_t nframes) {
+  if (!s || !name || nframes < 0 || (nframes > 0 && !ucell)) return 1;
+  CoordsSet* c = s->s.dsl.FindCoordsSet(name);
+  if (!c) {
+    s->err = std::string("no COORDS set '") + name + "'";
+    return 1;
+  }
+  if (nframes != 0 && nframes != c->nframes()) {
+    s->err = "chost_set_box: one unit cell per frame expected";
+    return 1;
+  }
+  c->box.assign(ucell, ucell + 9 * nframes);
+  return 0;
+}
+
+int chost_get_box(chost_session* s, const char* name, double* out, int64_t* nframes) {
+  if (!s || !name) return 1;
+  const CoordsSet* c = s->s.dsl.FindCoordsSet(name);
+  if (!c) {
+    s->err = std::string("no COORDS set '") + name + "'";
+    return 1;
+  }
+  if (nframes) *nframes = (int64_t)(c->box.size() / 9);
+  if (out && !c->box.empty()) memcpy(out, c->box.data(), sizeof(double) * c->box.size());
+  return 0;
+}
+
 int chost_add_reference(chost_session* s, const char* name, const float* xyz, int natom,
                         const double* mass) {
   if (!s || !name || !xyz || natom < 1) {

@@ -25,6 +25,12 @@ const char* chost_last_error(const chost_session* s);
  * natom masses (NULL: all 1). */
 int chost_add_coords(chost_session* s, const char* name, const float* xyz, int64_t nframes,
                      int64_t stride, int natom, const double* mass);
+/* Unit cells of a COORDS set's frames: 9 doubles per frame, the rows being
+ * the three cell vectors (Box::UnitCell).  Actions that rotate the frames
+ * rotate the cells with them (Box::RotateUcell).  chost_get_box: NULL out
+ * gives the frame count (0 without a box). */
+int chost_set_box(chost_session* s, const char* name, const double* ucell, int64_t nframes);
+int chost_get_box(chost_session* s, const char* name, double* out, int64_t* nframes);
 /* Reference frame ('ref <name>' / 'reference'): one frame of 3*natom floats. */
 int chost_add_reference(chost_session* s, const char* name, const float* xyz, int natom,
                         const double* mass);

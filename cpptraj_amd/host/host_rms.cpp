@@ -635,14 +635,18 @@ RetType Action_Rmsd::RunBatch(CoordsSet& crd, const AtomMask& tsel, int f0, int 
   // then the update is applied.
   const bool pr = perresReady_;
   const int prn = pr ? (int)prSeries_.size() : 0;
-  const bool needUT = rmatrices_ || moving || (pr && fit_);
+  // Box::RotateUcell after the rotate + translate update (src/Action_Rmsd.cpp:
+  // 381; Action_Align::DoAction, src/Action_Align.cpp:98).
+  const bool rotbox = fit_ && (modify == RMSD_MODIFY_ROT_TRANS || moving) &&
+                      crd.box.size() == 9 * (size_t)crd.nframes() && !crd.box.empty();
+  const bool needUT = rmatrices_ || moving || (pr && fit_) || rotbox;
   if (pr)
     for (int j = 0; j < prn; ++j) resRMS_[prSeries_[j]].data->v.resize(base_r + nf);
   struct SlotBufs {
     rmsd_ctx* ctx = nullptr;
     hipStream_t st = nullptr;
     float* dx = nullptr;
-    double *dr = nullptr, *dU = nullptr, *dT = nullptr, *dpr = nullptr;
+    double *dr = nullptr, *dU = nullptr, *dT = nullptr, *dpr = nullptr, *dbox = nullptr;
     PinnedDoubles hr, hU, hT, hpr;
     int c = -1;  // chunk in flight
   } sl[2];
@@ -656,8 +660,9 @@ RetType Action_Rmsd::RunBatch(CoordsSet& crd, const AtomMask& tsel, int f0, int 
     if (needUT) sl[k].dU = (double*)g.Buffer(t + ".U", sizeof(double) * 9 * chunk);
     if (tvecs_ || needUT) sl[k].dT = (double*)g.Buffer(t + ".T", sizeof(double) * 6 * chunk);
     if (pr) sl[k].dpr = (double*)g.Buffer(t + ".perres", sizeof(double) * (size_t)prn * chunk);
+    if (rotbox) sl[k].dbox = (double*)g.Buffer(t + ".box", sizeof(double) * 9 * chunk);
     if (!sl[k].dx || !sl[k].dr || (needUT && !sl[k].dU) || ((tvecs_ || needUT) && !sl[k].dT) ||
-        (pr && !sl[k].dpr))
+        (pr && !sl[k].dpr) || (rotbox && !sl[k].dbox))
       return ERR;
     sl[k].hr.resize(chunk);
     if (pr) sl[k].hpr.resize((size_t)prn * chunk);
@@ -728,6 +733,17 @@ RetType Action_Rmsd::RunBatch(CoordsSet& crd, const AtomMask& tsel, int f0, int 
                                          RMSD_MODIFY_ROT_TRANS),
                           "align"))
       return ERR;
+    if (rotbox) {
+      double* hbox = crd.box.data() + 9 * (size_t)(f0 + c0);
+      if (!hip_ok(hipMemcpyAsync(b.dbox, hbox, sizeof(double) * 9 * n, hipMemcpyHostToDevice,
+                                 b.st),
+                  "upload") ||
+          g.Check(rmsd_rotate_ucell(b.ctx, b.dU, n, b.dbox), "rms box") ||
+          !hip_ok(hipMemcpyAsync(hbox, b.dbox, sizeof(double) * 9 * n, hipMemcpyDeviceToHost,
+                                 b.st),
+                  "download"))
+        return ERR;
+    }
     if (pr && !hip_ok(hipMemcpyAsync(b.hpr.data(), b.dpr, sizeof(double) * (size_t)prn * n,
                                      hipMemcpyDeviceToHost, b.st),
                       "download"))

@@ -176,6 +176,16 @@ int rmsd_fit_to_ref(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames*
                     int ref_frame, int fit, int use_mass, int modify, double* rmsd, double* U,
                     double* tvec);
 
+/* Unit-cell rotation of the fit's coordinate update.
+ * Replaces Box::RotateUcell (src/Box.cpp:732-745) as Action_Rmsd::DoAction
+ * applies it after Trans_Rot_Trans in the default rotate + translate mode
+ * (src/Action_Rmsd.cpp:378-382): frame f's unit cell (ucell[9f .. 9f+8], rows
+ * are the three cell vectors, device memory) becomes row k' = U_f row k, with
+ * U_f the fit's Kabsch rotation (rmsd_fit_to_ref's U output).  Lengths,
+ * angles and volume are unchanged; the fractional cell is the inverse of the
+ * result.  Frames without a box are simply not passed. */
+int rmsd_rotate_ucell(rmsd_ctx* ctx, const double* U, int nframes, double* ucell);
+
 /* Per-frame rigid update of selected atoms.
  * Replaces Frame::Trans_Rot_Trans(AtomMask, ...) as Action_Align::DoAction
  * uses it for 'align ... move <mask>' (src/Action_Align.cpp): for every stored

@@ -113,6 +113,8 @@ class Oracle:
         L.orc_rmsd_nofit.restype = C.c_double
         L.orc_tri_index.argtypes = [C.c_int64] * 3
         L.orc_tri_index.restype = C.c_int64
+        L.orc_rotate_ucell.argtypes = [_dp, _dp, _dp]
+        L.orc_rotate_ucell.restype = None
         L.orc_full_index.argtypes = [C.c_int64] * 3
         L.orc_full_index.restype = C.c_int64
         L.orc_rms2d.argtypes = [_fp, C.c_int64, C.c_int, _fp, C.c_int64, C.c_int,
@@ -328,6 +330,12 @@ class Oracle:
             raise MemoryError("orc_avgcorr failed")
         return w, ct, sd
 
+    def rotate_ucell(self, U, ucell):
+        """Box::RotateUcell: rows of ucell (3x3) rotated by the Kabsch U."""
+        out = np.zeros(9)
+        self.L.orc_rotate_ucell(_d(_c64(U).ravel()), _d(_c64(ucell).ravel()), _d(out))
+        return out.reshape(3, 3)
+
     def hungarian(self, M):
         """Hungarian::Optimize restated: map[col] = row."""
         return _hungarian(self.L.orc_hungarian, M)
@@ -379,6 +387,12 @@ class RefLib:
         L.cref_avgcorr.restype = C.c_int
         L.cref_hungarian.argtypes = [_dp, C.c_int, _ip]
         L.cref_hungarian.restype = C.c_int
+        L.cref_rotate_ucell.argtypes = [_dp, _dp, _dp, _dp]
+        L.cref_rotate_ucell.restype = C.c_int
+        L.cref_ucell_from_xyzabg.argtypes = [_dp, _dp, _dp]
+        L.cref_ucell_from_xyzabg.restype = C.c_int
+        L.cref_dist2_image_nonortho.argtypes = [_dp, _dp, _dp, _dp]
+        L.cref_dist2_image_nonortho.restype = C.c_double
         L.cref_symm_frames.argtypes = [_fp, C.c_int64, C.c_int, _ip, C.c_int, _dp, _dp, C.c_int,
                                        C.c_int, C.c_int, _ip, _ip, _dp, _ip, _dp, _dp]
         L.cref_symm_frames.restype = C.c_int
@@ -524,6 +538,25 @@ class RefLib:
         self.L.cref_avgcorr(_f(fr), fr.shape[1], fr.shape[0], _i(s), nsel, _d(m), _d(rs),
                             int(use_mass), _i(w), len(w), _d(ct), _d(sd), nthreads)
         return ct, sd
+
+    def rotate_ucell(self, U, ucell):
+        """Compiled Box::SetupFromUcell + RotateUcell: (unit cell, fractional cell)."""
+        u, f = np.zeros(9), np.zeros(9)
+        if self.L.cref_rotate_ucell(_d(_c64(ucell).ravel()), _d(_c64(U).ravel()), _d(u), _d(f)):
+            raise ValueError("no box")
+        return u.reshape(3, 3), f.reshape(3, 3)
+
+    def ucell_from_xyzabg(self, box):
+        u, f = np.zeros(9), np.zeros(9)
+        if self.L.cref_ucell_from_xyzabg(_d(_c64(box).ravel()), _d(u), _d(f)):
+            raise ValueError("no box")
+        return u.reshape(3, 3), f.reshape(3, 3)
+
+    def dist_image(self, a, b, ucell, frac):
+        """Minimum-image distance (compiled DIST2_ImageNonOrtho)."""
+        return float(np.sqrt(self.L.cref_dist2_image_nonortho(
+            _d(_c64(a).ravel()), _d(_c64(b).ravel()), _d(_c64(ucell).ravel()),
+            _d(_c64(frac).ravel()))))
 
     def hungarian(self, M):
         """The reference's Hungarian::Optimize: map[col] = row."""

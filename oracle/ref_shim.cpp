@@ -22,6 +22,7 @@
 #include "qcprot.h"
 #include "Hungarian.h"
 #include "DistRoutines.h"
+#include "Box.h"
 
 namespace {
 
@@ -433,6 +434,42 @@ int cref_symm_frames(const float* frames, int64_t stride, int nframes, const int
     if (trans) for (int k = 0; k < 3; ++k) trans[3 * (int64_t)f + k] = fit ? tt[k] : 0.0;
   }
   return 0;
+}
+
+
+// Box::RotateUcell (src/Box.cpp:732-745) as Action_Rmsd::DoAction calls it
+// after the rotate + translate update (src/Action_Rmsd.cpp:381): the box is
+// set up from the unit cell (rows = cell vectors), rotated by the Kabsch U,
+// and its unit cell and recomputed fractional cell are returned.
+int cref_rotate_ucell(const double* ucell, const double* U, double* ucell_out, double* frac_out) {
+  Box b;
+  if (b.SetupFromUcell(ucell)) return 1;
+  b.RotateUcell(Matrix_3x3(U));
+  const double* u = b.UnitCell().Dptr();
+  const double* f = b.FracCell().Dptr();
+  for (int k = 0; k < 9; ++k) {
+    ucell_out[k] = u[k];
+    if (frac_out) frac_out[k] = f[k];
+  }
+  return 0;
+}
+
+// Box::SetupFromXyzAbg: unit and fractional cell from lengths and angles.
+int cref_ucell_from_xyzabg(const double* xyzabg, double* ucell_out, double* frac_out) {
+  Box b;
+  if (b.SetupFromXyzAbg(xyzabg)) return 1;
+  for (int k = 0; k < 9; ++k) {
+    ucell_out[k] = b.UnitCell().Dptr()[k];
+    frac_out[k] = b.FracCell().Dptr()[k];
+  }
+  return 0;
+}
+
+// DIST2_ImageNonOrtho (src/DistRoutines.cpp): squared minimum-image distance
+// in a general cell, as the distance action computes it with imaging.
+double cref_dist2_image_nonortho(const double* a, const double* b, const double* ucell,
+                                 const double* frac) {
+  return DIST2_ImageNonOrtho(a, b, Matrix_3x3(ucell), Matrix_3x3(frac));
 }
 
 }  // extern "C"

@@ -1172,3 +1172,17 @@ int orc_symm_frames(const float *frames, int64_t stride, int nframes, const int 
   }
   return err;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Box::RotateUcell (src/Box.cpp:732-745), applied after the rotate +
+ * translate update of Action_Rmsd::DoAction (src/Action_Rmsd.cpp:381): each
+ * unit-cell vector (a row of ucell) is rotated by the Kabsch U, row k' = U row k.
+ * The reference recomputes the fractional cell from the result; that is the
+ * inverse of ucell' and left to the caller.                                 */
+void orc_rotate_ucell(const double U[9], const double ucell[9], double out[9])
+{
+  for (int k = 0; k < 3; ++k)
+    for (int c = 0; c < 3; ++c)
+      out[3 * k + c] = U[3 * c] * ucell[3 * k] + U[3 * c + 1] * ucell[3 * k + 1] +
+                       U[3 * c + 2] * ucell[3 * k + 2];
+}

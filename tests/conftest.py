@@ -55,3 +55,31 @@ def tri_to_square(tri, F):
     M[iu] = tri
     M[(iu[1], iu[0])] = tri
     return M
+
+
+def ucell_from_box(box):
+    """Unit cell (rows = cell vectors) of an orthorhombic box (lengths, 90 deg
+    angles), as Box::SetupFromXyzAbg gives it for that case."""
+    box = np.asarray(box, np.float64)
+    assert np.allclose(box[3:], 90.0), box
+    return np.diag(box[:3])
+
+
+def min_image_distance(a, b, ucell):
+    """Minimum-image distance between points a and b in the periodic cell
+    `ucell` (rows = cell vectors): the fractional difference wrapped to the
+    nearest image, then the 27 neighbouring images searched (exact for any
+    cell whose reduced vectors are no longer than its diagonal; pinned against
+    the compiled DIST2_ImageNonOrtho in test_oracle.py)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    inv = np.linalg.inv(ucell)
+    f = (b - a) @ inv
+    f -= np.round(f)
+    best = np.inf
+    for i in (-1, 0, 1):
+        for j in (-1, 0, 1):
+            for k in (-1, 0, 1):
+                d = (f + np.array([i, j, k], np.float64)) @ ucell
+                best = min(best, float(d @ d))
+    return np.sqrt(best)

@@ -8,6 +8,7 @@ Run in the build container (where the reference tree exists):
 Inputs (read as data, never executed):
   test/tz2.parm7, test/tz2.nc                  (223 atoms, 101 frames)
   test/tz2.truncoct.parm7, test/tz2.truncoct.nc (5827 atoms, 10 frames)
+  test/tz2.ortho.parm7, test/tz2.ortho.nc       (5293 atoms, 10 frames, box)
   test/rGACC.full.parm7, test/rGACC.full.nc     (7622 atoms, 1 frame)
   test/Test_2DRMS/*.save, test/Test_RMSD/*.save (expected outputs)
   test/AFV.parm7, test/AFV.nc, test/Test_SymmRmsd/* (symmetric RMSD)
@@ -20,6 +21,11 @@ Outputs (numpy .npz, no pickles):
   tz2_rotate.npz   reference = tz2.nc frame 1, targets = tz2.truncoct.nc with
                    waters stripped, @CA mask, expected rotation matrices and the
                    expected rotated / translated coordinates
+  tz2_ortho.npz    tz2.ortho.nc (orthorhombic box) all 10 frames: the :1-13
+                   solute and the atoms of residues 559 and 1412, masses,
+                   box lengths and angles, and Test_RMSD's distances.agr.save
+                   (distance between the two residues without / with imaging
+                   after 'rms :1-13 first' rotated coordinates and box)
   tz2_camask.npz   tz2.nc all 101 frames, :2-12@CA atoms, expected 'nomod' and
                    'previous' RMSD series
   rgacc.npz        rGACC.full.nc frame 1, :1-4&!@H= atoms (9 identical frames ->
@@ -341,6 +347,35 @@ def main():
         rmatrices=read_series(os.path.join(T, "Test_RMSD", "rmatrices.dat.save")),
         norotate=read_crd(os.path.join(T, "Test_RMSD", "tz2.norotate.crd.save"), len(solute), 3),
         rotate=read_crd(os.path.join(T, "Test_RMSD", "tz2.rotate.crd.save"), len(solute), 3),
+    )
+
+    # ---- Test_RMSD imaging after unit-cell rotation (tz2.ortho) ----------------
+    # 'rms :1-13 first' rotates coordinates and box; the distances between the
+    # centres of mass of residues 559 and 1412 without and with imaging
+    # (distances.agr.save) then show whether the box was rotated with them.
+    # Kept: the solute (:1-13, the fit mask) and the two residues' atoms.
+    po = Parm(os.path.join(T, "tz2.ortho.parm7"))
+    fo = netcdf_file(os.path.join(T, "tz2.ortho.nc"), "r", mmap=False)
+    xo = np.array(fo.variables["coordinates"].data, dtype=np.float32)
+    box = np.concatenate([np.array(fo.variables["cell_lengths"].data, dtype=np.float64),
+                          np.array(fo.variables["cell_angles"].data, dtype=np.float64)], axis=1)
+    fo.close()
+    sol = po.select(res=[(1, 13)])
+    r559 = po.select(res=[(559, 559)])
+    r1412 = po.select(res=[(1412, 1412)])
+    keep = np.concatenate([sol, r559, r1412])
+    legends, dist = read_agr(os.path.join(T, "Test_RMSD", "distances.agr.save"))
+    assert legends == ["NoImage", "Image"], legends
+    np.savez_compressed(
+        os.path.join(OUT, "tz2_ortho.npz"),
+        frames=xo[:, keep, :].reshape(xo.shape[0], -1),
+        box=box,
+        nfit=np.int32(len(sol)),
+        res_a=np.arange(len(sol), len(sol) + len(r559), dtype=np.int32),
+        res_b=np.arange(len(sol) + len(r559), len(keep), dtype=np.int32),
+        mass=po.mass[keep],
+        dist_noimage=dist[0],
+        dist_image=dist[1],
     )
 
     # ---- Test_RMSD nomod / previous (:2-12@CA over 101 frames) ---------------

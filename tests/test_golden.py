@@ -9,7 +9,7 @@ precision: the tolerance is half a unit in the last printed digit.
 import numpy as np
 import pytest
 
-from conftest import tri_to_square
+from conftest import min_image_distance, tri_to_square, ucell_from_box
 
 HALF3 = 0.0005 + 1e-6   # %8.3f
 HALF4 = 0.00005 + 1e-7  # %12.4f / %8.4f
@@ -40,6 +40,9 @@ class OracleBackend:
         r, U, tv, fo = self.o.fit_frames(frames, rc, rt, sel=sel, mass=mass, use_mass=use_mass,
                                          modify=modify, want_frames=True)
         return r, U, tv, fo
+
+    def rotate_ucell(self, U, ucell):
+        return np.stack([self.o.rotate_ucell(u, c) for u, c in zip(U, ucell)])
 
     def perres(self, frames, fit_sel, res_off, atoms, center=False):
         """rmsd <fit_sel> perres against frame 0: fit, then per-residue no-fit
@@ -119,6 +122,12 @@ class GpuBackend:
                                modify={0: "nomod", 1: "rottrans", 2: "norotate"}[modify],
                                save_matrices=True, save_vectors=True, ctx=self.ctx)
         return (r.cpu().numpy(), U.cpu().numpy(), tv.cpu().numpy(), X.cpu().numpy())
+
+    def rotate_ucell(self, U, ucell):
+        import cpptraj_amd as ca
+        u = self._dev(ucell, self.t.float64)
+        ca.rotate_ucell(self._dev(U, self.t.float64), u, ctx=self.ctx)
+        return u.cpu().numpy()
 
     def perres(self, frames, fit_sel, res_off, atoms, center=False):
         import cpptraj_amd as ca
@@ -263,6 +272,36 @@ def test_rmsd_rotation_matrices_and_coordinates(golden, backend):
     _, _, _, f3 = backend.fit(f1, sel=d["sel_ca"], modify=1, ref_frame=ref,
                               ref_sel=d["sel_ca_ref"])
     assert np.abs(f3.reshape(10, -1, 3) - d["rotate"]).max() <= HALF3
+
+
+def test_rmsd_box_rotation_imaging(golden, backend):
+    """Test_RMSD 'imaging after unit cell rotation' (distances.agr.save):
+    'rms :1-13 first' on tz2.ortho rotates every atom and the unit cell
+    (Box::RotateUcell); the distance between the centres of mass of residues
+    559 and 1412 is then the same without imaging on frames where they are
+    close, and the minimum image in the rotated cell where they are not.
+    Without the cell rotation the imaged distances of frames 1-3 and 10
+    would come out wrong."""
+    g = golden("tz2_ortho")
+    X, box, nfit = g["frames"], g["box"], int(g["nfit"])
+    F = X.shape[0]
+    sel = np.arange(nfit, dtype=np.int32)
+    _, U, _, moved = backend.fit(X, sel=sel, modify=1)
+    U = np.asarray(U).reshape(F, 3, 3)
+    ucell = np.stack([ucell_from_box(b) for b in box])
+    rot = backend.rotate_ucell(U, ucell)
+    m = g["mass"]
+    Y = np.asarray(moved, np.float64).reshape(F, -1, 3)
+    ra, rb = g["res_a"], g["res_b"]
+    ca = (Y[:, ra] * m[ra, None]).sum(1) / m[ra].sum()
+    cb = (Y[:, rb] * m[rb, None]).sum(1) / m[rb].sum()
+    noimage = np.linalg.norm(cb - ca, axis=1)
+    image = np.array([min_image_distance(ca[f], cb[f], rot[f]) for f in range(F)])
+    assert np.abs(noimage - g["dist_noimage"]).max() <= HALF4
+    assert np.abs(image - g["dist_image"]).max() <= HALF4
+    # the unrotated cell gives other imaged distances: the rotation is observable
+    wrong = np.array([min_image_distance(ca[f], cb[f], ucell[f]) for f in range(F)])
+    assert np.abs(wrong - g["dist_image"]).max() > 0.1
 
 
 def test_rmsd_nomod_series(golden, backend):

@@ -10,7 +10,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, tri_to_square
+from conftest import GOLDEN, min_image_distance, tri_to_square, ucell_from_box
 
 pytestmark = pytest.mark.gpu
 TXT = os.path.join(GOLDEN, "text")
@@ -138,6 +138,66 @@ def test_rms_rotation(hs, golden, tmp_path):
     assert np.abs(a - b).max() <= 1.5e-9
     rot = hs.coords("WAT2").reshape(10, 220, 3)
     assert np.abs(rot - g["rotate"]).max() <= 5e-4 + 1e-6
+
+
+def ortho_distances(X, box, g):
+    """(no image, image) distances between the centres of mass of the fixture's
+    two residues, per frame, in the given cells."""
+    m = g["mass"]
+    Y = np.asarray(X, np.float64).reshape(len(X), -1, 3)
+    ra, rb = g["res_a"], g["res_b"]
+    ca = (Y[:, ra] * m[ra, None]).sum(1) / m[ra].sum()
+    cb = (Y[:, rb] * m[rb, None]).sum(1) / m[rb].sum()
+    return (np.linalg.norm(cb - ca, axis=1),
+            np.array([min_image_distance(ca[f], cb[f], box[f]) for f in range(len(X))]))
+
+
+@pytest.mark.parametrize("cmd", ["rms :1-13 first", "align :1-13", "rms :1-13 first chunk 3"])
+def test_rms_rotates_the_unit_cell(hs, golden, oracle, cmd):
+    """Test_RMSD 'imaging after unit cell rotation': 'rms :1-13 first' on a
+    COORDS set with a box rotates coordinates and unit cells (Box::RotateUcell,
+    src/Action_Rmsd.cpp:381; align: src/Action_Align.cpp:98); the distances of
+    distances.agr.save then follow, and the cells equal the oracle's whole
+    chain (its Jacobi U differs from the GPU's QCP U by ~5e-10) to 1e-7."""
+    g = golden("tz2_ortho")
+    X, nfit = g["frames"], int(g["nfit"])
+    F, n = X.shape[0], X.shape[1] // 3
+    ucell = np.stack([ucell_from_box(b) for b in g["box"]])
+    hs.add_coords("ortho", X, n, g["mass"])
+    hs.set_box("ortho", ucell)
+    hs.add_mask(":1-13", np.arange(nfit))
+    hs.run(cmd)
+    moved = hs.coords("ortho").reshape(F, n, 3)
+    box = hs.box("ortho")
+    noimage, image = ortho_distances(moved, box, g)
+    HALF4 = 0.00005 + 1e-7
+    assert np.abs(noimage - g["dist_noimage"]).max() <= HALF4
+    assert np.abs(image - g["dist_image"]).max() <= HALF4
+    ref = X[0].reshape(-1, 3)[:nfit].astype(np.float64)
+    rc, rt = oracle.center_on_origin(ref)
+    _, U_o, _, _ = oracle.fit_frames(X, rc, rt, sel=np.arange(nfit, dtype=np.int32))
+    exp = np.stack([oracle.rotate_ucell(U_o[f], ucell[f]) for f in range(F)])
+    assert np.abs(box - exp).max() <= 1e-7
+    assert not np.array_equal(box, ucell)
+
+
+def test_rotate_ucell_matches_oracle_on_the_fit_rotations(ctx, golden, oracle):
+    """rmsd_rotate_ucell against orc_rotate_ucell applied to the same U
+    (the fit's own rotations): the reference's operation order, <= 1e-12."""
+    import torch
+    import cpptraj_amd as ca
+    g = golden("tz2_ortho")
+    X, nfit = g["frames"], int(g["nfit"])
+    F = X.shape[0]
+    Xd = torch.as_tensor(X).cuda()
+    sel = torch.arange(nfit, dtype=torch.int32, device="cuda")
+    _, U, _ = ca.rmsd_fit(Xd, mask=sel, modify="rottrans", save_matrices=True, ctx=ctx)
+    ucell = np.stack([ucell_from_box(b) for b in g["box"]])
+    u = torch.as_tensor(ucell).cuda()
+    ca.rotate_ucell(U, u, ctx=ctx)
+    Uh = U.cpu().numpy()
+    exp = np.stack([oracle.rotate_ucell(Uh[f], ucell[f]) for f in range(F)])
+    assert np.abs(u.cpu().numpy() - exp).max() <= 1e-12
 
 
 # Test_RMSD 'nomod' and 'previous'.

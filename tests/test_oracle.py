@@ -191,6 +191,38 @@ def test_cluster_cache_identity_list_is_rms2d_triangle(oracle):
         assert np.abs(c - t).max() < 1e-6
 
 
+def test_rotate_ucell_bitexact(oracle, reflib):
+    """orc_rotate_ucell against the compiled Box::SetupFromUcell + RotateUcell
+    (src/Box.cpp:571-586, :732-745) on orthorhombic, triclinic and truncated
+    octahedral cells and random rotations; the fractional cell the reference
+    recomputes is the inverse of the rotated cell."""
+    rng = np.random.default_rng(61)
+    cells = [[35.26, 41.85, 36.17, 90, 90, 90], [40.0, 40.0, 40.0, 109.4712206, 109.4712206,
+                                                 109.4712206], [30.0, 33.0, 37.0, 80, 95, 110]]
+    for box in cells:
+        ucell, _ = reflib.ucell_from_xyzabg(box)
+        for U in synth.rotations(rng, 8):
+            a, frac = reflib.rotate_ucell(U, ucell)
+            assert same(oracle.rotate_ucell(U, ucell), a)
+            assert np.abs(frac @ a.T - np.eye(3)).max() < 1e-12 or \
+                np.abs(a.T @ frac - np.eye(3)).max() < 1e-12
+
+
+def test_min_image_matches_reference(reflib):
+    """The tests' minimum-image distance against the compiled
+    DIST2_ImageNonOrtho in rotated orthorhombic and in triclinic cells."""
+    from conftest import min_image_distance
+    rng = np.random.default_rng(62)
+    for box in ([35.26, 41.85, 36.17, 90, 90, 90], [30.0, 33.0, 37.0, 80, 95, 110]):
+        ucell, _ = reflib.ucell_from_xyzabg(box)
+        for U in synth.rotations(rng, 6):
+            cell, frac = reflib.rotate_ucell(U, ucell)
+            for _ in range(20):
+                a, b = rng.uniform(-60, 60, 3), rng.uniform(-60, 60, 3)
+                assert abs(min_image_distance(a, b, cell) - reflib.dist_image(a, b, cell, frac)) \
+                    < 1e-9
+
+
 def test_fit_frames_bitexact(oracle, reflib):
     X = synth.noisy(6, 20, 60, 1.0).reshape(20, -1)
     rng = np.random.default_rng(15)
