@@ -357,25 +357,29 @@ int pairwise_impl(rmsd_ctx* c, const FrameSetDev& A, const FrameSetDev& B, int t
   const WaveLayout lay = rmsd_pairwise_layout(prec);
   const int nw = lay.NWR * lay.NWC;
   unsigned long long* stats = nullptr;
-  int* flags = nullptr;
+  unsigned int* nflag = nullptr;
+  int* wlist = nullptr;
   unsigned int* masks = nullptr;
   if (bounded) {
-    flags = (int*)scratch(c, "pair.flags", sizeof(int) * (size_t)ntiles * nw, &st);
-    if (!flags) return st;
-    // Lane masks are written (and read) only for flagged waves: no clearing.
+    // The list of flagged waves and their lane masks are written (and read)
+    // only for flagged waves: no clearing.
+    wlist = (int*)scratch(c, "pair.wlist", sizeof(int) * (size_t)ntiles * nw, &st);
+    if (!wlist) return st;
     masks = (unsigned int*)scratch(c, "pair.masks",
                                    sizeof(unsigned int) * (size_t)ntiles * nw * 64, &st);
     if (!masks) return st;
-    stats = (unsigned long long*)scratch(c, "pair.stats", 2 * sizeof(unsigned long long), &st);
+    // {largest kept bound, pairs recomputed, (listed waves, work counter)}
+    stats = (unsigned long long*)scratch(c, "pair.stats", 3 * sizeof(unsigned long long), &st);
     if (!stats) return st;
-    // Fill kernels, not memsets: the sequence stays valid under graph capture.
-    HIP_TRY(c, rmsd_launch_fill_int(flags, ntiles * nw, 0, c->stream));
-    HIP_TRY(c, rmsd_launch_fill_int((int*)stats, 4, 0, c->stream));
+    nflag = (unsigned int*)(stats + 2);
+    // A fill kernel, not a memset: the sequence stays valid under graph capture.
+    HIP_TRY(c, rmsd_launch_fill_int((int*)stats, 6, 0, c->stream));
     p.nrmA = pa.nrm;
     p.nrmB = pb.nrm;
     p.ke0 = rmsd_bound_coef(prec, p.KB).e0;
     p.keep = rmsd_keep_split(c->tol);  // < 0: report only
-    p.flags = flags;
+    p.wlist = wlist;
+    p.nflag = nflag;
     p.masks = masks;
     p.stats = stats;
   }
@@ -395,7 +399,7 @@ int pairwise_impl(rmsd_ctx* c, const FrameSetDev& A, const FrameSetDev& B, int t
     f.total_mass = M;
     f.tiles = tiles;
     f.ntiles = ntiles;
-    f.flags = flags;
+    f.wlist = wlist;
     f.masks = masks;
     f.FA = FA;
     f.FB = FB;
@@ -403,7 +407,7 @@ int pairwise_impl(rmsd_ctx* c, const FrameSetDev& A, const FrameSetDev& B, int t
     f.clamp = p.clamp;
     f.out = out;
     f.stats = stats;
-    HIP_TRY(c, rmsd_launch_pair_fallback(f, lay, c->stream));
+    HIP_TRY(c, rmsd_launch_pair_fallback(f, lay, nflag, c->stream));
   }
   mark(c, 3);
   return RMSD_OK;

@@ -31,13 +31,14 @@ struct rmsd_batch {
   // mean-structure templates (MFMA forms): sample prefix over the active sets
   int* dsbase = nullptr;
   int total_samples = 0, maxk = 0;
-  // error bound (MFMA forms, context tolerance != 0 at creation): one flag per
-  // wave and 64 lane masks per wave of either shape's tile list, {largest kept
-  // bound, pairs recomputed}, and the per-set arguments of the FP64
-  // recomputation
+  // error bound (MFMA forms, context tolerance != 0 at creation): room to list
+  // every wave of either shape's tile list and its 64 lane masks, {largest
+  // kept bound, pairs recomputed, (listed waves, work counter)}, and the
+  // per-set arguments of the FP64 recomputation
   bool bounded = false;
   double tol = 0.0;
-  int* tflags = nullptr;
+  int* twlist = nullptr;
+  unsigned int* tnflag = nullptr;
   unsigned int* tmasks = nullptr;
   unsigned long long* stats = nullptr;
   FallbackArgs* dfall = nullptr;
@@ -150,17 +151,13 @@ int batch_enqueue(rmsd_batch* b, int v, hipStream_t st) {
   ba.counters = b->counters;
   if (b->ntiles[sh] == 0) return RMSD_OK;
   const WaveLayout lay = rmsd_batch_layout(b->prec, v);
-  if (b->bounded) {
-    // Fill kernels (no memset nodes in the captured graph); lane masks are
-    // written and read only for flagged waves.
-    HIP_TRY(c, rmsd_launch_fill_int(b->tflags, b->ntiles[sh] * lay.NWR * lay.NWC, 0, st));
-    HIP_TRY(c, rmsd_launch_fill_int((int*)b->stats, 4, 0, st));
-  }
+  if (b->bounded)  // a fill kernel (no memset nodes in the captured graph)
+    HIP_TRY(c, rmsd_launch_fill_int((int*)b->stats, 6, 0, st));
   if (v & 1) HIP_TRY(c, rmsd_launch_batch_reset(ba, st));
   HIP_TRY(c, rmsd_launch_pairwise_batch(b->prec, v, ba, st));
   if (b->bounded && b->tol > 0.0)  // tol < 0: report only
     HIP_TRY(c, rmsd_launch_pair_fallback_batch(b->dfall, b->dtiles[sh], b->ntiles[sh], lay,
-                                               b->tflags, b->tmasks, st));
+                                               b->twlist, b->tnflag, b->tmasks, st));
   return RMSD_OK;
 }
 
@@ -311,7 +308,7 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
   std::vector<int4> th[2];
   std::vector<int> qh[2];
   size_t o_args = 0, o_tiles[2] = {0, 0}, o_q[2] = {0, 0}, o_cnt = 0;
-  size_t o_setup = 0, o_prep = 0, o_fbase = 0, o_sbase = 0, o_flags = 0, o_stats = 0;
+  size_t o_setup = 0, o_prep = 0, o_fbase = 0, o_sbase = 0, o_wlist = 0, o_stats = 0;
   size_t o_fall = 0, o_masks = 0;
   size_t nwaves = 0;
   int nactive = 0;
@@ -345,12 +342,12 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
         nwaves = std::max<size_t>(nwaves, (size_t)b->ntiles[sh] * L.NWR * L.NWC);
       }
       nwaves = std::max<size_t>(nwaves, 1);
-      o_flags = off;
+      o_wlist = off;
       off += al(sizeof(int) * nwaves);
       o_masks = off;
       off += al(sizeof(unsigned int) * 64 * nwaves);
       o_stats = off;
-      off += al(2 * sizeof(unsigned long long));
+      off += al(3 * sizeof(unsigned long long));
       o_fall = off;
       off += al(sizeof(FallbackArgs) * std::max(1, nsets));
     }
@@ -372,9 +369,10 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
   b->bounded = mixed && ctx->tol != 0.0;
   b->tol = ctx->tol;
   if (mixed) {
-    b->tflags = (int*)(base + o_flags);
+    b->twlist = (int*)(base + o_wlist);
     b->tmasks = (unsigned int*)(base + o_masks);
     b->stats = (unsigned long long*)(base + o_stats);
+    b->tnflag = (unsigned int*)(b->stats + 2);
     b->dfall = (FallbackArgs*)(base + o_fall);
   }
   std::vector<FallbackArgs> fall(nsets);
@@ -414,7 +412,8 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
       p.nrmA = p.nrmB = b->nrm[s];
       p.ke0 = rmsd_bound_coef(b->prec, p.KB).e0;
       p.keep = rmsd_keep_split(b->tol);
-      p.flags = b->tflags;
+      p.wlist = b->twlist;
+      p.nflag = b->tnflag;
       p.masks = b->tmasks;
       p.stats = b->stats;
       FallbackArgs& f = fall[s];

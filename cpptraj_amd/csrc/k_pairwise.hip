@@ -106,18 +106,20 @@ __device__ __forceinline__ const u32x4* frag_ptr(const u32x4* base, int64_t fb, 
 
 // ---- FP64 epilogue --------------------------------------------------------------
 // Reassembles each pair's 3x3 covariance (plus the FP64 template terms for the
-// MFMA-reduced forms), forms E0 and runs QCP pair by pair in registers.  The
-// per-frame terms are read from gA/uA (row frames, indexed i - rbase) and
-// gB/uB (column frames, j - cbase): global memory or an LDS copy.  (Grouping
-// several pairs' Newton iterations was measured slower on the 50k-frame
-// matrix.)
+// MFMA-reduced forms), forms E0 and runs QCP pair by pair in registers, with
+// the per-frame terms read from memory (gA / uA / nrmA for row frames, gB / uB
+// / nrmB for column frames).  (Grouping several pairs' Newton iterations, and
+// staging the tile's per-frame terms in LDS, were both measured slower on the
+// 50k-frame matrix.)
 template <int PREC, int WR, int WC>
 __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
                                               typename PrecTraits<PREC>::acc_t (&acc)[WR][WC][3][3],
-                                              int64_t fbA, int64_t fbB, int lane,
-                                              const double* gA, const double* uA, int64_t rbase,
-                                              const double* gB, const double* uB, int64_t cbase,
-                                              int tile_id, int wave, int nwaves) {
+                                              int64_t fbA, int64_t fbB, int lane, int tile_id,
+                                              int wave, int nwaves) {
+  const double* gA = args.gA;
+  const double* gB = args.gB;
+  const double* uA = args.uA;
+  const double* uB = args.uB;
   const double M = args.total_mass[0];
   // Error bound (MFMA-reduced forms with a flag array).  One eps bounds the
   // pairs of a lane's column j with its 8 rows (the largest row coefficient
@@ -128,7 +130,7 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
   // 32) are the same in every tile shape with WR = 2, so which pairs are
   // recomputed -- and every result -- does not depend on the shape.
   static_assert(PREC == PREC_F64 || WR == 2, "the bound's row groups assume WR = 2");
-  const bool bounded = PREC != PREC_F64 && args.flags != nullptr && M >= RMSD_SMALL;
+  const bool bounded = PREC != PREC_F64 && args.wlist != nullptr && M >= RMSD_SMALL;
   float klo[WC], khi[WC], eps[WC];
   if (bounded) {
     double csr = 0.0, gr = 0.0;
@@ -139,7 +141,7 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
         const int64_t i = (fbA + r) * 16 + 4 * (lane >> 4) + v;
         if (i < args.FA) {
           csr = fmax(csr, args.nrmA[4 * i]);
-          gr = fmax(gr, gA[i - rbase]);
+          gr = fmax(gr, gA[i]);
         }
       }
 #pragma unroll
@@ -148,7 +150,7 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
       double csc = 0.0, gc = 0.0;
       if (j < args.FB) {
         csc = args.nrmB[4 * j];
-        gc = gB[j - cbase];
+        gc = gB[j];
       }
       eps[q] = __double2float_ru(csr * csc + args.ke0 / M * 0.5 * (gr + gc));
       rmsd_dev::keep_limits(eps[q], args.keep, &klo[q], &khi[q]);
@@ -173,10 +175,10 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
     double gj = 0.0, uj[9];
     const bool jok = j < args.FB;
     if (jok) {
-      gj = gB[j - cbase];
+      gj = gB[j];
       if (PREC != PREC_F64) {
 #pragma unroll
-        for (int k = 0; k < 9; ++k) uj[k] = uB[9 * (j - cbase) + k];
+        for (int k = 0; k < 9; ++k) uj[k] = uB[9 * j + k];
       }
     }
 #pragma unroll
@@ -195,7 +197,7 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
 #pragma unroll
           for (int d = 0; d < 3; ++d) S[3 * c + d] = acc_elem<PREC>(acc[r][q][c][d], v);
         if (PREC != PREC_F64) {
-          const double* ui = uA + 9 * (i - rbase);
+          const double* ui = uA + 9 * i;
 #pragma unroll
           for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -215,7 +217,7 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
 #pragma unroll
           for (int k = 0; k < 9; ++k) R[k] = S[k];
         }
-        const double E0 = 0.5 * (gA[i - rbase] + gj);
+        const double E0 = 0.5 * (gA[i] + gj);
         const int64_t idx = args.tri ? rmsd_dev::tri_index(args.FA, i, j) : i * args.FB + j;
         const float res = M < RMSD_SMALL ? -1.0f : (float)rmsd_dev::qcp_rmsd(R, E0, M, args.clamp);
         args.out[idx] = res;
@@ -236,7 +238,7 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
     const int64_t w = (int64_t)tile_id * nwaves + wave;
     if (__ballot(bits != 0)) {
       args.masks[w * 64 + lane] = bits;
-      if (lane == 0) args.flags[w] = 1;
+      if (lane == 0) args.wlist[atomicAdd(args.nflag, 1u)] = (int)w;
     }
     // The largest stated bound of the kept pairs (<= tol by construction).
     // Report only (tol = infinity): also the pairs below sqrt(eps)+.
@@ -248,188 +250,153 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
       if (omax[q] >= 0.0f && args.keep.tol == INFINITY)
         b = fmaxf(b, rmsd_dev::bound_any(eps[q], omax[q]));
     }
-    for (int off = 32; off > 0; off >>= 1) b = fmaxf(b, __shfl_xor(b, off, 64));
-    if (lane == 0 && b > 0.0f)
-      atomicMax(args.stats, (unsigned long long)__double_as_longlong((double)b));
+    const unsigned int bw = rmsd_dev::wave_max_u32(__float_as_uint(b));  // b >= 0
+    if (lane == 0 && bw != 0u)
+      atomicMax(args.stats, (unsigned long long)__double_as_longlong((double)__uint_as_float(bw)));
   }
 }
 
 // ---- FP64 recomputation of flagged pairs -------------------------------------------
-// One workgroup per tile of the schedule; tiles without a flagged wave leave at
-// once.  The pairs come from the contraction's lane masks (so both sides
-// agree on them by construction) and are listed in LDS with the tile's frames
-// they use.  Those frames are then staged chunk by chunk in LDS as centred
-// (and sqrt(mass)-scaled) FP64 coordinates, from the frames as stored and the
-// per-frame centroids of k_prep, and each thread sums FALLBACK_PPT listed
-// pairs over the chunk; the pairs are solved with the same FP64 QCP.  (The
-// tile's frames do not stay in L2 -- an XCD's concurrent tiles hold far more --
-// so per-pair loads from memory were latency-bound: 18 ms for 2.3e6 pairs.)
+// The contraction lists every wave with a pair over the tolerance (wlist,
+// nflag[0] entries) and leaves that wave's lane masks.  Here one wave
+// recomputes one 16 x 16 block of frame pairs of a listed wave on
+// v_mfma_f64_16x16x4f64: operands straight from the frames as stored, centred
+// with the per-frame centroids of k_prep (the row side weighted), nine FP64
+// accumulator blocks over all atoms, then the FP64 QCP of the block's flagged
+// pairs.  Flagged pairs come in dense blocks (pairs of frames close to each
+// other: near the diagonal of a trajectory's matrix), so whole blocks are
+// computed; blocks without a flagged pair are skipped.  Waves take blocks from
+// a counter (nflag[1]), so a fixed grid drains the list whatever its length.
 constexpr int FALLBACK_THREADS = 256;
-constexpr int FALLBACK_MAX_PAIRS = 64 * 96;   // largest tile of any schedule
-constexpr int FALLBACK_MAX_FRAMES = 64 + 96;  // its row + column frames
-constexpr int FALLBACK_KCH = 14;              // atoms per staged chunk
-constexpr int FALLBACK_PPT = 4;               // pairs per thread per round
-// staging elements per thread per chunk (all loads issued before any store)
-constexpr int FALLBACK_STAGE = (FALLBACK_MAX_FRAMES * FALLBACK_KCH + FALLBACK_THREADS - 1) /
-                               FALLBACK_THREADS;
+constexpr int FALLBACK_GRID = 2048;
 
-__device__ __forceinline__ void fallback_tile(const FallbackArgs& a, int2 tile, int64_t tile_id,
-                                              WaveLayout L) {
-  __shared__ unsigned short list[FALLBACK_MAX_PAIRS];
-  __shared__ int nlist, nused;
-  __shared__ short cidx[FALLBACK_MAX_FRAMES];   // tile frame slot -> staged index (-1: unused)
-  __shared__ short cslot[FALLBACK_MAX_FRAMES];  // staged index -> tile frame slot
-  __shared__ double cen[FALLBACK_MAX_FRAMES][3];
-  __shared__ double stage[FALLBACK_MAX_FRAMES][3][FALLBACK_KCH + 1];
-  const int nw = L.NWR * L.NWC;
+// Four consecutive atoms k, ..., k + 3 of row frame fi and column frame fj
+// (zero beyond the selection or the frame count) and their weights.
+struct FallbackQuad {
+  float x[4][3], y[4][3];
+  double w[4];
+};
+
+__device__ __forceinline__ void fallback_fetch(const FallbackArgs& a, int64_t fi, bool iok,
+                                               int64_t fj, bool jok, int k, FallbackQuad& f) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    f.x[s][0] = f.x[s][1] = f.x[s][2] = f.y[s][0] = f.y[s][1] = f.y[s][2] = 0.0f;
+    f.w[s] = 0.0;
+    if (k + s < a.A.nsel) {
+      f.w[s] = a.smass ? a.smass[k + s] : 1.0;
+      if (iok) load_atom(a.A, fi, k + s, f.x[s]);
+      if (jok) load_atom(a.B, fj, k + s, f.y[s]);
+    }
+  }
+}
+
+__device__ __forceinline__ void fallback_block(const FallbackArgs& a, int2 tile, int wave,
+                                               int blk, WaveLayout L, int lane,
+                                               const unsigned int* wmask) {
+  const int r = blk % L.WR, q = blk / L.WR;
+  const int wr = wave / L.NWC, wc = wave % L.NWC;
   const int tile_r = 16 * L.WR * L.NWR, tile_c = 16 * L.WC * L.NWC;
-  const int nslots = tile_r + tile_c;
-  const double M = a.total_mass[0];
-  if (threadIdx.x == 0) nlist = 0;
-  for (int s = threadIdx.x; s < nslots; s += FALLBACK_THREADS) cidx[s] = -1;
-  __syncthreads();
-  // Decode the lane masks: bit (q WR + r) 4 + v of lane l in wave w is the
-  // pair at row 16 (wr WR + r) + 4 (l / 16) + v, column 16 (wc WC + q) + l % 16
-  // of the tile (wr = w / NWC, wc = w % NWC).
-  if (threadIdx.x < nw * 64) {
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (a.flags[tile_id * nw + w]) {
-      unsigned int m = a.masks[(tile_id * nw + w) * 64 + lane];
-      const int wr = w / L.NWC, wc = w % L.NWC;
-      while (m) {
-        const int b = __builtin_ctz(m);
-        m &= m - 1;
-        const int v = b & 3, rq = b >> 2, r = rq % L.WR, q = rq / L.WR;
-        const int il = 16 * (wr * L.WR + r) + 4 * (lane >> 4) + v;
-        const int jl = 16 * (wc * L.WC + q) + (lane & 15);
-        list[atomicAdd(&nlist, 1)] = (unsigned short)(il * tile_c + jl);
-        cidx[il] = 0;  // marks: slot used
-        cidx[tile_r + jl] = 0;
-      }
-    }
+  const int64_t i0 = (int64_t)tile.x * tile_r + 16 * (wr * L.WR + r);
+  const int64_t j0 = (int64_t)tile.y * tile_c + 16 * (wc * L.WC + q);
+  // This lane's pairs in the FP64 block: row (lane / 16) + 4 v, column
+  // lane % 16.  Their flags sit in the contraction's FP32 layout (row
+  // 4 (lane / 16) + v): lane 16 v + lane % 16, bit 4 blk + lane / 16.
+  unsigned int fl = 0;
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+    fl |= ((wmask[16 * v + (lane & 15)] >> (4 * blk + (lane >> 4))) & 1u) << v;
+  if (__ballot(fl != 0) == 0) return;
+  const int64_t fi = i0 + (lane & 15), fj = j0 + (lane & 15);
+  const bool iok = fi < a.FA, jok = fj < a.FB;
+  double ci[3] = {0.0, 0.0, 0.0}, cj[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (iok) ci[c] = a.cenA[4 * fi + c];
+    if (jok) cj[c] = a.cenB[4 * fj + c];
   }
-  __syncthreads();
-  const int n = nlist;
-  if (n == 0) return;
-  if (threadIdx.x == 0) {
-    int c = 0;
-    for (int s = 0; s < nslots; ++s)
-      if (cidx[s] == 0) {
-        cidx[s] = (short)c;
-        cslot[c++] = (short)s;
-      }
-    nused = c;
-  }
-  __syncthreads();
-  const int nu = nused;
-  for (int c = threadIdx.x; c < nu; c += FALLBACK_THREADS) {
-    const int s = cslot[c];
-    const double* src = s < tile_r ? a.cenA + 4 * ((int64_t)tile.x * tile_r + s)
-                                   : a.cenB + 4 * ((int64_t)tile.y * tile_c + (s - tile_r));
-    cen[c][0] = src[0];
-    cen[c][1] = src[1];
-    cen[c][2] = src[2];
-  }
+  f64x4 acc[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) acc[c][d] = f64x4{0.0, 0.0, 0.0, 0.0};
+  // Atoms in steps of 16: in sub-step s lane l feeds atom k0 + 4 (l / 16) + s
+  // of row frame i0 + l % 16 (A, 16 x 4) and of column frame j0 + l % 16
+  // (B, 4 x 16), so each lane reads four consecutive atoms per step.  The next
+  // step's atoms are fetched before this step's 36 MFMAs.
   const int nsel = a.A.nsel;
-  for (int base = 0; base < n; base += FALLBACK_THREADS * FALLBACK_PPT) {
-    int si[FALLBACK_PPT], sj[FALLBACK_PPT];
-    double S[FALLBACK_PPT][9];
+  const int kq = 4 * (lane >> 4);
+  FallbackQuad nxt;
+  fallback_fetch(a, fi, iok, fj, jok, kq, nxt);
+  for (int k0 = 0; k0 < nsel; k0 += 16) {
+    const FallbackQuad cur = nxt;
+    fallback_fetch(a, fi, iok, fj, jok, k0 + 16 + kq, nxt);
 #pragma unroll
-    for (int u = 0; u < FALLBACK_PPT; ++u) {
-      const int q = base + threadIdx.x + FALLBACK_THREADS * u;
-      si[u] = sj[u] = -1;
-      if (q < n) {
-        const int p = list[q];
-        si[u] = cidx[p / tile_c];
-        sj[u] = cidx[tile_r + p % tile_c];
+    for (int s = 0; s < 4; ++s) {
+      double av[3], bv[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        av[c] = ((double)cur.x[s][c] - ci[c]) * cur.w[s];
+        bv[c] = (double)cur.y[s][c] - cj[c];
       }
-#pragma unroll
-      for (int e = 0; e < 9; ++e) S[u][e] = 0.0;
-    }
-    for (int k0 = 0; k0 < nsel; k0 += FALLBACK_KCH) {
-      __syncthreads();  // the previous chunk's reads (and cen) are done
-      float px[FALLBACK_STAGE][3];
-#pragma unroll
-      for (int t = 0; t < FALLBACK_STAGE; ++t) {
-        const int e = threadIdx.x + FALLBACK_THREADS * t;
-        const int c = e / FALLBACK_KCH, k = k0 + e % FALLBACK_KCH;
-        px[t][0] = px[t][1] = px[t][2] = 0.0f;
-        if (c < nu && k < nsel) {
-          const int s = cslot[c];
-          if (s < tile_r) load_atom(a.A, (int64_t)tile.x * tile_r + s, k, px[t]);
-          else load_atom(a.B, (int64_t)tile.y * tile_c + (s - tile_r), k, px[t]);
-        }
-      }
-#pragma unroll
-      for (int t = 0; t < FALLBACK_STAGE; ++t) {
-        const int e = threadIdx.x + FALLBACK_THREADS * t;
-        const int c = e / FALLBACK_KCH, kk = e % FALLBACK_KCH, k = k0 + kk;
-        if (c < nu) {
-          const double sw = (k < nsel && a.smass) ? sqrt(a.smass[k]) : 1.0;
-#pragma unroll
-          for (int d = 0; d < 3; ++d)
-            stage[c][d][kk] = k < nsel ? ((double)px[t][d] - cen[c][d]) * sw : 0.0;
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int u = 0; u < FALLBACK_PPT; ++u) {
-        if (si[u] < 0) continue;
-#pragma unroll
-        for (int kk = 0; kk < FALLBACK_KCH; ++kk) {
-          const double x0 = stage[si[u]][0][kk], x1 = stage[si[u]][1][kk], x2 = stage[si[u]][2][kk];
-          const double y0 = stage[sj[u]][0][kk], y1 = stage[sj[u]][1][kk], y2 = stage[sj[u]][2][kk];
-          S[u][0] += x0 * y0; S[u][1] += x0 * y1; S[u][2] += x0 * y2;
-          S[u][3] += x1 * y0; S[u][4] += x1 * y1; S[u][5] += x1 * y2;
-          S[u][6] += x2 * y0; S[u][7] += x2 * y1; S[u][8] += x2 * y2;
-        }
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < FALLBACK_PPT; ++u) {
-      const int q = base + threadIdx.x + FALLBACK_THREADS * u;
-      if (q >= n) continue;
-      const int p = list[q];
-      const int64_t i = (int64_t)tile.x * tile_r + p / tile_c;
-      const int64_t j = (int64_t)tile.y * tile_c + p % tile_c;
-      double R[9];
-      // As in the epilogue: rows are the reference in the triangle layout.
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) R[3 * c + d] = a.tri ? S[u][3 * d + c] : S[u][3 * c + d];
-      const double E0 = 0.5 * (a.gA[i] + a.gB[j]);
-      const int64_t idx = a.tri ? rmsd_dev::tri_index(a.FA, i, j) : i * a.FB + j;
-      a.out[idx] = (float)rmsd_dev::qcp_rmsd(R, E0, M, a.clamp);
+        for (int d = 0; d < 3; ++d)
+          acc[c][d] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[c], bv[d], acc[c][d], 0, 0, 0);
     }
   }
-  if (threadIdx.x == 0) atomicAdd(a.stats + 1, (unsigned long long)n);
+  const double M = a.total_mass[0];
+  unsigned long long nrec = 0;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    nrec += __popcll(__ballot((fl >> v) & 1u));
+    if (!((fl >> v) & 1u)) continue;
+    const int64_t i = i0 + (lane >> 4) + 4 * v, j = fj;
+    double R[9];
+    // As in the epilogue: rows are the reference in the triangle layout.
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) R[3 * c + d] = a.tri ? acc[d][c][v] : acc[c][d][v];
+    const double E0 = 0.5 * (a.gA[i] + a.gB[j]);
+    const int64_t idx = a.tri ? rmsd_dev::tri_index(a.FA, i, j) : i * a.FB + j;
+    a.out[idx] = (float)rmsd_dev::qcp_rmsd(R, E0, M, a.clamp);
+  }
+  if (lane == 0) atomicAdd(a.stats + 1, nrec);
 }
 
-// A tile with no flagged wave leaves before touching LDS.
-__device__ __forceinline__ bool tile_flagged(const int* flags, int64_t t, int nw) {
-  int any = 0;
-  for (int w = 0; w < nw; ++w) any |= flags[t * nw + w];
-  return any != 0;
+// Next 16 x 16 block for this wave (the same for every lane).
+__device__ __forceinline__ unsigned int fallback_next(unsigned int* nflag, int lane) {
+  unsigned int b = 0;
+  if (lane == 0) b = atomicAdd(nflag + 1, 1u);
+  return __shfl(b, 0, 64);
 }
 
-__global__ __launch_bounds__(FALLBACK_THREADS) void k_pair_fallback(FallbackArgs a, WaveLayout L) {
-  const int t = blockIdx.x;
-  if (t >= a.ntiles || !tile_flagged(a.flags, t, L.NWR * L.NWC)) return;
-  fallback_tile(a, a.tiles[t], t, L);
+__global__ __launch_bounds__(FALLBACK_THREADS, 2) void k_pair_fallback(FallbackArgs a, WaveLayout L,
+                                                                    unsigned int* nflag) {
+  const int lane = threadIdx.x & 63;
+  const int nw = L.NWR * L.NWC, nb = L.WR * L.WC;
+  const unsigned int total = nflag[0] * (unsigned int)nb;
+  for (unsigned int b = fallback_next(nflag, lane); b < total; b = fallback_next(nflag, lane)) {
+    const int w = a.wlist[b / nb];
+    fallback_block(a, a.tiles[w / nw], w % nw, (int)(b % nb), L, lane,
+                   a.masks + (int64_t)w * 64);
+  }
 }
 
-__global__ __launch_bounds__(FALLBACK_THREADS) void k_pair_fallback_batch(const FallbackArgs* fsets,
-                                                             const int4* tiles, int ntiles,
-                                                             WaveLayout L, const int* flags,
-                                                             const unsigned int* masks) {
-  const int t = blockIdx.x;
-  if (t >= ntiles || !tile_flagged(flags, t, L.NWR * L.NWC)) return;
-  const int4 tl = tiles[t];
-  FallbackArgs a = fsets[tl.x];
-  a.flags = flags;
-  a.masks = masks;
-  fallback_tile(a, make_int2(tl.y, tl.z), t, L);
+__global__ __launch_bounds__(FALLBACK_THREADS, 2) void k_pair_fallback_batch(
+    const FallbackArgs* fsets, const int4* tiles, WaveLayout L, const int* wlist,
+    unsigned int* nflag, const unsigned int* masks) {
+  const int lane = threadIdx.x & 63;
+  const int nw = L.NWR * L.NWC, nb = L.WR * L.WC;
+  const unsigned int total = nflag[0] * (unsigned int)nb;
+  for (unsigned int b = fallback_next(nflag, lane); b < total; b = fallback_next(nflag, lane)) {
+    const int w = wlist[b / nb];
+    const int4 tl = tiles[w / nw];
+    fallback_block(fsets[tl.x], make_int2(tl.y, tl.z), w % nw, (int)(b % nb), L, lane,
+                   masks + (int64_t)w * 64);
+  }
 }
 
 // One 64 x 64 tile of frame pairs: contraction over all atom chunks, then
@@ -505,8 +472,7 @@ __device__ __forceinline__ void pair_tile(const PairArgs& args, int2 tile, int t
           for (int d = 0; d < 3; ++d) mma<PREC>(acc[r][q][c][d], ca[r][c], cb[q][d]);
   }
 
-  pair_epilogue<PREC, WR, WC>(args, acc, fbA, fbB, lane, args.gA, args.uA, 0, args.gB, args.uB, 0,
-                              tile_id, wave, NWR * NWC);
+  pair_epilogue<PREC, WR, WC>(args, acc, fbA, fbB, lane, tile_id, wave, NWR * NWC);
 }
 
 template <int PREC, int WR, int WC, int NWR, int NWC>
@@ -636,26 +602,26 @@ static hipError_t launch_batch_shape(const BatchArgs& b, bool persist, hipStream
   return hipGetLastError();
 }
 
-static bool fallback_fits(WaveLayout L) {
-  const int tr = 16 * L.WR * L.NWR, tc = 16 * L.WC * L.NWC;
-  return tr * tc <= FALLBACK_MAX_PAIRS && tr + tc <= FALLBACK_MAX_FRAMES &&
-         L.NWR * L.NWC * 64 <= FALLBACK_THREADS && L.WR * L.WC * 4 <= 32;
-}
+// A wave's pairs must fit the 32 bits of a lane mask.
+static bool fallback_fits(WaveLayout L) { return L.WR * L.WC * 4 <= 32; }
 
-hipError_t rmsd_launch_pair_fallback(const FallbackArgs& a, WaveLayout L, hipStream_t s) {
+hipError_t rmsd_launch_pair_fallback(const FallbackArgs& a, WaveLayout L, unsigned int* nflag,
+                                     hipStream_t s) {
   if (a.ntiles <= 0) return hipSuccess;
   if (!fallback_fits(L)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_pair_fallback, dim3(a.ntiles), dim3(FALLBACK_THREADS), 0, s, a, L);
+  hipLaunchKernelGGL(k_pair_fallback, dim3(FALLBACK_GRID), dim3(FALLBACK_THREADS), 0, s, a, L,
+                     nflag);
   return hipGetLastError();
 }
 
 hipError_t rmsd_launch_pair_fallback_batch(const FallbackArgs* fsets, const int4* tiles,
-                                           int ntiles, WaveLayout L, const int* flags,
-                                           const unsigned int* masks, hipStream_t s) {
+                                           int ntiles, WaveLayout L, const int* wlist,
+                                           unsigned int* nflag, const unsigned int* masks,
+                                           hipStream_t s) {
   if (ntiles <= 0) return hipSuccess;
   if (!fallback_fits(L)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_pair_fallback_batch, dim3(ntiles), dim3(FALLBACK_THREADS), 0, s, fsets,
-                     tiles, ntiles, L, flags, masks);
+  hipLaunchKernelGGL(k_pair_fallback_batch, dim3(FALLBACK_GRID), dim3(FALLBACK_THREADS), 0, s,
+                     fsets, tiles, L, wlist, nflag, masks);
   return hipGetLastError();
 }
 

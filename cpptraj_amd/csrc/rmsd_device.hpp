@@ -176,6 +176,19 @@ __device__ __forceinline__ float bound_any(float e, float rmax) {
   return (b + (rmax + b) * 0x1p-23f) * (1.0f + 0x1p-20f);
 }
 
+// Largest value over the 64 lanes of a wave (the same in every lane): DPP
+// steps within quads, half rows and rows, then row broadcasts collect the
+// four rows in lane 63.  For non-negative floats pass their bits (same order).
+__device__ __forceinline__ unsigned int wave_max_u32(unsigned int v) {
+  v = max(v, (unsigned int)__builtin_amdgcn_update_dpp(0, (int)v, 0xb1, 0xf, 0xf, false));
+  v = max(v, (unsigned int)__builtin_amdgcn_update_dpp(0, (int)v, 0x4e, 0xf, 0xf, false));
+  v = max(v, (unsigned int)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xf, 0xf, false));
+  v = max(v, (unsigned int)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xf, 0xf, false));
+  v = max(v, (unsigned int)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));
+  v = max(v, (unsigned int)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));
+  return (unsigned int)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 // Optimal rotation for covariance S at eigenvalue lam, written as the Kabsch
 // U (x' = U x for the centred target).  Returns 0 when every adjugate column
 // is degenerate and the identity is returned (QCP fallback), 1 otherwise.

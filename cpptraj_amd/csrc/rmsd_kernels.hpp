@@ -120,19 +120,21 @@ struct PairArgs {
   int tri;              // packed upper triangle (A == B); else full FA x FB
   int clamp;            // 1: Kabsch-style clamp at 0; 0: QCP-style fabs
   float* out;
-  // Error bound of the MFMA-reduced forms (off when flags is null): per-frame
+  // Error bound of the MFMA-reduced forms (off when wlist is null): per-frame
   // norms (PrepArgs::nrm, first entry cs), the FP64 term's coefficient, the
-  // tolerance.  Wave w of tile t with a pair whose bound exceeds tol sets
-  // flags[t * waves + w] = 1 and stores each lane's pairs over it as a bit
-  // mask, masks[(t * waves + w) * 64 + lane] (bit (q WR + r) 4 + v: its
-  // accumulator block (r, q), item v); rmsd_launch_pair_fallback recomputes
-  // those pairs in FP64.  stats[0] collects the largest bound of the pairs
-  // that stay (as the bits of a non-negative double).
+  // tolerance split.  Wave w of tile t (w = t * waves + wave) with a pair whose
+  // bound exceeds tol stores each lane's pairs over it as a bit mask,
+  // masks[w * 64 + lane] (bit (q WR + r) 4 + v: its accumulator block (r, q),
+  // item v), and appends w to wlist (nflag[0] entries, nflag[0] zeroed before
+  // the launch); rmsd_launch_pair_fallback recomputes those pairs in FP64.
+  // stats[0] collects the largest bound of the pairs that stay (as the bits of
+  // a non-negative double).
   const double* nrmA = nullptr;
   const double* nrmB = nullptr;
   double ke0 = 0.0;
   rmsd_dev::KeepSplit keep{};
-  int* flags = nullptr;
+  int* wlist = nullptr;
+  unsigned int* nflag = nullptr;
   unsigned int* masks = nullptr;
   unsigned long long* stats = nullptr;
 };
@@ -144,9 +146,9 @@ struct WaveLayout {
 };
 
 // FP64 recomputation of the pairs whose bound exceeds the tolerance
-// (k_pairwise.hip, k_pair_fallback): one workgroup per tile of the schedule,
-// the pairs read from the contraction's wave flags and lane masks
-// (PairArgs), summed over the atoms in FP64 from LDS-staged frames.
+// (k_pairwise.hip, k_pair_fallback): one wave per 16 x 16 block of a wave the
+// contraction listed (PairArgs::wlist, masks), FP64 MFMA over the atoms of
+// the frames as stored.
 struct FallbackArgs {
   FrameSetDev A, B;
   const double* smass;   // covariance weights (null: unit)
@@ -157,7 +159,7 @@ struct FallbackArgs {
   const double* total_mass;
   const int2* tiles;
   int ntiles;
-  const int* flags;
+  const int* wlist;  // the contraction's flagged waves and their lane masks
   const unsigned int* masks;
   int FA, FB;
   int tri;
@@ -311,10 +313,13 @@ hipError_t rmsd_launch_prep_batch(int prec, const PrepArgs* sets, const int* fba
                                   int total_frames, hipStream_t s);
 hipError_t rmsd_launch_pairwise(int prec, const PairArgs& a, hipStream_t s);
 // Wave layout of rmsd_launch_pairwise / rmsd_launch_pairwise_batch variants
-// (flags and masks hold NWR * NWC waves per tile).
+// (wlist and masks hold NWR * NWC waves per tile).
 WaveLayout rmsd_pairwise_layout(int prec);
 WaveLayout rmsd_batch_layout(int prec, int variant);
-hipError_t rmsd_launch_pair_fallback(const FallbackArgs& a, WaveLayout w, hipStream_t s);
+// nflag: the contraction's count of listed waves, then a work counter (zeroed
+// with it).
+hipError_t rmsd_launch_pair_fallback(const FallbackArgs& a, WaveLayout w, unsigned int* nflag,
+                                     hipStream_t s);
 hipError_t rmsd_launch_fill_int(int* p, int n, int v, hipStream_t s);
 // Mean-structure template (k_prep.hip): the first template frame's T (from
 // rmsd_launch_setup) is replaced by the mean of `nsamp` evenly spaced frames
@@ -326,10 +331,11 @@ hipError_t rmsd_launch_template_mean(const SetupArgs& a, int nsamp, double* part
 hipError_t rmsd_launch_template_mean_batch(const SetupArgs* sets, const int* sbase, int nsets,
                                            int total_samples, int maxk, hipStream_t s);
 // FP64 recomputation over a batch's tile list: tile t = (set, row tile,
-// column tile, t), fsets one per set (their flags / masks are the plan's).
+// column tile, t), fsets one per set (wlist / nflag / masks are the plan's).
 hipError_t rmsd_launch_pair_fallback_batch(const FallbackArgs* fsets, const int4* tiles,
-                                           int ntiles, WaveLayout w, const int* flags,
-                                           const unsigned int* masks, hipStream_t s);
+                                           int ntiles, WaveLayout w, const int* wlist,
+                                           unsigned int* nflag, const unsigned int* masks,
+                                           hipStream_t s);
 // Column frames per pair tile of rmsd_launch_pairwise (rows: 64).
 int rmsd_pairwise_tile_cols(int prec);
 int rmsd_autocorr_chunks(int n, int* rows);
