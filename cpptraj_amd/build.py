@@ -27,7 +27,7 @@ HOSTDIR = os.path.join(HERE, "host")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["capi.hip", "capi_grad.hip", "capi_batch.hip", "k_prep.hip",
+SOURCES = ["capi.hip", "capi_grad.hip", "capi_batch.hip", "capi_noalloc.hip", "k_prep.hip",
            "k_pairwise.hip", "k_nofit.hip", "k_fit.hip", "k_grad.hip", "k_matrix.hip", "k_perres.hip", "k_avgcorr.hip",
            "k_symm.hip"]
 HOST_SOURCES = ["host_core.cpp", "host_rms.cpp", "host_capi.cpp"]

@@ -426,6 +426,12 @@ int rmsd_ctx_create(rmsd_ctx** ctx, int device, void* stream) {
   rmsd_ctx* c = new rmsd_ctx();
   c->device = device;
   c->stream = (hipStream_t)stream;
+  std::string err;
+  if (noalloc_init_device(device, &c->noalloc, &err) != RMSD_OK) {
+    fprintf(stderr, "rmsd_ctx_create: %s\n", err.c_str());
+    delete c;
+    return RMSD_ERR_DEVICE;
+  }
   *ctx = c;
   return RMSD_OK;
 }

@@ -69,6 +69,9 @@ struct rmsd_ctx {
   // 0 = off, < 0 = report only) and what the last call reported.
   double tol = rmsd_capi::DEFAULT_TOLERANCE;
   rmsd_capi::ErrorReport report;
+  // This device's allocation-free scratch (capi_noalloc.hip), looked up when
+  // the context is created.
+  void* noalloc = nullptr;
 };
 
 namespace rmsd_capi {
@@ -97,5 +100,8 @@ BoundCoef rmsd_bound_coef(int prec, int KB);
 rmsd_dev::KeepSplit rmsd_keep_split(double tol);
 // Frees a batch plan (capi_batch.hip); rmsd_ctx_destroy frees leftovers.
 void batch_free(rmsd_batch* b);
+// Address of the allocation-free scratch on `device` (current), looked up once
+// per device (capi_noalloc.hip).
+int noalloc_init_device(int device, void** ptr, std::string* err);
 
 }  // namespace rmsd_capi

@@ -176,6 +176,21 @@ int rmsd_fit_to_ref(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames*
                     int ref_frame, int fit, int use_mass, int modify, double* rmsd, double* U,
                     double* tvec);
 
+/* Allocation-free variant of rmsd_fit_to_ref: same arguments, same kernels,
+ * same results bit for bit.  No counterpart in the reference; it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * per-frame loops such as Action::DoAction in a serving path, harnesses that
+ * forbid hipMalloc).  Its working memory is a fixed per-GPU device array of
+ * RMSD_NOALLOC_SCRATCH_BYTES that the library holds until the process ends;
+ * one call needs about 32 nsel + 512 bytes plus 128 bytes per frame (248 with
+ * a coordinate update): larger calls are rejected with RMSD_ERR_ARG (split the
+ * frames; 64 MiB hold about 500,000 frames, 270,000 with the update).  All calls on one GPU share the array,
+ * so calls that overlap on one GPU (other streams or threads) race. */
+#define RMSD_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int rmsd_fit_to_ref_noalloc(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* ref,
+                            int ref_frame, int fit, int use_mass, int modify, double* rmsd,
+                            double* U, double* tvec);
+
 /* Unit-cell rotation of the fit's coordinate update.
  * Replaces Box::RotateUcell (src/Box.cpp:732-745) as Action_Rmsd::DoAction
  * applies it after Trans_Rot_Trans in the default rotate + translate mode
