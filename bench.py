@@ -113,17 +113,19 @@ def baseline_threads():
     return n
 
 
-def profiled_traffic(kernel, workload):
-    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3
-    counter summaries (profiles/*.md lines '<!-- traffic <kernel> <workload>
-    <bytes> -->', FETCH_SIZE + WRITE_SIZE scaled by a copy kernel), or None."""
+def profiled_value(tag, kernel, workload):
+    """A figure of `kernel` on `workload` from the committed rocprofv3 counter
+    summaries (profiles/*.md lines '<!-- <tag> <kernel> <workload> <value> -->'),
+    or None.  Tags: 'traffic' (HBM bytes per launch: memory reads by request
+    size, plus WRITE_SIZE), 'mfma_busy' (SQ_VALU_MFMA_BUSY_CYCLES over
+    4 SIMDs x CUs x GRBM_GUI_ACTIVE)."""
     import glob
     import re
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         with open(path) as fh:
-            for m in re.finditer(r"<!-- traffic (\S+) (\S+) (\d+) -->", fh.read()):
-                if m.group(1) == kernel and m.group(2) == workload:
-                    return int(m.group(3))
+            for m in re.finditer(r"<!-- (\S+) (\S+) (\S+) ([0-9.eE+-]+) -->", fh.read()):
+                if m.group(1) == tag and m.group(2) == kernel and m.group(3) == workload:
+                    return float(m.group(4))
     return None
 
 
@@ -685,7 +687,8 @@ def main():
         "kernels": res,
         "roofline": {"bound": "mfma", "achieved": algo_tflops, "peak": peak,
                      "unit": "TFLOP/s", "frac": algo_tflops / peak,
-                     "traffic": profiled_traffic(f"k_pairwise<{head}>", f"{F}x{n}"),
+                     "traffic": profiled_value("traffic", f"k_pairwise<{head}>", f"{F}x{n}"),
+                     "mfma_busy": profiled_value("mfma_busy", f"k_pairwise<{head}>", f"{F}x{n}"),
                      "kernel": f"k_pairwise<{head}>",
                      "algorithmic_flops_per_launch": npairs * flops_pair,
                      "executed_mfma_tflops": algo_tflops * terms},
