@@ -21,6 +21,7 @@ HBM bandwidth.
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -213,6 +214,7 @@ def fit_stream(ctx, F, n, steps, warmup):
     out["perres"] = perres_bench(ctx, X, d, m, steps, warmup)
     del X
     torch.cuda.empty_cache()
+    out["reftraj"] = reftraj_bench(ctx, 100000, n, steps, warmup)
     return out
 
 
@@ -251,6 +253,69 @@ def fit_cpu_baseline(X, ref_centered, ref_trans, target_s=6.0):
             "openmp_over_frames": {"value": k / dt, "unit": "frames/s", "cores": nt,
                                    "omp_proc_bind": os.environ.get("OMP_PROC_BIND")},
             "model": info["model"]}
+
+
+def reftraj_bench(ctx, F, n, steps, warmup):
+    """'rmsd ... reftraj': frame f of F frames x n atoms (FP32 SoA, device
+    resident) fitted to frame f of a second set, unweighted, nomod, one
+    rmsd_fit_pairs call.  Algorithmic bytes: both frames read once (24 n per
+    pair) + 8 B RMSD written.  CPU baseline: the compiled reference's per-pair
+    body (centre the reference frame, RMSD_CenteredRef) on evenly spaced pairs,
+    serial as the action runs."""
+    import torch
+    import cpptraj_amd as ca
+    from cpptraj_amd import synth
+
+    X, _ = synth.noisy_soa_torch(7, F, n, 1.0, ld=n)
+    Y, _ = synth.noisy_soa_torch(8, F, n, 1.0, ld=n)
+    dx = ca.frames_desc(X, soa=True)
+    dy = ca.frames_desc(Y, soa=True)
+    rmsd = torch.empty(F, dtype=torch.float64, device="cuda")
+    L = ctx._L
+    fn = lambda: ctx.check(L.rmsd_fit_pairs(ctx.handle, C.byref(dx), C.byref(dy), 1, 0, 0,
+                                            ca._ptr(rmsd), None, None))
+    ms = timed_steps(fn, steps, warmup)
+    _, main = profiled_main_ms(ctx, fn, 3)
+    algo = F * (24.0 * n + 8.0)
+    out = {"workload": f"{F} frames x {n} atoms FP32 SoA, frame f against frame f of a "
+                       "second set, fit, unweighted, nomod",
+           "frames_per_s": F / (ms / 1e3), "ms_per_step": ms, "kernel_ms": main,
+           "hbm_gbs": algo / (main / 1e3) / 1e9}
+    # parity on 64 pairs: the FP64 oracle on the host
+    from oracle.oracle import Oracle, RefLib, ref_available
+    o = Oracle()
+    idx = np.linspace(0, F - 1, 64).astype(np.int64)
+    aos = lambda T, ids: np.transpose(T[ids].cpu().numpy(), (0, 2, 1)).reshape(len(ids), -1)
+    Xs, Ys = aos(X, idx), aos(Y, idx)
+    fn()
+    torch.cuda.synchronize()
+    got = rmsd[idx].cpu().numpy()
+    exp = []
+    for k in range(len(idx)):
+        rc, rt = o.center_on_origin(Ys[k].reshape(n, 3).astype(np.float64))
+        r, _, _, _ = o.fit_frames(Xs[k:k + 1], rc, rt)
+        exp.append(r[0])
+    out["max_err_vs_oracle_64_pairs"] = float(np.abs(got - np.array(exp)).max())
+    if ref_available():
+        lib = RefLib()
+        k = 256
+        ids = np.linspace(0, F - 1, k).astype(np.int64)
+        t0 = time.perf_counter()
+        lib.fit_pairs(aos(X, ids), aos(Y, ids), nthreads=1)
+        rate = k / (time.perf_counter() - t0)
+        k = int(min(F, max(256, 5.0 * rate)))
+        ids = np.linspace(0, F - 1, k).astype(np.int64)
+        Xs, Ys = aos(X, ids), aos(Y, ids)
+        t0 = time.perf_counter()
+        lib.fit_pairs(Xs, Ys, nthreads=1)
+        out["cpu_baseline"] = {"value": k / (time.perf_counter() - t0), "unit": "frames/s",
+                               "cores": 1, "kind": "reference",
+                               "sample": f"{k} evenly spaced pairs, serial per frame as "
+                                         "Action_Rmsd::DoAction runs in 'reftraj' mode",
+                               "model": host_cpu_info()["model"]}
+    del X, Y
+    torch.cuda.empty_cache()
+    return out
 
 
 def perres_bench(ctx, X, d, m, steps, warmup, res_atoms=15):

@@ -100,6 +100,8 @@ def load_library():
     L.rmsd_cluster_cache.restype = C.c_int
     L.rmsd_fit_to_ref.argtypes = [P, FP, FP, C.c_int, C.c_int, C.c_int, C.c_int, P, P, P]
     L.rmsd_fit_to_ref.restype = C.c_int
+    L.rmsd_fit_pairs.argtypes = [P, FP, FP, C.c_int, C.c_int, C.c_int, P, P, P]
+    L.rmsd_fit_pairs.restype = C.c_int
     _optional(L)
     _lib = L
     return L
@@ -400,6 +402,29 @@ def rmsd_fit(frames, ref_frames=None, ref_index=0, mask=None, mass=None, ref_mas
     U = torch.empty((F, 3, 3), dtype=torch.float64, device=dev) if save_matrices else None
     tv = torch.empty((F, 6), dtype=torch.float64, device=dev) if save_vectors else None
     ctx.fit_desc(d, r, ref_index, fit, use_mass, MODIFY[modify], rmsd, U, tv)
+    return rmsd, U, tv
+
+
+def rmsd_fit_pairs(frames, ref_frames, mask=None, mass=None, ref_mask=None, ref_mass=None,
+                   fit=True, use_mass=False, modify="nomod", save_matrices=False,
+                   save_vectors=False, soa=False, natom=None, ctx: Context | None = None):
+    """Frame f of `frames` fitted to frame f of `ref_frames` (rmsd_fit_pairs:
+    the 'reftraj' reference mode; 'previous' on frames as stored is
+    ref_frames = frames shifted by one).  Same outputs as rmsd_fit."""
+    import torch
+
+    ctx = ctx or _ctx(frames.device.index or 0)
+    d = frames_desc(frames, mask, mass, soa=soa, natom=natom)
+    r = frames_desc(ref_frames, ref_mask if ref_mask is not None else mask,
+                    ref_mass if ref_mass is not None else mass, soa=soa, natom=natom)
+    F = d.nframes
+    dev = frames.device
+    rmsd = torch.empty(F, dtype=torch.float64, device=dev)
+    U = torch.empty((F, 3, 3), dtype=torch.float64, device=dev) if save_matrices else None
+    tv = torch.empty((F, 6), dtype=torch.float64, device=dev) if save_vectors else None
+    ctx.check(ctx._L.rmsd_fit_pairs(ctx.handle, C.byref(d), C.byref(r), int(bool(fit)),
+                                    int(bool(use_mass)), MODIFY[modify], _ptr(rmsd), _ptr(U),
+                                    _ptr(tv)))
     return rmsd, U, tv
 
 

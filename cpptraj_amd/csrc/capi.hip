@@ -661,6 +661,71 @@ int rmsd_fit_to_ref(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames*
   return RMSD_OK;
 }
 
+int rmsd_fit_pairs(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* refs, int fit,
+                   int use_mass, int modify, double* rmsd, double* U, double* tvec) {
+  if (!ctx) return RMSD_ERR_ARG;
+  int st;
+  if ((st = check_frames(ctx, frames, "frames")) != RMSD_OK) return st;
+  if ((st = check_frames(ctx, refs, "refs")) != RMSD_OK) return st;
+  if (refs->nsel != frames->nsel)
+    return fail(ctx, RMSD_ERR_ARG, "reference and target select different atom counts");
+  if (refs->nframes < frames->nframes)
+    return fail(ctx, RMSD_ERR_ARG, "fewer reference frames than target frames");
+  if (frames->nsel < 1 && frames->nframes > 0)
+    return fail(ctx, RMSD_ERR_ARG, "no atoms selected");
+  if (use_mass && (!frames->mass || !refs->mass))
+    return fail(ctx, RMSD_ERR_ARG, "use_mass needs target and reference masses");
+  if (modify < RMSD_MODIFY_NONE || modify > RMSD_MODIFY_TRANS)
+    return fail(ctx, RMSD_ERR_ARG, "unknown modify mode");
+  if (modify != RMSD_MODIFY_NONE && (frames->fidx || !fit))
+    return fail(ctx, RMSD_ERR_ARG,
+                "coordinate update needs a fit and frames in storage order (fidx NULL)");
+  if (!rmsd && frames->nframes > 0) return fail(ctx, RMSD_ERR_ARG, "rmsd is NULL");
+  if (frames->nframes == 0) return RMSD_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const int64_t nf = frames->nframes;
+  double* W = (double*)scratch(ctx, "pairfit.W", sizeof(double), &st);
+  if (!W) return st;
+  double* xform = nullptr;
+  if (modify != RMSD_MODIFY_NONE) {
+    xform = (double*)scratch(ctx, "pairfit.xform", sizeof(double) * 15 * nf, &st);
+    if (!xform) return st;
+  }
+  ctx->marks = 0;
+  mark(ctx, 0);
+  HIP_TRY(ctx, rmsd_launch_mass_total(use_mass ? frames->mass : nullptr, frames->nsel, W,
+                                      ctx->stream));
+  mark(ctx, 1);
+  PairFitArgs a;
+  a.T = to_dev(*frames);
+  a.R = to_dev(*refs);
+  a.tmass = use_mass ? frames->mass : nullptr;
+  a.rmass = use_mass ? refs->mass : nullptr;
+  a.W = W;
+  a.fit = fit ? 1 : 0;
+  a.clamp = 1;
+  a.rmsd = rmsd;
+  a.U = fit ? U : nullptr;
+  a.tvec = fit ? tvec : nullptr;
+  a.xform = xform;
+  HIP_TRY(ctx, rmsd_launch_fit_pairs(a, ctx->stream));
+  mark(ctx, 2);
+  if (modify != RMSD_MODIFY_NONE) {
+    // every pair is solved before any target is rewritten (stream order), so
+    // references may share the targets' storage ('previous')
+    ApplyArgs ap;
+    ap.xyz = const_cast<float*>(frames->xyz);
+    ap.stride = frames->stride;
+    ap.ld = frames->ld;
+    ap.natom = frames->natom;
+    ap.nframes = frames->nframes;
+    ap.xform = xform;
+    HIP_TRY(ctx, rmsd_launch_apply(ap, modify == RMSD_MODIFY_ROT_TRANS ? 1 : 2, ctx->stream));
+  }
+  mark(ctx, 3);
+  return RMSD_OK;
+}
+
 int rmsd_rotate_ucell(rmsd_ctx* ctx, const double* U, int nframes, double* ucell) {
   if (!ctx) return RMSD_ERR_ARG;
   if (nframes < 0 || (nframes > 0 && (!U || !ucell)))

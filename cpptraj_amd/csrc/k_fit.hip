@@ -488,6 +488,133 @@ __global__ void k_fit_solve(SolveArgs a, const double* shift) {
   }
 }
 
+// ---- pairs: frame f against reference frame f ---------------------------------
+// The 'reftraj' / 'previous' reference modes: every frame has its own
+// reference, so one workgroup per pair centres the reference (as k_ref_prep),
+// then takes the reference's sums (w y, w |y|^2) and the target's shifted
+// moments (as the moment kernels) in one pass over both frames, and solves as
+// k_fit_solve.  No fit: RMSD_NoFit in the reference's operation order.
+__global__ void k_mass_total(const double* mass, int nsel, double* W) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0.0;
+  if (mass) {
+    for (int k = 0; k < nsel; ++k) s += mass[k];
+  } else {
+    s = (double)nsel;
+  }
+  *W = s;
+}
+
+template <bool WEIGHTED>
+__global__ __launch_bounds__(FIT_THREADS) void k_fit_pairs(PairFitArgs a) {
+  __shared__ double red[17 * (FIT_THREADS / 64)];
+  const int64_t f = blockIdx.x;
+  const int n = a.T.nsel;
+  const double W = a.W[0];
+  if (!a.fit) {
+#pragma clang fp contract(off)
+    double v[1] = {0.0};
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+      const double dx = (double)fcoord(a.R, f, k, 0) - (double)fcoord(a.T, f, k, 0);
+      const double dy = (double)fcoord(a.R, f, k, 1) - (double)fcoord(a.T, f, k, 1);
+      const double dz = (double)fcoord(a.R, f, k, 2) - (double)fcoord(a.T, f, k, 2);
+      const double w = WEIGHTED ? a.tmass[k] : 1.0;
+      v[0] += (w * (dx * dx + dy * dy + dz * dz));
+    }
+    rmsd_dev::block_sum<1>(v, red);
+    if (threadIdx.x == 0) {
+      double r;
+      if (W < RMSD_SMALL) r = -1.0;
+      else if (v[0] < 0) r = 0.0;
+      else r = sqrt(v[0] / W);
+      a.rmsd[f] = r;
+    }
+    return;
+  }
+  // reference centroid (centring masses)
+  double c4[4] = {0, 0, 0, 0};
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const double m = a.rmass ? a.rmass[k] : 1.0;
+    c4[0] += (double)fcoord(a.R, f, k, 0) * m;
+    c4[1] += (double)fcoord(a.R, f, k, 1) * m;
+    c4[2] += (double)fcoord(a.R, f, k, 2) * m;
+    c4[3] += m;
+  }
+  rmsd_dev::block_sum<4>(c4, red);
+  double cen[3] = {0, 0, 0};
+  const double tot = a.rmass ? c4[3] : (double)n;
+  if (tot != 0.0) {
+    cen[0] = c4[0] / tot;
+    cen[1] = c4[1] / tot;
+    cen[2] = c4[2] / tot;
+  }
+  // target shifted by its first selected atom (an exact identity for the
+  // centred moments, so the sums carry the molecule's extent only)
+  const double sh[3] = {(double)fcoord(a.T, f, 0, 0), (double)fcoord(a.T, f, 0, 1),
+                        (double)fcoord(a.T, f, 0, 2)};
+  // v[0..12]: target moments (w x, w x y^T, w |x|^2); v[13..16]: w y, w |y|^2
+  double v[17];
+#pragma unroll
+  for (int i = 0; i < 17; ++i) v[i] = 0.0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const double y0 = (double)fcoord(a.R, f, k, 0) - cen[0];
+    const double y1 = (double)fcoord(a.R, f, k, 1) - cen[1];
+    const double y2 = (double)fcoord(a.R, f, k, 2) - cen[2];
+    const double w = WEIGHTED ? a.tmass[k] : 1.0;
+    double m13[13];
+#pragma unroll
+    for (int i = 0; i < 13; ++i) m13[i] = v[i];
+    accum<WEIGHTED>(m13, (double)fcoord(a.T, f, k, 0) - sh[0],
+                    (double)fcoord(a.T, f, k, 1) - sh[1], (double)fcoord(a.T, f, k, 2) - sh[2],
+                    w, y0, y1, y2);
+#pragma unroll
+    for (int i = 0; i < 13; ++i) v[i] = m13[i];
+    v[13] += w * y0;
+    v[14] += w * y1;
+    v[15] += w * y2;
+    v[16] += w * (y0 * y0 + y1 * y1 + y2 * y2);
+  }
+  rmsd_dev::block_sum<17>(v, red);
+  if (threadIdx.x != 0) return;
+  double U[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  double t1[3] = {0, 0, 0};
+  double r;
+  if (W < RMSD_SMALL) {
+    r = -1.0;
+  } else {
+    const double c0 = v[0] / W, c1 = v[1] / W, c2 = v[2] / W;
+    double R[9];
+    R[0] = v[3] - c0 * v[13]; R[1] = v[4] - c0 * v[14]; R[2] = v[5] - c0 * v[15];
+    R[3] = v[6] - c1 * v[13]; R[4] = v[7] - c1 * v[14]; R[5] = v[8] - c1 * v[15];
+    R[6] = v[9] - c2 * v[13]; R[7] = v[10] - c2 * v[14]; R[8] = v[11] - c2 * v[15];
+    const double sx = v[12] - W * (c0 * c0 + c1 * c1 + c2 * c2);
+    const double E0 = 0.5 * (sx + v[16]);
+    const rmsd_dev::QcpCoef co = rmsd_dev::qcp_coefficients(R);
+    const double lam = rmsd_dev::qcp_lambda(co, E0);
+    r = rmsd_dev::rmsd_from_lambda(E0, lam, W, a.clamp);
+    rmsd_dev::qcp_rotation(R, lam, U);
+    t1[0] = -(c0 + sh[0]);
+    t1[1] = -(c1 + sh[1]);
+    t1[2] = -(c2 + sh[2]);
+  }
+  a.rmsd[f] = r;
+  if (a.U)
+    for (int k = 0; k < 9; ++k) a.U[9 * f + k] = U[k];
+  if (a.tvec)
+    for (int k = 0; k < 3; ++k) {
+      a.tvec[6 * f + k] = t1[k];
+      a.tvec[6 * f + 3 + k] = cen[k];
+    }
+  if (a.xform) {
+    double* x = a.xform + 15 * f;
+    for (int k = 0; k < 9; ++k) x[k] = U[k];
+    for (int k = 0; k < 3; ++k) {
+      x[9 + k] = t1[k];
+      x[12 + k] = cen[k];
+    }
+  }
+}
+
 // ---- no-fit ------------------------------------------------------------------
 // sum w |y - x|^2 accumulated atom by atom per thread in the reference's
 // operation order (no contraction), then reduced over the block.
@@ -698,6 +825,22 @@ hipError_t rmsd_launch_nofit_frames(const NofitArgs& a, const double* consts, do
   else
     hipLaunchKernelGGL(k_nofit_frames<false>, dim3(a.fs.nframes), dim3(FIT_THREADS), 0, s, a, ldr,
                        consts, rmsd);
+  return hipGetLastError();
+}
+
+hipError_t rmsd_launch_mass_total(const double* mass, int nsel, double* W, hipStream_t s) {
+  hipLaunchKernelGGL(k_mass_total, dim3(1), dim3(64), 0, s, mass, nsel, W);
+  return hipGetLastError();
+}
+
+hipError_t rmsd_launch_fit_pairs(const PairFitArgs& a, hipStream_t s) {
+  if (a.T.nframes <= 0) return hipSuccess;
+  if (a.R.nframes < a.T.nframes || a.R.nsel != a.T.nsel || a.T.nsel <= 0)
+    return hipErrorInvalidValue;
+  if (a.tmass)
+    hipLaunchKernelGGL(k_fit_pairs<true>, dim3(a.T.nframes), dim3(FIT_THREADS), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_fit_pairs<false>, dim3(a.T.nframes), dim3(FIT_THREADS), 0, s, a);
   return hipGetLastError();
 }
 

@@ -201,6 +201,22 @@ struct SolveArgs {
   double* xform;            // 15 per frame (U, Trans, refTrans) for the coordinate update, or null
 };
 
+// Fit of target frame f of T to reference frame f of R, every f (k_fit.hip,
+// k_fit_pairs): the 'reftraj' and 'previous' reference modes of
+// ReferenceAction (src/ReferenceAction.h:73-88) as one launch.
+struct PairFitArgs {
+  FrameSetDev T, R;
+  const double* tmass;  // weights of the covariance and E0 (null: unit)
+  const double* rmass;  // centring masses of the reference (null: unit)
+  const double* W;      // sum of tmass, summed in atom order (device scalar)
+  int fit;              // 0: RMSD_NoFit
+  int clamp;
+  double* rmsd;
+  double* U;            // 9 per frame or null
+  double* tvec;         // 6 per frame or null
+  double* xform;        // 15 per frame for the coordinate update, or null
+};
+
 struct NofitArgs {
   FrameSetDev fs;
   const double* mass;
@@ -350,6 +366,9 @@ hipError_t rmsd_launch_pair_nofit_tiles(const FrameSetDev& A, const FrameSetDev&
                                         int tri, float* out, hipStream_t s);
 hipError_t rmsd_launch_fit_moments(const FitArgs& a, hipStream_t s);
 hipError_t rmsd_launch_fit_solve(const SolveArgs& a, hipStream_t s);
+// W = sum of the target masses (nsel when null), summed in atom order.
+hipError_t rmsd_launch_mass_total(const double* mass, int nsel, double* W, hipStream_t s);
+hipError_t rmsd_launch_fit_pairs(const PairFitArgs& a, hipStream_t s);
 hipError_t rmsd_launch_nofit_frames(const NofitArgs& a, const double* consts, double* rmsd,
                                     hipStream_t s);
 hipError_t rmsd_launch_apply(const ApplyArgs& a, int mode, hipStream_t s);

@@ -253,8 +253,10 @@ class Action_Rmsd {
   // or to the atoms of 'move <mask>' only.
   RetType InitAlign(ArgList& args, DataSetList& dsl, MaskResolver resolve, GpuSession& gpu);
   // Runs the action over every frame of `crd` (rewriting its coordinates
-  // unless 'nofit' or 'nomod').  Frames go to the library in batches when the
-  // reference is fixed (first / ref), one at a time otherwise.
+  // unless 'nofit' or 'nomod').  Frames go to the library in batches: against
+  // one fixed reference (first / ref), or each against its own (reftraj /
+  // previous, rmsd_fit_pairs); one at a time only for 'previous' / 'reftraj'
+  // with per-residue values or a move mask.
   RetType DoFrames(CoordsSet& crd);
   // Output files; with 'perres', also Action_Rmsd::Print (src/Action_Rmsd.cpp:
   // 423-470): the per-residue series (perresout, Grace) and their average and
@@ -266,6 +268,10 @@ class Action_Rmsd {
   RetType InitRef(ArgList& args, DataSetList& dsl, GpuSession& gpu);
   RetType RunBatch(CoordsSet& crd, const AtomMask& tsel, int f0, int nf, const float* refxyz,
                    const Topology& reftop, const std::string& refname);
+  // 'reftraj' (refset = the reference trajectory: frame f against its frame f)
+  // or 'previous' (refset = crd: frame f against frame f - 1 after this
+  // action's update of it; frame 0 against itself), in chunks of frames.
+  RetType RunPairs(CoordsSet& crd, const AtomMask& tsel, CoordsSet& refset, bool previous);
   // Action_Rmsd::perResSetup (src/Action_Rmsd.cpp:197-315): residue masks of
   // the target and reference topologies, one data set per residue.
   RetType PerResSetup(const Topology& tgt, const CoordsSet& tgtset, const Topology& ref,

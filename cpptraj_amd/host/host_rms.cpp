@@ -764,6 +764,151 @@ RetType Action_Rmsd::RunBatch(CoordsSet& crd, const AtomMask& tsel, int f0, int 
   return harvest(sl[0]) && harvest(sl[1]) ? OK : ERR;
 }
 
+// Every frame against its own reference (ReferenceAction::ActionRef /
+// PreviousRef, src/ReferenceAction.h:73-88) in chunks: one rmsd_fit_pairs
+// call per chunk.  'previous' fits the frames as stored (frame f against f - 1)
+// and composes the updates on the host: the reference of frame f is frame
+// f - 1 after its update G_(f-1) (a rigid motion, which leaves the RMSD
+// unchanged), so the fit onto it has U_f = L_(f-1) U_f and refTrans_f =
+// L_(f-1) (refTrans_f + Trans_(f-1)) + refTrans'_(f-1), L the applied
+// rotation (identity for the translate-only update).
+RetType Action_Rmsd::RunPairs(CoordsSet& crd, const AtomMask& tsel, CoordsSet& refset,
+                              bool previous) {
+  GpuSession& g = *gpu_;
+  const int nf = crd.nframes();
+  AtomMask rsel;
+  if (!resolve_or_all(resolve_, refMaskExpr_, refset, &rsel) || rsel.empty()) {
+    g.SetError("No reference atoms selected for [" + refMaskExpr_ + "]");
+    return ERR;
+  }
+  if (rsel.size() != tsel.size()) {
+    g.SetError("Number of atoms in target mask does not equal number of atoms in reference mask");
+    return ERR;
+  }
+  if (!previous && refset.nframes() < nf) {
+    g.SetError("reference trajectory has fewer frames than the target");
+    return ERR;
+  }
+  const int modify = !fit_ ? RMSD_MODIFY_NONE
+                           : (mode_ == ROT_AND_TRANS ? RMSD_MODIFY_ROT_TRANS
+                                                     : (mode_ == TRANS_ONLY ? RMSD_MODIFY_TRANS
+                                                                            : RMSD_MODIFY_NONE));
+  const bool compose = previous && modify != RMSD_MODIFY_NONE;
+  const bool rotbox = fit_ && modify == RMSD_MODIFY_ROT_TRANS &&
+                      crd.box.size() == 9 * (size_t)crd.nframes() && !crd.box.empty();
+  const bool needU = fit_ && (rmatrices_ || compose || rotbox);
+  const bool needT = fit_ && (tvecs_ || compose);
+  const size_t fbytes = sizeof(float) * (size_t)crd.stride;
+  int chunk = (int)std::max<size_t>(1, std::min<size_t>(nf, kStreamBytes / fbytes));
+  if (chunkFrames_ > 0) chunk = std::min(chunk, chunkFrames_);
+  const size_t base_r = rmsd_->v.size();
+  rmsd_->v.resize(base_r + nf);
+  const size_t base_u = rmatrices_ ? rmatrices_->v.size() : 0;
+  if (rmatrices_) rmatrices_->v.resize(base_u + 9 * (size_t)nf);
+  const size_t base_t = tvecs_ ? tvecs_->v.size() : 0;
+  if (tvecs_) tvecs_->v.resize(base_t + 6 * (size_t)nf);
+  double* dr = (double*)g.Buffer("pairs.rmsd", sizeof(double) * chunk);
+  double* dU = needU ? (double*)g.Buffer("pairs.U", sizeof(double) * 9 * chunk) : nullptr;
+  double* dT = needT ? (double*)g.Buffer("pairs.T", sizeof(double) * 6 * chunk) : nullptr;
+  double* dbox = rotbox ? (double*)g.Buffer("pairs.box", sizeof(double) * 9 * chunk) : nullptr;
+  if (!dr || (needU && !dU) || (needT && !dT) || (rotbox && !dbox)) return ERR;
+  std::vector<double> hr(chunk), hU(needU ? 9 * (size_t)chunk : 0),
+      hT(needT ? 6 * (size_t)chunk : 0);
+  // 'previous': frame f0 - 1 as stored, before this action moved it, ahead of
+  // the chunk's frames in one upload (for the first chunk frame 0 itself, its
+  // own reference).
+  std::vector<float> stage;
+  std::vector<float> prevOrig(crd.frame(0), crd.frame(0) + crd.stride);
+  double Lp[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, tp[3] = {0, 0, 0}, rp[3] = {0, 0, 0};
+  for (int f0 = 0; f0 < nf; f0 += chunk) {
+    const int n = std::min(chunk, nf - f0);
+    rmsd_frames T, R;
+    if (previous) {
+      stage.resize((size_t)(n + 1) * crd.stride);
+      std::copy(prevOrig.begin(), prevOrig.end(), stage.begin());
+      std::copy(crd.frame(f0), crd.frame(f0) + (size_t)n * crd.stride,
+                stage.begin() + crd.stride);
+      if (g.FramesOnDevice("pairs.prev", crd, rsel, &R, stage.data(), n + 1)) return ERR;
+      std::copy(crd.frame(f0 + n - 1), crd.frame(f0 + n - 1) + crd.stride, prevOrig.begin());
+      // targets: the same upload, one frame on (their own selection / masses)
+      if (g.FramesOnDevice("pairs.tsel", crd, tsel, &T, crd.frame(f0), 0)) return ERR;
+      T.xyz = R.xyz + crd.stride;
+      T.nframes = n;
+      R.nframes = n;
+    } else {
+      if (g.FramesOnDevice("pairs.tgt", crd, tsel, &T, crd.frame(f0), n) ||
+          g.FramesOnDevice("pairs.ref", refset, rsel, &R, refset.frame(f0), n))
+        return ERR;
+    }
+    if (g.Check(rmsd_fit_pairs(g.ctx(), &T, &R, fit_, useMass_,
+                               compose ? RMSD_MODIFY_NONE : modify, dr, dU, dT),
+                "rms"))
+      return ERR;
+    if (g.Download(hr.data(), dr, sizeof(double) * n) ||
+        (needU && g.Download(hU.data(), dU, sizeof(double) * 9 * n)) ||
+        (needT && g.Download(hT.data(), dT, sizeof(double) * 6 * n)))
+      return ERR;
+    if (compose) {
+      for (int f = 0; f < n; ++f) {
+        double* U = hU.data() + 9 * (size_t)f;
+        double* t = hT.data() + 6 * (size_t)f;
+        if (f0 + f > 0) {
+          double Uc[9], rc[3];
+          const double* L = modify == RMSD_MODIFY_ROT_TRANS ? Lp : nullptr;
+          for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+              Uc[3 * i + j] = L ? L[3 * i] * U[j] + L[3 * i + 1] * U[3 + j] + L[3 * i + 2] * U[6 + j]
+                                : U[3 * i + j];
+          const double v[3] = {t[3] + tp[0], t[4] + tp[1], t[5] + tp[2]};
+          for (int i = 0; i < 3; ++i)
+            rc[i] = (L ? L[3 * i] * v[0] + L[3 * i + 1] * v[1] + L[3 * i + 2] * v[2] : v[i]) +
+                    rp[i];
+          std::copy(Uc, Uc + 9, U);
+          std::copy(rc, rc + 3, t + 3);
+        }
+        std::copy(U, U + 9, Lp);
+        std::copy(t, t + 3, tp);
+        std::copy(t + 3, t + 6, rp);
+      }
+      rmsd_frames A = T;
+      A.fidx = nullptr;
+      if (g.Upload(dU, hU.data(), sizeof(double) * 9 * n) ||
+          g.Upload(dT, hT.data(), sizeof(double) * 6 * n) ||
+          g.Check(rmsd_transform(g.ctx(), &A, nullptr, 0, dU, dT, modify), "rms"))
+        return ERR;
+    }
+    if (rotbox) {
+      double* hbox = crd.box.data() + 9 * (size_t)f0;
+      if (g.Upload(dbox, hbox, sizeof(double) * 9 * n) ||
+          g.Check(rmsd_rotate_ucell(g.ctx(), dU, n, dbox), "rms box") ||
+          g.Download(hbox, dbox, sizeof(double) * 9 * n))
+        return ERR;
+    }
+    if (modify != RMSD_MODIFY_NONE &&
+        g.Download(crd.frame(f0), T.xyz, fbytes * n))
+      return ERR;
+    std::copy(hr.begin(), hr.begin() + n, rmsd_->v.begin() + base_r + f0);
+    if (rmatrices_)
+      std::copy(hU.begin(), hU.begin() + 9 * (size_t)n,
+                rmatrices_->v.begin() + base_u + 9 * (size_t)f0);
+    if (tvecs_)
+      for (int f = 0; f < n; ++f) {
+        double* o = tvecs_->v.data() + base_t + 6 * (size_t)(f0 + f);
+        for (int c = 0; c < 3; ++c) {
+          const double tr = hT[6 * f + c], rf = hT[6 * f + 3 + c];
+          if (tvecType_ == COMBINED) {
+            o[c] = tr + rf;
+            o[3 + c] = 0.0;
+          } else {
+            o[c] = tr;
+            o[3 + c] = rf;
+          }
+        }
+      }
+  }
+  return OK;
+}
+
 // Action_Align::Init (src/Action_Align.cpp): 'align <mask> [<refmask>]
 // [move <mask>] [mass]' plus the reference keywords; always fits and moves
 // (all atoms, or those of the move mask).  Its RMSD values are kept in a data
@@ -815,6 +960,7 @@ RetType Action_Rmsd::DoFrames(CoordsSet& crd) {
         g.SetError("reference trajectory has fewer frames than the target");
         return ERR;
       }
+      if (!perresReady_ && moveMaskExpr_.empty()) return RunPairs(crd, tsel, *refCrd_, false);
       for (int f = 0; f < nf; ++f)
         if (RunBatch(crd, tsel, f, 1, refCrd_->frame(f), refCrd_->top, refCrd_->name) != OK)
           return ERR;
@@ -822,6 +968,7 @@ RetType Action_Rmsd::DoFrames(CoordsSet& crd) {
     case PREVIOUS: {
       // Reference = the previous frame after this action's update of it
       // (ReferenceAction::PreviousRef, src/ReferenceAction.h:86-88).
+      if (!perresReady_ && moveMaskExpr_.empty()) return RunPairs(crd, tsel, crd, true);
       std::vector<float> prev(crd.frame(0), crd.frame(0) + crd.stride);
       for (int f = 0; f < nf; ++f) {
         if (RunBatch(crd, tsel, f, 1, prev.data(), crd.top, crd.name) != OK) return ERR;

@@ -176,6 +176,28 @@ int rmsd_fit_to_ref(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames*
                     int ref_frame, int fit, int use_mass, int modify, double* rmsd, double* U,
                     double* tvec);
 
+/* Fit of every target frame to its own reference frame: target f of `frames`
+ * against frame f of `refs` (refs->nframes >= frames->nframes; either set may
+ * use fidx).  Replaces the per-frame DoAction of Action_Rmsd
+ * (src/Action_Rmsd.cpp:361-417) in the 'reftraj' and 'previous' reference
+ * modes of ReferenceAction (src/ReferenceAction.h:73-88), where every frame has
+ * a different reference: one call per batch instead of one per frame.  The
+ * arguments and outputs mean what they mean for rmsd_fit_to_ref (the reference
+ * frame is centred with refs->mass, the target's masses weight the covariance;
+ * refTrans in tvec is frame f's reference centre).  With modify, the targets
+ * are rewritten after every pair is solved, so refs may share the targets'
+ * storage (refs->xyz = frames->xyz - frames->stride gives 'previous' on the
+ * frames as stored).  For 'previous' with a coordinate update the reference
+ * is the previous frame after its update: fit the frames as stored (modify
+ * NONE), then compose on the host U_f = L_(f-1) U_f and refTrans_f =
+ * L_(f-1) (refTrans_f + Trans_(f-1)) + refTrans'_(f-1), where L_(f-1) is the
+ * rotation applied to frame f-1 (its composed U for RMSD_MODIFY_ROT_TRANS, the
+ * identity for RMSD_MODIFY_TRANS) and refTrans'_(f-1) its composed refTrans
+ * (RMSD is unchanged by a rigid update of the reference), and apply them with
+ * rmsd_transform; the host layer does this. */
+int rmsd_fit_pairs(rmsd_ctx* ctx, const rmsd_frames* frames, const rmsd_frames* refs, int fit,
+                   int use_mass, int modify, double* rmsd, double* U, double* tvec);
+
 /* Allocation-free variant of rmsd_fit_to_ref: same arguments, same kernels,
  * same results bit for bit.  No counterpart in the reference; it is for
  * callers that cannot allocate device memory (code under HIP graph capture,

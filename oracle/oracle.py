@@ -379,6 +379,9 @@ class RefLib:
                                       _dp, _dp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp,
                                       _fp, C.c_int]
         L.cref_fit_frames.restype = C.c_int
+        L.cref_fit_pairs.argtypes = [_fp, _fp, C.c_int64, C.c_int, _ip, C.c_int, _dp, C.c_int,
+                                     _dp, C.c_int]
+        L.cref_fit_pairs.restype = C.c_int
         L.cref_perres.argtypes = [_fp, C.c_int64, C.c_int, C.c_int, _fp, C.c_int, C.c_int, _ip,
                                   _ip, _ip, _dp, _dp, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp]
         L.cref_perres.restype = C.c_int
@@ -506,6 +509,20 @@ class RefLib:
                                int(use_mass), modify, _d(rmsd), _d(U), _d(tv), _f(fo),
                                nthreads)
         return rmsd, U.reshape(nf, 3, 3), tv, fo
+
+    def fit_pairs(self, frames, ref_frames, sel=None, mass=None, use_mass=False, nthreads=0):
+        """'reftraj': frame f against reference frame f (centred per pair), nomod."""
+        fr = _c32(frames)
+        rf = _c32(ref_frames)
+        assert fr.shape == rf.shape
+        nf, natom = fr.shape[0], fr.shape[1] // 3
+        s = _ci(sel)
+        nsel = len(s) if s is not None else natom
+        m = _c64(mass) if mass is not None else np.ones(nsel)
+        rmsd = np.zeros(nf)
+        self.L.cref_fit_pairs(_f(fr), _f(rf), fr.shape[1], nf, _i(s), nsel, _d(m),
+                              int(use_mass), _d(rmsd), nthreads)
+        return rmsd
 
     def perres(self, frames, ref, res_off, tgt_atoms, ref_atoms=None, tmass=None,
                rmass=None, use_mass=False, center=False, U=None, tvec=None, modify=0):

@@ -265,6 +265,36 @@ int cref_fit_frames(const float* frames, int64_t stride, int nframes, int natom,
   return 0;
 }
 
+// 'reftraj' mode of Action_Rmsd::DoAction: for every pair the reference
+// frame's selected atoms centred with their masses (ReferenceAction::
+// SelectRefAtoms, src/ReferenceAction.cpp:155-169), then
+// Frame::RMSD_CenteredRef of the target frame (nomod).
+int cref_fit_pairs(const float* tgt, const float* ref, int64_t stride, int nframes,
+                   const int* sel, int nsel, const double* mass, int use_mass, double* rmsd,
+                   int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+#endif
+  {
+    std::vector<double> zr(3 * (size_t)nsel, 0.0), zt(3 * (size_t)nsel, 0.0);
+    Frame Ref = make_frame(zr.data(), mass, nsel);
+    Frame Tgt = make_frame(zt.data(), mass, nsel);
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (int f = 0; f < nframes; ++f) {
+      frame_from_floats(Ref, ref + (int64_t)f * stride, sel, nsel);
+      Ref.CenterOnOrigin(use_mass != 0);
+      frame_from_floats(Tgt, tgt + (int64_t)f * stride, sel, nsel);
+      Matrix_3x3 u(1.0);
+      Vec3 tr(0.0, 0.0, 0.0);
+      rmsd[f] = Tgt.RMSD_CenteredRef(Ref, u, tr, use_mass != 0);
+    }
+  }
+  return 0;
+}
+
 // Per-residue loop of Action_Rmsd::DoAction (src/Action_Rmsd.cpp:398-413):
 // the frame after the fit's update (Frame::Trans_Rot_Trans / Translate in
 // double), each residue copied out with its own masses, optionally centred

@@ -1,0 +1,156 @@
+"""rmsd_fit_pairs (every frame against its own reference frame) and the host
+layer's batched 'reftraj' / 'previous' modes built on it.
+
+Library: frame f of a set against frame f of a reference set, against the
+oracle's Action_Rmsd::DoAction (orc_fit_frames) on that pair alone: RMSD, U and
+vectors to 1e-9, updated coordinates to 1e-5 A (both stored as float);
+RMSD_NoFit pairs against orc_rmsd_nofit; references sharing the targets'
+storage ('previous' on the frames as stored) give the same results as copies.
+
+Host: 'rms ... previous' with the coordinate update against the reference's
+chain run frame by frame in FP64 with the oracle (each frame fitted to the
+previous frame after its update, as ReferenceAction::PreviousRef keeps it):
+RMSD (frame 0, fitted to itself, below 1e-6 in both), matrices and vectors
+to 1e-8, coordinates to 1e-4 A (FP32 storage of the updated frames); in one chunk and in chunks of 7 frames (the composition
+crosses chunk boundaries); rotate + translate and translate only.  'reftraj'
+against the oracle pair by pair, in chunks.
+"""
+import numpy as np
+import pytest
+
+from cpptraj_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def hs():
+    from cpptraj_amd.host import HostSession
+    s = HostSession(0)
+    yield s
+    s.close()
+
+
+def pair_oracle(oracle, X, Y, sel, mass, use_mass, modify=0):
+    """Frame f of X fitted to frame f of Y with the oracle, pair by pair."""
+    F, n = X.shape[0], X.shape[1] // 3
+    r = np.zeros(F)
+    U = np.zeros((F, 3, 3))
+    tv = np.zeros((F, 6))
+    fo = np.zeros_like(X)
+    for f in range(F):
+        ref = Y[f].reshape(n, 3).astype(np.float64)[sel]
+        rc, rt = oracle.center_on_origin(ref, mass, use_mass)
+        rr, UU, tt, ff = oracle.fit_frames(X[f:f + 1], rc, rt, sel=sel, mass=mass, fit=True,
+                                           use_mass=use_mass, modify=modify, want_frames=True)
+        r[f], U[f], tv[f], fo[f] = rr[0], UU[0], tt[0], ff[0]
+    return r, U, tv, fo
+
+
+@pytest.mark.parametrize("use_mass", [False, True])
+def test_pairs_match_the_oracle_pair_by_pair(ctx, oracle, use_mass):
+    import torch
+    import cpptraj_amd as ca
+    F, n = 40, 150
+    X = synth.noisy(31, F, n, 1.0).reshape(F, -1).astype(np.float32)
+    Y = synth.noisy(32, F, n, 1.0).reshape(F, -1).astype(np.float32)
+    sel = np.arange(1, n, 2, dtype=np.int32)
+    m = synth.masses(np.random.default_rng(3), len(sel))
+    exp = pair_oracle(oracle, X, Y, sel, m, use_mass, modify=1)
+    Xd = torch.as_tensor(X).cuda()
+    Yd = torch.as_tensor(Y).cuda()
+    s = torch.as_tensor(sel).cuda()
+    md = torch.as_tensor(m).cuda()
+    r, U, tv = ca.rmsd_fit_pairs(Xd, Yd, mask=s, mass=md, use_mass=use_mass, modify="rottrans",
+                                 save_matrices=True, save_vectors=True, ctx=ctx)
+    assert np.abs(r.cpu().numpy() - exp[0]).max() <= 1e-9
+    assert np.abs(U.cpu().numpy() - exp[1]).max() <= 1e-9
+    assert np.abs(tv.cpu().numpy() - exp[2]).max() <= 1e-9
+    assert np.abs(Xd.cpu().numpy() - exp[3]).max() <= 1e-5
+    # no fit: RMSD_NoFit of each pair
+    rn, _, _ = ca.rmsd_fit_pairs(torch.as_tensor(X).cuda(), Yd, mask=s, mass=md,
+                                 use_mass=use_mass, fit=False, ctx=ctx)
+    n_exp = np.array([oracle.rmsd_nofit(X[f].reshape(n, 3)[sel], Y[f].reshape(n, 3)[sel], m,
+                                        use_mass) for f in range(F)])
+    assert np.abs(rn.cpu().numpy() - n_exp).max() <= 1e-9
+
+
+def test_pairs_with_references_in_the_targets_storage(ctx):
+    """'previous' on frames as stored: targets X[1:], references X[:-1], views
+    of one tensor, with the coordinate update: every pair is solved before any
+    target is rewritten, so the results equal those on separate copies."""
+    import torch
+    import cpptraj_amd as ca
+    F, n = 33, 90
+    X = synth.noisy(41, F, n, 1.0).reshape(F, -1).astype(np.float32)
+    shared = torch.as_tensor(X).cuda()
+    r1, U1, t1 = ca.rmsd_fit_pairs(shared[1:], shared[:-1], modify="rottrans",
+                                   save_matrices=True, save_vectors=True, ctx=ctx)
+    tgt = torch.as_tensor(X[1:]).cuda()
+    ref = torch.as_tensor(X[:-1]).cuda()
+    r2, U2, t2 = ca.rmsd_fit_pairs(tgt, ref, modify="rottrans", save_matrices=True,
+                                   save_vectors=True, ctx=ctx)
+    assert torch.equal(r1, r2) and torch.equal(U1, U2) and torch.equal(t1, t2)
+    assert torch.equal(shared[1:], tgt)
+    assert torch.equal(shared[0], torch.as_tensor(X[0]).cuda())
+
+
+def previous_chain(oracle, X, sel, mass, use_mass, modify):
+    """The reference's 'previous' mode frame by frame in FP64: frame f fitted to
+    frame f - 1 after its update (frame 0 to itself), then updated."""
+    F, n = X.shape[0], X.shape[1] // 3
+    prev = X[0].reshape(n, 3).astype(np.float64)
+    r = np.zeros(F)
+    U = np.zeros((F, 3, 3))
+    tv = np.zeros((F, 6))
+    out = np.zeros((F, n, 3))
+    for f in range(F):
+        rc, rt = oracle.center_on_origin(prev[sel], mass, use_mass)
+        rr, UU, tt, _ = oracle.fit_frames(X[f:f + 1], rc, rt, sel=sel, mass=mass, fit=True,
+                                          use_mass=use_mass, modify=0)
+        x = X[f].reshape(n, 3).astype(np.float64)
+        if modify == "rottrans":
+            x = (x + tt[0, :3]) @ UU[0].T + tt[0, 3:]
+        elif modify == "norotate":
+            x = x + (tt[0, :3] + tt[0, 3:])
+        r[f], U[f], tv[f], out[f] = rr[0], UU[0], tt[0], x
+        prev = x
+    return r, U, tv, out
+
+
+@pytest.mark.parametrize("extra,modify", [("", "rottrans"), (" chunk 7", "rottrans"),
+                                          (" norotate chunk 7", "norotate"),
+                                          (" mass chunk 5", "rottrans")])
+def test_host_previous_composes_the_updates(hs, oracle, extra, modify):
+    F, n = 30, 120
+    X = synth.noisy(51, F, n, 2.0).reshape(F, -1).astype(np.float32)
+    m = synth.masses(np.random.default_rng(51), n)
+    sel = np.arange(0, n, 3, dtype=np.int32)
+    hs.add_coords("p", X, n, m)
+    hs.add_mask(":1-9", sel)
+    hs.run(f"rms P previous :1-9 savematrices savevectors separate{extra} crdset p")
+    use_mass = "mass" in extra
+    r, U, tv, out = previous_chain(oracle, X, sel, m[sel], use_mass, modify)
+    got = hs.series("P")
+    # frame 0 against itself: both are the rounding noise of a zero RMSD
+    # (sqrt of ~1e-14 relative), so only their size is compared
+    assert got[0] <= 1e-6 and r[0] <= 1e-6
+    assert np.abs(got[1:] - r[1:]).max() <= 1e-8
+    assert np.abs(hs.series("P[RM]").reshape(F, 3, 3) - U).max() <= 1e-8
+    assert np.abs(hs.series("P[TV]").reshape(F, 6) - tv).max() <= 1e-8
+    assert np.abs(hs.coords("p").reshape(F, n, 3) - out).max() <= 1e-4
+
+
+def test_host_reftraj_in_chunks(hs, oracle):
+    F, n = 25, 100
+    X = synth.noisy(61, F, n, 1.0).reshape(F, -1).astype(np.float32)
+    Y = synth.noisy(62, F, n, 1.0).reshape(F, -1).astype(np.float32)
+    sel = np.arange(n, dtype=np.int32)
+    hs.add_coords("t", X, n)
+    hs.add_coords("r", Y, n)
+    hs.add_mask(":1-100", sel)
+    hs.run("rms R reftraj r :1-100 savematrices chunk 6 crdset t")
+    r, U, _, fo = pair_oracle(oracle, X, Y, sel, None, False, modify=1)
+    assert np.abs(hs.series("R") - r).max() <= 1e-9
+    assert np.abs(hs.series("R[RM]").reshape(F, 3, 3) - U).max() <= 1e-9
+    assert np.abs(hs.coords("t").reshape(F, -1) - fo).max() <= 1e-5

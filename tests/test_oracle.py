@@ -512,3 +512,23 @@ def test_symm_without_groups_is_the_plain_fit(oracle):
                                        want_frames=True)
     assert same(rm, r2) and same(U, U2.reshape(-1, 9)) and same(tr, tv2[:, :3])
     assert np.array_equal(amap, np.tile(np.arange(50), (8, 1)))
+
+
+@pytest.mark.parametrize("use_mass", [False, True])
+def test_fit_pairs_shim_is_the_per_pair_fit(reflib, use_mass):
+    """cref_fit_pairs (the 'reftraj' CPU baseline) equals, bit for bit, the
+    compiled fit of each pair on its own: the reference frame centred with its
+    masses (Frame::CenterOnOrigin), then RMSD_CenteredRef."""
+    from cpptraj_amd import synth
+    F, n = 12, 60
+    X = synth.noisy(5, F, n, 1.0).reshape(F, -1).astype(np.float32)
+    Y = synth.noisy(6, F, n, 1.0).reshape(F, -1).astype(np.float32)
+    sel = np.arange(0, n, 2, dtype=np.int32)
+    m = synth.masses(np.random.default_rng(1), len(sel))
+    got = reflib.fit_pairs(X, Y, sel, m, use_mass)
+    exp = []
+    for f in range(F):
+        rc, rt = reflib.center_on_origin(Y[f].reshape(n, 3).astype(np.float64)[sel], m, use_mass)
+        r, _, _, _ = reflib.fit_frames(X[f:f + 1], rc, rt, sel=sel, mass=m, use_mass=use_mass)
+        exp.append(r[0])
+    assert np.array_equal(got, np.array(exp))
