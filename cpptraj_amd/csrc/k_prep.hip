@@ -357,9 +357,27 @@ __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
   }
 }
 
+// Workgroup -> frame.  The hardware deals consecutive workgroups round-robin
+// over the 8 XCDs, and the 16 frames of a packed frame block write interleaved
+// 16-byte slots of the same 1 KiB pieces; over 8 L2s each 128-byte line went
+// out in partial writes from 8 places.  Here the 16 frames of a block run on
+// one XCD, whose L2 merges their slots before the line is written back
+// (k_prep_batch 7.8 -> 5.9 ms on the 3 x 20,000-frame batch).  The map is a
+// bijection of [0, prep_grid(n)) onto itself.
+constexpr int PREP_XCDS = 8;
+__device__ __forceinline__ int64_t prep_frame(int64_t b) {
+  const int64_t xcd = b % PREP_XCDS, i = b / PREP_XCDS;
+  return ((i / 16) * PREP_XCDS + xcd) * 16 + (i % 16);
+}
+
+int64_t prep_grid(int64_t n) {
+  return (n + 16 * PREP_XCDS - 1) / (16 * PREP_XCDS) * 16 * PREP_XCDS;
+}
+
 template <int PREC>
 __global__ __launch_bounds__(PREP_THREADS) void k_prep(PrepArgs a) {
-  prep_body<PREC>(a, blockIdx.x);
+  const int64_t f = prep_frame(blockIdx.x);
+  if (f < a.Fp) prep_body<PREC>(a, f);
 }
 
 // Every padded frame of every set of a batch in one launch: workgroup b is
@@ -367,7 +385,10 @@ __global__ __launch_bounds__(PREP_THREADS) void k_prep(PrepArgs a) {
 template <int PREC>
 __global__ __launch_bounds__(PREP_THREADS) void k_prep_batch(const PrepArgs* sets,
                                                              const int* fbase, int nsets) {
-  const int b = blockIdx.x;
+  // every set's padded frame count is a multiple of 16, so no frame block
+  // straddles two sets
+  const int b = (int)prep_frame(blockIdx.x);
+  if (b >= fbase[nsets]) return;
   int lo = 0, hi = nsets - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -416,15 +437,15 @@ hipError_t rmsd_launch_prep_batch(int prec, const PrepArgs* sets, const int* fba
   if (nsets <= 0 || total_frames <= 0) return hipSuccess;
   switch (prec) {
     case PREC_F64:
-      hipLaunchKernelGGL(k_prep_batch<PREC_F64>, dim3(total_frames), dim3(PREP_THREADS), 0, s,
+      hipLaunchKernelGGL(k_prep_batch<PREC_F64>, dim3(prep_grid(total_frames)), dim3(PREP_THREADS), 0, s,
                          sets, fbase, nsets);
       break;
     case PREC_F32:
-      hipLaunchKernelGGL(k_prep_batch<PREC_F32>, dim3(total_frames), dim3(PREP_THREADS), 0, s,
+      hipLaunchKernelGGL(k_prep_batch<PREC_F32>, dim3(prep_grid(total_frames)), dim3(PREP_THREADS), 0, s,
                          sets, fbase, nsets);
       break;
     case PREC_BF16X3:
-      hipLaunchKernelGGL(k_prep_batch<PREC_BF16X3>, dim3(total_frames), dim3(PREP_THREADS), 0, s,
+      hipLaunchKernelGGL(k_prep_batch<PREC_BF16X3>, dim3(prep_grid(total_frames)), dim3(PREP_THREADS), 0, s,
                          sets, fbase, nsets);
       break;
     default: return hipErrorInvalidValue;
@@ -434,10 +455,14 @@ hipError_t rmsd_launch_prep_batch(int prec, const PrepArgs* sets, const int* fba
 
 hipError_t rmsd_launch_prep(const PrepArgs& a, hipStream_t s) {
   switch (a.prec) {
-    case PREC_F64: hipLaunchKernelGGL(k_prep<PREC_F64>, dim3(a.Fp), dim3(PREP_THREADS), 0, s, a); break;
-    case PREC_F32: hipLaunchKernelGGL(k_prep<PREC_F32>, dim3(a.Fp), dim3(PREP_THREADS), 0, s, a); break;
+    case PREC_F64:
+      hipLaunchKernelGGL(k_prep<PREC_F64>, dim3(prep_grid(a.Fp)), dim3(PREP_THREADS), 0, s, a);
+      break;
+    case PREC_F32:
+      hipLaunchKernelGGL(k_prep<PREC_F32>, dim3(prep_grid(a.Fp)), dim3(PREP_THREADS), 0, s, a);
+      break;
     case PREC_BF16X3:
-      hipLaunchKernelGGL(k_prep<PREC_BF16X3>, dim3(a.Fp), dim3(PREP_THREADS), 0, s, a);
+      hipLaunchKernelGGL(k_prep<PREC_BF16X3>, dim3(prep_grid(a.Fp)), dim3(PREP_THREADS), 0, s, a);
       break;
     default: return hipErrorInvalidValue;
   }
