@@ -124,6 +124,16 @@ class MatrixFlt {
   const float* data() const { return v_.data(); }
   size_t size() const { return v_.size(); }
   std::string name;
+  // Pairwise-cache bookkeeping (DataSet_PairwiseCache, src/DataSet_PairwiseCache.h):
+  // set on a cluster cache; `frames` are the cached frame numbers (row order)
+  // of a run over `ntotal` frames with sieve `sieve`.
+  bool pwcache = false;
+  size_t ntotal = 0;
+  int sieve = 1;
+  std::vector<int> frames;
+  // DataSet_PairwiseCache::CachedFramesMatch (src/DataSet_PairwiseCache.cpp:39-48):
+  // every frame of `want` is below ntotal and cached.
+  bool CachedFramesMatch(const std::vector<int>& want) const;
 
  private:
   Kind kind_ = FULL;
@@ -154,6 +164,7 @@ class DataSetList {
   MatrixFlt* AddMatrix(const std::string& name);
   Series* AddSeries(const std::string& name);
   const MatrixFlt* FindMatrix(const std::string& name) const;
+  MatrixFlt* FindPairCache(const std::string& name);  // a matrix with pwcache set
   const Series* FindSeries(const std::string& name) const;
   std::string GenerateDefaultName(const std::string& base);
 
@@ -358,11 +369,18 @@ class Analysis_RmsAvgCorr {
 
 // Cluster::Metric_RMS with the pairwise cache of Cluster::MetricArray:
 // 'cluster' arguments rms [<mask>] [mass] [nofit] [sieve <#>]
-// [pairdist <file>] [savepairdist]; builds and optionally saves the cache
-// (Cmatrix binary, src/Cluster/Cmatrix_Binary.cpp).  'srmsd [<mask>] [mass]'
-// caches Cluster::Metric_SRMSD's distances instead (symmetric groups from the
-// set's topology).  The clustering algorithms that consume the cache are not
-// part of this path.
+// [pairdist <name> [pairdistfile <file>]] [loadpairdist] [savepairdist]
+// [pwrecalc]; loads, builds and optionally saves the cache (Cmatrix binary,
+// src/Cluster/Cmatrix_Binary.cpp) with MetricArray::setupPairwiseCache's rules
+// (src/Cluster/MetricArray.cpp:104-272): with 'loadpairdist', or 'pairdist'
+// without 'savepairdist', an existing cache set of that name is used, else the
+// file is read (a loaded file is not saved again unless 'pwrecalc'), else the
+// distances are computed and saved.  A cache whose frames do not cover the
+// frames to cluster is an error, or is recomputed with 'pwrecalc'
+// (MetricArray::CacheDistances, :725-760).  'srmsd [<mask>] [mass]' caches
+// Cluster::Metric_SRMSD's distances instead (symmetric groups from the set's
+// topology).  The clustering algorithms that consume the cache are not part of
+// this path.
 class Metric_RMS {
  public:
   RetType Setup(ArgList& args, DataSetList& dsl, MaskResolver resolve, GpuSession& gpu);
@@ -373,11 +391,13 @@ class Metric_RMS {
   const std::vector<int>& FramesToCache() const { return frames_; }
 
  private:
+  RetType SaveCache();
   bool useMass_ = false, nofit_ = false, save_ = false, symm_ = false;
+  bool mismatchFatal_ = true;  // no 'pwrecalc'
   std::vector<int32_t> goff_, gatoms_;  // symmetric groups (selection indices)
   int sieve_ = 1;
   int precision_ = RMSD_PREC_FP64;
-  std::string maskExpr_, pairdist_;
+  std::string maskExpr_, pairdistFile_;
   CoordsSet* crd_ = nullptr;
   AtomMask sel_;
   std::vector<int> frames_;

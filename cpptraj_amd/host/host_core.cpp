@@ -160,6 +160,21 @@ const MatrixFlt* DataSetList::FindMatrix(const std::string& name) const {
   return nullptr;
 }
 
+MatrixFlt* DataSetList::FindPairCache(const std::string& name) {
+  for (auto it = mats_.rbegin(); it != mats_.rend(); ++it)
+    if ((*it)->pwcache && (*it)->name == name) return it->get();
+  return nullptr;
+}
+
+bool MatrixFlt::CachedFramesMatch(const std::vector<int>& want) const {
+  std::vector<char> present(ntotal, 0);
+  for (int f : frames)
+    if (f >= 0 && (size_t)f < ntotal) present[f] = 1;
+  for (int f : want)
+    if (f < 0 || (size_t)f >= ntotal || !present[f]) return false;
+  return true;
+}
+
 const Series* DataSetList::FindSeries(const std::string& name) const {
   for (auto it = series_.rbegin(); it != series_.rend(); ++it)
     if ((*it)->name == name) return it->get();
@@ -370,7 +385,9 @@ int Read(const std::string& path, MatrixFlt* tri, size_t* actual_nrows, int* sie
     for (size_t i = 0; ok && i < original; ++i)
       if (present[i] == 'F') frames->push_back((int)i);
   } else if (ok) {
-    for (uint64_t i = 0; i < nrows; ++i) frames->push_back((int)i);
+    // sieve 1: every frame of the run is present
+    // (DataIO_Cmatrix_Binary::ReadCmatrix, src/DataIO_Cmatrix_Binary.cpp:53-59)
+    for (uint64_t i = 0; i < original; ++i) frames->push_back((int)i);
   }
   fclose(f);
   if (!ok) return 1;

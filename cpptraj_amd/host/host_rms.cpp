@@ -1335,8 +1335,11 @@ RetType Metric_RMS::Setup(ArgList& args, DataSetList& dsl, MaskResolver resolve,
     gpu.SetError("cluster: " + perr);
     return ERR;
   }
-  pairdist_ = args.GetStringKey("pairdist");
+  std::string pdSet = args.GetStringKey("pairdist");
+  pairdistFile_ = args.GetStringKey("pairdistfile");
+  const bool load = args.hasKey("loadpairdist");
   save_ = args.hasKey("savepairdist");
+  mismatchFatal_ = !args.hasKey("pwrecalc");
   maskExpr_ = args.GetMaskNext();
   if (!resolve_or_all(resolve_, maskExpr_, *crd_, &sel_) || sel_.empty()) {
     gpu.SetError("cluster: no atoms selected for [" + maskExpr_ + "]");
@@ -1349,9 +1352,45 @@ RetType Metric_RMS::Setup(ArgList& args, DataSetList& dsl, MaskResolver resolve,
       return ERR;
     }
   }
-  std::string name = args.GetStringNext();
-  if (name.empty()) name = dsl.GenerateDefaultName("CpptrajPairDist");
-  cache_ = dsl.AddMatrix(name);
+  const std::string name = args.GetStringNext();
+  // MetricArray::setupPairwiseCache (src/Cluster/MetricArray.cpp:124-190).
+  static const char* kDefault = "CpptrajPairDist";
+  if (!pdSet.empty() && pairdistFile_.empty()) pairdistFile_ = pdSet;
+  cache_ = nullptr;
+  if (load || (!save_ && !pdSet.empty())) {
+    if (pdSet.empty()) pdSet = kDefault;
+    if (pairdistFile_.empty()) pairdistFile_ = kDefault;
+    if ((cache_ = dsl.FindPairCache(pdSet)) == nullptr) {
+      FILE* probe = fopen(pairdistFile_.c_str(), "rb");
+      if (probe) {
+        fclose(probe);
+        MatrixFlt tri;
+        size_t ntotal = 0;
+        int sv = 1;
+        std::vector<int> fr;
+        if (cmatrix::Read(pairdistFile_, &tri, &ntotal, &sv, &fr)) {
+          gpu.SetError("cluster: cannot read pairwise cache file " + pairdistFile_);
+          return ERR;
+        }
+        cache_ = dsl.AddMatrix(pdSet);
+        *cache_ = std::move(tri);
+        cache_->name = pdSet;
+        cache_->pwcache = true;
+        cache_->ntotal = ntotal;
+        cache_->sieve = sv;
+        cache_->frames = std::move(fr);
+        // a cache read from its file is not written back unless 'pwrecalc'
+        if (save_ && mismatchFatal_) save_ = false;
+      }
+    }
+    // neither a set nor a file: compute the distances and save them
+    if (!cache_) save_ = true;
+  }
+  if (!cache_) {
+    cache_ = dsl.AddMatrix(!pdSet.empty() ? pdSet
+                                          : !name.empty() ? name : dsl.GenerateDefaultName(kDefault));
+  }
+  if (save_ && pairdistFile_.empty()) pairdistFile_ = kDefault;
   // Regular sieve (Cluster::Sieve::SetFramesToCluster, src/Cluster/Sieve.cpp:42-55).
   frames_.clear();
   for (int f = 0; f < crd_->nframes(); f += sieve_) frames_.push_back(f);
@@ -1362,6 +1401,17 @@ RetType Metric_RMS::Setup(ArgList& args, DataSetList& dsl, MaskResolver resolve,
 // MetricArray.cpp:722-801): the triangle over framesToCache in one call.
 RetType Metric_RMS::CacheDistances() {
   GpuSession& g = *gpu_;
+  if (frames_.empty()) return OK;
+  if (cache_->pwcache && cache_->Nrows() > 0) {
+    // an existing cache: used when it holds every frame to cluster
+    // (MetricArray::CacheDistances, src/Cluster/MetricArray.cpp:729-747)
+    if (cache_->CachedFramesMatch(frames_)) return SaveCache();
+    if (mismatchFatal_) {
+      g.SetError("cluster: frames to cluster do not match those in existing cache '" +
+                 cache_->name + "' (specify 'pwrecalc' to recalculate)");
+      return ERR;
+    }
+  }
   rmsd_frames F;
   if (g.FramesOnDevice("cluster.crd", *crd_, sel_, &F)) return ERR;
   const int n = (int)frames_.size();
@@ -1388,12 +1438,21 @@ RetType Metric_RMS::CacheDistances() {
       return ERR;
     if (g.Download(cache_->data(), d, sizeof(float) * cache_->size())) return ERR;
   }
-  if (save_ || !pairdist_.empty()) {
-    const std::string path = pairdist_.empty() ? "CpptrajPairDist" : pairdist_;
-    if (cmatrix::Write(path, *cache_, crd_->nframes(), sieve_, frames_)) {
-      g.SetError("cannot write " + path);
-      return ERR;
-    }
+  // DataSet_PairwiseCache::SetupCache's bookkeeping (src/DataSet_PairwiseCache.cpp)
+  cache_->pwcache = true;
+  cache_->ntotal = (size_t)crd_->nframes();
+  cache_->sieve = sieve_;
+  cache_->frames = frames_;
+  return SaveCache();
+}
+
+// 'savepairdist' (or a 'pairdist' file that did not exist yet): the cache in
+// the Cmatrix binary format, over the cache's own frames.
+RetType Metric_RMS::SaveCache() {
+  if (!save_) return OK;
+  if (cmatrix::Write(pairdistFile_, *cache_, cache_->ntotal, cache_->sieve, cache_->frames)) {
+    gpu_->SetError("cannot write " + pairdistFile_);
+    return ERR;
   }
   return OK;
 }

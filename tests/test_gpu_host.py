@@ -244,6 +244,51 @@ def test_cluster_cache_cmatrix(hs, golden, oracle, tmp_path):
     assert raw[28 + 4 * 6:] == b"FTTFTTFTTF"
 
 
+def test_cluster_cache_loaded_from_file_or_set(hs, golden, oracle, tmp_path):
+    """MetricArray::setupPairwiseCache / CacheDistances: 'pairdist <name>
+    pairdistfile <file>' reads the file when no cache set of that name exists
+    (the planted values come back unchanged: nothing is recomputed); a cache
+    whose frames cover the frames to cluster is used as it is (a sieve-1 cache
+    for a sieve-3 run); a cache that does not cover them is an error, or is
+    recomputed with 'pwrecalc' (and rewritten with 'pwrecalc savepairdist');
+    a 'pairdist' file that does not exist is computed and saved; a cache set
+    already in the session is used again."""
+    from cpptraj_amd.host import RmsdError, read_cmatrix, write_cmatrix
+    g = load_tz2(hs, golden)
+    sel, m = g["sel_3_7"], g["mass"][g["sel_3_7"]]
+    path = str(tmp_path / "planted.cmatrix")
+    planted = np.arange(6, dtype=np.float32) * 0.25 + 1.0
+    write_cmatrix(path, planted, 4, 10, 3, np.array([0, 3, 6, 9], np.int32))
+    hs.run(f"cluster crdset tz2 rms :3-7 mass sieve 3 pairdist A pairdistfile {path}")
+    assert np.array_equal(hs.matrix("A"), planted)
+    # sieve 2 needs frames 2, 4, 8, which the file does not hold
+    with pytest.raises(RmsdError, match="do not match"):
+        hs.run(f"cluster crdset tz2 rms :3-7 mass sieve 2 pairdist B pairdistfile {path}")
+    hs.run(f"cluster crdset tz2 rms :3-7 mass sieve 2 pairdist C pairdistfile {path} pwrecalc")
+    exp2 = oracle.cluster_cache(g["frames"], np.arange(0, 10, 2, dtype=np.int32), sel, m,
+                                use_mass=True)
+    assert np.abs(hs.matrix("C") - exp2).max() <= 1e-6
+    assert np.array_equal(read_cmatrix(path)[0], planted)   # loaded: not written back
+    hs.run(f"cluster crdset tz2 rms :3-7 mass sieve 2 pairdist D pairdistfile {path} "
+           "pwrecalc savepairdist")
+    tri, actual, sieve, frames = read_cmatrix(path)
+    assert (actual, sieve, list(frames)) == (10, 2, [0, 2, 4, 6, 8])
+    assert np.array_equal(tri, hs.matrix("D"))
+    # a sieve-1 cache holds every frame of a sieve-3 run: used as it is
+    full = str(tmp_path / "full.cmatrix")
+    hs.run(f"cluster crdset tz2 rms :3-7 mass pairdist {full}")   # not there: computed, saved
+    exp1 = oracle.cluster_cache(g["frames"], np.arange(10, dtype=np.int32), sel, m, use_mass=True)
+    tri, actual, sieve, frames = read_cmatrix(full)
+    assert (actual, sieve, len(tri)) == (10, 1, 45)
+    assert np.abs(tri - exp1).max() <= 1e-6
+    write_cmatrix(full, planted[:1].repeat(45), 10, 10, 1, np.arange(10, dtype=np.int32))
+    hs.run(f"cluster crdset tz2 rms :3-7 mass sieve 3 loadpairdist pairdist E pairdistfile {full}")
+    assert np.array_equal(hs.matrix("E"), planted[:1].repeat(45))
+    # the set 'A' of the session is used before any file of that name
+    hs.run("cluster crdset tz2 rms :3-7 mass sieve 3 loadpairdist pairdist A")
+    assert np.array_equal(hs.matrix("A"), planted)
+
+
 @pytest.mark.parametrize("extra", ["", "mass savematrices savevectors separate", "norotate"])
 def test_rms_streamed_chunks_match_one_batch(hs, extra):
     """The host layer's double-buffered streaming (chunks on two transfer
