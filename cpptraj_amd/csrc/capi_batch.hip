@@ -261,8 +261,10 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
   b->outs.assign(outs, outs + nsets);
   const int CH = chunk_atoms(b->prec);
   // Frames padded to whole tiles of every batch shape (rows 64, columns up to
-  // rmsd_pairwise_tile_cols).
-  const int pad = std::lcm(TILE, rmsd_pairwise_tile_cols(b->prec));
+  // those of shape 0).
+  int tr0 = 0, tc0 = 0;
+  rmsd_batch_tile_shape(b->prec, 0, &tr0, &tc0);
+  const int pad = std::lcm(TILE, tc0);
   const size_t esz = b->prec == PREC_F64 ? 8 : (b->prec == PREC_F32 ? 4 : 2);
   // Layout of the single allocation.
   auto al = [](size_t x) { return (x + 255) & ~size_t(255); };

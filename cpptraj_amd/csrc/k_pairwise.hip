@@ -475,6 +475,114 @@ __device__ __forceinline__ void pair_tile(const PairArgs& args, int2 tile, int t
   pair_epilogue<PREC, WR, WC>(args, acc, fbA, fbB, lane, tile_id, wave, NWR * NWC);
 }
 
+// ---- Operands shared through LDS ---------------------------------------------------
+// One 64 x 64 tile of frame pairs per 4-wave workgroup (2 x 2 waves of 32 x 32
+// pairs).  Per chunk of atoms the workgroup needs NF fragments of 1 KiB (4 row
+// and 4 column frame blocks x 3 coordinates x PARTS).  They go from L2 straight
+// into LDS with global_load_lds_dwordx4 (64 lanes x 16 B = one fragment per
+// instruction, no register staging), so each fragment crosses the vector memory
+// path once per workgroup rather than once per wave that uses it: 111 instead
+// of 227 bytes per split-BF16 MFMA.  One 48 KiB LDS stage and at most 256
+// registers per wave let two workgroups share a CU (two waves per SIMD): while
+// one waits at a barrier or runs its FP64 epilogue, the other's MFMAs run.
+// Per chunk: own DMA landed + barrier (the chunk is in LDS), each wave reads
+// its 24 fragments, barrier (the stage is free again), DMA of the next chunk,
+// then the chunk's MFMAs from registers.  Chunk order and product order per
+// pair are those of pair_tile, so the results are bit-identical to the
+// register-tile kernels' (the batched path keeps those).
+
+// s_waitcnt immediates (gfx9 encoding: vmcnt in bits 3:0 and 15:14, expcnt
+// 6:4, lgkmcnt 11:8); a counter not named is not waited for.
+constexpr int WAIT_VMCNT0 = (7 << 4) | (15 << 8);
+constexpr int WAIT_LGKMCNT0 = 15 | (3 << 14) | (7 << 4);
+
+template <int PREC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_pairwise_shared(
+    PairArgs args) {
+  typedef PrecTraits<PREC> T;
+  typedef typename T::acc_t acc_t;
+  constexpr int PARTS = T::PARTS;
+  constexpr int WR = 2, WC = 2, NWC = 2;
+  constexpr int NF = 2 * 4 * 3 * PARTS;  // fragments per chunk and workgroup
+  constexpr int LPW = NF / 4;            // DMA instructions per wave and chunk
+  __shared__ u32x4 lds[NF][64];
+  // XCD-aware tile pick, as in k_pairwise.
+  const int w = blockIdx.x;
+  const int t = (w % args.nxcd) * args.per_xcd + (w / args.nxcd);
+  if (t >= args.ntiles) return;  // the whole workgroup, before any barrier
+  const int2 tile = args.tiles[t];
+  // Uniform wave index: the DMA and LDS addresses stay scalar.
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int wr = wave / NWC, wc = wave % NWC;
+  const int64_t fb0A = (int64_t)tile.x * 4, fb0B = (int64_t)tile.y * 4;
+  const int64_t fbA = fb0A + wr * WR, fbB = fb0B + wc * WC;
+  const u32x4* A = reinterpret_cast<const u32x4*>(args.A);
+  const u32x4* B = reinterpret_cast<const u32x4*>(args.B);
+  const int KB = args.KB;
+
+  // LDS fragment f = ((side * 4 + blk) * 3 + c) * PARTS + p; wave v fetches
+  // f = v * LPW ... v * LPW + LPW - 1.
+  auto fetch = [&](int kb) {
+#pragma unroll
+    for (int i = 0; i < LPW; ++i) {
+      const int f = wave * LPW + i;
+      const int p = f % PARTS, c = (f / PARTS) % 3, blk = (f / (3 * PARTS)) % 4;
+      const int side = f / (12 * PARTS);
+      const u32x4* src =
+          frag_ptr(side ? B : A, (side ? fb0B : fb0A) + blk, kb, KB, c, p, PARTS) + lane;
+      __builtin_amdgcn_global_load_lds((const void*)src,
+                                       (__attribute__((address_space(3))) void*)&lds[f][0], 16, 0,
+                                       0);
+    }
+  };
+
+  acc_t acc[WR][WC][3][3];
+#pragma unroll
+  for (int r = 0; r < WR; ++r)
+#pragma unroll
+    for (int q = 0; q < WC; ++q)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) acc[r][q][c][d] = acc_t{0, 0, 0, 0};
+
+  fetch(0);
+  for (int kb = 0; kb < KB; ++kb) {
+    // A bare s_barrier after explicit waits (a __syncthreads fence would also
+    // wait for what it need not); the asm's memory clobber keeps the compiler
+    // from moving LDS accesses across it.
+    __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);  // this wave's DMA of chunk kb landed
+    asm volatile("s_barrier" ::: "memory");   // ... and every wave's
+    u32x4 ca[WR][3][PARTS], cb[WC][3][PARTS];
+#pragma unroll
+    for (int r = 0; r < WR; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int p = 0; p < PARTS; ++p) ca[r][c][p] = lds[((wr * WR + r) * 3 + c) * PARTS + p][lane];
+#pragma unroll
+    for (int q = 0; q < WC; ++q)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int p = 0; p < PARTS; ++p)
+          cb[q][c][p] = lds[((4 + wc * WC + q) * 3 + c) * PARTS + p][lane];
+    __builtin_amdgcn_s_waitcnt(WAIT_LGKMCNT0);  // this wave's reads done
+    asm volatile("s_barrier" ::: "memory");     // ... and every wave's: stage free
+    if (kb + 1 < KB) fetch(kb + 1);
+#pragma unroll
+    for (int r = 0; r < WR; ++r)
+#pragma unroll
+      for (int q = 0; q < WC; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) mma<PREC>(acc[r][q][c][d], ca[r][c], cb[q][d]);
+  }
+  pair_epilogue<PREC, WR, WC>(args, acc, fbA, fbB, lane, t, wave, 4);
+}
+
 template <int PREC, int WR, int WC, int NWR, int NWC>
 __global__ __launch_bounds__(64 * NWR * NWC) void k_pairwise(PairArgs args) {
   // XCD-aware tile pick: the hardware deals workgroups round-robin over the
@@ -546,32 +654,42 @@ static hipError_t launch_tiles(const PairArgs& a, hipStream_t s) {
 }
 
 // Pair tile of the single-matrix kernel: 64 row frames x pairwise_tile_cols(prec)
-// column frames.  FP32 and split-BF16 waves own 32 x 48 pairs (fewer operand
-// bytes per MFMA than 32 x 32; measured 3 % (split-BF16) and 1 % (FP32) faster
-// on the 50k-frame matrix).
-int rmsd_pairwise_tile_cols(int prec) { return prec == PREC_F64 ? 64 : 96; }
+// column frames.  FP32 waves own 32 x 48 pairs of a register-tile kernel (fewer
+// operand bytes per MFMA than 32 x 32: 1 % faster on the 50k-frame matrix);
+// split-BF16 waves 32 x 32 pairs of k_pairwise_shared's 64 x 64 tiles.
+int rmsd_pairwise_tile_cols(int prec) { return prec == PREC_F32 ? 96 : 64; }
 
 // The <WR, WC, NWR, NWC> of the launches below.
 WaveLayout rmsd_pairwise_layout(int prec) {
-  return prec == PREC_F64 ? WaveLayout{1, 2, 4, 2} : WaveLayout{2, 3, 2, 2};
+  switch (prec) {
+    case PREC_F64: return WaveLayout{1, 2, 4, 2};
+    case PREC_F32: return WaveLayout{2, 3, 2, 2};
+    default: return WaveLayout{2, 2, 2, 2};
+  }
 }
 
 WaveLayout rmsd_batch_layout(int prec, int variant) {
-  if ((variant >> 1) == 0) return rmsd_pairwise_layout(prec);
+  if ((variant >> 1) == 0) return prec == PREC_F64 ? WaveLayout{1, 2, 4, 2} : WaveLayout{2, 3, 2, 2};
   return prec == PREC_F64 ? WaveLayout{1, 2, 2, 1} : WaveLayout{2, 2, 1, 1};
+}
+
+template <int PREC>
+static hipError_t launch_shared(const PairArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((k_pairwise_shared<PREC>), dim3(a.per_xcd * a.nxcd), dim3(256), 0, s, a);
+  return hipGetLastError();
 }
 
 hipError_t rmsd_launch_pairwise(int prec, const PairArgs& a, hipStream_t s) {
   switch (prec) {
     case PREC_F64: return launch_tiles<PREC_F64, 1, 2, 4, 2>(a, s);
     case PREC_F32: return launch_tiles<PREC_F32, 2, 3, 2, 2>(a, s);
-    case PREC_BF16X3: return launch_tiles<PREC_BF16X3, 2, 3, 2, 2>(a, s);
+    case PREC_BF16X3: return launch_shared<PREC_BF16X3>(a, s);
     default: return hipErrorInvalidValue;
   }
 }
 
-// Batched launches.  Variant v: tile shape v / 2 (shape 0: the single-matrix
-// tiles, 64 x 64 for FP64 and 64 x 96 for FP32 / BF16; shape 1: 32 x 32 pair
+// Batched launches.  Variant v: tile shape v / 2 (shape 0: the register-tile
+// kernels' 64 x 64 for FP64 and 64 x 96 for FP32 / BF16; shape 1: 32 x 32 pair
 // tiles of one or two waves), grid form for even v, persistent form for odd v.
 // The tile list must be built for the shape (rmsd_batch_tile_shape).
 template <int PREC, int WR, int WC, int NWR, int NWC>
@@ -636,7 +754,7 @@ int rmsd_batch_variant_count() { return 4; }
 void rmsd_batch_tile_shape(int prec, int variant, int* rows, int* cols) {
   if ((variant >> 1) == 0) {
     *rows = 64;
-    *cols = rmsd_pairwise_tile_cols(prec);
+    *cols = prec == PREC_F64 ? 64 : 96;
   } else {
     *rows = *cols = 32;
   }
