@@ -728,6 +728,8 @@ def main():
     flops_pair = 18.0 * n
     algo_tflops = npairs * flops_pair / (res[head]["contraction_ms"] / 1e3) / 1e12
     peak = {"bf16x3": PEAK_BF16_TFLOPS, "fp32": PEAK_FP32_TFLOPS, "fp64": PEAK_FP64_TFLOPS}[head]
+    # the contraction kernel of each precision (split-BF16: operands shared through LDS)
+    kname = f"k_pairwise_shared<{head}>" if head == "bf16x3" else f"k_pairwise<{head}>"
     terms = 3 if head == "bf16x3" else 1
     line = {
         "metric": METRIC,
@@ -752,9 +754,9 @@ def main():
         "kernels": res,
         "roofline": {"bound": "mfma", "achieved": algo_tflops, "peak": peak,
                      "unit": "TFLOP/s", "frac": algo_tflops / peak,
-                     "traffic": profiled_value("traffic", f"k_pairwise<{head}>", f"{F}x{n}"),
-                     "mfma_busy": profiled_value("mfma_busy", f"k_pairwise<{head}>", f"{F}x{n}"),
-                     "kernel": f"k_pairwise<{head}>",
+                     "traffic": profiled_value("traffic", kname, f"{F}x{n}"),
+                     "mfma_busy": profiled_value("mfma_busy", kname, f"{F}x{n}"),
+                     "kernel": kname,
                      "algorithmic_flops_per_launch": npairs * flops_pair,
                      "executed_mfma_tflops": algo_tflops * terms},
     }

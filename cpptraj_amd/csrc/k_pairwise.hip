@@ -496,21 +496,21 @@ __device__ __forceinline__ void pair_tile(const PairArgs& args, int2 tile, int t
 constexpr int WAIT_VMCNT0 = (7 << 4) | (15 << 8);
 constexpr int WAIT_LGKMCNT0 = 15 | (3 << 14) | (7 << 4);
 
+// Fragments of one chunk in the LDS stage of k_pairwise_shared.
+template <int PREC> constexpr int shared_frags() { return 2 * 4 * 3 * PrecTraits<PREC>::PARTS; }
+
+// One 64 x 64 tile (tile_id: its position in the tile list) by the calling
+// workgroup of 256 threads; lds: the workgroup's stage of shared_frags<PREC>()
+// fragments.  Every wave of the workgroup calls it for the same tile.
 template <int PREC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_pairwise_shared(
-    PairArgs args) {
+__device__ __forceinline__ void shared_tile(const PairArgs& args, int2 tile, int tile_id,
+                                            u32x4 (*lds)[64]) {
   typedef PrecTraits<PREC> T;
   typedef typename T::acc_t acc_t;
   constexpr int PARTS = T::PARTS;
   constexpr int WR = 2, WC = 2, NWC = 2;
-  constexpr int NF = 2 * 4 * 3 * PARTS;  // fragments per chunk and workgroup
-  constexpr int LPW = NF / 4;            // DMA instructions per wave and chunk
-  __shared__ u32x4 lds[NF][64];
-  // XCD-aware tile pick, as in k_pairwise.
-  const int w = blockIdx.x;
-  const int t = (w % args.nxcd) * args.per_xcd + (w / args.nxcd);
-  if (t >= args.ntiles) return;  // the whole workgroup, before any barrier
-  const int2 tile = args.tiles[t];
+  constexpr int NF = shared_frags<PREC>();  // fragments per chunk and workgroup
+  constexpr int LPW = NF / 4;               // DMA instructions per wave and chunk
   // Uniform wave index: the DMA and LDS addresses stay scalar.
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -580,7 +580,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
           for (int d = 0; d < 3; ++d) mma<PREC>(acc[r][q][c][d], ca[r][c], cb[q][d]);
   }
-  pair_epilogue<PREC, WR, WC>(args, acc, fbA, fbB, lane, t, wave, 4);
+  pair_epilogue<PREC, WR, WC>(args, acc, fbA, fbB, lane, tile_id, wave, 4);
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_pairwise_shared(
+    PairArgs args) {
+  __shared__ u32x4 lds[shared_frags<PREC>()][64];
+  // XCD-aware tile pick, as in k_pairwise.
+  const int w = blockIdx.x;
+  const int t = (w % args.nxcd) * args.per_xcd + (w / args.nxcd);
+  if (t >= args.ntiles) return;  // the whole workgroup, before any barrier
+  shared_tile<PREC>(args, args.tiles[t], t, lds);
 }
 
 template <int PREC, int WR, int WC, int NWR, int NWC>
@@ -634,6 +645,42 @@ __global__ __launch_bounds__(64 * NWR * NWC) void k_pairwise_batch_persist(Batch
   }
 }
 
+// The batched forms of k_pairwise_shared (grid and persistent, as above).
+template <int PREC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+k_pairwise_shared_batch(BatchArgs b) {
+  __shared__ u32x4 lds[shared_frags<PREC>()][64];
+  const int q = blockIdx.x % b.nq;
+  const int t = b.qbeg[q] + blockIdx.x / b.nq;
+  if (t >= b.qbeg[q + 1]) return;
+  const int4 tl = b.tiles[t];
+  const PairArgs args = b.sets[tl.x];
+  shared_tile<PREC>(args, make_int2(tl.y, tl.z), tl.w, lds);
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+k_pairwise_shared_batch_persist(BatchArgs b) {
+  __shared__ u32x4 lds[shared_frags<PREC>()][64];
+  __shared__ int slot;
+  const int q0 = blockIdx.x % b.nq;
+  for (int k = 0; k < b.nq; ++k) {
+    const int q = (q0 + k) % b.nq;
+    const int beg = b.qbeg[q], len = b.qbeg[q + 1] - beg;
+    for (;;) {
+      // also orders the previous tile's last LDS reads before the next DMA
+      __syncthreads();
+      if (threadIdx.x == 0) slot = atomicAdd(b.counters + q, 1);
+      __syncthreads();
+      const int pos = __builtin_amdgcn_readfirstlane(slot);
+      if (pos >= len) break;
+      const int4 tl = b.tiles[beg + pos];
+      const PairArgs args = b.sets[tl.x];
+      shared_tile<PREC>(args, make_int2(tl.y, tl.z), tl.w, lds);
+    }
+  }
+}
+
 __global__ void k_fill_int(int* p, int n, int v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -669,7 +716,7 @@ WaveLayout rmsd_pairwise_layout(int prec) {
 }
 
 WaveLayout rmsd_batch_layout(int prec, int variant) {
-  if ((variant >> 1) == 0) return prec == PREC_F64 ? WaveLayout{1, 2, 4, 2} : WaveLayout{2, 3, 2, 2};
+  if ((variant >> 1) == 0) return rmsd_pairwise_layout(prec);
   return prec == PREC_F64 ? WaveLayout{1, 2, 2, 1} : WaveLayout{2, 2, 1, 1};
 }
 
@@ -688,8 +735,9 @@ hipError_t rmsd_launch_pairwise(int prec, const PairArgs& a, hipStream_t s) {
   }
 }
 
-// Batched launches.  Variant v: tile shape v / 2 (shape 0: the register-tile
-// kernels' 64 x 64 for FP64 and 64 x 96 for FP32 / BF16; shape 1: 32 x 32 pair
+// Batched launches.  Variant v: tile shape v / 2 (shape 0: the single-matrix
+// kernels' tiles, 64 x 64 for FP64 and split-BF16 (k_pairwise_shared), 64 x 96
+// for FP32; shape 1: 32 x 32 pair
 // tiles of one or two waves), grid form for even v, persistent form for odd v.
 // The tile list must be built for the shape (rmsd_batch_tile_shape).
 template <int PREC, int WR, int WC, int NWR, int NWC>
@@ -754,7 +802,7 @@ int rmsd_batch_variant_count() { return 4; }
 void rmsd_batch_tile_shape(int prec, int variant, int* rows, int* cols) {
   if ((variant >> 1) == 0) {
     *rows = 64;
-    *cols = prec == PREC_F64 ? 64 : 96;
+    *cols = rmsd_pairwise_tile_cols(prec);
   } else {
     *rows = *cols = 32;
   }
@@ -762,6 +810,32 @@ void rmsd_batch_tile_shape(int prec, int variant, int* rows, int* cols) {
 
 hipError_t rmsd_launch_batch_reset(const BatchArgs& b, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, b.counters, b.nq, 0);
+  return hipGetLastError();
+}
+
+template <int PREC>
+static hipError_t launch_batch_shared(const BatchArgs& b, bool persist, hipStream_t s) {
+  if (!persist) {
+    const int64_t grid = (int64_t)b.nq * b.maxq;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_pairwise_shared_batch<PREC>), dim3((unsigned)grid), dim3(256), 0, s, b);
+    return hipGetLastError();
+  }
+  int dev = 0, ncu = 256, bpc = 1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+    return e;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &bpc, reinterpret_cast<const void*>(&k_pairwise_shared_batch_persist<PREC>), 256, 0);
+  if (e != hipSuccess) return e;
+  if (bpc < 1) bpc = 1;
+  int64_t grid = (int64_t)ncu * bpc;
+  const int64_t ntiles = (int64_t)b.nq * b.maxq;
+  if (grid > ntiles) grid = ntiles;
+  if (grid < b.nq) grid = b.nq;
+  hipLaunchKernelGGL((k_pairwise_shared_batch_persist<PREC>), dim3((unsigned)grid), dim3(256), 0, s,
+                     b);
   return hipGetLastError();
 }
 
@@ -777,7 +851,7 @@ hipError_t rmsd_launch_pairwise_batch(int prec, int variant, const BatchArgs& b,
       return shape == 0 ? launch_batch_shape<PREC_F32, 2, 3, 2, 2>(b, persist, s)
                         : launch_batch_shape<PREC_F32, 2, 2, 1, 1>(b, persist, s);
     case PREC_BF16X3:
-      return shape == 0 ? launch_batch_shape<PREC_BF16X3, 2, 3, 2, 2>(b, persist, s)
+      return shape == 0 ? launch_batch_shared<PREC_BF16X3>(b, persist, s)
                         : launch_batch_shape<PREC_BF16X3, 2, 2, 1, 1>(b, persist, s);
     default: return hipErrorInvalidValue;
   }
