@@ -111,28 +111,34 @@ __device__ __forceinline__ const u32x4* frag_ptr(const u32x4* base, int64_t fb, 
 // / nrmB for column frames).  (Grouping several pairs' Newton iterations, and
 // staging the tile's per-frame terms in LDS, were both measured slower on the
 // 50k-frame matrix.)
+// Error bound (MFMA-reduced forms with a flag array).  One eps bounds the
+// pairs of a lane's column j with its 8 rows (the largest row coefficient times
+// j's, plus the FP64 term at the largest E0), and keep_limits turns it into a
+// float interval of RMSDs whose stated bound is within tol: per pair two
+// compares.  Pairs outside it are recorded in the lane's mask.  The 8 rows
+// (4 (lane / 16) + v of two 16-frame blocks starting on a multiple of 32) are
+// the same in every tile shape with WR = 2, so which pairs are recomputed --
+// and every result -- does not depend on the shape.  Computed before the
+// contraction, so that the latency of its loads overlaps the first chunk's.
+template <int WC>
+struct BoundLimits {
+  bool on;
+  float klo[WC], khi[WC], eps[WC];
+};
+
 template <int PREC, int WR, int WC>
-__device__ __forceinline__ void pair_epilogue(const PairArgs& args,
-                                              typename PrecTraits<PREC>::acc_t (&acc)[WR][WC][3][3],
-                                              int64_t fbA, int64_t fbB, int lane, int tile_id,
-                                              int wave, int nwaves) {
+__device__ __forceinline__ BoundLimits<WC> bound_limits(const PairArgs& args, int64_t fbA,
+                                                        int64_t fbB, int lane) {
+  static_assert(PREC == PREC_F64 || WR == 2, "the bound's row groups assume WR = 2");
+  const double M = args.total_mass[0];
   const double* gA = args.gA;
   const double* gB = args.gB;
-  const double* uA = args.uA;
-  const double* uB = args.uB;
-  const double M = args.total_mass[0];
-  // Error bound (MFMA-reduced forms with a flag array).  One eps bounds the
-  // pairs of a lane's column j with its 8 rows (the largest row coefficient
-  // times j's, plus the FP64 term at the largest E0), and keep_limits turns it
-  // into a float interval of RMSDs whose stated bound is within tol: per pair
-  // two compares.  Pairs outside it are recorded in the lane's mask.  The 8
-  // rows (4 (lane / 16) + v of two 16-frame blocks starting on a multiple of
-  // 32) are the same in every tile shape with WR = 2, so which pairs are
-  // recomputed -- and every result -- does not depend on the shape.
-  static_assert(PREC == PREC_F64 || WR == 2, "the bound's row groups assume WR = 2");
-  const bool bounded = PREC != PREC_F64 && args.wlist != nullptr && M >= RMSD_SMALL;
-  float klo[WC], khi[WC], eps[WC];
-  if (bounded) {
+  BoundLimits<WC> L;
+  float* klo = L.klo;
+  float* khi = L.khi;
+  float* eps = L.eps;
+  L.on = PREC != PREC_F64 && args.wlist != nullptr && M >= RMSD_SMALL;
+  if (L.on) {
     double csr = 0.0, gr = 0.0;
 #pragma unroll
     for (int r = 0; r < WR; ++r)
@@ -156,6 +162,23 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
       rmsd_dev::keep_limits(eps[q], args.keep, &klo[q], &khi[q]);
     }
   }
+  return L;
+}
+
+template <int PREC, int WR, int WC>
+__device__ __forceinline__ void pair_epilogue(const PairArgs& args,
+                                              typename PrecTraits<PREC>::acc_t (&acc)[WR][WC][3][3],
+                                              int64_t fbA, int64_t fbB, int lane, int tile_id,
+                                              int wave, int nwaves, const BoundLimits<WC>& L) {
+  const double* gA = args.gA;
+  const double* gB = args.gB;
+  const double* uA = args.uA;
+  const double* uB = args.uB;
+  const double M = args.total_mass[0];
+  const bool bounded = L.on;
+  const float* klo = L.klo;
+  const float* khi = L.khi;
+  const float* eps = L.eps;
   unsigned int bits = 0;
   float kmin[WC], kmax[WC], omax[WC];
 #pragma unroll
@@ -447,6 +470,7 @@ __device__ __forceinline__ void pair_tile(const PairArgs& args, int2 tile, int t
   };
 
   load(0);
+  const BoundLimits<WC> bl = bound_limits<PREC, WR, WC>(args, fbA, fbB, lane);
   for (int kb = 0; kb < KB; ++kb) {
     u32x4 ca[WR][3][PARTS], cb[WC][3][PARTS];
 #pragma unroll
@@ -472,7 +496,7 @@ __device__ __forceinline__ void pair_tile(const PairArgs& args, int2 tile, int t
           for (int d = 0; d < 3; ++d) mma<PREC>(acc[r][q][c][d], ca[r][c], cb[q][d]);
   }
 
-  pair_epilogue<PREC, WR, WC>(args, acc, fbA, fbB, lane, tile_id, wave, NWR * NWC);
+  pair_epilogue<PREC, WR, WC>(args, acc, fbA, fbB, lane, tile_id, wave, NWR * NWC, bl);
 }
 
 // ---- Operands shared through LDS ---------------------------------------------------
@@ -548,6 +572,7 @@ __device__ __forceinline__ void shared_tile(const PairArgs& args, int2 tile, int
         for (int d = 0; d < 3; ++d) acc[r][q][c][d] = acc_t{0, 0, 0, 0};
 
   fetch(0);
+  const BoundLimits<WC> bl = bound_limits<PREC, WR, WC>(args, fbA, fbB, lane);
   for (int kb = 0; kb < KB; ++kb) {
     // A bare s_barrier after explicit waits (a __syncthreads fence would also
     // wait for what it need not); the asm's memory clobber keeps the compiler
@@ -580,7 +605,7 @@ __device__ __forceinline__ void shared_tile(const PairArgs& args, int2 tile, int
 #pragma unroll
           for (int d = 0; d < 3; ++d) mma<PREC>(acc[r][q][c][d], ca[r][c], cb[q][d]);
   }
-  pair_epilogue<PREC, WR, WC>(args, acc, fbA, fbB, lane, tile_id, wave, 4);
+  pair_epilogue<PREC, WR, WC>(args, acc, fbA, fbB, lane, tile_id, wave, 4, bl);
 }
 
 template <int PREC>
@@ -654,7 +679,8 @@ k_pairwise_shared_batch(BatchArgs b) {
   const int t = b.qbeg[q] + blockIdx.x / b.nq;
   if (t >= b.qbeg[q + 1]) return;
   const int4 tl = b.tiles[t];
-  const PairArgs args = b.sets[tl.x];
+  // by reference: the fields are read where they are used (scalar loads)
+  const PairArgs& args = b.sets[tl.x];
   shared_tile<PREC>(args, make_int2(tl.y, tl.z), tl.w, lds);
 }
 
@@ -675,7 +701,7 @@ k_pairwise_shared_batch_persist(BatchArgs b) {
       const int pos = __builtin_amdgcn_readfirstlane(slot);
       if (pos >= len) break;
       const int4 tl = b.tiles[beg + pos];
-      const PairArgs args = b.sets[tl.x];
+      const PairArgs& args = b.sets[tl.x];
       shared_tile<PREC>(args, make_int2(tl.y, tl.z), tl.w, lds);
     }
   }
