@@ -57,7 +57,7 @@ template <> struct PrecTraits<PREC_BF16X3> {
 
 // One 16-byte piece per lane (cached: row and column blocks are re-read by
 // neighbouring workgroups through L2).
-__device__ __forceinline__ u32x4 ld16(const u32x4* p) { return *p; }
+__device__ __forceinline__ u32x4 ld16(const u32x4* p) { return *gp(p); }
 
 template <int PREC>
 __device__ __forceinline__ void mma(typename PrecTraits<PREC>::acc_t& acc, const u32x4* a,
@@ -130,9 +130,9 @@ template <int PREC, int WR, int WC>
 __device__ __forceinline__ BoundLimits<WC> bound_limits(const PairArgs& args, int64_t fbA,
                                                         int64_t fbB, int lane) {
   static_assert(PREC == PREC_F64 || WR == 2, "the bound's row groups assume WR = 2");
-  const double M = args.total_mass[0];
-  const double* gA = args.gA;
-  const double* gB = args.gB;
+  const double M = gp(args.total_mass)[0];
+  const auto gA = gp(args.gA);
+  const auto gB = gp(args.gB);
   BoundLimits<WC> L;
   float* klo = L.klo;
   float* khi = L.khi;
@@ -146,7 +146,7 @@ __device__ __forceinline__ BoundLimits<WC> bound_limits(const PairArgs& args, in
       for (int v = 0; v < 4; ++v) {
         const int64_t i = (fbA + r) * 16 + 4 * (lane >> 4) + v;
         if (i < args.FA) {
-          csr = fmax(csr, args.nrmA[4 * i]);
+          csr = fmax(csr, gp(args.nrmA)[4 * i]);
           gr = fmax(gr, gA[i]);
         }
       }
@@ -155,7 +155,7 @@ __device__ __forceinline__ BoundLimits<WC> bound_limits(const PairArgs& args, in
       const int64_t j = (fbB + q) * 16 + (lane & 15);
       double csc = 0.0, gc = 0.0;
       if (j < args.FB) {
-        csc = args.nrmB[4 * j];
+        csc = gp(args.nrmB)[4 * j];
         gc = gB[j];
       }
       eps[q] = __double2float_ru(csr * csc + args.ke0 / M * 0.5 * (gr + gc));
@@ -170,11 +170,11 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
                                               typename PrecTraits<PREC>::acc_t (&acc)[WR][WC][3][3],
                                               int64_t fbA, int64_t fbB, int lane, int tile_id,
                                               int wave, int nwaves, const BoundLimits<WC>& L) {
-  const double* gA = args.gA;
-  const double* gB = args.gB;
-  const double* uA = args.uA;
-  const double* uB = args.uB;
-  const double M = args.total_mass[0];
+  const auto gA = gp(args.gA);
+  const auto gB = gp(args.gB);
+  const auto uA = gp(args.uA);
+  const auto uB = gp(args.uB);
+  const double M = gp(args.total_mass)[0];
   const bool bounded = L.on;
   const float* klo = L.klo;
   const float* khi = L.khi;
@@ -190,7 +190,7 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
   double Pm[9];
   if (PREC != PREC_F64) {
 #pragma unroll
-    for (int k = 0; k < 9; ++k) Pm[k] = args.P[k];
+    for (int k = 0; k < 9; ++k) Pm[k] = gp(args.P)[k];
   }
 #pragma unroll
   for (int q = 0; q < WC; ++q) {
@@ -220,7 +220,7 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
 #pragma unroll
           for (int d = 0; d < 3; ++d) S[3 * c + d] = acc_elem<PREC>(acc[r][q][c][d], v);
         if (PREC != PREC_F64) {
-          const double* ui = uA + 9 * i;
+          const auto ui = uA + 9 * i;
 #pragma unroll
           for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -243,7 +243,7 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
         const double E0 = 0.5 * (gA[i] + gj);
         const int64_t idx = args.tri ? rmsd_dev::tri_index(args.FA, i, j) : i * args.FB + j;
         const float res = M < RMSD_SMALL ? -1.0f : (float)rmsd_dev::qcp_rmsd(R, E0, M, args.clamp);
-        args.out[idx] = res;
+        gp(args.out)[idx] = res;
         if (bounded) {
           if (res >= klo[q] && res <= khi[q]) {  // NaN fails: recomputed in FP64
             kmin[q] = fminf(kmin[q], res);
@@ -260,8 +260,8 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
     // Flag and masks: one store per lane of a wave with a pair over tol.
     const int64_t w = (int64_t)tile_id * nwaves + wave;
     if (__ballot(bits != 0)) {
-      args.masks[w * 64 + lane] = bits;
-      if (lane == 0) args.wlist[atomicAdd(args.nflag, 1u)] = (int)w;
+      gp(args.masks)[w * 64 + lane] = bits;
+      if (lane == 0) gp(args.wlist)[atomicAdd(args.nflag, 1u)] = (int)w;
     }
     // The largest stated bound of the kept pairs (<= tol by construction).
     // Report only (tol = infinity): also the pairs below sqrt(eps)+.
@@ -307,7 +307,7 @@ __device__ __forceinline__ void fallback_fetch(const FallbackArgs& a, int64_t fi
     f.x[s][0] = f.x[s][1] = f.x[s][2] = f.y[s][0] = f.y[s][1] = f.y[s][2] = 0.0f;
     f.w[s] = 0.0;
     if (k + s < a.A.nsel) {
-      f.w[s] = a.smass ? a.smass[k + s] : 1.0;
+      f.w[s] = a.smass ? gp(a.smass)[k + s] : 1.0;
       if (iok) load_atom(a.A, fi, k + s, f.x[s]);
       if (jok) load_atom(a.B, fj, k + s, f.y[s]);
     }
@@ -335,8 +335,8 @@ __device__ __forceinline__ void fallback_block(const FallbackArgs& a, int2 tile,
   double ci[3] = {0.0, 0.0, 0.0}, cj[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    if (iok) ci[c] = a.cenA[4 * fi + c];
-    if (jok) cj[c] = a.cenB[4 * fj + c];
+    if (iok) ci[c] = gp(a.cenA)[4 * fi + c];
+    if (jok) cj[c] = gp(a.cenB)[4 * fj + c];
   }
   f64x4 acc[3][3];
 #pragma unroll
@@ -382,9 +382,9 @@ __device__ __forceinline__ void fallback_block(const FallbackArgs& a, int2 tile,
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int d = 0; d < 3; ++d) R[3 * c + d] = a.tri ? acc[d][c][v] : acc[c][d][v];
-    const double E0 = 0.5 * (a.gA[i] + a.gB[j]);
+    const double E0 = 0.5 * (gp(a.gA)[i] + gp(a.gB)[j]);
     const int64_t idx = a.tri ? rmsd_dev::tri_index(a.FA, i, j) : i * a.FB + j;
-    a.out[idx] = (float)rmsd_dev::qcp_rmsd(R, E0, M, a.clamp);
+    gp(a.out)[idx] = (float)rmsd_dev::qcp_rmsd(R, E0, M, a.clamp);
   }
   if (lane == 0) atomicAdd(a.stats + 1, nrec);
 }

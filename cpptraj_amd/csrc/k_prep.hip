@@ -47,7 +47,7 @@ __device__ __forceinline__ void centroid(const FrameSetDev& fs, int64_t f, const
                                          double* red, double cen[3]) {
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   for (int k = threadIdx.x; k < fs.nsel; k += blockDim.x) {
-    const double m = w ? w[k] : 1.0;
+    const double m = w ? gp(w)[k] : 1.0;
     float p[3];
     load_atom(fs, f, k, p);
     v[0] += m * (double)p[0];
@@ -66,7 +66,7 @@ __device__ __forceinline__ void centroid(const FrameSetDev& fs, int64_t f, const
 __device__ __forceinline__ void centred(const FrameSetDev& fs, int64_t f, int k, const double cen[3],
                                         const double* sw, double x[3]) {
   if (k < fs.nsel) {
-    const double s = sw ? sqrt(sw[k]) : 1.0;
+    const double s = sw ? sqrt(gp(sw)[k]) : 1.0;
     float p[3];
     load_atom(fs, f, k, p);
     x[0] = ((double)p[0] - cen[0]) * s;
@@ -236,15 +236,15 @@ __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
   if (f >= a.fs.nframes) {
     // Padding frames: zero operands so that padded rows and columns of a tile
     // contribute nothing.
-    if (a.nrm && threadIdx.x < 4) a.nrm[4 * f + threadIdx.x] = 0.0;
-    if (a.cen && threadIdx.x < 4) a.cen[4 * f + threadIdx.x] = 0.0;
+    if (a.nrm && threadIdx.x < 4) gp(a.nrm)[4 * f + threadIdx.x] = 0.0;
+    if (a.cen && threadIdx.x < 4) gp(a.cen)[4 * f + threadIdx.x] = 0.0;
     for (int k = threadIdx.x; k < Kp; k += blockDim.x)
       for (int c = 0; c < 3; ++c)
         for (int p = 0; p < chunk_parts(PREC); ++p) {
           const int64_t e = packed_elem<PREC>(f, k, c, p, a.KB);
-          if (PREC == PREC_F64) reinterpret_cast<double*>(a.packed)[e] = 0.0;
-          else if (PREC == PREC_F32) reinterpret_cast<float*>(a.packed)[e] = 0.0f;
-          else reinterpret_cast<__bf16*>(a.packed)[e] = (__bf16)0.0f;
+          if (PREC == PREC_F64) gp(reinterpret_cast<double*>(a.packed))[e] = 0.0;
+          else if (PREC == PREC_F32) gp(reinterpret_cast<float*>(a.packed))[e] = 0.0f;
+          else gp(reinterpret_cast<__bf16*>(a.packed))[e] = (__bf16)0.0f;
         }
     return;
   }
@@ -252,15 +252,15 @@ __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
   double cen[3];
   centroid(a.fs, f, a.cmass, red, cen);
   if (a.cen && threadIdx.x == 0) {
-    a.cen[4 * f] = cen[0];
-    a.cen[4 * f + 1] = cen[1];
-    a.cen[4 * f + 2] = cen[2];
-    a.cen[4 * f + 3] = 0.0;
+    gp(a.cen)[4 * f] = cen[0];
+    gp(a.cen)[4 * f + 1] = cen[1];
+    gp(a.cen)[4 * f + 2] = cen[2];
+    gp(a.cen)[4 * f + 3] = 0.0;
   }
 
   if (PREC == PREC_F64) {
     double g[1] = {0.0};
-    double* out = reinterpret_cast<double*>(a.packed);
+    const auto out = gp(reinterpret_cast<double*>(a.packed));
     for (int k = threadIdx.x; k < Kp; k += blockDim.x) {
       double x[3];
       centred(a.fs, f, k, cen, a.smass, x);
@@ -270,28 +270,33 @@ __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
       g[0] += x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
     }
     rmsd_dev::block_sum<1>(g, red);
-    if (threadIdx.x == 0) a.g[f] = g[0];
+    if (threadIdx.x == 0) gp(a.g)[f] = g[0];
     return;
   }
 
   // MFMA-reduced forms: align to the template in FP64.
   double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int k = threadIdx.x; k < a.fs.nsel; k += blockDim.x) {
-    double x[3];
-    centred(a.fs, f, k, cen, a.smass, x);
-    const double t0 = a.T[k], t1 = a.T[Kp + k], t2 = a.T[2 * Kp + k];
-    v[0] += x[0] * t0; v[1] += x[0] * t1; v[2] += x[0] * t2;
-    v[3] += x[1] * t0; v[4] += x[1] * t1; v[5] += x[1] * t2;
-    v[6] += x[2] * t0; v[7] += x[2] * t1; v[8] += x[2] * t2;
+  auto moments = [&](const double x[3], const double t[3]) {
+    v[0] += x[0] * t[0]; v[1] += x[0] * t[1]; v[2] += x[0] * t[2];
+    v[3] += x[1] * t[0]; v[4] += x[1] * t[1]; v[5] += x[1] * t[2];
+    v[6] += x[2] * t[0]; v[7] += x[2] * t[1]; v[8] += x[2] * t[2];
     v[9] += x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+  };
+  for (int k = threadIdx.x; k < a.fs.nsel; k += blockDim.x) {
+    double x[3], t[3];
+    centred(a.fs, f, k, cen, a.smass, x);
+    t[0] = gp(a.T)[k];
+    t[1] = gp(a.T)[Kp + k];
+    t[2] = gp(a.T)[2 * Kp + k];
+    moments(x, t);
   }
   rmsd_dev::block_sum<10>(v, red);
   // Every thread holds the block totals and solves the same FP64 QCP (same
   // inputs, same operations), so the rotation needs no broadcast or barrier.
-  if (threadIdx.x == 0) a.g[f] = v[9];
+  if (threadIdx.x == 0) gp(a.g)[f] = v[9];
   double U[9];
   {
-    const double E0 = 0.5 * (v[9] + a.gT[0]);
+    const double E0 = 0.5 * (v[9] + gp(a.gT)[0]);
     const rmsd_dev::QcpCoef co = rmsd_dev::qcp_coefficients(v);
     const double lam = rmsd_dev::qcp_lambda(co, E0);
     rmsd_dev::qcp_rotation(v, lam, U);
@@ -301,10 +306,7 @@ __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
   // sum_k D_c^4 per coordinate, and ||lo||^2 (the split's low parts; for FP32
   // the rounding residual D - float(D)).
   double u[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int k = threadIdx.x; k < Kp; k += blockDim.x) {
-    double x[3];
-    centred(a.fs, f, k, cen, a.smass, x);
-    const double t[3] = {a.T[k], a.T[Kp + k], a.T[2 * Kp + k]};
+  auto emit = [&](int k, const double x[3], const double t[3]) {
     double D[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -320,23 +322,31 @@ __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
       u[10 + c] += d2 * d2;
       if (PREC == PREC_F32) {
         const float df = (float)D[c];
-        reinterpret_cast<float*>(a.packed)[packed_elem<PREC>(f, k, c, 0, a.KB)] = df;
+        gp(reinterpret_cast<float*>(a.packed))[packed_elem<PREC>(f, k, c, 0, a.KB)] = df;
         const double r = D[c] - (double)df;
         u[13] += r * r;
       } else {
         const __bf16 hi = (__bf16)(float)D[c];
         const __bf16 lo = (__bf16)(float)(D[c] - (double)(float)hi);
-        __bf16* out = reinterpret_cast<__bf16*>(a.packed);
+        const auto out = gp(reinterpret_cast<__bf16*>(a.packed));
         out[packed_elem<PREC>(f, k, c, 0, a.KB)] = hi;
         out[packed_elem<PREC>(f, k, c, 1, a.KB)] = lo;
         const double l = (double)(float)lo;
         u[13] += l * l;
       }
     }
+  };
+  for (int k = threadIdx.x; k < Kp; k += blockDim.x) {
+    double x[3], t[3];
+    centred(a.fs, f, k, cen, a.smass, x);
+    t[0] = gp(a.T)[k];
+    t[1] = gp(a.T)[Kp + k];
+    t[2] = gp(a.T)[2 * Kp + k];
+    emit(k, x, t);
   }
   rmsd_dev::block_sum<14>(u, red);
   if (threadIdx.x == 0) {
-    for (int k = 0; k < 9; ++k) a.u[9 * f + k] = u[k];
+    for (int k = 0; k < 9; ++k) gp(a.u)[9 * f + k] = u[k];
     if (a.nrm) {
       // The error bound's per-frame coefficient (rmsd_dev::bound_keep).
       const double d2 = u[9];                                  // ||D||^2
@@ -348,11 +358,11 @@ __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
       const double sa = sqrt(k.acc);
       const double t = sa * sqrt(d2) + (k.mix / sa) * sqrt(l2);
       const double c2 = k.drop * l2 + k.rep * b2 + t * t;
-      const double M = a.total_mass ? a.total_mass[0] : 0.0;
-      a.nrm[4 * f] = M >= RMSD_SMALL ? sqrt(c2 * (2.0 * 1.7320508075688772 / M)) : 0.0;
-      a.nrm[4 * f + 1] = sqrt(d2);
-      a.nrm[4 * f + 2] = sqrt(b2);
-      a.nrm[4 * f + 3] = sqrt(l2);
+      const double M = a.total_mass ? gp(a.total_mass)[0] : 0.0;
+      gp(a.nrm)[4 * f] = M >= RMSD_SMALL ? sqrt(c2 * (2.0 * 1.7320508075688772 / M)) : 0.0;
+      gp(a.nrm)[4 * f + 1] = sqrt(d2);
+      gp(a.nrm)[4 * f + 2] = sqrt(b2);
+      gp(a.nrm)[4 * f + 3] = sqrt(l2);
     }
   }
 }
@@ -362,7 +372,7 @@ __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
 // 16-byte slots of the same 1 KiB pieces; over 8 L2s each 128-byte line went
 // out in partial writes from 8 places.  Here the 16 frames of a block run on
 // one XCD, whose L2 merges their slots before the line is written back
-// (k_prep_batch 7.8 -> 5.9 ms on the 3 x 20,000-frame batch).  The map is a
+// (k_prep_batch 7.8 -> 5.9 ms on the 64-trajectory batch).  The map is a
 // bijection of [0, prep_grid(n)) onto itself.
 constexpr int PREP_XCDS = 8;
 __device__ __forceinline__ int64_t prep_frame(int64_t b) {

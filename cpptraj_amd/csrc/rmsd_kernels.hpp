@@ -30,6 +30,16 @@ struct FrameSetDev {
   const int* fidx;  // frame f of the set is stored frame fidx[f] (null: f)
 };
 
+// p as a global-address-space pointer.  Kernels that read their arguments from a
+// device array (the batched ones) would otherwise access memory through flat
+// instructions (the compiler cannot tell where such a pointer points; flat
+// accesses also wait on the LDS counter and cannot use a scalar base); every
+// pointer of the argument sets addresses device memory.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gp(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
 __host__ __device__ inline int64_t frame_of(const FrameSetDev& fs, int64_t f) {
   return fs.fidx ? (int64_t)fs.fidx[f] : f;
 }
@@ -38,14 +48,15 @@ __host__ __device__ inline int64_t frame_of(const FrameSetDev& fs, int64_t f) {
 // 12-byte load: three 4-byte gathers would look its cache lines up three times.
 typedef float f32x3_a4 __attribute__((ext_vector_type(3), aligned(4)));
 __device__ __forceinline__ void load_atom(const FrameSetDev& fs, int64_t f, int k, float (&p)[3]) {
-  const int a = fs.sel ? fs.sel[k] : k;
-  const float* fr = fs.xyz + frame_of(fs, f) * fs.stride;
+  const int a = fs.sel ? gp(fs.sel)[k] : k;
+  const int64_t fo = fs.fidx ? (int64_t)gp(fs.fidx)[f] : f;
+  const auto fr = gp(fs.xyz + fo * fs.stride);
   if (fs.ld) {
     p[0] = fr[a];
     p[1] = fr[fs.ld + a];
     p[2] = fr[2 * fs.ld + a];
   } else {
-    const f32x3_a4 q = *reinterpret_cast<const f32x3_a4*>(fr + 3 * a);
+    const f32x3_a4 q = *gp(reinterpret_cast<const f32x3_a4*>(fs.xyz + fo * fs.stride + 3 * a));
     p[0] = q.x;
     p[1] = q.y;
     p[2] = q.z;
