@@ -492,7 +492,10 @@ __global__ void k_fit_solve(SolveArgs a, const double* shift) {
 
 // ---- pairs: frame f against reference frame f ---------------------------------
 // The 'reftraj' / 'previous' reference modes: every frame has its own
-// reference, so one workgroup per pair centres the reference (as k_ref_prep),
+// reference, so one wave per pair (sums reduced within the wave: 4.14 ms for
+// 100,000 pairs of 3,000 atoms against 4.57 ms with a 256-thread workgroup
+// per pair; four atoms' loads issued before use per lane measured the same)
+// centres the reference (as k_ref_prep),
 // then takes the reference's sums (w y, w |y|^2) and the target's shifted
 // moments (as the moment kernels) in one pass over both frames, and solves as
 // k_fit_solve.  No fit: RMSD_NoFit in the reference's operation order.
@@ -509,22 +512,24 @@ __global__ void k_mass_total(const double* mass, int nsel, double* W) {
 
 template <bool WEIGHTED>
 __global__ __launch_bounds__(FIT_THREADS) void k_fit_pairs(PairFitArgs a) {
-  __shared__ double red[17 * (FIT_THREADS / 64)];
-  const int64_t f = blockIdx.x;
+  // one pair per wave, reductions within the wave (no barriers)
+  const int lane = threadIdx.x & 63;
+  const int64_t f = (int64_t)blockIdx.x * (FIT_THREADS / 64) + (threadIdx.x >> 6);
+  if (f >= a.T.nframes) return;
   const int n = a.T.nsel;
   const double W = a.W[0];
   if (!a.fit) {
 #pragma clang fp contract(off)
     double v[1] = {0.0};
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    for (int k = lane; k < n; k += 64) {
       const double dx = (double)fcoord(a.R, f, k, 0) - (double)fcoord(a.T, f, k, 0);
       const double dy = (double)fcoord(a.R, f, k, 1) - (double)fcoord(a.T, f, k, 1);
       const double dz = (double)fcoord(a.R, f, k, 2) - (double)fcoord(a.T, f, k, 2);
       const double w = WEIGHTED ? a.tmass[k] : 1.0;
       v[0] += (w * (dx * dx + dy * dy + dz * dz));
     }
-    rmsd_dev::block_sum<1>(v, red);
-    if (threadIdx.x == 0) {
+    v[0] = rmsd_dev::wave_sum(v[0]);
+    if (lane == 0) {
       double r;
       if (W < RMSD_SMALL) r = -1.0;
       else if (v[0] < 0) r = 0.0;
@@ -535,14 +540,15 @@ __global__ __launch_bounds__(FIT_THREADS) void k_fit_pairs(PairFitArgs a) {
   }
   // reference centroid (centring masses)
   double c4[4] = {0, 0, 0, 0};
-  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+  for (int k = lane; k < n; k += 64) {
     const double m = a.rmass ? a.rmass[k] : 1.0;
     c4[0] += (double)fcoord(a.R, f, k, 0) * m;
     c4[1] += (double)fcoord(a.R, f, k, 1) * m;
     c4[2] += (double)fcoord(a.R, f, k, 2) * m;
     c4[3] += m;
   }
-  rmsd_dev::block_sum<4>(c4, red);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) c4[i] = rmsd_dev::wave_sum(c4[i]);
   double cen[3] = {0, 0, 0};
   const double tot = a.rmass ? c4[3] : (double)n;
   if (tot != 0.0) {
@@ -558,26 +564,35 @@ __global__ __launch_bounds__(FIT_THREADS) void k_fit_pairs(PairFitArgs a) {
   double v[17];
 #pragma unroll
   for (int i = 0; i < 17; ++i) v[i] = 0.0;
-  for (int k = threadIdx.x; k < n; k += blockDim.x) {
-    const double y0 = (double)fcoord(a.R, f, k, 0) - cen[0];
-    const double y1 = (double)fcoord(a.R, f, k, 1) - cen[1];
-    const double y2 = (double)fcoord(a.R, f, k, 2) - cen[2];
+  auto mom_add = [&](int k, const float (&r)[3], const float (&t)[3]) {
+    const double y0 = (double)r[0] - cen[0];
+    const double y1 = (double)r[1] - cen[1];
+    const double y2 = (double)r[2] - cen[2];
     const double w = WEIGHTED ? a.tmass[k] : 1.0;
     double m13[13];
 #pragma unroll
     for (int i = 0; i < 13; ++i) m13[i] = v[i];
-    accum<WEIGHTED>(m13, (double)fcoord(a.T, f, k, 0) - sh[0],
-                    (double)fcoord(a.T, f, k, 1) - sh[1], (double)fcoord(a.T, f, k, 2) - sh[2],
-                    w, y0, y1, y2);
+    accum<WEIGHTED>(m13, (double)t[0] - sh[0], (double)t[1] - sh[1], (double)t[2] - sh[2], w,
+                    y0, y1, y2);
 #pragma unroll
     for (int i = 0; i < 13; ++i) v[i] = m13[i];
     v[13] += w * y0;
     v[14] += w * y1;
     v[15] += w * y2;
     v[16] += w * (y0 * y0 + y1 * y1 + y2 * y2);
+  };
+  for (int k = lane; k < n; k += 64) {
+    float r[3], t[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      r[c] = fcoord(a.R, f, k, c);
+      t[c] = fcoord(a.T, f, k, c);
+    }
+    mom_add(k, r, t);
   }
-  rmsd_dev::block_sum<17>(v, red);
-  if (threadIdx.x != 0) return;
+#pragma unroll
+  for (int i = 0; i < 17; ++i) v[i] = rmsd_dev::wave_sum(v[i]);
+  if (lane != 0) return;
   double U[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
   double t1[3] = {0, 0, 0};
   double r;
@@ -839,10 +854,11 @@ hipError_t rmsd_launch_fit_pairs(const PairFitArgs& a, hipStream_t s) {
   if (a.T.nframes <= 0) return hipSuccess;
   if (a.R.nframes < a.T.nframes || a.R.nsel != a.T.nsel || a.T.nsel <= 0)
     return hipErrorInvalidValue;
+  const unsigned grid = (unsigned)((a.T.nframes + FIT_THREADS / 64 - 1) / (FIT_THREADS / 64));
   if (a.tmass)
-    hipLaunchKernelGGL(k_fit_pairs<true>, dim3(a.T.nframes), dim3(FIT_THREADS), 0, s, a);
+    hipLaunchKernelGGL(k_fit_pairs<true>, dim3(grid), dim3(FIT_THREADS), 0, s, a);
   else
-    hipLaunchKernelGGL(k_fit_pairs<false>, dim3(a.T.nframes), dim3(FIT_THREADS), 0, s, a);
+    hipLaunchKernelGGL(k_fit_pairs<false>, dim3(grid), dim3(FIT_THREADS), 0, s, a);
   return hipGetLastError();
 }
 
