@@ -75,6 +75,40 @@ def test_pairs_match_the_oracle_pair_by_pair(ctx, oracle, use_mass):
     assert np.abs(rn.cpu().numpy() - n_exp).max() <= 1e-9
 
 
+@pytest.mark.parametrize("layout", ["soa", "aos"])
+def test_pairs_four_atom_loads(ctx, oracle, layout):
+    """Sets whose layout allows 16-byte loads of four atoms (SoA rows and AoS
+    frames with a stride of a multiple of 4 floats, no selection), with a tail
+    of n % 4 atoms: the same results as the oracle pair by pair."""
+    import torch
+    import cpptraj_amd as ca
+    F, n = 37, 130  # 130 = 4 * 32 + 2; AoS stride 392 (padded to 4)
+    X = synth.noisy(71, F, n, 1.0).astype(np.float32)
+    Y = synth.noisy(72, F, n, 1.0).astype(np.float32)
+    m = synth.masses(np.random.default_rng(7), n)
+    sel = np.arange(n, dtype=np.int32)
+    for use_mass in (False, True):
+        exp = pair_oracle(oracle, X.reshape(F, -1), Y.reshape(F, -1), sel, m, use_mass)
+        if layout == "soa":
+            Xd = torch.as_tensor(synth.to_soa(X, 132)).cuda()
+            Yd = torch.as_tensor(synth.to_soa(Y, 132)).cuda()
+            kw = dict(soa=True, natom=n)
+        else:
+            Xp = np.zeros((F, 392), np.float32)
+            Yp = np.zeros((F, 392), np.float32)
+            Xp[:, :3 * n] = X.reshape(F, -1)
+            Yp[:, :3 * n] = Y.reshape(F, -1)
+            Xd = torch.as_tensor(Xp).cuda()
+            Yd = torch.as_tensor(Yp).cuda()
+            kw = dict(natom=n)
+        md = torch.as_tensor(m).cuda()
+        r, U, tv = ca.rmsd_fit_pairs(Xd, Yd, mass=md, use_mass=use_mass, save_matrices=True,
+                                     save_vectors=True, ctx=ctx, **kw)
+        assert np.abs(r.cpu().numpy() - exp[0]).max() <= 1e-9
+        assert np.abs(U.cpu().numpy() - exp[1]).max() <= 1e-9
+        assert np.abs(tv.cpu().numpy() - exp[2]).max() <= 1e-9
+
+
 def test_pairs_with_references_in_the_targets_storage(ctx):
     """'previous' on frames as stored: targets X[1:], references X[:-1], views
     of one tensor, with the coordinate update: every pair is solved before any
