@@ -36,47 +36,6 @@ __device__ __forceinline__ float fcoord(const FrameSetDev& fs, int64_t f, int k,
   return fs.ld ? fr[c * fs.ld + a] : fr[3 * a + c];
 }
 
-// Atoms 4q ... 4q + 3 of frame f as 16-byte loads (the pair kernel on 100,000
-// pairs of 3,000 SoA atoms: 1.54 ms, against 4.14 ms with one 4-byte load per
-// atom and coordinate and 4.57 ms with a 256-thread workgroup per pair): LAY 1
-// SoA (three rows,
-// ld and stride multiples of 4, 16-byte aligned), LAY 2 AoS (x0 y0 z0 x1 | y1 z1
-// x2 y2 | z2 x3 y3 z3, stride a multiple of 4, aligned); no selection.
-template <int LAY>
-__device__ __forceinline__ void fquad(const FrameSetDev& fs, int64_t f, int q, float (&x)[4],
-                                      float (&y)[4], float (&z)[4]) {
-  const float* fr = fs.xyz + frame_of(fs, f) * fs.stride;
-  if (LAY == 1) {
-    const f32x4 a = *reinterpret_cast<const f32x4*>(fr + 4 * q);
-    const f32x4 b = *reinterpret_cast<const f32x4*>(fr + fs.ld + 4 * q);
-    const f32x4 c = *reinterpret_cast<const f32x4*>(fr + 2 * fs.ld + 4 * q);
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      x[e] = a[e];
-      y[e] = b[e];
-      z[e] = c[e];
-    }
-  } else {
-    const f32x4* V = reinterpret_cast<const f32x4*>(fr + 12 * q);
-    const f32x4 p0 = V[0], p1 = V[1], p2 = V[2];
-    x[0] = p0.x; y[0] = p0.y; z[0] = p0.z;
-    x[1] = p0.w; y[1] = p1.x; z[1] = p1.y;
-    x[2] = p1.z; y[2] = p1.w; z[2] = p2.x;
-    x[3] = p2.y; y[3] = p2.z; z[3] = p2.w;
-  }
-}
-
-// The quad layout both frame sets of a pair kernel share (0: none).
-inline int pair_quad_layout(const FrameSetDev& a, const FrameSetDev& b) {
-  auto lay = [](const FrameSetDev& fs) {
-    if (fs.sel || ((uintptr_t)fs.xyz % 16) != 0 || (fs.stride % 4) != 0) return 0;
-    if (fs.ld > 0) return (fs.ld % 4) == 0 ? 1 : 0;
-    return 2;
-  };
-  const int la = lay(a);
-  return la == lay(b) ? la : 0;
-}
-
 // ---- reference ---------------------------------------------------------------
 // ref_out: SoA [3][ldr] FP64 (ldr = nsel rounded up to 4, zero padded).
 // consts: [0..2] sum w y, [3] sum w |y|^2, [4] W = sum w (sequential sum, so
@@ -590,7 +549,7 @@ __global__ __launch_bounds__(FIT_THREADS) void k_fit_pairs(PairFitArgs a) {
   const int n4 = LAY ? n >> 2 : 0;
   for (int q = lane; q < n4; q += 64) {
     float x[4], y[4], z[4];
-    fquad<LAY>(a.R, f, q, x, y, z);
+    load_quad(a.R, f, q, LAY, x, y, z);
 #pragma unroll
     for (int e = 0; e < 4; ++e) cen_add(4 * q + e, x[e], y[e], z[e]);
   }
@@ -632,8 +591,8 @@ __global__ __launch_bounds__(FIT_THREADS) void k_fit_pairs(PairFitArgs a) {
   };
   for (int q = lane; q < n4; q += 64) {
     float rx[4], ry[4], rz[4], tx[4], ty[4], tz[4];
-    fquad<LAY>(a.R, f, q, rx, ry, rz);
-    fquad<LAY>(a.T, f, q, tx, ty, tz);
+    load_quad(a.R, f, q, LAY, rx, ry, rz);
+    load_quad(a.T, f, q, LAY, tx, ty, tz);
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const float r[3] = {rx[e], ry[e], rz[e]}, t[3] = {tx[e], ty[e], tz[e]};
@@ -914,7 +873,7 @@ hipError_t rmsd_launch_fit_pairs(const PairFitArgs& a, hipStream_t s) {
   if (a.R.nframes < a.T.nframes || a.R.nsel != a.T.nsel || a.T.nsel <= 0)
     return hipErrorInvalidValue;
   const unsigned grid = (unsigned)((a.T.nframes + FIT_THREADS / 64 - 1) / (FIT_THREADS / 64));
-  const int lay = a.fit ? pair_quad_layout(a.T, a.R) : 0;
+  const int lay = a.fit && quad_layout(a.T) == quad_layout(a.R) ? quad_layout(a.T) : 0;
 #define RMSD_FIT_PAIRS(W, L) \
   hipLaunchKernelGGL((k_fit_pairs<W, L>), dim3(grid), dim3(FIT_THREADS), 0, s, a)
   if (a.tmass) {

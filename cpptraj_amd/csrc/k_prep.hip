@@ -77,6 +77,69 @@ __device__ __forceinline__ void centred(const FrameSetDev& fs, int64_t f, int k,
   }
 }
 
+// Atoms 4q ... 4q + 3 of frame f: one quad of 16-byte loads when the layout
+// allows it (quad_layout), else atom by atom; zeros beyond the selection.
+__device__ __forceinline__ void atoms_quad(const FrameSetDev& fs, int64_t f, int q, int lay,
+                                           float (&x)[4], float (&y)[4], float (&z)[4]) {
+  if (lay && 4 * q + 3 < fs.nsel) {
+    load_quad(fs, f, q, lay, x, y, z);
+    return;
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    float p[3] = {0.0f, 0.0f, 0.0f};
+    if (4 * q + e < fs.nsel) load_atom(fs, f, 4 * q + e, p);
+    x[e] = p[0];
+    y[e] = p[1];
+    z[e] = p[2];
+  }
+}
+
+// The template's atoms 4q ... 4q + 3 (rows of T, Kp a multiple of 4 and the
+// rows 32-byte aligned): two 16-byte loads per row.
+__device__ __forceinline__ void template_quad(const double* T, int Kp, int q, double (&t)[4][3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    typedef __attribute__((ext_vector_type(2))) double f64x2;
+    const auto row = gp(reinterpret_cast<const f64x2*>(T + (int64_t)c * Kp + 4 * q));
+    const f64x2 a = row[0], b = row[1];
+    t[0][c] = a.x;
+    t[1][c] = a.y;
+    t[2][c] = b.x;
+    t[3][c] = b.y;
+  }
+}
+
+// Weighted centroid of frame f over quads of atoms (the MFMA forms: each lane
+// adds atoms 4q ... 4q + 3 in order).
+__device__ __forceinline__ void centroid_quads(const FrameSetDev& fs, int64_t f, const double* w,
+                                               double* red, double cen[3]) {
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  const int lay = quad_layout(fs);
+  const int nq = (fs.nsel + 3) >> 2;
+  for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+    float x[4], y[4], z[4];
+    atoms_quad(fs, f, q, lay, x, y, z);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int k = 4 * q + e;
+      if (k < fs.nsel) {
+        const double m = w ? gp(w)[k] : 1.0;
+        v[0] += m * (double)x[e];
+        v[1] += m * (double)y[e];
+        v[2] += m * (double)z[e];
+        v[3] += m;
+      }
+    }
+  }
+  rmsd_dev::block_sum<4>(v, red);
+  const double tot = w ? v[3] : (double)fs.nsel;
+  const double inv = tot != 0.0 ? 1.0 / tot : 0.0;
+  cen[0] = v[0] * inv;
+  cen[1] = v[1] * inv;
+  cen[2] = v[2] * inv;
+}
+
 // ---- template / total weight -----------------------------------------------
 __device__ __forceinline__ void setup_body(const SetupArgs& a) {
   __shared__ double red[16 * (PREP_THREADS / 64)];
@@ -249,8 +312,14 @@ __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
     return;
   }
 
+  // MFMA forms on frames that allow 16-byte loads of four atoms: each lane
+  // takes quads of atoms (four per step; the 50,000 x 1,200 AoS stage: 1.05 ->
+  // 0.86 ms).  Otherwise lanes take consecutive atoms (coalesced 4-byte loads;
+  // quads of such atoms would spread each load over four times the lines).
+  const int lay = PREC == PREC_F64 ? 0 : quad_layout(a.fs);
   double cen[3];
-  centroid(a.fs, f, a.cmass, red, cen);
+  if (lay) centroid_quads(a.fs, f, a.cmass, red, cen);
+  else centroid(a.fs, f, a.cmass, red, cen);
   if (a.cen && threadIdx.x == 0) {
     gp(a.cen)[4 * f] = cen[0];
     gp(a.cen)[4 * f + 1] = cen[1];
@@ -282,13 +351,42 @@ __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
     v[6] += x[2] * t[0]; v[7] += x[2] * t[1]; v[8] += x[2] * t[2];
     v[9] += x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
   };
-  for (int k = threadIdx.x; k < a.fs.nsel; k += blockDim.x) {
-    double x[3], t[3];
-    centred(a.fs, f, k, cen, a.smass, x);
-    t[0] = gp(a.T)[k];
-    t[1] = gp(a.T)[Kp + k];
-    t[2] = gp(a.T)[2 * Kp + k];
-    moments(x, t);
+  const int nq = (a.fs.nsel + 3) >> 2;
+  // centred, weighted coordinates of atoms 4q ... 4q + 3 (zeros past nsel)
+  auto centred_quad = [&](int q, double (&x)[4][3]) {
+    float px[4], py[4], pz[4];
+    atoms_quad(a.fs, f, q, lay, px, py, pz);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int k = 4 * q + e;
+      if (k < a.fs.nsel) {
+        const double s = a.smass ? sqrt(gp(a.smass)[k]) : 1.0;
+        x[e][0] = ((double)px[e] - cen[0]) * s;
+        x[e][1] = ((double)py[e] - cen[1]) * s;
+        x[e][2] = ((double)pz[e] - cen[2]) * s;
+      } else {
+        x[e][0] = x[e][1] = x[e][2] = 0.0;
+      }
+    }
+  };
+  if (lay) {
+    for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+      double x[4][3], t[4][3];
+      centred_quad(q, x);
+      template_quad(a.T, Kp, q, t);
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (4 * q + e < a.fs.nsel) moments(x[e], t[e]);
+    }
+  } else {
+    for (int k = threadIdx.x; k < a.fs.nsel; k += blockDim.x) {
+      double x[3], t[3];
+      centred(a.fs, f, k, cen, a.smass, x);
+      t[0] = gp(a.T)[k];
+      t[1] = gp(a.T)[Kp + k];
+      t[2] = gp(a.T)[2 * Kp + k];
+      moments(x, t);
+    }
   }
   rmsd_dev::block_sum<10>(v, red);
   // Every thread holds the block totals and solves the same FP64 QCP (same
@@ -306,7 +404,8 @@ __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
   // sum_k D_c^4 per coordinate, and ||lo||^2 (the split's low parts; for FP32
   // the rounding residual D - float(D)).
   double u[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  auto emit = [&](int k, const double x[3], const double t[3]) {
+  // one atom: its D (accumulated into u) rounded to the stored parts
+  auto emit = [&](const double x[3], const double t[3], float (&hi)[3], float (&lo)[3]) {
     double D[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -322,27 +421,67 @@ __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
       u[10 + c] += d2 * d2;
       if (PREC == PREC_F32) {
         const float df = (float)D[c];
-        gp(reinterpret_cast<float*>(a.packed))[packed_elem<PREC>(f, k, c, 0, a.KB)] = df;
+        hi[c] = df;
         const double r = D[c] - (double)df;
         u[13] += r * r;
       } else {
-        const __bf16 hi = (__bf16)(float)D[c];
-        const __bf16 lo = (__bf16)(float)(D[c] - (double)(float)hi);
-        const auto out = gp(reinterpret_cast<__bf16*>(a.packed));
-        out[packed_elem<PREC>(f, k, c, 0, a.KB)] = hi;
-        out[packed_elem<PREC>(f, k, c, 1, a.KB)] = lo;
-        const double l = (double)(float)lo;
-        u[13] += l * l;
+        const __bf16 h = (__bf16)(float)D[c];
+        const __bf16 l = (__bf16)(float)(D[c] - (double)(float)h);
+        hi[c] = (float)h;
+        lo[c] = (float)l;
+        const double ld = (double)(float)l;
+        u[13] += ld * ld;
       }
     }
   };
-  for (int k = threadIdx.x; k < Kp; k += blockDim.x) {
+  // Kp is a multiple of the chunk, so a quad never straddles two chunks and its
+  // four values of one coordinate and part are consecutive in a 16-byte slot:
+  // one 8-byte (split-BF16) or 16-byte (FP32) store each.
+  for (int q = threadIdx.x; lay && q < (Kp >> 2); q += blockDim.x) {
+    double x[4][3], t[4][3];
+    centred_quad(q, x);
+    template_quad(a.T, Kp, q, t);
+    float hi[4][3], lo[4][3];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) emit(x[e], t[e], hi[e], lo[e]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (PREC == PREC_F32) {
+        f32x4_t v4 = {hi[0][c], hi[1][c], hi[2][c], hi[3][c]};
+        *gp(reinterpret_cast<f32x4_t*>(reinterpret_cast<float*>(a.packed) +
+                                       packed_elem<PREC>(f, 4 * q, c, 0, a.KB))) = v4;
+      } else {
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+          const float (&src)[4][3] = p ? lo : hi;
+          // bfloat16 bits: the top half of each float (exact: the values are bfloat16)
+          typedef __attribute__((ext_vector_type(2))) unsigned int u32x2;
+          const u32x2 w2 = {(__float_as_uint(src[0][c]) >> 16) | (__float_as_uint(src[1][c]) & 0xffff0000u),
+                            (__float_as_uint(src[2][c]) >> 16) | (__float_as_uint(src[3][c]) & 0xffff0000u)};
+          *gp(reinterpret_cast<u32x2*>(reinterpret_cast<__bf16*>(a.packed) +
+                                       packed_elem<PREC>(f, 4 * q, c, p, a.KB))) = w2;
+        }
+      }
+    }
+  }
+  for (int k = threadIdx.x; !lay && k < Kp; k += blockDim.x) {
     double x[3], t[3];
     centred(a.fs, f, k, cen, a.smass, x);
     t[0] = gp(a.T)[k];
     t[1] = gp(a.T)[Kp + k];
     t[2] = gp(a.T)[2 * Kp + k];
-    emit(k, x, t);
+    float hi[3], lo[3];
+    emit(x, t, hi, lo);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (PREC == PREC_F32) {
+        gp(reinterpret_cast<float*>(a.packed))[packed_elem<PREC>(f, k, c, 0, a.KB)] = hi[c];
+      } else {
+        const auto out = gp(reinterpret_cast<__bf16*>(a.packed));
+        out[packed_elem<PREC>(f, k, c, 0, a.KB)] = (__bf16)hi[c];
+        out[packed_elem<PREC>(f, k, c, 1, a.KB)] = (__bf16)lo[c];
+      }
+    }
   }
   rmsd_dev::block_sum<14>(u, red);
   if (threadIdx.x == 0) {
@@ -385,7 +524,7 @@ int64_t prep_grid(int64_t n) {
 }
 
 template <int PREC>
-__global__ __launch_bounds__(PREP_THREADS) void k_prep(PrepArgs a) {
+__global__ __launch_bounds__(PREP_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_prep(PrepArgs a) {
   const int64_t f = prep_frame(blockIdx.x);
   if (f < a.Fp) prep_body<PREC>(a, f);
 }
@@ -393,7 +532,7 @@ __global__ __launch_bounds__(PREP_THREADS) void k_prep(PrepArgs a) {
 // Every padded frame of every set of a batch in one launch: workgroup b is
 // frame b - fbase[s] of the set s with fbase[s] <= b < fbase[s + 1].
 template <int PREC>
-__global__ __launch_bounds__(PREP_THREADS) void k_prep_batch(const PrepArgs* sets,
+__global__ __launch_bounds__(PREP_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_prep_batch(const PrepArgs* sets,
                                                              const int* fbase, int nsets) {
   // every set's padded frame count is a multiple of 16, so no frame block
   // straddles two sets
