@@ -61,16 +61,17 @@ struct rmsd_batch {
 
 namespace {
 
-// Relative cost of one tile: its atom chunks plus the epilogue (QCP over the
-// tile's pairs) in units of one chunk, measured on the 50k x 1200 matrix.
-double tile_cost(int prec, int KB) {
-  const double epi = prec == PREC_BF16X3 ? 6.0 : 4.0;
-  return KB + epi;
-}
+constexpr size_t QUEUE_BLOCK = 64;  // tiles per block dealt to a queue
 
-// Tile queues for tile side S: sets sorted by cost per tile (descending, ties
-// by set index), each set's triangle tiles in grouped order, the concatenated
-// list cut into nq contiguous runs of equal total cost.
+// Tile queues for tile side S: sets sorted by atom chunks per tile (the cost
+// of a tile; descending, ties by set index), each set's triangle tiles in
+// grouped order, and the concatenated list dealt to the nq queues (one per
+// XCD) in blocks of QUEUE_BLOCK consecutive tiles, round-robin.  Every queue
+// then holds the same mix of heavy and light tiles, with no model of their
+// cost, while the tiles a queue's concurrent workgroups run stay neighbours in
+// one set (sharing row and column blocks in that XCD's L2).  Contiguous runs of
+// equal modelled cost, one per queue, measured 99.3 / 84.3 ms (grid /
+// persistent form) against 77.7 / 82.2 ms on the 64-trajectory batch.
 void build_queues(const rmsd_batch& b, int tr, int tc, std::vector<int4>* tiles,
                   std::vector<int>* qbeg, int* maxq) {
   const int ns = (int)b.sets.size();
@@ -80,29 +81,24 @@ void build_queues(const rmsd_batch& b, int tr, int tc, std::vector<int4>* tiles,
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
     return b.Kp[x] / CH > b.Kp[y] / CH;
   });
-  std::vector<double> cost;
   tiles->clear();
   for (int s : order) {
     const int F = b.sets[s].nframes;
     if (F < 2) continue;
     const int nbr = (F + tr - 1) / tr, nbc = (F + tc - 1) / tc;
-    const double c = tile_cost(b.prec, b.Kp[s] / CH);
-    for (int2 t : grouped_tiles(nbr, nbc, 1, tr, tc)) {
-      tiles->push_back(make_int4(s, t.x, t.y, 0));
-      cost.push_back(c);
-    }
+    for (int2 t : grouped_tiles(nbr, nbc, 1, tr, tc)) tiles->push_back(make_int4(s, t.x, t.y, 0));
   }
-  double total = 0.0;
-  for (double c : cost) total += c;
+  std::vector<int4> dealt;
+  dealt.reserve(tiles->size());
   qbeg->assign(b.nq + 1, 0);
-  double run = 0.0;
-  int q = 1;
-  for (size_t t = 0; t < cost.size() && q < b.nq; ++t) {
-    run += cost[t];
-    while (q < b.nq && run >= total * q / b.nq) (*qbeg)[q++] = (int)t + 1;
+  for (int q = 0; q < b.nq; ++q) {
+    (*qbeg)[q] = (int)dealt.size();
+    for (size_t b0 = (size_t)q * QUEUE_BLOCK; b0 < tiles->size(); b0 += (size_t)b.nq * QUEUE_BLOCK)
+      for (size_t t = b0; t < std::min(tiles->size(), b0 + QUEUE_BLOCK); ++t)
+        dealt.push_back((*tiles)[t]);
   }
-  while (q <= b.nq) (*qbeg)[q++] = (int)cost.size();
-  (*qbeg)[b.nq] = (int)cost.size();
+  (*qbeg)[b.nq] = (int)dealt.size();
+  *tiles = dealt;
   // w: the tile's position in the list (its flag of the error bound).
   for (size_t t = 0; t < tiles->size(); ++t) (*tiles)[t].w = (int)t;
   *maxq = 0;
