@@ -187,21 +187,18 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
     kmax[q] = 0.0f;
     omax[q] = -1.0f;  // >= 0 once a pair of the column is outside [klo, khi]
   }
-  double Pm[9];
-  if (PREC != PREC_F64) {
-#pragma unroll
-    for (int k = 0; k < 9; ++k) Pm[k] = gp(args.P)[k];
-  }
+  const double invM = 1.0 / M;
 #pragma unroll
   for (int q = 0; q < WC; ++q) {
     const int64_t j = (fbB + q) * 16 + (lane & 15);  // column frame
+    // uj: the column's template terms plus the constant P, added once per column
     double gj = 0.0, uj[9];
     const bool jok = j < args.FB;
     if (jok) {
       gj = gB[j];
       if (PREC != PREC_F64) {
 #pragma unroll
-        for (int k = 0; k < 9; ++k) uj[k] = uB[9 * j + k];
+        for (int k = 0; k < 9; ++k) uj[k] = gp(args.P)[k] + uB[9 * j + k];
       }
     }
 #pragma unroll
@@ -225,7 +222,7 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
           for (int c = 0; c < 3; ++c)
 #pragma unroll
             for (int d = 0; d < 3; ++d)
-              S[3 * c + d] += Pm[3 * c + d] + uj[3 * c + d] + ui[3 * d + c];
+              S[3 * c + d] += uj[3 * c + d] + ui[3 * d + c];
         }
         // Row frames are the reference in the triangle layout (the reference
         // stores element (nref, ntgt) with ntgt > nref) and the target in the
@@ -242,7 +239,10 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& args,
         }
         const double E0 = 0.5 * (gA[i] + gj);
         const int64_t idx = args.tri ? rmsd_dev::tri_index(args.FA, i, j) : i * args.FB + j;
-        const float res = M < RMSD_SMALL ? -1.0f : (float)rmsd_dev::qcp_rmsd(R, E0, M, args.clamp);
+        const float res = M < RMSD_SMALL ? -1.0f
+                          : PREC == PREC_F64
+                              ? (float)rmsd_dev::qcp_rmsd(R, E0, M, args.clamp)
+                              : (float)rmsd_dev::qcp_rmsd_fast(R, E0, invM, args.clamp);
         gp(args.out)[idx] = res;
         if (bounded) {
           if (res >= klo[q] && res <= khi[q]) {  // NaN fails: recomputed in FP64
@@ -300,8 +300,30 @@ struct FallbackQuad {
   double w[4];
 };
 
+// LAY != 0: both frame sets allow 16-byte loads of four atoms (quad_layout),
+// used wherever the four atoms lie within the selection (k is a multiple of 4).
+template <int LAY>
 __device__ __forceinline__ void fallback_fetch(const FallbackArgs& a, int64_t fi, bool iok,
                                                int64_t fj, bool jok, int k, FallbackQuad& f) {
+  if (LAY != 0 && k + 3 < a.A.nsel) {
+    float x[3][4], y[3][4];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) x[c][s] = y[c][s] = 0.0f;
+    if (iok) load_quad(a.A, fi, k >> 2, LAY, x[0], x[1], x[2]);
+    if (jok) load_quad(a.B, fj, k >> 2, LAY, y[0], y[1], y[2]);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      f.w[s] = a.smass ? gp(a.smass)[k + s] : 1.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        f.x[s][c] = x[c][s];
+        f.y[s][c] = y[c][s];
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     f.x[s][0] = f.x[s][1] = f.x[s][2] = f.y[s][0] = f.y[s][1] = f.y[s][2] = 0.0f;
@@ -314,6 +336,7 @@ __device__ __forceinline__ void fallback_fetch(const FallbackArgs& a, int64_t fi
   }
 }
 
+template <int LAY>
 __device__ __forceinline__ void fallback_block(const FallbackArgs& a, int2 tile, int wave,
                                                int blk, WaveLayout L, int lane,
                                                const unsigned int* wmask) {
@@ -350,10 +373,10 @@ __device__ __forceinline__ void fallback_block(const FallbackArgs& a, int2 tile,
   const int nsel = a.A.nsel;
   const int kq = 4 * (lane >> 4);
   FallbackQuad nxt;
-  fallback_fetch(a, fi, iok, fj, jok, kq, nxt);
+  fallback_fetch<LAY>(a, fi, iok, fj, jok, kq, nxt);
   for (int k0 = 0; k0 < nsel; k0 += 16) {
     const FallbackQuad cur = nxt;
-    fallback_fetch(a, fi, iok, fj, jok, k0 + 16 + kq, nxt);
+    fallback_fetch<LAY>(a, fi, iok, fj, jok, k0 + 16 + kq, nxt);
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       double av[3], bv[3];
@@ -389,6 +412,20 @@ __device__ __forceinline__ void fallback_block(const FallbackArgs& a, int2 tile,
   if (lane == 0) atomicAdd(a.stats + 1, nrec);
 }
 
+// fallback_block with four-atom loads where both frame sets allow them (the
+// layout is uniform: a property of the sets).
+__device__ __forceinline__ void fallback_any(const FallbackArgs& a, int2 tile, int wave, int blk,
+                                             WaveLayout L, int lane, const unsigned int* wmask) {
+  const int la = quad_layout(a.A);
+  const int lay = la == quad_layout(a.B) ? la : 0;
+  if (lay == 1)
+    fallback_block<1>(a, tile, wave, blk, L, lane, wmask);
+  else if (lay == 2)
+    fallback_block<2>(a, tile, wave, blk, L, lane, wmask);
+  else
+    fallback_block<0>(a, tile, wave, blk, L, lane, wmask);
+}
+
 // Next 16 x 16 block for this wave (the same for every lane).
 __device__ __forceinline__ unsigned int fallback_next(unsigned int* nflag, int lane) {
   unsigned int b = 0;
@@ -403,8 +440,7 @@ __global__ __launch_bounds__(FALLBACK_THREADS, 2) void k_pair_fallback(FallbackA
   const unsigned int total = nflag[0] * (unsigned int)nb;
   for (unsigned int b = fallback_next(nflag, lane); b < total; b = fallback_next(nflag, lane)) {
     const int w = a.wlist[b / nb];
-    fallback_block(a, a.tiles[w / nw], w % nw, (int)(b % nb), L, lane,
-                   a.masks + (int64_t)w * 64);
+    fallback_any(a, a.tiles[w / nw], w % nw, (int)(b % nb), L, lane, a.masks + (int64_t)w * 64);
   }
 }
 
@@ -417,8 +453,8 @@ __global__ __launch_bounds__(FALLBACK_THREADS, 2) void k_pair_fallback_batch(
   for (unsigned int b = fallback_next(nflag, lane); b < total; b = fallback_next(nflag, lane)) {
     const int w = wlist[b / nb];
     const int4 tl = tiles[w / nw];
-    fallback_block(fsets[tl.x], make_int2(tl.y, tl.z), w % nw, (int)(b % nb), L, lane,
-                   masks + (int64_t)w * 64);
+    fallback_any(fsets[tl.x], make_int2(tl.y, tl.z), w % nw, (int)(b % nb), L, lane,
+                 masks + (int64_t)w * 64);
   }
 }
 

@@ -110,6 +110,41 @@ __device__ __forceinline__ double qcp_rmsd(const double* S, double E0, double M,
   return rmsd_from_lambda(E0, qcp_lambda(k, E0), M, clamp);
 }
 
+// The same solve with fewer FP64 instructions, for the MFMA-reduced pairwise
+// forms, whose results carry a stated error bound against FP64 anyway: each
+// Newton quotient is the numerator times the hardware reciprocal of the
+// denominator after one Newton-Raphson refinement (relative error a few units
+// of 2^-53, against the IEEE division's half unit; Newton's iteration corrects
+// an inexact step, and the stopping rule is unchanged), and 1 / M is taken
+// once per launch.  The result moves by a few units in the last place of the
+// FP64 solve, which the bound's FP64 term (64 roundings of E0) covers.
+__device__ __forceinline__ double fast_quot(double n, double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  return n * r;
+}
+
+__device__ __forceinline__ double qcp_lambda_fast(const QcpCoef& k, double E0) {
+  double lam = E0;
+  for (int it = 0; it < 50; ++it) {
+    const double prev = lam;
+    const double l2 = lam * lam;
+    const double b = (l2 + k.c2) * lam;
+    const double a = b + k.c1;
+    lam -= fast_quot(a * lam + k.c0, 2.0 * l2 * lam + b + a);
+    if (fabs(lam - prev) < fabs(1e-11 * lam)) break;
+  }
+  return lam;
+}
+
+__device__ __forceinline__ double qcp_rmsd_fast(const double* S, double E0, double invM,
+                                                int clamp) {
+  const QcpCoef k = qcp_coefficients(S);
+  const double e = E0 - qcp_lambda_fast(k, E0);
+  if (clamp) return e > 0.0 ? sqrt(2.0 * e * invM) : 0.0;
+  return sqrt(fabs(2.0 * e * invM));
+}
+
 // Error bound of one pair of the MFMA-reduced forms against the FP64
 // computation.  The model (coefficients k*, per-frame norms ||D||, b, l from
 // k_prep) bounds the covariance error
