@@ -28,6 +28,7 @@ struct rmsd_batch {
   PrepArgs* dprep = nullptr;
   int* dfbase = nullptr;
   int nactive = 0, total_frames = 0;
+  int nquad = 0, quad_frames = 0;  // prep: the leading sets that allow quads
   // mean-structure templates (MFMA forms): sample prefix over the active sets
   int* dsbase = nullptr;
   int total_samples = 0, maxk = 0;
@@ -134,8 +135,8 @@ int batch_enqueue(rmsd_batch* b, int v, hipStream_t st) {
   if (b->prec != PREC_F64)
     HIP_TRY(c, rmsd_launch_template_mean_batch(b->dsetup, b->dsbase, b->nactive,
                                                b->total_samples, b->maxk, st));
-  HIP_TRY(c, rmsd_launch_prep_batch(b->prec, b->dprep, b->dfbase, b->nactive, b->total_frames,
-                                    st));
+  HIP_TRY(c, rmsd_launch_prep_batch(b->prec, b->dprep, b->dfbase, b->nactive, b->nquad,
+                                    b->quad_frames, b->total_frames, st));
   mark(c, 1);
   const int sh = v >> 1;
   BatchArgs ba;
@@ -482,6 +483,18 @@ int rmsd_batch_create(rmsd_ctx* ctx, const rmsd_frames* sets, int nsets, int mod
       fbase.push_back(fbase.back() + a.Fp);
       sbase.push_back(sbase.back() + nsamp[s]);
       b->maxk = std::max(b->maxk, 3 * b->Kp[s]);
+    }
+    // Prep order: the sets that allow 16-byte loads of four atoms first (their
+    // launch compiles the quad path; the rest run an atom-by-atom kernel).
+    {
+      std::vector<PrepArgs> q, r;
+      for (const PrepArgs& x : pa) (quad_layout(x.fs) ? q : r).push_back(x);
+      b->nquad = (int)q.size();
+      pa = q;
+      pa.insert(pa.end(), r.begin(), r.end());
+      fbase.assign(1, 0);
+      for (const PrepArgs& x : pa) fbase.push_back(fbase.back() + x.Fp);
+      b->quad_frames = fbase[b->nquad];
     }
     b->nactive = (int)su.size();
     b->total_frames = fbase.back();

@@ -291,9 +291,11 @@ __global__ __launch_bounds__(PREP_THREADS) void k_setup_batch(const SetupArgs* s
 }
 
 // ---- per-frame preparation -----------------------------------------------------
-template <int PREC>
+// QUADS = false compiles the atom-by-atom path only (fewer registers and no
+// branch to the quad code: the batched launch for sets without quads).
+template <int PREC, bool QUADS = true>
 __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
-  __shared__ double red[16 * (PREP_THREADS / 64)];
+  __shared__ double red[24 * (PREP_THREADS / 64)];
   const int Kp = a.Kp;
 
   if (f >= a.fs.nframes) {
@@ -316,18 +318,16 @@ __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
   // takes quads of atoms (four per step; the 50,000 x 1,200 AoS stage: 1.05 ->
   // 0.86 ms).  Otherwise lanes take consecutive atoms (coalesced 4-byte loads;
   // quads of such atoms would spread each load over four times the lines).
-  const int lay = PREC == PREC_F64 ? 0 : quad_layout(a.fs);
+  const int lay = PREC == PREC_F64 || !QUADS ? 0 : quad_layout(a.fs);
   double cen[3];
-  if (lay) centroid_quads(a.fs, f, a.cmass, red, cen);
-  else centroid(a.fs, f, a.cmass, red, cen);
-  if (a.cen && threadIdx.x == 0) {
-    gp(a.cen)[4 * f] = cen[0];
-    gp(a.cen)[4 * f + 1] = cen[1];
-    gp(a.cen)[4 * f + 2] = cen[2];
-    gp(a.cen)[4 * f + 3] = 0.0;
-  }
-
   if (PREC == PREC_F64) {
+    centroid(a.fs, f, a.cmass, red, cen);
+    if (a.cen && threadIdx.x == 0) {
+      gp(a.cen)[4 * f] = cen[0];
+      gp(a.cen)[4 * f + 1] = cen[1];
+      gp(a.cen)[4 * f + 2] = cen[2];
+      gp(a.cen)[4 * f + 3] = 0.0;
+    }
     double g[1] = {0.0};
     const auto out = gp(reinterpret_cast<double*>(a.packed));
     for (int k = threadIdx.x; k < Kp; k += blockDim.x) {
@@ -343,15 +343,86 @@ __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
     return;
   }
 
-  // MFMA-reduced forms: align to the template in FP64.
-  double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  auto moments = [&](const double x[3], const double t[3]) {
-    v[0] += x[0] * t[0]; v[1] += x[0] * t[1]; v[2] += x[0] * t[2];
-    v[3] += x[1] * t[0]; v[4] += x[1] * t[1]; v[5] += x[1] * t[2];
-    v[6] += x[2] * t[0]; v[7] += x[2] * t[1]; v[8] += x[2] * t[2];
-    v[9] += x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+  // MFMA-reduced forms: align to the template in FP64.  One pass over the
+  // frame gives the centroid and the moments about the origin: with s = sqrt(w)
+  // the fit weights and c the centroid,
+  //   M = sum s (x - c) t^T = sum s x t^T - c (sum s t)^T,
+  // (sum s t is zero up to rounding for a template of centred frames), and the
+  // QCP's Newton start E0 = (G + |T|^2) / 2 takes
+  //   G = sum w |x - c|^2 = sum w |x|^2 - 2 c . sum w x + |c|^2 sum w.
+  // The G stored for the pairs is summed from the centred coordinates below.
+  // v: sum wc x (3), sum wc, sum s x_a t_b (9), sum s t (3), sum w x (3),
+  // sum w, sum w |x|^2 (wc: centroid weights, w: fit weights).
+  double v[21];
+#pragma unroll
+  for (int i = 0; i < 21; ++i) v[i] = 0.0;
+  auto origin_moments = [&](int k, float px, float py, float pz, const double t[3]) {
+    const double wc = a.cmass ? gp(a.cmass)[k] : 1.0;
+    const double w = a.smass ? gp(a.smass)[k] : 1.0;
+    const double sq = a.smass ? sqrt(w) : 1.0;
+    const double x[3] = {(double)px, (double)py, (double)pz};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      v[c] += wc * x[c];
+      const double sx = sq * x[c];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) v[4 + 3 * c + d] += sx * t[d];
+      v[13 + c] += sq * t[c];
+      v[16 + c] += w * x[c];
+    }
+    v[3] += wc;
+    v[19] += w;
+    v[20] += w * (x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
   };
   const int nq = (a.fs.nsel + 3) >> 2;
+  if (lay) {
+    for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+      float px[4], py[4], pz[4];
+      double t[4][3];
+      atoms_quad(a.fs, f, q, lay, px, py, pz);
+      template_quad(a.T, Kp, q, t);
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (4 * q + e < a.fs.nsel) origin_moments(4 * q + e, px[e], py[e], pz[e], t[e]);
+    }
+  } else {
+    for (int k = threadIdx.x; k < a.fs.nsel; k += blockDim.x) {
+      float p[3];
+      load_atom(a.fs, f, k, p);
+      const double t[3] = {gp(a.T)[k], gp(a.T)[Kp + k], gp(a.T)[2 * Kp + k]};
+      origin_moments(k, p[0], p[1], p[2], t);
+    }
+  }
+  rmsd_dev::block_sum<21>(v, red);
+  {
+    const double tot = a.cmass ? v[3] : (double)a.fs.nsel;
+    const double inv = tot != 0.0 ? 1.0 / tot : 0.0;
+    cen[0] = v[0] * inv;
+    cen[1] = v[1] * inv;
+    cen[2] = v[2] * inv;
+  }
+  if (a.cen && threadIdx.x == 0) {
+    gp(a.cen)[4 * f] = cen[0];
+    gp(a.cen)[4 * f + 1] = cen[1];
+    gp(a.cen)[4 * f + 2] = cen[2];
+    gp(a.cen)[4 * f + 3] = 0.0;
+  }
+  // Every thread holds the block totals and solves the same FP64 QCP (same
+  // inputs, same operations), so the rotation needs no broadcast or barrier.
+  double U[9];
+  {
+    double Mc[9];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) Mc[3 * c + d] = v[4 + 3 * c + d] - cen[c] * v[13 + d];
+    const double G = v[20] - 2.0 * (cen[0] * v[16] + cen[1] * v[17] + cen[2] * v[18]) +
+                     (cen[0] * cen[0] + cen[1] * cen[1] + cen[2] * cen[2]) * v[19];
+    const double E0 = 0.5 * (G + gp(a.gT)[0]);
+    const rmsd_dev::QcpCoef co = rmsd_dev::qcp_coefficients(Mc);
+    const double lam = rmsd_dev::qcp_lambda(co, E0);
+    rmsd_dev::qcp_rotation(Mc, lam, U);
+  }
   // centred, weighted coordinates of atoms 4q ... 4q + 3 (zeros past nsel)
   auto centred_quad = [&](int q, double (&x)[4][3]) {
     float px[4], py[4], pz[4];
@@ -369,43 +440,15 @@ __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
       }
     }
   };
-  if (lay) {
-    for (int q = threadIdx.x; q < nq; q += blockDim.x) {
-      double x[4][3], t[4][3];
-      centred_quad(q, x);
-      template_quad(a.T, Kp, q, t);
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if (4 * q + e < a.fs.nsel) moments(x[e], t[e]);
-    }
-  } else {
-    for (int k = threadIdx.x; k < a.fs.nsel; k += blockDim.x) {
-      double x[3], t[3];
-      centred(a.fs, f, k, cen, a.smass, x);
-      t[0] = gp(a.T)[k];
-      t[1] = gp(a.T)[Kp + k];
-      t[2] = gp(a.T)[2 * Kp + k];
-      moments(x, t);
-    }
-  }
-  rmsd_dev::block_sum<10>(v, red);
-  // Every thread holds the block totals and solves the same FP64 QCP (same
-  // inputs, same operations), so the rotation needs no broadcast or barrier.
-  if (threadIdx.x == 0) gp(a.g)[f] = v[9];
-  double U[9];
-  {
-    const double E0 = 0.5 * (v[9] + gp(a.gT)[0]);
-    const rmsd_dev::QcpCoef co = rmsd_dev::qcp_coefficients(v);
-    const double lam = rmsd_dev::qcp_lambda(co, E0);
-    rmsd_dev::qcp_rotation(v, lam, U);
-  }
 
   // u = T^T D (9), then the sums behind the error bound's norms: ||D||^2,
   // sum_k D_c^4 per coordinate, and ||lo||^2 (the split's low parts; for FP32
   // the rounding residual D - float(D)).
-  double u[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // u[14]: G = sum |x~|^2, the frame's E0 term.
+  double u[15] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   // one atom: its D (accumulated into u) rounded to the stored parts
   auto emit = [&](const double x[3], const double t[3], float (&hi)[3], float (&lo)[3]) {
+    u[14] += x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
     double D[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -483,8 +526,9 @@ __device__ __forceinline__ void prep_body(const PrepArgs& a, int64_t f) {
       }
     }
   }
-  rmsd_dev::block_sum<14>(u, red);
+  rmsd_dev::block_sum<15>(u, red);
   if (threadIdx.x == 0) {
+    gp(a.g)[f] = u[14];
     for (int k = 0; k < 9; ++k) gp(a.u)[9 * f + k] = u[k];
     if (a.nrm) {
       // The error bound's per-frame coefficient (rmsd_dev::bound_keep).
@@ -531,12 +575,14 @@ __global__ __launch_bounds__(PREP_THREADS) __attribute__((amdgpu_waves_per_eu(4)
 
 // Every padded frame of every set of a batch in one launch: workgroup b is
 // frame b - fbase[s] of the set s with fbase[s] <= b < fbase[s + 1].
-template <int PREC>
-__global__ __launch_bounds__(PREP_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_prep_batch(const PrepArgs* sets,
-                                                             const int* fbase, int nsets) {
+// Frames fbase[0] ... fbase[nsets] - 1 of sets[0 .. nsets) (fbase: prefix of
+// the padded frame counts, which may start above zero).
+template <int PREC, bool QUADS>
+__global__ __launch_bounds__(PREP_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_prep_batch(
+    const PrepArgs* sets, const int* fbase, int nsets) {
   // every set's padded frame count is a multiple of 16, so no frame block
   // straddles two sets
-  const int b = (int)prep_frame(blockIdx.x);
+  const int b = fbase[0] + (int)prep_frame(blockIdx.x);
   if (b >= fbase[nsets]) return;
   int lo = 0, hi = nsets - 1;
   while (lo < hi) {
@@ -545,7 +591,7 @@ __global__ __launch_bounds__(PREP_THREADS) __attribute__((amdgpu_waves_per_eu(4)
     else hi = mid - 1;
   }
   const PrepArgs a = sets[lo];
-  prep_body<PREC>(a, b - fbase[lo]);
+  prep_body<PREC, QUADS>(a, b - fbase[lo]);
 }
 
 }  // namespace
@@ -582,20 +628,31 @@ hipError_t rmsd_launch_setup_batch(const SetupArgs* sets, int nsets, hipStream_t
 }
 
 hipError_t rmsd_launch_prep_batch(int prec, const PrepArgs* sets, const int* fbase, int nsets,
-                                  int total_frames, hipStream_t s) {
+                                  int nquad, int quad_frames, int total_frames, hipStream_t s) {
   if (nsets <= 0 || total_frames <= 0) return hipSuccess;
+  if (nquad < 0 || nquad > nsets || quad_frames < 0 || quad_frames > total_frames)
+    return hipErrorInvalidValue;
+  const int rest = total_frames - quad_frames;
   switch (prec) {
     case PREC_F64:
-      hipLaunchKernelGGL(k_prep_batch<PREC_F64>, dim3(prep_grid(total_frames)), dim3(PREP_THREADS), 0, s,
-                         sets, fbase, nsets);
+      hipLaunchKernelGGL((k_prep_batch<PREC_F64, false>), dim3(prep_grid(total_frames)),
+                         dim3(PREP_THREADS), 0, s, sets, fbase, nsets);
       break;
     case PREC_F32:
-      hipLaunchKernelGGL(k_prep_batch<PREC_F32>, dim3(prep_grid(total_frames)), dim3(PREP_THREADS), 0, s,
-                         sets, fbase, nsets);
+      if (nquad > 0)
+        hipLaunchKernelGGL((k_prep_batch<PREC_F32, true>), dim3(prep_grid(quad_frames)),
+                           dim3(PREP_THREADS), 0, s, sets, fbase, nquad);
+      if (nquad < nsets)
+        hipLaunchKernelGGL((k_prep_batch<PREC_F32, false>), dim3(prep_grid(rest)), dim3(PREP_THREADS),
+                           0, s, sets + nquad, fbase + nquad, nsets - nquad);
       break;
     case PREC_BF16X3:
-      hipLaunchKernelGGL(k_prep_batch<PREC_BF16X3>, dim3(prep_grid(total_frames)), dim3(PREP_THREADS), 0, s,
-                         sets, fbase, nsets);
+      if (nquad > 0)
+        hipLaunchKernelGGL((k_prep_batch<PREC_BF16X3, true>), dim3(prep_grid(quad_frames)),
+                           dim3(PREP_THREADS), 0, s, sets, fbase, nquad);
+      if (nquad < nsets)
+        hipLaunchKernelGGL((k_prep_batch<PREC_BF16X3, false>), dim3(prep_grid(rest)),
+                           dim3(PREP_THREADS), 0, s, sets + nquad, fbase + nquad, nsets - nquad);
       break;
     default: return hipErrorInvalidValue;
   }

@@ -337,7 +337,8 @@ __device__ __forceinline__ double wave_reduce16(const double (&v)[16]) {
 }
 
 // Sums NV doubles per thread over the whole block.  Every thread receives the
-// totals.  red must hold NV * (blockDim.x / 64) doubles of shared memory.
+// totals.  red must hold NV * (blockDim.x / 64) doubles of shared memory (one
+// wave: red is not touched, and there is no barrier).
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* red) {
   const int lane = threadIdx.x & 63;
@@ -345,6 +346,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* red) {
   const int nw = blockDim.x >> 6;
 #pragma unroll
   for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+  if (nw == 1) return;  // one wave: every lane already holds the totals
   if (lane == 0) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) red[wave * NV + i] = v[i];

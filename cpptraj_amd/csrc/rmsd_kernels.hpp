@@ -373,10 +373,13 @@ constexpr size_t AVGCORR_PREFIX_MAX = size_t(16) << 30;
 
 hipError_t rmsd_launch_setup(const SetupArgs& a, hipStream_t s);
 hipError_t rmsd_launch_prep(const PrepArgs& a, hipStream_t s);
-// Batched forms: one workgroup per set / per padded frame of every set.
+// Batched forms: one workgroup per set / per padded frame of every set.  The
+// prep takes the sets that allow 16-byte loads of four atoms first (nquad sets,
+// quad_frames padded frames: fbase[nquad]) and launches them and the rest
+// separately.
 hipError_t rmsd_launch_setup_batch(const SetupArgs* sets, int nsets, hipStream_t s);
 hipError_t rmsd_launch_prep_batch(int prec, const PrepArgs* sets, const int* fbase, int nsets,
-                                  int total_frames, hipStream_t s);
+                                  int nquad, int quad_frames, int total_frames, hipStream_t s);
 hipError_t rmsd_launch_pairwise(int prec, const PairArgs& a, hipStream_t s);
 // Wave layout of rmsd_launch_pairwise / rmsd_launch_pairwise_batch variants
 // (wlist and masks hold NWR * NWC waves per tile).
