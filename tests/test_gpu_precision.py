@@ -119,6 +119,43 @@ def test_recomputed_pairs_match_the_oracle(oracle, ctx, two_state):
     ctx.set_tolerance(TOL)
 
 
+@pytest.mark.parametrize("layout", ["aos", "aos_tail", "soa", "soa_odd_ld", "mask"])
+def test_recomputation_in_every_frame_layout(oracle, ctx, two_state, layout):
+    """The FP64 recomputation loads four atoms per 16-byte load where the frames
+    allow it (AoS with a row stride, or SoA with a row length, that is a
+    multiple of 4 floats, and no selection; atom by atom past the last whole
+    quad) and one atom at a time otherwise.  With every pair recomputed, each
+    layout gives the FP64 result (<= 1e-6 A of the oracle)."""
+    import torch
+    import cpptraj_amd as ca
+    X, state = two_state
+    F, n = 72, 1200
+    Y = np.ascontiguousarray(X[:F].reshape(F, n, 3))
+    kw, used, sel = {}, Y, None
+    if layout == "aos":
+        inp = dev(Y)
+    elif layout == "aos_tail":  # 1,197 atoms of 1,200-atom rows: 299 quads + 1
+        inp, kw, used = dev(Y), {"natom": 1197}, Y[:, :1197]
+    elif layout == "soa":  # (F, 3, 1200): ld a multiple of 4
+        inp, kw = dev(np.ascontiguousarray(Y.transpose(0, 2, 1))), {"soa": True}
+    elif layout == "soa_odd_ld":  # rows of 1,203 floats, 1,200 atoms used
+        S = np.zeros((F, 3, 1203), np.float32)
+        S[:, :, :n] = Y.transpose(0, 2, 1)
+        inp, kw = dev(S), {"soa": True, "natom": n}
+    else:  # every third atom selected
+        sel = np.arange(0, n, 3, dtype=np.int32)
+        inp, kw, used = dev(Y), {"mask": dev(sel)}, Y[:, sel]
+    ctx.set_tolerance(1e-12)
+    try:
+        got = ca.rms2d(inp, precision="bf16x3", ctx=ctx, **kw).cpu().numpy().astype(np.float64)
+        bound, nfp64 = ctx.error_report()
+    finally:
+        ctx.set_tolerance(TOL)
+    assert nfp64 == len(got) == F * (F - 1) // 2
+    exp = oracle.rms2d(np.ascontiguousarray(used).reshape(F, -1))
+    assert np.abs(got - exp).max() <= 1e-6
+
+
 @pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
 def test_tolerance_ladder_on_drift(ctx, precision):
     """Drifting trajectory: with tolerance t every pair is within t of FP64, for
