@@ -14,9 +14,12 @@ from cpptraj_amd import synth
 
 pytestmark = pytest.mark.gpu
 
-# (frames, atoms, selected) per set: ragged sizes, 0/1-frame sets, one atom.
+# (frames, atoms, selected) per set: ragged sizes, 0/1-frame sets, one atom;
+# (90, 128) and (41, 64) allow 16-byte loads of four atoms (AoS rows of a
+# multiple of 4 floats, no selection), so the plan's prep runs both its quad and
+# its atom-by-atom launch.
 SIZES = [(130, 257, None), (2, 5, None), (1, 40, None), (70, 33, 20), (200, 100, 100),
-         (65, 1, None), (0, 12, None), (97, 300, 151)]
+         (90, 128, None), (65, 1, None), (0, 12, None), (97, 300, 151), (41, 64, None)]
 
 
 def make_sets(seed, sizes=SIZES, masses=True):
