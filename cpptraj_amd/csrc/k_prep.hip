@@ -110,36 +110,6 @@ __device__ __forceinline__ void template_quad(const double* T, int Kp, int q, do
   }
 }
 
-// Weighted centroid of frame f over quads of atoms (the MFMA forms: each lane
-// adds atoms 4q ... 4q + 3 in order).
-__device__ __forceinline__ void centroid_quads(const FrameSetDev& fs, int64_t f, const double* w,
-                                               double* red, double cen[3]) {
-  double v[4] = {0.0, 0.0, 0.0, 0.0};
-  const int lay = quad_layout(fs);
-  const int nq = (fs.nsel + 3) >> 2;
-  for (int q = threadIdx.x; q < nq; q += blockDim.x) {
-    float x[4], y[4], z[4];
-    atoms_quad(fs, f, q, lay, x, y, z);
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int k = 4 * q + e;
-      if (k < fs.nsel) {
-        const double m = w ? gp(w)[k] : 1.0;
-        v[0] += m * (double)x[e];
-        v[1] += m * (double)y[e];
-        v[2] += m * (double)z[e];
-        v[3] += m;
-      }
-    }
-  }
-  rmsd_dev::block_sum<4>(v, red);
-  const double tot = w ? v[3] : (double)fs.nsel;
-  const double inv = tot != 0.0 ? 1.0 / tot : 0.0;
-  cen[0] = v[0] * inv;
-  cen[1] = v[1] * inv;
-  cen[2] = v[2] * inv;
-}
-
 // ---- template / total weight -----------------------------------------------
 __device__ __forceinline__ void setup_body(const SetupArgs& a) {
   __shared__ double red[16 * (PREP_THREADS / 64)];
