@@ -119,8 +119,9 @@ def test_recomputed_pairs_match_the_oracle(oracle, ctx, two_state):
     ctx.set_tolerance(TOL)
 
 
+@pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
 @pytest.mark.parametrize("layout", ["aos", "aos_tail", "soa", "soa_odd_ld", "mask"])
-def test_recomputation_in_every_frame_layout(oracle, ctx, two_state, layout):
+def test_recomputation_in_every_frame_layout(oracle, ctx, two_state, layout, precision):
     """The FP64 recomputation loads four atoms per 16-byte load where the frames
     allow it (AoS with a row stride, or SoA with a row length, that is a
     multiple of 4 floats, and no selection; atom by atom past the last whole
@@ -147,7 +148,7 @@ def test_recomputation_in_every_frame_layout(oracle, ctx, two_state, layout):
         inp, kw, used = dev(Y), {"mask": dev(sel)}, Y[:, sel]
     ctx.set_tolerance(1e-12)
     try:
-        got = ca.rms2d(inp, precision="bf16x3", ctx=ctx, **kw).cpu().numpy().astype(np.float64)
+        got = ca.rms2d(inp, precision=precision, ctx=ctx, **kw).cpu().numpy().astype(np.float64)
         bound, nfp64 = ctx.error_report()
     finally:
         ctx.set_tolerance(TOL)
