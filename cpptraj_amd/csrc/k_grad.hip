@@ -94,26 +94,28 @@ __global__ __launch_bounds__(256) void k_center(FrameSetDev fs, const double* w,
   if (threadIdx.x == 0) g[f] = e[0];
 }
 
-// One wave per 16 x 16 block of pairs (i target, j reference); cw[(i*FB + j)*10]:
-// c_ij and c_ij * U_ij (row-major), c_ij = G_ij / RMSD_ij (0 at RMSD = 0).
-// The covariances S_ij = sum_k w_k x~_i,k y~_j,k^T of the block run on
-// v_mfma_f64_16x16x4f64, nine accumulator blocks (x coordinate c by y
+// One workgroup per 16 x 16 block of pairs (i target, j reference);
+// cw[(i*FB + j)*10]: c_ij and c_ij * U_ij (row-major), c_ij = G_ij / RMSD_ij (0
+// at RMSD = 0).  The covariances S_ij = sum_k w_k x~_i,k y~_j,k^T of the block
+// run on v_mfma_f64_16x16x4f64, nine accumulator blocks (x coordinate c by y
 // coordinate d): in atom steps of 16, lane l feeds atoms k0 + 4 (l / 16) + s,
 // s = 0..3, of row frame i0 + l % 16 (A, 16 x 4) and of column frame j0 + l % 16
-// (B, 4 x 16), so that each lane reads four consecutive atoms of a coordinate
-// row.  Then per pair the FP64 QCP, its rotation and c_ij.  (One thread per
-// pair with a serial atom loop took 6.2 ms for 512 x 512 pairs of 2,000 atoms.)
+// (B, 4 x 16), four consecutive atoms of a coordinate row (two 16-byte loads
+// when n is even).  The four waves take a quarter of the atoms each; their
+// partial sums are added in a fixed order through LDS, and wave 0 solves the
+// block's pairs: FP64 QCP, rotation, c_ij.  (One thread per pair with a serial
+// atom loop took 6.2 ms for 512 x 512 pairs of 2,000 atoms; one wave per block
+// over all atoms 0.58 ms.)
 typedef __attribute__((ext_vector_type(4))) double f64x4_g;
-constexpr int ROT_WAVES = 4;
-__global__ __launch_bounds__(64 * ROT_WAVES) void k_pair_rot(const double* XA, const double* XB,
-                                                             const double* gA, const double* gB,
-                                                             const double* w, int n, int FA, int FB,
-                                                             const double* Mp, const double* G,
-                                                             double* cw) {
-  const int lane = threadIdx.x & 63;
+typedef __attribute__((ext_vector_type(2))) double f64x2_g;
+constexpr int ROT_SPLIT = 4;
+__global__ __launch_bounds__(64 * ROT_SPLIT) __attribute__((amdgpu_waves_per_eu(2))) void k_pair_rot(
+    const double* XA, const double* XB, const double* gA, const double* gB, const double* w, int n,
+    int FA, int FB, const double* Mp, const double* G, double* cw) {
+  __shared__ double part[2][36][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nbj = (FB + 15) / 16;
-  const int64_t blk = (int64_t)blockIdx.x * ROT_WAVES + (threadIdx.x >> 6);
-  if (blk >= (int64_t)((FA + 15) / 16) * nbj) return;  // whole waves; no barriers here
+  const int64_t blk = blockIdx.x;  // the grid is the block count: no workgroup exits early
   const int64_t i0 = (blk / nbj) * 16, j0 = (blk % nbj) * 16;
   const int64_t fi = i0 + (lane & 15), fj = j0 + (lane & 15);
   const bool iok = fi < FA, jok = fj < FB;
@@ -124,27 +126,74 @@ __global__ __launch_bounds__(64 * ROT_WAVES) void k_pair_rot(const double* XA, c
   for (int c = 0; c < 3; ++c)
 #pragma unroll
     for (int d = 0; d < 3; ++d) acc[c][d] = f64x4_g{0.0, 0.0, 0.0, 0.0};
+  const int per = ((n + 15) / 16 + ROT_SPLIT - 1) / ROT_SPLIT * 16;  // atoms per wave
+  const int kbeg = wave * per, kend = n < kbeg + per ? n : kbeg + per;
   const int kq = 4 * (lane >> 4);
-  for (int k0 = 0; k0 < n; k0 += 16) {
+  const bool pairs = (n & 1) == 0;  // rows of an even length: 16-byte aligned pairs of atoms
+  for (int k0 = kbeg; k0 < kend; k0 += 16) {
+    const int k = k0 + kq;
+    double av[4][3], bv[4][3];
+    if (pairs && k + 3 < kend) {
 #pragma unroll
-    for (int s4 = 0; s4 < 4; ++s4) {
-      const int k = k0 + kq + s4;
-      double av[3] = {0.0, 0.0, 0.0}, bv[3] = {0.0, 0.0, 0.0};
-      if (k < n) {
-        const double m = w ? w[k] : 1.0;
+      for (int c = 0; c < 3; ++c) {
+        f64x2_g a0 = {0.0, 0.0}, a1 = {0.0, 0.0}, b0 = {0.0, 0.0}, b1 = {0.0, 0.0};
+        if (iok) {
+          a0 = *reinterpret_cast<const f64x2_g*>(x + c * n + k);
+          a1 = *reinterpret_cast<const f64x2_g*>(x + c * n + k + 2);
+        }
+        if (jok) {
+          b0 = *reinterpret_cast<const f64x2_g*>(y + c * n + k);
+          b1 = *reinterpret_cast<const f64x2_g*>(y + c * n + k + 2);
+        }
+        av[0][c] = a0.x; av[1][c] = a0.y; av[2][c] = a1.x; av[3][c] = a1.y;
+        bv[0][c] = b0.x; bv[1][c] = b0.y; bv[2][c] = b1.x; bv[3][c] = b1.y;
+      }
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          if (iok) av[c] = m * x[c * n + k];
-          if (jok) bv[c] = y[c * n + k];
+      for (int s4 = 0; s4 < 4; ++s4) {
+        const double m = w ? w[k + s4] : 1.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) av[s4][c] *= m;
+      }
+    } else {
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) {
+        const int ks = k + s4;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) av[s4][c] = bv[s4][c] = 0.0;
+        if (ks < kend) {
+          const double m = w ? w[ks] : 1.0;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            if (iok) av[s4][c] = m * x[c * n + ks];
+            if (jok) bv[s4][c] = y[c * n + ks];
+          }
         }
       }
+    }
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4)
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int d = 0; d < 3; ++d)
-          acc[c][d] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[c], bv[d], acc[c][d], 0, 0, 0);
-    }
+          acc[c][d] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s4][c], bv[s4][d], acc[c][d], 0, 0, 0);
   }
+  // (acc0 + acc2) + (acc1 + acc3), in that order
+  if (wave >= 2)
+#pragma unroll
+    for (int e = 0; e < 36; ++e) part[wave - 2][e][lane] = acc[e / 12][(e / 4) % 3][e % 4];
+  __syncthreads();
+  if (wave < 2)
+#pragma unroll
+    for (int e = 0; e < 36; ++e) acc[e / 12][(e / 4) % 3][e % 4] += part[wave][e][lane];
+  __syncthreads();
+  if (wave == 1)
+#pragma unroll
+    for (int e = 0; e < 36; ++e) part[0][e][lane] = acc[e / 12][(e / 4) % 3][e % 4];
+  __syncthreads();
+  if (wave != 0) return;
+#pragma unroll
+  for (int e = 0; e < 36; ++e) acc[e / 12][(e / 4) % 3][e % 4] += part[0][e][lane];
   const double M = Mp[0];
   // The accumulators hold row (lane / 16) + 4 v, column lane % 16, in item v.
 #pragma unroll
@@ -283,8 +332,9 @@ hipError_t rmsd_launch_pair_grad(const double* XA, const double* XB, const doubl
                                  const double* M, const double* G, double* cw, double* gradA,
                                  double* gradB, hipStream_t s) {
   const int64_t nblk = (int64_t)((FA + 15) / 16) * ((FB + 15) / 16);
-  hipLaunchKernelGGL(k_pair_rot, dim3((unsigned)((nblk + ROT_WAVES - 1) / ROT_WAVES)),
-                     dim3(64 * ROT_WAVES), 0, s, XA, XB, gA, gB, w, n, FA, FB, M, G, cw);
+  if (nblk > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_pair_rot, dim3((unsigned)nblk), dim3(64 * ROT_SPLIT), 0, s, XA, XB, gA, gB,
+                     w, n, FA, FB, M, G, cw);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (gradA) {
