@@ -17,7 +17,8 @@
 //   k_pair_rot    per 16 x 16 block of pairs (i, j): covariances on FP64
 //                 MFMA, then per pair QCP and rotation; stores c_ij = G_ij /
 //                 RMSD_ij and c_ij U_ij (recomputed in backward)
-//   k_grad_rows / k_grad_cols   the partner sums of the block gradient
+//   k_grad_sum    the partner sums of the block gradient (rows, columns) on
+//                 FP64 MFMA
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "rmsd_device.hpp"
@@ -219,93 +220,91 @@ __global__ __launch_bounds__(64 * ROT_SPLIT) __attribute__((amdgpu_waves_per_eu(
   }
 }
 
-// grad_A[i][k] = w_k / M * (sum_j c_ij x~_i,k - sum_j (c_ij U_ij)^T y~_j,k).
-// Block = (8 rows i, 256 atoms); the partner loop streams y~_j once per block.
-constexpr int GR = 8;
-__global__ __launch_bounds__(256) void k_grad_rows(const double* XA, const double* XB,
-                                                   const double* cw, const double* w, int n, int FA,
-                                                   int FB, const double* Mp, double* grad) {
-  __shared__ double sc[GR][10];
+// The partner sums of the block gradient on FP64 MFMA.  For own frame o, atom
+// k and coordinate c:
+//   ROWS (own i, partners j):     O[i][c][k] = sum_j sum_b (c U)_ij[b][c] y~_j[b][k]
+//   columns (own j, partners i):  O[j][c][k] = sum_i sum_a (c U)_ij[c][a] x~_i[a][k]
+//   grad[o][k][c] = w_k / M (cs_o own~_o[c][k] - O[o][c][k]),  cs_o = sum_p c_op.
+// One wave per 16 own frames x 64 atoms.  Per step of 4 partners, lane l holds
+// the pair record (c, c U) of own frame o0 + l % 16 and partner p0 + l / 16
+// (five 16-byte loads): the A operands (16 x 4) of the nine (own coordinate,
+// partner coordinate) products; B (4 x 16) is the partner's coordinate row at
+// atoms k0 + 16 g + l % 16.  (A thread per atom with the coefficients of one
+// partner per LDS stage: 0.58 ms per side for 512 x 512 pairs of 2,000 atoms.)
+constexpr int GKT = 4;  // 16-atom tiles per wave
+template <bool ROWS>
+__global__ __launch_bounds__(256) void k_grad_sum(const double* XA, const double* XB,
+                                                  const double* cw, const double* w, int n, int FA,
+                                                  int FB, const double* Mp, double* grad) {
+  const int lane = threadIdx.x & 63;
+  const int NO = ROWS ? FA : FB, NP = ROWS ? FB : FA;  // own frames, partners
+  const double* XO = ROWS ? XA : XB;
+  const double* XP = ROWS ? XB : XA;
+  const int nkt = (n + 16 * GKT - 1) / (16 * GKT);
+  const int64_t wv = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (wv >= (int64_t)((NO + 15) / 16) * nkt) return;  // whole waves; no barriers here
+  const int o0 = (int)(wv / nkt) * 16, k0 = (int)(wv % nkt) * 16 * GKT;
+  const int ol = o0 + (lane & 15);
+  const bool ook = ol < NO;
+  f64x4_g acc[GKT][3];
+#pragma unroll
+  for (int g = 0; g < GKT; ++g)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[g][c] = f64x4_g{0.0, 0.0, 0.0, 0.0};
+  double csum = 0.0;
+  for (int p0 = 0; p0 < NP; p0 += 4) {
+    const int pl = p0 + (lane >> 4);
+    const bool pok = pl < NP;
+    double rec[10];
+#pragma unroll
+    for (int t = 0; t < 10; ++t) rec[t] = 0.0;
+    if (ook && pok) {
+      const int64_t pair = ROWS ? (int64_t)ol * FB + pl : (int64_t)pl * FB + ol;
+      const f64x2_g* r2 = reinterpret_cast<const f64x2_g*>(cw + pair * 10);
+#pragma unroll
+      for (int t = 0; t < 5; ++t) {
+        const f64x2_g v = r2[t];
+        rec[2 * t] = v.x;
+        rec[2 * t + 1] = v.y;
+      }
+    }
+    csum += rec[0];
+    const double* yp = XP + (int64_t)(pok ? pl : 0) * 3 * n;
+#pragma unroll
+    for (int g = 0; g < GKT; ++g) {
+      const int k = k0 + 16 * g + (lane & 15);
+      double bv[3] = {0.0, 0.0, 0.0};
+      if (pok && k < n)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) bv[b] = yp[b * n + k];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+          acc[g][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(ROWS ? rec[1 + 3 * b + c] : rec[1 + 3 * c + b],
+                                                           bv[b], acc[g][c], 0, 0, 0);
+    }
+  }
+  // cs of own frame o0 + r: lanes r, r + 16, r + 32, r + 48
+  csum += __shfl_xor(csum, 16, 64);
+  csum += __shfl_xor(csum, 32, 64);
   const double M = Mp[0];
-  const int i0 = blockIdx.y * GR;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  double acc[GR][3], cs[GR];
+  // The accumulators hold own frame o0 + (lane / 16) + 4 v, atom lane % 16, in item v.
 #pragma unroll
-  for (int a = 0; a < GR; ++a) acc[a][0] = acc[a][1] = acc[a][2] = cs[a] = 0.0;
-  for (int j = 0; j < FB; ++j) {
-    __syncthreads();
-    if (threadIdx.x < GR * 10) {
-      const int a = threadIdx.x / 10, e = threadIdx.x % 10;
-      sc[a][e] = (i0 + a < FA) ? cw[((int64_t)(i0 + a) * FB + j) * 10 + e] : 0.0;
+  for (int v = 0; v < 4; ++v) {
+    const double cs = __shfl(csum, (lane >> 4) + 4 * v, 64);
+    const int o = o0 + (lane >> 4) + 4 * v;
+    if (o >= NO) continue;
+    const double* xo = XO + (int64_t)o * 3 * n;
+#pragma unroll
+    for (int g = 0; g < GKT; ++g) {
+      const int k = k0 + 16 * g + (lane & 15);
+      if (k >= n) continue;
+      const double s = (w ? w[k] : 1.0) / M;
+      double* gp_ = grad + ((int64_t)o * n + k) * 3;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) gp_[c] = s * (cs * xo[c * n + k] - acc[g][c][v]);
     }
-    __syncthreads();
-    if (k >= n) continue;
-    const double* y = XB + (int64_t)j * 3 * n;
-    const double y0 = y[k], y1 = y[n + k], y2 = y[2 * n + k];
-#pragma unroll
-    for (int a = 0; a < GR; ++a) {
-      const double* W = &sc[a][1];  // c U, row-major; (c U)^T y
-      acc[a][0] += W[0] * y0 + W[3] * y1 + W[6] * y2;
-      acc[a][1] += W[1] * y0 + W[4] * y1 + W[7] * y2;
-      acc[a][2] += W[2] * y0 + W[5] * y1 + W[8] * y2;
-      cs[a] += sc[a][0];
-    }
-  }
-  if (k >= n) return;
-  const double s = (w ? w[k] : 1.0) / M;
-#pragma unroll
-  for (int a = 0; a < GR; ++a) {
-    const int i = i0 + a;
-    if (i >= FA) break;
-    const double* x = XA + (int64_t)i * 3 * n;
-    double* g = grad + ((int64_t)i * n + k) * 3;
-    g[0] = s * (cs[a] * x[k] - acc[a][0]);
-    g[1] = s * (cs[a] * x[n + k] - acc[a][1]);
-    g[2] = s * (cs[a] * x[2 * n + k] - acc[a][2]);
-  }
-}
-
-// grad_B[j][k] = w_k / M * (sum_i c_ij y~_j,k - sum_i (c_ij U_ij) x~_i,k).
-__global__ __launch_bounds__(256) void k_grad_cols(const double* XA, const double* XB,
-                                                   const double* cw, const double* w, int n, int FA,
-                                                   int FB, const double* Mp, double* grad) {
-  __shared__ double sc[GR][10];
-  const double M = Mp[0];
-  const int j0 = blockIdx.y * GR;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  double acc[GR][3], cs[GR];
-#pragma unroll
-  for (int a = 0; a < GR; ++a) acc[a][0] = acc[a][1] = acc[a][2] = cs[a] = 0.0;
-  for (int i = 0; i < FA; ++i) {
-    __syncthreads();
-    if (threadIdx.x < GR * 10) {
-      const int a = threadIdx.x / 10, e = threadIdx.x % 10;
-      sc[a][e] = (j0 + a < FB) ? cw[((int64_t)i * FB + (j0 + a)) * 10 + e] : 0.0;
-    }
-    __syncthreads();
-    if (k >= n) continue;
-    const double* x = XA + (int64_t)i * 3 * n;
-    const double x0 = x[k], x1 = x[n + k], x2 = x[2 * n + k];
-#pragma unroll
-    for (int a = 0; a < GR; ++a) {
-      const double* W = &sc[a][1];  // (c U) x
-      acc[a][0] += W[0] * x0 + W[1] * x1 + W[2] * x2;
-      acc[a][1] += W[3] * x0 + W[4] * x1 + W[5] * x2;
-      acc[a][2] += W[6] * x0 + W[7] * x1 + W[8] * x2;
-      cs[a] += sc[a][0];
-    }
-  }
-  if (k >= n) return;
-  const double s = (w ? w[k] : 1.0) / M;
-#pragma unroll
-  for (int a = 0; a < GR; ++a) {
-    const int j = j0 + a;
-    if (j >= FB) break;
-    const double* y = XB + (int64_t)j * 3 * n;
-    double* g = grad + ((int64_t)j * n + k) * 3;
-    g[0] = s * (cs[a] * y[k] - acc[a][0]);
-    g[1] = s * (cs[a] * y[n + k] - acc[a][1]);
-    g[2] = s * (cs[a] * y[2 * n + k] - acc[a][2]);
   }
 }
 
@@ -337,14 +336,17 @@ hipError_t rmsd_launch_pair_grad(const double* XA, const double* XB, const doubl
                      w, n, FA, FB, M, G, cw);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  const int nkt = (n + 16 * GKT - 1) / (16 * GKT);
   if (gradA) {
-    hipLaunchKernelGGL(k_grad_rows, dim3((n + 255) / 256, (FA + GR - 1) / GR), dim3(256), 0, s, XA,
-                       XB, cw, w, n, FA, FB, M, gradA);
+    const int64_t waves = (int64_t)((FA + 15) / 16) * nkt;
+    hipLaunchKernelGGL(k_grad_sum<true>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, XA, XB,
+                       cw, w, n, FA, FB, M, gradA);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   if (gradB) {
-    hipLaunchKernelGGL(k_grad_cols, dim3((n + 255) / 256, (FB + GR - 1) / GR), dim3(256), 0, s, XA,
-                       XB, cw, w, n, FA, FB, M, gradB);
+    const int64_t waves = (int64_t)((FB + 15) / 16) * nkt;
+    hipLaunchKernelGGL(k_grad_sum<false>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, XA, XB,
+                       cw, w, n, FA, FB, M, gradB);
     e = hipGetLastError();
   }
   return e;
