@@ -114,10 +114,21 @@ __device__ __forceinline__ void template_quad(const double* T, int Kp, int q, do
 __device__ __forceinline__ void setup_body(const SetupArgs& a) {
   __shared__ double red[16 * (PREP_THREADS / 64)];
   if (threadIdx.x == 0) {
-    // Sequential running total, rounded as the reference's per-pair sum.
+    // Sequential running total, rounded as the reference's per-pair sum; the
+    // weights are loaded 16 at a time so that their loads overlap.
     double tw = 0.0;
     if (a.smass) {
-      for (int k = 0; k < a.fs.nsel; ++k) tw += a.smass[k];
+      const auto m = gp(a.smass);
+      const int n = a.fs.nsel;
+      int k = 0;
+      for (; k + 16 <= n; k += 16) {
+        double v[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) v[e] = m[k + e];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) tw += v[e];
+      }
+      for (; k < n; ++k) tw += m[k];
     } else {
       tw = (double)a.fs.nsel;
     }
