@@ -225,13 +225,13 @@ __global__ __launch_bounds__(64 * ROT_SPLIT) __attribute__((amdgpu_waves_per_eu(
 //   ROWS (own i, partners j):     O[i][c][k] = sum_j sum_b (c U)_ij[b][c] y~_j[b][k]
 //   columns (own j, partners i):  O[j][c][k] = sum_i sum_a (c U)_ij[c][a] x~_i[a][k]
 //   grad[o][k][c] = w_k / M (cs_o own~_o[c][k] - O[o][c][k]),  cs_o = sum_p c_op.
-// One wave per 16 own frames x 64 atoms.  Per step of 4 partners, lane l holds
+// One wave per 16 own frames x 32 atoms.  Per step of 4 partners, lane l holds
 // the pair record (c, c U) of own frame o0 + l % 16 and partner p0 + l / 16
 // (five 16-byte loads): the A operands (16 x 4) of the nine (own coordinate,
 // partner coordinate) products; B (4 x 16) is the partner's coordinate row at
 // atoms k0 + 16 g + l % 16.  (A thread per atom with the coefficients of one
 // partner per LDS stage: 0.58 ms per side for 512 x 512 pairs of 2,000 atoms.)
-constexpr int GKT = 4;  // 16-atom tiles per wave
+constexpr int GKT = 2;  // 16-atom tiles per wave (4: 0.97 ms, 1: 0.90 ms for the whole call)
 template <bool ROWS>
 __global__ __launch_bounds__(256) void k_grad_sum(const double* XA, const double* XB,
                                                   const double* cw, const double* w, int n, int FA,
